@@ -1,0 +1,344 @@
+"""Python face of the oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads oracle/liboracle.so (the C restatement in oracle.c) and, when it has been
+built from the reference tree, oracle/_ref/libmidas_ref.so (the reference's
+own Fortran routines).  Dense solves use LAPACK through numpy (zgesv), the
+CBFM pipeline is restated with numpy (block solves, SVD, sigma-ratio
+truncation, Zc = C^T Z C, Vc, E = C alpha).
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may
+import this module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        d, i, v = C.c_double, C.c_int, C.c_void_p
+        _LIB.or_pi.restype = d
+        _LIB.or_pi.argtypes = [i]
+        _LIB.or_aca.restype = i
+        _LIB.or_aca_gen.restype = i
+    return _LIB
+
+
+def ref():
+    """The reference's own routines (oracle/_ref), or None when not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libmidas_ref.so")
+        if not os.path.exists(path):
+            return None
+        # lazy binding: one unused reference routine has an unresolved helper
+        _REF = C.CDLL(path, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    return _REF
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+class Cells:
+    """SoA cell table (the fields of the reference's `type Cell` the hot path reads)."""
+
+    def __init__(self, x, y, z, sc, eps, k0, quirk=1):
+        self.x, self.y, self.z = _f64(x), _f64(y), _f64(z)
+        self.sc = _f64(sc)
+        self.eps = _c128(eps)
+        self.n = len(self.x)
+        self.k0 = float(k0)
+        self.quirk = quirk
+        self.id = np.arange(1, self.n + 1, dtype=np.int32)
+        self.a = np.empty(self.n)
+        self.che = np.empty(self.n, np.complex128)
+        self.znnpp = np.empty(self.n, np.complex128)
+        self.kappa = np.empty(self.n)
+        lib().or_cell_params(C.c_int(self.n), _p(self.sc), _p(self.eps), C.c_double(self.k0), C.c_int(quirk),
+                             _p(self.a), _p(self.che), _p(self.znnpp), _p(self.kappa))
+
+    def subset(self, idx):
+        """Cells idx (0-based), keeping their global identifiers."""
+        c = object.__new__(Cells)
+        for f in ("x", "y", "z", "sc", "eps", "a", "che", "znnpp", "kappa", "id"):
+            setattr(c, f, np.ascontiguousarray(getattr(self, f)[idx]))
+        c.n, c.k0, c.quirk = len(c.x), self.k0, self.quirk
+        return c
+
+
+def pi(quirk=1):
+    return lib().or_pi(quirk)
+
+
+def mom_fill(cells: Cells):
+    n3 = 3 * cells.n
+    Z = np.empty((n3, n3), np.complex128, order="F")
+    lib().or_mom_fill(C.c_int(cells.n), _p(cells.x), _p(cells.y), _p(cells.z), _p(cells.kappa), _p(cells.che),
+                      _p(cells.znnpp), C.c_double(cells.k0), C.c_int(cells.quirk), _p(Z))
+    return Z
+
+
+def block_fill(c1: Cells, c2: Cells, fp32=True):
+    Z = np.empty((3 * c1.n, 3 * c2.n), np.complex128, order="F")
+    fn = lib().or_block_fill_f32 if fp32 else lib().or_block_fill_f64
+    fn(C.c_int(c1.n), _p(c1.id), _p(c1.x), _p(c1.y), _p(c1.z), C.c_int(c2.n), _p(c2.id), _p(c2.x), _p(c2.y),
+       _p(c2.z), _p(c2.kappa), _p(c2.che), _p(c2.znnpp), C.c_double(c1.k0), C.c_int(c1.quirk), _p(Z))
+    return Z
+
+
+def mom_rhs(cells: Cells, th_deg, ph_deg):
+    th, ph = _f64(th_deg), _f64(ph_deg)
+    V = np.empty((3 * cells.n, 2 * len(th)), np.complex128, order="F")
+    lib().or_mom_rhs(C.c_int(cells.n), _p(cells.x), _p(cells.y), _p(cells.z), C.c_int(len(th)), _p(th), _p(ph),
+                     C.c_double(cells.k0), C.c_int(cells.quirk), _p(V))
+    return V
+
+
+def incident_field_f32(cells: Cells, th_deg, ph_deg, t0=0, t1=None):
+    th, ph = _f64(th_deg), _f64(ph_deg)
+    t1 = len(th) if t1 is None else t1
+    E = np.empty((3 * cells.n, 2 * (t1 - t0)), np.complex128, order="F")
+    lib().or_incident_field_f32(C.c_int(cells.n), _p(cells.x), _p(cells.y), _p(cells.z), C.c_int(t0), C.c_int(t1),
+                                _p(th), _p(ph), C.c_double(cells.k0), C.c_int(cells.quirk), _p(E))
+    return E
+
+
+def ff_coeff(cells: Cells, th, ph):
+    ff = np.empty(cells.n, np.complex128)
+    lib().or_ff_coeff(C.c_int(cells.n), _p(cells.x), _p(cells.y), _p(cells.z), _p(cells.kappa), _p(cells.che),
+                      C.c_double(th), C.c_double(ph), C.c_double(cells.k0), C.c_int(cells.quirk), _p(ff))
+    return ff
+
+
+def scattering_matrix(cells: Cells, E, rth, rph, beta_deg=(0.0,)):
+    E = np.asfortranarray(E, dtype=np.complex128)
+    ntr = E.shape[1] // 2
+    rth, rph, beta = _f64(rth), _f64(rph), _f64(beta_deg)
+    S = np.empty((len(rth), 4 * ntr * len(beta)), np.complex128, order="F")
+    lib().or_scattering_matrix(C.c_int(cells.n), _p(cells.x), _p(cells.y), _p(cells.z), _p(cells.kappa),
+                               _p(cells.che), _p(E), C.c_int(ntr), C.c_int(len(rth)), _p(rth), _p(rph),
+                               C.c_int(len(beta)), _p(beta), C.c_double(cells.k0), C.c_int(cells.quirk), _p(S))
+    return S
+
+
+def cext_cabs(cells: Cells, E, th_deg, ph_deg, q6=1):
+    E = np.asfortranarray(E, dtype=np.complex128)
+    th, ph = _f64(th_deg), _f64(ph_deg)
+    ce = np.empty(len(th), np.complex128)
+    ca = np.empty(len(th), np.complex128)
+    lib().or_cext_cabs(C.c_int(cells.n), _p(cells.x), _p(cells.y), _p(cells.z), _p(cells.sc), _p(cells.che), _p(E),
+                       C.c_int(len(th)), _p(th), _p(ph), C.c_double(cells.k0), C.c_int(cells.quirk), C.c_int(q6),
+                       _p(ce), _p(ca))
+    return ce, ca
+
+
+def aca(cells: Cells, r0, nr, c0, nc, kmax=50, eps=1e-4, fp32=True):
+    """ACA of the block rows cells[r0:r0+nr] x columns cells[c0:c0+nc] (0-based)."""
+    U = np.zeros((3 * nr, kmax), np.complex128, order="F")
+    V = np.zeros((kmax, 3 * nc), np.complex128, order="F")
+    rows = np.zeros(kmax, np.int32)
+    cols = np.zeros(kmax, np.int32)
+    k = lib().or_aca(C.c_int(r0), C.c_int(nr), C.c_int(c0), C.c_int(nc), _p(cells.x), _p(cells.y), _p(cells.z),
+                     _p(cells.kappa), _p(cells.che), _p(cells.znnpp), C.c_double(cells.k0), C.c_int(cells.quirk),
+                     C.c_int(0 if fp32 else 1), C.c_int(kmax), C.c_double(eps), _p(U), _p(V), _p(rows), _p(cols))
+    kk = min(k, kmax)
+    return k, U[:, :kk], V[:kk, :], rows[:kk], cols[:kk]
+
+
+def aca_gen(A, kmax=50, eps=1e-4):
+    A = np.asfortranarray(A, dtype=np.complex128)
+    m, n = A.shape
+    U = np.zeros((m, kmax), np.complex128, order="F")
+    V = np.zeros((kmax, n), np.complex128, order="F")
+    k = lib().or_aca_gen(C.c_int(m), C.c_int(n), _p(A), C.c_int(kmax), C.c_double(eps), _p(U), _p(V))
+    kk = min(k, kmax)
+    return k, U[:, :kk], V[:kk, :]
+
+
+# --------------------------------------------------------------------------
+# Dense solve and CBFM pipeline (numpy restatement)
+# --------------------------------------------------------------------------
+
+def solve(Z, B):
+    """PZGESV semantics (Compute_EFields_MoM.f90:173): LU with partial pivoting."""
+    return np.linalg.solve(Z, B)
+
+
+def cbf_threshold(eps_cells, aca_variant):
+    """Compute_EFields_CBFME.f90:286-288 and Compute_EFields_CBFME_ACA.f90:124."""
+    if aca_variant:
+        return 1e-3
+    mmax = np.max(np.abs(np.sqrt(np.asarray(eps_cells))))
+    return max(10.0 ** (-3.0 - (np.ceil(mmax) - 2.0)), 1e-10)
+
+
+def cbfm(cells: Cells, blocks, ext, th_cbf, ph_cbf, th, ph, use_aca=False, kmax=50, eps_aca=1e-4,
+         threshold=None, fp32_blocks=True):
+    """Dense-block CBFM-E (Compute_EFields_CBFME.f90:348-449, 630-849, 896-1006)
+    or its ACA variant (Compute_EFields_CBFME_ACA.f90:296-385, 569-636).
+
+    blocks: list of 0-based cell-index arrays (contiguous ranges in block order);
+    ext: list of extension cell-index arrays.  Returns (E, K per block, C list).
+    """
+    if threshold is None:
+        threshold = cbf_threshold(cells.eps, use_aca)
+    Cs, Ks = [], []
+    for b, e in zip(blocks, ext):
+        idx = np.concatenate([b, e]).astype(np.int64)
+        cb = cells.subset(idx)
+        Zb = block_fill(cb, cb, fp32=fp32_blocks)
+        Eref = incident_field_f32(cb, th_cbf, ph_cbf)
+        Eb = np.linalg.solve(Zb, Eref)
+        U, s, _ = np.linalg.svd(Eb, full_matrices=False)
+        K = int(np.sum(s / s[0] >= threshold))
+        Cs.append(np.asfortranarray(U[: 3 * len(b), :K]))
+        Ks.append(K)
+    offs = np.concatenate([[0], np.cumsum(Ks)])
+    Kt = offs[-1]
+    Zc = np.zeros((Kt, Kt), np.complex128)
+    for i, bi in enumerate(blocks):
+        ci = cells.subset(bi)
+        for j, bj in enumerate(blocks):
+            cj = cells.subset(bj)
+            if use_aca and i != j:
+                k, U, V, _, _ = aca(cells, int(bi[0]), len(bi), int(bj[0]), len(bj), kmax, eps_aca)
+                if k < kmax:
+                    Zc[offs[i]:offs[i + 1], offs[j]:offs[j + 1]] = Cs[i].T @ (U @ (V @ Cs[j]))
+                    continue
+            Zij = block_fill(ci, cj, fp32=fp32_blocks)
+            Zc[offs[i]:offs[i + 1], offs[j]:offs[j + 1]] = Cs[i].T @ Zij @ Cs[j]
+    Vc = np.zeros((Kt, 2 * len(th)), np.complex128)
+    for i, bi in enumerate(blocks):
+        Vc[offs[i]:offs[i + 1]] = Cs[i].T @ incident_field_f32(cells.subset(bi), th, ph)
+    alpha = np.linalg.solve(Zc, Vc)
+    E = np.zeros((3 * cells.n, 2 * len(th)), np.complex128)
+    for i, bi in enumerate(blocks):
+        rows = (3 * np.asarray(bi)[:, None] + np.arange(3)[None, :]).ravel()
+        E[rows] = Cs[i] @ alpha[offs[i]:offs[i + 1]]
+    return E, Ks, Cs
+
+
+# --------------------------------------------------------------------------
+# The reference's own routines (oracle/_ref), same calling shapes as above
+# --------------------------------------------------------------------------
+
+def _ref_globals(r, k0, nbc=1, ntr=1, homogs=0, nb_it=50, eps_aca=1e-4):
+    r.ref_set_globals(C.c_double(k0), C.c_int(nbc), C.c_int(ntr), C.c_int(homogs), C.c_int(nb_it),
+                      C.c_double(eps_aca))
+
+
+def ref_cell_params(sc, eps, k0):
+    r = ref()
+    _ref_globals(r, k0)
+    sc, eps = _f64(sc), _c128(eps)
+    n = len(sc)
+    a, kap = np.empty(n), np.empty(n)
+    che, zn = np.empty(n, np.complex128), np.empty(n, np.complex128)
+    r.ref_cell_params(C.c_int(n), _p(sc), _p(eps), _p(a), _p(che), _p(zn), _p(kap))
+    return a, che, zn, kap
+
+
+def ref_block_fill(c1: Cells, c2: Cells):
+    r = ref()
+    _ref_globals(r, c1.k0)
+    Z = np.empty((3 * c1.n, 3 * c2.n), np.complex128, order="F")
+    r.ref_block_fill(C.c_int(c1.n), _p(c1.id), _p(c1.x), _p(c1.y), _p(c1.z), C.c_int(c2.n), _p(c2.id), _p(c2.x),
+                     _p(c2.y), _p(c2.z), _p(c2.kappa), _p(c2.che), _p(c2.znnpp), _p(Z))
+    return Z
+
+
+def ref_total_fill(c: Cells):
+    r = ref()
+    _ref_globals(r, c.k0, nbc=c.n)
+    Z = np.empty((3 * c.n, 3 * c.n), np.complex128, order="F")
+    r.ref_total_fill(C.c_int(c.n), _p(c.x), _p(c.y), _p(c.z), _p(c.kappa), _p(c.che), _p(c.znnpp), _p(Z))
+    return Z
+
+
+def ref_block_row(c: Cells, grow1, col_inf1, ncols):
+    r = ref()
+    _ref_globals(r, c.k0, nbc=c.n)
+    row = np.empty(ncols, np.complex128)
+    r.ref_block_row(C.c_int(c.n), _p(c.x), _p(c.y), _p(c.z), _p(c.kappa), _p(c.che), C.c_int(grow1),
+                    C.c_int(col_inf1), C.c_int(ncols), _p(row))
+    return row
+
+
+def ref_block_col(c: Cells, row_inf1, nrows, gcol1):
+    r = ref()
+    _ref_globals(r, c.k0, nbc=c.n)
+    col = np.empty(nrows, np.complex128)
+    r.ref_block_col(C.c_int(c.n), _p(c.x), _p(c.y), _p(c.z), _p(c.kappa), _p(c.che), C.c_int(row_inf1),
+                    C.c_int(nrows), C.c_int(gcol1), _p(col))
+    return col
+
+
+def ref_incident_field(c: Cells, th, ph, t0=1, t1=None):
+    r = ref()
+    th, ph = _f64(th), _f64(ph)
+    t1 = len(th) if t1 is None else t1
+    _ref_globals(r, c.k0, nbc=c.n, ntr=len(th))
+    E = np.empty((3 * c.n, 2 * (t1 - t0 + 1)), np.complex128, order="F")
+    r.ref_incident_field(C.c_int(c.n), _p(c.x), _p(c.y), _p(c.z), C.c_int(len(th)), _p(th), _p(ph), C.c_int(t0),
+                         C.c_int(t1), _p(E))
+    return E
+
+
+def ref_ff_coeff(c: Cells, th, ph):
+    r = ref()
+    _ref_globals(r, c.k0, nbc=c.n)
+    ff = np.empty(c.n, np.complex128)
+    r.ref_ff_coeff(C.c_int(c.n), _p(c.x), _p(c.y), _p(c.z), _p(c.kappa), _p(c.che), C.c_double(th), C.c_double(ph),
+                   _p(ff))
+    return ff
+
+
+def ref_aca(c: Cells, r0, nr, c0, nc, kmax=50, eps=1e-4):
+    """Reference ACA of cells[r0:r0+nr] x cells[c0:c0+nc] (0-based cell ranges)."""
+    r = ref()
+    _ref_globals(r, c.k0, nbc=c.n, nb_it=kmax, eps_aca=eps)
+    U = np.zeros((3 * nr, kmax), np.complex128, order="F")
+    V = np.zeros((kmax, 3 * nc), np.complex128, order="F")
+    k = C.c_int(0)
+    r.ref_aca(C.c_int(c.n), _p(c.x), _p(c.y), _p(c.z), _p(c.kappa), _p(c.che), C.c_int(3 * r0 + 1),
+              C.c_int(3 * c0 + 1), C.c_int(3 * nr), C.c_int(3 * nc), C.byref(k), _p(U), _p(V))
+    kk = min(k.value, kmax)
+    return k.value, U[:, :kk], V[:kk, :]
+
+
+def ref_aca_gen(A, kmax=50, eps=1e-4):
+    r = ref()
+    _ref_globals(r, 1.0, nb_it=kmax, eps_aca=eps)
+    A = np.asfortranarray(A, dtype=np.complex128)
+    m, n = A.shape
+    U = np.zeros((m, kmax), np.complex128, order="F")
+    V = np.zeros((kmax, n), np.complex128, order="F")
+    k = C.c_int(0)
+    r.ref_aca_gen(C.c_int(m), C.c_int(n), _p(A), C.c_int(kmax), C.byref(k), _p(U), _p(V))
+    kk = min(k.value, kmax)
+    return k.value, U[:, :kk], V[:kk, :]

@@ -1,0 +1,229 @@
+! C-callable driver around the reference's own hot-path routines.
+!
+! TEST INFRASTRUCTURE ONLY.  It is compiled by oracle/Makefile.ref together
+! with the unmodified reference sources (Initialization.f90, Common_variables.f90,
+! SetCellsParams.f90, Green_functions.f90, Incident_Field.f90,
+! Calcul_ACA_matrices.f90, read in place under the reference tree) into
+! oracle/_ref/libmidas_ref.so, which tests use to pin the oracle.  Nothing in
+! the product loads it.
+!
+! Each wrapper takes plain arrays (SoA cells, interleaved complex128), builds
+! the reference's `Cell` / `Dipole` records, sets the module globals the
+! routine reads, and calls the routine unchanged.
+module midas_ref_driver
+    use iso_c_binding
+    use Initialization
+    use common_variables
+    implicit none
+contains
+
+    subroutine build_cells(n, id, x, y, z, sc, kappa, che, znnpp, cells)
+        integer, intent(in) :: n
+        integer(c_int), intent(in) :: id(n)
+        real(c_double), intent(in) :: x(n), y(n), z(n), sc(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n), znnpp(n)
+        type(Cell), intent(out) :: cells(n)
+        integer :: c
+        do c = 1, n
+            cells(c)%n_cell = id(c)
+            cells(c)%n_block = 1
+            cells(c)%n_diel = 1
+            cells(c)%Dlamb_n = 0
+            cells(c)%lambda_n = 0d0
+            cells(c)%Sc = sc(c)
+            cells(c)%Xc = x(c)
+            cells(c)%Yc = y(c)
+            cells(c)%Zc = z(c)
+            cells(c)%a_n = 0d0
+            cells(c)%kappa_n = kappa(c)
+            cells(c)%m_n = (0d0, 0d0)
+            cells(c)%Eps_n = che(c) + 1d0
+            cells(c)%Che_n = che(c)
+            cells(c)%Znnpp = znnpp(c)
+        end do
+    end subroutine build_cells
+
+    subroutine ref_set_globals(k0, nbc, ntr, homogs_in, nb_it, eps_aca) bind(c, name="ref_set_globals")
+        real(c_double), value :: k0, eps_aca
+        integer(c_int), value :: nbc, ntr, homogs_in, nb_it
+        k_0 = k0
+        Nbc = nbc
+        NTr = ntr
+        homogs = homogs_in
+        Nb_it_max = nb_it
+        Epsilon_ACA = eps_aca
+        vrb_ACA = 0
+        rank = 0
+        nber_procs = 1
+    end subroutine ref_set_globals
+
+    ! SetCellsParams (SetCellsParams.f90:1-50)
+    subroutine ref_cell_params(n, sc, eps, a_out, che_out, znnpp_out, kappa_out) bind(c, name="ref_cell_params")
+        integer(c_int), value :: n
+        real(c_double), intent(in) :: sc(n)
+        complex(c_double_complex), intent(in) :: eps(n)
+        real(c_double), intent(out) :: a_out(n), kappa_out(n)
+        complex(c_double_complex), intent(out) :: che_out(n), znnpp_out(n)
+        interface
+            subroutine SetCellsParams(SimScatterer, Cells, UCells)
+                use Initialization
+                use common_variables
+                type(Scatterer), intent(inout) :: SimScatterer
+                type(Cell), dimension(Nbc), intent(inout) :: Cells
+                type(Cell), dimension(:), allocatable, intent(out) :: UCells
+            end subroutine SetCellsParams
+        end interface
+        type(Scatterer) :: sim
+        type(Cell), allocatable :: cells(:), ucells(:)
+        integer :: c
+        Nbc = n
+        allocate(cells(n))
+        do c = 1, n
+            cells(c)%n_cell = c
+            cells(c)%Sc = sc(c)
+            cells(c)%Eps_n = eps(c)
+            cells(c)%Xc = 0d0; cells(c)%Yc = 0d0; cells(c)%Zc = 0d0
+        end do
+        call SetCellsParams(sim, cells, ucells)
+        do c = 1, n
+            a_out(c) = ucells(c)%a_n
+            che_out(c) = ucells(c)%Che_n
+            znnpp_out(c) = ucells(c)%Znnpp
+            kappa_out(c) = ucells(c)%Kappa_n
+        end do
+    end subroutine ref_cell_params
+
+    ! Green_s_tr_partial (Green_functions.f90:96-173)
+    subroutine ref_block_fill(n1, id1, x1, y1, z1, n2, id2, x2, y2, z2, kappa2, che2, znnpp2, Z) &
+            bind(c, name="ref_block_fill")
+        integer(c_int), value :: n1, n2
+        integer(c_int), intent(in) :: id1(n1), id2(n2)
+        real(c_double), intent(in) :: x1(n1), y1(n1), z1(n1), x2(n2), y2(n2), z2(n2), kappa2(n2)
+        complex(c_double_complex), intent(in) :: che2(n2), znnpp2(n2)
+        complex(c_double_complex), intent(out) :: Z(3*n1, 3*n2)
+        type(Cell), allocatable :: c1(:), c2(:)
+        real(c_double), allocatable :: zero1(:)
+        complex(c_double_complex), allocatable :: zc1(:)
+        allocate(c1(n1), c2(n2), zero1(n1), zc1(n1))
+        zero1 = 0d0; zc1 = (0d0, 0d0)
+        call build_cells(n1, id1, x1, y1, z1, zero1, zero1, zc1, zc1, c1)
+        call build_cells(n2, id2, x2, y2, z2, kappa2, kappa2, che2, znnpp2, c2)
+        call Green_s_tr_partial(n1, c1, n2, c2, Z)
+    end subroutine ref_block_fill
+
+    ! Green_s_tr_total (Green_functions.f90:21-94)
+    subroutine ref_total_fill(n, x, y, z, kappa, che, znnpp, zmat) bind(c, name="ref_total_fill")
+        integer(c_int), value :: n
+        real(c_double), intent(in) :: x(n), y(n), z(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n), znnpp(n)
+        complex(c_double_complex), intent(out) :: zmat(3*n, 3*n)
+        type(Cell), allocatable :: cc(:)
+        integer(c_int), allocatable :: id(:)
+        integer :: c
+        allocate(cc(n), id(n))
+        id = [(c, c = 1, n)]
+        call build_cells(n, id, x, y, z, kappa, kappa, che, znnpp, cc)
+        Nbc = n
+        call Green_s_tr_total(cc, zmat)
+    end subroutine ref_total_fill
+
+    ! computeBlockRow / computeBlockCol (Green_functions.f90:905-1012, 1119-1225);
+    ! grow/gcol and the cursors are the reference's 1-based global indexes.
+    subroutine ref_block_row(n, x, y, z, kappa, che, grow, col_inf, ncols, row) bind(c, name="ref_block_row")
+        integer(c_int), value :: n, grow, col_inf, ncols
+        real(c_double), intent(in) :: x(n), y(n), z(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n)
+        complex(c_double_complex), intent(out) :: row(1, ncols)
+        type(Cell), allocatable :: cc(:)
+        integer(c_int), allocatable :: id(:)
+        integer :: c
+        allocate(cc(n), id(n))
+        id = [(c, c = 1, n)]
+        call build_cells(n, id, x, y, z, kappa, kappa, che, che, cc)
+        Nbc = n
+        call computeBlockRow(cc, grow, col_inf, ncols, row)
+    end subroutine ref_block_row
+
+    subroutine ref_block_col(n, x, y, z, kappa, che, row_inf, nrows, gcol, col) bind(c, name="ref_block_col")
+        integer(c_int), value :: n, row_inf, nrows, gcol
+        real(c_double), intent(in) :: x(n), y(n), z(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n)
+        complex(c_double_complex), intent(out) :: col(nrows, 1)
+        type(Cell), allocatable :: cc(:)
+        integer(c_int), allocatable :: id(:)
+        integer :: c
+        allocate(cc(n), id(n))
+        id = [(c, c = 1, n)]
+        call build_cells(n, id, x, y, z, kappa, kappa, che, che, cc)
+        Nbc = n
+        call computeBlockCol(cc, row_inf, nrows, gcol, col)
+    end subroutine ref_block_col
+
+    ! Incident_Field (Incident_Field.f90:1-63); transmitters t0..t1 are 1-based.
+    subroutine ref_incident_field(n, x, y, z, ntr, th, ph, t0, t1, E) bind(c, name="ref_incident_field")
+        integer(c_int), value :: n, ntr, t0, t1
+        real(c_double), intent(in) :: x(n), y(n), z(n), th(ntr), ph(ntr)
+        complex(c_double_complex), intent(out) :: E(3*n, 2*(t1-t0+1))
+        type(Cell), allocatable :: cc(:)
+        type(Dipole), allocatable :: tr(:)
+        integer(c_int), allocatable :: id(:)
+        real(c_double), allocatable :: zr(:)
+        complex(c_double_complex), allocatable :: zc(:)
+        integer :: c
+        allocate(cc(n), id(n), tr(ntr), zr(n), zc(n))
+        id = [(c, c = 1, n)]; zr = 0d0; zc = (0d0, 0d0)
+        call build_cells(n, id, x, y, z, zr, zr, zc, zc, cc)
+        do c = 1, ntr
+            tr(c)%theta = th(c)
+            tr(c)%phi = ph(c)
+        end do
+        call Incident_Field(1, n, cc, ntr, tr, t0, t1, E)
+    end subroutine ref_incident_field
+
+    ! GetFFieldCoeff (Green_functions.f90:1399-1438)
+    subroutine ref_ff_coeff(n, x, y, z, kappa, che, th, ph, ff) bind(c, name="ref_ff_coeff")
+        integer(c_int), value :: n
+        real(c_double), value :: th, ph
+        real(c_double), intent(in) :: x(n), y(n), z(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n)
+        complex(c_double_complex), intent(out) :: ff(n)
+        type(Cell), allocatable :: cc(:)
+        integer(c_int), allocatable :: id(:)
+        integer :: c
+        allocate(cc(n), id(n))
+        id = [(c, c = 1, n)]
+        call build_cells(n, id, x, y, z, kappa, kappa, che, che, cc)
+        call GetFFieldCoeff(n, cc, th, ph, ff)
+    end subroutine ref_ff_coeff
+
+    ! Calcul_MatrixZij_ACA (Calcul_ACA_matrices.f90:6-219); curs_* are the
+    ! reference's 1-based global row / column cursors.
+    subroutine ref_aca(n, x, y, z, kappa, che, curs_lig, curs_col, nrow, ncol, nb_iter, U, V) bind(c, name="ref_aca")
+        integer(c_int), value :: n, curs_lig, curs_col, nrow, ncol
+        real(c_double), intent(in) :: x(n), y(n), z(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n)
+        integer(c_int), intent(out) :: nb_iter
+        complex(c_double_complex), intent(out) :: U(nrow, Nb_it_max), V(Nb_it_max, ncol)
+        type(Cell), allocatable :: cc(:)
+        integer(c_int), allocatable :: id(:)
+        integer :: c, k
+        allocate(cc(n), id(n))
+        id = [(c, c = 1, n)]
+        call build_cells(n, id, x, y, z, kappa, kappa, che, che, cc)
+        Nbc = n
+        call Calcul_MatrixZij_ACA(cc, 1, 2, curs_lig, curs_col, nrow, ncol, k, U, V)
+        nb_iter = k
+    end subroutine ref_aca
+
+    ! Calcul_GenMatrix_ACA (Calcul_ACA_matrices.f90:222-424)
+    subroutine ref_aca_gen(m, n, A, nmax, nout, U, V) bind(c, name="ref_aca_gen")
+        integer(c_int), value :: m, n, nmax
+        complex(c_double_complex), intent(in) :: A(m, n)
+        integer(c_int), intent(out) :: nout
+        complex(c_double_complex), intent(out) :: U(m, nmax), V(nmax, n)
+        integer :: k
+        call Calcul_GenMatrix_ACA(m, n, A, nmax, k, U, V)
+        nout = k
+    end subroutine ref_aca_gen
+
+end module midas_ref_driver
