@@ -1,0 +1,183 @@
+/*
+ * midas_gpu.h -- C-ABI of the MI355X (gfx950) implementation of the MIDAS
+ * volume-integral-equation MoM hot path: dyadic Green's-function fill, dense
+ * complex solve (FP64 LU, or mixed-precision LU + FP64 iterative refinement),
+ * far-field scattering matrix, Cext/Cabs, ACA and the CBFM compressed solver.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns 0 on success, > 0 for a LAPACK-style singular pivot (1-based index,
+ * as PZGESV's INFO), < 0 for a bad argument or a HIP error; midas_last_error()
+ * gives the message.  Host-pointer entry points are synchronous: results are in
+ * host memory on return.  midas_dev_* entry points take device pointers and are
+ * ordered on the context's stream (midas_stream()).
+ *
+ * Conventions follow the reference (Fortran, column-major):
+ *   - unknown i = 3*cell + p (0-based here), p in {x, y, z};
+ *   - complex numbers are interleaved double (re, im) = Fortran COMPLEX(real64);
+ *   - matrices are column-major with an explicit leading dimension;
+ *   - angles are degrees, theta measured from +x (Incident_Field.f90:31-34);
+ *   - excitation columns 0..ntr-1 are V-pol, ntr..2ntr-1 H-pol
+ *     (Compute_EFields_MoM.f90:557-563).
+ */
+#ifndef MIDAS_GPU_H
+#define MIDAS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct midas_ctx midas_ctx;
+
+/* Quirk bits: reproduce the reference's precision quirks (default: all on). */
+#define MIDAS_Q_PI 1          /* Pi = Acos(-1.) of a single literal (Initialization.f90:39)          */
+#define MIDAS_Q_LITERALS 2    /* single (1./3.), (2./3.) literals (SetCellsParams.f90:37-44)         */
+#define MIDAS_Q_CEXT_SUM 4    /* component-sum Cext/Cabs (Compute_ExtAbsCsec_fromIntField.f90:53-60)  */
+#define MIDAS_Q_FP32_BLOCKS 8 /* FP32 intermediates of Green_s_tr_partial / Incident_Field (CBFM)    */
+#define MIDAS_Q_ALL 15
+
+/* Solver modes for the dense complex systems. */
+#define MIDAS_SOLVE_FP64 0        /* FP64 LU (PZGESV semantics): the plain baseline                 */
+#define MIDAS_SOLVE_C32_IR 1      /* complex-FP32 LU on FP32 MFMA + FP64 iterative refinement       */
+#define MIDAS_SOLVE_F16_IR 2      /* FP32 LU, trailing GEMM on FP16 MFMA (scaled, 1 term) + FP64 IR */
+#define MIDAS_SOLVE_F16X3_IR 3    /* ... FP16 split hi/lo, 3 product terms + FP64 IR                */
+#define MIDAS_SOLVE_BF16_IR 4     /* ... BF16, 1 term + FP64 IR                                     */
+#define MIDAS_SOLVE_BF16X3_IR 5   /* ... BF16 split hi/lo, 3 product terms + FP64 IR                */
+
+/* Cell table in structure-of-arrays form (the fields of the reference's
+ * `type Cell`, Initialization.f90:22-26, that the hot path reads).  If kappa,
+ * che and znnpp are NULL they are computed from sc and eps as SetCellsParams
+ * does; if given, they are used as they are.  id (n_cell) may be NULL (1..n). */
+typedef struct {
+    int n;
+    const double *x, *y, *z;  /* centres [m]                         */
+    const double *sc;         /* cell edge Sc [m]                    */
+    const double *eps;        /* relative permittivity, 2n doubles   */
+    const double *kappa;      /* Kappa_n (optional)                  */
+    const double *che;        /* Che_n = eps - 1, 2n (optional)      */
+    const double *znnpp;      /* Znnpp, 2n (optional)                */
+    const int *id;            /* n_cell identifiers (optional)       */
+} midas_cells_t;
+
+typedef struct {
+    int max_iter;      /* refinement sweeps (default 30)                                      */
+    double tol;        /* stop when the normwise backward error is <= tol (default 1e-14)    */
+    int nb;            /* LU panel width (0: default)                                        */
+} midas_solve_opts_t;
+
+typedef struct {
+    int info;          /* 0, or first exactly-zero pivot (1-based)                            */
+    int iterations;    /* refinement sweeps done (0 for MIDAS_SOLVE_FP64)                     */
+    double berr;       /* final max_j ||b_j - Z x_j||_inf / (||Z||_inf ||x_j||_inf + ||b_j||_inf) */
+    double anorm;      /* ||Z||_inf (PZLANGE 'I', Compute_EFields_MoM.f90:168)                */
+    double t_factor_ms, t_solve_ms, t_refine_ms;
+} midas_solve_info_t;
+
+/* ---------------- context ---------------- */
+int midas_init(int device, midas_ctx **ctx);
+int midas_finalize(midas_ctx *ctx);
+const char *midas_last_error(void);
+void *midas_stream(midas_ctx *ctx);           /* hipStream_t of the context */
+
+/* ---------------- a1: cell constants ----------------
+ * SetCellsParams (SetCellsParams.f90:37-48): a_n, Che_n, Znnpp, Kappa_n. */
+int midas_cell_params(midas_ctx *ctx, int n, const double *sc, const double *eps, double k0, int quirks,
+                      double *a_n, double *che, double *znnpp, double *kappa);
+
+/* ---------------- a2: MoM matrix ----------------
+ * Z (3n x 3n, ldz) = compute_Green_s_tr_single_element for every (row, column)
+ * (Compute_EFields_MoM.f90:138-145, 433-523), FP64. */
+int midas_mom_fill(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, double *Z, long ldz);
+
+/* ---------------- a7: block matrix ----------------
+ * Green_s_tr_partial (Green_functions.f90:96-173): Z (3*b1.n x 3*b2.n) between
+ * receiver cells b1 and source cells b2; diagonal blocks where identifiers
+ * coincide.  quirks & MIDAS_Q_FP32_BLOCKS reproduces the FP32 intermediates. */
+int midas_block_fill(midas_ctx *ctx, const midas_cells_t *b1, const midas_cells_t *b2, double k0, int quirks,
+                     double *Z, long ldz);
+
+/* ---------------- a3 / a10: plane-wave excitations ----------------
+ * V (3n x 2ntr, ldv).  ref32 = 0: Incident_Field_single_element
+ * (Compute_EFields_MoM.f90:529-598, FP64); ref32 = 1: Incident_Field
+ * (Incident_Field.f90:1-63, FP32 locals). */
+int midas_plane_wave(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int ntr, const double *th,
+                     const double *ph, int ref32, double *V, long ldv);
+
+/* ---------------- a4: dense solve ----------------
+ * PZGESV (Compute_EFields_MoM.f90:173): X = Z^-1 B, overwriting B (n x nrhs).
+ * Z is not modified. */
+int midas_solve(midas_ctx *ctx, int n, int nrhs, const double *Z, long ldz, double *B, long ldb, int mode,
+                const midas_solve_opts_t *opts, midas_solve_info_t *info);
+
+/* ---------------- a5: far-field scattering matrix ----------------
+ * GetFFieldCoeff + contraction + projection (Green_functions.f90:1399-1438,
+ * Compute_EFields_MoM.f90:306-369, Compute_Scattering_Matrices.f90:122-207).
+ * E (3n x 2ntr, lde); S (nrx x 4*ntr*npol, lds), column 4*npol*t + 4*ip +
+ * {Vv, Vh, Hv, Hh}; npol = 1 with beta = 0 is the MoM S_total layout. */
+int midas_scattering_matrix(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, const double *E,
+                            long lde, int ntr, int nrx, const double *rth, const double *rph, int npol,
+                            const double *beta, double *S, long lds);
+
+/* ---------------- a6: extinction / absorption ----------------
+ * Compute_ExtAbsCsec_fromIntField (Compute_ExtAbsCsec_fromIntField.f90:45-64);
+ * C_ext, C_abs are complex (2*ntr doubles) as in the reference. */
+int midas_cext_cabs(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, const double *E, long lde,
+                    int ntr, const double *th, const double *ph, double *C_ext, double *C_abs);
+
+/* ---------------- Compute_EFields_MoM drop-in ----------------
+ * Compute_EFields_MoM(Cells, Transmitters, Receivers, S_total, C_ext, C_abs)
+ * (Compute_EFields_MoM.f90:1-19): fill, solve, S_total (nrx x 4ntr),
+ * C_ext/C_abs (ntr).  E (3n x 2ntr) is returned when non-NULL. */
+int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int ntr,
+                              const double *tr_th, const double *tr_ph, int nrx, const double *rx_th,
+                              const double *rx_ph, int mode, const midas_solve_opts_t *opts, double *S_total,
+                              double *C_ext, double *C_abs, double *E, midas_solve_info_t *info);
+
+/* ---------------- a14: ACA of one off-diagonal block ----------------
+ * Calcul_MatrixZij_ACA (Calcul_ACA_matrices.f90:6-219): rows of cells
+ * [r0, r0+nr), columns of cells [c0, c0+nc) of the full matrix over `cells`,
+ * generated on the fly.  U (3nr x kmax), V (kmax x 3nc); *k_out as the
+ * reference's nb_iter_out (kmax+1 when not converged). */
+int midas_aca_block(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int r0, int nr, int c0,
+                    int nc, int kmax, double eps, double *U, double *V, int *k_out);
+
+/* ---------------- a16: ACA of an explicit matrix ----------------
+ * Calcul_GenMatrix_ACA (Calcul_ACA_matrices.f90:222-424). */
+int midas_aca_matrix(midas_ctx *ctx, int m, int n, const double *A, long lda, int kmax, double eps, double *U,
+                     double *V, int *k_out);
+
+/* ---------------- a7-a15: CBFM-E (dense blocks) and CBFM-E + ACA ----------------
+ * Compute_EFields_CBFME (Compute_EFields_CBFME.f90:1-1143) with the dense
+ * block solve (SR = 0), and Compute_EFields_CBFME_ACA (use_aca = 1,
+ * Compute_EFields_CBFME_ACA.f90:1-1017).  Blocks are contiguous cell ranges
+ * block_start[b] .. block_start[b+1]-1 (cells already in block order, as
+ * Division_blocks leaves them); extension cells of block b are
+ * ext_cells[ext_start[b] .. ext_start[b+1]-1].  CBF excitations are
+ * ntr_cbf plane waves (th_cbf, ph_cbf), the problem excitations ntr plane
+ * waves.  threshold <= 0 selects the reference's rule.  E (3n x 2ntr);
+ * K_out[b] = number of CBFs of block b. */
+typedef struct {
+    int nblocks;
+    const int *block_start;   /* nblocks + 1 */
+    const int *ext_start;     /* nblocks + 1 */
+    const int *ext_cells;     /* 0-based cell indices */
+} midas_blocks_t;
+
+int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks, double k0,
+                               int quirks, int ntr_cbf, const double *th_cbf, const double *ph_cbf, int ntr,
+                               const double *tr_th, const double *tr_ph, int use_aca, int kmax, double eps_aca,
+                               double threshold, int mode, double *E, int *K_out, midas_solve_info_t *info);
+
+/* ---------------- device-pointer variants (inputs resident in HBM) ----------------
+ * Same semantics as above; every pointer is device memory; ordered on
+ * midas_stream(ctx).  Z may be NULL in midas_dev_mom_solve when only the
+ * FP32 copy is needed (mixed modes then fill W directly). */
+int midas_dev_mom_fill(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, double *Z, long ldz,
+                       float *W32, long ldw);
+int midas_dev_mom_solve(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, int ntr,
+                        const double *dth, const double *dph, int mode, const midas_solve_opts_t *opts, double *Z,
+                        long ldz, double *X, long ldx, midas_solve_info_t *info, double *t_fill_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MIDAS_GPU_H */

@@ -1,0 +1,34 @@
+/*
+ * midas_gpu_test.h -- kernel-level entry points used by the test suite to
+ * check the building blocks of the solver in isolation (no reference
+ * counterpart; not part of the drop-in interface).  Host pointers,
+ * synchronous, same return convention as midas_gpu.h.
+ */
+#ifndef MIDAS_GPU_TEST_H
+#define MIDAS_GPU_TEST_H
+#include "midas_gpu.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* C = alpha op(A) op(B) + beta C, complex, column-major; prec 64 (ZGEMM on
+ * FP64 MFMA) or 32 (CGEMM on FP32 MFMA, operands given as complex128 and
+ * rounded); transA/transB: 0 = N, 1 = T. */
+int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double *alpha, const double *A, long lda,
+                    int transA, const double *B, long ldb, int transB, const double *beta, double *C, long ldc);
+
+/* C -= A B with the split FP16/BF16 trailing-update kernel (mode as
+ * TrailingMode: 1 f16, 3 f16x3, 11 bf16, 13 bf16x3); operands complex128,
+ * rounded to complex FP32 before the call. */
+int midas_test_trailing(midas_ctx *ctx, int mode, int m, int n, int k, const double *A, long lda, const double *B,
+                        long ldb, double *C, long ldc);
+
+/* Matrix-core lane layout probe: which = 0 (f64 16x16x4), 1 (f32 16x16x4),
+ * 2 (f16 16x16x32), 3 (bf16 16x16x32).  out receives the 16x16 product of
+ * A(i,k) = i + 1 + 0.5 k and B(k,j) = (k + 1) + 0.25 j (column-major 16x16). */
+int midas_test_mfma_probe(midas_ctx *ctx, int which, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

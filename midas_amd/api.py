@@ -1,0 +1,321 @@
+"""Host-side mirror of the MIDAS hot-path interfaces over libmidas_gpu.so.
+
+Names follow the reference routines they replace:
+  compute_efields_mom      Compute_EFields_MoM        (Compute_EFields_MoM.f90:1-19)
+  compute_efields_cbfme    Compute_EFields_CBFME(_ACA) (Compute_EFields_CBFME.f90:1-21,
+                                                        Compute_EFields_CBFME_ACA.f90:1-18)
+  calcul_matrixzij_aca     Calcul_MatrixZij_ACA       (Calcul_ACA_matrices.f90:6-20)
+  calcul_genmatrix_aca     Calcul_GenMatrix_ACA       (Calcul_ACA_matrices.f90:222-236)
+  set_cells_params         SetCellsParams             (SetCellsParams.f90:1-50)
+  green_s_tr_partial       Green_s_tr_partial         (Green_functions.f90:96-173)
+  incident_field           Incident_Field             (Incident_Field.f90:1-63)
+plus the building blocks (mom_fill, solve, scattering_matrix, cext_cabs).
+
+Every call runs on the GPU through the C-ABI declared in include/midas_gpu.h;
+there is no CPU fallback: if the library is missing or no GPU is present the
+calls raise.  Arrays are numpy, complex128 where the reference uses
+COMPLEX(real64), matrices Fortran-ordered.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmidas_gpu.so")
+
+Q_PI, Q_LITERALS, Q_CEXT_SUM, Q_FP32_BLOCKS = 1, 2, 4, 8
+Q_ALL = 15
+
+SOLVE_FP64, SOLVE_C32_IR, SOLVE_F16_IR, SOLVE_F16X3_IR, SOLVE_BF16_IR, SOLVE_BF16X3_IR = range(6)
+MODE_NAMES = {SOLVE_FP64: "fp64", SOLVE_C32_IR: "c32+ir", SOLVE_F16_IR: "f16+ir", SOLVE_F16X3_IR: "f16x3+ir",
+              SOLVE_BF16_IR: "bf16+ir", SOLVE_BF16X3_IR: "bf16x3+ir"}
+
+EXPORTS = (
+    "midas_init", "midas_finalize", "midas_last_error", "midas_stream", "midas_cell_params", "midas_mom_fill",
+    "midas_block_fill", "midas_plane_wave", "midas_solve", "midas_scattering_matrix", "midas_cext_cabs",
+    "midas_compute_efields_mom", "midas_aca_block", "midas_aca_matrix", "midas_compute_efields_cbfm",
+    "midas_dev_mom_fill", "midas_dev_mom_solve",
+)
+
+
+class MidasError(RuntimeError):
+    pass
+
+
+class cells_t(C.Structure):
+    _fields_ = [("n", C.c_int), ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("sc", C.c_void_p),
+                ("eps", C.c_void_p), ("kappa", C.c_void_p), ("che", C.c_void_p), ("znnpp", C.c_void_p),
+                ("id", C.c_void_p)]
+
+
+class solve_opts_t(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("nb", C.c_int)]
+
+
+class solve_info_t(C.Structure):
+    _fields_ = [("info", C.c_int), ("iterations", C.c_int), ("berr", C.c_double), ("anorm", C.c_double),
+                ("t_factor_ms", C.c_double), ("t_solve_ms", C.c_double), ("t_refine_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class blocks_t(C.Structure):
+    _fields_ = [("nblocks", C.c_int), ("block_start", C.c_void_p), ("ext_start", C.c_void_p),
+                ("ext_cells", C.c_void_p)]
+
+
+_LIB = None
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise MidasError(f"{LIBPATH} not built: run `python -m midas_amd.build`")
+        lib = C.CDLL(LIBPATH)
+        lib.midas_last_error.restype = C.c_char_p
+        lib.midas_stream.restype = C.c_void_p
+        _LIB = lib
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(r, what):
+    if r < 0:
+        raise MidasError(f"{what} failed ({r}): {load().midas_last_error().decode()}")
+    return r
+
+
+class CellTable:
+    """SoA cell table (reference `type Cell` fields the hot path reads)."""
+
+    def __init__(self, x, y, z, sc, eps, kappa=None, che=None, znnpp=None, ids=None):
+        f = lambda a: np.ascontiguousarray(a, np.float64)
+        self.x, self.y, self.z, self.sc = f(x), f(y), f(z), f(sc)
+        self.eps = np.ascontiguousarray(eps, np.complex128)
+        self.kappa = None if kappa is None else f(kappa)
+        self.che = None if che is None else np.ascontiguousarray(che, np.complex128)
+        self.znnpp = None if znnpp is None else np.ascontiguousarray(znnpp, np.complex128)
+        self.ids = None if ids is None else np.ascontiguousarray(ids, np.int32)
+        self.n = len(self.x)
+
+    @classmethod
+    def from_scatterer(cls, s):
+        return cls(s.x, s.y, s.z, s.sc, s.eps)
+
+    def subset(self, idx):
+        idx = np.asarray(idx)
+        g = lambda a: None if a is None else a[idx]
+        ids = (idx + 1) if self.ids is None else self.ids[idx]
+        return CellTable(self.x[idx], self.y[idx], self.z[idx], self.sc[idx], self.eps[idx], g(self.kappa),
+                         g(self.che), g(self.znnpp), ids)
+
+    def struct(self):
+        s = cells_t(self.n, _p(self.x), _p(self.y), _p(self.z), _p(self.sc), _p(self.eps), _p(self.kappa),
+                    _p(self.che), _p(self.znnpp), _p(self.ids))
+        s._keep = self
+        return s
+
+
+def _cells(c):
+    return c if isinstance(c, CellTable) else CellTable.from_scatterer(c)
+
+
+class Midas:
+    """One context (one GPU, one stream)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.ctx = C.c_void_p()
+        _check(self.lib.midas_init(C.c_int(device), C.byref(self.ctx)), "midas_init")
+
+    def close(self):
+        if self.ctx:
+            self.lib.midas_finalize(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return self.lib.midas_stream(self.ctx)
+
+    # -- a1 ---------------------------------------------------------------
+    def set_cells_params(self, sc, eps, k0, quirks=Q_ALL):
+        sc = np.ascontiguousarray(sc, np.float64)
+        eps = np.ascontiguousarray(eps, np.complex128)
+        n = len(sc)
+        a, kap = np.empty(n), np.empty(n)
+        che, zn = np.empty(n, np.complex128), np.empty(n, np.complex128)
+        _check(self.lib.midas_cell_params(self.ctx, C.c_int(n), _p(sc), _p(eps), C.c_double(k0), C.c_int(quirks),
+                                          _p(a), _p(che), _p(zn), _p(kap)), "cell_params")
+        return a, che, zn, kap
+
+    # -- a2 / a7 ------------------------------------------------------------
+    def mom_fill(self, cells, k0, quirks=Q_ALL):
+        cells = _cells(cells)
+        n3 = 3 * cells.n
+        Z = np.empty((n3, n3), np.complex128, order="F")
+        cs = cells.struct()
+        _check(self.lib.midas_mom_fill(self.ctx, C.byref(cs), C.c_double(k0), C.c_int(quirks), _p(Z), C.c_long(n3)),
+               "mom_fill")
+        return Z
+
+    def green_s_tr_partial(self, c1, c2, k0, quirks=Q_ALL):
+        c1, c2 = _cells(c1), _cells(c2)
+        Z = np.empty((3 * c1.n, 3 * c2.n), np.complex128, order="F")
+        s1, s2 = c1.struct(), c2.struct()
+        _check(self.lib.midas_block_fill(self.ctx, C.byref(s1), C.byref(s2), C.c_double(k0), C.c_int(quirks), _p(Z),
+                                         C.c_long(3 * c1.n)), "block_fill")
+        return Z
+
+    # -- a3 / a10 -----------------------------------------------------------
+    def plane_wave(self, cells, k0, th, ph, ref32=False, quirks=Q_ALL):
+        cells = _cells(cells)
+        th, ph = np.ascontiguousarray(th, np.float64), np.ascontiguousarray(ph, np.float64)
+        V = np.empty((3 * cells.n, 2 * len(th)), np.complex128, order="F")
+        cs = cells.struct()
+        _check(self.lib.midas_plane_wave(self.ctx, C.byref(cs), C.c_double(k0), C.c_int(quirks), C.c_int(len(th)),
+                                         _p(th), _p(ph), C.c_int(int(ref32)), _p(V), C.c_long(3 * cells.n)),
+               "plane_wave")
+        return V
+
+    def incident_field(self, cells, k0, th, ph, quirks=Q_ALL):
+        return self.plane_wave(cells, k0, th, ph, ref32=True, quirks=quirks)
+
+    # -- a4 -----------------------------------------------------------------
+    def solve(self, Z, B, mode=SOLVE_FP64, max_iter=0, tol=0.0, nb=0):
+        Z = np.asfortranarray(Z, np.complex128)
+        X = np.array(B, np.complex128, order="F", copy=True)
+        if X.ndim == 1:
+            X = X.reshape(-1, 1, order="F")
+        n, nrhs = X.shape
+        opts = solve_opts_t(max_iter, tol, nb)
+        info = solve_info_t()
+        r = _check(self.lib.midas_solve(self.ctx, C.c_int(n), C.c_int(nrhs), _p(Z), C.c_long(Z.shape[0]), _p(X),
+                                        C.c_long(n), C.c_int(mode), C.byref(opts), C.byref(info)), "solve")
+        d = info.as_dict()
+        d["info"] = r
+        return X, d
+
+    # -- a5 / a6 ------------------------------------------------------------
+    def scattering_matrix(self, cells, k0, E, rth, rph, beta=(0.0,), quirks=Q_ALL):
+        cells = _cells(cells)
+        E = np.asfortranarray(E, np.complex128)
+        ntr = E.shape[1] // 2
+        rth, rph = np.ascontiguousarray(rth, np.float64), np.ascontiguousarray(rph, np.float64)
+        beta = np.ascontiguousarray(beta, np.float64)
+        S = np.empty((len(rth), 4 * ntr * len(beta)), np.complex128, order="F")
+        cs = cells.struct()
+        _check(self.lib.midas_scattering_matrix(self.ctx, C.byref(cs), C.c_double(k0), C.c_int(quirks), _p(E),
+                                                C.c_long(E.shape[0]), C.c_int(ntr), C.c_int(len(rth)), _p(rth),
+                                                _p(rph), C.c_int(len(beta)), _p(beta), _p(S), C.c_long(len(rth))),
+               "scattering_matrix")
+        return S
+
+    def cext_cabs(self, cells, k0, E, th, ph, quirks=Q_ALL):
+        cells = _cells(cells)
+        E = np.asfortranarray(E, np.complex128)
+        th, ph = np.ascontiguousarray(th, np.float64), np.ascontiguousarray(ph, np.float64)
+        ce = np.empty(len(th), np.complex128)
+        ca = np.empty(len(th), np.complex128)
+        cs = cells.struct()
+        _check(self.lib.midas_cext_cabs(self.ctx, C.byref(cs), C.c_double(k0), C.c_int(quirks), _p(E),
+                                        C.c_long(E.shape[0]), C.c_int(len(th)), _p(th), _p(ph), _p(ce), _p(ca)),
+               "cext_cabs")
+        return ce, ca
+
+    # -- Compute_EFields_MoM -------------------------------------------------
+    def compute_efields_mom(self, cells, k0, transmitters, receivers, mode=SOLVE_FP64, quirks=Q_ALL, want_E=True,
+                            max_iter=0, tol=0.0, nb=0):
+        cells = _cells(cells)
+        tth, tph = (np.ascontiguousarray(a, np.float64) for a in transmitters)
+        rth, rph = (np.ascontiguousarray(a, np.float64) for a in receivers)
+        ntr, nrx = len(tth), len(rth)
+        S = np.empty((nrx, 4 * ntr), np.complex128, order="F")
+        ce = np.empty(ntr, np.complex128)
+        ca = np.empty(ntr, np.complex128)
+        E = np.empty((3 * cells.n, 2 * ntr), np.complex128, order="F") if want_E else None
+        opts = solve_opts_t(max_iter, tol, nb)
+        info = solve_info_t()
+        cs = cells.struct()
+        r = _check(self.lib.midas_compute_efields_mom(self.ctx, C.byref(cs), C.c_double(k0), C.c_int(quirks),
+                                                      C.c_int(ntr), _p(tth), _p(tph), C.c_int(nrx), _p(rth), _p(rph),
+                                                      C.c_int(mode), C.byref(opts), _p(S), _p(ce), _p(ca), _p(E),
+                                                      C.byref(info)), "compute_efields_mom")
+        d = info.as_dict()
+        d["info"] = r
+        return S, ce, ca, E, d
+
+    # -- a14 / a16 ----------------------------------------------------------
+    def calcul_matrixzij_aca(self, cells, k0, r0, nr, c0, nc, kmax=50, eps=1e-4, quirks=Q_ALL):
+        cells = _cells(cells)
+        U = np.zeros((3 * nr, kmax), np.complex128, order="F")
+        V = np.zeros((kmax, 3 * nc), np.complex128, order="F")
+        k = C.c_int(0)
+        cs = cells.struct()
+        _check(self.lib.midas_aca_block(self.ctx, C.byref(cs), C.c_double(k0), C.c_int(quirks), C.c_int(r0),
+                                        C.c_int(nr), C.c_int(c0), C.c_int(nc), C.c_int(kmax), C.c_double(eps), _p(U),
+                                        _p(V), C.byref(k)), "aca_block")
+        kk = min(k.value, kmax)
+        return k.value, U[:, :kk], V[:kk, :]
+
+    def calcul_genmatrix_aca(self, A, kmax=50, eps=1e-4):
+        A = np.asfortranarray(A, np.complex128)
+        m, n = A.shape
+        U = np.zeros((m, kmax), np.complex128, order="F")
+        V = np.zeros((kmax, n), np.complex128, order="F")
+        k = C.c_int(0)
+        _check(self.lib.midas_aca_matrix(self.ctx, C.c_int(m), C.c_int(n), _p(A), C.c_long(m), C.c_int(kmax),
+                                         C.c_double(eps), _p(U), _p(V), C.byref(k)), "aca_matrix")
+        kk = min(k.value, kmax)
+        return k.value, U[:, :kk], V[:kk, :]
+
+    # -- Compute_EFields_CBFME(_ACA) -----------------------------------------
+    def compute_efields_cbfme(self, cells, k0, blocks, ext, cbf_dirs, transmitters, use_aca=False, kmax=50,
+                              eps_aca=1e-4, threshold=0.0, mode=SOLVE_FP64, quirks=Q_ALL):
+        cells = _cells(cells)
+        nb = len(blocks)
+        starts = np.zeros(nb + 1, np.int32)
+        for b, blk in enumerate(blocks):
+            blk = np.asarray(blk)
+            if len(blk) == 0 or np.any(np.diff(blk) != 1):
+                raise MidasError("blocks must be non-empty contiguous cell ranges in block order")
+            if b > 0 and blk[0] != starts[b]:
+                raise MidasError("blocks must tile the cells in order")
+            starts[b + 1] = blk[-1] + 1
+        if starts[-1] != cells.n:
+            raise MidasError("blocks must cover every cell")
+        es = np.zeros(nb + 1, np.int32)
+        for b, e in enumerate(ext):
+            es[b + 1] = es[b] + len(e)
+        ec = np.ascontiguousarray(np.concatenate([np.asarray(e, np.int32) for e in ext]) if es[-1] else
+                                  np.zeros(1, np.int32), np.int32)
+        bt = blocks_t(nb, _p(starts), _p(es), _p(ec))
+        cth, cph = (np.ascontiguousarray(a, np.float64) for a in cbf_dirs)
+        tth, tph = (np.ascontiguousarray(a, np.float64) for a in transmitters)
+        E = np.empty((3 * cells.n, 2 * len(tth)), np.complex128, order="F")
+        K = np.zeros(nb, np.int32)
+        info = solve_info_t()
+        cs = cells.struct()
+        r = _check(self.lib.midas_compute_efields_cbfm(self.ctx, C.byref(cs), C.byref(bt), C.c_double(k0),
+                                                       C.c_int(quirks), C.c_int(len(cth)), _p(cth), _p(cph),
+                                                       C.c_int(len(tth)), _p(tth), _p(tph), C.c_int(int(use_aca)),
+                                                       C.c_int(kmax), C.c_double(eps_aca), C.c_double(threshold),
+                                                       C.c_int(mode), _p(E), _p(K), C.byref(info)),
+                   "compute_efields_cbfm")
+        d = info.as_dict()
+        d["info"] = r
+        return E, K, d

@@ -1,0 +1,845 @@
+// C-ABI of the MIDAS gfx950 library (declared in include/midas_gpu.h).
+#include "midas_common.h"
+#include "midas_lu.h"
+#include "../../include/midas_gpu.h"
+#include "../../include/midas_gpu_test.h"
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace midas {
+
+static thread_local char g_err[1024] = "";
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+const char *last_error() { return g_err; }
+
+// ---------------------------------------------------------------------------
+// small device buffer owner
+// ---------------------------------------------------------------------------
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    int alloc(size_t b)
+    {
+        release();
+        if (b == 0) b = 16;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("hipMalloc(%zu bytes): %s", b, hipGetErrorString(e));
+            return -100 - (int)e;
+        }
+        bytes = b;
+        return 0;
+    }
+    void release()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename X> X *as() const { return (X *)p; }
+};
+
+#define TRY(x)                  \
+    do {                        \
+        int r_ = (x);           \
+        if (r_) return r_;      \
+    } while (0)
+
+// blocking copy ordered after the work already queued on st
+static int copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t st)
+{
+    MIDAS_HIP(hipMemcpyAsync(dst, src, bytes, kind, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+static int upload(DBuf &b, const void *h, size_t bytes, hipStream_t st)
+{
+    TRY(b.alloc(bytes));
+    if (h && bytes) MIDAS_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, st));
+    return 0;
+}
+
+}  // namespace midas
+
+using namespace midas;
+
+struct midas_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+};
+
+// Device-resident cell table with derived constants.
+struct DevCellsOwned {
+    DBuf x, y, z, sc, eps, kappa, che, znnpp, kchi, id, a;
+    DevCells v{};
+    // host cells -> device; constants computed on the GPU unless supplied
+    int from_host(const midas_cells_t *c, double k0, int quirks, hipStream_t st)
+    {
+        MIDAS_CHECK(c && c->n > 0 && c->x && c->y && c->z, -1, "cells: empty or missing coordinates");
+        const int n = c->n;
+        const size_t d = sizeof(double) * n, z2 = 2 * d;
+        TRY(upload(x, c->x, d, st));
+        TRY(upload(y, c->y, d, st));
+        TRY(upload(this->z, c->z, d, st));
+        TRY(upload(sc, c->sc, d, st));
+        std::vector<int> ids;
+        if (!c->id) {
+            ids.resize(n);
+            for (int i = 0; i < n; ++i) ids[i] = i + 1;
+        }
+        TRY(upload(id, c->id ? (const void *)c->id : (const void *)ids.data(), sizeof(int) * n, st));
+        TRY(kappa.alloc(d));
+        TRY(che.alloc(z2));
+        TRY(znnpp.alloc(z2));
+        TRY(kchi.alloc(z2));
+        TRY(a.alloc(d));
+        if (c->kappa && c->che && c->znnpp) {
+            MIDAS_HIP(hipMemcpyAsync(kappa.p, c->kappa, d, hipMemcpyHostToDevice, st));
+            MIDAS_HIP(hipMemcpyAsync(che.p, c->che, z2, hipMemcpyHostToDevice, st));
+            MIDAS_HIP(hipMemcpyAsync(znnpp.p, c->znnpp, z2, hipMemcpyHostToDevice, st));
+            std::vector<double> kc(2 * n);
+            for (int i = 0; i < n; ++i) {
+                kc[2 * i] = c->kappa[i] * c->che[2 * i];
+                kc[2 * i + 1] = c->kappa[i] * c->che[2 * i + 1];
+            }
+            MIDAS_HIP(hipMemcpyAsync(kchi.p, kc.data(), z2, hipMemcpyHostToDevice, st));
+            MIDAS_HIP(hipStreamSynchronize(st));
+        } else {
+            MIDAS_CHECK(c->sc && c->eps, -1, "cells: need sc and eps (or kappa, che, znnpp)");
+            TRY(upload(eps, c->eps, z2, st));
+            TRY(launch_cell_params(n, sc.as<double>(), eps.as<double>(), k0, quirks, a.as<double>(),
+                                   che.as<double>(), znnpp.as<double>(), kappa.as<double>(), kchi.as<double>(), st));
+        }
+        v = DevCells{n, x.as<double>(), y.as<double>(), this->z.as<double>(), sc.as<double>(), kappa.as<double>(),
+                     che.as<double>(), znnpp.as<double>(), kchi.as<double>(), id.as<int>()};
+        return 0;
+    }
+    // device pointers supplied by the caller; constants computed here
+    int from_device(const midas_cells_t *c, double k0, int quirks, hipStream_t st)
+    {
+        MIDAS_CHECK(c && c->n > 0 && c->x && c->y && c->z && c->sc && c->eps, -1, "dev cells: missing arrays");
+        const int n = c->n;
+        const size_t d = sizeof(double) * n, z2 = 2 * d;
+        TRY(kappa.alloc(d));
+        TRY(che.alloc(z2));
+        TRY(znnpp.alloc(z2));
+        TRY(kchi.alloc(z2));
+        TRY(a.alloc(d));
+        const int *ids = c->id;
+        if (!ids) {
+            std::vector<int> h(n);
+            for (int i = 0; i < n; ++i) h[i] = i + 1;
+            TRY(upload(id, h.data(), sizeof(int) * n, st));
+            MIDAS_HIP(hipStreamSynchronize(st));
+            ids = id.as<int>();
+        }
+        TRY(launch_cell_params(n, c->sc, c->eps, k0, quirks, a.as<double>(), che.as<double>(), znnpp.as<double>(),
+                               kappa.as<double>(), kchi.as<double>(), st));
+        v = DevCells{n, c->x, c->y, c->z, c->sc, kappa.as<double>(), che.as<double>(), znnpp.as<double>(),
+                     kchi.as<double>(), ids};
+        return 0;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// device helpers for the solver
+// ---------------------------------------------------------------------------
+__global__ void k_z2c(const double2 *Z, long ldz, float2 *W, long ldw, int m, int n)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (r < m && c < n) {
+        double2 v = Z[(long)r + (long)c * ldz];
+        W[(long)r + (long)c * ldw] = make_float2((float)v.x, (float)v.y);
+    }
+}
+__global__ void k_c2z_add(const float2 *D, long ldd, double2 *X, long ldx, int m, int n, int accumulate)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (r < m && c < n) {
+        float2 d = D[(long)r + (long)c * ldd];
+        double2 *x = X + (long)r + (long)c * ldx;
+        if (accumulate) *x = make_double2(x->x + (double)d.x, x->y + (double)d.y);
+        else *x = make_double2((double)d.x, (double)d.y);
+    }
+}
+__global__ void k_copyz(const double2 *S, long lds, double2 *D, long ldd, int m, int n)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (r < m && c < n) D[(long)r + (long)c * ldd] = S[(long)r + (long)c * lds];
+}
+// ||Z||_inf: max over rows of sum |Z_ij| (one thread per row, coalesced along rows)
+__global__ void k_rowabs(const double2 *Z, long ldz, int n, double *rowsum)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < n; ++c) {
+        double2 v = Z[(long)r + (long)c * ldz];
+        s += sqrt(v.x * v.x + v.y * v.y);
+    }
+    rowsum[r] = s;
+}
+// per-column max modulus of a (m x n) block -> out[c]
+__global__ void k_colmax(const double2 *X, long ldx, int m, double *out)
+{
+    int c = blockIdx.x;
+    double mx = 0.0;
+    for (int r = threadIdx.x; r < m; r += blockDim.x) {
+        double2 v = X[(long)r + (long)c * ldx];
+        mx = fmax(mx, sqrt(v.x * v.x + v.y * v.y));
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
+}
+__global__ void k_max_reduce(const double *v, int n, double *out)
+{
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = fmax(mx, v[i]);
+    __shared__ double red[256];
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+static int trail_mode(int mode)
+{
+    switch (mode) {
+    case MIDAS_SOLVE_C32_IR: return TRAIL_NATIVE;
+    case MIDAS_SOLVE_F16_IR: return TRAIL_F16_1;
+    case MIDAS_SOLVE_F16X3_IR: return TRAIL_F16_3;
+    case MIDAS_SOLVE_BF16_IR: return TRAIL_BF16_1;
+    case MIDAS_SOLVE_BF16X3_IR: return TRAIL_BF16_3;
+    }
+    return -1;
+}
+
+struct Timer {
+    hipEvent_t a, b;
+    hipStream_t st;
+    explicit Timer(hipStream_t s) : st(s)
+    {
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, st);
+    }
+    double ms()
+    {
+        hipEventRecord(b, st);
+        hipEventSynchronize(b);
+        float t = 0;
+        hipEventElapsedTime(&t, a, b);
+        hipEventRecord(a, st);
+        return t;
+    }
+    ~Timer()
+    {
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+    }
+};
+
+static int default_nb(int n, int mode) { return mode == MIDAS_SOLVE_FP64 ? 128 : 256; }
+
+// Solve Z X = B on the device.  Z (n x n, ldz) FP64; if W32 is non-NULL it
+// already holds the FP32 copy of Z (mixed modes).  For MIDAS_SOLVE_FP64 Z is
+// factored in place when z_inplace, else copied.  X (n x nrhs, ldx) receives
+// the solution; B (n x nrhs, ldb) is read only.
+static int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32, long ldw, const double2 *B,
+                     long ldb, double2 *X, long ldx, int mode, const midas_solve_opts_t *opts, midas_solve_info_t *info,
+                     bool z_inplace)
+{
+    hipStream_t st = ctx->st;
+    midas_solve_info_t inf{};
+    int max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 30;
+    double tol = (opts && opts->tol > 0) ? opts->tol : 1e-14;
+    int nb = (opts && opts->nb > 0) ? opts->nb : default_nb(n, mode);
+    nb = nb > n ? n : nb;
+    MIDAS_CHECK(n > 0 && nrhs > 0, -1, "solve: n=%d nrhs=%d", n, nrhs);
+    MIDAS_CHECK(mode >= 0 && mode <= 5, -1, "solve: unknown mode %d", mode);
+    DBuf ipiv_d, perm_d, norms, tmpn;
+    TRY(ipiv_d.alloc(sizeof(int) * n));
+    TRY(perm_d.alloc(sizeof(int) * n));
+    TRY(norms.alloc(sizeof(double) * (n + nrhs * 2 + 4)));
+    std::vector<int> ipiv(n), perm(n);
+    double *rowsum = norms.as<double>();
+    double *colx = rowsum + n, *colr = colx + nrhs, *scal = colr + nrhs;
+
+    // ||Z||_inf before factoring (PZLANGE 'I')
+    hipLaunchKernelGGL(k_rowabs, dim3((n + 255) / 256), dim3(256), 0, st, Z, ldz, n, rowsum);
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, st, rowsum, n, scal);
+    MIDAS_HIP(hipGetLastError());
+    MIDAS_HIP(hipMemcpyAsync(&inf.anorm, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+
+    Timer tm(st);
+    if (mode == MIDAS_SOLVE_FP64) {
+        DBuf zc;
+        double2 *A = Z;
+        long lda = ldz;
+        if (!z_inplace) {
+            TRY(zc.alloc(sizeof(double2) * (size_t)n * n));
+            A = zc.as<double2>();
+            lda = n;
+            hipLaunchKernelGGL(k_copyz, dim3((n + 255) / 256, n), dim3(256), 0, st, Z, ldz, A, lda, n, n);
+        }
+        LuWork<double> w;
+        TRY(w.alloc(n, nb));
+        TrailingGemm<double> tg;
+        int r = getrf<double>(A, lda, n, ipiv_d.as<int>(), w, st, tg);
+        if (r) { w.release(); return r; }
+        MIDAS_HIP(hipMemcpyAsync(&inf.info, w.info, sizeof(int), hipMemcpyDeviceToHost, st));
+        inf.t_factor_ms = tm.ms();
+        TRY(copy_sync(ipiv.data(), ipiv_d.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+        ipiv_to_perm(ipiv.data(), n, perm.data());
+        TRY(copy_sync(perm_d.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_copyz, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, X, ldx, n, nrhs);
+        DBuf t2;
+        TRY(t2.alloc(sizeof(double2) * (size_t)n * nrhs));
+        r = getrs<double>(A, lda, n, perm_d.as<int>(), X, ldx, nrhs, t2.as<double2>(), nb, st);
+        w.release();
+        if (r) return r;
+        inf.t_solve_ms = tm.ms();
+    } else {
+        DBuf wc;
+        float2 *A = W32;
+        long lda = ldw;
+        if (!A) {
+            TRY(wc.alloc(sizeof(float2) * (size_t)n * n));
+            A = wc.as<float2>();
+            lda = n;
+            hipLaunchKernelGGL(k_z2c, dim3((n + 255) / 256, n), dim3(256), 0, st, Z, ldz, A, lda, n, n);
+            MIDAS_HIP(hipGetLastError());
+        }
+        LuWork<float> w;
+        TRY(w.alloc(n, nb));
+        DBuf scale;
+        TRY(scale.alloc(sizeof(float) * n));
+        TrailingGemm<float> tg;
+        tg.mode = trail_mode(mode);
+        tg.scratch = scale.p;
+        int r = getrf<float>(A, lda, n, ipiv_d.as<int>(), w, st, tg);
+        if (r) { w.release(); return r; }
+        MIDAS_HIP(hipMemcpyAsync(&inf.info, w.info, sizeof(int), hipMemcpyDeviceToHost, st));
+        inf.t_factor_ms = tm.ms();
+        TRY(copy_sync(ipiv.data(), ipiv_d.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+        ipiv_to_perm(ipiv.data(), n, perm.data());
+        TRY(copy_sync(perm_d.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        // refinement: X = 0, R = B; repeat D = LU^-1 R (FP32), X += D, R = B - Z X (FP64)
+        DBuf R, D, t2;
+        TRY(R.alloc(sizeof(double2) * (size_t)n * nrhs));
+        TRY(D.alloc(sizeof(float2) * (size_t)n * nrhs));
+        TRY(t2.alloc(sizeof(float2) * (size_t)n * nrhs));
+        dim3 g2((n + 255) / 256, nrhs);
+        hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R.as<double2>(), (long)n, n, nrhs);
+        std::vector<double> hx(nrhs), hr(nrhs), hb(nrhs);
+        hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, B, ldb, n, colr);
+        TRY(copy_sync(hb.data(), colr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+        double berr = 1e300;
+        int it = 0;
+        for (; it < max_iter; ++it) {
+            hipLaunchKernelGGL(k_z2c, g2, dim3(256), 0, st, R.as<double2>(), (long)n, D.as<float2>(), (long)n, n, nrhs);
+            r = getrs<float>(A, lda, n, perm_d.as<int>(), D.as<float2>(), (long)n, nrhs, t2.as<float2>(), nb, st);
+            if (r) { w.release(); return r; }
+            hipLaunchKernelGGL(k_c2z_add, g2, dim3(256), 0, st, D.as<float2>(), (long)n, X, ldx, n, nrhs, it > 0);
+            hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R.as<double2>(), (long)n, n, nrhs);
+            r = launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R.p, n, st);
+            if (r) { w.release(); return r; }
+            hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, X, ldx, n, colx);
+            hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, R.as<double2>(), (long)n, n, colr);
+            MIDAS_HIP(hipGetLastError());
+            TRY(copy_sync(hx.data(), colx, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+            TRY(copy_sync(hr.data(), colr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+            double b = 0.0;
+            for (int c = 0; c < nrhs; ++c) {
+                double den = inf.anorm * hx[c] + hb[c];
+                double e = den > 0 ? hr[c] / den : 0.0;
+                b = e > b ? e : b;
+            }
+            if (it == 0) inf.t_solve_ms = tm.ms();
+            bool stalled = (it > 2 && b > 0.5 * berr);
+            berr = b;
+            if (berr <= tol || stalled) { ++it; break; }
+        }
+        inf.iterations = it;
+        inf.berr = berr;
+        w.release();
+        inf.t_refine_ms = tm.ms();
+    }
+    if (info) *info = inf;
+    return inf.info;
+}
+
+namespace midas {
+int mfma_probe(int which, double *dout, hipStream_t st);
+int launch_trail16(int mode, int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
+                   long ldc, float *scale, hipStream_t st);
+}
+
+// ===========================================================================
+// extern "C"
+// ===========================================================================
+extern "C" {
+
+const char *midas_last_error(void) { return midas::last_error(); }
+
+int midas_init(int device, midas_ctx **ctx)
+{
+    MIDAS_CHECK(ctx, -1, "midas_init: ctx is NULL");
+    int ndev = 0;
+    MIDAS_HIP(hipGetDeviceCount(&ndev));
+    MIDAS_CHECK(device >= 0 && device < ndev, -1, "midas_init: device %d of %d", device, ndev);
+    MIDAS_HIP(hipSetDevice(device));
+    midas_ctx *c = new midas_ctx;
+    c->device = device;
+    hipError_t e = hipStreamCreate(&c->st);
+    if (e != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreate: %s", hipGetErrorString(e));
+        return -100 - (int)e;
+    }
+    *ctx = c;
+    return 0;
+}
+
+int midas_finalize(midas_ctx *ctx)
+{
+    if (!ctx) return 0;
+    hipStreamSynchronize(ctx->st);
+    hipStreamDestroy(ctx->st);
+    delete ctx;
+    return 0;
+}
+
+void *midas_stream(midas_ctx *ctx) { return ctx ? (void *)ctx->st : nullptr; }
+
+int midas_cell_params(midas_ctx *ctx, int n, const double *sc, const double *eps, double k0, int quirks, double *a_n,
+                      double *che, double *znnpp, double *kappa)
+{
+    MIDAS_CHECK(ctx && n > 0 && sc && eps, -1, "cell_params: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    DBuf dsc, deps, da, dche, dzn, dk, dkc;
+    TRY(upload(dsc, sc, sizeof(double) * n, st));
+    TRY(upload(deps, eps, sizeof(double) * 2 * n, st));
+    TRY(da.alloc(sizeof(double) * n));
+    TRY(dche.alloc(sizeof(double) * 2 * n));
+    TRY(dzn.alloc(sizeof(double) * 2 * n));
+    TRY(dk.alloc(sizeof(double) * n));
+    TRY(dkc.alloc(sizeof(double) * 2 * n));
+    TRY(launch_cell_params(n, dsc.as<double>(), deps.as<double>(), k0, quirks, da.as<double>(), dche.as<double>(),
+                           dzn.as<double>(), dk.as<double>(), dkc.as<double>(), st));
+    if (a_n) MIDAS_HIP(hipMemcpyAsync(a_n, da.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (che) MIDAS_HIP(hipMemcpyAsync(che, dche.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
+    if (znnpp) MIDAS_HIP(hipMemcpyAsync(znnpp, dzn.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
+    if (kappa) MIDAS_HIP(hipMemcpyAsync(kappa, dk.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int midas_mom_fill(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, double *Z, long ldz)
+{
+    MIDAS_CHECK(ctx && cells && Z, -1, "mom_fill: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    const int n = cells->n, n3 = 3 * cells->n;
+    MIDAS_CHECK(ldz >= n3, -1, "mom_fill: ldz %ld < %d", ldz, n3);
+    DevCellsOwned dc;
+    TRY(dc.from_host(cells, k0, quirks, ctx->st));
+    DBuf dz;
+    TRY(dz.alloc(sizeof(double2) * (size_t)n3 * n3));
+    TRY(launch_fill(dc.v, CellSet{n, 0, nullptr}, CellSet{n, 0, nullptr}, k0, quirks, false, dz.as<double2>(), n3,
+                    nullptr, 0, ctx->st));
+    MIDAS_HIP(hipMemcpy2DAsync(Z, sizeof(double2) * ldz, dz.p, sizeof(double2) * n3, sizeof(double2) * n3, n3,
+                               hipMemcpyDeviceToHost, ctx->st));
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int midas_block_fill(midas_ctx *ctx, const midas_cells_t *b1, const midas_cells_t *b2, double k0, int quirks,
+                     double *Z, long ldz)
+{
+    MIDAS_CHECK(ctx && b1 && b2 && Z, -1, "block_fill: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    const int n1 = b1->n, n2 = b2->n;
+    MIDAS_CHECK(ldz >= 3L * n1, -1, "block_fill: ldz too small");
+    // one device table: receivers first, then sources
+    std::vector<double> x(n1 + n2), y(n1 + n2), z(n1 + n2), sc(n1 + n2), kap(n1 + n2), che(2 * (n1 + n2)),
+        zn(2 * (n1 + n2));
+    std::vector<int> id(n1 + n2);
+    // constants: compute each set's own (receivers' constants are not used by the element)
+    std::vector<double> a(n1 + n2);
+    auto fillset = [&](const midas_cells_t *c, int off) -> int {
+        for (int i = 0; i < c->n; ++i) {
+            x[off + i] = c->x[i];
+            y[off + i] = c->y[i];
+            z[off + i] = c->z[i];
+            sc[off + i] = c->sc ? c->sc[i] : 0.0;
+            id[off + i] = c->id ? c->id[i] : i + 1;
+        }
+        if (c->kappa && c->che && c->znnpp) {
+            for (int i = 0; i < c->n; ++i) {
+                kap[off + i] = c->kappa[i];
+                che[2 * (off + i)] = c->che[2 * i];
+                che[2 * (off + i) + 1] = c->che[2 * i + 1];
+                zn[2 * (off + i)] = c->znnpp[2 * i];
+                zn[2 * (off + i) + 1] = c->znnpp[2 * i + 1];
+            }
+            return 0;
+        }
+        MIDAS_CHECK(c->sc && c->eps, -1, "block_fill: need sc/eps or constants");
+        return midas_cell_params(ctx, c->n, c->sc, c->eps, k0, quirks, a.data() + off, che.data() + 2 * off,
+                                 zn.data() + 2 * off, kap.data() + off);
+    };
+    TRY(fillset(b1, 0));
+    TRY(fillset(b2, n1));
+    midas_cells_t all{n1 + n2, x.data(), y.data(), z.data(), sc.data(), nullptr, kap.data(), che.data(), zn.data(),
+                      id.data()};
+    DevCellsOwned dc;
+    TRY(dc.from_host(&all, k0, quirks, ctx->st));
+    DBuf dz;
+    TRY(dz.alloc(sizeof(double2) * (size_t)3 * n1 * 3 * n2));
+    TRY(launch_fill(dc.v, CellSet{n1, 0, nullptr}, CellSet{n2, n1, nullptr}, k0, quirks,
+                    (quirks & MIDAS_Q_FP32_BLOCKS) != 0, dz.as<double2>(), 3L * n1, nullptr, 0, ctx->st));
+    MIDAS_HIP(hipMemcpy2DAsync(Z, sizeof(double2) * ldz, dz.p, sizeof(double2) * 3 * n1, sizeof(double2) * 3 * n1,
+                               3 * n2, hipMemcpyDeviceToHost, ctx->st));
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int midas_plane_wave(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int ntr, const double *th,
+                     const double *ph, int ref32, double *V, long ldv)
+{
+    MIDAS_CHECK(ctx && cells && th && ph && V && ntr > 0, -1, "plane_wave: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    const int n = cells->n;
+    MIDAS_CHECK(ldv >= 3L * n, -1, "plane_wave: ldv too small");
+    DevCellsOwned dc;
+    midas_cells_t geo = *cells;
+    std::vector<double> zr;
+    if (!(geo.kappa && geo.che && geo.znnpp) && !(geo.sc && geo.eps)) {
+        // only the geometry enters the excitation: neutral constants
+        zr.assign(2 * (size_t)n, 0.0);
+        geo.kappa = zr.data();
+        geo.che = zr.data();
+        geo.znnpp = zr.data();
+    }
+    TRY(dc.from_host(&geo, k0, quirks, ctx->st));
+    DBuf dth, dph, dv;
+    TRY(upload(dth, th, sizeof(double) * ntr, ctx->st));
+    TRY(upload(dph, ph, sizeof(double) * ntr, ctx->st));
+    TRY(dv.alloc(sizeof(double2) * 3L * n * 2 * ntr));
+    TRY(launch_rhs(dc.v, CellSet{n, 0, nullptr}, ntr, dth.as<double>(), dph.as<double>(), k0, quirks, ref32 != 0,
+                   dv.as<double2>(), 3L * n, ctx->st));
+    MIDAS_HIP(hipMemcpy2DAsync(V, sizeof(double2) * ldv, dv.p, sizeof(double2) * 3 * n, sizeof(double2) * 3 * n,
+                               2 * ntr, hipMemcpyDeviceToHost, ctx->st));
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int midas_solve(midas_ctx *ctx, int n, int nrhs, const double *Z, long ldz, double *B, long ldb, int mode,
+                const midas_solve_opts_t *opts, midas_solve_info_t *info)
+{
+    MIDAS_CHECK(ctx && Z && B && n > 0 && nrhs > 0 && ldz >= n && ldb >= n, -1, "solve: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    DBuf dz, db, dx;
+    TRY(dz.alloc(sizeof(double2) * (size_t)n * n));
+    TRY(db.alloc(sizeof(double2) * (size_t)n * nrhs));
+    TRY(dx.alloc(sizeof(double2) * (size_t)n * nrhs));
+    MIDAS_HIP(hipMemcpy2DAsync(dz.p, sizeof(double2) * n, Z, sizeof(double2) * ldz, sizeof(double2) * n, n,
+                               hipMemcpyHostToDevice, st));
+    MIDAS_HIP(hipMemcpy2DAsync(db.p, sizeof(double2) * n, B, sizeof(double2) * ldb, sizeof(double2) * n, nrhs,
+                               hipMemcpyHostToDevice, st));
+    int r = solve_dev(ctx, n, nrhs, dz.as<double2>(), n, nullptr, 0, db.as<double2>(), n, dx.as<double2>(), n, mode,
+                      opts, info, true);
+    if (r < 0) return r;
+    MIDAS_HIP(hipMemcpy2DAsync(B, sizeof(double2) * ldb, dx.p, sizeof(double2) * n, sizeof(double2) * n, nrhs,
+                               hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return r;
+}
+
+int midas_scattering_matrix(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, const double *E,
+                            long lde, int ntr, int nrx, const double *rth, const double *rph, int npol,
+                            const double *beta, double *S, long lds)
+{
+    MIDAS_CHECK(ctx && cells && E && rth && rph && S && ntr > 0 && nrx > 0 && npol > 0, -1,
+                "scattering_matrix: bad arguments");
+    MIDAS_CHECK(lde >= 3L * cells->n && lds >= nrx, -1, "scattering_matrix: leading dimension too small");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    const int n = cells->n;
+    DevCellsOwned dc;
+    TRY(dc.from_host(cells, k0, quirks, st));
+    std::vector<double> b0(npol, 0.0);
+    DBuf dE, drt, drp, db, dS, FF, E3, G;
+    TRY(dE.alloc(sizeof(double2) * 3L * n * 2 * ntr));
+    MIDAS_HIP(hipMemcpy2DAsync(dE.p, sizeof(double2) * 3 * n, E, sizeof(double2) * lde, sizeof(double2) * 3 * n,
+                               2 * ntr, hipMemcpyHostToDevice, st));
+    TRY(upload(drt, rth, sizeof(double) * nrx, st));
+    TRY(upload(drp, rph, sizeof(double) * nrx, st));
+    TRY(upload(db, beta ? beta : b0.data(), sizeof(double) * npol, st));
+    TRY(dS.alloc(sizeof(double2) * (size_t)nrx * 4 * ntr * npol));
+    TRY(FF.alloc(sizeof(double2) * (size_t)nrx * n));
+    TRY(E3.alloc(sizeof(double2) * (size_t)n * 6 * ntr));
+    TRY(G.alloc(sizeof(double2) * (size_t)nrx * 6 * ntr));
+    TRY(far_field(dc.v, dE.as<double2>(), 3L * n, ntr, nrx, drt.as<double>(), drp.as<double>(), npol,
+                  db.as<double>(), k0, quirks, dS.as<double2>(), nrx, FF.as<double2>(), E3.as<double2>(),
+                  G.as<double2>(), st));
+    MIDAS_HIP(hipMemcpy2DAsync(S, sizeof(double2) * lds, dS.p, sizeof(double2) * nrx, sizeof(double2) * nrx,
+                               4L * ntr * npol, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int midas_cext_cabs(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, const double *E, long lde,
+                    int ntr, const double *th, const double *ph, double *C_ext, double *C_abs)
+{
+    MIDAS_CHECK(ctx && cells && E && th && ph && C_ext && C_abs && ntr > 0, -1, "cext_cabs: bad arguments");
+    MIDAS_CHECK(lde >= 3L * cells->n, -1, "cext_cabs: lde too small");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    const int n = cells->n;
+    DevCellsOwned dc;
+    TRY(dc.from_host(cells, k0, quirks, st));
+    DBuf dE, dth, dph, dce, dca;
+    TRY(dE.alloc(sizeof(double2) * 3L * n * 2 * ntr));
+    MIDAS_HIP(hipMemcpy2DAsync(dE.p, sizeof(double2) * 3 * n, E, sizeof(double2) * lde, sizeof(double2) * 3 * n,
+                               2 * ntr, hipMemcpyHostToDevice, st));
+    TRY(upload(dth, th, sizeof(double) * ntr, st));
+    TRY(upload(dph, ph, sizeof(double) * ntr, st));
+    TRY(dce.alloc(sizeof(double2) * ntr));
+    TRY(dca.alloc(sizeof(double2) * ntr));
+    TRY(cext_cabs(dc.v, dE.as<double2>(), 3L * n, ntr, dth.as<double>(), dph.as<double>(), k0, quirks,
+                  dce.as<double>(), dca.as<double>(), st));
+    MIDAS_HIP(hipMemcpyAsync(C_ext, dce.p, sizeof(double2) * ntr, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipMemcpyAsync(C_abs, dca.p, sizeof(double2) * ntr, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Fill (FP64 Z, and the FP32 copy W for mixed modes) + RHS + solve, all on
+// the device; X (3n x 2ntr).  Shared by the drop-in and the device entry point.
+static int mom_solve_dev(midas_ctx *ctx, const DevCells &cells, double k0, int quirks, int ntr, const double *dth,
+                         const double *dph, int mode, const midas_solve_opts_t *opts, double2 *Z, long ldz,
+                         double2 *X, long ldx, midas_solve_info_t *info, double *t_fill_ms)
+{
+    hipStream_t st = ctx->st;
+    const int n = cells.n, n3 = 3 * n;
+    DBuf W, V;
+    float2 *w32 = nullptr;
+    if (mode != MIDAS_SOLVE_FP64) {
+        TRY(W.alloc(sizeof(float2) * (size_t)n3 * n3));
+        w32 = W.as<float2>();
+    }
+    TRY(V.alloc(sizeof(double2) * (size_t)n3 * 2 * ntr));
+    Timer tm(st);
+    TRY(launch_fill(cells, CellSet{n, 0, nullptr}, CellSet{n, 0, nullptr}, k0, quirks, false, Z, ldz, w32, n3, st));
+    TRY(launch_rhs(cells, CellSet{n, 0, nullptr}, ntr, dth, dph, k0, quirks, false, V.as<double2>(), n3, st));
+    double tf = tm.ms();
+    if (t_fill_ms) *t_fill_ms = tf;
+    return solve_dev(ctx, n3, 2 * ntr, Z, ldz, w32, n3, V.as<double2>(), n3, X, ldx, mode, opts, info, true);
+}
+
+int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int ntr,
+                              const double *tr_th, const double *tr_ph, int nrx, const double *rx_th,
+                              const double *rx_ph, int mode, const midas_solve_opts_t *opts, double *S_total,
+                              double *C_ext, double *C_abs, double *E, midas_solve_info_t *info)
+{
+    MIDAS_CHECK(ctx && cells && tr_th && tr_ph && ntr > 0, -1, "compute_efields_mom: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    const int n = cells->n, n3 = 3 * n;
+    DevCellsOwned dc;
+    TRY(dc.from_host(cells, k0, quirks, st));
+    DBuf Z, X, dth, dph;
+    TRY(Z.alloc(sizeof(double2) * (size_t)n3 * n3));
+    TRY(X.alloc(sizeof(double2) * (size_t)n3 * 2 * ntr));
+    TRY(upload(dth, tr_th, sizeof(double) * ntr, st));
+    TRY(upload(dph, tr_ph, sizeof(double) * ntr, st));
+    int r = mom_solve_dev(ctx, dc.v, k0, quirks, ntr, dth.as<double>(), dph.as<double>(), mode, opts, Z.as<double2>(),
+                          n3, X.as<double2>(), n3, info, nullptr);
+    if (r < 0) return r;
+    int singular = r;
+    Z.release();
+    if (S_total && nrx > 0) {
+        DBuf drt, drp, db, dS, FF, E3, G;
+        double b0 = 0.0;
+        TRY(upload(drt, rx_th, sizeof(double) * nrx, st));
+        TRY(upload(drp, rx_ph, sizeof(double) * nrx, st));
+        TRY(upload(db, &b0, sizeof(double), st));
+        TRY(dS.alloc(sizeof(double2) * (size_t)nrx * 4 * ntr));
+        TRY(FF.alloc(sizeof(double2) * (size_t)nrx * n));
+        TRY(E3.alloc(sizeof(double2) * (size_t)n * 6 * ntr));
+        TRY(G.alloc(sizeof(double2) * (size_t)nrx * 6 * ntr));
+        TRY(far_field(dc.v, X.as<double2>(), n3, ntr, nrx, drt.as<double>(), drp.as<double>(), 1, db.as<double>(), k0,
+                      quirks, dS.as<double2>(), nrx, FF.as<double2>(), E3.as<double2>(), G.as<double2>(), st));
+        MIDAS_HIP(hipMemcpyAsync(S_total, dS.p, sizeof(double2) * (size_t)nrx * 4 * ntr, hipMemcpyDeviceToHost, st));
+        MIDAS_HIP(hipStreamSynchronize(st));
+    }
+    if (C_ext && C_abs) {
+        DBuf dce, dca;
+        TRY(dce.alloc(sizeof(double2) * ntr));
+        TRY(dca.alloc(sizeof(double2) * ntr));
+        TRY(cext_cabs(dc.v, X.as<double2>(), n3, ntr, dth.as<double>(), dph.as<double>(), k0, quirks, dce.as<double>(),
+                      dca.as<double>(), st));
+        MIDAS_HIP(hipMemcpyAsync(C_ext, dce.p, sizeof(double2) * ntr, hipMemcpyDeviceToHost, st));
+        MIDAS_HIP(hipMemcpyAsync(C_abs, dca.p, sizeof(double2) * ntr, hipMemcpyDeviceToHost, st));
+    }
+    if (E) MIDAS_HIP(hipMemcpyAsync(E, X.p, sizeof(double2) * (size_t)n3 * 2 * ntr, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return singular;
+}
+
+int midas_dev_mom_fill(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, double *Z, long ldz,
+                       float *W32, long ldw)
+{
+    MIDAS_CHECK(ctx && dcells && (Z || W32), -1, "dev_mom_fill: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    const int n = dcells->n;
+    DevCellsOwned dc;
+    TRY(dc.from_device(dcells, k0, quirks, ctx->st));
+    TRY(launch_fill(dc.v, CellSet{n, 0, nullptr}, CellSet{n, 0, nullptr}, k0, quirks, false, (double2 *)Z, ldz,
+                    (float2 *)W32, ldw, ctx->st));
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int midas_dev_mom_solve(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, int ntr, const double *dth,
+                        const double *dph, int mode, const midas_solve_opts_t *opts, double *Z, long ldz, double *X,
+                        long ldx, midas_solve_info_t *info, double *t_fill_ms)
+{
+    MIDAS_CHECK(ctx && dcells && dth && dph && Z && X && ntr > 0, -1, "dev_mom_solve: bad arguments");
+    MIDAS_CHECK(ldz >= 3L * dcells->n && ldx >= 3L * dcells->n, -1, "dev_mom_solve: leading dimension too small");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    DevCellsOwned dc;
+    TRY(dc.from_device(dcells, k0, quirks, ctx->st));
+    int r = mom_solve_dev(ctx, dc.v, k0, quirks, ntr, dth, dph, mode, opts, (double2 *)Z, ldz, (double2 *)X, ldx, info,
+                          t_fill_ms);
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// kernel-level test hooks (include/midas_gpu_test.h)
+// ---------------------------------------------------------------------------
+__global__ void k_c2z(const float2 *S, double2 *D, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) D[i] = make_double2(S[i].x, S[i].y);
+}
+__global__ void k_z2c1(const double2 *S, float2 *D, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) D[i] = make_float2((float)S[i].x, (float)S[i].y);
+}
+
+static int upload_mat(DBuf &b, const double *h, int rows, int cols, long ld, hipStream_t st)
+{
+    TRY(b.alloc(sizeof(double2) * (size_t)rows * cols));
+    MIDAS_HIP(hipMemcpy2DAsync(b.p, sizeof(double2) * rows, h, sizeof(double2) * ld, sizeof(double2) * rows, cols,
+                               hipMemcpyHostToDevice, st));
+    return 0;
+}
+static int to_c64(DBuf &d32, const DBuf &d64, long cnt, hipStream_t st)
+{
+    TRY(d32.alloc(sizeof(float2) * cnt));
+    hipLaunchKernelGGL(k_z2c1, dim3((cnt + 255) / 256), dim3(256), 0, st, d64.as<double2>(), d32.as<float2>(), cnt);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double *alpha, const double *A, long lda,
+                    int transA, const double *B, long ldb, int transB, const double *beta, double *C, long ldc)
+{
+    MIDAS_CHECK(ctx && A && B && C && alpha && beta && (prec == 64 || prec == 32), -1, "test_gemm: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    int ar = transA ? k : m, ac = transA ? m : k, br = transB ? n : k, bc = transB ? k : n;
+    DBuf dA, dB, dC;
+    TRY(upload_mat(dA, A, ar, ac, lda, st));
+    TRY(upload_mat(dB, B, br, bc, ldb, st));
+    TRY(upload_mat(dC, C, m, n, ldc, st));
+    if (prec == 64) {
+        TRY(launch_cgemm<double>(m, n, k, z64{alpha[0], alpha[1]}, dA.p, ar, transA, dB.p, br, transB,
+                                 z64{beta[0], beta[1]}, dC.p, m, st));
+    } else {
+        DBuf a32, b32, c32;
+        TRY(to_c64(a32, dA, (long)ar * ac, st));
+        TRY(to_c64(b32, dB, (long)br * bc, st));
+        TRY(to_c64(c32, dC, (long)m * n, st));
+        TRY(launch_cgemm<float>(m, n, k, midas::c32{(float)alpha[0], (float)alpha[1]}, a32.p, ar, transA, b32.p, br,
+                                transB, midas::c32{(float)beta[0], (float)beta[1]}, c32.p, m, st));
+        hipLaunchKernelGGL(k_c2z, dim3(((long)m * n + 255) / 256), dim3(256), 0, st, c32.as<float2>(),
+                           dC.as<double2>(), (long)m * n);
+    }
+    MIDAS_HIP(hipMemcpy2DAsync(C, sizeof(double2) * ldc, dC.p, sizeof(double2) * m, sizeof(double2) * m, n,
+                               hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int midas_test_trailing(midas_ctx *ctx, int mode, int m, int n, int k, const double *A, long lda, const double *B,
+                        long ldb, double *C, long ldc)
+{
+    MIDAS_CHECK(ctx && A && B && C, -1, "test_trailing: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    DBuf dA, dB, dC, a32, b32, c32, sc;
+    TRY(upload_mat(dA, A, m, k, lda, st));
+    TRY(upload_mat(dB, B, k, n, ldb, st));
+    TRY(upload_mat(dC, C, m, n, ldc, st));
+    TRY(to_c64(a32, dA, (long)m * k, st));
+    TRY(to_c64(b32, dB, (long)k * n, st));
+    TRY(to_c64(c32, dC, (long)m * n, st));
+    TRY(sc.alloc(sizeof(float) * n));
+    TRY(launch_trail16(mode, m, n, k, a32.as<float2>(), m, b32.as<float2>(), k, c32.as<float2>(), m, sc.as<float>(),
+                       st));
+    hipLaunchKernelGGL(k_c2z, dim3(((long)m * n + 255) / 256), dim3(256), 0, st, c32.as<float2>(), dC.as<double2>(),
+                       (long)m * n);
+    MIDAS_HIP(hipMemcpy2DAsync(C, sizeof(double2) * ldc, dC.p, sizeof(double2) * m, sizeof(double2) * m, n,
+                               hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int midas_test_mfma_probe(midas_ctx *ctx, int which, double *out)
+{
+    MIDAS_CHECK(ctx && out && which >= 0 && which <= 3, -1, "mfma_probe: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    DBuf d;
+    TRY(d.alloc(sizeof(double) * 256));
+    TRY(mfma_probe(which, d.as<double>(), ctx->st));
+    MIDAS_HIP(hipMemcpyAsync(out, d.p, sizeof(double) * 256, hipMemcpyDeviceToHost, ctx->st));
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+}  // extern "C"

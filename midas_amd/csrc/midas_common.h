@@ -1,0 +1,94 @@
+// Shared device/host helpers for the MIDAS gfx950 library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#define MIDAS_PI_EXACT 3.14159265358979323846
+// Quirk bits (the reference's constant/rounding quirks, switchable)
+#define MIDAS_Q_PI 1          // Pi = acos(-1.0f) widened to double (Initialization.f90:39)
+#define MIDAS_Q_LITERALS 2    // (1./3.), (2./3.) single literals (SetCellsParams.f90:37-44)
+#define MIDAS_Q_CEXT_SUM 4    // sum-of-components Cext/Cabs (Compute_ExtAbsCsec_fromIntField.f90:53-60)
+#define MIDAS_Q_FP32_BLOCKS 8 // FP32 intermediates in CBFM/ACA fills and RHS (Green_functions.f90:109)
+
+namespace midas {
+
+void set_error(const char *fmt, ...);
+const char *last_error();
+
+#define MIDAS_HIP(call)                                                                       \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            ::midas::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+            return -100 - (int)e_;                                                            \
+        }                                                                                     \
+    } while (0)
+
+#define MIDAS_CHECK(cond, code, ...)             \
+    do {                                         \
+        if (!(cond)) {                           \
+            ::midas::set_error(__VA_ARGS__);     \
+            return (code);                       \
+        }                                        \
+    } while (0)
+
+inline double host_pi(int quirks) { return (quirks & MIDAS_Q_PI) ? (double)acosf(-1.0f) : MIDAS_PI_EXACT; }
+
+// ---------------------------------------------------------------------------
+// complex arithmetic on plain structs (interleaved re, im)
+// ---------------------------------------------------------------------------
+template <typename T> struct cplx { T r, i; };
+typedef cplx<double> z64;
+typedef cplx<float> c32;
+
+template <typename T> __host__ __device__ inline cplx<T> cmk(T r, T i) { return {r, i}; }
+template <typename T> __host__ __device__ inline cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.r + b.r, a.i + b.i}; }
+template <typename T> __host__ __device__ inline cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.r - b.r, a.i - b.i}; }
+template <typename T> __host__ __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b)
+{
+    return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+template <typename T> __host__ __device__ inline cplx<T> cscale(cplx<T> a, T s) { return {a.r * s, a.i * s}; }
+template <typename T> __host__ __device__ inline T cabs2(cplx<T> a) { return a.r * a.r + a.i * a.i; }
+template <typename T> __host__ __device__ inline cplx<T> cinv(cplx<T> a)
+{
+    T d = (T)1 / (a.r * a.r + a.i * a.i);
+    return {a.r * d, -a.i * d};
+}
+template <typename T> __host__ __device__ inline cplx<T> cconj(cplx<T> a) { return {a.r, -a.i}; }
+
+// Device cell table (structure of arrays) with per-cell EM constants.
+struct DevCells {
+    int n;
+    const double *x, *y, *z;   // centres
+    const double *sc;          // cell size
+    const double *kappa;       // Kappa_n
+    const double *che;         // Che_n = eps - 1, interleaved complex
+    const double *znnpp;       // self term, interleaved complex
+    const double *kchi;        // Kappa_n * Che_n, interleaved complex
+    const int *id;             // cell identifier (n_cell)
+};
+
+// A set of cells: contiguous range [start, start+n) or an index list.
+struct CellSet {
+    int n;          // cells in the set
+    int start;      // first cell when idx == nullptr
+    const int *idx; // optional 0-based cell indices
+    __host__ __device__ int cell(int k) const { return idx ? idx[k] : start + k; }
+};
+
+// launchers (kernel files)
+int launch_cell_params(int n, const double *sc, const double *eps, double k0, int quirks, double *a, double *che,
+                       double *znnpp, double *kappa, double *kchi, hipStream_t st);
+int launch_fill(const DevCells &cells, CellSet R, CellSet S, double k0, int quirks, bool ref32, double2 *Z64,
+                long ld64, float2 *W32, long ld32, hipStream_t st);
+int launch_rhs(const DevCells &cells, CellSet R, int ntr, const double *th, const double *ph, double k0, int quirks,
+               bool ref32, double2 *V, long ld, hipStream_t st);
+int far_field(const DevCells &cells, const double2 *E, long ld, int T, int nrx, const double *rth,
+              const double *rph, int npol, const double *beta, double k0, int quirks, double2 *S, long lds,
+              double2 *FF, double2 *E3, double2 *G, hipStream_t st);
+int cext_cabs(const DevCells &cells, const double2 *E, long ld, int T, const double *th, const double *ph, double k0,
+              int quirks, double *cext, double *cabs, hipStream_t st);
+
+}  // namespace midas

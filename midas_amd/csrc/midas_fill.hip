@@ -1,0 +1,194 @@
+// Matrix fill, cell constants and plane-wave right-hand sides (gfx950).
+//
+//  k_cell_params : SetCellsParams.f90:37-48 (per cell a_n, Che_n, Znnpp, Kappa_n)
+//  k_fill        : compute_Green_s_tr_single_element (Compute_EFields_MoM.f90:433-523)
+//                  for every pair of a receiver set x source set, and
+//                  Green_s_tr_partial (Green_functions.f90:96-173) with FP32
+//                  intermediates when `ref32` is set.
+//  k_rhs         : Incident_Field_single_element (Compute_EFields_MoM.f90:529-598)
+//                  and Incident_Field (Incident_Field.f90:1-63, FP32 locals).
+//
+// One thread per cell pair writes the whole 3x3 block: the pair's distance,
+// e^{ikr} and dyad factors are computed once instead of nine times as in the
+// reference's per-entry routine.  A wavefront runs along receiver cells, so
+// each of the 3 columns a wave touches is written as one contiguous
+// 64 x 48-byte run; the source cell's constants are wave-uniform.
+#include "midas_common.h"
+#include "midas_elem.h"
+
+namespace midas {
+
+__global__ void k_cell_params(int n, const double *__restrict__ sc, const double *__restrict__ eps, double k0,
+                              double pi, double third, double twothird, double *a_out, double *che_out,
+                              double *znnpp_out, double *kappa_out, double *kchi_out)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double a = sc[c] * pow(0.75 / pi, third);
+    z64 che = {eps[2 * c] - 1.0, eps[2 * c + 1]};
+    double s, co;
+    sincos(k0 * a, &s, &co);
+    // ((2/3) e^{ika} (1 - ika) - 1) * che
+    z64 e = {twothird * co, twothird * s};
+    z64 t = cmul(e, z64{1.0, -k0 * a});
+    t.r -= 1.0;
+    z64 zn = cmul(t, che);
+    double kap = 4.0 * pi * (s - k0 * a * co) / pow(k0, 3.0);
+    a_out[c] = a;
+    che_out[2 * c] = che.r;
+    che_out[2 * c + 1] = che.i;
+    znnpp_out[2 * c] = zn.r;
+    znnpp_out[2 * c + 1] = zn.i;
+    kappa_out[c] = kap;
+    kchi_out[2 * c] = kap * che.r;
+    kchi_out[2 * c + 1] = kap * che.i;
+}
+
+// Z64 (ld64) and/or W32 (interleaved complex float, ld32) receive the block
+// (3*R.n x 3*S.n).  Diagonal cell pairs are detected by identifier equality.
+template <bool REF32>
+__global__ __launch_bounds__(256) void k_fill(DevCells cells, CellSet R, CellSet S, double k0, double pi,
+                                              double2 *__restrict__ Z64, long ld64, float2 *__restrict__ W32,
+                                              long ld32)
+{
+    const int o = blockIdx.x * 64 + threadIdx.x;
+    const int sl = blockIdx.y * 4 + threadIdx.y;
+    if (o >= R.n || sl >= S.n) return;
+    const int go = R.cell(o), gs = S.cell(sl);
+    z64 e[9];
+    if (cells.id[go] == cells.id[gs]) {
+        z64 d = {1.0 - cells.znnpp[2 * gs], -cells.znnpp[2 * gs + 1]};
+#pragma unroll
+        for (int q = 0; q < 9; ++q) e[q] = {0.0, 0.0};
+        e[0] = e[4] = e[8] = d;
+    } else {
+        double rx = cells.x[go] - cells.x[gs], ry = cells.y[go] - cells.y[gs], rz = cells.z[go] - cells.z[gs];
+        if (REF32) {
+            dyad_ref32(rx, ry, rz, k0, pi, cells.kappa[gs], z64{cells.che[2 * gs], cells.che[2 * gs + 1]}, e);
+        } else {
+            Dyad64 d = dyad64(rx, ry, rz, k0, 0.25 / pi, z64{cells.kchi[2 * gs], cells.kchi[2 * gs + 1]});
+#pragma unroll
+            for (int ps = 0; ps < 3; ++ps)
+#pragma unroll
+                for (int po = 0; po < 3; ++po) e[3 * ps + po] = dyad_entry(d, po, ps);
+        }
+    }
+#pragma unroll
+    for (int ps = 0; ps < 3; ++ps) {
+        const long col = 3L * sl + ps;
+        if (Z64) {
+            double2 *dst = Z64 + col * ld64 + 3L * o;
+#pragma unroll
+            for (int po = 0; po < 3; ++po) dst[po] = make_double2(e[3 * ps + po].r, e[3 * ps + po].i);
+        }
+        if (W32) {
+            float2 *dst = W32 + col * ld32 + 3L * o;
+#pragma unroll
+            for (int po = 0; po < 3; ++po) dst[po] = make_float2((float)e[3 * ps + po].r, (float)e[3 * ps + po].i);
+        }
+    }
+}
+
+// Plane-wave excitation: column t (V-pol) and t+ntr (H-pol) of V (3*R.n x 2*ntr).
+// ref32: K11 and the field components are single precision (Incident_Field.f90:17).
+template <bool REF32>
+__global__ void k_rhs(DevCells cells, CellSet R, int ntr, const double *__restrict__ th_deg,
+                      const double *__restrict__ ph_deg, double k0, double pi, double2 *__restrict__ V, long ld)
+{
+#pragma clang fp contract(off)
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blockIdx.y;
+    if (c >= R.n) return;
+    int g = R.cell(c);
+    double th = th_deg[t] * pi / 180.0, ph = ph_deg[t] * pi / 180.0;
+    double sth, cth, sph, cph;
+    sincos(th, &sth, &cth);
+    sincos(ph, &sph, &cph);
+    double kx = k0 * cth, ky = k0 * sth * cph, kz = k0 * sth * sph;
+    if (REF32) {
+        kx = (double)(float)kx;
+        ky = (double)(float)ky;
+        kz = (double)(float)kz;
+    }
+    // exp(i kx X) exp(i ky Y) exp(i kz Z) as the reference forms it
+    double s1, c1, s2, c2, s3, c3;
+    sincos(kx * cells.x[g], &s1, &c1);
+    sincos(ky * cells.y[g], &s2, &c2);
+    sincos(kz * cells.z[g], &s3, &c3);
+    z64 e = cmul(cmul(z64{c1, s1}, z64{c2, s2}), z64{c3, s3});
+    z64 v[3] = {cscale(e, -sth), cscale(cscale(e, cth), cph), cscale(cscale(e, cth), sph)};
+    z64 h[3] = {{0.0, 0.0}, cscale(e, -sph), cscale(e, cph)};
+    double2 *dv = V + (long)t * ld + 3L * c;
+    double2 *dh = V + (long)(t + ntr) * ld + 3L * c;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        if (REF32) {
+            dv[p] = make_double2((double)(float)v[p].r, (double)(float)v[p].i);
+            dh[p] = make_double2((double)(float)h[p].r, (double)(float)h[p].i);
+        } else {
+            dv[p] = make_double2(v[p].r, v[p].i);
+            dh[p] = make_double2(h[p].r, h[p].i);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int launch_cell_params(int n, const double *sc, const double *eps, double k0, int quirks, double *a, double *che,
+                       double *znnpp, double *kappa, double *kchi, hipStream_t st)
+{
+    double pi = host_pi(quirks);
+    double third = (quirks & MIDAS_Q_LITERALS) ? (double)(1.0f / 3.0f) : 1.0 / 3.0;
+    double twothird = (quirks & MIDAS_Q_LITERALS) ? (double)(2.0f / 3.0f) : 2.0 / 3.0;
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_cell_params, dim3((n + 255) / 256), dim3(256), 0, st, n, sc, eps, k0, pi, third, twothird,
+                       a, che, znnpp, kappa, kchi);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_fill(const DevCells &cells, CellSet R, CellSet S, double k0, int quirks, bool ref32, double2 *Z64,
+                long ld64, float2 *W32, long ld32, hipStream_t st)
+{
+    if (R.n <= 0 || S.n <= 0) return 0;
+    MIDAS_CHECK(!Z64 || ld64 >= 3L * R.n, -1, "fill: ld64 %ld < 3*%d", ld64, R.n);
+    MIDAS_CHECK(!W32 || ld32 >= 3L * R.n, -1, "fill: ld32 %ld < 3*%d", ld32, R.n);
+    MIDAS_CHECK((S.n + 3) / 4 <= 65535 * 64, -1, "fill: source set too large");
+    double pi = host_pi(quirks);
+    dim3 blk(64, 4);
+    // grid.y is limited to 65535: split the source set in chunks
+    const int ymax = 65535;
+    for (int s0 = 0; s0 < S.n; s0 += ymax * 4) {
+        CellSet Sc = S;
+        int cnt = S.n - s0 < ymax * 4 ? S.n - s0 : ymax * 4;
+        Sc.n = cnt;
+        if (S.idx) Sc.idx = S.idx + s0; else Sc.start = S.start + s0;
+        dim3 grd((R.n + 63) / 64, (cnt + 3) / 4);
+        double2 *z = Z64 ? Z64 + 3L * s0 * ld64 : nullptr;
+        float2 *w = W32 ? W32 + 3L * s0 * ld32 : nullptr;
+        if (ref32)
+            hipLaunchKernelGGL(k_fill<true>, grd, blk, 0, st, cells, R, Sc, k0, pi, z, ld64, w, ld32);
+        else
+            hipLaunchKernelGGL(k_fill<false>, grd, blk, 0, st, cells, R, Sc, k0, pi, z, ld64, w, ld32);
+        MIDAS_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+int launch_rhs(const DevCells &cells, CellSet R, int ntr, const double *th, const double *ph, double k0, int quirks,
+               bool ref32, double2 *V, long ld, hipStream_t st)
+{
+    if (R.n <= 0 || ntr <= 0) return 0;
+    MIDAS_CHECK(ntr <= 65535, -1, "rhs: too many directions in one launch");
+    double pi = host_pi(quirks);
+    dim3 grd((R.n + 255) / 256, ntr);
+    if (ref32)
+        hipLaunchKernelGGL(k_rhs<true>, grd, dim3(256), 0, st, cells, R, ntr, th, ph, k0, pi, V, ld);
+    else
+        hipLaunchKernelGGL(k_rhs<false>, grd, dim3(256), 0, st, cells, R, ntr, th, ph, k0, pi, V, ld);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace midas

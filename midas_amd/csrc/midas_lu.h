@@ -1,0 +1,49 @@
+// Dense LU workspace and entry points (see midas_lu.hip).
+#pragma once
+#include "midas_common.h"
+
+namespace midas {
+
+constexpr int LU_IB = 16;   // inner panel width (column-step kernels)
+
+template <typename T> struct LuWork {
+    int n = 0, nb = 128, maxblk = 0;
+    int jb = 0;                      // first column of the current inner block
+    T *pmax = nullptr;               // per-workgroup pivot candidates
+    int *pidx = nullptr;
+    int *counter = nullptr;          // last-workgroup counter
+    int *info = nullptr;             // first exactly-zero pivot (1-based), 0 if none
+    cplx<T> *pinv = nullptr;         // 1 / current pivot
+    cplx<T> *linv = nullptr;         // nb x nb inverse of the unit-lower diagonal block
+    cplx<T> *tmp = nullptr;          // nb x n staging for the U12 solve
+    int alloc(int n, int nb);
+    void release();
+};
+
+// Trailing update C(m x n) -= A(m x k) B(k x n), complex, column-major.
+enum TrailingMode { TRAIL_NATIVE = 0, TRAIL_F16_1 = 1, TRAIL_F16_3 = 3, TRAIL_BF16_1 = 11, TRAIL_BF16_3 = 13 };
+
+template <typename T> struct TrailingGemm {
+    int mode = TRAIL_NATIVE;
+    void *scratch = nullptr;        // per-column scales for the split kernels
+    int operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C, long ldc,
+                   hipStream_t st) const;
+};
+
+template <typename T> struct cvec;
+template <> struct cvec<double> { typedef double2 t; };
+template <> struct cvec<float> { typedef float2 t; };
+
+template <typename T>
+int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipStream_t st, TrailingGemm<T> tg);
+template <typename T>
+int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, typename cvec<T>::t *B, long ldb, int nrhs,
+          typename cvec<T>::t *tmp, int nb, hipStream_t st);
+void ipiv_to_perm(const int *ipiv, int n, int *perm);
+
+template <typename T>
+int launch_cgemm(int m, int n, int k, cplx<T> alpha, const void *A, long lda, int tA, const void *B, long ldb,
+                 int tB, cplx<T> beta, void *C, long ldc, hipStream_t st, int batch = 1, long sA = 0, long sB = 0,
+                 long sC = 0);
+
+}  // namespace midas

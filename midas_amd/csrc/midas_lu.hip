@@ -1,0 +1,415 @@
+// Dense complex LU with partial pivoting and triangular solves (gfx950).
+//
+// Replaces ScaLAPACK PZGESV (Compute_EFields_MoM.f90:173, Compute_EFields_CBFME.f90:912)
+// on one GPU: right-looking blocked LU, column-major, interleaved complex.
+//
+//   outer panel  nb columns : factored in inner blocks of IB columns;
+//   inner column step       : one fused kernel per column -- scale + rank-1 update
+//                             for the previous column, then the pivot search of
+//                             this column; the last workgroup to finish reduces
+//                             the per-workgroup maxima, swaps the pivot row across
+//                             the outer panel and records ipiv (LAPACK getf2 order);
+//   inner block update      : IBxIB unit-lower solve + GEMM on the rest of the panel;
+//   outside the panel       : row swaps (laswp), U12 = L11^-1 A12 through an
+//                             explicit L11^-1 and a GEMM, trailing GEMM A22 -= L21 U12.
+//
+// T = double is the FP64 baseline; T = float holds the complex-FP32 factors of
+// the mixed-precision solvers, whose trailing GEMM runs either on FP32 MFMA or
+// on the FP16/BF16 split kernels (midas_gemm16.hip).
+#include "midas_common.h"
+#include "midas_lu.h"
+
+namespace midas {
+
+constexpr int STEP_THREADS = 256;
+constexpr int STEP_ROWS = 1024;   // rows per workgroup in the column-step kernel
+
+// ---------------------------------------------------------------------------
+// Fused column step.  On entry column j-1 has been pivoted (row j-1 holds the
+// pivot row).  Rows [j, n): scale column j-1 by 1/pivot, update columns
+// [j, jend) with the rank-1 term, then find this workgroup's max |A(i, j)|.
+// The last workgroup reduces, swaps rows j <-> p over columns [pc0, pc1),
+// stores ipiv[j] = p and 1/pivot for the next step.  With j == jend only the
+// scaling of column j-1 is done.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t *A, long lda, int n, int j,
+                                                             int jend, int pc0, int pc1, int *ipiv,
+                                                             LuWork<T> w)
+{
+    typedef typename cvec<T>::t V;
+    __shared__ T smax[STEP_THREADS];
+    __shared__ int sidx[STEP_THREADS];
+    __shared__ V urow[64];
+    __shared__ int last;
+    const int tid = threadIdx.x;
+    const int r0 = j + blockIdx.x * STEP_ROWS;
+    const int r1 = min(n, r0 + STEP_ROWS);
+    const int ncol = jend - j;   // columns to update
+    cplx<T> pinv = {0, 0};
+    if (j > w.jb) {
+        pinv = w.pinv[0];
+        for (int c = tid; c < ncol; c += STEP_THREADS) urow[c] = A[(long)(j - 1) + (long)(j + c) * lda];
+    }
+    __syncthreads();
+    T best = (T)-1;
+    int bidx = 0x7fffffff;
+    for (int i = r0 + tid; i < r1; i += STEP_THREADS) {
+        if (j > w.jb) {
+            V a = A[(long)i + (long)(j - 1) * lda];
+            cplx<T> l = cmul(cplx<T>{a.x, a.y}, pinv);
+            A[(long)i + (long)(j - 1) * lda] = V{l.r, l.i};
+            for (int c = 0; c < ncol; ++c) {
+                V *p = A + (long)i + (long)(j + c) * lda;
+                V v = *p;
+                cplx<T> u = {urow[c].x, urow[c].y};
+                cplx<T> r = csub(cplx<T>{v.x, v.y}, cmul(l, u));
+                *p = V{r.r, r.i};
+            }
+        }
+        if (ncol > 0) {
+            V v = A[(long)i + (long)j * lda];
+            // |Re| + |Im|, the complex pivot measure of LAPACK i?amax (dcabs1)
+            T m2 = fabs(v.x) + fabs(v.y);
+            if (m2 > best || (m2 == best && i < bidx)) { best = m2; bidx = i; }
+        }
+    }
+    if (ncol == 0) return;
+    smax[tid] = best;
+    sidx[tid] = bidx;
+    __syncthreads();
+    for (int s = STEP_THREADS / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            T o = smax[tid + s];
+            int oi = sidx[tid + s];
+            if (o > smax[tid] || (o == smax[tid] && oi < sidx[tid])) { smax[tid] = o; sidx[tid] = oi; }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        w.pmax[blockIdx.x] = smax[0];
+        w.pidx[blockIdx.x] = sidx[0];
+        __threadfence();
+        int done = atomicAdd(w.counter, 1);
+        last = (done == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // last workgroup: global argmax (first index on ties, as i?amax)
+    if (tid == 0) {
+        T b = (T)-1;
+        int bi = j;
+        for (int q = 0; q < (int)gridDim.x; ++q) {
+            T v = ((volatile T *)w.pmax)[q];
+            int vi = ((volatile int *)w.pidx)[q];
+            if (v > b || (v == b && vi < bi)) { b = v; bi = vi; }
+        }
+        sidx[0] = bi;
+        ipiv[j] = bi;
+        *w.counter = 0;
+    }
+    __syncthreads();
+    const int p = sidx[0];
+    if (p != j) {
+        for (int c = pc0 + tid; c < pc1; c += STEP_THREADS) {
+            V *a = A + (long)j + (long)c * lda;
+            V *b = A + (long)p + (long)c * lda;
+            V t = *a;
+            *a = *b;
+            *b = t;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        V d = A[(long)j + (long)j * lda];
+        cplx<T> dv = {d.x, d.y};
+        if (d.x == (T)0 && d.y == (T)0) {
+            // exactly singular: record the first zero pivot (LAPACK INFO), keep going
+            atomicCAS(w.info, 0, j + 1);
+            w.pinv[0] = cplx<T>{0, 0};
+        } else {
+            w.pinv[0] = cinv(dv);
+        }
+    }
+}
+
+// Forward substitution with the unit-lower IBxIB block L(jb:jend, jb:jend) on
+// rows [jb, jend) of columns [c0, c1): one thread per column.
+template <typename T>
+__global__ void k_trsm_lunit_small(typename cvec<T>::t *A, long lda, int jb, int ib, int c0, int c1)
+{
+    typedef typename cvec<T>::t V;
+    int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= c1) return;
+    V *x = A + (long)jb + (long)c * lda;
+    const V *L = A + (long)jb + (long)jb * lda;
+    for (int r = 1; r < ib; ++r) {
+        cplx<T> acc = {x[r].x, x[r].y};
+        for (int q = 0; q < r; ++q) {
+            V l = L[(long)r + (long)q * lda];
+            acc = csub(acc, cmul(cplx<T>{l.x, l.y}, cplx<T>{x[q].x, x[q].y}));
+        }
+        x[r] = V{acc.r, acc.i};
+    }
+}
+
+// Apply the row interchanges ipiv[k0..k1) to columns [c0, c1) (and [c2, c3)).
+template <typename T>
+__global__ void k_laswp(typename cvec<T>::t *A, long lda, const int *ipiv, int k0, int k1, int c0, int c1, int c2,
+                        int c3)
+{
+    typedef typename cvec<T>::t V;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int w0 = c1 - c0;
+    int c = t < w0 ? c0 + t : c2 + (t - w0);
+    if (t >= w0 + (c3 - c2)) return;
+    V *col = A + (long)c * lda;
+    for (int j = k0; j < k1; ++j) {
+        int p = ipiv[j];
+        if (p != j) {
+            V a = col[j];
+            col[j] = col[p];
+            col[p] = a;
+        }
+    }
+}
+
+// Explicit inverse of the unit-lower nb x nb diagonal block: X = L^-1,
+// one thread per column of X, forward substitution.  X is dense nb x nb.
+template <typename T>
+__global__ void k_linv(const typename cvec<T>::t *A, long lda, int k, int nb, typename cvec<T>::t *X)
+{
+    typedef typename cvec<T>::t V;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nb) return;
+    const V *L = A + (long)k + (long)k * lda;
+    V *x = X + (long)c * nb;
+    for (int r = 0; r < nb; ++r) x[r] = V{(T)(r == c), 0};
+    for (int r = c + 1; r < nb; ++r) {
+        cplx<T> acc = {x[r].x, x[r].y};
+        for (int q = c; q < r; ++q) {
+            V l = L[(long)r + (long)q * lda];
+            acc = csub(acc, cmul(cplx<T>{l.x, l.y}, cplx<T>{x[q].x, x[q].y}));
+        }
+        x[r] = V{acc.r, acc.i};
+    }
+}
+
+// Copy a (rows x cols) block between column-major arrays.
+template <typename T>
+__global__ void k_copy2d(const typename cvec<T>::t *S, long lds, typename cvec<T>::t *D, long ldd, int rows, int cols)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (r < rows && c < cols) D[(long)r + (long)c * ldd] = S[(long)r + (long)c * lds];
+}
+
+template <typename T> static cplx<T> cone() { return {(T)1, (T)0}; }
+template <typename T> static cplx<T> cmone() { return {(T)-1, (T)0}; }
+template <typename T> static cplx<T> czero() { return {(T)0, (T)0}; }
+
+// Panel [k, k+nb) of an n x n matrix.
+template <typename T>
+static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ipiv, LuWork<T> &w, hipStream_t st)
+{
+    for (int jb = k; jb < k + nb; jb += LU_IB) {
+        int jend = min(jb + LU_IB, k + nb);
+        w.jb = jb;
+        for (int j = jb; j <= jend; ++j) {
+            int rows = n - j;
+            int nblk = rows > 0 ? (rows + STEP_ROWS - 1) / STEP_ROWS : 1;
+            MIDAS_CHECK(nblk <= w.maxblk, -2, "lu: step workspace too small");
+            hipLaunchKernelGGL(k_getf2_step<T>, dim3(nblk), dim3(STEP_THREADS), 0, st, A, lda, n, j,
+                               j < jend ? jend : j, k, k + nb, ipiv, w);
+        }
+        MIDAS_HIP(hipGetLastError());
+        // rest of the outer panel: columns [jend, k+nb)
+        int rc = k + nb - jend;
+        if (rc > 0) {
+            hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((rc + 63) / 64), dim3(64), 0, st, A, lda, jb, jend - jb,
+                               jend, k + nb);
+            MIDAS_HIP(hipGetLastError());
+            if (n - jend > 0) {
+                int r = launch_cgemm<T>(n - jend, rc, jend - jb, cmone<T>(), A + jend + (long)jb * lda, lda, 0,
+                                        A + jb + (long)jend * lda, lda, 0, cone<T>(), A + jend + (long)jend * lda,
+                                        lda, st, 1, 0, 0, 0);
+                if (r) return r;
+            }
+        }
+    }
+    return 0;
+}
+
+template <typename T>
+int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipStream_t st, TrailingGemm<T> tg)
+{
+    typedef typename cvec<T>::t V;
+    MIDAS_HIP(hipMemsetAsync(w.counter, 0, sizeof(int), st));
+    MIDAS_HIP(hipMemsetAsync(w.info, 0, sizeof(int), st));
+    for (int k = 0; k < n; k += w.nb) {
+        int nb = min(w.nb, n - k);
+        int r = panel<T>(A, lda, n, k, nb, ipiv, w, st);
+        if (r) return r;
+        // swaps outside the panel: columns [0, k) and [k+nb, n)
+        int ncols = k + (n - k - nb);
+        if (ncols > 0) {
+            hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + 255) / 256), dim3(256), 0, st, A, lda, ipiv, k, k + nb, 0,
+                               k, k + nb, n);
+            MIDAS_HIP(hipGetLastError());
+        }
+        int nr = n - k - nb;
+        if (nr <= 0) continue;
+        // U12 = L11^-1 A12
+        hipLaunchKernelGGL(k_linv<T>, dim3((nb + 63) / 64), dim3(64), 0, st, A, lda, k, nb, (V *)w.linv);
+        hipLaunchKernelGGL(k_copy2d<T>, dim3((nb + 255) / 256, nr), dim3(256), 0, st, A + k + (long)(k + nb) * lda,
+                           lda, (V *)w.tmp, (long)nb, nb, nr);
+        MIDAS_HIP(hipGetLastError());
+        r = launch_cgemm<T>(nb, nr, nb, cone<T>(), w.linv, nb, 0, w.tmp, nb, 0, czero<T>(),
+                            A + k + (long)(k + nb) * lda, lda, st, 1, 0, 0, 0);
+        if (r) return r;
+        // A22 -= L21 U12
+        r = tg(nr, nr, nb, A + (k + nb) + (long)k * lda, lda, A + k + (long)(k + nb) * lda, lda,
+               A + (k + nb) + (long)(k + nb) * lda, lda, st);
+        if (r) return r;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves with the factors (getrs), nrhs columns of B (ldb).
+// ---------------------------------------------------------------------------
+// Diagonal-block solve: rows [k, k+nb) of B; one thread per RHS column.
+template <typename T, bool LOWER>
+__global__ void k_trsv_block(const typename cvec<T>::t *A, long lda, int k, int nb, typename cvec<T>::t *B, long ldb,
+                             int nrhs)
+{
+    typedef typename cvec<T>::t V;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nrhs) return;
+    V *x = B + (long)k + (long)c * ldb;
+    const V *D = A + (long)k + (long)k * lda;
+    if (LOWER) {
+        for (int r = 1; r < nb; ++r) {
+            cplx<T> acc = {x[r].x, x[r].y};
+            for (int q = 0; q < r; ++q) {
+                V l = D[(long)r + (long)q * lda];
+                acc = csub(acc, cmul(cplx<T>{l.x, l.y}, cplx<T>{x[q].x, x[q].y}));
+            }
+            x[r] = V{acc.r, acc.i};
+        }
+    } else {
+        for (int r = nb - 1; r >= 0; --r) {
+            cplx<T> acc = {x[r].x, x[r].y};
+            for (int q = r + 1; q < nb; ++q) {
+                V u = D[(long)r + (long)q * lda];
+                acc = csub(acc, cmul(cplx<T>{u.x, u.y}, cplx<T>{x[q].x, x[q].y}));
+            }
+            V d = D[(long)r + (long)r * lda];
+            acc = cmul(acc, cinv(cplx<T>{d.x, d.y}));
+            x[r] = V{acc.r, acc.i};
+        }
+    }
+}
+
+// B <- P B with the final permutation perm (B2[i] = B[perm[i]]).
+template <typename T>
+__global__ void k_permute_rows(const typename cvec<T>::t *B, long ldb, typename cvec<T>::t *D, long ldd,
+                               const int *perm, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i < n) D[(long)i + (long)c * ldd] = B[(long)perm[i] + (long)c * ldb];
+}
+
+template <typename T>
+int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, typename cvec<T>::t *B, long ldb,
+          int nrhs, typename cvec<T>::t *tmp, int nb, hipStream_t st)
+{
+    typedef typename cvec<T>::t V;
+    if (n <= 0 || nrhs <= 0) return 0;
+    MIDAS_CHECK(nrhs <= 65535, -1, "getrs: nrhs too large for one launch");
+    hipLaunchKernelGGL(k_permute_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, tmp, (long)n, perm,
+                       n);
+    hipLaunchKernelGGL(k_copy2d<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, tmp, (long)n, B, ldb, n, nrhs);
+    MIDAS_HIP(hipGetLastError());
+    for (int k = 0; k < n; k += nb) {
+        int b = min(nb, n - k);
+        hipLaunchKernelGGL((k_trsv_block<T, true>), dim3((nrhs + 63) / 64), dim3(64), 0, st, A, lda, k, b, B, ldb,
+                           nrhs);
+        MIDAS_HIP(hipGetLastError());
+        if (n - k - b > 0) {
+            int r = launch_cgemm<T>(n - k - b, nrhs, b, cmone<T>(), A + (k + b) + (long)k * lda, lda, 0, B + k, ldb,
+                                    0, cone<T>(), B + k + b, ldb, st, 1, 0, 0, 0);
+            if (r) return r;
+        }
+    }
+    for (int k = ((n - 1) / nb) * nb; k >= 0; k -= nb) {
+        int b = min(nb, n - k);
+        hipLaunchKernelGGL((k_trsv_block<T, false>), dim3((nrhs + 63) / 64), dim3(64), 0, st, A, lda, k, b, B, ldb,
+                           nrhs);
+        MIDAS_HIP(hipGetLastError());
+        if (k > 0) {
+            int r = launch_cgemm<T>(k, nrhs, b, cmone<T>(), A + (long)k * lda, lda, 0, B + k, ldb, 0, cone<T>(), B,
+                                    ldb, st, 1, 0, 0, 0);
+            if (r) return r;
+        }
+    }
+    return 0;
+}
+
+// Final row permutation from LAPACK-style interchanges: perm[i] is the
+// original row that ends up in position i.
+void ipiv_to_perm(const int *ipiv, int n, int *perm)
+{
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int i = 0; i < n; ++i) {
+        int p = ipiv[i];
+        if (p != i) {
+            int t = perm[i];
+            perm[i] = perm[p];
+            perm[p] = t;
+        }
+    }
+}
+
+template <typename T>
+int LuWork<T>::alloc(int n_, int nb_)
+{
+    n = n_;
+    nb = nb_;
+    maxblk = (n + STEP_ROWS - 1) / STEP_ROWS + 1;
+    MIDAS_HIP(hipMalloc(&pmax, sizeof(T) * maxblk));
+    MIDAS_HIP(hipMalloc(&pidx, sizeof(int) * maxblk));
+    MIDAS_HIP(hipMalloc(&counter, sizeof(int) * 2));
+    info = counter + 1;
+    MIDAS_HIP(hipMalloc(&pinv, sizeof(cplx<T>)));
+    MIDAS_HIP(hipMalloc(&linv, sizeof(cplx<T>) * (size_t)nb * nb));
+    MIDAS_HIP(hipMalloc(&tmp, sizeof(cplx<T>) * (size_t)nb * (size_t)(n > 0 ? n : 1)));
+    return 0;
+}
+
+template <typename T>
+void LuWork<T>::release()
+{
+    hipFree(pmax);
+    hipFree(pidx);
+    hipFree(counter);
+    hipFree(pinv);
+    hipFree(linv);
+    hipFree(tmp);
+    pmax = nullptr;
+    pidx = counter = info = nullptr;
+    pinv = nullptr;
+    linv = tmp = nullptr;
+}
+
+template struct LuWork<double>;
+template struct LuWork<float>;
+template int getrf<double>(double2 *, long, int, int *, LuWork<double> &, hipStream_t, TrailingGemm<double>);
+template int getrf<float>(float2 *, long, int, int *, LuWork<float> &, hipStream_t, TrailingGemm<float>);
+template int getrs<double>(const double2 *, long, int, const int *, double2 *, long, int, double2 *, int,
+                           hipStream_t);
+template int getrs<float>(const float2 *, long, int, const int *, float2 *, long, int, float2 *, int, hipStream_t);
+
+}  // namespace midas

@@ -1,0 +1,102 @@
+"""GPU unit tests of the matrix-core building blocks against numpy."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from midas_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mid():
+    m = api.Midas(0)
+    yield m
+    m.close()
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _probe_expected(which):
+    K = 4 if which < 2 else 32
+    i = np.arange(16)[:, None]
+    k = np.arange(K)[None, :]
+    A = ((i + 2 * k) % 7 - 3).astype(float)
+    kk = np.arange(K)[:, None]
+    j = np.arange(16)[None, :]
+    B = ((3 * kk + j) % 5 - 2).astype(float)
+    return A @ B
+
+
+@pytest.mark.parametrize("which", [0, 1, 2, 3])
+def test_mfma_lane_layout(mid, which):
+    print(f"\n[case] mfma probe {which}", flush=True)
+    out = np.zeros(256)
+    r = mid.lib.midas_test_mfma_probe(mid.ctx, C.c_int(which), _p(out))
+    assert r == 0
+    D = out.reshape(16, 16, order="F")
+    E = _probe_expected(which)
+    if not np.array_equal(D, E):
+        print("got\n", D[:4, :6], "\nexpected\n", E[:4, :6])
+        print("transposed match:", np.array_equal(D, E.T))
+    assert np.array_equal(D, E)
+
+
+def _gemm(mid, prec, A, B, C0, alpha, beta, tA, tB):
+    m, n = C0.shape
+    k = A.shape[0] if tA else A.shape[1]
+    Cm = np.asfortranarray(C0.copy())
+    a = np.array([alpha.real, alpha.imag])
+    b = np.array([beta.real, beta.imag])
+    A = np.asfortranarray(A)
+    B = np.asfortranarray(B)
+    r = mid.lib.midas_test_gemm(mid.ctx, C.c_int(prec), C.c_int(m), C.c_int(n), C.c_int(k), _p(a), _p(A),
+                                C.c_long(A.shape[0]), C.c_int(tA), _p(B), C.c_long(B.shape[0]), C.c_int(tB), _p(b),
+                                _p(Cm), C.c_long(m))
+    assert r == 0, mid.lib.midas_last_error()
+    return Cm
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("shape", [(64, 64, 16), (100, 37, 53), (1, 5, 200), (130, 2, 300)])
+@pytest.mark.parametrize("tA,tB", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_gemm(mid, prec, shape, tA, tB):
+    m, n, k = shape
+    print(f"\n[case] gemm prec={prec} m={m} n={n} k={k} tA={tA} tB={tB}", flush=True)
+    rng = np.random.default_rng(m * 1000 + n + k)
+    cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    A = cr(k, m) if tA else cr(m, k)
+    B = cr(n, k) if tB else cr(k, n)
+    C0 = cr(m, n)
+    alpha, beta = 0.7 - 0.2j, -1.1 + 0.4j
+    got = _gemm(mid, prec, A, B, C0, alpha, beta, tA, tB)
+    ref = alpha * ((A.T if tA else A) @ (B.T if tB else B)) + beta * C0
+    err = np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * k + np.abs(C0).max())
+    print("max scaled err", err)
+    assert err < (1e-15 if prec == 64 else 3e-7)
+
+
+@pytest.mark.parametrize("mode,tol", [(1, 4e-3), (3, 1e-5), (11, 3e-2), (13, 2e-4)])
+def test_trailing_split(mid, mode, tol):
+    m, n, k = 200, 150, 96
+    print(f"\n[case] trailing split mode={mode}", flush=True)
+    rng = np.random.default_rng(mode)
+    cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    A = cr(m, k)
+    A /= np.abs(A).max() * 1.5      # |Re|,|Im| <= 1 as for LU multipliers
+    B = cr(k, n) * np.exp(rng.normal(size=(1, n)) * 3)   # columns of very different scale
+    C0 = cr(m, n)
+    Cm = np.asfortranarray(C0.copy())
+    A, B = np.asfortranarray(A), np.asfortranarray(B)
+    r = mid.lib.midas_test_trailing(mid.ctx, C.c_int(mode), C.c_int(m), C.c_int(n), C.c_int(k), _p(A), C.c_long(m),
+                                    _p(B), C.c_long(k), _p(Cm), C.c_long(m))
+    assert r == 0
+    ref = C0 - A @ B
+    # error relative to |A||B| per column (the scale the split precision resolves)
+    colscale = (np.abs(A).max() * np.abs(B).max(0) * k)[None, :]
+    err = (np.abs(Cm - ref) / (colscale + np.abs(C0).max() * 1e-7)).max()
+    print("mode", mode, "scaled err", err)
+    assert err < tol
