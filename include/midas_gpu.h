@@ -78,6 +78,15 @@ int midas_finalize(midas_ctx *ctx);
 const char *midas_last_error(void);
 void *midas_stream(midas_ctx *ctx);           /* hipStream_t of the context */
 
+/* Live kernel accounting: midas_profile(ctx, 1) starts recording HIP events
+ * around the trailing-update GEMM launches of the LU (kind 0) and the matrix
+ * fill launches (kind 1) on the context's stream; midas_profile_query sums
+ * their count, elapsed time, algorithmic flops and algorithmic bytes.  No
+ * reference counterpart (the reference prints wall-clock phase timers,
+ * Calcul_time_spent.f90:1-60). */
+int midas_profile(midas_ctx *ctx, int enable);
+int midas_profile_query(midas_ctx *ctx, int kind, int *launches, double *total_ms, double *flops, double *bytes);
+
 /* ---------------- a1: cell constants ----------------
  * SetCellsParams (SetCellsParams.f90:37-48): a_n, Che_n, Znnpp, Kappa_n. */
 int midas_cell_params(midas_ctx *ctx, int n, const double *sc, const double *eps, double k0, int quirks,

@@ -22,6 +22,12 @@ void set_error(const char *fmt, ...)
 }
 const char *last_error() { return g_err; }
 
+KernelProfile &profile()
+{
+    static KernelProfile p;
+    return p;
+}
+
 // ---------------------------------------------------------------------------
 // small device buffer owner
 // ---------------------------------------------------------------------------
@@ -323,7 +329,7 @@ static int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, floa
         hipLaunchKernelGGL(k_copyz, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, X, ldx, n, nrhs);
         DBuf t2;
         TRY(t2.alloc(sizeof(double2) * (size_t)n * nrhs));
-        r = getrs<double>(A, lda, n, perm_d.as<int>(), X, ldx, nrhs, t2.as<double2>(), nb, st);
+        r = getrs<double>(A, lda, n, perm_d.as<int>(), w, X, ldx, nrhs, t2.as<double2>(), st);
         w.release();
         if (r) return r;
         inf.t_solve_ms = tm.ms();
@@ -366,7 +372,7 @@ static int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, floa
         int it = 0;
         for (; it < max_iter; ++it) {
             hipLaunchKernelGGL(k_z2c, g2, dim3(256), 0, st, R.as<double2>(), (long)n, D.as<float2>(), (long)n, n, nrhs);
-            r = getrs<float>(A, lda, n, perm_d.as<int>(), D.as<float2>(), (long)n, nrhs, t2.as<float2>(), nb, st);
+            r = getrs<float>(A, lda, n, perm_d.as<int>(), w, D.as<float2>(), (long)n, nrhs, t2.as<float2>(), st);
             if (r) { w.release(); return r; }
             hipLaunchKernelGGL(k_c2z_add, g2, dim3(256), 0, st, D.as<float2>(), (long)n, X, ldx, n, nrhs, it > 0);
             hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R.as<double2>(), (long)n, n, nrhs);
@@ -439,6 +445,42 @@ int midas_finalize(midas_ctx *ctx)
 }
 
 void *midas_stream(midas_ctx *ctx) { return ctx ? (void *)ctx->st : nullptr; }
+
+int midas_profile(midas_ctx *ctx, int enable)
+{
+    MIDAS_CHECK(ctx, -1, "profile: ctx is NULL");
+    KernelProfile &p = profile();
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    for (auto &r : p.recs) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    p.recs.clear();
+    p.on = enable != 0;
+    return 0;
+}
+
+int midas_profile_query(midas_ctx *ctx, int kind, int *launches, double *total_ms, double *flops, double *bytes)
+{
+    MIDAS_CHECK(ctx, -1, "profile_query: ctx is NULL");
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    int cnt = 0;
+    double ms = 0, f = 0, b = 0;
+    for (auto &r : profile().recs) {
+        if (r.kind != kind) continue;
+        float t = 0;
+        MIDAS_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        ++cnt;
+        ms += t;
+        f += r.flops;
+        b += r.bytes;
+    }
+    if (launches) *launches = cnt;
+    if (total_ms) *total_ms = ms;
+    if (flops) *flops = f;
+    if (bytes) *bytes = b;
+    return 0;
+}
 
 int midas_cell_params(midas_ctx *ctx, int n, const double *sc, const double *eps, double k0, int quirks, double *a_n,
                       double *che, double *znnpp, double *kappa)

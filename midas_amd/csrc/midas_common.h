@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 
 #define MIDAS_PI_EXACT 3.14159265358979323846
 // Quirk bits (the reference's constant/rounding quirks, switchable)
@@ -68,6 +69,39 @@ struct DevCells {
     const double *znnpp;       // self term, interleaved complex
     const double *kchi;        // Kappa_n * Che_n, interleaved complex
     const int *id;             // cell identifier (n_cell)
+};
+
+// Live kernel accounting (HIP events around selected launches), used by the
+// benchmark to time the dominant kernel on the stream it runs on.
+struct KernelProfile {
+    bool on = false;
+    struct Rec { hipEvent_t a, b; double flops, bytes; int kind; };
+    std::vector<Rec> recs;
+};
+KernelProfile &profile();
+// kinds
+enum { PROF_TRAIL = 0, PROF_FILL = 1, PROF_PANEL = 2 };
+struct ProfScope {
+    hipEvent_t a{}, b{};
+    hipStream_t st;
+    double flops, bytes;
+    int kind;
+    bool on;
+    ProfScope(hipStream_t s, int k, double f, double by) : st(s), flops(f), bytes(by), kind(k), on(profile().on)
+    {
+        if (on) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, st);
+        }
+    }
+    ~ProfScope()
+    {
+        if (on) {
+            hipEventRecord(b, st);
+            profile().recs.push_back({a, b, flops, bytes, kind});
+        }
+    }
 };
 
 // A set of cells: contiguous range [start, start+n) or an index list.

@@ -156,6 +156,8 @@ int launch_fill(const DevCells &cells, CellSet R, CellSet S, double k0, int quir
     MIDAS_CHECK(!W32 || ld32 >= 3L * R.n, -1, "fill: ld32 %ld < 3*%d", ld32, R.n);
     MIDAS_CHECK((S.n + 3) / 4 <= 65535 * 64, -1, "fill: source set too large");
     double pi = host_pi(quirks);
+    const double elems = 9.0 * R.n * (double)S.n;
+    ProfScope ps(st, PROF_FILL, 125.0 * R.n * (double)S.n, elems * ((Z64 ? 16.0 : 0.0) + (W32 ? 8.0 : 0.0)));
     dim3 blk(64, 4);
     // grid.y is limited to 65535: split the source set in chunks
     const int ymax = 65535;

@@ -222,6 +222,7 @@ template <>
 int TrailingGemm<double>::operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C,
                                      long ldc, hipStream_t st) const
 {
+    ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 16.0 * (2.0 * m * n + (double)m * k + (double)k * n));
     return launch_cgemm<double>(m, n, k, z64{-1.0, 0.0}, A, lda, 0, B, ldb, 0, z64{1.0, 0.0}, C, ldc, st);
 }
 
@@ -229,6 +230,7 @@ template <>
 int TrailingGemm<float>::operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C,
                                     long ldc, hipStream_t st) const
 {
+    ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 8.0 * (2.0 * m * n + (double)m * k + (double)k * n));
     if (mode == TRAIL_NATIVE)
         return launch_cgemm<float>(m, n, k, c32{-1.f, 0.f}, A, lda, 0, B, ldb, 0, c32{1.f, 0.f}, C, ldc, st);
     return launch_trail16(mode, m, n, k, (const float2 *)A, lda, (const float2 *)B, ldb, (float2 *)C, ldc,

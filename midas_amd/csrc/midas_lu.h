@@ -14,8 +14,8 @@ template <typename T> struct LuWork {
     int *counter = nullptr;          // last-workgroup counter
     int *info = nullptr;             // first exactly-zero pivot (1-based), 0 if none
     cplx<T> *pinv = nullptr;         // 1 / current pivot
-    cplx<T> *linv = nullptr;         // nb x nb inverse of the unit-lower diagonal block
-    cplx<T> *tmp = nullptr;          // nb x n staging for the U12 solve
+    cplx<T> *linv = nullptr;         // inverses of the diagonal blocks of L (nb x nb each)
+    cplx<T> *uinv = nullptr;         // inverses of the diagonal blocks of U
     int alloc(int n, int nb);
     void release();
 };
@@ -36,9 +36,11 @@ template <> struct cvec<float> { typedef float2 t; };
 
 template <typename T>
 int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipStream_t st, TrailingGemm<T> tg);
+// Solve with the factors of getrf (same LuWork: it holds the diagonal-block
+// inverses); tmp needs max(n, nb) x nrhs elements.
 template <typename T>
-int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, typename cvec<T>::t *B, long ldb, int nrhs,
-          typename cvec<T>::t *tmp, int nb, hipStream_t st);
+int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, const LuWork<T> &w, typename cvec<T>::t *B,
+          long ldb, int nrhs, typename cvec<T>::t *tmp, hipStream_t st);
 void ipiv_to_perm(const int *ipiv, int n, int *perm);
 
 template <typename T>

@@ -22,7 +22,7 @@
 namespace midas {
 
 constexpr int STEP_THREADS = 256;
-constexpr int STEP_ROWS = 1024;   // rows per workgroup in the column-step kernel
+constexpr int STEP_ROWS = 512;    // rows per workgroup in the column-step kernel
 
 // ---------------------------------------------------------------------------
 // Fused column step.  On entry column j-1 has been pivoted (row j-1 holds the
@@ -96,18 +96,32 @@ __global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t
     __syncthreads();
     if (!last) return;
     __threadfence();
-    // last workgroup: global argmax (first index on ties, as i?amax)
-    if (tid == 0) {
+    // last workgroup: global argmax over the per-workgroup candidates
+    // (first index on ties, as i?amax)
+    {
         T b = (T)-1;
-        int bi = j;
-        for (int q = 0; q < (int)gridDim.x; ++q) {
+        int bi = 0x7fffffff;
+        for (int q = tid; q < (int)gridDim.x; q += STEP_THREADS) {
             T v = ((volatile T *)w.pmax)[q];
             int vi = ((volatile int *)w.pidx)[q];
             if (v > b || (v == b && vi < bi)) { b = v; bi = vi; }
         }
-        sidx[0] = bi;
-        ipiv[j] = bi;
-        *w.counter = 0;
+        smax[tid] = b;
+        sidx[tid] = bi;
+        __syncthreads();
+        for (int s2 = STEP_THREADS / 2; s2 > 0; s2 >>= 1) {
+            if (tid < s2) {
+                T o = smax[tid + s2];
+                int oi = sidx[tid + s2];
+                if (o > smax[tid] || (o == smax[tid] && oi < sidx[tid])) { smax[tid] = o; sidx[tid] = oi; }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            if (sidx[0] == 0x7fffffff) sidx[0] = j;
+            ipiv[j] = sidx[0];
+            *w.counter = 0;
+        }
     }
     __syncthreads();
     const int p = sidx[0];
@@ -175,24 +189,49 @@ __global__ void k_laswp(typename cvec<T>::t *A, long lda, const int *ipiv, int k
     }
 }
 
-// Explicit inverse of the unit-lower nb x nb diagonal block: X = L^-1,
-// one thread per column of X, forward substitution.  X is dense nb x nb.
-template <typename T>
-__global__ void k_linv(const typename cvec<T>::t *A, long lda, int k, int nb, typename cvec<T>::t *X)
+// Inverses of all diagonal nb x nb blocks of the factors, one thread per
+// (block, column): unit-lower L (LOWER) or upper U.  Inv holds block b as a
+// dense nb x nb column-major matrix at Inv + b*nb*nb (rows/cols beyond a short
+// last block are left untouched).
+template <typename T, bool LOWER>
+__global__ void k_tri_inv(const typename cvec<T>::t *A, long lda, int n, int nb, typename cvec<T>::t *Inv)
 {
     typedef typename cvec<T>::t V;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nb) return;
-    const V *L = A + (long)k + (long)k * lda;
-    V *x = X + (long)c * nb;
-    for (int r = 0; r < nb; ++r) x[r] = V{(T)(r == c), 0};
-    for (int r = c + 1; r < nb; ++r) {
-        cplx<T> acc = {x[r].x, x[r].y};
-        for (int q = c; q < r; ++q) {
-            V l = L[(long)r + (long)q * lda];
-            acc = csub(acc, cmul(cplx<T>{l.x, l.y}, cplx<T>{x[q].x, x[q].y}));
+    const int b = blockIdx.x;
+    const int c = blockIdx.y * blockDim.x + threadIdx.x;
+    const int k = b * nb;
+    const int bs = min(nb, n - k);
+    if (c >= bs) return;
+    const V *D = A + (long)k + (long)k * lda;
+    V *x = Inv + (long)b * nb * nb + (long)c * nb;
+    if (LOWER) {
+        for (int r = 0; r < c; ++r) x[r] = V{0, 0};
+        x[c] = V{1, 0};
+        for (int r = c + 1; r < bs; ++r) {
+            cplx<T> acc = {0, 0};
+            for (int q = c; q < r; ++q) {
+                V l = D[(long)r + (long)q * lda];
+                V xq = x[q];
+                acc = csub(acc, cmul(cplx<T>{l.x, l.y}, cplx<T>{xq.x, xq.y}));
+            }
+            x[r] = V{acc.r, acc.i};
         }
-        x[r] = V{acc.r, acc.i};
+    } else {
+        for (int r = c + 1; r < bs; ++r) x[r] = V{0, 0};
+        V d = D[(long)c + (long)c * lda];
+        cplx<T> xc = cinv(cplx<T>{d.x, d.y});
+        x[c] = V{xc.r, xc.i};
+        for (int r = c - 1; r >= 0; --r) {
+            cplx<T> acc = {0, 0};
+            for (int q = r + 1; q <= c; ++q) {
+                V u = D[(long)r + (long)q * lda];
+                V xq = x[q];
+                acc = csub(acc, cmul(cplx<T>{u.x, u.y}, cplx<T>{xq.x, xq.y}));
+            }
+            V dr = D[(long)r + (long)r * lda];
+            acc = cmul(acc, cinv(cplx<T>{dr.x, dr.y}));
+            x[r] = V{acc.r, acc.i};
+        }
     }
 }
 
@@ -260,58 +299,36 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
         }
         int nr = n - k - nb;
         if (nr <= 0) continue;
-        // U12 = L11^-1 A12
-        hipLaunchKernelGGL(k_linv<T>, dim3((nb + 63) / 64), dim3(64), 0, st, A, lda, k, nb, (V *)w.linv);
-        hipLaunchKernelGGL(k_copy2d<T>, dim3((nb + 255) / 256, nr), dim3(256), 0, st, A + k + (long)(k + nb) * lda,
-                           lda, (V *)w.tmp, (long)nb, nb, nr);
-        MIDAS_HIP(hipGetLastError());
-        r = launch_cgemm<T>(nb, nr, nb, cone<T>(), w.linv, nb, 0, w.tmp, nb, 0, czero<T>(),
-                            A + k + (long)(k + nb) * lda, lda, st, 1, 0, 0, 0);
-        if (r) return r;
+        // U12 = L11^-1 A12, blocked by the inner panel width
+        for (int jb = k; jb < k + nb; jb += LU_IB) {
+            int jend = min(jb + LU_IB, k + nb);
+            hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((nr + 63) / 64), dim3(64), 0, st, A, lda, jb, jend - jb,
+                               k + nb, n);
+            MIDAS_HIP(hipGetLastError());
+            if (jend < k + nb) {
+                r = launch_cgemm<T>(k + nb - jend, nr, jend - jb, cmone<T>(), A + jend + (long)jb * lda, lda, 0,
+                                    A + jb + (long)(k + nb) * lda, lda, 0, cone<T>(), A + jend + (long)(k + nb) * lda,
+                                    lda, st, 1, 0, 0, 0);
+                if (r) return r;
+            }
+        }
         // A22 -= L21 U12
         r = tg(nr, nr, nb, A + (k + nb) + (long)k * lda, lda, A + k + (long)(k + nb) * lda, lda,
                A + (k + nb) + (long)(k + nb) * lda, lda, st);
         if (r) return r;
     }
+    // inverses of the diagonal blocks, used by getrs
+    const int nblk = (n + w.nb - 1) / w.nb;
+    dim3 g(nblk, (w.nb + 63) / 64);
+    hipLaunchKernelGGL((k_tri_inv<T, true>), g, dim3(64), 0, st, A, lda, n, w.nb, (V *)w.linv);
+    hipLaunchKernelGGL((k_tri_inv<T, false>), g, dim3(64), 0, st, A, lda, n, w.nb, (V *)w.uinv);
+    MIDAS_HIP(hipGetLastError());
     return 0;
 }
 
 // ---------------------------------------------------------------------------
 // Triangular solves with the factors (getrs), nrhs columns of B (ldb).
 // ---------------------------------------------------------------------------
-// Diagonal-block solve: rows [k, k+nb) of B; one thread per RHS column.
-template <typename T, bool LOWER>
-__global__ void k_trsv_block(const typename cvec<T>::t *A, long lda, int k, int nb, typename cvec<T>::t *B, long ldb,
-                             int nrhs)
-{
-    typedef typename cvec<T>::t V;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nrhs) return;
-    V *x = B + (long)k + (long)c * ldb;
-    const V *D = A + (long)k + (long)k * lda;
-    if (LOWER) {
-        for (int r = 1; r < nb; ++r) {
-            cplx<T> acc = {x[r].x, x[r].y};
-            for (int q = 0; q < r; ++q) {
-                V l = D[(long)r + (long)q * lda];
-                acc = csub(acc, cmul(cplx<T>{l.x, l.y}, cplx<T>{x[q].x, x[q].y}));
-            }
-            x[r] = V{acc.r, acc.i};
-        }
-    } else {
-        for (int r = nb - 1; r >= 0; --r) {
-            cplx<T> acc = {x[r].x, x[r].y};
-            for (int q = r + 1; q < nb; ++q) {
-                V u = D[(long)r + (long)q * lda];
-                acc = csub(acc, cmul(cplx<T>{u.x, u.y}, cplx<T>{x[q].x, x[q].y}));
-            }
-            V d = D[(long)r + (long)r * lda];
-            acc = cmul(acc, cinv(cplx<T>{d.x, d.y}));
-            x[r] = V{acc.r, acc.i};
-        }
-    }
-}
-
 // B <- P B with the final permutation perm (B2[i] = B[perm[i]]).
 template <typename T>
 __global__ void k_permute_rows(const typename cvec<T>::t *B, long ldb, typename cvec<T>::t *D, long ldd,
@@ -323,35 +340,44 @@ __global__ void k_permute_rows(const typename cvec<T>::t *B, long ldb, typename 
 }
 
 template <typename T>
-int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, typename cvec<T>::t *B, long ldb,
-          int nrhs, typename cvec<T>::t *tmp, int nb, hipStream_t st)
+int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, const LuWork<T> &w,
+          typename cvec<T>::t *B, long ldb, int nrhs, typename cvec<T>::t *tmp, hipStream_t st)
 {
     typedef typename cvec<T>::t V;
+    const int nb = w.nb;
     if (n <= 0 || nrhs <= 0) return 0;
     MIDAS_CHECK(nrhs <= 65535, -1, "getrs: nrhs too large for one launch");
     hipLaunchKernelGGL(k_permute_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, tmp, (long)n, perm,
                        n);
     hipLaunchKernelGGL(k_copy2d<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, tmp, (long)n, B, ldb, n, nrhs);
     MIDAS_HIP(hipGetLastError());
+    // forward: B_k <- Linv_k B_k ; B_{>k} -= L_{>k,k} B_k
     for (int k = 0; k < n; k += nb) {
         int b = min(nb, n - k);
-        hipLaunchKernelGGL((k_trsv_block<T, true>), dim3((nrhs + 63) / 64), dim3(64), 0, st, A, lda, k, b, B, ldb,
+        const V *Li = (const V *)w.linv + (long)(k / nb) * nb * nb;
+        int r = launch_cgemm<T>(b, nrhs, b, cone<T>(), Li, nb, 0, B + k, ldb, 0, czero<T>(), tmp, nb, st, 1, 0, 0, 0);
+        if (r) return r;
+        hipLaunchKernelGGL(k_copy2d<T>, dim3((b + 255) / 256, nrhs), dim3(256), 0, st, tmp, (long)nb, B + k, ldb, b,
                            nrhs);
         MIDAS_HIP(hipGetLastError());
         if (n - k - b > 0) {
-            int r = launch_cgemm<T>(n - k - b, nrhs, b, cmone<T>(), A + (k + b) + (long)k * lda, lda, 0, B + k, ldb,
-                                    0, cone<T>(), B + k + b, ldb, st, 1, 0, 0, 0);
+            r = launch_cgemm<T>(n - k - b, nrhs, b, cmone<T>(), A + (k + b) + (long)k * lda, lda, 0, B + k, ldb, 0,
+                                cone<T>(), B + k + b, ldb, st, 1, 0, 0, 0);
             if (r) return r;
         }
     }
+    // backward: B_k <- Uinv_k B_k ; B_{<k} -= U_{<k,k} B_k
     for (int k = ((n - 1) / nb) * nb; k >= 0; k -= nb) {
         int b = min(nb, n - k);
-        hipLaunchKernelGGL((k_trsv_block<T, false>), dim3((nrhs + 63) / 64), dim3(64), 0, st, A, lda, k, b, B, ldb,
+        const V *Ui = (const V *)w.uinv + (long)(k / nb) * nb * nb;
+        int r = launch_cgemm<T>(b, nrhs, b, cone<T>(), Ui, nb, 0, B + k, ldb, 0, czero<T>(), tmp, nb, st, 1, 0, 0, 0);
+        if (r) return r;
+        hipLaunchKernelGGL(k_copy2d<T>, dim3((b + 255) / 256, nrhs), dim3(256), 0, st, tmp, (long)nb, B + k, ldb, b,
                            nrhs);
         MIDAS_HIP(hipGetLastError());
         if (k > 0) {
-            int r = launch_cgemm<T>(k, nrhs, b, cmone<T>(), A + (long)k * lda, lda, 0, B + k, ldb, 0, cone<T>(), B,
-                                    ldb, st, 1, 0, 0, 0);
+            r = launch_cgemm<T>(k, nrhs, b, cmone<T>(), A + (long)k * lda, lda, 0, B + k, ldb, 0, cone<T>(), B, ldb, st,
+                                1, 0, 0, 0);
             if (r) return r;
         }
     }
@@ -384,8 +410,9 @@ int LuWork<T>::alloc(int n_, int nb_)
     MIDAS_HIP(hipMalloc(&counter, sizeof(int) * 2));
     info = counter + 1;
     MIDAS_HIP(hipMalloc(&pinv, sizeof(cplx<T>)));
-    MIDAS_HIP(hipMalloc(&linv, sizeof(cplx<T>) * (size_t)nb * nb));
-    MIDAS_HIP(hipMalloc(&tmp, sizeof(cplx<T>) * (size_t)nb * (size_t)(n > 0 ? n : 1)));
+    const size_t nblk = (size_t)(n + nb - 1) / nb;
+    MIDAS_HIP(hipMalloc(&linv, sizeof(cplx<T>) * nblk * nb * nb));
+    MIDAS_HIP(hipMalloc(&uinv, sizeof(cplx<T>) * nblk * nb * nb));
     return 0;
 }
 
@@ -397,19 +424,20 @@ void LuWork<T>::release()
     hipFree(counter);
     hipFree(pinv);
     hipFree(linv);
-    hipFree(tmp);
+    hipFree(uinv);
     pmax = nullptr;
     pidx = counter = info = nullptr;
     pinv = nullptr;
-    linv = tmp = nullptr;
+    linv = uinv = nullptr;
 }
 
 template struct LuWork<double>;
 template struct LuWork<float>;
 template int getrf<double>(double2 *, long, int, int *, LuWork<double> &, hipStream_t, TrailingGemm<double>);
 template int getrf<float>(float2 *, long, int, int *, LuWork<float> &, hipStream_t, TrailingGemm<float>);
-template int getrs<double>(const double2 *, long, int, const int *, double2 *, long, int, double2 *, int,
-                           hipStream_t);
-template int getrs<float>(const float2 *, long, int, const int *, float2 *, long, int, float2 *, int, hipStream_t);
+template int getrs<double>(const double2 *, long, int, const int *, const LuWork<double> &, double2 *, long, int,
+                           double2 *, hipStream_t);
+template int getrs<float>(const float2 *, long, int, const int *, const LuWork<float> &, float2 *, long, int,
+                          float2 *, hipStream_t);
 
 }  // namespace midas
