@@ -24,8 +24,10 @@ int midas_test_trailing(midas_ctx *ctx, int mode, int m, int n, int k, const dou
                         long ldb, double *C, long ldc);
 
 /* Matrix-core lane layout probe: which = 0 (f64 16x16x4), 1 (f32 16x16x4),
- * 2 (f16 16x16x32), 3 (bf16 16x16x32).  out receives the 16x16 product of
- * A(i,k) = i + 1 + 0.5 k and B(k,j) = (k + 1) + 0.25 j (column-major 16x16). */
+ * 2 (f16 16x16x32), 3 (bf16 16x16x32), 4 (f16 32x32x16), 5 (bf16 32x32x16).
+ * out receives the product of A(i,k) = (i + 2k) % 7 - 3 and
+ * B(k,j) = (3k + j) % 5 - 2 read back with the lane layout the kernels assume
+ * (column-major 16x16, or 32x32 for which >= 4). */
 int midas_test_mfma_probe(midas_ctx *ctx, int which, double *out);
 
 #ifdef __cplusplus

@@ -347,7 +347,7 @@ static int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, floa
         LuWork<float> w;
         TRY(w.alloc(n, nb));
         DBuf scale;
-        TRY(scale.alloc(sizeof(float) * n));
+        TRY(scale.alloc(trail16_scratch_bytes(n, n, nb)));
         TrailingGemm<float> tg;
         tg.mode = trail_mode(mode);
         tg.scratch = scale.p;
@@ -406,7 +406,7 @@ static int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, floa
 namespace midas {
 int mfma_probe(int which, double *dout, hipStream_t st);
 int launch_trail16(int mode, int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
-                   long ldc, float *scale, hipStream_t st);
+                   long ldc, void *scratch, hipStream_t st);
 }
 
 // ===========================================================================
@@ -861,9 +861,8 @@ int midas_test_trailing(midas_ctx *ctx, int mode, int m, int n, int k, const dou
     TRY(to_c64(a32, dA, (long)m * k, st));
     TRY(to_c64(b32, dB, (long)k * n, st));
     TRY(to_c64(c32, dC, (long)m * n, st));
-    TRY(sc.alloc(sizeof(float) * n));
-    TRY(launch_trail16(mode, m, n, k, a32.as<float2>(), m, b32.as<float2>(), k, c32.as<float2>(), m, sc.as<float>(),
-                       st));
+    TRY(sc.alloc(trail16_scratch_bytes(m, n, k)));
+    TRY(launch_trail16(mode, m, n, k, a32.as<float2>(), m, b32.as<float2>(), k, c32.as<float2>(), m, sc.p, st));
     hipLaunchKernelGGL(k_c2z, dim3(((long)m * n + 255) / 256), dim3(256), 0, st, c32.as<float2>(), dC.as<double2>(),
                        (long)m * n);
     MIDAS_HIP(hipMemcpy2DAsync(C, sizeof(double2) * ldc, dC.p, sizeof(double2) * m, sizeof(double2) * m, n,
@@ -874,12 +873,13 @@ int midas_test_trailing(midas_ctx *ctx, int mode, int m, int n, int k, const dou
 
 int midas_test_mfma_probe(midas_ctx *ctx, int which, double *out)
 {
-    MIDAS_CHECK(ctx && out && which >= 0 && which <= 3, -1, "mfma_probe: bad arguments");
+    MIDAS_CHECK(ctx && out && which >= 0 && which <= 5, -1, "mfma_probe: bad arguments");
     MIDAS_HIP(hipSetDevice(ctx->device));
     DBuf d;
-    TRY(d.alloc(sizeof(double) * 256));
+    const int cnt = which >= 4 ? 1024 : 256;
+    TRY(d.alloc(sizeof(double) * cnt));
     TRY(mfma_probe(which, d.as<double>(), ctx->st));
-    MIDAS_HIP(hipMemcpyAsync(out, d.p, sizeof(double) * 256, hipMemcpyDeviceToHost, ctx->st));
+    MIDAS_HIP(hipMemcpyAsync(out, d.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, ctx->st));
     MIDAS_HIP(hipStreamSynchronize(ctx->st));
     return 0;
 }

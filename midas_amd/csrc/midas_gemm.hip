@@ -122,15 +122,23 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, cplx<T> alph
                 fbi[t] = Bi[kr][wn + 16 * t + (lane & 15)];
                 fbn[t] = -fbi[t];
             }
+            // independent accumulators back to back (no MFMA waits on the previous one)
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    accr[a][b] = mfma16<T>::op(far[a], fbr[b], accr[a][b]);
-                    accr[a][b] = mfma16<T>::op(fai[a], fbn[b], accr[a][b]);
-                    acci[a][b] = mfma16<T>::op(far[a], fbi[b], acci[a][b]);
-                    acci[a][b] = mfma16<T>::op(fai[a], fbr[b], acci[a][b]);
-                }
+                for (int b = 0; b < 2; ++b) accr[a][b] = mfma16<T>::op(far[a], fbr[b], accr[a][b]);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) acci[a][b] = mfma16<T>::op(far[a], fbi[b], acci[a][b]);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) accr[a][b] = mfma16<T>::op(fai[a], fbn[b], accr[a][b]);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) acci[a][b] = mfma16<T>::op(fai[a], fbr[b], acci[a][b]);
         }
     }
     // Epilogue: 16x16 MFMA output, lane holds 4 rows (mfma16<T>::row) of column lane%16.
@@ -205,6 +213,29 @@ __global__ void k_mfma_probe(int which, double *out)
     } else if (which == 1) {
         int i = l & 15, k = l >> 4;
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(pa(i, k), pb(k, l & 15), acc, 0, 0, 0);
+    } else if (which >= 4) {
+        // 32x32x16: lane l supplies A(l%32, 8*(l/32)+t) and B(8*(l/32)+t, l%32);
+        // D(8*(q/4) + 4*(l/32) + q%4, l%32) lands in register q
+        typedef float f16x __attribute__((ext_vector_type(16)));
+        f16x d = {0};
+        int i = l & 31, kb = 8 * (l >> 5);
+        if (which == 4) {
+            h8p a, b;
+            for (int t = 0; t < 8; ++t) {
+                a[t] = (_Float16)pa(i, kb + t);
+                b[t] = (_Float16)pb(kb + t, l & 31);
+            }
+            d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, d, 0, 0, 0);
+        } else {
+            b8p a, b;
+            for (int t = 0; t < 8; ++t) {
+                a[t] = (__bf16)pa(i, kb + t);
+                b[t] = (__bf16)pb(kb + t, l & 31);
+            }
+            d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, d, 0, 0, 0);
+        }
+        for (int q = 0; q < 16; ++q) out[(8 * (q >> 2) + 4 * (l >> 5) + (q & 3)) + 32 * (l & 31)] = d[q];
+        return;
     } else {
         int i = l & 15, kb = 8 * (l >> 4);
         if (which == 2) {

@@ -1,51 +1,58 @@
 // Split-precision trailing update of the complex-FP32 LU on FP16 / BF16
-// matrix cores (gfx950 v_mfma_f32_16x16x32_{f16,bf16}).
+// matrix cores (gfx950 v_mfma_f32_32x32x16_{f16,bf16}).
 //
 //   C(m x n) -= A(m x k) B(k x n)        C, A, B complex FP32 (interleaved)
 //
-// A = L21 has |Re|,|Im| <= 1 (partial pivoting bounds |Re|+|Im| of every
-// multiplier by 1).  B = U12 is scaled per column by an exact power of two so
-// that its entries are <= 1 in magnitude; the epilogue undoes the scale.
+// A = L21: partial pivoting on |Re|+|Im| bounds every multiplier's parts by
+// sqrt(2), so A needs no scaling.  B = U12 is scaled per column by an exact
+// power of two (entries <= 1 in magnitude); the epilogue undoes the scale.
 // Each operand element x is split as hi = rn16(x), lo = rn16(x - hi):
 //   terms = 1 :  A_hi B_hi
 //   terms = 3 :  A_hi B_hi + A_hi B_lo + A_lo B_hi        (~FP32-level products)
-// The split is done while staging tiles into LDS, so no converted copies of
-// the panels are kept in HBM.  Accumulation is FP32 in the MFMA accumulators;
-// the complex product uses 4 real MFMA products (4M) per term pair.
+// Accumulation is FP32 in the MFMA accumulators; the complex product uses 4
+// real products (4M) per term pair.
+//
+// Layout: two small split kernels write the operands once per update as
+// half-precision planes (plane = {hi, lo} x {re, im}) in a tile-major order,
+// so each 128-row x 16-k stage tile is one contiguous 4 KB run.  The GEMM
+// computes C^T = B^T A^T: the MFMA's row operand comes from B, its column
+// operand from A, so a lane's output column is a row of C and the epilogue's
+// read-modify-write of C is 256-byte contiguous per column.  Workgroup tile
+// 128 x 128 (four waves of 64 x 64 = 2 x 2 MFMA tiles of 32 x 32), LDS double
+// buffered, one stage = 16 k.  Tiles are ordered so that consecutive
+// workgroups of one XCD share a column block of B (L2 reuse).
 #include "midas_common.h"
 #include "midas_lu.h"
-#include <hip/hip_fp16.h>
-#include <hip/hip_bf16.h>
 
 namespace midas {
 
+typedef float f16v __attribute__((ext_vector_type(16)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef short s8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
 
-constexpr int H_M = 64, H_N = 64, H_K = 32, H_PAD = 8;
+constexpr int T_M = 128, T_N = 128, T_K = 16, T_ROW = T_K + 8;   // LDS row: 16 halves + 8 pad (48 B)
 
 template <bool BF> struct half_t;
 template <> struct half_t<false> {
-    typedef _Float16 t;
-    __device__ static inline t cvt(float x) { return (_Float16)x; }
-    __device__ static inline float back(t x) { return (float)x; }
+    __device__ static inline u16 cvt(float x) { return __builtin_bit_cast(u16, (_Float16)x); }
+    __device__ static inline float back(u16 x) { return (float)__builtin_bit_cast(_Float16, x); }
 };
 template <> struct half_t<true> {
-    typedef __bf16 t;
-    __device__ static inline t cvt(float x) { return (__bf16)x; }
-    __device__ static inline float back(t x) { return (float)x; }
+    __device__ static inline u16 cvt(float x) { return __builtin_bit_cast(u16, (__bf16)x); }
+    __device__ static inline float back(u16 x) { return (float)__builtin_bit_cast(__bf16, x); }
 };
 
-template <bool BF> __device__ inline f4v mma(s8 a, s8 b, f4v c);
-template <> __device__ inline f4v mma<false>(s8 a, s8 b, f4v c)
+template <bool BF> __device__ inline f16v mma32(s8 a, s8 b, f16v c);
+template <> __device__ inline f16v mma32<false>(s8 a, s8 b, f16v c)
 {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
 }
-template <> __device__ inline f4v mma<true>(s8 a, s8 b, f4v c)
+template <> __device__ inline f16v mma32<true>(s8 a, s8 b, f16v c)
 {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a), __builtin_bit_cast(b8, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, a), __builtin_bit_cast(b8, b), c, 0, 0, 0);
 }
 
 __device__ inline s8 neg8(s8 v)
@@ -56,166 +63,333 @@ __device__ inline s8 neg8(s8 v)
     return __builtin_bit_cast(s8, u);
 }
 
-// Per-column power-of-two scale of B: s_j = 2^-e with max(|Re|,|Im|) of column j in [2^(e-1), 2^e).
-__global__ void k_colscale(const float2 *B, long ldb, int k, int n, float *scale)
+// plane element offset (in halves): plane pc, row, k with tile-major order
+__host__ __device__ inline long plane_off(int pc, int row, int k, int rtiles, int kst)
 {
-    int j = blockIdx.x;
-    float m = 0.f;
-    for (int r = threadIdx.x; r < k; r += blockDim.x) {
-        float2 v = B[(long)r + (long)j * ldb];
-        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    return ((((long)pc * rtiles + row / T_M) * kst + k / T_K) * T_M + row % T_M) * T_K + k % T_K;
+}
+
+// A (m x K, lda) -> planes; thread per (row, group of 8 k)
+template <bool BF, int P>
+__global__ void k_split_a(const float2 *__restrict__ A, long lda, int m, int K, int rtiles, int kst, u16 *__restrict__ out)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k0 = blockIdx.y * 8;
+    if (row >= rtiles * T_M) return;
+    u16 h[2][2][8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        float2 v = (row < m && k0 + t < K) ? A[(long)row + (long)(k0 + t) * lda] : float2{0.f, 0.f};
+        u16 hr = half_t<BF>::cvt(v.x), hi = half_t<BF>::cvt(v.y);
+        h[0][0][t] = hr;
+        h[0][1][t] = hi;
+        if (P == 2) {
+            h[1][0][t] = half_t<BF>::cvt(v.x - half_t<BF>::back(hr));
+            h[1][1][t] = half_t<BF>::cvt(v.y - half_t<BF>::back(hi));
+        }
     }
-    __shared__ float red[256];
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            s8 v;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) v[t] = (short)h[p][c][t];
+            *(s8 *)(out + plane_off(p * 2 + c, row, k0, rtiles, kst)) = v;
+        }
+}
+
+// B (K x n, ldb) -> per-column power-of-two scale + planes; one wave per column
+template <bool BF, int P>
+__global__ __launch_bounds__(64) void k_split_b(const float2 *__restrict__ B, long ldb, int K, int n, int rtiles,
+                                                int kst, u16 *__restrict__ out, float *__restrict__ scale)
+{
+    const int col = blockIdx.x;
+    const int t = threadIdx.x;
+    const int k0 = 8 * t;
+    const int Kp = kst * T_K;
+    float2 v[8];
+    float mx = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        v[q] = (col < n && k0 + q < K) ? B[(long)(k0 + q) + (long)col * ldb] : float2{0.f, 0.f};
+        mx = fmaxf(mx, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
     }
-    if (threadIdx.x == 0) {
-        int e = 0;
-        float mm = red[0];
-        if (mm > 0.f && isfinite(mm)) frexpf(mm, &e);
-        scale[j] = ldexpf(1.0f, -e);
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    int e = 0;
+    if (mx > 0.f && isfinite(mx)) frexpf(mx, &e);
+    const float s = ldexpf(1.0f, -e);
+    if (t == 0 && col < n) scale[col] = s;
+    if (k0 >= Kp) return;
+    u16 h[2][2][8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        float xr = v[q].x * s, xi = v[q].y * s;
+        u16 hr = half_t<BF>::cvt(xr), hi = half_t<BF>::cvt(xi);
+        h[0][0][q] = hr;
+        h[0][1][q] = hi;
+        if (P == 2) {
+            h[1][0][q] = half_t<BF>::cvt(xr - half_t<BF>::back(hr));
+            h[1][1][q] = half_t<BF>::cvt(xi - half_t<BF>::back(hi));
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            s8 w;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) w[q] = (short)h[p][c][q];
+            *(s8 *)(out + plane_off(p * 2 + c, col, k0, rtiles, kst)) = w;
+        }
+}
+
+// C -= A B from the planes.  TERMS pairs: (hi,hi) [, (B hi, A lo), (B lo, A hi)].
+// Stage tiles go straight from global memory to LDS (global_load_lds_dwordx4:
+// each wave moves 1 KB of a 4 KB plane tile, no register staging) through a
+// ring of NS LDS slots: the copies of the next NS-1 stages are in flight while
+// a stage computes, which covers the memory latency with one wave per SIMD.
+// LDS rows are 32 B (16 halves), unpadded: the fragment reads (lane -> row
+// l%32, 16-byte chunk l/32) and the copies are bank-conflict free.
+template <int SPS, int NPL> struct StageLds {
+    u16 a[NPL][SPS][T_M][T_K];
+    u16 b[NPL][SPS][T_N][T_K];
+};
+
+// Four LDS ring slots as distinct objects, so the compiler can tell a slot
+// being refilled by the copy engine from the slot being read.
+__shared__ __attribute__((aligned(16))) unsigned char g_slot0[32768];
+__shared__ __attribute__((aligned(16))) unsigned char g_slot1[32768];
+__shared__ __attribute__((aligned(16))) unsigned char g_slot2[32768];
+__shared__ __attribute__((aligned(16))) unsigned char g_slot3[32768];
+template <int I, typename S> __device__ inline S *ring_slot()
+{
+    if constexpr (I == 0) return (S *)g_slot0;
+    else if constexpr (I == 1) return (S *)g_slot1;
+    else if constexpr (I == 2) return (S *)g_slot2;
+    else return (S *)g_slot3;
+}
+
+// s_waitcnt for vector-memory count <= N only (gfx9 encoding)
+#define MIDAS_WAIT_VM(N) __builtin_amdgcn_s_waitcnt(((N) & 15) | (((N) >> 4) << 14) | (7 << 4) | (15 << 8))
+
+template <bool BF, int P, int TERMS>
+__global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int mt, int nt, const u16 *__restrict__ Ap,
+                                                    const u16 *__restrict__ Bp, const float *__restrict__ bscale,
+                                                    float2 *__restrict__ C, long ldc)
+{
+    constexpr int NPL = 2 * P;          // planes per operand
+    constexpr int SPS = P == 1 ? 2 : 1; // 16-k granules per stage
+    constexpr int NS = 4;               // LDS ring slots
+    constexpr int IPS = NPL * SPS * 2;  // copy instructions per wave per stage
+    typedef StageLds<SPS, NPL> Stage;
+    static_assert(sizeof(Stage) <= 32768, "stage larger than an LDS slot");
+
+    // Tile order.  Each XCD (workgroup ids are dealt round-robin to the 8
+    // XCDs) walks a contiguous range of the sequence L; L runs over groups of
+    // 16 row tiles, all column tiles of a group, row tiles fastest.  A group's
+    // A tiles (16 x 128 KB) stay in the XCD's L2 while its column tiles stream.
+    const int total = mt * nt;
+    const int id = blockIdx.x;
+    const int full = (total / 8) * 8;
+    const int L = id < full ? (id % 8) * (full / 8) + id / 8 : id;
+    constexpr int GROUP = 16;
+    const int grp = L / (GROUP * nt);
+    const int rows_g = min(GROUP, mt - grp * GROUP);
+    const int li = L - grp * GROUP * nt;
+    const int tm = grp * GROUP + li % rows_g, tn = li / rows_g;
+    const int m0 = tm * T_M, n0 = tn * T_N;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar)
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const int nst = kst / SPS;
+
+    f16v accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                accr[a][b][q] = 0.f;
+                acci[a][b][q] = 0.f;
+            }
+
+    // One wave copies 1 KB (64 lanes x 16 B) of a plane tile into LDS at
+    // M0 + lane*16.  Written as one asm statement that also sets M0, so the
+    // copy is invisible to the compiler's wait-count pass (which would
+    // otherwise drain every copy in flight before each LDS read); completion
+    // is awaited explicitly with MIDAS_WAIT_VM before the slot is read.  The
+    // instruction has no register result, so nothing is left in flight in a
+    // register.  Nothing else in this kernel uses M0.
+    typedef __attribute__((address_space(3))) void lds_void;
+    auto copy16 = [&](const u16 *gsrc, u16 *ldst) {
+        const unsigned lds = (unsigned)(size_t)(lds_void *)ldst;
+        asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(gsrc), "s"(lds) : "memory");
+    };
+    auto dma = [&](Stage *dst, int st) {
+#pragma unroll
+        for (int pc = 0; pc < NPL; ++pc)
+#pragma unroll
+            for (int g = 0; g < SPS; ++g) {
+                const int ks = st * SPS + g;
+                // scalar tile base + per-lane 16-byte offset
+                const u16 *ga = Ap + ((((long)pc * mt + tm) * kst + ks) * T_M) * T_K + wave * 512;
+                const u16 *gb = Bp + ((((long)pc * nt + tn) * kst + ks) * T_N) * T_K + wave * 512;
+                copy16(ga + lane * 8, &dst->a[pc][g][0][0] + wave * 512);
+                copy16(gb + lane * 8, &dst->b[pc][g][0][0] + wave * 512);
+            }
+    };
+    auto compute = [&](const Stage *src) {
+        const int r32 = lane & 31, kc = (lane >> 5) * 8;
+#pragma unroll
+        for (int g = 0; g < SPS; ++g) {
+            // MFMA row operand from B (rows = columns of C), column operand from A;
+            // B fragments for the whole granule, A fragments one row tile at a time
+            s8 fb[NPL][2], fbn[P][2];
+#pragma unroll
+            for (int pc = 0; pc < NPL; ++pc)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) fb[pc][t] = *(const s8 *)&src->b[pc][g][wn + 32 * t + r32][kc];
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) fbn[p][t] = neg8(fb[2 * p + 1][t]);
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                s8 fa[NPL];
+#pragma unroll
+                for (int pc = 0; pc < NPL; ++pc) fa[pc] = *(const s8 *)&src->a[pc][g][wm + 32 * a + r32][kc];
+                // four independent accumulators per product part (the b tiles x re/im)
+#pragma unroll
+                for (int tp = 0; tp < TERMS; ++tp) {
+                    const int pb = (tp == 2) ? 1 : 0;   // B plane set (hi / lo)
+                    const int pa = (tp == 1) ? 1 : 0;   // A plane set
+                    // C^T: Dr = Br^T Ar^T - Bi^T Ai^T, Di = Br^T Ai^T + Bi^T Ar^T
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        accr[a][b] = mma32<BF>(fb[2 * pb][b], fa[2 * pa], accr[a][b]);
+                        acci[a][b] = mma32<BF>(fb[2 * pb][b], fa[2 * pa + 1], acci[a][b]);
+                    }
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        accr[a][b] = mma32<BF>(fbn[pb][b], fa[2 * pa + 1], accr[a][b]);
+                        acci[a][b] = mma32<BF>(fb[2 * pb + 1][b], fa[2 * pa], acci[a][b]);
+                    }
+                }
+            }
+        }
+    };
+    // one ring step: wait for stage s, refill the slot freed by stage s-1, compute stage s
+    auto step = [&](int s, const Stage *cur, Stage *refill) {
+        if (s + NS - 2 < nst) MIDAS_WAIT_VM(IPS * (NS - 2));
+        else MIDAS_WAIT_VM(0);
+        // plain barrier: this wave's copies of stage s are complete (wait
+        // above); a fenced __syncthreads would also drain the copies still
+        // in flight for the later stages
+        asm volatile("s_barrier" ::: "memory");
+        if (s + NS - 1 < nst) dma(refill, s + NS - 1);
+        compute(cur);
+    };
+
+    // nst is a multiple of NS (the planes are zero-padded)
+    Stage *s0 = ring_slot<0, Stage>(), *s1 = ring_slot<1, Stage>(), *s2 = ring_slot<2, Stage>(),
+          *s3 = ring_slot<3, Stage>();
+    dma(s0, 0);
+    dma(s1, 1);
+    dma(s2, 2);
+    for (int s = 0; s < nst; s += NS) {
+        step(s, s0, s3);
+        step(s + 1, s1, s0);
+        step(s + 2, s2, s1);
+        step(s + 3, s3, s2);
+    }
+    // epilogue: 32x32 output, lane holds column lane%32 of D = row of C;
+    // register q holds D row 8*(q/4) + 4*(lane/32) + q%4 = column of C.
+    // All loads of a row tile are issued before any use (one wave per SIMD:
+    // nothing else hides their latency); C is streamed non-temporally so it
+    // does not evict the operand planes other workgroups still read.
+    float inv[2][16];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int col = n0 + wn + 32 * b + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+            inv[b][q] = col < n ? bscale[col] : 1.0f;
+        }
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) inv[b][q] = 1.0f / inv[b][q];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int row = m0 + wm + 32 * a + (lane & 31);
+        double cv[2][16];
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int col = n0 + wn + 32 * b + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                cv[b][q] = (row < m && col < n)
+                               ? __builtin_nontemporal_load((const double *)(C + (long)row + (long)col * ldc))
+                               : 0.0;
+            }
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int col = n0 + wn + 32 * b + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                if (row < m && col < n) {
+                    float2 c0 = __builtin_bit_cast(float2, cv[b][q]);
+                    float2 c1 = float2{c0.x - accr[a][b][q] * inv[b][q], c0.y - acci[a][b][q] * inv[b][q]};
+                    __builtin_nontemporal_store(__builtin_bit_cast(double, c1), (double *)(C + (long)row + (long)col * ldc));
+                }
+            }
     }
 }
 
-template <bool BF, int TERMS>
-__global__ __launch_bounds__(256) void k_trail16(int m, int n, int k, const float2 *__restrict__ A, long lda,
-                                                 const float2 *__restrict__ B, long ldb, float2 *__restrict__ C,
-                                                 long ldc, const float *__restrict__ bscale)
+// 16-k granules, rounded up to a multiple of 8 = ring slots x granules per stage
+// for both variants (split planes are zero-padded)
+static inline int kstages(int k) { return ((k + 8 * T_K - 1) / (8 * T_K)) * 8; }
+
+size_t trail16_scratch_bytes(int mmax, int nmax, int kmax)
 {
-    typedef typename half_t<BF>::t H;
-    constexpr int NP = TERMS == 1 ? 1 : 2;   // planes per component (hi, lo)
-    // A planes [p][re/im][row][k], B planes [p][re/im][col][k]; k contiguous
-    __shared__ __attribute__((aligned(16))) H As[NP][2][H_M][H_K + H_PAD];
-    __shared__ __attribute__((aligned(16))) H Bs[NP][2][H_N][H_K + H_PAD];
+    const size_t rt_m = (mmax + T_M - 1) / T_M, rt_n = (nmax + T_N - 1) / T_N, kst = kstages(kmax);
+    return 2 * 4 * (rt_m * T_M + rt_n * T_N) * kst * T_K + 4 * rt_n * T_N + 256;
+}
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int i0 = blockIdx.x * H_M, j0 = blockIdx.y * H_N;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-
-    f4v accr[2][2], acci[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            accr[a][b] = f4v{0, 0, 0, 0};
-            acci[a][b] = f4v{0, 0, 0, 0};
-        }
-
-    // staging: A tile 64 x 32 complex = 2048 values, 8 per thread; B tile 32 x 64 the same
-    float2 ra[8], rb[8];
-    float bs[8];
-    auto gload = [&](int k0) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            int r = tid & 63, kk = (tid >> 6) + 4 * q;   // A: rows contiguous
-            int gi = i0 + r, gk = k0 + kk;
-            ra[q] = (gi < m && gk < k) ? A[(long)gi + (long)gk * lda] : float2{0.f, 0.f};
-            int kb = tid & 31, c = (tid >> 5) + 8 * q;     // B: k contiguous
-            int gj = j0 + c, gkb = k0 + kb;
-            rb[q] = (gj < n && gkb < k) ? B[(long)gkb + (long)gj * ldb] : float2{0.f, 0.f};
-        }
-    };
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        int gj = j0 + (tid >> 5) + 8 * q;
-        bs[q] = gj < n ? bscale[gj] : 1.0f;
-    }
-    auto sstore = [&]() {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            int r = tid & 63, kk = (tid >> 6) + 4 * q;
-            H hr = half_t<BF>::cvt(ra[q].x), hi = half_t<BF>::cvt(ra[q].y);
-            As[0][0][r][kk] = hr;
-            As[0][1][r][kk] = hi;
-            if (NP == 2) {
-                As[NP - 1][0][r][kk] = half_t<BF>::cvt(ra[q].x - half_t<BF>::back(hr));
-                As[NP - 1][1][r][kk] = half_t<BF>::cvt(ra[q].y - half_t<BF>::back(hi));
-            }
-            int kb = tid & 31, c = (tid >> 5) + 8 * q;
-            float xr = rb[q].x * bs[q], xi = rb[q].y * bs[q];
-            H br = half_t<BF>::cvt(xr), bi = half_t<BF>::cvt(xi);
-            Bs[0][0][c][kb] = br;
-            Bs[0][1][c][kb] = bi;
-            if (NP == 2) {
-                Bs[NP - 1][0][c][kb] = half_t<BF>::cvt(xr - half_t<BF>::back(br));
-                Bs[NP - 1][1][c][kb] = half_t<BF>::cvt(xi - half_t<BF>::back(bi));
-            }
-        }
-    };
-
-    gload(0);
-    for (int k0 = 0; k0 < k; k0 += H_K) {
-        __syncthreads();
-        sstore();
-        __syncthreads();
-        if (k0 + H_K < k) gload(k0 + H_K);
-        const int ko = 8 * (lane >> 4);
-        s8 fa[NP][2][2], fb[NP][2][2], fbn[NP][2];
-#pragma unroll
-        for (int p = 0; p < NP; ++p)
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    fa[p][c][t] = *(const s8 *)&As[p][c][wm + 16 * t + (lane & 15)][ko];
-                    fb[p][c][t] = *(const s8 *)&Bs[p][c][wn + 16 * t + (lane & 15)][ko];
-                }
-                fbn[p][t] = neg8(fb[p][1][t]);
-            }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                // term pairs (pa, pb): (0,0) [, (0,1), (1,0)]
-#pragma unroll
-                for (int tp = 0; tp < TERMS; ++tp) {
-                    const int pa = (tp == 2) ? NP - 1 : 0;
-                    const int pb = (tp == 1) ? NP - 1 : 0;
-                    accr[a][b] = mma<BF>(fa[pa][0][a], fb[pb][0][b], accr[a][b]);
-                    accr[a][b] = mma<BF>(fa[pa][1][a], fbn[pb][b], accr[a][b]);
-                    acci[a][b] = mma<BF>(fa[pa][0][a], fb[pb][1][b], acci[a][b]);
-                    acci[a][b] = mma<BF>(fa[pa][1][a], fb[pb][0][b], acci[a][b]);
-                }
-            }
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            int col = j0 + wn + 16 * b + (lane & 15);
-            if (col >= n) continue;
-            float inv = 1.0f / bscale[col];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                int row = i0 + wm + 16 * a + 4 * (lane >> 4) + q;
-                if (row < m) {
-                    float2 *pc = C + (long)row + (long)col * ldc;
-                    float2 c0 = *pc;
-                    *pc = float2{c0.x - accr[a][b][q] * inv, c0.y - acci[a][b][q] * inv};
-                }
-            }
-        }
+template <bool BF, int P, int TERMS>
+static int run_trail(int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C, long ldc,
+                     char *scratch, hipStream_t st)
+{
+    const int mt = (m + T_M - 1) / T_M, nt = (n + T_N - 1) / T_N, kst = kstages(k);
+    u16 *Ap = (u16 *)scratch;
+    u16 *Bp = Ap + (size_t)2 * P * mt * T_M * kst * T_K;
+    float *sc = (float *)(Bp + (size_t)2 * P * nt * T_N * kst * T_K);
+    MIDAS_CHECK(kst * T_K <= 512, -1, "trail16: k = %d above 512", k);
+    hipLaunchKernelGGL((k_split_a<BF, P>), dim3(mt * T_M / 256 + 1, kst * 2), dim3(256), 0, st, A, lda, m, k, mt, kst,
+                       Ap);
+    hipLaunchKernelGGL((k_split_b<BF, P>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, kst, Bp, sc);
+    hipLaunchKernelGGL((k_trail16<BF, P, TERMS>), dim3(mt * nt), dim3(256), 0, st, m, n, kst, mt, nt, Ap, Bp, sc, C,
+                       ldc);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
 }
 
 int launch_trail16(int mode, int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
-                   long ldc, float *scale, hipStream_t st)
+                   long ldc, void *scratch, hipStream_t st)
 {
     if (m <= 0 || n <= 0 || k <= 0) return 0;
-    MIDAS_CHECK((n + H_N - 1) / H_N <= 65535, -1, "trail16: n too large");
-    hipLaunchKernelGGL(k_colscale, dim3(n), dim3(256), 0, st, B, ldb, k, n, scale);
-    dim3 grd((m + H_M - 1) / H_M, (n + H_N - 1) / H_N);
+    MIDAS_CHECK(scratch, -1, "trail16: no scratch");
+    char *s = (char *)scratch;
     switch (mode) {
-    case TRAIL_F16_1: hipLaunchKernelGGL((k_trail16<false, 1>), grd, dim3(256), 0, st, m, n, k, A, lda, B, ldb, C, ldc, scale); break;
-    case TRAIL_F16_3: hipLaunchKernelGGL((k_trail16<false, 3>), grd, dim3(256), 0, st, m, n, k, A, lda, B, ldb, C, ldc, scale); break;
-    case TRAIL_BF16_1: hipLaunchKernelGGL((k_trail16<true, 1>), grd, dim3(256), 0, st, m, n, k, A, lda, B, ldb, C, ldc, scale); break;
-    case TRAIL_BF16_3: hipLaunchKernelGGL((k_trail16<true, 3>), grd, dim3(256), 0, st, m, n, k, A, lda, B, ldb, C, ldc, scale); break;
-    default: MIDAS_CHECK(false, -1, "trail16: bad mode %d", mode);
+    case TRAIL_F16_1: return run_trail<false, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st);
+    case TRAIL_F16_3: return run_trail<false, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st);
+    case TRAIL_BF16_1: return run_trail<true, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st);
+    case TRAIL_BF16_3: return run_trail<true, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st);
     }
-    MIDAS_HIP(hipGetLastError());
-    return 0;
+    MIDAS_CHECK(false, -1, "trail16: bad mode %d", mode);
 }
 
 template <>
@@ -233,8 +407,8 @@ int TrailingGemm<float>::operator()(int m, int n, int k, const void *A, long lda
     ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 8.0 * (2.0 * m * n + (double)m * k + (double)k * n));
     if (mode == TRAIL_NATIVE)
         return launch_cgemm<float>(m, n, k, c32{-1.f, 0.f}, A, lda, 0, B, ldb, 0, c32{1.f, 0.f}, C, ldc, st);
-    return launch_trail16(mode, m, n, k, (const float2 *)A, lda, (const float2 *)B, ldb, (float2 *)C, ldc,
-                          (float *)scratch, st);
+    return launch_trail16(mode, m, n, k, (const float2 *)A, lda, (const float2 *)B, ldb, (float2 *)C, ldc, scratch,
+                          st);
 }
 
 }  // namespace midas

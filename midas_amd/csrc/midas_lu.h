@@ -16,6 +16,8 @@ template <typename T> struct LuWork {
     cplx<T> *pinv = nullptr;         // 1 / current pivot
     cplx<T> *linv = nullptr;         // inverses of the diagonal blocks of L (nb x nb each)
     cplx<T> *uinv = nullptr;         // inverses of the diagonal blocks of U
+    hipStream_t pstream = nullptr;   // panel stream (look-ahead), default priority
+    hipEvent_t ev_panel = nullptr, ev_next = nullptr;
     int alloc(int n, int nb);
     void release();
 };
@@ -25,10 +27,12 @@ enum TrailingMode { TRAIL_NATIVE = 0, TRAIL_F16_1 = 1, TRAIL_F16_3 = 3, TRAIL_BF
 
 template <typename T> struct TrailingGemm {
     int mode = TRAIL_NATIVE;
-    void *scratch = nullptr;        // per-column scales for the split kernels
+    void *scratch = nullptr;        // split operand planes + column scales (trail16_scratch_bytes)
     int operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C, long ldc,
                    hipStream_t st) const;
 };
+
+size_t trail16_scratch_bytes(int mmax, int nmax, int kmax);
 
 template <typename T> struct cvec;
 template <> struct cvec<double> { typedef double2 t; };

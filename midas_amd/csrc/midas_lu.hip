@@ -280,16 +280,27 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
     return 0;
 }
 
+// Look-ahead: the panel of step k+1 is factored on a second stream while the
+// main stream applies the rest of step k's trailing update.  Per step, on the
+// main stream: wait for panel k, row swaps outside it, U12 = L11^-1 A12,
+// update of the next panel's columns (then signal the panel stream), update
+// of the remaining columns.  The panel stream only touches the next panel's
+// columns, which the remaining update never reads or writes.
 template <typename T>
 int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipStream_t st, TrailingGemm<T> tg)
 {
     typedef typename cvec<T>::t V;
+    hipStream_t ps = w.pstream;
     MIDAS_HIP(hipMemsetAsync(w.counter, 0, sizeof(int), st));
     MIDAS_HIP(hipMemsetAsync(w.info, 0, sizeof(int), st));
+    MIDAS_HIP(hipEventRecord(w.ev_next, st));
+    MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
+    int r = panel<T>(A, lda, n, 0, min(w.nb, n), ipiv, w, ps);
+    if (r) return r;
+    MIDAS_HIP(hipEventRecord(w.ev_panel, ps));
     for (int k = 0; k < n; k += w.nb) {
         int nb = min(w.nb, n - k);
-        int r = panel<T>(A, lda, n, k, nb, ipiv, w, st);
-        if (r) return r;
+        MIDAS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
         // swaps outside the panel: columns [0, k) and [k+nb, n)
         int ncols = k + (n - k - nb);
         if (ncols > 0) {
@@ -312,11 +323,23 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
                 if (r) return r;
             }
         }
-        // A22 -= L21 U12
-        r = tg(nr, nr, nb, A + (k + nb) + (long)k * lda, lda, A + k + (long)(k + nb) * lda, lda,
+        // A22 -= L21 U12: first the next panel's columns, then the rest
+        const int nb1 = min(w.nb, nr);
+        r = tg(nr, nb1, nb, A + (k + nb) + (long)k * lda, lda, A + k + (long)(k + nb) * lda, lda,
                A + (k + nb) + (long)(k + nb) * lda, lda, st);
         if (r) return r;
+        MIDAS_HIP(hipEventRecord(w.ev_next, st));
+        MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
+        r = panel<T>(A, lda, n, k + nb, nb1, ipiv, w, ps);
+        if (r) return r;
+        MIDAS_HIP(hipEventRecord(w.ev_panel, ps));
+        if (nr > nb1) {
+            r = tg(nr, nr - nb1, nb, A + (k + nb) + (long)k * lda, lda, A + k + (long)(k + nb + nb1) * lda, lda,
+                   A + (k + nb) + (long)(k + nb + nb1) * lda, lda, st);
+            if (r) return r;
+        }
     }
+    MIDAS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
     // inverses of the diagonal blocks, used by getrs
     const int nblk = (n + w.nb - 1) / w.nb;
     dim3 g(nblk, (w.nb + 63) / 64);
@@ -413,6 +436,9 @@ int LuWork<T>::alloc(int n_, int nb_)
     const size_t nblk = (size_t)(n + nb - 1) / nb;
     MIDAS_HIP(hipMalloc(&linv, sizeof(cplx<T>) * nblk * nb * nb));
     MIDAS_HIP(hipMalloc(&uinv, sizeof(cplx<T>) * nblk * nb * nb));
+    MIDAS_HIP(hipStreamCreate(&pstream));
+    MIDAS_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+    MIDAS_HIP(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming));
     return 0;
 }
 
@@ -425,6 +451,14 @@ void LuWork<T>::release()
     hipFree(pinv);
     hipFree(linv);
     hipFree(uinv);
+    if (pstream) {
+        hipStreamSynchronize(pstream);
+        hipStreamDestroy(pstream);
+    }
+    if (ev_panel) hipEventDestroy(ev_panel);
+    if (ev_next) hipEventDestroy(ev_next);
+    pstream = nullptr;
+    ev_panel = ev_next = nullptr;
     pmax = nullptr;
     pidx = counter = info = nullptr;
     pinv = nullptr;

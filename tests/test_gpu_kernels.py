@@ -21,23 +21,25 @@ def _p(a):
 
 
 def _probe_expected(which):
-    K = 4 if which < 2 else 32
-    i = np.arange(16)[:, None]
+    K = {0: 4, 1: 4, 2: 32, 3: 32, 4: 16, 5: 16}[which]
+    M = 32 if which >= 4 else 16
+    i = np.arange(M)[:, None]
     k = np.arange(K)[None, :]
     A = ((i + 2 * k) % 7 - 3).astype(float)
     kk = np.arange(K)[:, None]
-    j = np.arange(16)[None, :]
+    j = np.arange(M)[None, :]
     B = ((3 * kk + j) % 5 - 2).astype(float)
     return A @ B
 
 
-@pytest.mark.parametrize("which", [0, 1, 2, 3])
+@pytest.mark.parametrize("which", [0, 1, 2, 3, 4, 5])
 def test_mfma_lane_layout(mid, which):
     print(f"\n[case] mfma probe {which}", flush=True)
-    out = np.zeros(256)
+    M = 32 if which >= 4 else 16
+    out = np.zeros(M * M)
     r = mid.lib.midas_test_mfma_probe(mid.ctx, C.c_int(which), _p(out))
     assert r == 0
-    D = out.reshape(16, 16, order="F")
+    D = out.reshape(M, M, order="F")
     E = _probe_expected(which)
     if not np.array_equal(D, E):
         print("got\n", D[:4, :6], "\nexpected\n", E[:4, :6])
@@ -81,7 +83,7 @@ def test_gemm(mid, prec, shape, tA, tB):
 
 @pytest.mark.parametrize("mode,tol", [(1, 4e-3), (3, 1e-5), (11, 3e-2), (13, 2e-4)])
 def test_trailing_split(mid, mode, tol):
-    m, n, k = 200, 150, 96
+    m, n, k = 300, 250, 96
     print(f"\n[case] trailing split mode={mode}", flush=True)
     rng = np.random.default_rng(mode)
     cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
