@@ -37,7 +37,7 @@ EXPORTS = (
     "midas_init", "midas_finalize", "midas_last_error", "midas_stream", "midas_cell_params", "midas_mom_fill",
     "midas_block_fill", "midas_plane_wave", "midas_solve", "midas_scattering_matrix", "midas_cext_cabs",
     "midas_compute_efields_mom", "midas_aca_block", "midas_aca_matrix", "midas_compute_efields_cbfm",
-    "midas_dev_mom_fill", "midas_dev_mom_solve",
+    "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_profile", "midas_profile_query",
 )
 
 

@@ -1,6 +1,7 @@
 // C-ABI of the MIDAS gfx950 library (declared in include/midas_gpu.h).
 #include "midas_common.h"
 #include "midas_lu.h"
+#include "midas_host.h"
 #include "../../include/midas_gpu.h"
 #include "../../include/midas_gpu_test.h"
 
@@ -28,140 +29,9 @@ KernelProfile &profile()
     return p;
 }
 
-// ---------------------------------------------------------------------------
-// small device buffer owner
-// ---------------------------------------------------------------------------
-struct DBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() { release(); }
-    int alloc(size_t b)
-    {
-        release();
-        if (b == 0) b = 16;
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) {
-            p = nullptr;
-            set_error("hipMalloc(%zu bytes): %s", b, hipGetErrorString(e));
-            return -100 - (int)e;
-        }
-        bytes = b;
-        return 0;
-    }
-    void release()
-    {
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <typename X> X *as() const { return (X *)p; }
-};
-
-#define TRY(x)                  \
-    do {                        \
-        int r_ = (x);           \
-        if (r_) return r_;      \
-    } while (0)
-
-// blocking copy ordered after the work already queued on st
-static int copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t st)
-{
-    MIDAS_HIP(hipMemcpyAsync(dst, src, bytes, kind, st));
-    MIDAS_HIP(hipStreamSynchronize(st));
-    return 0;
-}
-
-static int upload(DBuf &b, const void *h, size_t bytes, hipStream_t st)
-{
-    TRY(b.alloc(bytes));
-    if (h && bytes) MIDAS_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, st));
-    return 0;
-}
-
 }  // namespace midas
 
 using namespace midas;
-
-struct midas_ctx {
-    int device = 0;
-    hipStream_t st = nullptr;
-};
-
-// Device-resident cell table with derived constants.
-struct DevCellsOwned {
-    DBuf x, y, z, sc, eps, kappa, che, znnpp, kchi, id, a;
-    DevCells v{};
-    // host cells -> device; constants computed on the GPU unless supplied
-    int from_host(const midas_cells_t *c, double k0, int quirks, hipStream_t st)
-    {
-        MIDAS_CHECK(c && c->n > 0 && c->x && c->y && c->z, -1, "cells: empty or missing coordinates");
-        const int n = c->n;
-        const size_t d = sizeof(double) * n, z2 = 2 * d;
-        TRY(upload(x, c->x, d, st));
-        TRY(upload(y, c->y, d, st));
-        TRY(upload(this->z, c->z, d, st));
-        TRY(upload(sc, c->sc, d, st));
-        std::vector<int> ids;
-        if (!c->id) {
-            ids.resize(n);
-            for (int i = 0; i < n; ++i) ids[i] = i + 1;
-        }
-        TRY(upload(id, c->id ? (const void *)c->id : (const void *)ids.data(), sizeof(int) * n, st));
-        TRY(kappa.alloc(d));
-        TRY(che.alloc(z2));
-        TRY(znnpp.alloc(z2));
-        TRY(kchi.alloc(z2));
-        TRY(a.alloc(d));
-        if (c->kappa && c->che && c->znnpp) {
-            MIDAS_HIP(hipMemcpyAsync(kappa.p, c->kappa, d, hipMemcpyHostToDevice, st));
-            MIDAS_HIP(hipMemcpyAsync(che.p, c->che, z2, hipMemcpyHostToDevice, st));
-            MIDAS_HIP(hipMemcpyAsync(znnpp.p, c->znnpp, z2, hipMemcpyHostToDevice, st));
-            std::vector<double> kc(2 * n);
-            for (int i = 0; i < n; ++i) {
-                kc[2 * i] = c->kappa[i] * c->che[2 * i];
-                kc[2 * i + 1] = c->kappa[i] * c->che[2 * i + 1];
-            }
-            MIDAS_HIP(hipMemcpyAsync(kchi.p, kc.data(), z2, hipMemcpyHostToDevice, st));
-            MIDAS_HIP(hipStreamSynchronize(st));
-        } else {
-            MIDAS_CHECK(c->sc && c->eps, -1, "cells: need sc and eps (or kappa, che, znnpp)");
-            TRY(upload(eps, c->eps, z2, st));
-            TRY(launch_cell_params(n, sc.as<double>(), eps.as<double>(), k0, quirks, a.as<double>(),
-                                   che.as<double>(), znnpp.as<double>(), kappa.as<double>(), kchi.as<double>(), st));
-        }
-        v = DevCells{n, x.as<double>(), y.as<double>(), this->z.as<double>(), sc.as<double>(), kappa.as<double>(),
-                     che.as<double>(), znnpp.as<double>(), kchi.as<double>(), id.as<int>()};
-        return 0;
-    }
-    // device pointers supplied by the caller; constants computed here
-    int from_device(const midas_cells_t *c, double k0, int quirks, hipStream_t st)
-    {
-        MIDAS_CHECK(c && c->n > 0 && c->x && c->y && c->z && c->sc && c->eps, -1, "dev cells: missing arrays");
-        const int n = c->n;
-        const size_t d = sizeof(double) * n, z2 = 2 * d;
-        TRY(kappa.alloc(d));
-        TRY(che.alloc(z2));
-        TRY(znnpp.alloc(z2));
-        TRY(kchi.alloc(z2));
-        TRY(a.alloc(d));
-        const int *ids = c->id;
-        if (!ids) {
-            std::vector<int> h(n);
-            for (int i = 0; i < n; ++i) h[i] = i + 1;
-            TRY(upload(id, h.data(), sizeof(int) * n, st));
-            MIDAS_HIP(hipStreamSynchronize(st));
-            ids = id.as<int>();
-        }
-        TRY(launch_cell_params(n, c->sc, c->eps, k0, quirks, a.as<double>(), che.as<double>(), znnpp.as<double>(),
-                               kappa.as<double>(), kchi.as<double>(), st));
-        v = DevCells{n, c->x, c->y, c->z, c->sc, kappa.as<double>(), che.as<double>(), znnpp.as<double>(),
-                     kchi.as<double>(), ids};
-        return 0;
-    }
-};
 
 // ---------------------------------------------------------------------------
 // device helpers for the solver
@@ -248,38 +118,13 @@ static int trail_mode(int mode)
     return -1;
 }
 
-struct Timer {
-    hipEvent_t a, b;
-    hipStream_t st;
-    explicit Timer(hipStream_t s) : st(s)
-    {
-        hipEventCreate(&a);
-        hipEventCreate(&b);
-        hipEventRecord(a, st);
-    }
-    double ms()
-    {
-        hipEventRecord(b, st);
-        hipEventSynchronize(b);
-        float t = 0;
-        hipEventElapsedTime(&t, a, b);
-        hipEventRecord(a, st);
-        return t;
-    }
-    ~Timer()
-    {
-        hipEventDestroy(a);
-        hipEventDestroy(b);
-    }
-};
-
 static int default_nb(int n, int mode) { return mode == MIDAS_SOLVE_FP64 ? 128 : 256; }
 
 // Solve Z X = B on the device.  Z (n x n, ldz) FP64; if W32 is non-NULL it
 // already holds the FP32 copy of Z (mixed modes).  For MIDAS_SOLVE_FP64 Z is
 // factored in place when z_inplace, else copied.  X (n x nrhs, ldx) receives
 // the solution; B (n x nrhs, ldb) is read only.
-static int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32, long ldw, const double2 *B,
+int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32, long ldw, const double2 *B,
                      long ldb, double2 *X, long ldx, int mode, const midas_solve_opts_t *opts, midas_solve_info_t *info,
                      bool z_inplace)
 {

@@ -1,0 +1,327 @@
+// Characteristic-basis-function method (CBFM-E) on gfx950.
+//
+//  Compute_EFields_CBFME      (Compute_EFields_CBFME.f90:348-449, 630-849, 896-1006)
+//      with the dense block solve (SR = 0), and
+//  Compute_EFields_CBFME_ACA  (Compute_EFields_CBFME_ACA.f90:296-385, 569-636).
+//
+// Pipeline (all on the device, one context stream):
+//   1. per block b: cells = block cells then its extension cells (s_b of them);
+//      Z_b = Green_s_tr_partial(ext block, ext block) and the CBF excitations
+//      E_b = Incident_Field(ext block, CBF directions) (FP32 intermediates when
+//      the FP32-blocks quirk is set); X_b = Z_b^-1 E_b by the FP64 LU of this
+//      library; SVD X_b = U S W^H (rocSOLVER gesvd, left vectors only);
+//      K_b = #{sigma_d / sigma_1 >= threshold}; C_b = U(1:3 n_b, 1:K_b).
+//   2. reduced matrix Zc(K x K), block (i, j) = C_i^T Z_ij C_j (plain transpose,
+//      as the reference's ZGEMM('T', ...)):
+//        dense path: one fill of the column strip Z(:, block j) (3N x 3n_j),
+//                    T = Z(:, j) C_j as one GEMM, then C_i^T T(rows of i) per i;
+//        ACA path:   diagonal blocks dense; every off-diagonal pair (i, j) by ACA
+//                    (one workgroup per pair, a block column per launch batch),
+//                    then C_i^T (U (V C_j)) when the ACA rank is below kmax, and
+//                    the dense product otherwise (the reference's fallback).
+//   3. Vc = C_i^T E_inc(block i) (FP32 Incident_Field quirk), alpha = Zc^-1 Vc
+//      (FP64 LU or mixed-precision LU + FP64 refinement, the `mode`), and
+//      E(block i) = C_i alpha_i.
+#include "midas_aca.h"
+#include "midas_host.h"
+#include "midas_lu.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace midas {
+
+#define MIDAS_ROC(call)                                                                             \
+    do {                                                                                            \
+        rocblas_status s_ = (call);                                                                 \
+        if (s_ != rocblas_status_success) {                                                         \
+            ::midas::set_error("%s:%d %s: rocblas status %d", __FILE__, __LINE__, #call, (int)s_); \
+            return -90;                                                                             \
+        }                                                                                           \
+    } while (0)
+
+struct RocHandle {
+    rocblas_handle h = nullptr;
+    ~RocHandle()
+    {
+        if (h) rocblas_destroy_handle(h);
+    }
+};
+
+// Compute_EFields_CBFME.f90:286-288 (m = sqrt(eps)) and Compute_EFields_CBFME_ACA.f90:124
+// (a single-precision literal 1e-3 widened to double).
+static double cbf_threshold_rule(const midas_cells_t *c, int use_aca)
+{
+    if (use_aca) return (double)1e-3f;
+    double mx = 0.0;
+    for (int i = 0; i < c->n; ++i) mx = std::max(mx, std::sqrt(std::hypot(c->eps[2 * i], c->eps[2 * i + 1])));
+    return std::max(std::pow(10.0, -3.0 - (std::ceil(mx) - 2.0)), (double)1e-10f);
+}
+
+// (m x n) column-major sub-block copy, device to device
+static int copy2d(const double2 *S, long lds, double2 *D, long ldd, int m, int n, hipStream_t st)
+{
+    if (m <= 0 || n <= 0) return 0;
+    MIDAS_HIP(hipMemcpy2DAsync(D, sizeof(double2) * ldd, S, sizeof(double2) * lds, sizeof(double2) * m, n,
+                               hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
+typedef cplx<double> zc;
+static const zc ONE = {1.0, 0.0}, ZERO = {0.0, 0.0};
+
+}  // namespace midas
+
+extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks,
+                                          double k0, int quirks, int ntr_cbf, const double *th_cbf,
+                                          const double *ph_cbf, int ntr, const double *tr_th, const double *tr_ph,
+                                          int use_aca, int kmax, double eps_aca, double threshold, int mode,
+                                          double *E, int *K_out, midas_solve_info_t *info)
+{
+    MIDAS_CHECK(ctx && cells && blocks && E && th_cbf && ph_cbf && tr_th && tr_ph, -1, "cbfm: bad arguments");
+    const int N = cells->n, nb = blocks->nblocks;
+    MIDAS_CHECK(N > 0 && nb > 0 && blocks->block_start && blocks->ext_start, -1, "cbfm: empty cells or blocks");
+    MIDAS_CHECK(ntr_cbf > 0 && ntr > 0, -1, "cbfm: ntr_cbf %d, ntr %d", ntr_cbf, ntr);
+    MIDAS_CHECK(mode >= 0 && mode <= 5, -1, "cbfm: unknown solver mode %d", mode);
+    if (use_aca)
+        MIDAS_CHECK(kmax >= 1 && kmax <= ACA_KMAX_SUPPORTED && eps_aca > 0, -1, "cbfm: kmax %d (1..%d), eps %g", kmax,
+                    ACA_KMAX_SUPPORTED, eps_aca);
+    const int *bs = blocks->block_start, *es = blocks->ext_start;
+    MIDAS_CHECK(bs[0] == 0 && bs[nb] == N && es[0] == 0, -1, "cbfm: block_start must run 0..n, ext_start from 0");
+    for (int b = 0; b < nb; ++b) {
+        MIDAS_CHECK(bs[b + 1] > bs[b] && es[b + 1] >= es[b], -1, "cbfm: block %d empty or tables not increasing", b);
+        for (int e = es[b]; e < es[b + 1]; ++e) {
+            const int c = blocks->ext_cells[e];
+            MIDAS_CHECK(c >= 0 && c < N && (c < bs[b] || c >= bs[b + 1]), -1,
+                        "cbfm: extension cell %d of block %d outside the cells or inside the block", c, b);
+        }
+    }
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    const bool ref32 = (quirks & MIDAS_Q_FP32_BLOCKS) != 0;
+    MIDAS_CHECK(threshold > 0 || cells->eps, -1, "cbfm: the threshold rule needs eps");
+    const double thr = threshold > 0 ? threshold : cbf_threshold_rule(cells, use_aca);
+    const long N3 = 3L * N;
+    const int T2c = 2 * ntr_cbf, T2 = 2 * ntr;
+
+    DevCellsOwned dc;
+    TRY(dc.from_host(cells, k0, quirks, st));
+    DBuf dthc, dphc, dtt, dtp;
+    TRY(upload(dthc, th_cbf, sizeof(double) * ntr_cbf, st));
+    TRY(upload(dphc, ph_cbf, sizeof(double) * ntr_cbf, st));
+    TRY(upload(dtt, tr_th, sizeof(double) * ntr, st));
+    TRY(upload(dtp, tr_ph, sizeof(double) * ntr, st));
+
+    // ---- extended cell lists of every block (block cells first)
+    std::vector<int> nbk(nb), sb(nb), ioff(nb + 1, 0);
+    std::vector<long> coff(nb + 1, 0);
+    int smax = 0, nmax = 0;
+    for (int b = 0; b < nb; ++b) {
+        nbk[b] = bs[b + 1] - bs[b];
+        sb[b] = nbk[b] + es[b + 1] - es[b];
+        ioff[b + 1] = ioff[b] + sb[b];
+        coff[b + 1] = coff[b] + 3L * nbk[b] * std::min(3 * sb[b], T2c);
+        smax = std::max(smax, sb[b]);
+        nmax = std::max(nmax, nbk[b]);
+    }
+    std::vector<int> allidx(ioff[nb]);
+    for (int b = 0; b < nb; ++b) {
+        for (int i = 0; i < nbk[b]; ++i) allidx[ioff[b] + i] = bs[b] + i;
+        for (int e = es[b]; e < es[b + 1]; ++e) allidx[ioff[b] + nbk[b] + (e - es[b])] = blocks->ext_cells[e];
+    }
+    DBuf didx;
+    TRY(upload(didx, allidx.data(), sizeof(int) * allidx.size(), st));
+
+    // ---- 1. characteristic basis functions
+    const long mmax = 3L * smax;
+    const int rmax = (int)std::min<long>(mmax, T2c);
+    DBuf Zb, Eref, Xb, Usv, Sv, Esup, dinfo, Call;
+    TRY(Zb.alloc(sizeof(double2) * mmax * mmax));
+    TRY(Eref.alloc(sizeof(double2) * mmax * T2c));
+    TRY(Xb.alloc(sizeof(double2) * mmax * T2c));
+    TRY(Usv.alloc(sizeof(double2) * mmax * rmax));
+    TRY(Sv.alloc(sizeof(double) * rmax));
+    TRY(Esup.alloc(sizeof(double) * rmax));
+    TRY(dinfo.alloc(sizeof(int)));
+    TRY(Call.alloc(sizeof(double2) * std::max<long>(coff[nb], 1)));
+    RocHandle rh;
+    MIDAS_ROC(rocblas_create_handle(&rh.h));
+    MIDAS_ROC(rocblas_set_stream(rh.h, st));
+    std::vector<int> K(nb);
+    std::vector<double> hs(rmax);
+    int worst_info = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int m = 3 * sb[b], r = std::min(m, T2c);
+        CellSet S{sb[b], 0, didx.as<int>() + ioff[b]};
+        TRY(launch_fill(dc.v, S, S, k0, quirks, ref32, Zb.as<double2>(), m, nullptr, 0, st));
+        TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32,
+                       Eref.as<double2>(), m, st));
+        midas_solve_info_t bi{};
+        int rr = solve_dev(ctx, m, T2c, Zb.as<double2>(), m, nullptr, 0, Eref.as<double2>(), m, Xb.as<double2>(), m,
+                           MIDAS_SOLVE_FP64, nullptr, &bi, true);
+        if (rr < 0) return rr;
+        if (bi.info > 0 && worst_info == 0) worst_info = bi.info;
+        MIDAS_ROC(rocsolver_zgesvd(rh.h, rocblas_svect_singular, rocblas_svect_none, m, T2c,
+                                   (rocblas_double_complex *)Xb.p, m, Sv.as<double>(),
+                                   (rocblas_double_complex *)Usv.p, m, nullptr, 1, Esup.as<double>(),
+                                   rocblas_outofplace, dinfo.as<int>()));
+        int sinfo = 0;
+        MIDAS_HIP(hipMemcpyAsync(&sinfo, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        TRY(copy_sync(hs.data(), Sv.p, sizeof(double) * r, hipMemcpyDeviceToHost, st));
+        MIDAS_CHECK(sinfo == 0, -2, "cbfm: SVD of block %d did not converge (info %d)", b, sinfo);
+        int kb = 0;
+        for (int d = 0; d < r; ++d)
+            if (hs[0] > 0 && hs[d] / hs[0] >= thr) ++kb;
+        K[b] = kb;
+        TRY(copy2d(Usv.as<double2>(), m, Call.as<double2>() + coff[b], 3L * nbk[b], 3 * nbk[b], kb, st));
+    }
+    Zb.release();
+    Eref.release();
+    Xb.release();
+    Usv.release();
+    std::vector<long> offs(nb + 1, 0);
+    int kbmax = 0;
+    for (int b = 0; b < nb; ++b) {
+        offs[b + 1] = offs[b] + K[b];
+        kbmax = std::max(kbmax, K[b]);
+    }
+    const long Kt = offs[nb];
+    MIDAS_CHECK(Kt > 0, -3, "cbfm: no basis function passed the threshold %g", thr);
+    auto Cb = [&](int b) { return Call.as<double2>() + coff[b]; };
+
+    // ---- 2. reduced matrix
+    DBuf Zc;
+    TRY(Zc.alloc(sizeof(double2) * (size_t)Kt * Kt));
+    double2 *zc_ = Zc.as<double2>();
+    const CellSet ALL{N, 0, nullptr};
+    if (!use_aca) {
+        DBuf Zs, Tt;
+        TRY(Zs.alloc(sizeof(double2) * N3 * 3 * nmax));
+        TRY(Tt.alloc(sizeof(double2) * N3 * std::max(kbmax, 1)));
+        for (int j = 0; j < nb; ++j) {
+            if (!K[j]) continue;
+            TRY(launch_fill(dc.v, ALL, CellSet{nbk[j], bs[j], nullptr}, k0, quirks, ref32, Zs.as<double2>(), N3,
+                            nullptr, 0, st));
+            TRY(launch_cgemm<double>(N3, K[j], 3 * nbk[j], ONE, Zs.p, N3, 0, Cb(j), 3L * nbk[j], 0, ZERO, Tt.p, N3,
+                                     st));
+            for (int i = 0; i < nb; ++i)
+                if (K[i])
+                    TRY(launch_cgemm<double>(K[i], K[j], 3 * nbk[i], ONE, Cb(i), 3L * nbk[i], 1,
+                                             Tt.as<double2>() + 3L * bs[i], N3, 0, ZERO,
+                                             zc_ + offs[i] + offs[j] * Kt, Kt, st));
+        }
+    } else {
+        // dense products C_i^T Z_ij C_j for the diagonal and for ACA fallbacks
+        DBuf Zij, Mi;
+        TRY(Zij.alloc(sizeof(double2) * 9L * nmax * nmax));
+        TRY(Mi.alloc(sizeof(double2) * (size_t)std::max(kbmax, 1) * 3 * nmax));
+        auto dense_pair = [&](int i, int j) -> int {
+            TRY(launch_fill(dc.v, CellSet{nbk[i], bs[i], nullptr}, CellSet{nbk[j], bs[j], nullptr}, k0, quirks, ref32,
+                            Zij.as<double2>(), 3L * nbk[i], nullptr, 0, st));
+            TRY(launch_cgemm<double>(K[i], 3 * nbk[j], 3 * nbk[i], ONE, Cb(i), 3L * nbk[i], 1, Zij.p, 3L * nbk[i], 0,
+                                     ZERO, Mi.p, K[i], st));
+            TRY(launch_cgemm<double>(K[i], K[j], 3 * nbk[j], ONE, Mi.p, K[i], 0, Cb(j), 3L * nbk[j], 0, ZERO,
+                                     zc_ + offs[i] + offs[j] * Kt, Kt, st));
+            return 0;
+        };
+        for (int j = 0; j < nb; ++j)
+            if (K[j]) TRY(dense_pair(j, j));
+        // off-diagonal pairs by ACA, a batch of block columns per launch
+        const double budget = 4.0e9;   // bytes of U and Vt per batch
+        DBuf P, Q;
+        TRY(P.alloc(sizeof(double2) * (size_t)kmax * std::max(kbmax, 1)));
+        TRY(Q.alloc(sizeof(double2) * 3L * nmax * std::max(kbmax, 1)));
+        int j0 = 0;
+        while (j0 < nb) {
+            std::vector<AcaJob> jobs;
+            std::vector<int> pi_, pj_;
+            long uo = 0, vo = 0;
+            int j1 = j0;
+            while (j1 < nb) {
+                long add_u = 0, add_v = 0;
+                std::vector<AcaJob> col;
+                for (int i = 0; i < nb; ++i) {
+                    if (i == j1 || !K[i] || !K[j1]) continue;
+                    AcaJob jb{3 * nbk[i], 3 * nbk[j1], bs[i], bs[j1], uo + add_u, vo + add_v};
+                    add_u += 3L * nbk[i] * kmax;
+                    add_v += 3L * nbk[j1] * kmax;
+                    col.push_back(jb);
+                    pi_.push_back(i);
+                    pj_.push_back(j1);
+                }
+                if (j1 > j0 && (double)(uo + add_u + vo + add_v) * sizeof(double2) > budget) {
+                    pi_.resize(jobs.size());
+                    pj_.resize(jobs.size());
+                    break;
+                }
+                jobs.insert(jobs.end(), col.begin(), col.end());
+                uo += add_u;
+                vo += add_v;
+                ++j1;
+            }
+            if (!jobs.empty()) {
+                DBuf dj, Ub, Vb, dk;
+                TRY(upload(dj, jobs.data(), sizeof(AcaJob) * jobs.size(), st));
+                TRY(Ub.alloc(sizeof(double2) * std::max(uo, 1L)));
+                TRY(Vb.alloc(sizeof(double2) * std::max(vo, 1L)));
+                TRY(dk.alloc(sizeof(int) * jobs.size()));
+                TRY(launch_aca_cells(dc.v, ref32, k0, host_pi(quirks), dj.as<AcaJob>(), (int)jobs.size(),
+                                     Ub.as<double2>(), Vb.as<double2>(), kmax, eps_aca, dk.as<int>(), st));
+                std::vector<int> hk(jobs.size());
+                TRY(copy_sync(hk.data(), dk.p, sizeof(int) * hk.size(), hipMemcpyDeviceToHost, st));
+                for (size_t q = 0; q < jobs.size(); ++q) {
+                    const int i = pi_[q], j = pj_[q];
+                    const int kk = std::min(hk[q], kmax);
+                    if (kk >= kmax) {   // not rank-deficient enough: dense block
+                        TRY(dense_pair(i, j));
+                        continue;
+                    }
+                    const double2 *U = Ub.as<double2>() + jobs[q].u_off, *Vt = Vb.as<double2>() + jobs[q].v_off;
+                    // P = V C_j (kk x K_j), Q = U P (3n_i x K_j), Zc_ij = C_i^T Q
+                    TRY(launch_cgemm<double>(kk, K[j], 3 * nbk[j], ONE, Vt, 3L * nbk[j], 1, Cb(j), 3L * nbk[j], 0,
+                                             ZERO, P.p, kmax, st));
+                    TRY(launch_cgemm<double>(3 * nbk[i], K[j], kk, ONE, U, 3L * nbk[i], 0, P.p, kmax, 0, ZERO, Q.p,
+                                             3L * nbk[i], st));
+                    TRY(launch_cgemm<double>(K[i], K[j], 3 * nbk[i], ONE, Cb(i), 3L * nbk[i], 1, Q.p, 3L * nbk[i], 0,
+                                             ZERO, zc_ + offs[i] + offs[j] * Kt, Kt, st));
+                }
+                MIDAS_HIP(hipStreamSynchronize(st));   // U, V, P, Q are reused by the next batch
+            }
+            j0 = j1;
+        }
+    }
+
+    // ---- 3. excitation, reduced solve, fields
+    DBuf Einc, Vc, Al, dE;
+    TRY(Einc.alloc(sizeof(double2) * N3 * T2));
+    TRY(Vc.alloc(sizeof(double2) * (size_t)Kt * T2));
+    TRY(Al.alloc(sizeof(double2) * (size_t)Kt * T2));
+    TRY(launch_rhs(dc.v, ALL, ntr, dtt.as<double>(), dtp.as<double>(), k0, quirks, ref32, Einc.as<double2>(), N3,
+                   st));
+    for (int i = 0; i < nb; ++i)
+        if (K[i])
+            TRY(launch_cgemm<double>(K[i], T2, 3 * nbk[i], ONE, Cb(i), 3L * nbk[i], 1, Einc.as<double2>() + 3L * bs[i],
+                                     N3, 0, ZERO, Vc.as<double2>() + offs[i], Kt, st));
+    midas_solve_info_t inf{};
+    int r = solve_dev(ctx, (int)Kt, T2, zc_, Kt, nullptr, 0, Vc.as<double2>(), Kt, Al.as<double2>(), Kt, mode, nullptr,
+                      &inf, true);
+    if (r < 0) return r;
+    Zc.release();
+    Vc.release();
+    TRY(dE.alloc(sizeof(double2) * N3 * T2));
+    MIDAS_HIP(hipMemsetAsync(dE.p, 0, dE.bytes, st));
+    for (int i = 0; i < nb; ++i)
+        if (K[i])
+            TRY(launch_cgemm<double>(3 * nbk[i], T2, K[i], ONE, Cb(i), 3L * nbk[i], 0, Al.as<double2>() + offs[i], Kt,
+                                     0, ZERO, dE.as<double2>() + 3L * bs[i], N3, st));
+    MIDAS_HIP(hipMemcpyAsync(E, dE.p, dE.bytes, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    if (K_out)
+        for (int b = 0; b < nb; ++b) K_out[b] = K[b];
+    if (info) *info = inf;
+    return inf.info > 0 ? inf.info : worst_info;
+}

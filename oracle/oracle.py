@@ -111,6 +111,22 @@ def block_fill(c1: Cells, c2: Cells, fp32=True):
     return Z
 
 
+def block_row(cells: Cells, grow, c0, ncols):
+    """computeBlockRow: row `grow` (0-based global) over cells c0 .. c0+ncols/3-1, FP32 intermediates."""
+    row = np.empty(ncols, np.complex128)
+    lib().or_block_row_f32(C.c_int(grow), C.c_int(c0), C.c_int(ncols), _p(cells.x), _p(cells.y), _p(cells.z),
+                           _p(cells.kappa), _p(cells.che), C.c_double(cells.k0), C.c_int(cells.quirk), _p(row))
+    return row
+
+
+def block_col(cells: Cells, gcol, r0, nrows):
+    """computeBlockCol: column `gcol` (0-based global) over cells r0 .. r0+nrows/3-1."""
+    col = np.empty(nrows, np.complex128)
+    lib().or_block_col_f32(C.c_int(gcol), C.c_int(r0), C.c_int(nrows), _p(cells.x), _p(cells.y), _p(cells.z),
+                           _p(cells.kappa), _p(cells.che), C.c_double(cells.k0), C.c_int(cells.quirk), _p(col))
+    return col
+
+
 def mom_rhs(cells: Cells, th_deg, ph_deg):
     th, ph = _f64(th_deg), _f64(ph_deg)
     V = np.empty((3 * cells.n, 2 * len(th)), np.complex128, order="F")
@@ -190,11 +206,12 @@ def solve(Z, B):
 
 
 def cbf_threshold(eps_cells, aca_variant):
-    """Compute_EFields_CBFME.f90:286-288 and Compute_EFields_CBFME_ACA.f90:124."""
+    """Compute_EFields_CBFME.f90:286-288 (m = sqrt(eps)) and Compute_EFields_CBFME_ACA.f90:124.
+    The literals 1e-3 and 1e-10 are single precision widened to double."""
     if aca_variant:
-        return 1e-3
+        return float(np.float32(1e-3))
     mmax = np.max(np.abs(np.sqrt(np.asarray(eps_cells))))
-    return max(10.0 ** (-3.0 - (np.ceil(mmax) - 2.0)), 1e-10)
+    return max(10.0 ** (-3.0 - (np.ceil(mmax) - 2.0)), float(np.float32(1e-10)))
 
 
 def cbfm(cells: Cells, blocks, ext, th_cbf, ph_cbf, th, ph, use_aca=False, kmax=50, eps_aca=1e-4,

@@ -1,0 +1,94 @@
+"""CPU checks of the C-ABI library and the host-side code: the built library
+loads without a GPU and exports every function the public headers declare,
+the ctypes mirror binds exactly those, the geometry generators produce the
+workloads they promise, and bench.py refuses multi-GPU runs.  No compute call
+is made (there is no GPU here)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("midas_gpu.h", "midas_gpu_test.h")]
+
+
+def declared(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(midas_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    path = os.path.join(ROOT, "midas_amd", "libmidas_gpu.so")
+    if not os.path.exists(path):
+        pytest.skip("libmidas_gpu.so not built (run __graft_entry__.build())")
+    return C.CDLL(path)
+
+
+def test_headers_declare_entry_points():
+    names = declared(HEADERS[0])
+    for must in ("midas_init", "midas_mom_fill", "midas_solve", "midas_compute_efields_mom",
+                 "midas_compute_efields_cbfm", "midas_aca_block", "midas_aca_matrix", "midas_block_fill",
+                 "midas_plane_wave", "midas_scattering_matrix", "midas_cext_cabs"):
+        assert must in names
+
+
+@pytest.mark.parametrize("header", HEADERS, ids=os.path.basename)
+def test_library_exports_every_declared_symbol(lib, header):
+    missing = [n for n in declared(header) if not hasattr(lib, n)]
+    assert not missing, f"declared in {os.path.basename(header)} but not exported: {missing}"
+
+
+def test_library_exports_are_unmangled_c(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(ROOT, "midas_amd", "libmidas_gpu.so")],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    pub = set(declared(HEADERS[0])) | set(declared(HEADERS[1]))
+    assert pub <= exported
+
+
+def test_api_mirror_binds_declared_names():
+    from midas_amd import api
+    assert set(api.EXPORTS) <= set(declared(HEADERS[0])) | set(declared(HEADERS[1]))
+    assert set(declared(HEADERS[0])) <= set(api.EXPORTS)
+
+
+def test_no_oracle_in_product():
+    """The product never imports or links the oracle."""
+    pkg = os.path.join(ROOT, "midas_amd")
+    for dirpath, _, files in os.walk(pkg):
+        if "_build" in dirpath:
+            continue
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".cpp", ".f90")):
+                src = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in re.sub(r"#.*|//.*|!.*", "", src).lower(), f
+    out = subprocess.run(["ldd", os.path.join(ROOT, "midas_amd", "libmidas_gpu.so")], capture_output=True,
+                         text=True).stdout
+    assert "oracle" not in out and "midas_ref" not in out
+
+
+def test_geometry_workloads():
+    from midas_amd import geometry as g
+    s = g.sphere(size_param=2.0, nd=16)
+    assert s.n == 2176                                 # the reference lattice rule at Nd = 16
+    assert g.sphere(size_param=2.0, nd=15).n == 1791
+    d = g.dlca_aggregate(nmono=8, cells_per_mono=40, seed=3)
+    assert len(d.blocks) == 8
+    assert sorted(np.concatenate(d.blocks).tolist()) == list(range(d.n))
+    for b in d.blocks:                                 # blocks are contiguous cell ranges
+        assert np.array_equal(b, np.arange(b[0], b[0] + len(b)))
+    for b, e in zip(d.blocks, d.ext):                  # extension cells lie outside their block
+        assert not set(e.tolist()) & set(b.tolist())
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stdout

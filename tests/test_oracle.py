@@ -1,0 +1,277 @@
+"""Pins the oracle (CPU restatement, oracle/oracle.c) before it is trusted as the
+checker of the HIP library:
+
+  * against the reference's own routines -- golden vectors in
+    tests/golden/ref_vectors.npz written by tests/golden/make_golden.py from
+    oracle/_ref, and live against oracle/_ref when that is built here;
+  * against what the mathematics fixes: closed forms, the static limit,
+    complex symmetry, transversality, a hand-computed two-cell case, LAPACK
+    identities, exact-rank recovery of ACA, and CBFM == MoM without truncation;
+  * and a sensitivity check: the golden comparison fails for an oracle run
+    with the reference's single-precision pi switched off (quirk Q1).
+"""
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD, allow_pickle=False)
+
+
+def _cells(o, g, p, quirk=1):
+    return o.Cells(g[p + "_x"], g[p + "_y"], g[p + "_z"], g[p + "_sc"], g[p + "_eps"], float(g[p + "_k0"]),
+                   quirk=quirk)
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+# ----------------------------------------------------------------------------
+# golden vectors of the reference routines
+# ----------------------------------------------------------------------------
+
+def test_cell_params_golden(oracle, gold):
+    c = oracle.Cells(np.zeros(16), np.zeros(16), np.zeros(16), gold["cp_sc"], gold["cp_eps"], float(gold["cp_k0"]))
+    assert np.array_equal(c.a, gold["cp_a"])
+    assert np.array_equal(c.che, gold["cp_che"])
+    assert np.max(np.abs(c.znnpp - gold["cp_zn"]) / np.abs(gold["cp_zn"]).max()) < 1e-15
+    assert np.max(np.abs(c.kappa - gold["cp_kap"]) / gold["cp_kap"]) < 1e-13
+
+
+def test_block_fill_f32_golden(oracle, gold):
+    """Green_s_tr_partial with its single-precision intermediates (Q3): bit-exact."""
+    c = _cells(oracle, gold, "g1")
+    r, cc = gold["g1_part_rows"], gold["g1_part_cols"]
+    assert np.array_equal(oracle.block_fill(c.subset(r), c.subset(cc), fp32=True), gold["g1_partial"])
+    assert np.array_equal(oracle.block_fill(c.subset(cc), c.subset(cc), fp32=True), gold["g1_partial_diag"])
+
+
+def test_total_fill_golden(oracle, gold):
+    """The whole matrix through the same element routine equals Green_s_tr_total."""
+    c = _cells(oracle, gold, "g1")
+    assert np.array_equal(oracle.block_fill(c, c, fp32=True), gold["g1_total"])
+    # the FP64 MoM element (Compute_EFields_MoM.f90:433-523) agrees to single precision
+    Z = oracle.mom_fill(c)
+    assert np.abs(Z - gold["g1_total"]).max() / np.abs(Z).max() < 3e-6
+
+
+def test_block_row_col_golden(oracle, gold):
+    """computeBlockRow / computeBlockCol.  Arguments as the reference takes them:
+    1-based global row (column) and the 0-based offset of the first column (row)
+    whose cell is offset/3 + 1.  The [1, 1, 90] case crosses the self term, where
+    the reference divides by r = 0 (NaN); ACA never asks for it."""
+    c = _cells(oracle, gold, "g1")
+    rows = np.concatenate([oracle.block_row(c, g - 1, off // 3, nc) for g, off, nc in gold["g1_row_args"]])
+    cols = np.concatenate([oracle.block_col(c, g - 1, off // 3, nr) for off, nr, g in gold["g1_col_args"]])
+    assert np.array_equal(rows, gold["g1_rows"], equal_nan=True)
+    assert np.array_equal(cols, gold["g1_cols"], equal_nan=True)
+    # away from the self term they are rows and columns of Green_s_tr_partial
+    Z = oracle.block_fill(c, c, fp32=True)
+    assert np.array_equal(oracle.block_row(c, 57, 1, 30), Z[57, 3:33])
+    assert np.array_equal(oracle.block_col(c, 6, 10, 45), Z[30:75, 6])
+
+
+def test_incident_field_golden(oracle, gold):
+    c = _cells(oracle, gold, "g1")
+    E = oracle.incident_field_f32(c, gold["g1_th"], gold["g1_ph"])
+    assert np.array_equal(E, gold["g1_einc"])
+    # the FP64 plane wave agrees to single precision
+    V = oracle.mom_rhs(c, gold["g1_th"], gold["g1_ph"])
+    assert np.abs(V - E).max() < 1e-6
+
+
+def test_ff_coeff_golden(oracle, gold):
+    c = _cells(oracle, gold, "g1")
+    for i, (t, p) in enumerate(zip(gold["g1_th"], gold["g1_ph"])):
+        ff = oracle.ff_coeff(c, float(t), float(p))
+        assert _rel(ff, gold["g1_ff"][i]) < 1e-14
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_aca_golden(oracle, gold, case):
+    """Calcul_MatrixZij_ACA: same rank, pivots and factors (FP32 rows/columns, Q3/Q8).
+    The factors agree to 1e-10, not bit for bit: the FP64 norms and the complex
+    division by the pivot round differently in C and in the Fortran runtime."""
+    c = _cells(oracle, gold, "g2")
+    r0, nr, c0, nc, km = map(int, gold["g2_aca_args"][case])
+    eps = float(gold["g2_aca_eps"][case])
+    k, U, V, _, _ = oracle.aca(c, r0, nr, c0, nc, kmax=km, eps=eps, fp32=True)
+    assert k == int(gold[f"g2_aca{case}_k"])
+    assert _rel(U, gold[f"g2_aca{case}_U"]) < 1e-10
+    assert _rel(V, gold[f"g2_aca{case}_V"]) < 1e-10
+
+
+def test_aca_gen_golden(oracle, gold):
+    k, U, V = oracle.aca_gen(gold["gen_A"], kmax=20, eps=1e-4)
+    assert k == int(gold["gen_k"])
+    # the 7th term resolves the 1e-7 noise, which amplifies rounding: compare the
+    # product tightly and the factors loosely
+    assert _rel(U @ V, gold["gen_U"] @ gold["gen_V"]) < 1e-13
+    assert _rel(U, gold["gen_U"]) < 1e-7 and _rel(V, gold["gen_V"]) < 1e-7
+    assert _rel(U[:, :6], gold["gen_U"][:, :6]) < 1e-12
+
+
+def test_golden_sensitive_to_pi_quirk(oracle, gold):
+    """An oracle with the double-precision pi (quirk Q1 off) must fail the golden check."""
+    c = _cells(oracle, gold, "g1", quirk=0)
+    r, cc = gold["g1_part_rows"], gold["g1_part_cols"]
+    assert not np.array_equal(oracle.block_fill(c.subset(r), c.subset(cc), fp32=True), gold["g1_partial"])
+    c0 = oracle.Cells(np.zeros(16), np.zeros(16), np.zeros(16), gold["cp_sc"], gold["cp_eps"],
+                      float(gold["cp_k0"]), quirk=0)
+    assert not np.array_equal(c0.a, gold["cp_a"])
+
+
+# ----------------------------------------------------------------------------
+# live comparison with oracle/_ref (skipped when the reference tree is absent)
+# ----------------------------------------------------------------------------
+
+def test_live_ref_dlca(oracle, ref_lib):
+    from midas_amd import geometry as g
+    s = g.dlca_aggregate(nmono=3, cells_per_mono=50, seed=11)
+    c = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    a, che, zn, kap = oracle.ref_cell_params(s.sc, s.eps, s.k0)
+    assert np.array_equal(a, c.a) and np.array_equal(che, c.che)
+    b0, b1 = s.blocks[0], s.blocks[1]
+    assert np.array_equal(oracle.block_fill(c.subset(b0), c.subset(b1)), oracle.ref_block_fill(c.subset(b0),
+                                                                                            c.subset(b1)))
+    th, ph = np.array([0.0, 71.0]), np.array([15.0, 250.0])
+    assert np.array_equal(oracle.incident_field_f32(c, th, ph), oracle.ref_incident_field(c, th, ph))
+    k, U, V, _, _ = oracle.aca(c, int(b0[0]), len(b0), int(b1[0]), len(b1))
+    kr, Ur, Vr = oracle.ref_aca(c, int(b0[0]), len(b0), int(b1[0]), len(b1))
+    assert k == kr and _rel(U, Ur) < 1e-10 and _rel(V, Vr) < 1e-10
+
+
+# ----------------------------------------------------------------------------
+# mathematics
+# ----------------------------------------------------------------------------
+
+def test_cell_params_static_limit(oracle):
+    """k a -> 0: Znnpp -> -chi/3 and kappa -> (4 pi / 3) a^3 = Sc^3 (exact pi)."""
+    sc = np.array([1e-9, 2e-9])
+    eps = np.array([2.0 + 0.5j, 4.0 + 0j])
+    c = oracle.Cells(np.zeros(2), np.zeros(2), np.zeros(2), sc, eps, 1.0, quirk=0)
+    assert np.allclose(c.znnpp, -(eps - 1) / 3, rtol=1e-12)
+    assert np.allclose(c.kappa, sc ** 3, rtol=1e-9)
+    assert np.allclose(c.a, sc * (3 / (4 * np.pi)) ** (1 / 3), rtol=1e-15)
+
+
+def _two_cell(k0, d, sc, eps):
+    """Z for two cells on the x axis, evaluated by hand from the element formula."""
+    pi = np.pi
+    a = sc * (3 / (4 * pi)) ** (1 / 3)
+    kap = 4 * pi * (np.sin(k0 * a) - k0 * a * np.cos(k0 * a)) / k0 ** 3
+    chi = eps - 1
+    zn = (2 / 3 * np.exp(1j * k0 * a) * (1 - 1j * k0 * a) - 1) * chi
+    r = d
+    gr = np.exp(1j * k0 * r) / (4 * pi * r ** 2)
+    tau = 1j * k0 - 1 / r
+    A = gr * (tau + k0 ** 2 * r)
+    Cc = gr / r * (-k0 ** 2 - 3 * tau / r)
+    F = A * np.eye(3) + Cc * np.diag([r * r, 0, 0])   # r along x
+    Z = np.zeros((6, 6), complex)
+    Z[:3, :3] = Z[3:, 3:] = (1 - zn) * np.eye(3)
+    Z[:3, 3:] = Z[3:, :3] = -F * kap * chi
+    return Z
+
+
+def test_mom_fill_two_cells_by_hand(oracle):
+    k0, d, sc, eps = 2 * np.pi / 1e-3, 1.7e-4, 1e-4, 2.2 + 0.3j
+    c = oracle.Cells(np.array([0.0, d]), np.zeros(2), np.zeros(2), np.full(2, sc), np.full(2, eps), k0, quirk=0)
+    Z = oracle.mom_fill(c)
+    Zh = _two_cell(k0, d, sc, eps)
+    assert np.abs(Z - Zh).max() < 1e-12 * np.abs(Zh).max()
+    # a dropped C r r^T term or a sign flip would show here:
+    assert abs(Z[0, 3]) != pytest.approx(abs(Z[1, 4]), rel=1e-3)
+
+
+def test_mom_fill_complex_symmetric(oracle):
+    from midas_amd import geometry as g
+    s = g.sphere(size_param=1.5, nd=6)
+    c = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    Z = oracle.mom_fill(c)
+    assert np.abs(Z - Z.T).max() < 1e-13 * np.abs(Z).max()
+    for p in range(3):
+        for q in range(3):
+            if p != q:
+                assert np.all(Z[p::3, q::3].diagonal() == 0)
+
+
+def test_plane_wave_unit_transverse(oracle):
+    from midas_amd import geometry as g
+    s = g.sphere(size_param=1.0, nd=4)
+    c = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    th, ph = np.array([0.0, 30.0, 95.0, 180.0]), np.array([0.0, 45.0, 300.0, 10.0])
+    V = oracle.mom_rhs(c, th, ph)
+    pq = oracle.pi(1)                                   # the single-precision pi of the reference
+    t, f = th * pq / 180.0, ph * pq / 180.0
+    # theta from +x: propagation direction (cos t, sin t cos f, sin t sin f)
+    kh = np.stack([np.cos(t), np.sin(t) * np.cos(f), np.sin(t) * np.sin(f)])
+    for j in range(2 * len(th)):
+        e = V[:, j].reshape(-1, 3)
+        assert np.allclose(np.linalg.norm(e, axis=1), 1.0, atol=1e-14)
+        assert np.abs(e @ kh[:, j % len(th)]).max() < 1e-14
+    for j in range(len(th)):       # V and H are orthogonal
+        assert abs(np.vdot(V[:3, j], V[:3, j + len(th)])) < 1e-14
+
+
+def test_solve_backward_error(oracle):
+    from midas_amd import geometry as g
+    s = g.sphere(size_param=2.0, nd=6)
+    c = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    Z = oracle.mom_fill(c)
+    B = oracle.mom_rhs(c, [0.0], [0.0])
+    X = oracle.solve(Z, B)
+    assert np.linalg.norm(Z @ X - B) / (np.linalg.norm(Z) * np.linalg.norm(X)) < 1e-14
+
+
+def test_aca_recovers_exact_rank(oracle):
+    rng = np.random.default_rng(5)
+    for r in (1, 3, 8):
+        A = (rng.normal(size=(40, r)) + 1j * rng.normal(size=(40, r))) @ (
+            rng.normal(size=(r, 33)) + 1j * rng.normal(size=(r, 33)))
+        k, U, V = oracle.aca_gen(A, kmax=30, eps=1e-10)
+        assert np.linalg.norm(A - U @ V) < 1e-9 * np.linalg.norm(A)
+        assert r <= k <= r + 1
+
+
+def test_aca_block_accuracy(oracle):
+    from midas_amd import geometry as g
+    s = g.dlca_aggregate(nmono=4, cells_per_mono=60, seed=3)
+    c = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    b0, b3 = s.blocks[0], s.blocks[3]
+    Zb = oracle.block_fill(c.subset(b0), c.subset(b3), fp32=False)
+    k, U, V, _, _ = oracle.aca(c, int(b0[0]), len(b0), int(b3[0]), len(b3), kmax=50, eps=1e-4, fp32=False)
+    assert k < 50
+    assert np.linalg.norm(Zb - U @ V) < 1e-3 * np.linalg.norm(Zb)
+
+
+def test_cbfm_without_truncation_is_mom(oracle):
+    """Threshold 0, no extension and 2 T_c >= 3 s_i CBFs per block: CBFM spans the
+    whole space, so it reproduces the MoM solution (Compute_EFields_CBFME.f90)."""
+    from midas_amd import geometry as g
+    s = g.dlca_aggregate(nmono=3, cells_per_mono=12, seed=4)
+    c = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    nmax = max(len(b) for b in s.blocks)
+    dirs = g.directions(3 * nmax)                       # 2 T_c = 6 nmax >= 3 nmax
+    th, ph = np.array([0.0, 50.0]), np.array([0.0, 120.0])
+    E, Ks, _ = oracle.cbfm(c, s.blocks, [np.array([], int)] * len(s.blocks), dirs[0], dirs[1], th, ph,
+                           threshold=1e-300, fp32_blocks=False)
+    assert Ks == [3 * len(b) for b in s.blocks]
+    Eo = np.linalg.solve(oracle.mom_fill(c), oracle.mom_rhs(c, th, ph))
+    Eo32 = np.linalg.solve(oracle.mom_fill(c), oracle.incident_field_f32(c, th, ph))
+    assert _rel(E, Eo32) < 1e-10
+    assert _rel(E, Eo) < 1e-6
+
+
+def test_cbf_threshold_rule(oracle):
+    assert oracle.cbf_threshold(np.array([2.0 + 0j]), True) == float(np.float32(1e-3))
+    # |sqrt(eps)| max 1.41 -> ceil 2 -> 1e-3 ; max 3.2 -> ceil 4 -> 1e-5
+    assert oracle.cbf_threshold(np.array([2.0 + 0j]), False) == pytest.approx(1e-3)
+    assert oracle.cbf_threshold(np.array([2.0 + 0j, 10.24 + 0j]), False) == pytest.approx(1e-5)
+    assert oracle.cbf_threshold(np.array([1e12 + 0j]), False) == float(np.float32(1e-10))

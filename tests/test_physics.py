@@ -1,0 +1,82 @@
+"""Physics pins of the oracle's far-field and cross-section routines (a
+discretised sphere against the Mie series, energy balance, optical theorem).
+These hold to the discretisation error (~1e-2), as in the reference's own
+validation against Mie theory; they catch a wrong sign, a missing factor, a
+swapped polarisation column or a wrong angle convention, which the golden
+vectors of individual routines would not."""
+import numpy as np
+import pytest
+from scipy.integrate import trapezoid
+from scipy.special import spherical_jn, spherical_yn
+
+
+def mie_q(x, m):
+    """Qext, Qsca of a homogeneous sphere (Bohren & Huffman, 'Absorption and
+    Scattering of Light by Small Particles', 1983, BHMIE: downward recurrence
+    for the logarithmic derivative D_n(mx))."""
+    nstop = int(x + 4 * x ** (1 / 3) + 2)
+    n = np.arange(1, nstop + 1)
+    mx = m * x
+    nmx = int(max(nstop, abs(mx))) + 16
+    D = np.zeros(nmx + 1, complex)
+    for k in range(nmx, 0, -1):
+        D[k - 1] = k / mx - 1 / (D[k] + k / mx)
+    D = D[1:nstop + 1]
+    psi, psim1 = x * spherical_jn(n, x), x * spherical_jn(n - 1, x)
+    xi = psi + 1j * x * spherical_yn(n, x)
+    xim1 = psim1 + 1j * x * spherical_yn(n - 1, x)
+    a = ((D / m + n / x) * psi - psim1) / ((D / m + n / x) * xi - xim1)
+    b = ((D * m + n / x) * psi - psim1) / ((D * m + n / x) * xi - xim1)
+    qext = 2 / x ** 2 * np.sum((2 * n + 1) * (a + b).real)
+    qsca = 2 / x ** 2 * np.sum((2 * n + 1) * (abs(a) ** 2 + abs(b) ** 2))
+    return qext, qsca
+
+
+def test_mie_series_small_particle_limit():
+    """Rayleigh limit of the Mie routine itself: Qsca -> 8/3 x^4 |(m^2-1)/(m^2+2)|^2."""
+    x, m = 0.01, 1.5 + 0.0j
+    _, qs = mie_q(x, m)
+    assert qs == pytest.approx(8 / 3 * x ** 4 * abs((m * m - 1) / (m * m + 2)) ** 2, rel=1e-3)
+
+
+@pytest.fixture(scope="module")
+def sphere_solution(oracle):
+    from midas_amd import geometry as g
+    s = g.sphere(size_param=2.0, nd=12)
+    c = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    th, ph = np.array([0.0]), np.array([0.0])
+    E = oracle.solve(oracle.mom_fill(c), oracle.mom_rhs(c, th, ph))
+    return s, c, E, th, ph
+
+
+def test_cext_cabs_vs_mie(oracle, sphere_solution):
+    s, c, E, th, ph = sphere_solution
+    ce, ca = oracle.cext_cabs(c, E, th, ph, q6=0)
+    a_eq = (3 * s.n * s.sc[0] ** 3 / (4 * np.pi)) ** (1 / 3)        # equal-volume sphere
+    qe, qs = mie_q(s.k0 * a_eq, np.sqrt(s.eps[0]))
+    area = np.pi * a_eq ** 2
+    assert ce[0].real == pytest.approx(qe * area, rel=2e-2)
+    assert ca[0].real == pytest.approx((qe - qs) * area, rel=3e-2)
+    # Q6: for a single plane wave the component-sum formula equals the dot product here
+    ce6, ca6 = oracle.cext_cabs(c, E, th, ph, q6=1)
+    assert ce6[0] == pytest.approx(ce[0], rel=1e-12)
+
+
+def test_energy_balance_and_optical_theorem(oracle, sphere_solution):
+    """(1/k^2) * integral of |S_Vv|^2 + |S_Vh|^2 over the sphere = Cext - Cabs,
+    and (4 pi / k^2) Im S_Vv(forward) = Cext."""
+    s, c, E, th, ph = sphere_solution
+    ce, ca = oracle.cext_cabs(c, E, th, ph, q6=0)
+    nt, nph = 61, 36
+    tt = np.linspace(0.0, 180.0, nt)
+    T, P = np.meshgrid(tt, np.arange(nph) * 360.0 / nph, indexing="ij")
+    S = oracle.scattering_matrix(c, E, T.ravel(), P.ravel())
+    for cols in ((0, 1), (2, 3)):     # V incidence: Vv, Vh; H incidence: Hv, Hh
+        I = (np.abs(S[:, cols[0]]) ** 2 + np.abs(S[:, cols[1]]) ** 2).reshape(nt, nph).mean(1) * 2 * np.pi
+        csca = trapezoid(I * np.sin(np.radians(tt)), np.radians(tt)) / s.k0 ** 2
+        assert csca == pytest.approx((ce[0] - ca[0]).real, rel=1.5e-2)
+    S0 = oracle.scattering_matrix(c, E, [0.0], [0.0])
+    assert 4 * np.pi / s.k0 ** 2 * S0[0, 0].imag == pytest.approx(ce[0].real, rel=2e-2)
+    assert 4 * np.pi / s.k0 ** 2 * S0[0, 3].imag == pytest.approx(ce[0].real, rel=2e-2)
+    # no cross-polarised forward scattering from a sphere
+    assert abs(S0[0, 1]) < 1e-6 * abs(S0[0, 0])
