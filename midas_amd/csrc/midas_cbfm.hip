@@ -9,7 +9,13 @@
 //      Z_b = Green_s_tr_partial(ext block, ext block) and the CBF excitations
 //      E_b = Incident_Field(ext block, CBF directions) (FP32 intermediates when
 //      the FP32-blocks quirk is set); X_b = Z_b^-1 E_b by the FP64 LU of this
-//      library; SVD X_b = U S W^H (rocSOLVER gesvd, left vectors only);
+//      library.  The reference takes the SVD X_b = U S W^H (GESVD, JOB='U');
+//      only sigma and the span of the leading left vectors enter the result,
+//      so here they come from the Gram matrix X^H X (or X X^H when X is wide):
+//      one GEMM per block and ONE batched Hermitian eigensolver call
+//      (rocSOLVER heevd, strided batched) for all blocks, U = X W S^-1.  For
+//      thresholds below 1e-5 the squared spectrum is not resolved well enough
+//      and each block takes rocSOLVER gesvd instead.
 //      K_b = #{sigma_d / sigma_1 >= threshold}; C_b = U(1:3 n_b, 1:K_b).
 //   2. reduced matrix Zc(K x K), block (i, j) = C_i^T Z_ij C_j (plain transpose,
 //      as the reference's ZGEMM('T', ...)):
@@ -31,6 +37,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
 #include <vector>
 
 namespace midas {
@@ -74,6 +83,34 @@ static int copy2d(const double2 *S, long lds, double2 *D, long ldd, int m, int n
 typedef cplx<double> zc;
 static const zc ONE = {1.0, 0.0}, ZERO = {0.0, 0.0};
 
+// The Gram route squares the singular values: FP64 resolves sigma_d / sigma_1
+// to a few digits down to ~1e-6 (eigenvalue ratio 1e-12 against an absolute
+// eigenvalue error of ~n eps lambda_1); smaller thresholds take the SVD.
+constexpr double CBF_GRAM_MIN_THRESHOLD = 1e-5;
+
+// out(:, c) = W(0:rows, rpad-1-c) * (scale ? 1/sqrt(lambda[rpad-1-c]) : 1),
+// c < K: the eigenvectors of the K largest eigenvalues (ascending storage),
+// largest first.
+__global__ void k_pick_eigvecs(const double2 *W, long ldw, const double *lam, int rpad, int rows, int K, int scale,
+                               double2 *out, long ldo)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r >= rows || c >= K) return;
+    const int src = rpad - 1 - c;
+    const double s = scale ? 1.0 / sqrt(lam[src]) : 1.0;
+    double2 v = W[(long)src * ldw + r];
+    out[(long)c * ldo + r] = make_double2(v.x * s, v.y * s);
+}
+static int launch_pick_eigvecs(const double2 *W, long ldw, const double *lam, int rpad, int rows, int K, bool scale,
+                               double2 *out, long ldo, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pick_eigvecs, dim3((rows + 255) / 256, K), dim3(256), 0, st, W, ldw, lam, rpad, rows, K,
+                       (int)scale, out, ldo);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
 }  // namespace midas
 
 extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks,
@@ -108,6 +145,12 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
     const long N3 = 3L * N;
     const int T2c = 2 * ntr_cbf, T2 = 2 * ntr;
 
+    // MIDAS_VERBOSE=1 prints the time of each phase (synchronising between them)
+    const bool verbose = std::getenv("MIDAS_VERBOSE") && std::atoi(std::getenv("MIDAS_VERBOSE")) > 0;
+    std::unique_ptr<Timer> tm(verbose ? new Timer(st) : nullptr);
+    auto phase = [&](const char *name) {
+        if (tm) std::fprintf(stderr, "[cbfm] %-28s %10.2f ms\n", name, tm->ms());
+    };
     DevCellsOwned dc;
     TRY(dc.from_host(cells, k0, quirks, st));
     DBuf dthc, dphc, dtt, dtp;
@@ -137,52 +180,118 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
     TRY(upload(didx, allidx.data(), sizeof(int) * allidx.size(), st));
 
     // ---- 1. characteristic basis functions
+    // 1a. block solutions X_b = Z_b^-1 E_b (FP64 LU), kept side by side (ld 3 s_b)
     const long mmax = 3L * smax;
-    const int rmax = (int)std::min<long>(mmax, T2c);
-    DBuf Zb, Eref, Xb, Usv, Sv, Esup, dinfo, Call;
-    TRY(Zb.alloc(sizeof(double2) * mmax * mmax));
-    TRY(Eref.alloc(sizeof(double2) * mmax * T2c));
-    TRY(Xb.alloc(sizeof(double2) * mmax * T2c));
-    TRY(Usv.alloc(sizeof(double2) * mmax * rmax));
-    TRY(Sv.alloc(sizeof(double) * rmax));
-    TRY(Esup.alloc(sizeof(double) * rmax));
-    TRY(dinfo.alloc(sizeof(int)));
+    std::vector<long> xoff(nb + 1, 0);
+    int rpad = 1;
+    for (int b = 0; b < nb; ++b) {
+        xoff[b + 1] = xoff[b] + 3L * sb[b] * T2c;
+        rpad = std::max(rpad, std::min(3 * sb[b], T2c));
+    }
+    DBuf Call, Xall;
     TRY(Call.alloc(sizeof(double2) * std::max<long>(coff[nb], 1)));
+    TRY(Xall.alloc(sizeof(double2) * xoff[nb]));
+    auto Xb = [&](int b) { return Xall.as<double2>() + xoff[b]; };
+    int worst_info = 0;
+    {
+        DBuf Zb, Eref;
+        TRY(Zb.alloc(sizeof(double2) * mmax * mmax));
+        TRY(Eref.alloc(sizeof(double2) * mmax * T2c));
+        for (int b = 0; b < nb; ++b) {
+            const int m = 3 * sb[b];
+            CellSet S{sb[b], 0, didx.as<int>() + ioff[b]};
+            TRY(launch_fill(dc.v, S, S, k0, quirks, ref32, Zb.as<double2>(), m, nullptr, 0, st));
+            TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32,
+                           Eref.as<double2>(), m, st));
+            midas_solve_info_t bi{};
+            int rr = solve_dev(ctx, m, T2c, Zb.as<double2>(), m, nullptr, 0, Eref.as<double2>(), m, Xb(b), m,
+                               MIDAS_SOLVE_FP64, nullptr, &bi, true);
+            if (rr < 0) return rr;
+            if (bi.info > 0 && worst_info == 0) worst_info = bi.info;
+        }
+    }
+    phase("block fills + LU solves");
+    // 1b. left singular vectors of X_b with sigma_d / sigma_1 >= threshold.
+    std::vector<int> K(nb);
     RocHandle rh;
     MIDAS_ROC(rocblas_create_handle(&rh.h));
     MIDAS_ROC(rocblas_set_stream(rh.h, st));
-    std::vector<int> K(nb);
-    std::vector<double> hs(rmax);
-    int worst_info = 0;
-    for (int b = 0; b < nb; ++b) {
-        const int m = 3 * sb[b], r = std::min(m, T2c);
-        CellSet S{sb[b], 0, didx.as<int>() + ioff[b]};
-        TRY(launch_fill(dc.v, S, S, k0, quirks, ref32, Zb.as<double2>(), m, nullptr, 0, st));
-        TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32,
-                       Eref.as<double2>(), m, st));
-        midas_solve_info_t bi{};
-        int rr = solve_dev(ctx, m, T2c, Zb.as<double2>(), m, nullptr, 0, Eref.as<double2>(), m, Xb.as<double2>(), m,
-                           MIDAS_SOLVE_FP64, nullptr, &bi, true);
-        if (rr < 0) return rr;
-        if (bi.info > 0 && worst_info == 0) worst_info = bi.info;
-        MIDAS_ROC(rocsolver_zgesvd(rh.h, rocblas_svect_singular, rocblas_svect_none, m, T2c,
-                                   (rocblas_double_complex *)Xb.p, m, Sv.as<double>(),
-                                   (rocblas_double_complex *)Usv.p, m, nullptr, 1, Esup.as<double>(),
-                                   rocblas_outofplace, dinfo.as<int>()));
-        int sinfo = 0;
-        MIDAS_HIP(hipMemcpyAsync(&sinfo, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        TRY(copy_sync(hs.data(), Sv.p, sizeof(double) * r, hipMemcpyDeviceToHost, st));
-        MIDAS_CHECK(sinfo == 0, -2, "cbfm: SVD of block %d did not converge (info %d)", b, sinfo);
-        int kb = 0;
-        for (int d = 0; d < r; ++d)
-            if (hs[0] > 0 && hs[d] / hs[0] >= thr) ++kb;
-        K[b] = kb;
-        TRY(copy2d(Usv.as<double2>(), m, Call.as<double2>() + coff[b], 3L * nbk[b], 3 * nbk[b], kb, st));
+    if (thr >= CBF_GRAM_MIN_THRESHOLD) {
+        // Gram route: G_b = X^H X (T2c <= m_b, then U = X W S^-1) or X X^H
+        // (T2c > m_b, then U = W); every block's G zero-padded to rpad and
+        // decomposed by one batched Hermitian eigensolver call.
+        const long gs = (long)rpad * rpad;
+        DBuf G, D, Eo, dinfo, Wk;
+        TRY(G.alloc(sizeof(double2) * gs * nb));
+        TRY(D.alloc(sizeof(double) * (size_t)rpad * nb));
+        TRY(Eo.alloc(sizeof(double) * (size_t)rpad * nb));
+        TRY(dinfo.alloc(sizeof(int) * nb));
+        TRY(Wk.alloc(sizeof(double2) * (size_t)T2c * T2c));
+        MIDAS_HIP(hipMemsetAsync(G.p, 0, G.bytes, st));
+        for (int b = 0; b < nb; ++b) {
+            const int m = 3 * sb[b];
+            double2 *Gb = G.as<double2>() + b * gs;
+            if (T2c <= m)
+                TRY(launch_cgemm<double>(T2c, T2c, m, ONE, Xb(b), m, 2, Xb(b), m, 0, ZERO, Gb, rpad, st));
+            else
+                TRY(launch_cgemm<double>(m, m, T2c, ONE, Xb(b), m, 0, Xb(b), m, 2, ZERO, Gb, rpad, st));
+        }
+        MIDAS_ROC(rocsolver_zheevd_strided_batched(rh.h, rocblas_evect_original, rocblas_fill_upper, rpad,
+                                                   (rocblas_double_complex *)G.p, rpad, gs, D.as<double>(), rpad,
+                                                   Eo.as<double>(), rpad, dinfo.as<int>(), nb));
+        std::vector<double> hd((size_t)rpad * nb);
+        std::vector<int> hi(nb);
+        MIDAS_HIP(hipMemcpyAsync(hi.data(), dinfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+        TRY(copy_sync(hd.data(), D.p, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, st));
+        for (int b = 0; b < nb; ++b) {
+            MIDAS_CHECK(hi[b] == 0, -2, "cbfm: eigensolver failed on block %d (info %d)", b, hi[b]);
+            const int m = 3 * sb[b], r = std::min(m, T2c);
+            const double *lam = hd.data() + (size_t)b * rpad;   // ascending
+            const double s1 = std::sqrt(std::max(lam[rpad - 1], 0.0));
+            int kb = 0;
+            for (int d = 0; d < r; ++d)
+                if (s1 > 0 && std::sqrt(std::max(lam[rpad - 1 - d], 0.0)) / s1 >= thr) ++kb;
+            K[b] = kb;
+            if (!kb) continue;
+            const double2 *Wb = G.as<double2>() + b * gs;
+            if (T2c <= m) {
+                TRY(launch_pick_eigvecs(Wb, rpad, D.as<double>() + (size_t)b * rpad, rpad, T2c, kb, true,
+                                        Wk.as<double2>(), T2c, st));
+                TRY(launch_cgemm<double>(3 * nbk[b], kb, T2c, ONE, Xb(b), m, 0, Wk.p, T2c, 0, ZERO,
+                                         Call.as<double2>() + coff[b], 3L * nbk[b], st));
+            } else {
+                TRY(launch_pick_eigvecs(Wb, rpad, D.as<double>() + (size_t)b * rpad, rpad, 3 * nbk[b], kb, false,
+                                        Call.as<double2>() + coff[b], 3L * nbk[b], st));
+            }
+        }
+    } else {
+        // thresholds below what the Gram route resolves: one-by-one SVD
+        DBuf Usv, Sv, Esup, dinfo;
+        const long rmax = std::min<long>(mmax, T2c);
+        TRY(Usv.alloc(sizeof(double2) * mmax * rmax));
+        TRY(Sv.alloc(sizeof(double) * rmax));
+        TRY(Esup.alloc(sizeof(double) * rmax));
+        TRY(dinfo.alloc(sizeof(int)));
+        std::vector<double> hs(rmax);
+        for (int b = 0; b < nb; ++b) {
+            const int m = 3 * sb[b], r = std::min(m, T2c);
+            MIDAS_ROC(rocsolver_zgesvd(rh.h, rocblas_svect_singular, rocblas_svect_none, m, T2c,
+                                       (rocblas_double_complex *)Xb(b), m, Sv.as<double>(),
+                                       (rocblas_double_complex *)Usv.p, m, nullptr, 1, Esup.as<double>(),
+                                       rocblas_outofplace, dinfo.as<int>()));
+            int sinfo = 0;
+            MIDAS_HIP(hipMemcpyAsync(&sinfo, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            TRY(copy_sync(hs.data(), Sv.p, sizeof(double) * r, hipMemcpyDeviceToHost, st));
+            MIDAS_CHECK(sinfo == 0, -2, "cbfm: SVD of block %d did not converge (info %d)", b, sinfo);
+            int kb = 0;
+            for (int d = 0; d < r; ++d)
+                if (hs[0] > 0 && hs[d] / hs[0] >= thr) ++kb;
+            K[b] = kb;
+            TRY(copy2d(Usv.as<double2>(), m, Call.as<double2>() + coff[b], 3L * nbk[b], 3 * nbk[b], kb, st));
+        }
     }
-    Zb.release();
-    Eref.release();
-    Xb.release();
-    Usv.release();
+    phase("CBF singular subspaces");
+    Xall.release();
     std::vector<long> offs(nb + 1, 0);
     int kbmax = 0;
     for (int b = 0; b < nb; ++b) {
@@ -295,6 +404,7 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
         }
     }
 
+    phase(use_aca ? "reduced matrix (ACA)" : "reduced matrix (dense)");
     // ---- 3. excitation, reduced solve, fields
     DBuf Einc, Vc, Al, dE;
     TRY(Einc.alloc(sizeof(double2) * N3 * T2));
@@ -310,6 +420,7 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
     int r = solve_dev(ctx, (int)Kt, T2, zc_, Kt, nullptr, 0, Vc.as<double2>(), Kt, Al.as<double2>(), Kt, mode, nullptr,
                       &inf, true);
     if (r < 0) return r;
+    phase("reduced solve");
     Zc.release();
     Vc.release();
     TRY(dE.alloc(sizeof(double2) * N3 * T2));
@@ -320,6 +431,8 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                                      0, ZERO, dE.as<double2>() + 3L * bs[i], N3, st));
     MIDAS_HIP(hipMemcpyAsync(E, dE.p, dE.bytes, hipMemcpyDeviceToHost, st));
     MIDAS_HIP(hipStreamSynchronize(st));
+    phase("fields + copy out");
+    if (tm) std::fprintf(stderr, "[cbfm] K total %ld over %d blocks\n", Kt, nb);
     if (K_out)
         for (int b = 0; b < nb; ++b) K_out[b] = K[b];
     if (info) *info = inf;

@@ -96,11 +96,11 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, cplx<T> alph
             if (!tA) { ar = tid & 63; ac = (tid >> 6) + 4 * q; }
             else { ac = tid & 15; ar = (tid >> 4) + 16 * q; }
             Ar[ac][ar] = ra[q].x;
-            Ai[ac][ar] = ra[q].y;
+            Ai[ac][ar] = tA == 2 ? -ra[q].y : ra[q].y;
             if (!tB) { br = tid & 15; bc = (tid >> 4) + 16 * q; }
             else { bc = tid & 63; br = (tid >> 6) + 4 * q; }
             Br[br][bc] = rb[q].x;
-            Bi[br][bc] = rb[q].y;
+            Bi[br][bc] = tB == 2 ? -rb[q].y : rb[q].y;
         }
     };
 

@@ -47,6 +47,7 @@ int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, const 
           long ldb, int nrhs, typename cvec<T>::t *tmp, hipStream_t st);
 void ipiv_to_perm(const int *ipiv, int n, int *perm);
 
+// C = alpha op(A) op(B) + beta C; op: 0 none, 1 transpose, 2 conjugate transpose.
 template <typename T>
 int launch_cgemm(int m, int n, int k, cplx<T> alpha, const void *A, long lda, int tA, const void *B, long ldb,
                  int tB, cplx<T> beta, void *C, long ldc, hipStream_t st, int batch = 1, long sA = 0, long sB = 0,

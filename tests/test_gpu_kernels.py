@@ -64,7 +64,7 @@ def _gemm(mid, prec, A, B, C0, alpha, beta, tA, tB):
 
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("shape", [(64, 64, 16), (100, 37, 53), (1, 5, 200), (130, 2, 300)])
-@pytest.mark.parametrize("tA,tB", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("tA,tB", [(0, 0), (1, 0), (0, 1), (1, 1), (2, 0), (0, 2), (2, 1)])
 def test_gemm(mid, prec, shape, tA, tB):
     m, n, k = shape
     print(f"\n[case] gemm prec={prec} m={m} n={n} k={k} tA={tA} tB={tB}", flush=True)
@@ -75,7 +75,8 @@ def test_gemm(mid, prec, shape, tA, tB):
     C0 = cr(m, n)
     alpha, beta = 0.7 - 0.2j, -1.1 + 0.4j
     got = _gemm(mid, prec, A, B, C0, alpha, beta, tA, tB)
-    ref = alpha * ((A.T if tA else A) @ (B.T if tB else B)) + beta * C0
+    op = lambda X, t: X if t == 0 else (X.T if t == 1 else X.conj().T)   # 2: conjugate transpose
+    ref = alpha * (op(A, tA) @ op(B, tB)) + beta * C0
     err = np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * k + np.abs(C0).max())
     print("max scaled err", err)
     assert err < (1e-15 if prec == 64 else 3e-7)
