@@ -6,12 +6,13 @@ one x-polarised plane wave (2 right-hand sides: V and H).  Synthetic seeded
 geometry; inputs resident in HBM when the timed region starts.
 
 One step = FP64 matrix fill (+ its FP32 copy) + excitation + complex-FP32 LU
-whose trailing updates run on FP16 matrix cores (split hi/lo, 3 products) +
-FP64 iterative refinement to a backward error <= 1e-14.  Outside the timed
+whose trailing updates run on FP16 matrix cores (per-column scaled operands,
+1 product; --mode f16x3 splits hi/lo and takes 3 products) + FP64 iterative
+refinement to a backward error <= 1e-14.  Outside the timed
 steps the plain path (FP64 fill + FP64 LU) is run once to give the speedup and
 the relative internal-field error of the mixed path against it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N] [--mode f16x3]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N] [--mode f16]
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -32,6 +33,11 @@ HBM_GBS = 8000.0
 MODES = {"fp64": 0, "c32": 1, "f16": 2, "f16x3": 3, "bf16": 4, "bf16x3": 5}
 MODE_DTYPE = {"fp64": "f64", "c32": "f32", "f16": "f16", "f16x3": "f16", "bf16": "bf16", "bf16x3": "bf16"}
 MODE_TERMS = {"fp64": 1, "c32": 1, "f16": 1, "f16x3": 3, "bf16": 1, "bf16x3": 3}
+# HBM bytes per k_trail16 launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors
+# a 1 GiB device copy reads on the same counters), keyed by (cells, mode, nb);
+# other configurations report null.
+TRAFFIC_PER_LAUNCH = {}
 
 
 def log(*a):
@@ -79,10 +85,13 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=20000)
-    ap.add_argument("--mode", default="f16x3", choices=[m for m in MODES if m != "fp64"])
+    ap.add_argument("--mode", default="f16", choices=[m for m in MODES if m != "fp64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample-cells", type=int, default=3500)
+    ap.add_argument("--nb", type=int, default=0, help="LU panel width (0: library default)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy 1 GiB device to device (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -99,6 +108,13 @@ def main():
     n3 = 3 * n
     log(f"[bench] {s.name}: n = {n3} unknowns, Z = {n3 * n3 * 16 / 1e9:.1f} GB")
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copy probe: 1 GiB read + 1 GiB written")
+        del a, b
     t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
     dx, dy, dz, dsc = t(s.x), t(s.y), t(s.z), t(s.sc)
     deps = t(np.ascontiguousarray(s.eps).view(np.float64))
@@ -111,7 +127,7 @@ def main():
     cells = api.cells_t(n, ptr(dx), ptr(dy), ptr(dz), ptr(dsc), ptr(deps), None, None, None, None)
 
     def run(mode, out):
-        opts = api.solve_opts_t(30, 1e-14, 0)
+        opts = api.solve_opts_t(30, 1e-14, args.nb if mode != 0 else 0)
         info = api.solve_info_t()
         tf = C.c_double(0)
         r = m.lib.midas_dev_mom_solve(m.ctx, C.byref(cells), C.c_double(s.k0), C.c_int(api.Q_ALL), C.c_int(1),
@@ -163,13 +179,31 @@ def main():
         "ir_iterations": infos[-1][0].iterations, "ir_berr": infos[-1][0].berr,
     }
     dt = MODE_DTYPE[args.mode]
-    trail_tf = trail_flops / (trail_ms * 1e-3) / 1e12 if trail_ms > 0 else 0.0
+    trail_bytes = tr[3].value
+    trail_s = trail_ms * 1e-3
+    trail_tf = trail_flops / trail_s / 1e12 if trail_ms > 0 else 0.0
+    trail_gbs = trail_bytes / trail_s / 1e9 if trail_ms > 0 else 0.0
+    # the trailing update reads and writes its C block once per launch: with a
+    # panel of k columns its intensity is 8 m n k / (16 m n + planes) ~ k/2
+    # flop/B, below the MFMA ridge (peak / 8 TB/s) for k <= 512 -> HBM-bound
+    intensity = trail_flops / trail_bytes if trail_bytes else 0.0
+    ridge = PEAK_TFLOPS[dt] * 1e12 / (HBM_GBS * 1e9)
+    hbm_bound = intensity < ridge
+    traffic = TRAFFIC_PER_LAUNCH.get((args.cells, args.mode, args.nb))
     result["roofline"] = {
-        "kernel": "trailing-update GEMM of the LU (k_trail16 / k_cgemm)", "bound": "mfma",
-        "achieved": round(trail_tf, 2), "peak": PEAK_TFLOPS[dt], "unit": "TFLOP/s",
-        "frac": round(trail_tf / PEAK_TFLOPS[dt], 4), "traffic": None,
+        "kernel": "k_trail16 (LU trailing update, split-precision MFMA)" if dt in ("f16", "bf16")
+        else "k_cgemm (LU trailing update)",
+        "bound": "hbm" if hbm_bound else "mfma",
+        "achieved": round(trail_gbs if hbm_bound else trail_tf, 2),
+        "peak": HBM_GBS if hbm_bound else PEAK_TFLOPS[dt],
+        "unit": "GB/s" if hbm_bound else "TFLOP/s",
+        "frac": round((trail_gbs / HBM_GBS) if hbm_bound else (trail_tf / PEAK_TFLOPS[dt]), 4),
+        "traffic": traffic,
         "launches": trail_launches, "avg_launch_ms": round(trail_ms / max(trail_launches, 1), 4),
+        "alg_bytes_per_launch": trail_bytes / max(trail_launches, 1),
         "alg_flops_per_launch": trail_flops / max(trail_launches, 1),
+        "intensity_flop_per_byte": round(intensity, 1), "ridge_flop_per_byte": round(ridge, 1),
+        "mfma_achieved_tflops": round(trail_tf, 2), "mfma_peak_tflops": PEAK_TFLOPS[dt],
         "hw_mfma_flops_per_alg_flop": MODE_TERMS[args.mode],
     }
     result["fill_roofline"] = {

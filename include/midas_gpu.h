@@ -160,10 +160,15 @@ int midas_aca_matrix(midas_ctx *ctx, int m, int n, const double *A, long lda, in
  * Compute_EFields_CBFME_ACA.f90:1-1017).  Blocks are contiguous cell ranges
  * block_start[b] .. block_start[b+1]-1 (cells already in block order, as
  * Division_blocks leaves them); extension cells of block b are
- * ext_cells[ext_start[b] .. ext_start[b+1]-1].  CBF excitations are
- * ntr_cbf plane waves (th_cbf, ph_cbf), the problem excitations ntr plane
- * waves.  threshold <= 0 selects the reference's rule.  E (3n x 2ntr);
- * K_out[b] = number of CBFs of block b. */
+ * ext_cells[ext_start[b] .. ext_start[b+1]-1] (CBFM_Blocks_Ext).  CBF
+ * excitations are ntr_cbf plane waves (th_cbf, ph_cbf: Transmitters_CBFM of
+ * getTransmitters_CBFM), the problem excitations ntr plane waves.
+ * threshold <= 0 selects the reference's rule (1e-3 with ACA, else
+ * max(10^(-3-(ceil(max|m|)-2)), 1e-10) with |m| = sqrt(|eps|)).
+ * Outputs: E (3n x 2ntr, lde); K_out[b] = number of CBFs of block b
+ * (K_patchs); C_job (optional, may be NULL): the CBFs, block b's in rows
+ * 3*block_start[b] .. and columns 0..K_b-1 (C_job_patchs), ldc >= 3n and
+ * kcap columns; the call fails if some K_b exceeds kcap (K_b <= 2*ntr_cbf). */
 typedef struct {
     int nblocks;
     const int *block_start;   /* nblocks + 1 */
@@ -174,7 +179,8 @@ typedef struct {
 int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks, double k0,
                                int quirks, int ntr_cbf, const double *th_cbf, const double *ph_cbf, int ntr,
                                const double *tr_th, const double *tr_ph, int use_aca, int kmax, double eps_aca,
-                               double threshold, int mode, double *E, int *K_out, midas_solve_info_t *info);
+                               double threshold, int mode, double *E, long lde, int *K_out, double *C_job, long ldc,
+                               int kcap, midas_solve_info_t *info);
 
 /* ---------------- device-pointer variants (inputs resident in HBM) ----------------
  * Same semantics as above; every pointer is device memory; ordered on

@@ -285,7 +285,8 @@ class Midas:
 
     # -- Compute_EFields_CBFME(_ACA) -----------------------------------------
     def compute_efields_cbfme(self, cells, k0, blocks, ext, cbf_dirs, transmitters, use_aca=False, kmax=50,
-                              eps_aca=1e-4, threshold=0.0, mode=SOLVE_FP64, quirks=Q_ALL):
+                              eps_aca=1e-4, threshold=0.0, mode=SOLVE_FP64, quirks=Q_ALL,
+                              return_cbfs=False):
         cells = _cells(cells)
         nb = len(blocks)
         starts = np.zeros(nb + 1, np.int32)
@@ -308,14 +309,20 @@ class Midas:
         tth, tph = (np.ascontiguousarray(a, np.float64) for a in transmitters)
         E = np.empty((3 * cells.n, 2 * len(tth)), np.complex128, order="F")
         K = np.zeros(nb, np.int32)
+        kcap = 2 * len(cth)
+        Cj = np.zeros((3 * cells.n, kcap), np.complex128, order="F") if return_cbfs else None
         info = solve_info_t()
         cs = cells.struct()
         r = _check(self.lib.midas_compute_efields_cbfm(self.ctx, C.byref(cs), C.byref(bt), C.c_double(k0),
                                                        C.c_int(quirks), C.c_int(len(cth)), _p(cth), _p(cph),
                                                        C.c_int(len(tth)), _p(tth), _p(tph), C.c_int(int(use_aca)),
                                                        C.c_int(kmax), C.c_double(eps_aca), C.c_double(threshold),
-                                                       C.c_int(mode), _p(E), _p(K), C.byref(info)),
+                                                       C.c_int(mode), _p(E), C.c_long(3 * cells.n), _p(K),
+                                                       None if Cj is None else _p(Cj), C.c_long(3 * cells.n),
+                                                       C.c_int(kcap), C.byref(info)),
                    "compute_efields_cbfm")
         d = info.as_dict()
         d["info"] = r
+        if return_cbfs:
+            return E, K, d, Cj[:, :max(int(K.max()), 1)]
         return E, K, d

@@ -118,7 +118,7 @@ static int trail_mode(int mode)
     return -1;
 }
 
-static int default_nb(int n, int mode) { return mode == MIDAS_SOLVE_FP64 ? 128 : 256; }
+static int default_nb(int n, int mode) { return mode == MIDAS_SOLVE_FP64 ? 128 : 512; }
 
 // Solve Z X = B on the device.  Z (n x n, ldz) FP64; if W32 is non-NULL it
 // already holds the FP32 copy of Z (mixed modes).  For MIDAS_SOLVE_FP64 Z is

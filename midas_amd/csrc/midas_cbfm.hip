@@ -117,10 +117,13 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                                           double k0, int quirks, int ntr_cbf, const double *th_cbf,
                                           const double *ph_cbf, int ntr, const double *tr_th, const double *tr_ph,
                                           int use_aca, int kmax, double eps_aca, double threshold, int mode,
-                                          double *E, int *K_out, midas_solve_info_t *info)
+                                          double *E, long lde, int *K_out, double *C_job, long ldc, int kcap,
+                                          midas_solve_info_t *info)
 {
     MIDAS_CHECK(ctx && cells && blocks && E && th_cbf && ph_cbf && tr_th && tr_ph, -1, "cbfm: bad arguments");
     const int N = cells->n, nb = blocks->nblocks;
+    MIDAS_CHECK(lde >= 3L * N, -1, "cbfm: lde %ld < 3n", lde);
+    MIDAS_CHECK(!C_job || (ldc >= 3L * N && kcap > 0), -1, "cbfm: C_job needs ldc >= 3n and kcap > 0");
     MIDAS_CHECK(N > 0 && nb > 0 && blocks->block_start && blocks->ext_start, -1, "cbfm: empty cells or blocks");
     MIDAS_CHECK(ntr_cbf > 0 && ntr > 0, -1, "cbfm: ntr_cbf %d, ntr %d", ntr_cbf, ntr);
     MIDAS_CHECK(mode >= 0 && mode <= 5, -1, "cbfm: unknown solver mode %d", mode);
@@ -429,7 +432,18 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
         if (K[i])
             TRY(launch_cgemm<double>(3 * nbk[i], T2, K[i], ONE, Cb(i), 3L * nbk[i], 0, Al.as<double2>() + offs[i], Kt,
                                      0, ZERO, dE.as<double2>() + 3L * bs[i], N3, st));
-    MIDAS_HIP(hipMemcpyAsync(E, dE.p, dE.bytes, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipMemcpy2DAsync(E, sizeof(double2) * lde, dE.p, sizeof(double2) * N3, sizeof(double2) * N3, T2,
+                               hipMemcpyDeviceToHost, st));
+    if (C_job) {
+        MIDAS_CHECK(kbmax <= kcap, -1, "cbfm: a block has %d CBFs, more than kcap = %d", kbmax, kcap);
+        std::vector<double2> zero((size_t)ldc, make_double2(0.0, 0.0));
+        for (int c = 0; c < kcap; ++c) std::copy(zero.begin(), zero.end(), (double2 *)C_job + (size_t)c * ldc);
+        for (int b = 0; b < nb; ++b)
+            if (K[b])
+                MIDAS_HIP(hipMemcpy2DAsync((double2 *)C_job + 3L * bs[b], sizeof(double2) * ldc, Cb(b),
+                                           sizeof(double2) * 3 * nbk[b], sizeof(double2) * 3 * nbk[b], K[b],
+                                           hipMemcpyDeviceToHost, st));
+    }
     MIDAS_HIP(hipStreamSynchronize(st));
     phase("fields + copy out");
     if (tm) std::fprintf(stderr, "[cbfm] K total %ld over %d blocks\n", Kt, nb);

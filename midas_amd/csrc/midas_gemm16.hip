@@ -371,6 +371,10 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
     hipLaunchKernelGGL((k_split_a<BF, P>), dim3(mt * T_M / 256 + 1, kst * 2), dim3(256), 0, st, A, lda, m, k, mt, kst,
                        Ap);
     hipLaunchKernelGGL((k_split_b<BF, P>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, kst, Bp, sc);
+    // live accounting of the GEMM kernel alone: algorithmic flops 8 m n k
+    // (complex multiply-add); bytes = C read and written (complex FP32) plus
+    // one read of the operand planes (P planes of 2 B per real component)
+    ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 16.0 * m * n + 4.0 * P * ((double)m + n) * k);
     hipLaunchKernelGGL((k_trail16<BF, P, TERMS>), dim3(mt * nt), dim3(256), 0, st, m, n, kst, mt, nt, Ap, Bp, sc, C,
                        ldc);
     MIDAS_HIP(hipGetLastError());
@@ -404,9 +408,10 @@ template <>
 int TrailingGemm<float>::operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C,
                                     long ldc, hipStream_t st) const
 {
-    ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 8.0 * (2.0 * m * n + (double)m * k + (double)k * n));
-    if (mode == TRAIL_NATIVE)
+    if (mode == TRAIL_NATIVE) {
+        ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 8.0 * (2.0 * m * n + (double)m * k + (double)k * n));
         return launch_cgemm<float>(m, n, k, c32{-1.f, 0.f}, A, lda, 0, B, ldb, 0, c32{1.f, 0.f}, C, ldc, st);
+    }
     return launch_trail16(mode, m, n, k, (const float2 *)A, lda, (const float2 *)B, ldb, (float2 *)C, ldc, scratch,
                           st);
 }

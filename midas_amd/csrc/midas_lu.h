@@ -16,6 +16,7 @@ template <typename T> struct LuWork {
     cplx<T> *pinv = nullptr;         // 1 / current pivot
     cplx<T> *linv = nullptr;         // inverses of the diagonal blocks of L (nb x nb each)
     cplx<T> *uinv = nullptr;         // inverses of the diagonal blocks of U
+    cplx<T> *u12 = nullptr;          // nb x n staging for U12 = L11^-1 A12
     hipStream_t pstream = nullptr;   // panel stream (look-ahead), default priority
     hipEvent_t ev_panel = nullptr, ev_next = nullptr;
     int alloc(int n, int nb);

@@ -10,8 +10,9 @@
 //                             the per-workgroup maxima, swaps the pivot row across
 //                             the outer panel and records ipiv (LAPACK getf2 order);
 //   inner block update      : IBxIB unit-lower solve + GEMM on the rest of the panel;
-//   outside the panel       : row swaps (laswp), U12 = L11^-1 A12 through an
-//                             explicit L11^-1 and a GEMM, trailing GEMM A22 -= L21 U12.
+//   end of a panel          : L11^-1 of its diagonal block (one wave per column);
+//   outside the panel       : row swaps (laswp), U12 = L11^-1 A12 as one GEMM with
+//                             that inverse, trailing GEMM A22 -= L21 U12.
 //
 // T = double is the FP64 baseline; T = float holds the complex-FP32 factors of
 // the mixed-precision solvers, whose trailing GEMM runs either on FP32 MFMA or
@@ -189,50 +190,75 @@ __global__ void k_laswp(typename cvec<T>::t *A, long lda, const int *ipiv, int k
     }
 }
 
-// Inverses of all diagonal nb x nb blocks of the factors, one thread per
-// (block, column): unit-lower L (LOWER) or upper U.  Inv holds block b as a
-// dense nb x nb column-major matrix at Inv + b*nb*nb (rows/cols beyond a short
-// last block are left untouched).
+// Inverse of one triangular bs x bs diagonal block D (bs <= 512), one wave per
+// column of the inverse.  Column-oriented substitution: lane l owns rows
+// l, l+64, ..; at step q the solved x_q is broadcast and every row updates with
+// column q of D (a unit-stride read).  LOWER: unit lower L, x = L^-1 e_c;
+// otherwise upper U with its diagonal, x = U^-1 e_c.  Inv is ldi x bs.
+constexpr int TRINV_R = 8;   // rows per lane
 template <typename T, bool LOWER>
-__global__ void k_tri_inv(const typename cvec<T>::t *A, long lda, int n, int nb, typename cvec<T>::t *Inv)
+__global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D, long lda, int bs,
+                                                    typename cvec<T>::t *Inv, long ldi)
 {
     typedef typename cvec<T>::t V;
-    const int b = blockIdx.x;
-    const int c = blockIdx.y * blockDim.x + threadIdx.x;
-    const int k = b * nb;
-    const int bs = min(nb, n - k);
-    if (c >= bs) return;
-    const V *D = A + (long)k + (long)k * lda;
-    V *x = Inv + (long)b * nb * nb + (long)c * nb;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= bs) return;   // wave-uniform
+    cplx<T> x[TRINV_R];
+#pragma unroll
+    for (int t = 0; t < TRINV_R; ++t) x[t] = cplx<T>{(T)((lane + 64 * t) == c ? 1 : 0), (T)0};
+    auto bcast = [&](int q) {
+        cplx<T> v = {(T)0, (T)0};
+#pragma unroll
+        for (int t = 0; t < TRINV_R; ++t)
+            if (t == (q >> 6)) v = x[t];
+        return cplx<T>{__shfl(v.r, q & 63, 64), __shfl(v.i, q & 63, 64)};
+    };
     if (LOWER) {
-        for (int r = 0; r < c; ++r) x[r] = V{0, 0};
-        x[c] = V{1, 0};
-        for (int r = c + 1; r < bs; ++r) {
-            cplx<T> acc = {0, 0};
-            for (int q = c; q < r; ++q) {
-                V l = D[(long)r + (long)q * lda];
-                V xq = x[q];
-                acc = csub(acc, cmul(cplx<T>{l.x, l.y}, cplx<T>{xq.x, xq.y}));
+        for (int q = c; q < bs - 1; ++q) {
+            const cplx<T> xq = bcast(q);
+#pragma unroll
+            for (int t = 0; t < TRINV_R; ++t) {
+                const int r = lane + 64 * t;
+                if (r > q && r < bs) {
+                    V l = D[(long)r + (long)q * lda];
+                    x[t] = csub(x[t], cmul(cplx<T>{l.x, l.y}, xq));
+                }
             }
-            x[r] = V{acc.r, acc.i};
         }
     } else {
-        for (int r = c + 1; r < bs; ++r) x[r] = V{0, 0};
-        V d = D[(long)c + (long)c * lda];
-        cplx<T> xc = cinv(cplx<T>{d.x, d.y});
-        x[c] = V{xc.r, xc.i};
-        for (int r = c - 1; r >= 0; --r) {
-            cplx<T> acc = {0, 0};
-            for (int q = r + 1; q <= c; ++q) {
-                V u = D[(long)r + (long)q * lda];
-                V xq = x[q];
-                acc = csub(acc, cmul(cplx<T>{u.x, u.y}, cplx<T>{xq.x, xq.y}));
+        for (int q = c; q >= 0; --q) {
+            // x_q /= U(q, q) on the owning lane, then broadcast
+            V d = D[(long)q + (long)q * lda];
+            const cplx<T> dq = cinv(cplx<T>{d.x, d.y});
+#pragma unroll
+            for (int t = 0; t < TRINV_R; ++t)
+                if (lane + 64 * t == q) x[t] = cmul(x[t], dq);
+            const cplx<T> xq = bcast(q);
+#pragma unroll
+            for (int t = 0; t < TRINV_R; ++t) {
+                const int r = lane + 64 * t;
+                if (r < q) {
+                    V u = D[(long)r + (long)q * lda];
+                    x[t] = csub(x[t], cmul(cplx<T>{u.x, u.y}, xq));
+                }
             }
-            V dr = D[(long)r + (long)r * lda];
-            acc = cmul(acc, cinv(cplx<T>{dr.x, dr.y}));
-            x[r] = V{acc.r, acc.i};
         }
     }
+#pragma unroll
+    for (int t = 0; t < TRINV_R; ++t) {
+        const int r = lane + 64 * t;
+        if (r < bs) Inv[(long)r + (long)c * ldi] = V{x[t].r, x[t].i};
+    }
+}
+
+template <typename T, bool LOWER>
+static int trinv(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T>::t *Inv, long ldi, hipStream_t st)
+{
+    MIDAS_CHECK(bs <= 64 * TRINV_R, -1, "trinv: block %d above %d", bs, 64 * TRINV_R);
+    hipLaunchKernelGGL((k_trinv_wave<T, LOWER>), dim3((bs + 3) / 4), dim3(256), 0, st, D, lda, bs, Inv, ldi);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
 }
 
 // Copy a (rows x cols) block between column-major arrays.
@@ -277,6 +303,27 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
             }
         }
     }
+    // inverse of the unit-lower diagonal block L11 of this panel: U12 = L11^-1 A12
+    // becomes one GEMM on the main stream, and getrs reuses it
+    return trinv<T, true>(A + k + (long)k * lda, lda, nb, (typename cvec<T>::t *)w.linv + (long)(k / w.nb) * w.nb * w.nb,
+                          w.nb, st);
+}
+
+// U12(:, c0:c1) = L11^-1 A12(:, c0:c1) through the workspace, for panel k.
+template <typename T>
+static int u12_solve(typename cvec<T>::t *A, long lda, int k, int nb, int c0, int c1, LuWork<T> &w, hipStream_t st)
+{
+    typedef typename cvec<T>::t V;
+    const int nc = c1 - c0;
+    if (nc <= 0) return 0;
+    const V *Li = (const V *)w.linv + (long)(k / w.nb) * w.nb * w.nb;
+    V *tmp = (V *)w.u12;
+    int r = launch_cgemm<T>(nb, nc, nb, cone<T>(), Li, w.nb, 0, A + k + (long)c0 * lda, lda, 0, czero<T>(), tmp, nb,
+                            st, 1, 0, 0, 0);
+    if (r) return r;
+    hipLaunchKernelGGL(k_copy2d<T>, dim3((nb + 255) / 256, nc), dim3(256), 0, st, tmp, (long)nb,
+                       A + k + (long)c0 * lda, lda, nb, nc);
+    MIDAS_HIP(hipGetLastError());
     return 0;
 }
 
@@ -310,21 +357,11 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
         }
         int nr = n - k - nb;
         if (nr <= 0) continue;
-        // U12 = L11^-1 A12, blocked by the inner panel width
-        for (int jb = k; jb < k + nb; jb += LU_IB) {
-            int jend = min(jb + LU_IB, k + nb);
-            hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((nr + 63) / 64), dim3(64), 0, st, A, lda, jb, jend - jb,
-                               k + nb, n);
-            MIDAS_HIP(hipGetLastError());
-            if (jend < k + nb) {
-                r = launch_cgemm<T>(k + nb - jend, nr, jend - jb, cmone<T>(), A + jend + (long)jb * lda, lda, 0,
-                                    A + jb + (long)(k + nb) * lda, lda, 0, cone<T>(), A + jend + (long)(k + nb) * lda,
-                                    lda, st, 1, 0, 0, 0);
-                if (r) return r;
-            }
-        }
-        // A22 -= L21 U12: first the next panel's columns, then the rest
+        // The next panel's columns first (U12 = L11^-1 A12, then A22 -= L21 U12),
+        // so that the panel stream can start on them; then the rest.
         const int nb1 = min(w.nb, nr);
+        r = u12_solve<T>(A, lda, k, nb, k + nb, k + nb + nb1, w, st);
+        if (r) return r;
         r = tg(nr, nb1, nb, A + (k + nb) + (long)k * lda, lda, A + k + (long)(k + nb) * lda, lda,
                A + (k + nb) + (long)(k + nb) * lda, lda, st);
         if (r) return r;
@@ -334,18 +371,20 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
         if (r) return r;
         MIDAS_HIP(hipEventRecord(w.ev_panel, ps));
         if (nr > nb1) {
+            r = u12_solve<T>(A, lda, k, nb, k + nb + nb1, n, w, st);
+            if (r) return r;
             r = tg(nr, nr - nb1, nb, A + (k + nb) + (long)k * lda, lda, A + k + (long)(k + nb + nb1) * lda, lda,
                    A + (k + nb) + (long)(k + nb + nb1) * lda, lda, st);
             if (r) return r;
         }
     }
     MIDAS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
-    // inverses of the diagonal blocks, used by getrs
-    const int nblk = (n + w.nb - 1) / w.nb;
-    dim3 g(nblk, (w.nb + 63) / 64);
-    hipLaunchKernelGGL((k_tri_inv<T, true>), g, dim3(64), 0, st, A, lda, n, w.nb, (V *)w.linv);
-    hipLaunchKernelGGL((k_tri_inv<T, false>), g, dim3(64), 0, st, A, lda, n, w.nb, (V *)w.uinv);
-    MIDAS_HIP(hipGetLastError());
+    // inverses of the diagonal blocks of U, used by getrs (those of L come from the panels)
+    for (int k = 0; k < n; k += w.nb) {
+        r = trinv<T, false>(A + k + (long)k * lda, lda, min(w.nb, n - k), (V *)w.uinv + (long)(k / w.nb) * w.nb * w.nb,
+                            w.nb, st);
+        if (r) return r;
+    }
     return 0;
 }
 
@@ -436,6 +475,7 @@ int LuWork<T>::alloc(int n_, int nb_)
     const size_t nblk = (size_t)(n + nb - 1) / nb;
     MIDAS_HIP(hipMalloc(&linv, sizeof(cplx<T>) * nblk * nb * nb));
     MIDAS_HIP(hipMalloc(&uinv, sizeof(cplx<T>) * nblk * nb * nb));
+    MIDAS_HIP(hipMalloc(&u12, sizeof(cplx<T>) * (size_t)nb * n));
     MIDAS_HIP(hipStreamCreate(&pstream));
     MIDAS_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
     MIDAS_HIP(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming));
@@ -451,6 +491,7 @@ void LuWork<T>::release()
     hipFree(pinv);
     hipFree(linv);
     hipFree(uinv);
+    hipFree(u12);
     if (pstream) {
         hipStreamSynchronize(pstream);
         hipStreamDestroy(pstream);
@@ -462,7 +503,7 @@ void LuWork<T>::release()
     pmax = nullptr;
     pidx = counter = info = nullptr;
     pinv = nullptr;
-    linv = uinv = nullptr;
+    linv = uinv = u12 = nullptr;
 }
 
 template struct LuWork<double>;

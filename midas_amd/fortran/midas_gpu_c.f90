@@ -1,0 +1,125 @@
+! Fortran (iso_c_binding) face of libmidas_gpu.so, declared from
+! include/midas_gpu.h.  Used by the drop-in subroutines in midas_dropin.f90;
+! it depends on nothing of the reference and compiles on its own.
+module midas_gpu_c
+    use iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: MIDAS_Q_PI = 1, MIDAS_Q_LITERALS = 2, MIDAS_Q_CEXT_SUM = 4, &
+                                 MIDAS_Q_FP32_BLOCKS = 8, MIDAS_Q_ALL = 15
+    integer(c_int), parameter :: MIDAS_SOLVE_FP64 = 0, MIDAS_SOLVE_C32_IR = 1, MIDAS_SOLVE_F16_IR = 2, &
+                                 MIDAS_SOLVE_F16X3_IR = 3, MIDAS_SOLVE_BF16_IR = 4, MIDAS_SOLVE_BF16X3_IR = 5
+
+    type, bind(c) :: midas_cells_t
+        integer(c_int) :: n
+        type(c_ptr) :: x, y, z, sc, eps, kappa, che, znnpp, id
+    end type midas_cells_t
+
+    type, bind(c) :: midas_solve_opts_t
+        integer(c_int) :: max_iter
+        real(c_double) :: tol
+        integer(c_int) :: nb
+    end type midas_solve_opts_t
+
+    type, bind(c) :: midas_solve_info_t
+        integer(c_int) :: info, iterations
+        real(c_double) :: berr, anorm, t_factor_ms, t_solve_ms, t_refine_ms
+    end type midas_solve_info_t
+
+    type, bind(c) :: midas_blocks_t
+        integer(c_int) :: nblocks
+        type(c_ptr) :: block_start, ext_start, ext_cells
+    end type midas_blocks_t
+
+    interface
+        integer(c_int) function midas_init(device, ctx) bind(c, name="midas_init")
+            import :: c_int, c_ptr
+            integer(c_int), value :: device
+            type(c_ptr), intent(out) :: ctx
+        end function midas_init
+
+        integer(c_int) function midas_finalize(ctx) bind(c, name="midas_finalize")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function midas_finalize
+
+        type(c_ptr) function midas_last_error() bind(c, name="midas_last_error")
+            import :: c_ptr
+        end function midas_last_error
+
+        integer(c_int) function midas_compute_efields_mom(ctx, cells, k0, quirks, ntr, tr_th, tr_ph, nrx, rx_th, &
+                                                          rx_ph, mode, opts, s_total, c_ext, c_abs, e, info) &
+            bind(c, name="midas_compute_efields_mom")
+            import :: c_int, c_ptr, c_double, midas_cells_t, midas_solve_info_t
+            type(c_ptr), value :: ctx
+            type(midas_cells_t), intent(in) :: cells
+            real(c_double), value :: k0
+            integer(c_int), value :: quirks, ntr, nrx, mode
+            real(c_double), intent(in) :: tr_th(*), tr_ph(*), rx_th(*), rx_ph(*)
+            type(c_ptr), value :: opts, s_total, c_ext, c_abs, e
+            type(midas_solve_info_t), intent(out) :: info
+        end function midas_compute_efields_mom
+
+        integer(c_int) function midas_compute_efields_cbfm(ctx, cells, blocks, k0, quirks, ntr_cbf, th_cbf, ph_cbf, &
+                                                           ntr, tr_th, tr_ph, use_aca, kmax, eps_aca, threshold, &
+                                                           mode, e, lde, k_out, c_job, ldc, kcap, info) &
+            bind(c, name="midas_compute_efields_cbfm")
+            import :: c_int, c_long, c_ptr, c_double, midas_cells_t, midas_blocks_t, midas_solve_info_t
+            type(c_ptr), value :: ctx
+            type(midas_cells_t), intent(in) :: cells
+            type(midas_blocks_t), intent(in) :: blocks
+            real(c_double), value :: k0, eps_aca, threshold
+            integer(c_int), value :: quirks, ntr_cbf, ntr, use_aca, kmax, mode, kcap
+            real(c_double), intent(in) :: th_cbf(*), ph_cbf(*), tr_th(*), tr_ph(*)
+            type(c_ptr), value :: e, c_job
+            integer(c_long), value :: lde, ldc
+            integer(c_int), intent(out) :: k_out(*)
+            type(midas_solve_info_t), intent(out) :: info
+        end function midas_compute_efields_cbfm
+
+        integer(c_int) function midas_aca_block(ctx, cells, k0, quirks, r0, nr, c0, nc, kmax, eps, u, v, k_out) &
+            bind(c, name="midas_aca_block")
+            import :: c_int, c_ptr, c_double, midas_cells_t
+            type(c_ptr), value :: ctx
+            type(midas_cells_t), intent(in) :: cells
+            real(c_double), value :: k0, eps
+            integer(c_int), value :: quirks, r0, nr, c0, nc, kmax
+            type(c_ptr), value :: u, v
+            integer(c_int), intent(out) :: k_out
+        end function midas_aca_block
+
+        integer(c_int) function midas_aca_matrix(ctx, m, n, a, lda, kmax, eps, u, v, k_out) &
+            bind(c, name="midas_aca_matrix")
+            import :: c_int, c_long, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: m, n, kmax
+            type(c_ptr), value :: a, u, v
+            integer(c_long), value :: lda
+            real(c_double), value :: eps
+            integer(c_int), intent(out) :: k_out
+        end function midas_aca_matrix
+
+        integer(c_size_t) function c_strlen(s) bind(c, name="strlen")
+            import :: c_size_t, c_ptr
+            type(c_ptr), value :: s
+        end function c_strlen
+    end interface
+
+contains
+
+    ! The library's last error message as a Fortran string.
+    function midas_error_message() result(msg)
+        character(len=:), allocatable :: msg
+        type(c_ptr) :: p
+        character(kind=c_char), pointer :: cs(:)
+        integer :: i, n
+        p = midas_last_error()
+        n = int(c_strlen(p))
+        call c_f_pointer(p, cs, [n])
+        allocate(character(len=n) :: msg)
+        do i = 1, n
+            msg(i:i) = cs(i)
+        end do
+    end function midas_error_message
+
+end module midas_gpu_c

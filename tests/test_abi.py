@@ -92,3 +92,32 @@ def test_bench_refuses_multi_gpu():
                        text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in r.stdout
+
+
+def _flang():
+    import shutil
+    for c in (shutil.which("flang"), "/opt/rocm/lib/llvm/bin/flang", "/opt/rocm/llvm/bin/flang"):
+        if c and os.path.exists(c):
+            return c
+    pytest.skip("flang not available")
+
+
+def test_fortran_binding_module_compiles(tmp_path):
+    """The iso_c_binding module the drop-in uses compiles on its own."""
+    fc = _flang()
+    r = subprocess.run([fc, "-c", os.path.join(ROOT, "midas_amd", "fortran", "midas_gpu_c.f90")], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_fortran_dropin_compiles_against_reference_modules(tmp_path):
+    """The drop-in subroutines compile against the reference's own modules
+    (Initialization, common_variables) as oracle/_ref builds them."""
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    if not os.path.exists(os.path.join(ref, "initialization.mod")):
+        pytest.skip("oracle/_ref modules not built (needs the reference tree)")
+    fc = _flang()
+    for f in ("midas_gpu_c.f90", "midas_dropin.f90"):
+        r = subprocess.run([fc, "-c", f"-I{ref}", f"-I{tmp_path}", os.path.join(ROOT, "midas_amd", "fortran", f)],
+                           cwd=tmp_path, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr

@@ -108,11 +108,19 @@ def test_cbfm_vs_oracle(mid, oracle, dlca, use_aca):
     oc = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
     th_c, ph_c = _cbf_dirs(40)
     tth, tph = np.array([0.0, 35.0]), np.array([0.0, 80.0])
-    E, K, info = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=use_aca)
-    Eo, Ko, _ = oracle.cbfm(oc, s.blocks, s.ext, th_c, ph_c, tth, tph, use_aca=use_aca)
+    E, K, info, Cj = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=use_aca,
+                                               return_cbfs=True)
+    Eo, Ko, Cs = oracle.cbfm(oc, s.blocks, s.ext, th_c, ph_c, tth, tph, use_aca=use_aca)
     print("K", list(K), "oracle", Ko, "E err", _rel(E, Eo), info)
     assert list(K) == list(Ko)
     assert _rel(E, Eo) < 1e-9
+    # the CBFs span the oracle's (the basis inside the span is not unique)
+    for b, blk in enumerate(s.blocks):
+        rows = slice(3 * int(blk[0]), 3 * int(blk[0]) + 3 * len(blk))
+        Cg = Cj[rows, :K[b]]
+        coef, *_ = np.linalg.lstsq(Cg, Cs[b], rcond=None)
+        assert _rel(Cg @ coef, Cs[b]) < 1e-9
+        assert np.all(Cj[rows, K[b]:] == 0)
     # CBFM approximates the MoM solution of the same discretisation
     Emom = np.linalg.solve(oracle.mom_fill(oc), oracle.mom_rhs(oc, tth, tph))
     print("CBFM vs MoM", _rel(E, Emom))
