@@ -24,6 +24,8 @@
 #include "midas_common.h"
 #include "midas_lu.h"
 
+#include <algorithm>
+
 namespace midas {
 
 typedef float f16v __attribute__((ext_vector_type(16)));
@@ -174,7 +176,8 @@ template <int I, typename S> __device__ inline S *ring_slot()
 #define MIDAS_WAIT_VM(N) __builtin_amdgcn_s_waitcnt(((N) & 15) | (((N) >> 4) << 14) | (7 << 4) | (15 << 8))
 
 template <bool BF, int P, int TERMS>
-__global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int mt, int nt, const u16 *__restrict__ Ap,
+__global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int mt, int nt, int group,
+                                                    const u16 *__restrict__ Ap,
                                                     const u16 *__restrict__ Bp, const float *__restrict__ bscale,
                                                     float2 *__restrict__ C, long ldc)
 {
@@ -187,17 +190,17 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
 
     // Tile order.  Each XCD (workgroup ids are dealt round-robin to the 8
     // XCDs) walks a contiguous range of the sequence L; L runs over groups of
-    // 16 row tiles, all column tiles of a group, row tiles fastest.  A group's
-    // A tiles (16 x 128 KB) stay in the XCD's L2 while its column tiles stream.
+    // `group` row tiles, all column tiles of a group, row tiles fastest.  The
+    // host sizes `group` so that a group's A tiles take about half of the
+    // XCD's 4 MB L2 and stay there while its column tiles stream.
     const int total = mt * nt;
     const int id = blockIdx.x;
     const int full = (total / 8) * 8;
     const int L = id < full ? (id % 8) * (full / 8) + id / 8 : id;
-    constexpr int GROUP = 16;
-    const int grp = L / (GROUP * nt);
-    const int rows_g = min(GROUP, mt - grp * GROUP);
-    const int li = L - grp * GROUP * nt;
-    const int tm = grp * GROUP + li % rows_g, tn = li / rows_g;
+    const int grp = L / (group * nt);
+    const int rows_g = min(group, mt - grp * group);
+    const int li = L - grp * group * nt;
+    const int tm = grp * group + li % rows_g, tn = li / rows_g;
     const int m0 = tm * T_M, n0 = tn * T_N;
 
     const int tid = threadIdx.x, lane = tid & 63;
@@ -375,8 +378,11 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
     // (complex multiply-add); bytes = C read and written (complex FP32) plus
     // one read of the operand planes (P planes of 2 B per real component)
     ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 16.0 * m * n + 4.0 * P * ((double)m + n) * k);
-    hipLaunchKernelGGL((k_trail16<BF, P, TERMS>), dim3(mt * nt), dim3(256), 0, st, m, n, kst, mt, nt, Ap, Bp, sc, C,
-                       ldc);
+    // row tiles per group: A tiles of the group in ~2 MB of the XCD's L2
+    const long a_tile = (long)T_M * kst * T_K * 2 * P * 2;
+    const int group = (int)std::max(2L, std::min(16L, (2L << 20) / a_tile));
+    hipLaunchKernelGGL((k_trail16<BF, P, TERMS>), dim3(mt * nt), dim3(256), 0, st, m, n, kst, mt, nt, group, Ap, Bp,
+                       sc, C, ldc);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }

@@ -190,13 +190,15 @@ __global__ void k_laswp(typename cvec<T>::t *A, long lda, const int *ipiv, int k
     }
 }
 
-// Inverse of one triangular bs x bs diagonal block D (bs <= 512), one wave per
+// Inverse of one triangular bs x bs diagonal block D (bs <= 64 R), one wave per
 // column of the inverse.  Column-oriented substitution: lane l owns rows
 // l, l+64, ..; at step q the solved x_q is broadcast and every row updates with
-// column q of D (a unit-stride read).  LOWER: unit lower L, x = L^-1 e_c;
-// otherwise upper U with its diagonal, x = U^-1 e_c.  Inv is ldi x bs.
-constexpr int TRINV_R = 8;   // rows per lane
-template <typename T, bool LOWER>
+// column q of D (a unit-stride read).  The columns of D are loaded QC at a time
+// ahead of the dependent chain of broadcasts, so the loop waits for memory once
+// per QC steps.  LOWER: unit lower L, x = L^-1 e_c; otherwise upper U with its
+// diagonal, x = U^-1 e_c.  Inv is ldi x bs.
+constexpr int TRINV_QC = 8;
+template <typename T, bool LOWER, int R>
 __global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D, long lda, int bs,
                                                     typename cvec<T>::t *Inv, long ldi)
 {
@@ -204,49 +206,65 @@ __global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D
     const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (c >= bs) return;   // wave-uniform
-    cplx<T> x[TRINV_R];
+    cplx<T> x[R];
 #pragma unroll
-    for (int t = 0; t < TRINV_R; ++t) x[t] = cplx<T>{(T)((lane + 64 * t) == c ? 1 : 0), (T)0};
+    for (int t = 0; t < R; ++t) x[t] = cplx<T>{(T)((lane + 64 * t) == c ? 1 : 0), (T)0};
     auto bcast = [&](int q) {
         cplx<T> v = {(T)0, (T)0};
 #pragma unroll
-        for (int t = 0; t < TRINV_R; ++t)
+        for (int t = 0; t < R; ++t)
             if (t == (q >> 6)) v = x[t];
         return cplx<T>{__shfl(v.r, q & 63, 64), __shfl(v.i, q & 63, 64)};
     };
+    V col[TRINV_QC][R];
     if (LOWER) {
-        for (int q = c; q < bs - 1; ++q) {
-            const cplx<T> xq = bcast(q);
+        for (int q0 = c; q0 < bs - 1; q0 += TRINV_QC) {
 #pragma unroll
-            for (int t = 0; t < TRINV_R; ++t) {
-                const int r = lane + 64 * t;
-                if (r > q && r < bs) {
-                    V l = D[(long)r + (long)q * lda];
-                    x[t] = csub(x[t], cmul(cplx<T>{l.x, l.y}, xq));
+            for (int u = 0; u < TRINV_QC; ++u)
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    const int q = q0 + u, r = lane + 64 * t;
+                    col[u][t] = (q < bs - 1 && r > q && r < bs) ? D[(long)r + (long)q * lda] : V{0, 0};
                 }
+#pragma unroll
+            for (int u = 0; u < TRINV_QC; ++u) {
+                const int q = q0 + u;
+                if (q >= bs - 1) break;
+                const cplx<T> xq = bcast(q);
+#pragma unroll
+                for (int t = 0; t < R; ++t) x[t] = csub(x[t], cmul(cplx<T>{col[u][t].x, col[u][t].y}, xq));
             }
         }
     } else {
-        for (int q = c; q >= 0; --q) {
-            // x_q /= U(q, q) on the owning lane, then broadcast
-            V d = D[(long)q + (long)q * lda];
-            const cplx<T> dq = cinv(cplx<T>{d.x, d.y});
+        V dg[TRINV_QC];
+        for (int q0 = c; q0 >= 0; q0 -= TRINV_QC) {
 #pragma unroll
-            for (int t = 0; t < TRINV_R; ++t)
-                if (lane + 64 * t == q) x[t] = cmul(x[t], dq);
-            const cplx<T> xq = bcast(q);
+            for (int u = 0; u < TRINV_QC; ++u) {
+                const int q = q0 - u;
+                dg[u] = q >= 0 ? D[(long)q + (long)q * lda] : V{1, 0};
 #pragma unroll
-            for (int t = 0; t < TRINV_R; ++t) {
-                const int r = lane + 64 * t;
-                if (r < q) {
-                    V u = D[(long)r + (long)q * lda];
-                    x[t] = csub(x[t], cmul(cplx<T>{u.x, u.y}, xq));
+                for (int t = 0; t < R; ++t) {
+                    const int r = lane + 64 * t;
+                    col[u][t] = (q >= 0 && r < q) ? D[(long)r + (long)q * lda] : V{0, 0};
                 }
+            }
+#pragma unroll
+            for (int u = 0; u < TRINV_QC; ++u) {
+                const int q = q0 - u;
+                if (q < 0) break;
+                // x_q /= U(q, q) on the owning lane, then broadcast
+                const cplx<T> dq = cinv(cplx<T>{dg[u].x, dg[u].y});
+#pragma unroll
+                for (int t = 0; t < R; ++t)
+                    if (lane + 64 * t == q) x[t] = cmul(x[t], dq);
+                const cplx<T> xq = bcast(q);
+#pragma unroll
+                for (int t = 0; t < R; ++t) x[t] = csub(x[t], cmul(cplx<T>{col[u][t].x, col[u][t].y}, xq));
             }
         }
     }
 #pragma unroll
-    for (int t = 0; t < TRINV_R; ++t) {
+    for (int t = 0; t < R; ++t) {
         const int r = lane + 64 * t;
         if (r < bs) Inv[(long)r + (long)c * ldi] = V{x[t].r, x[t].i};
     }
@@ -255,8 +273,14 @@ __global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D
 template <typename T, bool LOWER>
 static int trinv(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T>::t *Inv, long ldi, hipStream_t st)
 {
-    MIDAS_CHECK(bs <= 64 * TRINV_R, -1, "trinv: block %d above %d", bs, 64 * TRINV_R);
-    hipLaunchKernelGGL((k_trinv_wave<T, LOWER>), dim3((bs + 3) / 4), dim3(256), 0, st, D, lda, bs, Inv, ldi);
+    MIDAS_CHECK(bs <= 512, -1, "trinv: block %d above 512", bs);
+    const dim3 g((bs + 3) / 4), b(256);
+    if (bs <= 128)
+        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 2>), g, b, 0, st, D, lda, bs, Inv, ldi);
+    else if (bs <= 256)
+        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 4>), g, b, 0, st, D, lda, bs, Inv, ldi);
+    else
+        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 8>), g, b, 0, st, D, lda, bs, Inv, ldi);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
@@ -366,10 +390,9 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
                A + (k + nb) + (long)(k + nb) * lda, lda, st);
         if (r) return r;
         MIDAS_HIP(hipEventRecord(w.ev_next, st));
-        MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
-        r = panel<T>(A, lda, n, k + nb, nb1, ipiv, w, ps);
-        if (r) return r;
-        MIDAS_HIP(hipEventRecord(w.ev_panel, ps));
+        // The rest of the update is enqueued before the panel: a panel is
+        // hundreds of small launches, and the host takes milliseconds to issue
+        // them; issued first, the big update runs while they are issued.
         if (nr > nb1) {
             r = u12_solve<T>(A, lda, k, nb, k + nb + nb1, n, w, st);
             if (r) return r;
@@ -377,6 +400,10 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
                    A + (k + nb) + (long)(k + nb + nb1) * lda, lda, st);
             if (r) return r;
         }
+        MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
+        r = panel<T>(A, lda, n, k + nb, nb1, ipiv, w, ps);
+        if (r) return r;
+        MIDAS_HIP(hipEventRecord(w.ev_panel, ps));
     }
     MIDAS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
     // inverses of the diagonal blocks of U, used by getrs (those of L come from the panels)
