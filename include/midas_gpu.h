@@ -168,7 +168,12 @@ int midas_aca_matrix(midas_ctx *ctx, int m, int n, const double *A, long lda, in
  * Outputs: E (3n x 2ntr, lde); K_out[b] = number of CBFs of block b
  * (K_patchs); C_job (optional, may be NULL): the CBFs, block b's in rows
  * 3*block_start[b] .. and columns 0..K_b-1 (C_job_patchs), ldc >= 3n and
- * kcap columns; the call fails if some K_b exceeds kcap (K_b <= 2*ntr_cbf). */
+ * kcap columns; the call fails if some K_b exceeds kcap (K_b <= 2*ntr_cbf).
+ * sr = 1 applies the reference's SR sparsification to every extended block
+ * before its solve (setfSR with res_sr percent, SR_Green_s_tr_partial;
+ * getParameters_CBFM.f90:113-168, Green_functions.f90:290-806); the masked
+ * block is solved by the dense LU instead of PARDISO.  fsr_out[b] and
+ * nnz_out[b] (optional) receive the chosen fSR and the kept entries. */
 typedef struct {
     int nblocks;
     const int *block_start;   /* nblocks + 1 */
@@ -180,7 +185,8 @@ int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const
                                int quirks, int ntr_cbf, const double *th_cbf, const double *ph_cbf, int ntr,
                                const double *tr_th, const double *tr_ph, int use_aca, int kmax, double eps_aca,
                                double threshold, int mode, double *E, long lde, int *K_out, double *C_job, long ldc,
-                               int kcap, midas_solve_info_t *info);
+                               int kcap, int sr, double res_sr, double *fsr_out, long *nnz_out,
+                               midas_solve_info_t *info);
 
 /* ---------------- device-pointer variants (inputs resident in HBM) ----------------
  * Same semantics as above; every pointer is device memory; ordered on

@@ -286,7 +286,7 @@ class Midas:
     # -- Compute_EFields_CBFME(_ACA) -----------------------------------------
     def compute_efields_cbfme(self, cells, k0, blocks, ext, cbf_dirs, transmitters, use_aca=False, kmax=50,
                               eps_aca=1e-4, threshold=0.0, mode=SOLVE_FP64, quirks=Q_ALL,
-                              return_cbfs=False):
+                              return_cbfs=False, sr=False, res_sr=0.01):
         cells = _cells(cells)
         nb = len(blocks)
         starts = np.zeros(nb + 1, np.int32)
@@ -311,6 +311,8 @@ class Midas:
         K = np.zeros(nb, np.int32)
         kcap = 2 * len(cth)
         Cj = np.zeros((3 * cells.n, kcap), np.complex128, order="F") if return_cbfs else None
+        fsr = np.zeros(nb, np.float64)
+        nnz = np.zeros(nb, np.int64)
         info = solve_info_t()
         cs = cells.struct()
         r = _check(self.lib.midas_compute_efields_cbfm(self.ctx, C.byref(cs), C.byref(bt), C.c_double(k0),
@@ -319,10 +321,13 @@ class Midas:
                                                        C.c_int(kmax), C.c_double(eps_aca), C.c_double(threshold),
                                                        C.c_int(mode), _p(E), C.c_long(3 * cells.n), _p(K),
                                                        None if Cj is None else _p(Cj), C.c_long(3 * cells.n),
-                                                       C.c_int(kcap), C.byref(info)),
+                                                       C.c_int(kcap), C.c_int(int(sr)), C.c_double(res_sr),
+                                                       _p(fsr), _p(nnz), C.byref(info)),
                    "compute_efields_cbfm")
         d = info.as_dict()
         d["info"] = r
+        if sr:
+            d["fsr"], d["nnz"] = fsr, nnz
         if return_cbfs:
             return E, K, d, Cj[:, :max(int(K.max()), 1)]
         return E, K, d

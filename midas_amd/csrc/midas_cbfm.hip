@@ -6,7 +6,8 @@
 //
 // Pipeline (all on the device, one context stream):
 //   1. per block b: cells = block cells then its extension cells (s_b of them);
-//      Z_b = Green_s_tr_partial(ext block, ext block) and the CBF excitations
+//      Z_b = Green_s_tr_partial(ext block, ext block) -- with sr, sparsified by
+//      the reference's SR mask and fSR search -- and the CBF excitations
 //      E_b = Incident_Field(ext block, CBF directions) (FP32 intermediates when
 //      the FP32-blocks quirk is set); X_b = Z_b^-1 E_b by the FP64 LU of this
 //      library.  The reference takes the SVD X_b = U S W^H (GESVD, JOB='U');
@@ -29,6 +30,7 @@
 //      (FP64 LU or mixed-precision LU + FP64 refinement, the `mode`), and
 //      E(block i) = C_i alpha_i.
 #include "midas_aca.h"
+#include "midas_elem.h"
 #include "midas_host.h"
 #include "midas_lu.h"
 
@@ -111,6 +113,109 @@ static int launch_pick_eigvecs(const double2 *W, long ldw, const double *lam, in
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// SR (sparsified) block solves: setfSR (getParameters_CBFM.f90:113-168) with
+// Green_s_tr_partial_FN and SR_Green_s_tr_partial_FN (Green_functions.f90:
+// 176-287, 564-806), and the mask of SR_Green_s_tr_partial (290-561).  The
+// reference hands the masked block to PARDISO; here the same masked matrix is
+// solved by the dense LU (a direct solve of the same linear system).
+// ---------------------------------------------------------------------------
+constexpr int NFSR = 11;
+struct FsrList { double f[NFSR]; };
+static const FsrList FSR_TESTS = {{1e2, 1e3, 2e3, 3e3, 4e3, 5e3, 6e3, 7e3, 8e3, 9e3, 1e4}};
+struct SrStats {
+    double fn2;                     // squared Frobenius norm of the block
+    double g2[NFSR];                // squared norm of the SR mask, per fSR
+    unsigned long long nnz[NFSR];   // entries kept, per fSR
+};
+
+// One thread per (receiver, source) pair of block b's extended cell list.
+__global__ __launch_bounds__(64) void k_sr_stats(DevCells cells, CellSet S, double k0, double pi, FsrList fl,
+                                                 SrStats *out)
+{
+    const int o = blockIdx.x * 64 + threadIdx.x;
+    const int sl = blockIdx.y;
+    double fn2 = 0.0, g2[NFSR];
+    unsigned nz[NFSR];
+#pragma unroll
+    for (int j = 0; j < NFSR; ++j) {
+        g2[j] = 0.0;
+        nz[j] = 0;
+    }
+    if (o < S.n) {
+        const int go = S.cell(o), gs = S.cell(sl), g0 = S.cell(0);
+        const double v0 = hypot(1.0 - cells.znnpp[2 * g0], -cells.znnpp[2 * g0 + 1]);
+        if (o == sl) {
+            const double a = hypot(1.0 - cells.znnpp[2 * gs], -cells.znnpp[2 * gs + 1]);
+            fn2 = 3.0 * (a * a);
+#pragma unroll
+            for (int j = 0; j < NFSR; ++j) {
+                g2[j] = 3.0 * (a * a);
+                nz[j] = 3;
+            }
+        } else {
+#pragma clang fp contract(off)
+            float2 F[3][3], f;
+            dyad_ref32_F(cells.x[go] - cells.x[gs], cells.y[go] - cells.y[gs], cells.z[go] - cells.z[gs], k0, pi,
+                         cells.kappa[gs], z64{cells.che[2 * gs], cells.che[2 * gs + 1]}, F, f);
+            const float axx = hypotf(F[0][0].x, F[0][0].y), axy = hypotf(F[0][1].x, F[0][1].y);
+            const float axz = hypotf(F[0][2].x, F[0][2].y), ayy = hypotf(F[1][1].x, F[1][1].y);
+            const float ayz = hypotf(F[1][2].x, F[1][2].y), azz = hypotf(F[2][2].x, F[2][2].y);
+            const float af = hypotf(f.x, f.y);
+            float t = axx * axx;
+            t = t + 2.0f * (axy * axy);
+            t = t + 2.0f * (axz * axz);
+            t = t + ayy * ayy;
+            t = t + 2.0f * (ayz * ayz);
+            t = t + azz * azz;
+            t = t * (af * af);
+            fn2 = (double)t;
+            for (int po = 0; po < 3; ++po)
+                for (int ps = 0; ps < 3; ++ps) {
+                    const z64 g = neg_fprod(F[po][ps], f);
+                    const float a = hypotf((float)g.r, (float)g.i);
+                    const float a2 = a * a;
+#pragma unroll
+                    for (int j = 0; j < NFSR; ++j)
+                        if ((double)a > v0 / fl.f[j]) {
+                            g2[j] += (double)a2;
+                            ++nz[j];
+                        }
+                }
+        }
+    }
+    // wave sums, then one atomic per value per wave
+    auto wsum = [](double v) {
+        for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+        return v;
+    };
+    fn2 = wsum(fn2);
+    if (threadIdx.x == 0) atomicAdd(&out->fn2, fn2);
+#pragma unroll
+    for (int j = 0; j < NFSR; ++j) {
+        double gs = wsum(g2[j]);
+        double ns = wsum((double)nz[j]);
+        if (threadIdx.x == 0) {
+            atomicAdd(&out->g2[j], gs);
+            atomicAdd(&out->nnz[j], (unsigned long long)ns);
+        }
+    }
+}
+
+// Zero the entries of a filled (m x m) block that the SR mask drops: off the
+// self pairs, |g| (single precision) <= |1 - Znnpp(first cell)| / fsr.
+__global__ void k_sr_mask(double2 *Z, long ld, int m, DevCells cells, CellSet S, double fsr)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= m || i / 3 == j / 3) return;
+    const int g0 = S.cell(0);
+    const double thr = hypot(1.0 - cells.znnpp[2 * g0], -cells.znnpp[2 * g0 + 1]) / fsr;
+    double2 *p = Z + (long)j * ld + i;
+    const double2 v = *p;
+    if (!((double)hypotf((float)v.x, (float)v.y) > thr)) *p = make_double2(0.0, 0.0);
+}
+
 }  // namespace midas
 
 extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks,
@@ -118,6 +223,7 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                                           const double *ph_cbf, int ntr, const double *tr_th, const double *tr_ph,
                                           int use_aca, int kmax, double eps_aca, double threshold, int mode,
                                           double *E, long lde, int *K_out, double *C_job, long ldc, int kcap,
+                                          int sr, double res_sr, double *fsr_out, long *nnz_out,
                                           midas_solve_info_t *info)
 {
     MIDAS_CHECK(ctx && cells && blocks && E && th_cbf && ph_cbf && tr_th && tr_ph, -1, "cbfm: bad arguments");
@@ -196,6 +302,35 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
     TRY(Xall.alloc(sizeof(double2) * xoff[nb]));
     auto Xb = [&](int b) { return Xall.as<double2>() + xoff[b]; };
     int worst_info = 0;
+    // SR: the fSR of every block from one pass of norms (setfSR)
+    std::vector<double> fsr(nb, 0.0);
+    if (sr) {
+        DBuf stats;
+        TRY(stats.alloc(sizeof(SrStats) * nb));
+        MIDAS_HIP(hipMemsetAsync(stats.p, 0, stats.bytes, st));
+        for (int b = 0; b < nb; ++b) {
+            CellSet S{sb[b], 0, didx.as<int>() + ioff[b]};
+            hipLaunchKernelGGL(k_sr_stats, dim3((sb[b] + 63) / 64, sb[b]), dim3(64), 0, st, dc.v, S, k0,
+                               host_pi(quirks), FSR_TESTS, stats.as<SrStats>() + b);
+        }
+        MIDAS_HIP(hipGetLastError());
+        std::vector<SrStats> hs(nb);
+        TRY(copy_sync(hs.data(), stats.p, sizeof(SrStats) * nb, hipMemcpyDeviceToHost, st));
+        for (int b = 0; b < nb; ++b) {
+            const double fn = std::sqrt(hs[b].fn2);
+            double res = 1e3;
+            int j = 0;
+            unsigned long long nz = 0;
+            while (j < NFSR && res > res_sr) {
+                fsr[b] = FSR_TESTS.f[j];
+                nz = hs[b].nnz[j];
+                res = std::fabs(fn - std::sqrt(hs[b].g2[j])) / fn * 100.0;
+                ++j;
+            }
+            if (fsr_out) fsr_out[b] = fsr[b];
+            if (nnz_out) nnz_out[b] = (long)nz;
+        }
+    }
     {
         DBuf Zb, Eref;
         TRY(Zb.alloc(sizeof(double2) * mmax * mmax));
@@ -204,6 +339,11 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
             const int m = 3 * sb[b];
             CellSet S{sb[b], 0, didx.as<int>() + ioff[b]};
             TRY(launch_fill(dc.v, S, S, k0, quirks, ref32, Zb.as<double2>(), m, nullptr, 0, st));
+            if (sr) {
+                hipLaunchKernelGGL(k_sr_mask, dim3((m + 255) / 256, m), dim3(256), 0, st, Zb.as<double2>(), (long)m,
+                                   m, dc.v, S, fsr[b]);
+                MIDAS_HIP(hipGetLastError());
+            }
             TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32,
                            Eref.as<double2>(), m, st));
             midas_solve_info_t bi{};
@@ -213,7 +353,7 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
             if (bi.info > 0 && worst_info == 0) worst_info = bi.info;
         }
     }
-    phase("block fills + LU solves");
+    phase(sr ? "SR masks + block fills + LU solves" : "block fills + LU solves");
     // 1b. left singular vectors of X_b with sigma_d / sigma_1 >= threshold.
     std::vector<int> K(nb);
     RocHandle rh;

@@ -52,8 +52,9 @@ __device__ inline z64 dyad_entry(const Dyad64 &d, int po, int ps)
 // ---------------------------------------------------------------------------
 __device__ inline float2 f2(double r, double i) { return make_float2((float)r, (float)i); }
 
-__device__ inline void dyad_ref32(double rx, double ry, double rz, double k0, double pi, double kappa, z64 che,
-                                  z64 out[9])
+// The single-precision dyad F (row p, column q) and f_kapChe of one pair.
+__device__ inline void dyad_ref32_F(double rx, double ry, double rz, double k0, double pi, double kappa, z64 che,
+                                    float2 F[3][3], float2 &f)
 {
 #pragma clang fp contract(off)
     double r = sqrt(rx * rx + ry * ry + rz * rz);
@@ -62,9 +63,8 @@ __device__ inline void dyad_ref32(double rx, double ry, double rz, double k0, do
     double den = 4.0 * pi * (r * r);
     float2 gr = f2(c / den, s / den);
     float2 tau = f2(-1.0 / r, k0);
-    float2 f = f2(kappa * che.r, kappa * che.i);
+    f = f2(kappa * che.r, kappa * che.i);
     double rv[3] = {rx, ry, rz};
-    float2 F[3][3];
     for (int p = 0; p < 3; ++p) {
         double rp = rv[p];
         double t1 = k0 * k0 * (r - rp * rp / r);
@@ -84,13 +84,24 @@ __device__ inline void dyad_ref32(double rx, double ry, double rz, double k0, do
             double br = -k0 * k0 - (double)tau3.x / r, bi = -(double)tau3.y / r;
             F[p][q] = F[q][p] = f2(gr_r * br - gr_i * bi, gr_r * bi + gr_i * br);
         }
+}
+
+// -F * f_kapChe as a complex(4) product, widened (Green_functions.f90:160-167)
+__device__ inline z64 neg_fprod(float2 a, float2 f)
+{
+#pragma clang fp contract(off)
+    float re = a.x * f.x - a.y * f.y;
+    float im = a.x * f.y + a.y * f.x;
+    return {-(double)re, -(double)im};
+}
+
+__device__ inline void dyad_ref32(double rx, double ry, double rz, double k0, double pi, double kappa, z64 che,
+                                  z64 out[9])
+{
+    float2 F[3][3], f;
+    dyad_ref32_F(rx, ry, rz, k0, pi, kappa, che, F, f);
     for (int ps = 0; ps < 3; ++ps)
-        for (int po = 0; po < 3; ++po) {
-            float2 a = F[po][ps];
-            float re = a.x * f.x - a.y * f.y;
-            float im = a.x * f.y + a.y * f.x;
-            out[3 * ps + po] = {-(double)re, -(double)im};
-        }
+        for (int po = 0; po < 3; ++po) out[3 * ps + po] = neg_fprod(F[po][ps], f);
 }
 
 }  // namespace midas

@@ -7,7 +7,8 @@
 ! output writers) is unchanged.
 !
 !   Compute_EFields_MoM        Compute_EFields_MoM.f90:1-19
-!   Compute_EFields_CBFME      Compute_EFields_CBFME.f90:1-21  (dense block solves, SR = 0)
+!   Compute_EFields_CBFME      Compute_EFields_CBFME.f90:1-21  (SR mask as the reference
+!                              builds it when SR = 1, solved by the dense LU instead of PARDISO)
 !   Compute_EFields_CBFME_ACA  Compute_EFields_CBFME_ACA.f90:1-18
 !   Calcul_MatrixZij_ACA       Calcul_ACA_matrices.f90:6-20
 !   Calcul_GenMatrix_ACA       Calcul_ACA_matrices.f90:222-236
@@ -187,7 +188,8 @@ SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,Transmitters,aca_
     r = midas_compute_efields_cbfm(midas_context, cs, bl, k_0, MIDAS_Q_ALL, int(size(tcbf), c_int), thc, phc, &
                                    int(NTr, c_int), tth, tph, int(aca_on, c_int), int(Nb_it_max, c_int), &
                                    Epsilon_ACA, threshold, midas_mode, c_loc(E_total), int(3 * Nbc, c_long), kk, &
-                                   c_loc(C_b), int(3 * Nbc, c_long), int(kcap, c_int), inf)
+                                   c_loc(C_b), int(3 * Nbc, c_long), int(kcap, c_int), int(SR, c_int), res_SR, &
+                                   c_null_ptr, c_null_ptr, inf)
     call midas_check(r, 'Compute_EFields_CBFME')
     if (r > 0 .and. rank == 0) write (*, '(a,i10)') ' SINGULAR MATRIX, INFO = ', r
     K_b = kk

@@ -62,16 +62,17 @@ module midas_gpu_c
 
         integer(c_int) function midas_compute_efields_cbfm(ctx, cells, blocks, k0, quirks, ntr_cbf, th_cbf, ph_cbf, &
                                                            ntr, tr_th, tr_ph, use_aca, kmax, eps_aca, threshold, &
-                                                           mode, e, lde, k_out, c_job, ldc, kcap, info) &
+                                                           mode, e, lde, k_out, c_job, ldc, kcap, sr, res_sr, &
+                                                           fsr_out, nnz_out, info) &
             bind(c, name="midas_compute_efields_cbfm")
             import :: c_int, c_long, c_ptr, c_double, midas_cells_t, midas_blocks_t, midas_solve_info_t
             type(c_ptr), value :: ctx
             type(midas_cells_t), intent(in) :: cells
             type(midas_blocks_t), intent(in) :: blocks
-            real(c_double), value :: k0, eps_aca, threshold
-            integer(c_int), value :: quirks, ntr_cbf, ntr, use_aca, kmax, mode, kcap
+            real(c_double), value :: k0, eps_aca, threshold, res_sr
+            integer(c_int), value :: quirks, ntr_cbf, ntr, use_aca, kmax, mode, kcap, sr
             real(c_double), intent(in) :: th_cbf(*), ph_cbf(*), tr_th(*), tr_ph(*)
-            type(c_ptr), value :: e, c_job
+            type(c_ptr), value :: e, c_job, fsr_out, nnz_out
             integer(c_long), value :: lde, ldc
             integer(c_int), intent(out) :: k_out(*)
             type(midas_solve_info_t), intent(out) :: info

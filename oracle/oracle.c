@@ -34,6 +34,10 @@
  *   or_aca                : reference Calcul_MatrixZij_ACA in oracle/_ref
  *                           (pivots, rank, U, V).
  *   or_aca_gen            : reference Calcul_GenMatrix_ACA in oracle/_ref.
+ *   or_block_fn           : reference Green_s_tr_partial_FN in oracle/_ref.
+ *   or_sr_block           : reference SR_Green_s_tr_partial (CSR) and
+ *                           SR_Green_s_tr_partial_FN in oracle/_ref; the fSR
+ *                           search built on them against setfSR.
  */
 #include <complex.h>
 #include <math.h>
@@ -161,16 +165,17 @@ static inline fc fmulc(fc a, fc b)
 static inline fc tofc(zc v) { return (float)creal(v) + I * (float)cimag(v); }
 static inline zc tozc(fc v) { return (double)crealf(v) + I * (double)cimagf(v); }
 
-/* The nine dyad entries for one cell pair, Green_functions.f90:139-167.
- * out[3*ps+po] is the entry (row component po, column component ps). */
-static void dyad_f32(double rx, double ry, double rz, double k0, double pi, double kappa, zc che, zc out[9])
+/* The single-precision dyad F (Fxx .. Fzz) and f_kapChe of one cell pair,
+ * Green_functions.f90:139-160 (the same expressions appear in the _FN and SR
+ * routines). */
+static void dyad_F32(double rx, double ry, double rz, double k0, double pi, double kappa, zc che, fc F[3][3],
+                     fc *fout)
 {
     double r = sqrt(rx * rx + ry * ry + rz * rz);
     fc gr = tofc(cexp(I * k0 * r) / (4.0 * pi * r * r));
     fc tau = tofc(I * k0 - 1.0 / r);
-    fc f = tofc(kappa * che);
+    *fout = tofc(kappa * che);
     double rv[3] = {rx, ry, rz};
-    fc F[3][3];
     for (int p = 0; p < 3; ++p) {
         double rp = rv[p];
         F[p][p] = tofc(tozc(gr) * (tozc(tau) + k0 * k0 * (r - rp * rp / r) - 3.0 * rp * rp / (r * r) * tozc(tau)));
@@ -182,6 +187,14 @@ static void dyad_f32(double rx, double ry, double rz, double k0, double pi, doub
             zc v = tozc(gr) / r * (rv[p] * rv[q]) * (-k0 * k0 - tozc(tau3) / r);
             F[p][q] = F[q][p] = tofc(v);
         }
+}
+
+/* The nine dyad entries for one cell pair, Green_functions.f90:139-167.
+ * out[3*ps+po] is the entry (row component po, column component ps). */
+static void dyad_f32(double rx, double ry, double rz, double k0, double pi, double kappa, zc che, zc out[9])
+{
+    fc F[3][3], f;
+    dyad_F32(rx, ry, rz, k0, pi, kappa, che, F, &f);
     for (int ps = 0; ps < 3; ++ps)
         for (int po = 0; po < 3; ++po) out[3 * ps + po] = tozc(-fmulc(F[po][ps], f));
 }
@@ -574,3 +587,85 @@ int or_aca_gen(int m, int n, const double *A, int kmax, double eps, double *U, d
     return aca_core(m, n, kmax, eps, &g, gen_row, gen_col, (zc *)U, (zc *)V, NULL, NULL);
 }
 
+
+/* ---------------------------------------------------------------------------
+ * SR (sparsified) block of the CBF solves.
+ * ------------------------------------------------------------------------- */
+
+/* Green_s_tr_partial_FN (Green_functions.f90:176-239, homogs = 0): Frobenius
+ * norm of the block over cells 0..n-1 (receivers = sources).  Each pair's term
+ * (|Fxx|^2 + 2|Fxy|^2 + 2|Fxz|^2 + |Fyy|^2 + 2|Fyz|^2 + |Fzz|^2) |f|^2 is
+ * single precision, accumulated in double; a self pair adds 3 |1 - Znnpp|^2. */
+double or_block_fn(int n, const int *id, const double *x, const double *y, const double *z, const double *kappa,
+                   const double *che, const double *znnpp, double k0, int quirk)
+{
+    const double pi = or_pi(quirk);
+    double val = 0.0;
+    for (int s = 0; s < n; ++s)
+        for (int o = 0; o < n; ++o) {
+            if (id[o] == id[s]) {
+                double a = cabs(1.0 - ld(znnpp, s));
+                val += 3.0 * (a * a);
+            } else {
+                fc F[3][3], f;
+                dyad_F32(x[o] - x[s], y[o] - y[s], z[o] - z[s], k0, pi, kappa[s], ld(che, s), F, &f);
+                volatile float axx = cabsf(F[0][0]), axy = cabsf(F[0][1]), axz = cabsf(F[0][2]);
+                volatile float ayy = cabsf(F[1][1]), ayz = cabsf(F[1][2]), azz = cabsf(F[2][2]), af = cabsf(f);
+                volatile float t = axx * axx;
+                t = t + 2.0f * (axy * axy);
+                t = t + 2.0f * (axz * axz);
+                t = t + ayy * ayy;
+                t = t + 2.0f * (ayz * ayz);
+                t = t + azz * azz;
+                t = t * (af * af);
+                val += (double)t;
+            }
+        }
+    return sqrt(val);
+}
+
+/* The SR mask of SR_Green_s_tr_partial(_FN) (Green_functions.f90:290-561,
+ * 564-806, homogs = 0): threshold = |1 - Znnpp(first cell)| / fsr; an
+ * off-cell entry g (single precision) is kept when |g| > threshold; the
+ * diagonal of a self pair is always kept, its cross-component zeros never.
+ * Returns nnz; *fn = sqrt(sum of |g|^2 over kept entries) (single-precision
+ * squares accumulated in double, rows in order); when Zout is given the kept
+ * entries are written into the dense (3n x 3n) block, zeros elsewhere. */
+long or_sr_block(int n, const double *x, const double *y, const double *z, const double *kappa, const double *che,
+                 const double *znnpp, double k0, int quirk, double fsr, double *fn, double *Zout)
+{
+    const double pi = or_pi(quirk);
+    const double thr = cabs(1.0 - ld(znnpp, 0)) / fsr;
+    const long ld3 = 3L * n;
+    if (Zout) memset(Zout, 0, sizeof(double) * 2 * ld3 * ld3);
+    long nnz = 0;
+    double g2 = 0.0;
+    for (int o = 0; o < n; ++o)
+        for (int po = 0; po < 3; ++po) {
+            const long row = 3L * o + po;
+            for (int s = 0; s < n; ++s) {
+                if (s == o) {
+                    zc d = 1.0 - ld(znnpp, o);
+                    double a = cabs(d);
+                    ++nnz;
+                    g2 += a * a;
+                    if (Zout) st(Zout, row + row * ld3, d);
+                    continue;
+                }
+                zc e[9];
+                dyad_f32(x[o] - x[s], y[o] - y[s], z[o] - z[s], k0, pi, kappa[s], ld(che, s), e);
+                for (int ps = 0; ps < 3; ++ps) {
+                    zc g = e[3 * ps + po];
+                    volatile float a = cabsf((float)creal(g) + I * (float)cimag(g));
+                    if ((double)a > thr) {
+                        ++nnz;
+                        volatile float a2 = a * a;
+                        g2 += (double)a2;
+                        if (Zout) st(Zout, row + (3L * s + ps) * ld3, g);
+                    }
+                }
+            }
+        }
+    if (fn) *fn = sqrt(g2);
+    return nnz;
+}

@@ -37,6 +37,8 @@ def lib():
         _LIB.or_pi.argtypes = [i]
         _LIB.or_aca.restype = i
         _LIB.or_aca_gen.restype = i
+        _LIB.or_block_fn.restype = d
+        _LIB.or_sr_block.restype = C.c_long
     return _LIB
 
 
@@ -53,7 +55,7 @@ def ref():
 
 
 def _p(a):
-    return a.ctypes.data_as(C.c_void_p)
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def _f64(a):
@@ -196,6 +198,62 @@ def aca_gen(A, kmax=50, eps=1e-4):
     return k, U[:, :kk], V[:kk, :]
 
 
+def block_fn(cells: Cells):
+    """Green_s_tr_partial_FN of the block over all of `cells` (receivers = sources)."""
+    return lib().or_block_fn(C.c_int(cells.n), _p(cells.id), _p(cells.x), _p(cells.y), _p(cells.z), _p(cells.kappa),
+                             _p(cells.che), _p(cells.znnpp), C.c_double(cells.k0), C.c_int(cells.quirk))
+
+
+def sr_block(cells: Cells, fsr, dense=True):
+    """SR mask of the block (SR_Green_s_tr_partial / _FN): (nnz, Frobenius norm, dense masked Z or None)."""
+    n3 = 3 * cells.n
+    Z = np.empty((n3, n3), np.complex128, order="F") if dense else None
+    fn = C.c_double(0)
+    nnz = lib().or_sr_block(C.c_int(cells.n), _p(cells.x), _p(cells.y), _p(cells.z), _p(cells.kappa), _p(cells.che),
+                            _p(cells.znnpp), C.c_double(cells.k0), C.c_int(cells.quirk), C.c_double(fsr), C.byref(fn),
+                            _p(Z))
+    return nnz, fn.value, Z
+
+
+FSR_TESTS = (1e2, 1e3, 2e3, 3e3, 4e3, 5e3, 6e3, 7e3, 8e3, 9e3, 1e4)
+
+
+def setfsr(cells: Cells, res_sr=0.01):
+    """setfSR (getParameters_CBFM.f90:113-168): the first fSR of the list whose
+    mask loses at most res_sr percent of the block's Frobenius norm (else the
+    last).  Returns (fsr, nnz)."""
+    fn = block_fn(cells)
+    res, j, f, nnz = 1e3, 0, FSR_TESTS[0], 0
+    while j < len(FSR_TESTS) and res > res_sr:
+        f = FSR_TESTS[j]
+        nnz, fns, _ = sr_block(cells, f, dense=False)
+        res = abs(fn - fns) / fn * 100.0
+        j += 1
+    return f, nnz
+
+
+def transmitters_un(niws):
+    """getTransmitters_CBFM (getParameters_CBFM.f90:1-110) with distr_iws = 1:
+    the grid uniform in cos(theta) and phi closest to niws directions
+    (theta from +x, degrees).  The grid-size expression is single precision in
+    the reference (integer and default-real operands)."""
+    f32 = np.float32
+    delta = float(f32(f32(540.0) ** f32(2.0)) + f32(4 * (niws - 1) * (360 * 180)))
+    step = (540 + np.sqrt(delta)) / (2 * (niws - 1))
+    nth = int(np.rint(180.0 / step)) + 1
+    nph = int(np.rint(360.0 / step)) + 1
+    pq = pi(1)
+    step_ph = 360.0 / (nph - 1)
+    c0 = np.cos(0.0 * pq / 180.0)
+    step_th = (np.cos(180.0 * pq / 180.0) - c0) / (nth - 1)
+    th, ph = [], []
+    for i in range(nph):
+        for k in range(nth):
+            ph.append(i * step_ph)
+            th.append(np.arccos(1.0 * k * step_th + c0) * 180.0 / pq)
+    return np.array(th), np.array(ph)
+
+
 # --------------------------------------------------------------------------
 # Dense solve and CBFM pipeline (numpy restatement)
 # --------------------------------------------------------------------------
@@ -215,7 +273,7 @@ def cbf_threshold(eps_cells, aca_variant):
 
 
 def cbfm(cells: Cells, blocks, ext, th_cbf, ph_cbf, th, ph, use_aca=False, kmax=50, eps_aca=1e-4,
-         threshold=None, fp32_blocks=True):
+         threshold=None, fp32_blocks=True, sr=False, res_sr=0.01, fsr_out=None):
     """Dense-block CBFM-E (Compute_EFields_CBFME.f90:348-449, 630-849, 896-1006)
     or its ACA variant (Compute_EFields_CBFME_ACA.f90:296-385, 569-636).
 
@@ -228,7 +286,13 @@ def cbfm(cells: Cells, blocks, ext, th_cbf, ph_cbf, th, ph, use_aca=False, kmax=
     for b, e in zip(blocks, ext):
         idx = np.concatenate([b, e]).astype(np.int64)
         cb = cells.subset(idx)
-        Zb = block_fill(cb, cb, fp32=fp32_blocks)
+        if sr:   # SR mask (Compute_EFields_CBFME.f90:398-409), solved densely
+            f, _ = setfsr(cb, res_sr)
+            _, _, Zb = sr_block(cb, f)
+            if fsr_out is not None:
+                fsr_out.append(f)
+        else:
+            Zb = block_fill(cb, cb, fp32=fp32_blocks)
         Eref = incident_field_f32(cb, th_cbf, ph_cbf)
         Eb = np.linalg.solve(Zb, Eref)
         U, s, _ = np.linalg.svd(Eb, full_matrices=False)
@@ -359,3 +423,50 @@ def ref_aca_gen(A, kmax=50, eps=1e-4):
     r.ref_aca_gen(C.c_int(m), C.c_int(n), _p(A), C.c_int(kmax), C.byref(k), _p(U), _p(V))
     kk = min(k.value, kmax)
     return k.value, U[:, :kk], V[:kk, :]
+
+
+def _ref_block_args(c: Cells):
+    return (C.c_int(c.n), _p(c.id), _p(c.x), _p(c.y), _p(c.z), _p(c.kappa), _p(c.che), _p(c.znnpp))
+
+
+def ref_block_fn(c: Cells):
+    r = ref()
+    _ref_globals(r, c.k0, nbc=c.n)
+    fn = C.c_double(0)
+    r.ref_block_fn(*_ref_block_args(c), C.byref(fn))
+    return fn.value
+
+
+def ref_sr_fn(c: Cells, fsr):
+    r = ref()
+    _ref_globals(r, c.k0, nbc=c.n)
+    fn, nnz = C.c_double(0), C.c_int(0)
+    r.ref_sr_fn(*_ref_block_args(c), C.c_double(fsr), C.byref(nnz), C.byref(fn))
+    return nnz.value, fn.value
+
+
+def ref_sr_csr(c: Cells, fsr):
+    """SR_Green_s_tr_partial as (val, irow, icol), 1-based CSR as the reference builds it."""
+    nnz, _ = ref_sr_fn(c, fsr)
+    r = ref()
+    val = np.zeros(nnz, np.complex128)
+    irow = np.zeros(3 * c.n + 1, np.int32)
+    icol = np.zeros(nnz, np.int32)
+    r.ref_sr_csr(*_ref_block_args(c), C.c_double(fsr), C.c_int(nnz), _p(val), _p(irow), _p(icol))
+    return val, irow, icol
+
+
+def ref_setfsr(c: Cells, res_sr=0.01):
+    r = ref()
+    _ref_globals(r, c.k0, nbc=c.n)
+    f, nnz = C.c_double(0), C.c_int(0)
+    r.ref_setfsr(*_ref_block_args(c), C.c_double(res_sr), C.byref(f), C.byref(nnz))
+    return f.value, nnz.value
+
+
+def ref_transmitters_un(niws, nmax=100000):
+    r = ref()
+    th, ph = np.zeros(nmax), np.zeros(nmax)
+    n = C.c_int(0)
+    r.ref_transmitters_un(C.c_int(niws), C.c_int(nmax), _p(th), _p(ph), C.byref(n))
+    return th[:n.value].copy(), ph[:n.value].copy()

@@ -3,7 +3,8 @@
 ! TEST INFRASTRUCTURE ONLY.  It is compiled by oracle/Makefile.ref together
 ! with the unmodified reference sources (Initialization.f90, Common_variables.f90,
 ! SetCellsParams.f90, Green_functions.f90, Incident_Field.f90,
-! Calcul_ACA_matrices.f90, read in place under the reference tree) into
+! Calcul_ACA_matrices.f90, getParameters_CBFM.f90, read in place under the
+! reference tree) into
 ! oracle/_ref/libmidas_ref.so, which tests use to pin the oracle.  Nothing in
 ! the product loads it.
 !
@@ -225,5 +226,93 @@ contains
         call Calcul_GenMatrix_ACA(m, n, A, nmax, k, U, V)
         nout = k
     end subroutine ref_aca_gen
+
+    ! Green_s_tr_partial_FN (Green_functions.f90:176-287): Frobenius norm of a block
+    subroutine ref_block_fn(n, id, x, y, z, kappa, che, znnpp, fn) bind(c, name="ref_block_fn")
+        integer(c_int), value :: n
+        integer(c_int), intent(in) :: id(n)
+        real(c_double), intent(in) :: x(n), y(n), z(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n), znnpp(n)
+        real(c_double), intent(out) :: fn
+        type(Cell), allocatable :: cc(:)
+        allocate(cc(n))
+        call build_cells(n, id, x, y, z, kappa, kappa, che, znnpp, cc)
+        call Green_s_tr_partial_FN(n, cc, n, cc, fn)
+    end subroutine ref_block_fn
+
+    ! SR_Green_s_tr_partial_FN (Green_functions.f90:564-806): nnz and norm of the SR mask
+    subroutine ref_sr_fn(n, id, x, y, z, kappa, che, znnpp, fsr, nnz, fn) bind(c, name="ref_sr_fn")
+        integer(c_int), value :: n
+        integer(c_int), intent(in) :: id(n)
+        real(c_double), intent(in) :: x(n), y(n), z(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n), znnpp(n)
+        real(c_double), value :: fsr
+        integer(c_int), intent(out) :: nnz
+        real(c_double), intent(out) :: fn
+        type(Cell), allocatable :: cc(:)
+        integer :: k
+        allocate(cc(n))
+        call build_cells(n, id, x, y, z, kappa, kappa, che, znnpp, cc)
+        call SR_Green_s_tr_partial_FN(n, cc, fsr, k, fn)
+        nnz = k
+    end subroutine ref_sr_fn
+
+    ! SR_Green_s_tr_partial (Green_functions.f90:290-561): the CSR matrix (1-based irow, icol)
+    subroutine ref_sr_csr(n, id, x, y, z, kappa, che, znnpp, fsr, nnz, val, irow, icol) bind(c, name="ref_sr_csr")
+        integer(c_int), value :: n, nnz
+        integer(c_int), intent(in) :: id(n)
+        real(c_double), intent(in) :: x(n), y(n), z(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n), znnpp(n)
+        real(c_double), value :: fsr
+        complex(c_double_complex), intent(out) :: val(nnz)
+        integer(c_int), intent(out) :: irow(3 * n + 1), icol(nnz)
+        type(Cell), allocatable :: cc(:)
+        allocate(cc(n))
+        call build_cells(n, id, x, y, z, kappa, kappa, che, znnpp, cc)
+        irow = 0
+        call SR_Green_s_tr_partial(n, cc, fsr, nnz, val, irow, icol)
+    end subroutine ref_sr_csr
+
+    ! setfSR (getParameters_CBFM.f90:113-168): the fSR search of one block
+    subroutine ref_setfsr(n, id, x, y, z, kappa, che, znnpp, res_sr_in, fsr, nnz) bind(c, name="ref_setfsr")
+        integer(c_int), value :: n
+        integer(c_int), intent(in) :: id(n)
+        real(c_double), intent(in) :: x(n), y(n), z(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n), znnpp(n)
+        real(c_double), value :: res_sr_in
+        real(c_double), intent(out) :: fsr
+        integer(c_int), intent(out) :: nnz
+        type(Cell), allocatable :: cc(:)
+        real(c_double) :: f
+        integer :: k
+        allocate(cc(n))
+        call build_cells(n, id, x, y, z, kappa, kappa, che, znnpp, cc)
+        res_SR = res_sr_in
+        call setfSR(1, n, cc, 0, f, k)
+        fsr = f
+        nnz = k
+    end subroutine ref_setfsr
+
+    ! getTransmitters_CBFM (getParameters_CBFM.f90:1-110), uniform cos(theta)/phi grid
+    subroutine ref_transmitters_un(niws_in, nmax, th, ph, ntr_out) bind(c, name="ref_transmitters_un")
+        integer(c_int), value :: niws_in, nmax
+        real(c_double), intent(out) :: th(nmax), ph(nmax)
+        integer(c_int), intent(out) :: ntr_out
+        interface
+            subroutine getTransmitters_CBFM(Transmitters_CBFM)
+                use Initialization
+                type(Dipole), dimension(:), allocatable, intent(out) :: Transmitters_CBFM
+            end subroutine getTransmitters_CBFM
+        end interface
+        type(Dipole), allocatable :: t(:)
+        integer :: m
+        distr_iws = 1
+        Niws = niws_in
+        call getTransmitters_CBFM(t)
+        ntr_out = size(t)
+        m = min(nmax, size(t))
+        th(1:m) = t(1:m)%theta
+        ph(1:m) = t(1:m)%phi
+    end subroutine ref_transmitters_un
 
 end module midas_ref_driver

@@ -13,6 +13,10 @@ Routines captured (reference file:line):
   GetFFieldCoeff            Green_functions.f90:1399-1438
   Calcul_MatrixZij_ACA      Calcul_ACA_matrices.f90:6-219 (converged and k = kmax)
   Calcul_GenMatrix_ACA      Calcul_ACA_matrices.f90:222-424
+  Green_s_tr_partial_FN     Green_functions.f90:176-287
+  SR_Green_s_tr_partial(_FN) Green_functions.f90:290-806 (CSR mask, norm)
+  setfSR                    getParameters_CBFM.f90:113-168
+  getTransmitters_CBFM      getParameters_CBFM.f90:1-110 (uniform grid)
 
 The fixture holds data only (inputs and expected outputs).
 """
@@ -106,6 +110,25 @@ def main():
     A += 1e-7 * (rng.normal(size=(m, n)) + 1j * rng.normal(size=(m, n)))
     k, U, V = o.ref_aca_gen(A, kmax=20, eps=1e-4)
     d.update(gen_A=A, gen_k=np.int64(k), gen_U=U, gen_V=V)
+
+    # --- SR mask of an extended block (two materials) and the fSR search -----
+    sr_idx = np.concatenate([np.arange(0, 48), np.arange(88, 100)])
+    cb = rc2.subset(sr_idx)
+    d["sr_idx"] = sr_idx
+    d["sr_fn"] = np.float64(o.ref_block_fn(cb))
+    d["sr_fsr_list"] = np.array([1e2, 1e3, 5e3, 1e4])
+    d["sr_nnz"] = np.array([o.ref_sr_fn(cb, f)[0] for f in d["sr_fsr_list"]], np.int64)
+    d["sr_fns"] = np.array([o.ref_sr_fn(cb, f)[1] for f in d["sr_fsr_list"]])
+    val, irow, icol = o.ref_sr_csr(cb, 1e3)
+    d["sr_csr_val"], d["sr_csr_irow"], d["sr_csr_icol"] = val, irow, icol
+    for res in (0.01, 0.001):
+        f, nnz = o.ref_setfsr(cb, res)
+        d[f"sr_setfsr_{res}"] = np.array([f, nnz])
+
+    # --- CBF excitation grid (uniform in cos theta and phi) -------------------
+    for ni in (94, 422):
+        th, ph = o.ref_transmitters_un(ni)
+        d[f"un_{ni}_th"], d[f"un_{ni}_ph"] = th, ph
 
     np.savez_compressed(OUT, **d)
     print("wrote", OUT, os.path.getsize(OUT), "bytes;", {k_: v.shape for k_, v in d.items() if np.ndim(v) > 1})

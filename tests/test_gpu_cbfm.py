@@ -153,3 +153,26 @@ def test_cbfm_without_truncation_is_mom(mid, oracle):
     Eo = np.linalg.solve(oracle.mom_fill(oc), oracle.mom_rhs(oc, tth, tph))
     print("err vs MoM", _rel(E, Eo))
     assert _rel(E, Eo) < 1e-10
+
+
+@pytest.mark.parametrize("use_aca", [False, True])
+def test_cbfm_sr_vs_oracle(mid, oracle, dlca, use_aca):
+    """SR-sparsified block solves (the reference's default CBFM-E): same fSR,
+    nnz and K per block as the oracle, same field."""
+    s = dlca
+    _case(f"cbfm SR use_aca={use_aca} n={3 * s.n}")
+    oc = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    th_c, ph_c = oracle.transmitters_un(94)
+    tth, tph = np.array([0.0, 35.0]), np.array([0.0, 80.0])
+    E, K, info = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=use_aca,
+                                           sr=True, res_sr=0.01)
+    fs = []
+    Eo, Ko, _ = oracle.cbfm(oc, s.blocks, s.ext, th_c, ph_c, tth, tph, use_aca=use_aca, sr=True, res_sr=0.01,
+                            fsr_out=fs)
+    nnz_o = [oracle.sr_block(oc.subset(np.concatenate([b, e])), f, dense=False)[0]
+             for b, e, f in zip(s.blocks, s.ext, fs)]
+    print("fSR", list(info["fsr"]), fs, "nnz", list(info["nnz"]), nnz_o, "K", list(K), Ko, "E err", _rel(E, Eo))
+    assert list(info["fsr"]) == fs
+    assert list(info["nnz"]) == nnz_o
+    assert list(K) == list(Ko)
+    assert _rel(E, Eo) < 1e-9

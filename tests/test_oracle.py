@@ -275,3 +275,85 @@ def test_cbf_threshold_rule(oracle):
     assert oracle.cbf_threshold(np.array([2.0 + 0j]), False) == pytest.approx(1e-3)
     assert oracle.cbf_threshold(np.array([2.0 + 0j, 10.24 + 0j]), False) == pytest.approx(1e-5)
     assert oracle.cbf_threshold(np.array([1e12 + 0j]), False) == float(np.float32(1e-10))
+
+
+# ----------------------------------------------------------------------------
+# SR sparsification of the CBF block solves and the CBF excitation grid
+# ----------------------------------------------------------------------------
+
+def _sr_cells(oracle, gold):
+    return _cells(oracle, gold, "g2").subset(gold["sr_idx"])
+
+
+def test_sr_norms_golden(oracle, gold):
+    """Green_s_tr_partial_FN and SR_Green_s_tr_partial_FN: nnz exact, norms to 1e-14."""
+    cb = _sr_cells(oracle, gold)
+    assert oracle.block_fn(cb) == pytest.approx(float(gold["sr_fn"]), rel=1e-14)
+    for f, nnz, fns in zip(gold["sr_fsr_list"], gold["sr_nnz"], gold["sr_fns"]):
+        n2, fn2, _ = oracle.sr_block(cb, float(f), dense=False)
+        assert n2 == int(nnz)
+        assert fn2 == pytest.approx(float(fns), rel=1e-14)
+
+
+def test_sr_mask_golden(oracle, gold):
+    """SR_Green_s_tr_partial: the dense form of the oracle's mask equals the reference CSR."""
+    cb = _sr_cells(oracle, gold)
+    val, irow, icol = gold["sr_csr_val"], gold["sr_csr_irow"], gold["sr_csr_icol"]
+    n3 = 3 * cb.n
+    D = np.zeros((n3, n3), complex)
+    for r in range(n3):
+        for q in range(irow[r] - 1, irow[r + 1] - 1):
+            D[r, icol[q] - 1] = val[q]
+    _, _, Z = oracle.sr_block(cb, 1e3)
+    assert np.array_equal(D != 0, Z != 0)
+    off = ~np.kron(np.eye(cb.n, dtype=bool), np.ones((3, 3), bool))
+    assert np.array_equal(D[off], Z[off])                 # single-precision entries: exact
+    assert np.abs(D - Z).max() < 1e-14                    # self terms (Znnpp to the last bit)
+
+
+def test_setfsr_golden(oracle, gold):
+    cb = _sr_cells(oracle, gold)
+    for res in (0.01, 0.001):
+        f, nnz = oracle.setfsr(cb, res)
+        assert (f, nnz) == (float(gold[f"sr_setfsr_{res}"][0]), int(gold[f"sr_setfsr_{res}"][1]))
+
+
+def test_sr_mask_properties(oracle, gold):
+    """The mask only drops entries: larger fSR keeps a superset; the kept part
+    of the full block is what it keeps."""
+    cb = _sr_cells(oracle, gold)
+    full = oracle.block_fill(cb, cb, fp32=True)
+    prev = None
+    for f in (1e2, 1e3, 1e4):
+        nnz, _, Z = oracle.sr_block(cb, f)
+        keep = Z != 0
+        assert np.array_equal(Z[keep], full[keep])
+        assert nnz == int(keep.sum())
+        if prev is not None:
+            assert np.all(keep[prev])
+        prev = keep
+
+
+def test_transmitters_un_golden(oracle, gold):
+    """getTransmitters_CBFM, uniform grid: same count and directions."""
+    for ni in (94, 422):
+        th, ph = oracle.transmitters_un(ni)
+        assert len(th) == len(gold[f"un_{ni}_th"])
+        assert np.abs(th - gold[f"un_{ni}_th"]).max() < 1e-12
+        assert np.abs(ph - gold[f"un_{ni}_ph"]).max() < 1e-12
+
+
+def test_cbfm_sr_close_to_dense(oracle):
+    """res_sr = 0 walks the whole fSR list (fSR = 1e4, the finest mask); the
+    SR solution then stays within the mask's norm loss of the dense one."""
+    from midas_amd import geometry as g
+    s = g.dlca_aggregate(nmono=2, cells_per_mono=25, seed=6)
+    c = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    th_c, ph_c = g.directions(30)
+    th, ph = np.array([0.0]), np.array([0.0])
+    Ed, Kd, _ = oracle.cbfm(c, s.blocks, s.ext, th_c, ph_c, th, ph)
+    fs = []
+    Es, Ks, _ = oracle.cbfm(c, s.blocks, s.ext, th_c, ph_c, th, ph, sr=True, res_sr=0.0, fsr_out=fs)
+    assert fs == [1e4] * len(s.blocks)
+    nnz, _, _ = oracle.sr_block(c.subset(np.concatenate([s.blocks[0], s.ext[0]])), 1e4, dense=False)
+    assert _rel(Es, Ed) < 1e-3        # the 1e4 mask still drops the weakest couplings
