@@ -202,6 +202,13 @@ __global__ __launch_bounds__(64) void k_sr_stats(DevCells cells, CellSet S, doub
     }
 }
 
+// Unit diagonal on the padding rows/columns [m, mp) of an mp x mp matrix.
+__global__ void k_unit_diag(double2 *Z, long ld, int m, int mp)
+{
+    const int i = m + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < mp) Z[(long)i * ld + i] = make_double2(1.0, 0.0);
+}
+
 // Zero the entries of a filled (m x m) block that the SR mask drops: off the
 // self pairs, |g| (single precision) <= |1 - Znnpp(first cell)| / fsr.
 __global__ void k_sr_mask(double2 *Z, long ld, int m, DevCells cells, CellSet S, double fsr)
@@ -289,18 +296,19 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
     TRY(upload(didx, allidx.data(), sizeof(int) * allidx.size(), st));
 
     // ---- 1. characteristic basis functions
-    // 1a. block solutions X_b = Z_b^-1 E_b (FP64 LU), kept side by side (ld 3 s_b)
+    // 1a. block solutions X_b = Z_b^-1 E_b (FP64 LU).  All blocks are solved
+    // as one batch: each Z_b is embedded in an mp x mp matrix (mp = 3 max s_b)
+    // with a unit diagonal on the padding and E_b padded with zero rows, so the
+    // padded solution rows are zero and every LU kernel launch serves the whole
+    // batch.  X_b is kept at stride mp * 2T_c, leading dimension mp.
     const long mmax = 3L * smax;
-    std::vector<long> xoff(nb + 1, 0);
+    const long mp = mmax;
     int rpad = 1;
-    for (int b = 0; b < nb; ++b) {
-        xoff[b + 1] = xoff[b] + 3L * sb[b] * T2c;
-        rpad = std::max(rpad, std::min(3 * sb[b], T2c));
-    }
+    for (int b = 0; b < nb; ++b) rpad = std::max(rpad, std::min(3 * sb[b], T2c));
     DBuf Call, Xall;
     TRY(Call.alloc(sizeof(double2) * std::max<long>(coff[nb], 1)));
-    TRY(Xall.alloc(sizeof(double2) * xoff[nb]));
-    auto Xb = [&](int b) { return Xall.as<double2>() + xoff[b]; };
+    TRY(Xall.alloc(sizeof(double2) * mp * T2c * nb));
+    auto Xb = [&](int b) { return Xall.as<double2>() + (long)b * mp * T2c; };
     int worst_info = 0;
     // SR: the fSR of every block from one pass of norms (setfSR)
     std::vector<double> fsr(nb, 0.0);
@@ -332,25 +340,49 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
         }
     }
     {
-        DBuf Zb, Eref;
-        TRY(Zb.alloc(sizeof(double2) * mmax * mmax));
-        TRY(Eref.alloc(sizeof(double2) * mmax * T2c));
-        for (int b = 0; b < nb; ++b) {
-            const int m = 3 * sb[b];
-            CellSet S{sb[b], 0, didx.as<int>() + ioff[b]};
-            TRY(launch_fill(dc.v, S, S, k0, quirks, ref32, Zb.as<double2>(), m, nullptr, 0, st));
-            if (sr) {
-                hipLaunchKernelGGL(k_sr_mask, dim3((m + 255) / 256, m), dim3(256), 0, st, Zb.as<double2>(), (long)m,
-                                   m, dc.v, S, fsr[b]);
+        const long zbytes = (long)sizeof(double2) * mp * mp;
+        const int chunk = (int)std::max(1L, std::min((long)nb, (long)(24e9 / (double)zbytes)));
+        const int nb_lu = 128;
+        DBuf Zall, ipiv, perm, tmp;
+        TRY(Zall.alloc(zbytes * chunk));
+        TRY(ipiv.alloc(sizeof(int) * mp * chunk));
+        TRY(perm.alloc(sizeof(int) * mp * chunk));
+        TRY(tmp.alloc(sizeof(double2) * std::max<long>(mp, nb_lu) * T2c * chunk));
+        std::vector<int> hinfo(chunk);
+        for (int b0 = 0; b0 < nb; b0 += chunk) {
+            const int nc = std::min(chunk, nb - b0);
+            MIDAS_HIP(hipMemsetAsync(Zall.p, 0, zbytes * nc, st));
+            MIDAS_HIP(hipMemsetAsync(Xb(b0), 0, sizeof(double2) * mp * T2c * nc, st));
+            for (int q = 0; q < nc; ++q) {
+                const int b = b0 + q, m = 3 * sb[b];
+                double2 *Zq = Zall.as<double2>() + (long)q * mp * mp;
+                CellSet S{sb[b], 0, didx.as<int>() + ioff[b]};
+                TRY(launch_fill(dc.v, S, S, k0, quirks, ref32, Zq, mp, nullptr, 0, st));
+                if (sr)
+                    hipLaunchKernelGGL(k_sr_mask, dim3((m + 255) / 256, m), dim3(256), 0, st, Zq, mp, m, dc.v, S,
+                                       fsr[b]);
+                if (m < mp)
+                    hipLaunchKernelGGL(k_unit_diag, dim3((int)((mp - m + 255) / 256)), dim3(256), 0, st, Zq, mp, m,
+                                       (int)mp);
                 MIDAS_HIP(hipGetLastError());
+                TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32, Xb(b), mp,
+                               st));
             }
-            TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32,
-                           Eref.as<double2>(), m, st));
-            midas_solve_info_t bi{};
-            int rr = solve_dev(ctx, m, T2c, Zb.as<double2>(), m, nullptr, 0, Eref.as<double2>(), m, Xb(b), m,
-                               MIDAS_SOLVE_FP64, nullptr, &bi, true);
-            if (rr < 0) return rr;
-            if (bi.info > 0 && worst_info == 0) worst_info = bi.info;
+            LuWork<double> w;
+            TRY(w.alloc((int)mp, nb_lu, nc, mp * mp));
+            TrailingGemm<double> tg;
+            tg.batch = nc;
+            tg.stride = mp * mp;
+            int r = getrf<double>(Zall.as<double2>(), mp, (int)mp, ipiv.as<int>(), w, st, tg);
+            if (!r) r = ipiv_to_perm_dev(ipiv.as<int>(), perm.as<int>(), (int)mp, nc, st);
+            if (!r)
+                r = getrs<double>(Zall.as<double2>(), mp, (int)mp, perm.as<int>(), w, Xb(b0), mp, T2c,
+                                  tmp.as<double2>(), st, mp * T2c);
+            if (!r) r = copy_sync(hinfo.data(), w.info, sizeof(int) * nc, hipMemcpyDeviceToHost, st);
+            w.release();
+            if (r) return r;
+            for (int q = 0; q < nc; ++q)
+                if (hinfo[q] > 0 && worst_info == 0) worst_info = hinfo[q];
         }
     }
     phase(sr ? "SR masks + block fills + LU solves" : "block fills + LU solves");
@@ -375,9 +407,9 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
             const int m = 3 * sb[b];
             double2 *Gb = G.as<double2>() + b * gs;
             if (T2c <= m)
-                TRY(launch_cgemm<double>(T2c, T2c, m, ONE, Xb(b), m, 2, Xb(b), m, 0, ZERO, Gb, rpad, st));
+                TRY(launch_cgemm<double>(T2c, T2c, m, ONE, Xb(b), mp, 2, Xb(b), mp, 0, ZERO, Gb, rpad, st));
             else
-                TRY(launch_cgemm<double>(m, m, T2c, ONE, Xb(b), m, 0, Xb(b), m, 2, ZERO, Gb, rpad, st));
+                TRY(launch_cgemm<double>(m, m, T2c, ONE, Xb(b), mp, 0, Xb(b), mp, 2, ZERO, Gb, rpad, st));
         }
         MIDAS_ROC(rocsolver_zheevd_strided_batched(rh.h, rocblas_evect_original, rocblas_fill_upper, rpad,
                                                    (rocblas_double_complex *)G.p, rpad, gs, D.as<double>(), rpad,
@@ -400,7 +432,7 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
             if (T2c <= m) {
                 TRY(launch_pick_eigvecs(Wb, rpad, D.as<double>() + (size_t)b * rpad, rpad, T2c, kb, true,
                                         Wk.as<double2>(), T2c, st));
-                TRY(launch_cgemm<double>(3 * nbk[b], kb, T2c, ONE, Xb(b), m, 0, Wk.p, T2c, 0, ZERO,
+                TRY(launch_cgemm<double>(3 * nbk[b], kb, T2c, ONE, Xb(b), mp, 0, Wk.p, T2c, 0, ZERO,
                                          Call.as<double2>() + coff[b], 3L * nbk[b], st));
             } else {
                 TRY(launch_pick_eigvecs(Wb, rpad, D.as<double>() + (size_t)b * rpad, rpad, 3 * nbk[b], kb, false,
@@ -419,7 +451,7 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
         for (int b = 0; b < nb; ++b) {
             const int m = 3 * sb[b], r = std::min(m, T2c);
             MIDAS_ROC(rocsolver_zgesvd(rh.h, rocblas_svect_singular, rocblas_svect_none, m, T2c,
-                                       (rocblas_double_complex *)Xb(b), m, Sv.as<double>(),
+                                       (rocblas_double_complex *)Xb(b), (int)mp, Sv.as<double>(),
                                        (rocblas_double_complex *)Usv.p, m, nullptr, 1, Esup.as<double>(),
                                        rocblas_outofplace, dinfo.as<int>()));
             int sinfo = 0;

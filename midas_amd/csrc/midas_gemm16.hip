@@ -406,8 +406,9 @@ template <>
 int TrailingGemm<double>::operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C,
                                      long ldc, hipStream_t st) const
 {
-    ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 16.0 * (2.0 * m * n + (double)m * k + (double)k * n));
-    return launch_cgemm<double>(m, n, k, z64{-1.0, 0.0}, A, lda, 0, B, ldb, 0, z64{1.0, 0.0}, C, ldc, st);
+    ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k * batch, 16.0 * (2.0 * m * n + (double)m * k + (double)k * n) * batch);
+    return launch_cgemm<double>(m, n, k, z64{-1.0, 0.0}, A, lda, 0, B, ldb, 0, z64{1.0, 0.0}, C, ldc, st, batch,
+                                stride, stride, stride);
 }
 
 template <>
@@ -415,9 +416,11 @@ int TrailingGemm<float>::operator()(int m, int n, int k, const void *A, long lda
                                     long ldc, hipStream_t st) const
 {
     if (mode == TRAIL_NATIVE) {
-        ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 8.0 * (2.0 * m * n + (double)m * k + (double)k * n));
-        return launch_cgemm<float>(m, n, k, c32{-1.f, 0.f}, A, lda, 0, B, ldb, 0, c32{1.f, 0.f}, C, ldc, st);
+        ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k * batch, 8.0 * (2.0 * m * n + (double)m * k + (double)k * n) * batch);
+        return launch_cgemm<float>(m, n, k, c32{-1.f, 0.f}, A, lda, 0, B, ldb, 0, c32{1.f, 0.f}, C, ldc, st, batch,
+                                   stride, stride, stride);
     }
+    MIDAS_CHECK(batch == 1, -1, "trail16: batched split-precision updates are not supported");
     return launch_trail16(mode, m, n, k, (const float2 *)A, lda, (const float2 *)B, ldb, (float2 *)C, ldc, scratch,
                           st);
 }

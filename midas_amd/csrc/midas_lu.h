@@ -8,6 +8,8 @@ constexpr int LU_IB = 16;   // inner panel width (column-step kernels)
 
 template <typename T> struct LuWork {
     int n = 0, nb = 128, maxblk = 0;
+    int batch = 1;                   // matrices factored together (same n), A stride sA
+    long sA = 0;
     int jb = 0;                      // first column of the current inner block
     T *pmax = nullptr;               // per-workgroup pivot candidates
     int *pidx = nullptr;
@@ -19,8 +21,9 @@ template <typename T> struct LuWork {
     cplx<T> *u12 = nullptr;          // nb x n staging for U12 = L11^-1 A12
     hipStream_t pstream = nullptr;   // panel stream (look-ahead), default priority
     hipEvent_t ev_panel = nullptr, ev_next = nullptr;
-    int alloc(int n, int nb);
+    int alloc(int n, int nb, int batch = 1, long sA = 0);
     void release();
+    long inv_stride() const { return (long)((n + nb - 1) / nb) * nb * nb; }   // diagonal-block inverses per matrix
 };
 
 // Trailing update C(m x n) -= A(m x k) B(k x n), complex, column-major.
@@ -29,6 +32,8 @@ enum TrailingMode { TRAIL_NATIVE = 0, TRAIL_F16_1 = 1, TRAIL_F16_3 = 3, TRAIL_BF
 template <typename T> struct TrailingGemm {
     int mode = TRAIL_NATIVE;
     void *scratch = nullptr;        // split operand planes + column scales (trail16_scratch_bytes)
+    int batch = 1;                  // batched factorisations: matrices at stride `stride`
+    long stride = 0;
     int operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C, long ldc,
                    hipStream_t st) const;
 };
@@ -42,11 +47,13 @@ template <> struct cvec<float> { typedef float2 t; };
 template <typename T>
 int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipStream_t st, TrailingGemm<T> tg);
 // Solve with the factors of getrf (same LuWork: it holds the diagonal-block
-// inverses); tmp needs max(n, nb) x nrhs elements.
+// inverses); tmp needs max(n, nb) x nrhs elements per matrix.  With
+// w.batch > 1 every matrix (stride w.sA) solves its own B (stride sB).
 template <typename T>
 int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, const LuWork<T> &w, typename cvec<T>::t *B,
-          long ldb, int nrhs, typename cvec<T>::t *tmp, hipStream_t st);
+          long ldb, int nrhs, typename cvec<T>::t *tmp, hipStream_t st, long sB = 0);
 void ipiv_to_perm(const int *ipiv, int n, int *perm);
+int ipiv_to_perm_dev(const int *ipiv, int *perm, int n, int batch, hipStream_t st);
 
 // C = alpha op(A) op(B) + beta C; op: 0 none, 1 transpose, 2 conjugate transpose.
 template <typename T>

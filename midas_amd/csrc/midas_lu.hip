@@ -44,12 +44,20 @@ __global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t
     __shared__ V urow[64];
     __shared__ int last;
     const int tid = threadIdx.x;
+    // matrix of the batch
+    const int zb = blockIdx.y;
+    A += zb * w.sA;
+    ipiv += (long)zb * n;
+    T *pmax = w.pmax + (long)zb * w.maxblk;
+    int *pidx = w.pidx + (long)zb * w.maxblk;
+    int *counter = w.counter + zb;
+    cplx<T> *pinvp = w.pinv + zb;
     const int r0 = j + blockIdx.x * STEP_ROWS;
     const int r1 = min(n, r0 + STEP_ROWS);
     const int ncol = jend - j;   // columns to update
     cplx<T> pinv = {0, 0};
     if (j > w.jb) {
-        pinv = w.pinv[0];
+        pinv = pinvp[0];
         for (int c = tid; c < ncol; c += STEP_THREADS) urow[c] = A[(long)(j - 1) + (long)(j + c) * lda];
     }
     __syncthreads();
@@ -88,10 +96,10 @@ __global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t
         __syncthreads();
     }
     if (tid == 0) {
-        w.pmax[blockIdx.x] = smax[0];
-        w.pidx[blockIdx.x] = sidx[0];
+        pmax[blockIdx.x] = smax[0];
+        pidx[blockIdx.x] = sidx[0];
         __threadfence();
-        int done = atomicAdd(w.counter, 1);
+        int done = atomicAdd(counter, 1);
         last = (done == (int)gridDim.x - 1);
     }
     __syncthreads();
@@ -103,8 +111,8 @@ __global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t
         T b = (T)-1;
         int bi = 0x7fffffff;
         for (int q = tid; q < (int)gridDim.x; q += STEP_THREADS) {
-            T v = ((volatile T *)w.pmax)[q];
-            int vi = ((volatile int *)w.pidx)[q];
+            T v = ((volatile T *)pmax)[q];
+            int vi = ((volatile int *)pidx)[q];
             if (v > b || (v == b && vi < bi)) { b = v; bi = vi; }
         }
         smax[tid] = b;
@@ -121,7 +129,7 @@ __global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t
         if (tid == 0) {
             if (sidx[0] == 0x7fffffff) sidx[0] = j;
             ipiv[j] = sidx[0];
-            *w.counter = 0;
+            *counter = 0;
         }
     }
     __syncthreads();
@@ -141,10 +149,10 @@ __global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t
         cplx<T> dv = {d.x, d.y};
         if (d.x == (T)0 && d.y == (T)0) {
             // exactly singular: record the first zero pivot (LAPACK INFO), keep going
-            atomicCAS(w.info, 0, j + 1);
-            w.pinv[0] = cplx<T>{0, 0};
+            atomicCAS(w.info + zb, 0, j + 1);
+            pinvp[0] = cplx<T>{0, 0};
         } else {
-            w.pinv[0] = cinv(dv);
+            pinvp[0] = cinv(dv);
         }
     }
 }
@@ -152,9 +160,10 @@ __global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t
 // Forward substitution with the unit-lower IBxIB block L(jb:jend, jb:jend) on
 // rows [jb, jend) of columns [c0, c1): one thread per column.
 template <typename T>
-__global__ void k_trsm_lunit_small(typename cvec<T>::t *A, long lda, int jb, int ib, int c0, int c1)
+__global__ void k_trsm_lunit_small(typename cvec<T>::t *A, long lda, long sA, int jb, int ib, int c0, int c1)
 {
     typedef typename cvec<T>::t V;
+    A += blockIdx.y * sA;
     int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= c1) return;
     V *x = A + (long)jb + (long)c * lda;
@@ -171,10 +180,12 @@ __global__ void k_trsm_lunit_small(typename cvec<T>::t *A, long lda, int jb, int
 
 // Apply the row interchanges ipiv[k0..k1) to columns [c0, c1) (and [c2, c3)).
 template <typename T>
-__global__ void k_laswp(typename cvec<T>::t *A, long lda, const int *ipiv, int k0, int k1, int c0, int c1, int c2,
-                        int c3)
+__global__ void k_laswp(typename cvec<T>::t *A, long lda, long sA, const int *ipiv, int n, int k0, int k1, int c0,
+                        int c1, int c2, int c3)
 {
     typedef typename cvec<T>::t V;
+    A += blockIdx.y * sA;
+    ipiv += (long)blockIdx.y * n;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int w0 = c1 - c0;
     int c = t < w0 ? c0 + t : c2 + (t - w0);
@@ -199,10 +210,12 @@ __global__ void k_laswp(typename cvec<T>::t *A, long lda, const int *ipiv, int k
 // diagonal, x = U^-1 e_c.  Inv is ldi x bs.
 constexpr int TRINV_QC = 8;
 template <typename T, bool LOWER, int R>
-__global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D, long lda, int bs,
-                                                    typename cvec<T>::t *Inv, long ldi)
+__global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D, long lda, long sD, int bs,
+                                                    typename cvec<T>::t *Inv, long ldi, long sI)
 {
     typedef typename cvec<T>::t V;
+    D += blockIdx.y * sD;
+    Inv += blockIdx.y * sI;
     const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (c >= bs) return;   // wave-uniform
@@ -271,26 +284,30 @@ __global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D
 }
 
 template <typename T, bool LOWER>
-static int trinv(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T>::t *Inv, long ldi, hipStream_t st)
+static int trinv(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T>::t *Inv, long ldi, hipStream_t st,
+                 int batch = 1, long sD = 0, long sI = 0)
 {
     MIDAS_CHECK(bs <= 512, -1, "trinv: block %d above 512", bs);
-    const dim3 g((bs + 3) / 4), b(256);
+    const dim3 g((bs + 3) / 4, batch), b(256);
     if (bs <= 128)
-        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 2>), g, b, 0, st, D, lda, bs, Inv, ldi);
+        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 2>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI);
     else if (bs <= 256)
-        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 4>), g, b, 0, st, D, lda, bs, Inv, ldi);
+        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 4>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI);
     else
-        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 8>), g, b, 0, st, D, lda, bs, Inv, ldi);
+        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 8>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
 
 // Copy a (rows x cols) block between column-major arrays.
 template <typename T>
-__global__ void k_copy2d(const typename cvec<T>::t *S, long lds, typename cvec<T>::t *D, long ldd, int rows, int cols)
+__global__ void k_copy2d(const typename cvec<T>::t *S, long lds, typename cvec<T>::t *D, long ldd, int rows, int cols,
+                         long sS = 0, long sD = 0)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     int c = blockIdx.y;
+    S += blockIdx.z * sS;
+    D += blockIdx.z * sD;
     if (r < rows && c < cols) D[(long)r + (long)c * ldd] = S[(long)r + (long)c * lds];
 }
 
@@ -309,20 +326,20 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
             int rows = n - j;
             int nblk = rows > 0 ? (rows + STEP_ROWS - 1) / STEP_ROWS : 1;
             MIDAS_CHECK(nblk <= w.maxblk, -2, "lu: step workspace too small");
-            hipLaunchKernelGGL(k_getf2_step<T>, dim3(nblk), dim3(STEP_THREADS), 0, st, A, lda, n, j,
+            hipLaunchKernelGGL(k_getf2_step<T>, dim3(nblk, w.batch), dim3(STEP_THREADS), 0, st, A, lda, n, j,
                                j < jend ? jend : j, k, k + nb, ipiv, w);
         }
         MIDAS_HIP(hipGetLastError());
         // rest of the outer panel: columns [jend, k+nb)
         int rc = k + nb - jend;
         if (rc > 0) {
-            hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((rc + 63) / 64), dim3(64), 0, st, A, lda, jb, jend - jb,
-                               jend, k + nb);
+            hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((rc + 63) / 64, w.batch), dim3(64), 0, st, A, lda, w.sA, jb,
+                               jend - jb, jend, k + nb);
             MIDAS_HIP(hipGetLastError());
             if (n - jend > 0) {
                 int r = launch_cgemm<T>(n - jend, rc, jend - jb, cmone<T>(), A + jend + (long)jb * lda, lda, 0,
                                         A + jb + (long)jend * lda, lda, 0, cone<T>(), A + jend + (long)jend * lda,
-                                        lda, st, 1, 0, 0, 0);
+                                        lda, st, w.batch, w.sA, w.sA, w.sA);
                 if (r) return r;
             }
         }
@@ -330,7 +347,7 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
     // inverse of the unit-lower diagonal block L11 of this panel: U12 = L11^-1 A12
     // becomes one GEMM on the main stream, and getrs reuses it
     return trinv<T, true>(A + k + (long)k * lda, lda, nb, (typename cvec<T>::t *)w.linv + (long)(k / w.nb) * w.nb * w.nb,
-                          w.nb, st);
+                          w.nb, st, w.batch, w.sA, w.inv_stride());
 }
 
 // U12(:, c0:c1) = L11^-1 A12(:, c0:c1) through the workspace, for panel k.
@@ -342,11 +359,12 @@ static int u12_solve(typename cvec<T>::t *A, long lda, int k, int nb, int c0, in
     if (nc <= 0) return 0;
     const V *Li = (const V *)w.linv + (long)(k / w.nb) * w.nb * w.nb;
     V *tmp = (V *)w.u12;
+    const long st12 = (long)w.nb * w.n;
     int r = launch_cgemm<T>(nb, nc, nb, cone<T>(), Li, w.nb, 0, A + k + (long)c0 * lda, lda, 0, czero<T>(), tmp, nb,
-                            st, 1, 0, 0, 0);
+                            st, w.batch, w.inv_stride(), w.sA, st12);
     if (r) return r;
-    hipLaunchKernelGGL(k_copy2d<T>, dim3((nb + 255) / 256, nc), dim3(256), 0, st, tmp, (long)nb,
-                       A + k + (long)c0 * lda, lda, nb, nc);
+    hipLaunchKernelGGL(k_copy2d<T>, dim3((nb + 255) / 256, nc, w.batch), dim3(256), 0, st, tmp, (long)nb,
+                       A + k + (long)c0 * lda, lda, nb, nc, st12, w.sA);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
@@ -362,8 +380,9 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
 {
     typedef typename cvec<T>::t V;
     hipStream_t ps = w.pstream;
-    MIDAS_HIP(hipMemsetAsync(w.counter, 0, sizeof(int), st));
-    MIDAS_HIP(hipMemsetAsync(w.info, 0, sizeof(int), st));
+    MIDAS_CHECK(w.batch == 1 || tg.batch == w.batch, -1, "getrf: trailing GEMM not batched like the factorisation");
+    MIDAS_HIP(hipMemsetAsync(w.counter, 0, sizeof(int) * w.batch, st));
+    MIDAS_HIP(hipMemsetAsync(w.info, 0, sizeof(int) * w.batch, st));
     MIDAS_HIP(hipEventRecord(w.ev_next, st));
     MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
     int r = panel<T>(A, lda, n, 0, min(w.nb, n), ipiv, w, ps);
@@ -375,8 +394,8 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
         // swaps outside the panel: columns [0, k) and [k+nb, n)
         int ncols = k + (n - k - nb);
         if (ncols > 0) {
-            hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + 255) / 256), dim3(256), 0, st, A, lda, ipiv, k, k + nb, 0,
-                               k, k + nb, n);
+            hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + 255) / 256, w.batch), dim3(256), 0, st, A, lda, w.sA, ipiv, n,
+                               k, k + nb, 0, k, k + nb, n);
             MIDAS_HIP(hipGetLastError());
         }
         int nr = n - k - nb;
@@ -409,7 +428,7 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
     // inverses of the diagonal blocks of U, used by getrs (those of L come from the panels)
     for (int k = 0; k < n; k += w.nb) {
         r = trinv<T, false>(A + k + (long)k * lda, lda, min(w.nb, n - k), (V *)w.uinv + (long)(k / w.nb) * w.nb * w.nb,
-                            w.nb, st);
+                            w.nb, st, w.batch, w.sA, w.inv_stride());
         if (r) return r;
     }
     return 0;
@@ -418,40 +437,73 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
 // ---------------------------------------------------------------------------
 // Triangular solves with the factors (getrs), nrhs columns of B (ldb).
 // ---------------------------------------------------------------------------
-// B <- P B with the final permutation perm (B2[i] = B[perm[i]]).
+// B <- P B with the final permutation perm (B2[i] = B[perm[i]]); batch on z.
 template <typename T>
-__global__ void k_permute_rows(const typename cvec<T>::t *B, long ldb, typename cvec<T>::t *D, long ldd,
-                               const int *perm, int n)
+__global__ void k_permute_rows(const typename cvec<T>::t *B, long ldb, long sB, typename cvec<T>::t *D, long ldd,
+                               long sD, const int *perm, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int c = blockIdx.y;
+    B += blockIdx.z * sB;
+    D += blockIdx.z * sD;
+    perm += (long)blockIdx.z * n;
     if (i < n) D[(long)i + (long)c * ldd] = B[(long)perm[i] + (long)c * ldb];
 }
 
+// perm from LAPACK interchanges on the device, one thread per matrix
+__global__ void k_ipiv_to_perm(const int *ipiv, int *perm, int n, int batch)
+{
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= batch) return;
+    ipiv += (long)z * n;
+    perm += (long)z * n;
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int i = 0; i < n; ++i) {
+        const int p = ipiv[i];
+        if (p != i) {
+            const int t = perm[i];
+            perm[i] = perm[p];
+            perm[p] = t;
+        }
+    }
+}
+
+int ipiv_to_perm_dev(const int *ipiv, int *perm, int n, int batch, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ipiv_to_perm, dim3((batch + 63) / 64), dim3(64), 0, st, ipiv, perm, n, batch);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+// Solve with the factors of getrf for every matrix of the batch: B (n x nrhs,
+// ldb, stride sB per matrix); tmp needs max(n, nb) x nrhs per matrix.
 template <typename T>
 int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, const LuWork<T> &w,
-          typename cvec<T>::t *B, long ldb, int nrhs, typename cvec<T>::t *tmp, hipStream_t st)
+          typename cvec<T>::t *B, long ldb, int nrhs, typename cvec<T>::t *tmp, hipStream_t st, long sB)
 {
     typedef typename cvec<T>::t V;
-    const int nb = w.nb;
+    const int nb = w.nb, bt = w.batch;
+    const long sA = w.sA, sI = w.inv_stride(), sT = (long)max(n, nb) * nrhs;
     if (n <= 0 || nrhs <= 0) return 0;
     MIDAS_CHECK(nrhs <= 65535, -1, "getrs: nrhs too large for one launch");
-    hipLaunchKernelGGL(k_permute_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, tmp, (long)n, perm,
-                       n);
-    hipLaunchKernelGGL(k_copy2d<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, tmp, (long)n, B, ldb, n, nrhs);
+    hipLaunchKernelGGL(k_permute_rows<T>, dim3((n + 255) / 256, nrhs, bt), dim3(256), 0, st, B, ldb, sB, tmp, (long)n,
+                       sT, perm, n);
+    hipLaunchKernelGGL(k_copy2d<T>, dim3((n + 255) / 256, nrhs, bt), dim3(256), 0, st, tmp, (long)n, B, ldb, n, nrhs,
+                       sT, sB);
     MIDAS_HIP(hipGetLastError());
     // forward: B_k <- Linv_k B_k ; B_{>k} -= L_{>k,k} B_k
     for (int k = 0; k < n; k += nb) {
         int b = min(nb, n - k);
         const V *Li = (const V *)w.linv + (long)(k / nb) * nb * nb;
-        int r = launch_cgemm<T>(b, nrhs, b, cone<T>(), Li, nb, 0, B + k, ldb, 0, czero<T>(), tmp, nb, st, 1, 0, 0, 0);
+        int r = launch_cgemm<T>(b, nrhs, b, cone<T>(), Li, nb, 0, B + k, ldb, 0, czero<T>(), tmp, nb, st, bt, sI, sB,
+                                sT);
         if (r) return r;
-        hipLaunchKernelGGL(k_copy2d<T>, dim3((b + 255) / 256, nrhs), dim3(256), 0, st, tmp, (long)nb, B + k, ldb, b,
-                           nrhs);
+        hipLaunchKernelGGL(k_copy2d<T>, dim3((b + 255) / 256, nrhs, bt), dim3(256), 0, st, tmp, (long)nb, B + k, ldb,
+                           b, nrhs, sT, sB);
         MIDAS_HIP(hipGetLastError());
         if (n - k - b > 0) {
             r = launch_cgemm<T>(n - k - b, nrhs, b, cmone<T>(), A + (k + b) + (long)k * lda, lda, 0, B + k, ldb, 0,
-                                cone<T>(), B + k + b, ldb, st, 1, 0, 0, 0);
+                                cone<T>(), B + k + b, ldb, st, bt, sA, sB, sB);
             if (r) return r;
         }
     }
@@ -459,14 +511,15 @@ int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, const 
     for (int k = ((n - 1) / nb) * nb; k >= 0; k -= nb) {
         int b = min(nb, n - k);
         const V *Ui = (const V *)w.uinv + (long)(k / nb) * nb * nb;
-        int r = launch_cgemm<T>(b, nrhs, b, cone<T>(), Ui, nb, 0, B + k, ldb, 0, czero<T>(), tmp, nb, st, 1, 0, 0, 0);
+        int r = launch_cgemm<T>(b, nrhs, b, cone<T>(), Ui, nb, 0, B + k, ldb, 0, czero<T>(), tmp, nb, st, bt, sI, sB,
+                                sT);
         if (r) return r;
-        hipLaunchKernelGGL(k_copy2d<T>, dim3((b + 255) / 256, nrhs), dim3(256), 0, st, tmp, (long)nb, B + k, ldb, b,
-                           nrhs);
+        hipLaunchKernelGGL(k_copy2d<T>, dim3((b + 255) / 256, nrhs, bt), dim3(256), 0, st, tmp, (long)nb, B + k, ldb,
+                           b, nrhs, sT, sB);
         MIDAS_HIP(hipGetLastError());
         if (k > 0) {
             r = launch_cgemm<T>(k, nrhs, b, cmone<T>(), A + (long)k * lda, lda, 0, B + k, ldb, 0, cone<T>(), B, ldb, st,
-                                1, 0, 0, 0);
+                                bt, sA, sB, sB);
             if (r) return r;
         }
     }
@@ -489,20 +542,21 @@ void ipiv_to_perm(const int *ipiv, int n, int *perm)
 }
 
 template <typename T>
-int LuWork<T>::alloc(int n_, int nb_)
+int LuWork<T>::alloc(int n_, int nb_, int batch_, long sA_)
 {
     n = n_;
     nb = nb_;
+    batch = batch_;
+    sA = sA_;
     maxblk = (n + STEP_ROWS - 1) / STEP_ROWS + 1;
-    MIDAS_HIP(hipMalloc(&pmax, sizeof(T) * maxblk));
-    MIDAS_HIP(hipMalloc(&pidx, sizeof(int) * maxblk));
-    MIDAS_HIP(hipMalloc(&counter, sizeof(int) * 2));
-    info = counter + 1;
-    MIDAS_HIP(hipMalloc(&pinv, sizeof(cplx<T>)));
-    const size_t nblk = (size_t)(n + nb - 1) / nb;
-    MIDAS_HIP(hipMalloc(&linv, sizeof(cplx<T>) * nblk * nb * nb));
-    MIDAS_HIP(hipMalloc(&uinv, sizeof(cplx<T>) * nblk * nb * nb));
-    MIDAS_HIP(hipMalloc(&u12, sizeof(cplx<T>) * (size_t)nb * n));
+    MIDAS_HIP(hipMalloc(&pmax, sizeof(T) * maxblk * batch));
+    MIDAS_HIP(hipMalloc(&pidx, sizeof(int) * maxblk * batch));
+    MIDAS_HIP(hipMalloc(&counter, sizeof(int) * 2 * batch));
+    info = counter + batch;
+    MIDAS_HIP(hipMalloc(&pinv, sizeof(cplx<T>) * batch));
+    MIDAS_HIP(hipMalloc(&linv, sizeof(cplx<T>) * inv_stride() * batch));
+    MIDAS_HIP(hipMalloc(&uinv, sizeof(cplx<T>) * inv_stride() * batch));
+    MIDAS_HIP(hipMalloc(&u12, sizeof(cplx<T>) * (size_t)nb * n * batch));
     MIDAS_HIP(hipStreamCreate(&pstream));
     MIDAS_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
     MIDAS_HIP(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming));
@@ -538,8 +592,8 @@ template struct LuWork<float>;
 template int getrf<double>(double2 *, long, int, int *, LuWork<double> &, hipStream_t, TrailingGemm<double>);
 template int getrf<float>(float2 *, long, int, int *, LuWork<float> &, hipStream_t, TrailingGemm<float>);
 template int getrs<double>(const double2 *, long, int, const int *, const LuWork<double> &, double2 *, long, int,
-                           double2 *, hipStream_t);
+                           double2 *, hipStream_t, long);
 template int getrs<float>(const float2 *, long, int, const int *, const LuWork<float> &, float2 *, long, int,
-                          float2 *, hipStream_t);
+                          float2 *, hipStream_t, long);
 
 }  // namespace midas
