@@ -21,11 +21,13 @@
 //   2. reduced matrix Zc(K x K), block (i, j) = C_i^T Z_ij C_j (plain transpose,
 //      as the reference's ZGEMM('T', ...)):
 //        dense path: one fill of the column strip Z(:, block j) (3N x 3n_j),
-//                    T = Z(:, j) C_j as one GEMM, then C_i^T T(rows of i) per i;
+//                    T = Z(:, j) C_j as one GEMM, then every C_i^T T(rows of i)
+//                    in one grouped launch of small GEMM tiles;
 //        ACA path:   diagonal blocks dense; every off-diagonal pair (i, j) by ACA
 //                    (one workgroup per pair, a block column per launch batch),
-//                    then C_i^T (U (V C_j)) when the ACA rank is below kmax, and
-//                    the dense product otherwise (the reference's fallback).
+//                    then (C_i^T U)(V C_j) for all pairs in two grouped launches
+//                    when the ACA rank is below kmax, and the dense product
+//                    otherwise (the reference's fallback).
 //   3. Vc = C_i^T E_inc(block i) (FP32 Incident_Field quirk), alpha = Zc^-1 Vc
 //      (FP64 LU or mixed-precision LU + FP64 refinement, the `mode`), and
 //      E(block i) = C_i alpha_i.
@@ -110,6 +112,96 @@ static int launch_pick_eigvecs(const double2 *W, long ldw, const double *lam, in
     hipLaunchKernelGGL(k_pick_eigvecs, dim3((rows + 255) / 256, K), dim3(256), 0, st, W, ldw, lam, rpad, rows, K,
                        (int)scale, out, ldo);
     MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped small GEMMs: every job is one 32 x 32 tile of C = op(A) B, complex
+// FP64, op = none or plain transpose.  The reduced-matrix assembly has
+// thousands of independent products of a few tens of columns (C_i^T T, C_i^T U,
+// V C_j, L R); one launch carries all of them instead of one launch each.
+// ---------------------------------------------------------------------------
+struct GJob {
+    const double2 *A, *B;
+    double2 *C;
+    int lda, ldb, ldc;
+    int m, n, k;    // op(A) m x k, B k x n
+    int tA;         // 1: op(A) = A^T (A stored k x m)
+    int i0, j0;     // tile origin in C
+};
+
+constexpr int GT = 32, GK = 16;
+__global__ __launch_bounds__(256) void k_ggemm(const GJob *__restrict__ jobs)
+{
+    const GJob jb = jobs[blockIdx.x];
+    __shared__ double2 sA[GK][GT + 1], sB[GK][GT + 1];
+    const int t = threadIdx.x;
+    const int tx = t & 15, ty = t >> 4;   // outputs (i0 + tx + 16 p, j0 + ty + 16 q)
+    z64 acc[2][2] = {};
+    for (int k0 = 0; k0 < jb.k; k0 += GK) {
+        // op(A) tile: GT rows x GK k
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int idx = t + 256 * e;
+            int kk, aa;
+            if (jb.tA) { kk = idx & 15; aa = idx >> 4; }   // A stored k x m: unit stride along k
+            else { aa = idx & 31; kk = idx >> 5; }          // A stored m x k: unit stride along m
+            const int a = jb.i0 + aa, kg = k0 + kk;
+            double2 v = make_double2(0.0, 0.0);
+            if (a < jb.m && kg < jb.k)
+                v = jb.tA ? jb.A[(long)a * jb.lda + kg] : jb.A[(long)kg * jb.lda + a];
+            sA[kk][aa] = v;
+        }
+        // B tile: GK k x GT cols
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int idx = t + 256 * e;
+            const int kk = idx & 15, bb = idx >> 4;
+            const int b = jb.j0 + bb, kg = k0 + kk;
+            sB[kk][bb] = (b < jb.n && kg < jb.k) ? jb.B[(long)b * jb.ldb + kg] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; ++kk)
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const double2 x = sA[kk][tx + 16 * p], y = sB[kk][ty + 16 * q];
+                    acc[p][q].r += x.x * y.x - x.y * y.y;
+                    acc[p][q].i += x.x * y.y + x.y * y.x;
+                }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int a = jb.i0 + tx + 16 * p, b = jb.j0 + ty + 16 * q;
+            if (a < jb.m && b < jb.n) jb.C[(long)b * jb.ldc + a] = make_double2(acc[p][q].r, acc[p][q].i);
+        }
+}
+
+// All tiles of C = op(A) B (m x n) appended to a job list.
+static void add_gemm_tiles(std::vector<GJob> &jobs, const double2 *A, int lda, int tA, const double2 *B, int ldb,
+                           double2 *C, int ldc, int m, int n, int k)
+{
+    for (int j0 = 0; j0 < n; j0 += GT)
+        for (int i0 = 0; i0 < m; i0 += GT) jobs.push_back(GJob{A, B, C, lda, ldb, ldc, m, n, k, tA, i0, j0});
+}
+
+static int run_ggemm(const std::vector<GJob> &jobs, DBuf &djobs, hipStream_t st)
+{
+    if (jobs.empty()) return 0;
+    TRY(djobs.alloc(sizeof(GJob) * jobs.size()));
+    MIDAS_HIP(hipMemcpyAsync(djobs.p, jobs.data(), sizeof(GJob) * jobs.size(), hipMemcpyHostToDevice, st));
+    for (size_t q0 = 0; q0 < jobs.size(); q0 += 1u << 30) {
+        const unsigned cnt = (unsigned)std::min<size_t>(jobs.size() - q0, 1u << 30);
+        hipLaunchKernelGGL(k_ggemm, dim3(cnt), dim3(256), 0, st, djobs.as<GJob>() + q0);
+    }
+    MIDAS_HIP(hipGetLastError());
+    // the host job list may be reused right after: wait for the copy
+    MIDAS_HIP(hipStreamSynchronize(st));
     return 0;
 }
 
@@ -482,21 +574,41 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
     TRY(Zc.alloc(sizeof(double2) * (size_t)Kt * Kt));
     double2 *zc_ = Zc.as<double2>();
     const CellSet ALL{N, 0, nullptr};
+    DBuf djobs;
+    std::vector<GJob> jobs;
     if (!use_aca) {
+        // T(:, block j) = Z(:, block j) C_j for a chunk of block columns (one
+        // strip fill and one GEMM each), then every C_i^T T(rows of i, j) as
+        // one grouped launch
+        const long tbudget = 8L << 30;
+        const int tcols = (int)std::max<long>(kbmax, std::min<long>(Kt, tbudget / ((long)sizeof(double2) * N3)));
         DBuf Zs, Tt;
         TRY(Zs.alloc(sizeof(double2) * N3 * 3 * nmax));
-        TRY(Tt.alloc(sizeof(double2) * N3 * std::max(kbmax, 1)));
-        for (int j = 0; j < nb; ++j) {
-            if (!K[j]) continue;
-            TRY(launch_fill(dc.v, ALL, CellSet{nbk[j], bs[j], nullptr}, k0, quirks, ref32, Zs.as<double2>(), N3,
-                            nullptr, 0, st));
-            TRY(launch_cgemm<double>(N3, K[j], 3 * nbk[j], ONE, Zs.p, N3, 0, Cb(j), 3L * nbk[j], 0, ZERO, Tt.p, N3,
-                                     st));
-            for (int i = 0; i < nb; ++i)
-                if (K[i])
-                    TRY(launch_cgemm<double>(K[i], K[j], 3 * nbk[i], ONE, Cb(i), 3L * nbk[i], 1,
-                                             Tt.as<double2>() + 3L * bs[i], N3, 0, ZERO,
-                                             zc_ + offs[i] + offs[j] * Kt, Kt, st));
+        TRY(Tt.alloc(sizeof(double2) * N3 * std::max(tcols, 1)));
+        int j0 = 0;
+        while (j0 < nb) {
+            int j1 = j0, used = 0;
+            std::vector<int> tcol(nb, 0);
+            while (j1 < nb && (j1 == j0 || used + K[j1] <= tcols)) {
+                tcol[j1] = used;
+                used += K[j1];
+                ++j1;
+            }
+            jobs.clear();
+            for (int j = j0; j < j1; ++j) {
+                if (!K[j]) continue;
+                TRY(launch_fill(dc.v, ALL, CellSet{nbk[j], bs[j], nullptr}, k0, quirks, ref32, Zs.as<double2>(), N3,
+                                nullptr, 0, st));
+                double2 *Tj = Tt.as<double2>() + (long)tcol[j] * N3;
+                TRY(launch_cgemm<double>(N3, K[j], 3 * nbk[j], ONE, Zs.p, N3, 0, Cb(j), 3L * nbk[j], 0, ZERO, Tj, N3,
+                                         st));
+                for (int i = 0; i < nb; ++i)
+                    if (K[i])
+                        add_gemm_tiles(jobs, Cb(i), 3 * nbk[i], 1, Tj + 3L * bs[i], (int)N3,
+                                       zc_ + offs[i] + offs[j] * Kt, (int)Kt, K[i], K[j], 3 * nbk[i]);
+            }
+            TRY(run_ggemm(jobs, djobs, st));
+            j0 = j1;
         }
     } else {
         // dense products C_i^T Z_ij C_j for the diagonal and for ACA fallbacks
@@ -514,71 +626,86 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
         };
         for (int j = 0; j < nb; ++j)
             if (K[j]) TRY(dense_pair(j, j));
-        // off-diagonal pairs by ACA, a batch of block columns per launch
-        const double budget = 4.0e9;   // bytes of U and Vt per batch
-        DBuf P, Q;
-        TRY(P.alloc(sizeof(double2) * (size_t)kmax * std::max(kbmax, 1)));
-        TRY(Q.alloc(sizeof(double2) * 3L * nmax * std::max(kbmax, 1)));
+        // off-diagonal pairs by ACA, a batch of block columns per launch; then
+        // Zc_ij = (C_i^T U)(V C_j) through two grouped launches
+        const double budget = 4.0e9;   // bytes of U, Vt and the small factors per batch
         int j0 = 0;
         while (j0 < nb) {
-            std::vector<AcaJob> jobs;
+            std::vector<AcaJob> ajobs;
             std::vector<int> pi_, pj_;
             long uo = 0, vo = 0;
             int j1 = j0;
             while (j1 < nb) {
                 long add_u = 0, add_v = 0;
                 std::vector<AcaJob> col;
+                std::vector<int> ci;
                 for (int i = 0; i < nb; ++i) {
                     if (i == j1 || !K[i] || !K[j1]) continue;
                     AcaJob jb{3 * nbk[i], 3 * nbk[j1], bs[i], bs[j1], uo + add_u, vo + add_v};
                     add_u += 3L * nbk[i] * kmax;
                     add_v += 3L * nbk[j1] * kmax;
                     col.push_back(jb);
+                    ci.push_back(i);
+                }
+                if (j1 > j0 && (double)(uo + add_u + vo + add_v) * sizeof(double2) > budget) break;
+                ajobs.insert(ajobs.end(), col.begin(), col.end());
+                for (int i : ci) {
                     pi_.push_back(i);
                     pj_.push_back(j1);
                 }
-                if (j1 > j0 && (double)(uo + add_u + vo + add_v) * sizeof(double2) > budget) {
-                    pi_.resize(jobs.size());
-                    pj_.resize(jobs.size());
-                    break;
-                }
-                jobs.insert(jobs.end(), col.begin(), col.end());
                 uo += add_u;
                 vo += add_v;
                 ++j1;
             }
-            if (!jobs.empty()) {
-                DBuf dj, Ub, Vb, dk;
-                TRY(upload(dj, jobs.data(), sizeof(AcaJob) * jobs.size(), st));
+            if (!ajobs.empty()) {
+                DBuf dj, Ub, Vb, dk, LR;
+                TRY(upload(dj, ajobs.data(), sizeof(AcaJob) * ajobs.size(), st));
                 TRY(Ub.alloc(sizeof(double2) * std::max(uo, 1L)));
                 TRY(Vb.alloc(sizeof(double2) * std::max(vo, 1L)));
-                TRY(dk.alloc(sizeof(int) * jobs.size()));
-                TRY(launch_aca_cells(dc.v, ref32, k0, host_pi(quirks), dj.as<AcaJob>(), (int)jobs.size(),
+                TRY(dk.alloc(sizeof(int) * ajobs.size()));
+                TRY(launch_aca_cells(dc.v, ref32, k0, host_pi(quirks), dj.as<AcaJob>(), (int)ajobs.size(),
                                      Ub.as<double2>(), Vb.as<double2>(), kmax, eps_aca, dk.as<int>(), st));
-                std::vector<int> hk(jobs.size());
+                std::vector<int> hk(ajobs.size());
                 TRY(copy_sync(hk.data(), dk.p, sizeof(int) * hk.size(), hipMemcpyDeviceToHost, st));
-                for (size_t q = 0; q < jobs.size(); ++q) {
-                    const int i = pi_[q], j = pj_[q];
+                // L_ij = C_i^T U (K_i x k) and R_ij = V C_j (k x K_j) in one scratch
+                std::vector<long> lo(ajobs.size()), ro(ajobs.size());
+                long lr = 0;
+                for (size_t q = 0; q < ajobs.size(); ++q) {
                     const int kk = std::min(hk[q], kmax);
-                    if (kk >= kmax) {   // not rank-deficient enough: dense block
-                        TRY(dense_pair(i, j));
-                        continue;
-                    }
-                    const double2 *U = Ub.as<double2>() + jobs[q].u_off, *Vt = Vb.as<double2>() + jobs[q].v_off;
-                    // P = V C_j (kk x K_j), Q = U P (3n_i x K_j), Zc_ij = C_i^T Q
-                    TRY(launch_cgemm<double>(kk, K[j], 3 * nbk[j], ONE, Vt, 3L * nbk[j], 1, Cb(j), 3L * nbk[j], 0,
-                                             ZERO, P.p, kmax, st));
-                    TRY(launch_cgemm<double>(3 * nbk[i], K[j], kk, ONE, U, 3L * nbk[i], 0, P.p, kmax, 0, ZERO, Q.p,
-                                             3L * nbk[i], st));
-                    TRY(launch_cgemm<double>(K[i], K[j], 3 * nbk[i], ONE, Cb(i), 3L * nbk[i], 1, Q.p, 3L * nbk[i], 0,
-                                             ZERO, zc_ + offs[i] + offs[j] * Kt, Kt, st));
+                    if (kk >= kmax) continue;
+                    lo[q] = lr;
+                    lr += (long)K[pi_[q]] * kk;
+                    ro[q] = lr;
+                    lr += (long)kk * K[pj_[q]];
                 }
-                MIDAS_HIP(hipStreamSynchronize(st));   // U, V, P, Q are reused by the next batch
+                TRY(LR.alloc(sizeof(double2) * std::max(lr, 1L)));
+                jobs.clear();
+                for (size_t q = 0; q < ajobs.size(); ++q) {
+                    const int i = pi_[q], j = pj_[q], kk = std::min(hk[q], kmax);
+                    if (kk >= kmax) continue;
+                    const double2 *U = Ub.as<double2>() + ajobs[q].u_off, *Vt = Vb.as<double2>() + ajobs[q].v_off;
+                    add_gemm_tiles(jobs, Cb(i), 3 * nbk[i], 1, U, 3 * nbk[i], LR.as<double2>() + lo[q], K[i], K[i], kk,
+                                   3 * nbk[i]);
+                    add_gemm_tiles(jobs, Vt, 3 * nbk[j], 1, Cb(j), 3 * nbk[j], LR.as<double2>() + ro[q], kk, kk, K[j],
+                                   3 * nbk[j]);
+                }
+                TRY(run_ggemm(jobs, djobs, st));
+                jobs.clear();
+                for (size_t q = 0; q < ajobs.size(); ++q) {
+                    const int i = pi_[q], j = pj_[q], kk = std::min(hk[q], kmax);
+                    if (kk >= kmax) continue;
+                    add_gemm_tiles(jobs, LR.as<double2>() + lo[q], K[i], 0, LR.as<double2>() + ro[q], kk,
+                                   zc_ + offs[i] + offs[j] * Kt, (int)Kt, K[i], K[j], kk);
+                }
+                TRY(run_ggemm(jobs, djobs, st));
+                // not rank-deficient enough: dense blocks (Compute_EFields_CBFME_ACA.f90:609-620)
+                for (size_t q = 0; q < ajobs.size(); ++q)
+                    if (std::min(hk[q], kmax) >= kmax) TRY(dense_pair(pi_[q], pj_[q]));
+                MIDAS_HIP(hipStreamSynchronize(st));   // U, V, L, R are freed with this batch
             }
             j0 = j1;
         }
     }
-
     phase(use_aca ? "reduced matrix (ACA)" : "reduced matrix (dense)");
     // ---- 3. excitation, reduced solve, fields
     DBuf Einc, Vc, Al, dE;
