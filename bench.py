@@ -79,6 +79,92 @@ def cpu_baseline(full_cells, sample_cells, full_s_n3):
             "fill_s": fill_s, "solve_s": solve_s}
 
 
+def cbfm_cpu_baseline(s, th_c, ph_c, nsample_blocks=3, nsample_pairs=6):
+    """The oracle's CBFM-E + ACA on the host, timed per unit on a sample and
+    scaled: extended-block solve + SVD (x number of blocks), ACA of an
+    off-diagonal pair with its products (x number of pairs), and the reduced
+    dense solve at its full size (numpy/OpenBLAS zgesv)."""
+    import numpy as np
+    from oracle import oracle as o
+    oc = o.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    nb = len(s.blocks)
+    t0 = time.perf_counter()
+    ks = []
+    for b in range(nsample_blocks):
+        idx = np.concatenate([s.blocks[b], s.ext[b]])
+        cb = oc.subset(idx)
+        f, _ = o.setfsr(cb)
+        _, _, Zb = o.sr_block(cb, f)
+        Eb = np.linalg.solve(Zb, o.incident_field_f32(cb, th_c, ph_c))
+        sv = np.linalg.svd(Eb, compute_uv=False)
+        ks.append(int(np.sum(sv / sv[0] >= o.cbf_threshold(s.eps, True))))
+    t_block = (time.perf_counter() - t0) / nsample_blocks
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    K = int(np.mean(ks))
+    for _ in range(nsample_pairs):
+        i, j = rng.choice(nb, 2, replace=False)
+        bi, bj = s.blocks[i], s.blocks[j]
+        k, U, V, _, _ = o.aca(oc, int(bi[0]), len(bi), int(bj[0]), len(bj))
+        Ci = rng.normal(size=(3 * len(bi), K)) + 0j
+        Cj = rng.normal(size=(3 * len(bj), K)) + 0j
+        Ci.T @ (U @ (V @ Cj))
+    t_pair = (time.perf_counter() - t0) / nsample_pairs
+    kt = K * nb
+    A = np.random.default_rng(1).normal(size=(kt, kt)) + 1j * np.eye(kt)
+    t0 = time.perf_counter()
+    np.linalg.solve(A, np.ones((kt, 2), complex))
+    t_red = time.perf_counter() - t0
+    total = t_block * nb + t_pair * nb * (nb - 1) + t_red
+    return {"value": total, "unit": "s", "cores": os.cpu_count(), "kind": "port",
+            "sample": f"{nsample_blocks} extended blocks (SR mask + zgesv + SVD, {t_block:.2f} s each) x {nb}; "
+                      f"{nsample_pairs} ACA pairs ({t_pair:.3f} s each) x {nb * (nb - 1)}; reduced zgesv at "
+                      f"K = {kt} ({t_red:.2f} s)"}
+
+
+def run_cbfm(args):
+    """C3: DLCA aggregate, 64 monomers of ~300 cells, CBFM-E with SR-sparsified
+    block solves and ACA (eps 1e-4, kmax 50) between blocks, reduced system by
+    the mixed-precision solver, 422 CBF directions on the reference's uniform
+    grid, one plane wave."""
+    import numpy as np
+    from midas_amd import api, geometry as g
+    s = g.dlca_aggregate(nmono=64, cells_per_mono=300, seed=3)
+    th_c, ph_c = g.cbf_directions_uniform(422)   # the reference's uniform grid: 435 directions
+    tth, tph = np.array([0.0]), np.array([0.0])
+    mode = MODES[args.mode]
+    m = api.Midas(0)
+    run = lambda: m.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True, sr=True,
+                                          res_sr=0.01, mode=mode)
+    for _ in range(args.warmup):
+        run()
+    ts = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        E, K, info = run()
+        ts.append(time.perf_counter() - t0)
+        log(f"[bench] cbfm step {k + 1}: {ts[-1]:.3f} s, K = {int(np.sum(K))}")
+    E64, _, _ = m.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True, sr=True,
+                                        res_sr=0.01, mode=0)
+    step_s = float(np.mean(ts))
+    res = {"metric": "CBFM-E+ACA wall-s (C3)", "value": round(step_s, 4), "unit": "s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": f"f32-quirk blocks, f64 LU/SVD/ACA, {MODE_DTYPE[args.mode]} reduced LU + f64 IR",
+           "data": "synthetic (seeded DLCA aggregate)",
+           "config": {"workload": f"C3 DLCA 64 monomers, {s.n} cells, {3 * s.n} unknowns, SR blocks, ACA 1e-4, "
+                                  f"{len(th_c)} CBF directions, 1 plane wave", "solver": args.mode,
+                      "parallelism": "1 GPU"},
+           "K_total": int(np.sum(K)), "reduced_iterations": info["iterations"],
+           "rel_field_err_vs_fp64_reduced": float(np.linalg.norm(E - E64) / np.linalg.norm(E64))}
+    if not args.no_cpu:
+        log("[bench] CPU baseline sample (CBFM)")
+        res["cpu_baseline"] = cbfm_cpu_baseline(s, th_c, ph_c)
+        res["speedup_vs_cpu"] = round(res["cpu_baseline"]["value"] / step_s, 1)
+    m.close()
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +176,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample-cells", type=int, default=3500)
     ap.add_argument("--nb", type=int, default=0, help="LU panel width (0: library default)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
+                    help="c2: MoM fill + solve (default); c3: CBFM-E + ACA on a DLCA aggregate")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device to device (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -97,6 +185,8 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "c3":
+        return run_cbfm(args)
 
     import numpy as np
     import torch

@@ -611,21 +611,55 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
             j0 = j1;
         }
     } else {
-        // dense products C_i^T Z_ij C_j for the diagonal and for ACA fallbacks
-        DBuf Zij, Mi;
-        TRY(Zij.alloc(sizeof(double2) * 9L * nmax * nmax));
-        TRY(Mi.alloc(sizeof(double2) * (size_t)std::max(kbmax, 1) * 3 * nmax));
-        auto dense_pair = [&](int i, int j) -> int {
-            TRY(launch_fill(dc.v, CellSet{nbk[i], bs[i], nullptr}, CellSet{nbk[j], bs[j], nullptr}, k0, quirks, ref32,
-                            Zij.as<double2>(), 3L * nbk[i], nullptr, 0, st));
-            TRY(launch_cgemm<double>(K[i], 3 * nbk[j], 3 * nbk[i], ONE, Cb(i), 3L * nbk[i], 1, Zij.p, 3L * nbk[i], 0,
-                                     ZERO, Mi.p, K[i], st));
-            TRY(launch_cgemm<double>(K[i], K[j], 3 * nbk[j], ONE, Mi.p, K[i], 0, Cb(j), 3L * nbk[j], 0, ZERO,
-                                     zc_ + offs[i] + offs[j] * Kt, Kt, st));
+        // dense products C_i^T Z_ij C_j for the diagonal and for ACA fallbacks:
+        // the blocks Z_ij are filled side by side, then M = C_i^T Z_ij and
+        // Zc_ij = M C_j for all of them in two grouped launches
+        std::vector<std::pair<int, int>> dense;
+        auto dense_pairs = [&]() -> int {
+            const long zbudget = 4L << 30;
+            size_t q0 = 0;
+            while (q0 < dense.size()) {
+                size_t q1 = q0;
+                long zused = 0, mused = 0;
+                std::vector<long> zo, mo;
+                while (q1 < dense.size()) {
+                    const int i = dense[q1].first, j = dense[q1].second;
+                    const long zb = 9L * nbk[i] * nbk[j];
+                    if (q1 > q0 && (zused + zb) * (long)sizeof(double2) > zbudget) break;
+                    zo.push_back(zused);
+                    mo.push_back(mused);
+                    zused += zb;
+                    mused += (long)K[i] * 3 * nbk[j];
+                    ++q1;
+                }
+                DBuf Zd, Md;
+                TRY(Zd.alloc(sizeof(double2) * zused));
+                TRY(Md.alloc(sizeof(double2) * std::max(mused, 1L)));
+                jobs.clear();
+                for (size_t q = q0; q < q1; ++q) {
+                    const int i = dense[q].first, j = dense[q].second;
+                    double2 *Zq = Zd.as<double2>() + zo[q - q0];
+                    TRY(launch_fill(dc.v, CellSet{nbk[i], bs[i], nullptr}, CellSet{nbk[j], bs[j], nullptr}, k0, quirks,
+                                    ref32, Zq, 3L * nbk[i], nullptr, 0, st));
+                    add_gemm_tiles(jobs, Cb(i), 3 * nbk[i], 1, Zq, 3 * nbk[i], Md.as<double2>() + mo[q - q0], K[i],
+                                   K[i], 3 * nbk[j], 3 * nbk[i]);
+                }
+                TRY(run_ggemm(jobs, djobs, st));
+                jobs.clear();
+                for (size_t q = q0; q < q1; ++q) {
+                    const int i = dense[q].first, j = dense[q].second;
+                    add_gemm_tiles(jobs, Md.as<double2>() + mo[q - q0], K[i], 0, Cb(j), 3 * nbk[j],
+                                   zc_ + offs[i] + offs[j] * Kt, (int)Kt, K[i], K[j], 3 * nbk[j]);
+                }
+                TRY(run_ggemm(jobs, djobs, st));
+                q0 = q1;
+            }
+            dense.clear();
             return 0;
         };
         for (int j = 0; j < nb; ++j)
-            if (K[j]) TRY(dense_pair(j, j));
+            if (K[j]) dense.push_back({j, j});
+        TRY(dense_pairs());
         // off-diagonal pairs by ACA, a batch of block columns per launch; then
         // Zc_ij = (C_i^T U)(V C_j) through two grouped launches
         const double budget = 4.0e9;   // bytes of U, Vt and the small factors per batch
@@ -700,7 +734,8 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                 TRY(run_ggemm(jobs, djobs, st));
                 // not rank-deficient enough: dense blocks (Compute_EFields_CBFME_ACA.f90:609-620)
                 for (size_t q = 0; q < ajobs.size(); ++q)
-                    if (std::min(hk[q], kmax) >= kmax) TRY(dense_pair(pi_[q], pj_[q]));
+                    if (std::min(hk[q], kmax) >= kmax) dense.push_back({pi_[q], pj_[q]});
+                TRY(dense_pairs());
                 MIDAS_HIP(hipStreamSynchronize(st));   // U, V, L, R are freed with this batch
             }
             j0 = j1;

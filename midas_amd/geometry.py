@@ -93,6 +93,24 @@ def directions(n):
     return th, ph
 
 
+def cbf_directions_uniform(niws):
+    """CBF excitation directions of getTransmitters_CBFM with distr_iws = 1
+    (getParameters_CBFM.f90:1-110): the grid uniform in cos(theta) and phi whose
+    size is closest to `niws`, phi outer, theta inner, degrees, theta from +x.
+    The grid-size expression is single precision in the reference (integer
+    and default-real operands) and pi is the single-precision acos(-1.)."""
+    f32 = np.float32
+    delta = float(f32(f32(540.0) ** f32(2.0)) + f32(4 * (niws - 1) * (360 * 180)))
+    step = (540 + math.sqrt(delta)) / (2 * (niws - 1))
+    nth = int(np.rint(180.0 / step)) + 1
+    nph = int(np.rint(360.0 / step)) + 1
+    pq = float(f32(math.pi))
+    c0 = math.cos(0.0)
+    step_th = (math.cos(180.0 * pq / 180.0) - c0) / (nth - 1)
+    th = np.degrees(np.arccos(c0 + step_th * np.arange(nth))) * (math.pi / pq)
+    return np.tile(th, nph), np.repeat(np.arange(nph) * (360.0 / (nph - 1)), nth)
+
+
 def _lattice(extent, sc):
     g = np.arange(-extent, extent + 1e-12, sc)
     g = g[:-1] + sc / 2 if len(g) > 1 else g

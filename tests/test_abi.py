@@ -121,3 +121,14 @@ def test_fortran_dropin_compiles_against_reference_modules(tmp_path):
         r = subprocess.run([fc, "-c", f"-I{ref}", f"-I{tmp_path}", os.path.join(ROOT, "midas_amd", "fortran", f)],
                            cwd=tmp_path, capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_cbf_direction_grid_matches_reference(oracle):
+    """The package's CBF direction grid equals the reference's getTransmitters_CBFM
+    grid (pinned in the oracle tests against the compiled routine)."""
+    from midas_amd import geometry as g
+    for ni in (94, 200, 422):
+        th, ph = g.cbf_directions_uniform(ni)
+        tho, pho = oracle.transmitters_un(ni)
+        assert len(th) == len(tho)
+        assert np.abs(th - tho).max() < 1e-12 and np.abs(ph - pho).max() < 1e-12
