@@ -62,18 +62,6 @@ __global__ void k_copyz(const double2 *S, long lds, double2 *D, long ldd, int m,
     int c = blockIdx.y;
     if (r < m && c < n) D[(long)r + (long)c * ldd] = S[(long)r + (long)c * lds];
 }
-// ||Z||_inf: max over rows of sum |Z_ij| (one thread per row, coalesced along rows)
-__global__ void k_rowabs(const double2 *Z, long ldz, int n, double *rowsum)
-{
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    double s = 0.0;
-    for (int c = 0; c < n; ++c) {
-        double2 v = Z[(long)r + (long)c * ldz];
-        s += sqrt(v.x * v.x + v.y * v.y);
-    }
-    rowsum[r] = s;
-}
 // per-column max modulus of a (m x n) block -> out[c]
 __global__ void k_colmax(const double2 *X, long ldx, int m, double *out)
 {
@@ -144,8 +132,11 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
     double *rowsum = norms.as<double>();
     double *colx = rowsum + n, *colr = colx + nrhs, *scal = colr + nrhs;
 
-    // ||Z||_inf before factoring (PZLANGE 'I')
-    hipLaunchKernelGGL(k_rowabs, dim3((n + 255) / 256), dim3(256), 0, st, Z, ldz, n, rowsum);
+    // ||Z||_inf before factoring (PZLANGE 'I'); the scratch also serves the
+    // few-RHS residuals of refinement
+    DBuf zx;
+    TRY(zx.alloc(zx_scratch_bytes(n, nrhs <= 4 ? nrhs : 1)));
+    TRY(launch_rowabs(Z, ldz, n, rowsum, zx.p, st));
     hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, st, rowsum, n, scal);
     MIDAS_HIP(hipGetLastError());
     MIDAS_HIP(hipMemcpyAsync(&inf.anorm, scal, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -220,8 +211,12 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
             r = getrs<float>(A, lda, n, perm_d.as<int>(), w, D.as<float2>(), (long)n, nrhs, t2.as<float2>(), st);
             if (r) { w.release(); return r; }
             hipLaunchKernelGGL(k_c2z_add, g2, dim3(256), 0, st, D.as<float2>(), (long)n, X, ldx, n, nrhs, it > 0);
-            hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R.as<double2>(), (long)n, n, nrhs);
-            r = launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R.p, n, st);
+            if (nrhs <= 4) {
+                r = launch_residual_few(Z, ldz, n, X, ldx, nrhs, B, ldb, R.as<double2>(), n, zx.p, st);
+            } else {
+                hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R.as<double2>(), (long)n, n, nrhs);
+                r = launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R.p, n, st);
+            }
             if (r) { w.release(); return r; }
             hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, X, ldx, n, colx);
             hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, R.as<double2>(), (long)n, n, colr);

@@ -168,6 +168,14 @@ struct Timer {
     }
 };
 
+// One-pass kernels over the FP64 matrix (midas_resid.hip)
+namespace midas {
+size_t zx_scratch_bytes(int n, int nrhs);
+int launch_residual_few(const double2 *Z, long ldz, int n, const double2 *X, long ldx, int nrhs, const double2 *B,
+                        long ldb, double2 *R, long ldr, void *scratch, hipStream_t st);
+int launch_rowabs(const double2 *Z, long ldz, int n, double *rowsum, void *scratch, hipStream_t st);
+}  // namespace midas
+
 // Dense solve on the device (FP64 LU or mixed-precision LU + FP64 refinement):
 // Z (n x n, ldz) FP64; W32 optional FP32 copy of Z for mixed modes; X = Z^-1 B.
 int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32, long ldw, const double2 *B,
