@@ -57,13 +57,6 @@ template <> __device__ inline f16v mma32<true>(s8 a, s8 b, f16v c)
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, a), __builtin_bit_cast(b8, b), c, 0, 0, 0);
 }
 
-__device__ inline s8 neg8(s8 v)
-{
-    typedef unsigned u4 __attribute__((ext_vector_type(4)));
-    u4 u = __builtin_bit_cast(u4, v);
-    u ^= (u4){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-    return __builtin_bit_cast(s8, u);
-}
 
 // plane element offset (in halves): plane pc, row, k with tile-major order
 __host__ __device__ inline long plane_off(int pc, int row, int k, int rtiles, int kst)
@@ -208,7 +201,9 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
     const int nst = kst / SPS;
 
-    f16v accr[2][2], acci[2][2];
+    // real part as two accumulators (Br Ar and Bi Ai, subtracted in the
+    // epilogue), so no operand needs a sign flip in the inner loop
+    f16v accr[2][2], acci[2][2], accn[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -217,6 +212,7 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
             for (int q = 0; q < 16; ++q) {
                 accr[a][b][q] = 0.f;
                 acci[a][b][q] = 0.f;
+                accn[a][b][q] = 0.f;
             }
 
     // One wave copies 1 KB (64 lanes x 16 B) of a plane tile into LDS at
@@ -250,15 +246,11 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
         for (int g = 0; g < SPS; ++g) {
             // MFMA row operand from B (rows = columns of C), column operand from A;
             // B fragments for the whole granule, A fragments one row tile at a time
-            s8 fb[NPL][2], fbn[P][2];
+            s8 fb[NPL][2];
 #pragma unroll
             for (int pc = 0; pc < NPL; ++pc)
 #pragma unroll
                 for (int t = 0; t < 2; ++t) fb[pc][t] = *(const s8 *)&src->b[pc][g][wn + 32 * t + r32][kc];
-#pragma unroll
-            for (int p = 0; p < P; ++p)
-#pragma unroll
-                for (int t = 0; t < 2; ++t) fbn[p][t] = neg8(fb[2 * p + 1][t]);
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
                 s8 fa[NPL];
@@ -269,7 +261,7 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
                 for (int tp = 0; tp < TERMS; ++tp) {
                     const int pb = (tp == 2) ? 1 : 0;   // B plane set (hi / lo)
                     const int pa = (tp == 1) ? 1 : 0;   // A plane set
-                    // C^T: Dr = Br^T Ar^T - Bi^T Ai^T, Di = Br^T Ai^T + Bi^T Ar^T
+                    // C^T: Dr = Br^T Ar^T - Bi^T Ai^T (accr - accn), Di = Br^T Ai^T + Bi^T Ar^T
 #pragma unroll
                     for (int b = 0; b < 2; ++b) {
                         accr[a][b] = mma32<BF>(fb[2 * pb][b], fa[2 * pa], accr[a][b]);
@@ -277,7 +269,7 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
                     }
 #pragma unroll
                     for (int b = 0; b < 2; ++b) {
-                        accr[a][b] = mma32<BF>(fbn[pb][b], fa[2 * pa + 1], accr[a][b]);
+                        accn[a][b] = mma32<BF>(fb[2 * pb + 1][b], fa[2 * pa + 1], accn[a][b]);
                         acci[a][b] = mma32<BF>(fb[2 * pb + 1][b], fa[2 * pa], acci[a][b]);
                     }
                 }
@@ -302,7 +294,27 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
     dma(s0, 0);
     dma(s1, 1);
     dma(s2, 2);
-    for (int s = 0; s < nst; s += NS) {
+    for (int s = 0; s < nst - NS; s += NS) {
+        step(s, s0, s3);
+        step(s + 1, s1, s0);
+        step(s + 2, s2, s1);
+        step(s + 3, s3, s2);
+    }
+    // C of the first row tile is loaded ahead of the last ring pass, so its
+    // latency overlaps the last MFMAs
+    double pre[2][16];
+    {
+        const int row = m0 + wm + (lane & 31);
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int col = n0 + wn + 32 * b + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                pre[b][q] = (row < m && col < n)
+                                ? __builtin_nontemporal_load((const double *)(C + (long)row + (long)col * ldc))
+                                : 0.0;
+            }
+        const int s = nst - NS;
         step(s, s0, s3);
         step(s + 1, s1, s0);
         step(s + 2, s2, s1);
@@ -334,9 +346,11 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const int col = n0 + wn + 32 * b + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                cv[b][q] = (row < m && col < n)
-                               ? __builtin_nontemporal_load((const double *)(C + (long)row + (long)col * ldc))
-                               : 0.0;
+                if (a == 0) cv[b][q] = pre[b][q];
+                else
+                    cv[b][q] = (row < m && col < n)
+                                   ? __builtin_nontemporal_load((const double *)(C + (long)row + (long)col * ldc))
+                                   : 0.0;
             }
 #pragma unroll
         for (int b = 0; b < 2; ++b)
@@ -345,7 +359,8 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
                 const int col = n0 + wn + 32 * b + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
                 if (row < m && col < n) {
                     float2 c0 = __builtin_bit_cast(float2, cv[b][q]);
-                    float2 c1 = float2{c0.x - accr[a][b][q] * inv[b][q], c0.y - acci[a][b][q] * inv[b][q]};
+                    const float dr = accr[a][b][q] - accn[a][b][q];
+                    float2 c1 = float2{c0.x - dr * inv[b][q], c0.y - acci[a][b][q] * inv[b][q]};
                     __builtin_nontemporal_store(__builtin_bit_cast(double, c1), (double *)(C + (long)row + (long)col * ldc));
                 }
             }
