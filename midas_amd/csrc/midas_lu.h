@@ -11,10 +11,15 @@ template <typename T> struct LuWork {
     int batch = 1;                   // matrices factored together (same n), A stride sA
     long sA = 0;
     int jb = 0;                      // first column of the current inner block
-    T *pmax = nullptr;               // per-workgroup pivot candidates
+    T *pmax = nullptr;               // per-workgroup pivot candidates (two buffers)
     int *pidx = nullptr;
     int *counter = nullptr;          // last-workgroup counter
     int *info = nullptr;             // first exactly-zero pivot (1-based), 0 if none
+    unsigned *tag = nullptr;         // step tag per workgroup of the inner-block kernel, per matrix
+    int *fault = nullptr;            // set when an inner-block wait gave up
+    unsigned tag_base = 0;           // column steps issued so far (host count)
+    cplx<T> *cand = nullptr;         // candidate pivot rows, one per workgroup (two buffers)
+    cplx<T> *rowj = nullptr;         // row j before its interchange (two buffers)
     cplx<T> *pinv = nullptr;         // 1 / current pivot
     cplx<T> *linv = nullptr;         // inverses of the diagonal blocks of L (nb x nb each)
     cplx<T> *uinv = nullptr;         // inverses of the diagonal blocks of U

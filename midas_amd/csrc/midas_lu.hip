@@ -4,12 +4,14 @@
 // on one GPU: right-looking blocked LU, column-major, interleaved complex.
 //
 //   outer panel  nb columns : factored in inner blocks of IB columns;
-//   inner column step       : one fused kernel per column -- scale + rank-1 update
-//                             for the previous column, then the pivot search of
-//                             this column; the last workgroup to finish reduces
-//                             the per-workgroup maxima, swaps the pivot row across
-//                             the outer panel and records ipiv (LAPACK getf2 order);
-//   inner block update      : IBxIB unit-lower solve + GEMM on the rest of the panel;
+//   inner block             : one launch for its IB column steps (k_getf2_block):
+//                             rows held in registers, one grid barrier per column
+//                             for the pivot search (LAPACK getf2 order); then the
+//                             block's interchanges on the other panel columns and
+//                             the IBxIB unit-lower solve (k_swap_trsm);
+//                             (for matrices too tall for one co-resident grid:
+//                             one fused kernel per column, k_getf2_step)
+//   inner block update      : GEMM on the rest of the panel;
 //   end of a panel          : L11^-1 of its diagonal block (one wave per column);
 //   outside the panel       : row swaps (laswp), U12 = L11^-1 A12 as one GEMM with
 //                             that inverse, trailing GEMM A22 -= L21 U12.
@@ -155,6 +157,287 @@ __global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t
             pinvp[0] = cinv(dv);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Inner block [jb, jb+ncols) (ncols <= IB) in one launch.  Thread t of
+// workgroup b holds row jb + 256 b + t of the block's columns in registers for
+// all IB column steps.  Per column j:
+//   local argmax of |Re|+|Im| over rows >= j  ->  publish (max, row), the
+//   candidate row's IB values and, in workgroup 0, row j; wait until those
+//   stores are done, then publish the step tag of this workgroup
+//   wait until every workgroup's tag has reached this step
+//   every workgroup reduces the candidates (first row on ties, as i?amax),
+//   takes the pivot row from the winner's published copy, applies the
+//   interchange to the rows it holds, and updates its rows below j (scale by
+//   1/pivot, rank-1 update of the columns right of j).
+// All exchanged data goes through device-scope loads and stores, so no cache
+// write-back is needed; the tags grow across launches (host count `base`).
+// Published data is double-buffered by column parity: a workgroup can be at
+// most one step ahead of any reader.  The other panel columns are interchanged
+// afterwards by k_swap_trsm.  The FP32 kernel stays within 64 VGPRs, so its
+// waves fit next to the trailing GEMM's (448 registers per lane) instead of
+// displacing them.  Workgroups of a matrix wait on each other and must become
+// co-resident; a wait that runs out sets w.fault (reported by getrf) and every
+// wave still exits.
+// ---------------------------------------------------------------------------
+constexpr int BLK_T = 256;          // threads = rows per workgroup
+constexpr int BLK_MAXWG = 1024;     // workgroups per matrix that may wait on each other
+constexpr int BLK_SPIN = 1 << 21;   // polls before a wait gives up (about two seconds)
+
+// 4/8/16-byte loads and stores that are coherent across the XCDs (device-scope
+// relaxed atomics)
+template <typename X> __device__ inline X ld_coh(const X *p)
+{
+    if constexpr (sizeof(X) == 4) {
+        return __builtin_bit_cast(X, __hip_atomic_load((unsigned *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    } else if constexpr (sizeof(X) == 8) {
+        return __builtin_bit_cast(
+            X, __hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    } else {
+        static_assert(sizeof(X) == 16, "ld_coh: 4, 8 or 16 bytes");
+        struct P2 { unsigned long long a, b; } v;
+        v.a = __hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v.b = __hip_atomic_load((unsigned long long *)p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return __builtin_bit_cast(X, v);
+    }
+}
+template <typename X> __device__ inline void st_coh(X *p, X x)
+{
+    if constexpr (sizeof(X) == 4) {
+        __hip_atomic_store((unsigned *)p, __builtin_bit_cast(unsigned, x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if constexpr (sizeof(X) == 8) {
+        __hip_atomic_store((unsigned long long *)p, __builtin_bit_cast(unsigned long long, x), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        static_assert(sizeof(X) == 16, "st_coh: 4, 8 or 16 bytes");
+        struct P2 { unsigned long long a, b; } v = __builtin_bit_cast(P2, x);
+        __hip_atomic_store((unsigned long long *)p, v.a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store((unsigned long long *)p + 1, v.b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+// wait until this wave's stores have completed (device-scope stores are then
+// visible to every workgroup)
+__device__ inline void stores_done() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+template <typename T> __device__ inline bool pivot_better(T v, int i, T bv, int bi)
+{
+    return v > bv || (v == bv && i < bi);
+}
+
+// argmax over the workgroup; every thread gets the result.
+template <typename T> __device__ inline void argmax_block(T &v, int &i, T *rv, int *ri)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const T ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(i, off, 64);
+        if (pivot_better(ov, oi, v, i)) { v = ov; i = oi; }
+    }
+    if ((threadIdx.x & 63) == 0) { rv[threadIdx.x >> 6] = v; ri[threadIdx.x >> 6] = i; }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BLK_T / 64; ++q)
+        if (pivot_better(rv[q], ri[q], v, i)) { v = rv[q]; i = ri[q]; }
+    __syncthreads();
+}
+
+template <typename T, int WPE>
+__global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t *A, long lda, int n, int jb,
+                                                            int ncols, int *ipiv, LuWork<T> w, unsigned base)
+{
+    typedef typename cvec<T>::t V;
+    __shared__ V urow[LU_IB];
+    __shared__ T rv[BLK_T / 64];
+    __shared__ int ri[BLK_T / 64];
+    const int tid = threadIdx.x, b = blockIdx.x, nwg = gridDim.x, zb = blockIdx.y;
+    A += zb * w.sA;
+    ipiv += (long)zb * n;
+    V *cand = (V *)w.cand + (long)zb * 2 * w.maxblk * LU_IB;
+    V *rowj = (V *)w.rowj + (long)zb * 2 * LU_IB;
+    T *bmax = w.pmax + (long)zb * 2 * w.maxblk;
+    int *bidx = w.pidx + (long)zb * 2 * w.maxblk;
+    unsigned *tag = w.tag + (long)zb * w.maxblk;
+    const int r0 = jb + b * BLK_T, i = r0 + tid;
+    const bool mine = i < n;
+    bool waited_out = false;
+
+    // a[t]: this thread's row.  The block's columns rotate one place left after
+    // every step, so the current column is always t = 0 and every register
+    // index is a constant; after IB steps they are back in place.
+    V *Ab = A + r0 + (long)jb * lda;   // uniform base: per-thread offsets stay 32-bit
+    V a[LU_IB];
+#pragma unroll
+    for (int t = 0; t < LU_IB; ++t) a[t] = (mine && t < ncols) ? Ab[(long)t * lda + tid] : V{0, 0};
+
+    for (int c = 0; c < LU_IB; ++c) {
+        if (c < ncols) {
+            const int j = jb + c, par = c & 1;
+            const unsigned want = base + (unsigned)c + 1u;
+            // ---- local candidate, published -----------------------------------
+            T best = (T)-1;
+            int bi = 0x7fffffff;
+            if (mine && i >= j) {
+                best = fabs(a[0].x) + fabs(a[0].y);
+                bi = i;
+            }
+            argmax_block(best, bi, rv, ri);
+            bi = __builtin_amdgcn_readfirstlane(bi);   // uniform: keeps address math scalar
+            if (tid == 0) {
+                st_coh(bmax + par * w.maxblk + b, best);
+                st_coh(bidx + par * w.maxblk + b, bi);
+            }
+            if (bi == i) {
+#pragma unroll
+                for (int t = 0; t < LU_IB; ++t) st_coh(cand + ((long)par * w.maxblk + b) * LU_IB + t, a[t]);
+            }
+            // row j lies in workgroup 0, thread c
+            if (b == 0 && tid == c) {
+#pragma unroll
+                for (int t = 0; t < LU_IB; ++t) st_coh(rowj + par * LU_IB + t, a[t]);
+            }
+            stores_done();
+            __syncthreads();
+            if (tid == 0) st_coh(tag + b, want);
+            // ---- wait for every workgroup of this matrix ----------------------
+            for (int q = tid; q < nwg; q += BLK_T) {
+                int spins = 0;
+                while (!waited_out && (int)(ld_coh(tag + q) - want) < 0) {
+                    if (++spins > BLK_SPIN) {
+                        st_coh(w.fault, 1);
+                        waited_out = true;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __syncthreads();
+            // ---- global pivot -------------------------------------------------
+            T gv = (T)-1;
+            int gi = 0x7fffffff;
+            for (int q = tid; q < nwg; q += BLK_T) {
+                const T v = ld_coh(bmax + par * w.maxblk + q);
+                const int vi = ld_coh(bidx + par * w.maxblk + q);
+                if (pivot_better(v, vi, gv, gi)) { gv = v; gi = vi; }
+            }
+            argmax_block(gv, gi, rv, ri);
+            const int p = __builtin_amdgcn_readfirstlane(gi == 0x7fffffff ? j : gi);
+            if (tid < LU_IB)
+                urow[tid] = p == j ? ld_coh(rowj + par * LU_IB + tid)
+                                   : ld_coh(cand + ((long)par * w.maxblk + (p - jb) / BLK_T) * LU_IB + tid);
+            __syncthreads();
+            if (b == 0 && tid == 0) ipiv[j] = p;
+            if (p != j) {
+                // row p takes row j's old values, row j the pivot row
+                if (i == p) {
+#pragma unroll
+                    for (int t = 0; t < LU_IB; ++t) a[t] = ld_coh(rowj + par * LU_IB + t);
+                }
+                if (i == j) {
+#pragma unroll
+                    for (int t = 0; t < LU_IB; ++t) a[t] = urow[t];
+                }
+            }
+            // ---- scale and rank-1 update of the rows below j ------------------
+            if (mine && i > j) {
+                const V d = urow[0];
+                cplx<T> pinv = {(T)0, (T)0};
+                if (d.x != (T)0 || d.y != (T)0) pinv = cinv(cplx<T>{d.x, d.y});
+                const cplx<T> l = cmul(cplx<T>{a[0].x, a[0].y}, pinv);
+                a[0] = V{l.r, l.i};
+                const int nright = ncols - c;   // positions 1 .. nright-1: the columns right of j
+#pragma unroll
+                for (int t = 1; t < LU_IB; ++t)
+                    if (t < nright) {
+                        const cplx<T> r = csub(cplx<T>{a[t].x, a[t].y}, cmul(l, cplx<T>{urow[t].x, urow[t].y}));
+                        a[t] = V{r.r, r.i};
+                    }
+            }
+            // exactly singular: record the first zero pivot (LAPACK INFO), keep going
+            if (b == 0 && tid == 0 && urow[0].x == (T)0 && urow[0].y == (T)0) atomicCAS(w.info + zb, 0, j + 1);
+            __syncthreads();
+        }
+        // rotate left by one column
+        const V a0 = a[0];
+#pragma unroll
+        for (int t = 0; t < LU_IB - 1; ++t) a[t] = a[t + 1];
+        a[LU_IB - 1] = a0;
+    }
+
+    if (mine) {
+#pragma unroll
+        for (int t = 0; t < LU_IB; ++t)
+            if (t < ncols) Ab[(long)t * lda + tid] = a[t];
+    }
+}
+
+// Interchanges ipiv[jb, jend) on panel columns [k, jb) and [jend, k1); then,
+// on [jend, k1), forward substitution with the unit-lower block
+// L(jb:jend, jb:jend) on rows [jb, jend).  One thread per column.  The
+// interchanges are composed first into at most 2 IB (position, source row)
+// pairs, so each column issues all its loads before its stores.
+template <typename T>
+__global__ __launch_bounds__(64) void k_swap_trsm(typename cvec<T>::t *A, long lda, long sA, const int *ipiv, int n,
+                                                  int jb, int jend, int k, int k1)
+{
+    typedef typename cvec<T>::t V;
+    __shared__ int pos[2 * LU_IB], src[2 * LU_IB];
+    __shared__ int npair;
+    A += blockIdx.y * sA;
+    ipiv += (long)blockIdx.y * n;
+    if (threadIdx.x == 0) {
+        int m = 0;
+        for (int j = jb; j < jend; ++j) {
+            const int p = ipiv[j];
+            if (p == j) continue;
+            int sj = -1, sp = -1;
+            for (int s = 0; s < m; ++s) {
+                if (pos[s] == j) sj = s;
+                if (pos[s] == p) sp = s;
+            }
+            if (sj < 0) { pos[m] = j; src[m] = j; sj = m++; }
+            if (sp < 0) { pos[m] = p; src[m] = p; sp = m++; }
+            const int t = src[sj];
+            src[sj] = src[sp];
+            src[sp] = t;
+        }
+        npair = m;
+    }
+    __syncthreads();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nl = jb - k;
+    if (t >= nl + (k1 - jend)) return;
+    const int c = t < nl ? k + t : jend + (t - nl);
+    V *col = A + (long)c * lda;
+    const int m = npair;
+    cplx<T> v[2 * LU_IB];   // (a plain struct: an array of double2 is not kept in registers)
+#pragma unroll
+    for (int s = 0; s < 2 * LU_IB; ++s)
+        if (s < m) {
+            const V x = col[src[s]];
+            v[s] = cplx<T>{x.x, x.y};
+        }
+#pragma unroll
+    for (int s = 0; s < 2 * LU_IB; ++s)
+        if (s < m) col[pos[s]] = V{v[s].r, v[s].i};
+    if (c < jend) return;
+    const int ib = jend - jb;
+    const V *L = A + (long)jb + (long)jb * lda;
+    cplx<T> x[LU_IB];
+#pragma unroll
+    for (int r = 0; r < LU_IB; ++r)
+        if (r < ib) x[r] = cplx<T>{col[jb + r].x, col[jb + r].y};
+#pragma unroll
+    for (int r = 1; r < LU_IB; ++r) {
+        if (r >= ib) continue;
+#pragma unroll
+        for (int q = 0; q < r; ++q) {
+            const V l = L[(long)r + (long)q * lda];
+            x[r] = csub(x[r], cmul(cplx<T>{l.x, l.y}, x[q]));
+        }
+    }
+#pragma unroll
+    for (int r = 1; r < LU_IB; ++r)
+        if (r < ib) col[jb + r] = V{x[r].r, x[r].i};
 }
 
 // Forward substitution with the unit-lower IBxIB block L(jb:jend, jb:jend) on
@@ -319,23 +602,46 @@ template <typename T> static cplx<T> czero() { return {(T)0, (T)0}; }
 template <typename T>
 static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ipiv, LuWork<T> &w, hipStream_t st)
 {
+    // one launch per inner block while the workgroups of a matrix fit on the
+    // device together, else one launch per column (MIDAS_LU_COLUMN_STEPS=1
+    // forces the latter, so that both stay tested)
+    const char *cs_env = getenv("MIDAS_LU_COLUMN_STEPS");
+    const bool column_steps = cs_env && atoi(cs_env) != 0;
+    const bool fused = !column_steps && (n - k + BLK_T - 1) / BLK_T <= BLK_MAXWG;
     for (int jb = k; jb < k + nb; jb += LU_IB) {
         int jend = min(jb + LU_IB, k + nb);
-        w.jb = jb;
-        for (int j = jb; j <= jend; ++j) {
-            int rows = n - j;
-            int nblk = rows > 0 ? (rows + STEP_ROWS - 1) / STEP_ROWS : 1;
-            MIDAS_CHECK(nblk <= w.maxblk, -2, "lu: step workspace too small");
-            hipLaunchKernelGGL(k_getf2_step<T>, dim3(nblk, w.batch), dim3(STEP_THREADS), 0, st, A, lda, n, j,
-                               j < jend ? jend : j, k, k + nb, ipiv, w);
-        }
-        MIDAS_HIP(hipGetLastError());
-        // rest of the outer panel: columns [jend, k+nb)
         int rc = k + nb - jend;
-        if (rc > 0) {
-            hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((rc + 63) / 64, w.batch), dim3(64), 0, st, A, lda, w.sA, jb,
-                               jend - jb, jend, k + nb);
+        if (fused) {
+            const int nblk = (n - jb + BLK_T - 1) / BLK_T;
+            MIDAS_CHECK(nblk <= w.maxblk, -2, "lu: step workspace too small");
+            // FP32: at most 64 VGPRs (8 waves per SIMD), see k_getf2_block
+            constexpr int wpe = sizeof(T) == 4 ? 8 : 4;
+            hipLaunchKernelGGL((k_getf2_block<T, wpe>), dim3(nblk, w.batch), dim3(BLK_T), 0, st, A, lda, n, jb,
+                               jend - jb, ipiv, w, w.tag_base);
+            w.tag_base += (unsigned)(jend - jb);
+            const int nc = (jb - k) + rc;
+            if (nc > 0)
+                hipLaunchKernelGGL(k_swap_trsm<T>, dim3((nc + 63) / 64, w.batch), dim3(64), 0, st, A, lda, w.sA, ipiv,
+                                   n, jb, jend, k, k + nb);
             MIDAS_HIP(hipGetLastError());
+        } else {
+            w.jb = jb;
+            for (int j = jb; j <= jend; ++j) {
+                int rows = n - j;
+                int nblk = rows > 0 ? (rows + STEP_ROWS - 1) / STEP_ROWS : 1;
+                MIDAS_CHECK(nblk <= w.maxblk, -2, "lu: step workspace too small");
+                hipLaunchKernelGGL(k_getf2_step<T>, dim3(nblk, w.batch), dim3(STEP_THREADS), 0, st, A, lda, n, j,
+                                   j < jend ? jend : j, k, k + nb, ipiv, w);
+            }
+            MIDAS_HIP(hipGetLastError());
+            if (rc > 0) {
+                hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((rc + 63) / 64, w.batch), dim3(64), 0, st, A, lda, w.sA,
+                                   jb, jend - jb, jend, k + nb);
+                MIDAS_HIP(hipGetLastError());
+            }
+        }
+        // rest of the outer panel: columns [jend, k+nb)
+        if (rc > 0) {
             if (n - jend > 0) {
                 int r = launch_cgemm<T>(n - jend, rc, jend - jb, cmone<T>(), A + jend + (long)jb * lda, lda, 0,
                                         A + jb + (long)jend * lda, lda, 0, cone<T>(), A + jend + (long)jend * lda,
@@ -381,8 +687,9 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
     typedef typename cvec<T>::t V;
     hipStream_t ps = w.pstream;
     MIDAS_CHECK(w.batch == 1 || tg.batch == w.batch, -1, "getrf: trailing GEMM not batched like the factorisation");
-    MIDAS_HIP(hipMemsetAsync(w.counter, 0, sizeof(int) * w.batch, st));
-    MIDAS_HIP(hipMemsetAsync(w.info, 0, sizeof(int) * w.batch, st));
+    // last-workgroup counters, info, the fault flag and the step tags
+    MIDAS_HIP(hipMemsetAsync(w.counter, 0, sizeof(int) * (2 * w.batch + 1 + w.maxblk * w.batch), st));
+    w.tag_base = 0;
     MIDAS_HIP(hipEventRecord(w.ev_next, st));
     MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
     int r = panel<T>(A, lda, n, 0, min(w.nb, n), ipiv, w, ps);
@@ -431,6 +738,10 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
                             w.nb, st, w.batch, w.sA, w.inv_stride());
         if (r) return r;
     }
+    int fault = 0;
+    MIDAS_HIP(hipMemcpyAsync(&fault, w.fault, sizeof(int), hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    MIDAS_CHECK(fault == 0, -3, "lu: a panel wait timed out (workgroups not co-resident)");
     return 0;
 }
 
@@ -548,11 +859,15 @@ int LuWork<T>::alloc(int n_, int nb_, int batch_, long sA_)
     nb = nb_;
     batch = batch_;
     sA = sA_;
-    maxblk = (n + STEP_ROWS - 1) / STEP_ROWS + 1;
-    MIDAS_HIP(hipMalloc(&pmax, sizeof(T) * maxblk * batch));
-    MIDAS_HIP(hipMalloc(&pidx, sizeof(int) * maxblk * batch));
-    MIDAS_HIP(hipMalloc(&counter, sizeof(int) * 2 * batch));
+    maxblk = (n + BLK_T - 1) / BLK_T + 1;   // workgroups of either panel kernel
+    MIDAS_HIP(hipMalloc(&pmax, sizeof(T) * 2 * maxblk * batch));
+    MIDAS_HIP(hipMalloc(&pidx, sizeof(int) * 2 * maxblk * batch));
+    MIDAS_HIP(hipMalloc(&counter, sizeof(int) * (2 * batch + 1 + maxblk * batch)));
     info = counter + batch;
+    fault = counter + 2 * batch;
+    tag = (unsigned *)(counter + 2 * batch + 1);
+    MIDAS_HIP(hipMalloc(&cand, sizeof(cplx<T>) * 2 * maxblk * LU_IB * batch));
+    MIDAS_HIP(hipMalloc(&rowj, sizeof(cplx<T>) * 2 * LU_IB * batch));
     MIDAS_HIP(hipMalloc(&pinv, sizeof(cplx<T>) * batch));
     MIDAS_HIP(hipMalloc(&linv, sizeof(cplx<T>) * inv_stride() * batch));
     MIDAS_HIP(hipMalloc(&uinv, sizeof(cplx<T>) * inv_stride() * batch));
@@ -573,6 +888,8 @@ void LuWork<T>::release()
     hipFree(linv);
     hipFree(uinv);
     hipFree(u12);
+    hipFree(cand);
+    hipFree(rowj);
     if (pstream) {
         hipStreamSynchronize(pstream);
         hipStreamDestroy(pstream);
@@ -582,9 +899,10 @@ void LuWork<T>::release()
     pstream = nullptr;
     ev_panel = ev_next = nullptr;
     pmax = nullptr;
-    pidx = counter = info = nullptr;
+    pidx = counter = info = fault = nullptr;
+    tag = nullptr;
     pinv = nullptr;
-    linv = uinv = u12 = nullptr;
+    linv = uinv = u12 = cand = rowj = nullptr;
 }
 
 template struct LuWork<double>;

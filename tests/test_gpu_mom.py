@@ -123,6 +123,42 @@ def test_solve_fp64_random_pivoting(mid):
     assert np.linalg.norm(A @ X - B) / (np.linalg.norm(A) * np.linalg.norm(X)) < 1e-14
 
 
+@pytest.mark.parametrize("mode", [api.SOLVE_FP64, api.SOLVE_C32_IR])
+def test_lu_panel_paths_agree(mid, mode, monkeypatch):
+    """The one-launch-per-inner-block panel (grid barrier per column) and the
+    one-launch-per-column panel pick the same pivots: same solution, n over
+    three 512-row workgroups and not a multiple of the panel or inner block."""
+    rng = np.random.default_rng(11)
+    n = 1301
+    _case(f"lu panel paths {api.MODE_NAMES[mode]} n={n}")
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    B = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    X1, i1 = mid.solve(A, B, mode)
+    monkeypatch.setenv("MIDAS_LU_COLUMN_STEPS", "1")
+    X2, i2 = mid.solve(A, B, mode)
+    monkeypatch.delenv("MIDAS_LU_COLUMN_STEPS")
+    Xo = np.linalg.solve(A, B)
+    print("fused vs column-step", _rel(X1, X2), "vs numpy", _rel(X1, Xo), _rel(X2, Xo))
+    assert i1["info"] == 0 and i2["info"] == 0
+    assert _rel(X1, X2) < 1e-12
+    assert _rel(X1, Xo) < 1e-10
+
+
+@pytest.mark.parametrize("column_steps", ["0", "1"])
+def test_lu_exactly_singular(mid, column_steps, monkeypatch):
+    """An exactly zero column makes U(j, j) exactly zero: the factorisation
+    reports the first such j (1-based), as LAPACK ZGETRF INFO."""
+    rng = np.random.default_rng(12)
+    n = 600
+    _case(f"lu exactly singular column_steps={column_steps} n={n}")
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    A[:, 37] = 0.0
+    monkeypatch.setenv("MIDAS_LU_COLUMN_STEPS", column_steps)
+    _, info = mid.solve(A, np.ones((n, 1), complex), api.SOLVE_FP64)
+    monkeypatch.delenv("MIDAS_LU_COLUMN_STEPS")
+    assert info["info"] == 38
+
+
 @pytest.mark.parametrize("mode", [api.SOLVE_C32_IR, api.SOLVE_F16_IR, api.SOLVE_F16X3_IR, api.SOLVE_BF16_IR,
                                   api.SOLVE_BF16X3_IR])
 def test_solve_mixed(mid, oracle, mode):
