@@ -34,9 +34,17 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+# Per-file flags.  The split-precision GEMM sizes its LDS so that two of its
+# workgroups and one LU panel workgroup share a CU; the compiler's promotion of
+# private arrays into LDS would take that room.
+FILE_FLAGS = {
+    "midas_gemm16.hip": ["-mllvm", "-disable-promote-alloca-to-lds"],
+}
+
+
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [hipcc(), *HIPCC_FLAGS, "-c", src, "-o", obj]
+    cmd = [hipcc(), *HIPCC_FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

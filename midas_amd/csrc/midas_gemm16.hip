@@ -94,8 +94,10 @@ __global__ void k_split_a(const float2 *__restrict__ A, long lda, int m, int K, 
         }
 }
 
-// B (K x n, ldb) -> per-column power-of-two scale + planes; one wave per column
-template <bool BF, int P>
+// B (K x n, ldb) -> per-column power-of-two scale + planes; one wave per column.
+// NEG (single term only): a third plane holds -Bi, so the real part of the
+// product is Br Ar + (-Bi) Ai in one accumulator.
+template <bool BF, int P, bool NEG = false>
 __global__ __launch_bounds__(64) void k_split_b(const float2 *__restrict__ B, long ldb, int K, int n, int rtiles,
                                                 int kst, u16 *__restrict__ out, float *__restrict__ scale)
 {
@@ -137,6 +139,13 @@ __global__ __launch_bounds__(64) void k_split_b(const float2 *__restrict__ B, lo
             for (int q = 0; q < 8; ++q) w[q] = (short)h[p][c][q];
             *(s8 *)(out + plane_off(p * 2 + c, col, k0, rtiles, kst)) = w;
         }
+    if constexpr (NEG) {
+        static_assert(P == 1, "negated plane: single term only");
+        s8 w;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[q] = (short)(h[0][1][q] ^ 0x8000);   // sign bit of the half
+        *(s8 *)(out + plane_off(2, col, k0, rtiles, kst)) = w;
+    }
 }
 
 // C -= A B from the planes.  TERMS pairs: (hi,hi) [, (B hi, A lo), (B lo, A hi)].
@@ -367,6 +376,166 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
     }
 }
 
+// ---------------------------------------------------------------------------
+// Single-term variant with two workgroups per CU.  With one workgroup per CU
+// the read-modify-write of C at the end of each tile and the LDS / barrier
+// waits of the main loop leave the matrix cores idle; a second resident
+// workgroup fills those gaps.  To fit two: the real part accumulates
+// Br Ar + (-Bi) Ai in one set (B carries a negated imaginary plane), so a wave
+// holds 2 x 4 accumulator tiles; stages are 16 k deep in a ring of three 20 KB
+// slots (60 KB per workgroup); registers stay within 224 per lane, which
+// leaves room on each SIMD for a 64-register wave of the LU panel kernel.
+// A planes: Ar, Ai.  B planes: Br, Bi, -Bi.
+// ---------------------------------------------------------------------------
+struct Stage2 {
+    u16 a[2][T_M][T_K];
+    u16 b[3][T_N][T_K];
+};
+__shared__ __attribute__((aligned(16))) unsigned char g2_slot0[sizeof(Stage2)];
+__shared__ __attribute__((aligned(16))) unsigned char g2_slot1[sizeof(Stage2)];
+__shared__ __attribute__((aligned(16))) unsigned char g2_slot2[sizeof(Stage2)];
+
+template <bool BF>
+__global__ __launch_bounds__(256, 2) void k_trail16x(int m, int n, int kst, int mt, int nt, int group,
+                                                     const u16 *__restrict__ Ap, const u16 *__restrict__ Bp,
+                                                     const float *__restrict__ bscale, float2 *__restrict__ C, long ldc)
+{
+    constexpr int IPS = 5;   // copy instructions per wave per stage (one 1 KB part of each of 5 plane tiles)
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    __shared__ float sinv[T_N];
+
+    // tile order as in k_trail16
+    const int total = mt * nt;
+    const int id = blockIdx.x;
+    const int full = (total / 8) * 8;
+    const int L = id < full ? (id % 8) * (full / 8) + id / 8 : id;
+    const int grp = L / (group * nt);
+    const int rows_g = min(group, mt - grp * group);
+    const int li = L - grp * group * nt;
+    const int tm = grp * group + li % rows_g, tn = li / rows_g;
+    const int m0 = tm * T_M, n0 = tn * T_N;
+
+    // 1 / column scale of the tile's columns, read by the epilogue from LDS
+    // (a global load there would also wait for the C loads in flight); the
+    // first ring barrier orders these stores before any read
+    if (tid < T_N) sinv[tid] = 1.0f / (n0 + tid < n ? bscale[n0 + tid] : 1.0f);
+
+    f16v accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                accr[a][b][q] = 0.f;
+                acci[a][b][q] = 0.f;
+            }
+
+    // copies straight to LDS, as in k_trail16 (one asm statement sets M0)
+    typedef __attribute__((address_space(3))) void lds_void;
+    auto copy16 = [&](const u16 *gsrc, u16 *ldst) {
+        const unsigned lds = (unsigned)(size_t)(lds_void *)ldst;
+        asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(gsrc), "s"(lds) : "memory");
+    };
+    auto dma = [&](Stage2 *dst, int ks) {
+#pragma unroll
+        for (int pc = 0; pc < 2; ++pc)
+            copy16(Ap + ((((long)pc * mt + tm) * kst + ks) * T_M) * T_K + wave * 512 + lane * 8,
+                   &dst->a[pc][0][0] + wave * 512);
+#pragma unroll
+        for (int pc = 0; pc < 3; ++pc)
+            copy16(Bp + ((((long)pc * nt + tn) * kst + ks) * T_N) * T_K + wave * 512 + lane * 8,
+                   &dst->b[pc][0][0] + wave * 512);
+    };
+    auto compute = [&](const Stage2 *src) {
+        const int r32 = lane & 31, kc = (lane >> 5) * 8;
+        s8 fb[3][2];
+#pragma unroll
+        for (int pc = 0; pc < 3; ++pc)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) fb[pc][t] = *(const s8 *)&src->b[pc][wn + 32 * t + r32][kc];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            s8 fa[2];
+#pragma unroll
+            for (int pc = 0; pc < 2; ++pc) fa[pc] = *(const s8 *)&src->a[pc][wm + 32 * a + r32][kc];
+            // C^T: Dr = Br^T Ar^T + (-Bi)^T Ai^T, Di = Br^T Ai^T + Bi^T Ar^T
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                accr[a][b] = mma32<BF>(fb[0][b], fa[0], accr[a][b]);
+                acci[a][b] = mma32<BF>(fb[0][b], fa[1], acci[a][b]);
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                accr[a][b] = mma32<BF>(fb[2][b], fa[1], accr[a][b]);
+                acci[a][b] = mma32<BF>(fb[1][b], fa[0], acci[a][b]);
+            }
+        }
+    };
+    // ring of three: at step s, stage s+1 is in flight; refill the slot of
+    // stage s-1 (computed by every wave before the barrier) with stage s+2
+    auto step = [&](int s, const Stage2 *cur, Stage2 *refill) {
+        if (s + 1 < kst) MIDAS_WAIT_VM(IPS);
+        else MIDAS_WAIT_VM(0);
+        asm volatile("s_barrier" ::: "memory");
+        if (s + 2 < kst) dma(refill, s + 2);
+        compute(cur);
+    };
+    Stage2 *s0 = (Stage2 *)g2_slot0, *s1 = (Stage2 *)g2_slot1, *s2 = (Stage2 *)g2_slot2;
+
+    // Epilogue: four 32 x 32 blocks e (a = e & 1, b = e >> 1).  Lane holds
+    // column lane%32 of D (a row of C); register q holds D row
+    // 8*(q/4) + 4*(lane/32) + q%4 (a column of C).  The column's q part is
+    // uniform (scalar bases), the lane part a 32-bit offset.  C is streamed
+    // non-temporally; two blocks are in flight: each block's loads are issued
+    // before the previous block is written.
+    const int loff = (lane & 31) + 4 * (lane >> 5) * (int)ldc;
+    auto cload = [&](int e, double *cv) {
+        const int row = m0 + wm + 32 * (e & 1) + (lane & 31);
+        const double *Cb = (const double *)(C + (long)(n0 + wn + 32 * (e >> 1)) * ldc + m0 + wm + 32 * (e & 1));
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int col = n0 + wn + 32 * (e >> 1) + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+            cv[q] = (row < m && col < n) ? __builtin_nontemporal_load(Cb + (long)(8 * (q >> 2) + (q & 3)) * ldc + loff)
+                                         : 0.0;
+        }
+    };
+    auto cstore = [&](int e, const double *cv, const f16v &dr, const f16v &di) {
+        const int row = m0 + wm + 32 * (e & 1) + (lane & 31);
+        double *Cb = (double *)(C + (long)(n0 + wn + 32 * (e >> 1)) * ldc + m0 + wm + 32 * (e & 1));
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int col = n0 + wn + 32 * (e >> 1) + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+            if (row < m && col < n) {
+                const float inv = sinv[col - n0];
+                const float2 c0 = __builtin_bit_cast(float2, cv[q]);
+                const float2 c1 = float2{c0.x - dr[q] * inv, c0.y - di[q] * inv};
+                __builtin_nontemporal_store(__builtin_bit_cast(double, c1),
+                                            Cb + (long)(8 * (q >> 2) + (q & 3)) * ldc + loff);
+            }
+        }
+    };
+
+    dma(s0, 0);
+    if (kst > 1) dma(s1, 1);
+    for (int s = 0; s < kst; s += 3) {
+        step(s, s0, s2);
+        if (s + 1 < kst) step(s + 1, s1, s0);
+        if (s + 2 < kst) step(s + 2, s2, s1);
+    }
+    double cv0[16], cv1[16];
+    cload(0, cv0);
+    cload(1, cv1);
+    cstore(0, cv0, accr[0][0], acci[0][0]);
+    cload(2, cv0);
+    cstore(1, cv1, accr[1][0], acci[1][0]);
+    cload(3, cv1);
+    cstore(2, cv0, accr[0][1], acci[0][1]);
+    cstore(3, cv1, accr[1][1], acci[1][1]);
+}
+
 // 16-k granules, rounded up to a multiple of 8 = ring slots x granules per stage
 // for both variants (split planes are zero-padded)
 static inline int kstages(int k) { return ((k + 8 * T_K - 1) / (8 * T_K)) * 8; }
@@ -386,6 +555,26 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
     u16 *Bp = Ap + (size_t)2 * P * mt * T_M * kst * T_K;
     float *sc = (float *)(Bp + (size_t)2 * P * nt * T_N * kst * T_K);
     MIDAS_CHECK(kst * T_K <= 512, -1, "trail16: k = %d above 512", k);
+    if constexpr (P == 1) {
+        // two workgroups per CU (MIDAS_TRAIL_1WG=1 selects the one-workgroup kernel)
+        const char *e = getenv("MIDAS_TRAIL_1WG");
+        if (!(e && atoi(e) != 0)) {
+            const int ks = (k + T_K - 1) / T_K;   // 16-k stages, no ring padding
+            u16 *Bq = Ap + (size_t)2 * mt * T_M * ks * T_K;
+            float *sq = (float *)(Bq + (size_t)3 * nt * T_N * ks * T_K);
+            hipLaunchKernelGGL((k_split_a<BF, 1>), dim3(mt * T_M / 256 + 1, ks * 2), dim3(256), 0, st, A, lda, m, k,
+                               mt, ks, Ap);
+            hipLaunchKernelGGL((k_split_b<BF, 1, true>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, ks, Bq,
+                               sq);
+            ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 16.0 * m * n + 4.0 * ((double)m + 1.5 * n) * k);
+            const long a_tile = (long)T_M * ks * T_K * 2 * 2;
+            const int group = (int)std::max(2L, std::min(16L, (2L << 20) / a_tile));
+            hipLaunchKernelGGL((k_trail16x<BF>), dim3(mt * nt), dim3(256), 0, st, m, n, ks, mt, nt, group, Ap, Bq, sq,
+                               C, ldc);
+            MIDAS_HIP(hipGetLastError());
+            return 0;
+        }
+    }
     hipLaunchKernelGGL((k_split_a<BF, P>), dim3(mt * T_M / 256 + 1, kst * 2), dim3(256), 0, st, A, lda, m, k, mt, kst,
                        Ap);
     hipLaunchKernelGGL((k_split_b<BF, P>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, kst, Bp, sc);

@@ -163,10 +163,11 @@ __global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t
 // Inner block [jb, jb+ncols) (ncols <= IB) in one launch.  Thread t of
 // workgroup b holds row jb + 256 b + t of the block's columns in registers for
 // all IB column steps.  Per column j:
-//   local argmax of |Re|+|Im| over rows >= j  ->  publish (max, row), the
-//   candidate row's IB values and, in workgroup 0, row j; wait until those
-//   stores are done, then publish the step tag of this workgroup
-//   wait until every workgroup's tag has reached this step
+//   local argmax of |Re|+|Im| over rows >= j  ->  publish the candidate
+//   row's IB values and, in workgroup 0, row j; wait until those stores are
+//   done, then publish (max, row) with the step tag (FP32: one 64-bit record
+//   tag | row | max; FP64: max and row, then a separate tag)
+//   wait until every workgroup's record of this step is there
 //   every workgroup reduces the candidates (first row on ties, as i?amax),
 //   takes the pivot row from the winner's published copy, applies the
 //   interchange to the rows it holds, and updates its rows below j (scale by
@@ -261,6 +262,9 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
     const int r0 = jb + b * BLK_T, i = r0 + tid;
     const bool mine = i < n;
     bool waited_out = false;
+    // the panel is the critical path: its waves take issue slots first on
+    // SIMDs shared with the trailing update
+    __builtin_amdgcn_s_setprio(3);
 
     // a[t]: this thread's row.  The block's columns rotate one place left after
     // every step, so the current column is always t = 0 and every register
@@ -283,10 +287,6 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
             }
             argmax_block(best, bi, rv, ri);
             bi = __builtin_amdgcn_readfirstlane(bi);   // uniform: keeps address math scalar
-            if (tid == 0) {
-                st_coh(bmax + par * w.maxblk + b, best);
-                st_coh(bidx + par * w.maxblk + b, bi);
-            }
             if (bi == i) {
 #pragma unroll
                 for (int t = 0; t < LU_IB; ++t) st_coh(cand + ((long)par * w.maxblk + b) * LU_IB + t, a[t]);
@@ -298,44 +298,72 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
             }
             stores_done();
             __syncthreads();
-            if (tid == 0) st_coh(tag + b, want);
-            // ---- wait for every workgroup of this matrix ----------------------
-            for (int q = tid; q < nwg; q += BLK_T) {
-                int spins = 0;
-                while (!waited_out && (int)(ld_coh(tag + q) - want) < 0) {
-                    if (++spins > BLK_SPIN) {
-                        st_coh(w.fault, 1);
-                        waited_out = true;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            __syncthreads();
-            // ---- global pivot -------------------------------------------------
             T gv = (T)-1;
             int gi = 0x7fffffff;
-            for (int q = tid; q < nwg; q += BLK_T) {
-                const T v = ld_coh(bmax + par * w.maxblk + q);
-                const int vi = ld_coh(bidx + par * w.maxblk + q);
-                if (pivot_better(v, vi, gv, gi)) { gv = v; gi = vi; }
+            if constexpr (sizeof(T) == 4) {
+                // one 64-bit record: step tag (12 bits) | row (20 bits) | max (float bits)
+                unsigned long long *rec = (unsigned long long *)w.pmax + (long)zb * 2 * w.maxblk;
+                if (tid == 0)
+                    st_coh(rec + par * w.maxblk + b,
+                           ((unsigned long long)(want & 0xfffu) << 52) |
+                               ((unsigned long long)(bi == 0x7fffffff ? 0xfffffu : (unsigned)bi) << 32) |
+                               __builtin_bit_cast(unsigned, (float)best));
+                // ---- wait for every workgroup's record of this step -----------
+                for (int q = tid; q < nwg; q += BLK_T) {
+                    int spins = 0;
+                    unsigned long long r = ld_coh(rec + par * w.maxblk + q);
+                    while (!waited_out && (unsigned)(r >> 52) != (want & 0xfffu)) {
+                        if (++spins > BLK_SPIN) {
+                            st_coh(w.fault, 1);
+                            waited_out = true;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        r = ld_coh(rec + par * w.maxblk + q);
+                    }
+                    const unsigned ri32 = (unsigned)(r >> 32) & 0xfffffu;
+                    const int vi = ri32 == 0xfffffu ? 0x7fffffff : (int)ri32;
+                    const T v = (T)__builtin_bit_cast(float, (unsigned)r);
+                    if (pivot_better(v, vi, gv, gi)) { gv = v; gi = vi; }
+                }
+            } else {
+                if (tid == 0) {
+                    st_coh(bmax + par * w.maxblk + b, best);
+                    st_coh(bidx + par * w.maxblk + b, bi);
+                    stores_done();
+                    st_coh(tag + b, want);
+                }
+                // ---- wait for every workgroup's tag of this step --------------
+                for (int q = tid; q < nwg; q += BLK_T) {
+                    int spins = 0;
+                    while (!waited_out && (int)(ld_coh(tag + q) - want) < 0) {
+                        if (++spins > BLK_SPIN) {
+                            st_coh(w.fault, 1);
+                            waited_out = true;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    const T v = ld_coh(bmax + par * w.maxblk + q);
+                    const int vi = ld_coh(bidx + par * w.maxblk + q);
+                    if (pivot_better(v, vi, gv, gi)) { gv = v; gi = vi; }
+                }
             }
+            // ---- global pivot -------------------------------------------------
             argmax_block(gv, gi, rv, ri);
             const int p = __builtin_amdgcn_readfirstlane(gi == 0x7fffffff ? j : gi);
             if (tid < LU_IB)
                 urow[tid] = p == j ? ld_coh(rowj + par * LU_IB + tid)
                                    : ld_coh(cand + ((long)par * w.maxblk + (p - jb) / BLK_T) * LU_IB + tid);
+            // row p takes row j's old values (loaded alongside the pivot row)
+            if (p != j && i == p) {
+#pragma unroll
+                for (int t = 0; t < LU_IB; ++t) a[t] = ld_coh(rowj + par * LU_IB + t);
+            }
             __syncthreads();
             if (b == 0 && tid == 0) ipiv[j] = p;
-            if (p != j) {
-                // row p takes row j's old values, row j the pivot row
-                if (i == p) {
+            // row j takes the pivot row
+            if (p != j && i == j) {
 #pragma unroll
-                    for (int t = 0; t < LU_IB; ++t) a[t] = ld_coh(rowj + par * LU_IB + t);
-                }
-                if (i == j) {
-#pragma unroll
-                    for (int t = 0; t < LU_IB; ++t) a[t] = urow[t];
-                }
+                for (int t = 0; t < LU_IB; ++t) a[t] = urow[t];
             }
             // ---- scale and rank-1 update of the rows below j ------------------
             if (mine && i > j) {
@@ -440,6 +468,62 @@ __global__ __launch_bounds__(64) void k_swap_trsm(typename cvec<T>::t *A, long l
         if (r < ib) col[jb + r] = V{x[r].r, x[r].i};
 }
 
+// Rank-IB update of the rest of the outer panel after an inner block:
+//   A(jend:n, jend:c1) -= L(jend:n, jb:jend) U(jb:jend, jend:c1).
+// A stream over the panel's remaining columns (each element read and written
+// once): thread = row, its IB multipliers in registers, the workgroup's
+// UPD_COLS columns of U in LDS (read as broadcasts).  FP32: kept under 104
+// registers (5 waves per SIMD) so its waves fit beside the trailing GEMM's.
+constexpr int UPD_COLS = 64;
+template <typename T, int WPE>
+__global__ __launch_bounds__(256, WPE) void k_panel_update(typename cvec<T>::t *A, long lda, long sA, int n, int jb,
+                                                           int ib, int jend, int c1)
+{
+    typedef typename cvec<T>::t V;
+    __shared__ cplx<T> u[UPD_COLS][LU_IB];
+    __builtin_amdgcn_s_setprio(3);   // panel critical path (see k_getf2_block)
+    A += blockIdx.z * sA;
+    const int c0 = jend + blockIdx.y * UPD_COLS;
+    const int nc = min(UPD_COLS, c1 - c0);
+    for (int e = threadIdx.x; e < UPD_COLS * LU_IB; e += 256) {
+        const int c = e / LU_IB, t = e % LU_IB;
+        V x = V{0, 0};
+        if (c < nc && t < ib) x = A[(long)(jb + t) + (long)(c0 + c) * lda];
+        u[c][t] = cplx<T>{x.x, x.y};
+    }
+    __syncthreads();
+    const int r = jend + blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    cplx<T> l[LU_IB];
+#pragma unroll
+    for (int t = 0; t < LU_IB; ++t) {
+        V x = V{0, 0};
+        if (t < ib) x = A[(long)r + (long)(jb + t) * lda];
+        l[t] = cplx<T>{x.x, x.y};
+    }
+    // columns in batches: all loads of a batch are issued before its stores
+    // (a store could alias a later load, so the compiler would not hoist them)
+    constexpr int CB = sizeof(T) == 4 ? 16 : 4;
+    V *col = A + (long)r + (long)c0 * lda;
+    for (int cb = 0; cb < nc; cb += CB) {
+        cplx<T> x[CB];
+#pragma unroll
+        for (int q = 0; q < CB; ++q) {
+            V v = V{0, 0};
+            if (cb + q < nc) v = col[(long)(cb + q) * lda];
+            x[q] = cplx<T>{v.x, v.y};
+        }
+#pragma unroll
+        for (int q = 0; q < CB; ++q)
+            if (cb + q < nc) {
+                cplx<T> acc = x[q];
+#pragma unroll
+                for (int t = 0; t < LU_IB; ++t) acc = csub(acc, cmul(l[t], u[cb + q][t]));
+                col[(long)(cb + q) * lda] = V{acc.r, acc.i};
+            }
+    }
+}
+
 // Forward substitution with the unit-lower IBxIB block L(jb:jend, jb:jend) on
 // rows [jb, jend) of columns [c0, c1): one thread per column.
 template <typename T>
@@ -489,14 +573,16 @@ __global__ void k_laswp(typename cvec<T>::t *A, long lda, long sA, const int *ip
 // l, l+64, ..; at step q the solved x_q is broadcast and every row updates with
 // column q of D (a unit-stride read).  The columns of D are loaded QC at a time
 // ahead of the dependent chain of broadcasts, so the loop waits for memory once
-// per QC steps.  LOWER: unit lower L, x = L^-1 e_c; otherwise upper U with its
-// diagonal, x = U^-1 e_c.  Inv is ldi x bs.
-constexpr int TRINV_QC = 8;
+// per QC steps; QC shrinks as R grows so that the lower (panel-stream) FP32
+// variant stays within 96 registers and fits beside the trailing update.
+// LOWER: unit lower L, x = L^-1 e_c; otherwise upper U with its diagonal,
+// x = U^-1 e_c.  Inv is ldi x bs.
 template <typename T, bool LOWER, int R>
 __global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D, long lda, long sD, int bs,
                                                     typename cvec<T>::t *Inv, long ldi, long sI)
 {
     typedef typename cvec<T>::t V;
+    constexpr int TRINV_QC = R >= 8 ? 2 : (R >= 4 ? 4 : 8);
     D += blockIdx.y * sD;
     Inv += blockIdx.y * sI;
     const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -607,7 +693,8 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
     // forces the latter, so that both stay tested)
     const char *cs_env = getenv("MIDAS_LU_COLUMN_STEPS");
     const bool column_steps = cs_env && atoi(cs_env) != 0;
-    const bool fused = !column_steps && (n - k + BLK_T - 1) / BLK_T <= BLK_MAXWG;
+    const bool fused = !column_steps && (n - k + BLK_T - 1) / BLK_T <= BLK_MAXWG &&
+                       (sizeof(T) == 8 || n < 0xfffff);   // FP32 records hold a 20-bit row
     for (int jb = k; jb < k + nb; jb += LU_IB) {
         int jend = min(jb + LU_IB, k + nb);
         int rc = k + nb - jend;
@@ -641,13 +728,12 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
             }
         }
         // rest of the outer panel: columns [jend, k+nb)
-        if (rc > 0) {
-            if (n - jend > 0) {
-                int r = launch_cgemm<T>(n - jend, rc, jend - jb, cmone<T>(), A + jend + (long)jb * lda, lda, 0,
-                                        A + jb + (long)jend * lda, lda, 0, cone<T>(), A + jend + (long)jend * lda,
-                                        lda, st, w.batch, w.sA, w.sA, w.sA);
-                if (r) return r;
-            }
+        if (rc > 0 && n - jend > 0) {
+            constexpr int wpe = sizeof(T) == 4 ? 5 : 4;
+            hipLaunchKernelGGL((k_panel_update<T, wpe>),
+                               dim3((n - jend + 255) / 256, (rc + UPD_COLS - 1) / UPD_COLS, w.batch), dim3(256), 0, st,
+                               A, lda, w.sA, n, jb, jend - jb, jend, k + nb);
+            MIDAS_HIP(hipGetLastError());
         }
     }
     // inverse of the unit-lower diagonal block L11 of this panel: U12 = L11^-1 A12
@@ -689,6 +775,7 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
     MIDAS_CHECK(w.batch == 1 || tg.batch == w.batch, -1, "getrf: trailing GEMM not batched like the factorisation");
     // last-workgroup counters, info, the fault flag and the step tags
     MIDAS_HIP(hipMemsetAsync(w.counter, 0, sizeof(int) * (2 * w.batch + 1 + w.maxblk * w.batch), st));
+    MIDAS_HIP(hipMemsetAsync(w.pmax, 0, 8 * 2 * w.maxblk * w.batch, st));   // tagged records
     w.tag_base = 0;
     MIDAS_HIP(hipEventRecord(w.ev_next, st));
     MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
@@ -860,7 +947,7 @@ int LuWork<T>::alloc(int n_, int nb_, int batch_, long sA_)
     batch = batch_;
     sA = sA_;
     maxblk = (n + BLK_T - 1) / BLK_T + 1;   // workgroups of either panel kernel
-    MIDAS_HIP(hipMalloc(&pmax, sizeof(T) * 2 * maxblk * batch));
+    MIDAS_HIP(hipMalloc(&pmax, 8 * 2 * maxblk * batch));   // FP32: 64-bit records of k_getf2_block
     MIDAS_HIP(hipMalloc(&pidx, sizeof(int) * 2 * maxblk * batch));
     MIDAS_HIP(hipMalloc(&counter, sizeof(int) * (2 * batch + 1 + maxblk * batch)));
     info = counter + batch;

@@ -103,3 +103,33 @@ def test_trailing_split(mid, mode, tol):
     err = (np.abs(Cm - ref) / (colscale + np.abs(C0).max() * 1e-7)).max()
     print("mode", mode, "scaled err", err)
     assert err < tol
+
+
+@pytest.mark.parametrize("one_wg", ["0", "1"])
+@pytest.mark.parametrize("shape", [(257, 130, 45), (640, 384, 512), (129, 1, 16)])
+@pytest.mark.parametrize("mode,tol", [(1, 4e-3), (11, 3e-2)])
+def test_trailing_split_kernels(mid, mode, tol, shape, one_wg, monkeypatch):
+    """Single-term split update: the two-workgroups-per-CU kernel (default) and
+    the one-workgroup kernel (MIDAS_TRAIL_1WG=1) on ragged shapes, k not a
+    multiple of the stage depth, and k at the panel width 512."""
+    m, n, k = shape
+    print(f"\n[case] trailing split kernels mode={mode} shape={shape} one_wg={one_wg}", flush=True)
+    rng = np.random.default_rng(m + n + k)
+    cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    A = cr(m, k)
+    A /= np.abs(A).max() * 1.5
+    B = cr(k, n) * np.exp(rng.normal(size=(1, n)) * 3)
+    C0 = cr(m, n)
+    Cm = np.asfortranarray(C0.copy())
+    A, B = np.asfortranarray(A), np.asfortranarray(B)
+    monkeypatch.setenv("MIDAS_TRAIL_1WG", one_wg)
+    r = mid.lib.midas_test_trailing(mid.ctx, C.c_int(mode), C.c_int(m), C.c_int(n), C.c_int(k), _p(A), C.c_long(m),
+                                    _p(B), C.c_long(k), _p(Cm), C.c_long(m))
+    monkeypatch.delenv("MIDAS_TRAIL_1WG")
+    assert r == 0
+    ref = C0 - A @ B
+    colscale = (np.abs(A).max() * np.abs(B).max(0) * k)[None, :]
+    err = (np.abs(Cm - ref) / (colscale + np.abs(C0).max() * 1e-7)).max()
+    print("scaled err", err)
+    assert err < tol
+
