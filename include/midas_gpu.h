@@ -111,6 +111,15 @@ int midas_block_fill(midas_ctx *ctx, const midas_cells_t *b1, const midas_cells_
 int midas_plane_wave(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int ntr, const double *th,
                      const double *ph, int ref32, double *V, long ldv);
 
+/* ---------------- a10: spherical-wave CBF excitations ----------------
+ * Incident_Field_Spherical (Incident_Field.f90:65-166) over the n cells given
+ * (an extended block, block cells first): sources at 10 max(R_block, lambda)
+ * from the cells' centroid in the directions (th, ph); V (3n x 2ntr, ldv).
+ * ref32 = 1 rounds the phase and the components to complex(4) as the
+ * reference does. */
+int midas_spherical_wave(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int ntr, const double *th,
+                         const double *ph, int ref32, double *V, long ldv);
+
 /* ---------------- a4: dense solve ----------------
  * PZGESV (Compute_EFields_MoM.f90:173): X = Z^-1 B, overwriting B (n x nrhs).
  * Z is not modified. */
@@ -161,8 +170,12 @@ int midas_aca_matrix(midas_ctx *ctx, int m, int n, const double *A, long lda, in
  * block_start[b] .. block_start[b+1]-1 (cells already in block order, as
  * Division_blocks leaves them); extension cells of block b are
  * ext_cells[ext_start[b] .. ext_start[b+1]-1] (CBFM_Blocks_Ext).  CBF
- * excitations are ntr_cbf plane waves (th_cbf, ph_cbf: Transmitters_CBFM of
- * getTransmitters_CBFM), the problem excitations ntr plane waves.
+ * excitations come from the ntr_cbf directions th_cbf, ph_cbf
+ * (Transmitters_CBFM of getTransmitters_CBFM): iws = 1 plane waves
+ * (Incident_Field), iws = 2 or 3 spherical waves from sources 10 block radii
+ * around each extended block (Incident_Field_Spherical; type_iws,
+ * Compute_EFields_CBFME.f90:363-369).  The problem excitations are ntr plane
+ * waves.
  * threshold <= 0 selects the reference's rule (1e-3 with ACA, else
  * max(10^(-3-(ceil(max|m|)-2)), 1e-10) with |m| = sqrt(|eps|)).
  * Outputs: E (3n x 2ntr, lde); K_out[b] = number of CBFs of block b
@@ -182,7 +195,7 @@ typedef struct {
 } midas_blocks_t;
 
 int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks, double k0,
-                               int quirks, int ntr_cbf, const double *th_cbf, const double *ph_cbf, int ntr,
+                               int quirks, int ntr_cbf, const double *th_cbf, const double *ph_cbf, int iws, int ntr,
                                const double *tr_th, const double *tr_ph, int use_aca, int kmax, double eps_aca,
                                double threshold, int mode, double *E, long lde, int *K_out, double *C_job, long ldc,
                                int kcap, int sr, double res_sr, double *fsr_out, long *nnz_out,

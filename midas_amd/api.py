@@ -35,7 +35,7 @@ MODE_NAMES = {SOLVE_FP64: "fp64", SOLVE_C32_IR: "c32+ir", SOLVE_F16_IR: "f16+ir"
 
 EXPORTS = (
     "midas_init", "midas_finalize", "midas_last_error", "midas_stream", "midas_cell_params", "midas_mom_fill",
-    "midas_block_fill", "midas_plane_wave", "midas_solve", "midas_scattering_matrix", "midas_cext_cabs",
+    "midas_block_fill", "midas_plane_wave", "midas_spherical_wave", "midas_solve", "midas_scattering_matrix", "midas_cext_cabs",
     "midas_compute_efields_mom", "midas_aca_block", "midas_aca_matrix", "midas_compute_efields_cbfm",
     "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_profile", "midas_profile_query",
 )
@@ -195,6 +195,17 @@ class Midas:
     def incident_field(self, cells, k0, th, ph, quirks=Q_ALL):
         return self.plane_wave(cells, k0, th, ph, ref32=True, quirks=quirks)
 
+    # -- a10: spherical-wave CBF excitation (Incident_Field_Spherical) ------
+    def spherical_wave(self, cells, k0, th, ph, ref32=True, quirks=Q_ALL):
+        cells = _cells(cells)
+        th, ph = np.ascontiguousarray(th, np.float64), np.ascontiguousarray(ph, np.float64)
+        V = np.empty((3 * cells.n, 2 * len(th)), np.complex128, order="F")
+        cs = cells.struct()
+        _check(self.lib.midas_spherical_wave(self.ctx, C.byref(cs), C.c_double(k0), C.c_int(quirks),
+                                             C.c_int(len(th)), _p(th), _p(ph), C.c_int(int(ref32)), _p(V),
+                                             C.c_long(3 * cells.n)), "spherical_wave")
+        return V
+
     # -- a4 -----------------------------------------------------------------
     def solve(self, Z, B, mode=SOLVE_FP64, max_iter=0, tol=0.0, nb=0):
         Z = np.asfortranarray(Z, np.complex128)
@@ -286,7 +297,7 @@ class Midas:
     # -- Compute_EFields_CBFME(_ACA) -----------------------------------------
     def compute_efields_cbfme(self, cells, k0, blocks, ext, cbf_dirs, transmitters, use_aca=False, kmax=50,
                               eps_aca=1e-4, threshold=0.0, mode=SOLVE_FP64, quirks=Q_ALL,
-                              return_cbfs=False, sr=False, res_sr=0.01):
+                              return_cbfs=False, sr=False, res_sr=0.01, iws=1):
         cells = _cells(cells)
         nb = len(blocks)
         starts = np.zeros(nb + 1, np.int32)
@@ -317,7 +328,8 @@ class Midas:
         cs = cells.struct()
         r = _check(self.lib.midas_compute_efields_cbfm(self.ctx, C.byref(cs), C.byref(bt), C.c_double(k0),
                                                        C.c_int(quirks), C.c_int(len(cth)), _p(cth), _p(cph),
-                                                       C.c_int(len(tth)), _p(tth), _p(tph), C.c_int(int(use_aca)),
+                                                       C.c_int(iws), C.c_int(len(tth)), _p(tth), _p(tph),
+                                                       C.c_int(int(use_aca)),
                                                        C.c_int(kmax), C.c_double(eps_aca), C.c_double(threshold),
                                                        C.c_int(mode), _p(E), C.c_long(3 * cells.n), _p(K),
                                                        None if Cj is None else _p(Cj), C.c_long(3 * cells.n),

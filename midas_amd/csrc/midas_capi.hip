@@ -445,6 +445,37 @@ int midas_plane_wave(midas_ctx *ctx, const midas_cells_t *cells, double k0, int 
     return 0;
 }
 
+int midas_spherical_wave(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int ntr, const double *th,
+                         const double *ph, int ref32, double *V, long ldv)
+{
+    MIDAS_CHECK(ctx && cells && th && ph && V && ntr > 0 && cells->n > 0, -1, "spherical_wave: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    const int n = cells->n;
+    MIDAS_CHECK(ldv >= 3L * n, -1, "spherical_wave: ldv too small");
+    DevCellsOwned dc;
+    midas_cells_t geo = *cells;
+    std::vector<double> zr;
+    if (!(geo.kappa && geo.che && geo.znnpp) && !(geo.sc && geo.eps)) {
+        zr.assign(2 * (size_t)n, 0.0);
+        geo.kappa = zr.data();
+        geo.che = zr.data();
+        geo.znnpp = zr.data();
+    }
+    TRY(dc.from_host(&geo, k0, quirks, ctx->st));
+    double cen[3], rs;
+    sph_source_frame(cells->x, cells->y, cells->z, nullptr, n, k0, quirks, cen, &rs);
+    DBuf dth, dph, dv;
+    TRY(upload(dth, th, sizeof(double) * ntr, ctx->st));
+    TRY(upload(dph, ph, sizeof(double) * ntr, ctx->st));
+    TRY(dv.alloc(sizeof(double2) * 3L * n * 2 * ntr));
+    TRY(launch_rhs_sph(dc.v, CellSet{n, 0, nullptr}, ntr, dth.as<double>(), dph.as<double>(), k0, quirks, ref32 != 0,
+                       cen, rs, dv.as<double2>(), 3L * n, ctx->st));
+    MIDAS_HIP(hipMemcpy2DAsync(V, sizeof(double2) * ldv, dv.p, sizeof(double2) * 3 * n, sizeof(double2) * 3 * n,
+                               2 * ntr, hipMemcpyDeviceToHost, ctx->st));
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
 int midas_solve(midas_ctx *ctx, int n, int nrhs, const double *Z, long ldz, double *B, long ldb, int mode,
                 const midas_solve_opts_t *opts, midas_solve_info_t *info)
 {

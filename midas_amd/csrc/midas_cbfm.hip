@@ -319,7 +319,8 @@ __global__ void k_sr_mask(double2 *Z, long ld, int m, DevCells cells, CellSet S,
 
 extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks,
                                           double k0, int quirks, int ntr_cbf, const double *th_cbf,
-                                          const double *ph_cbf, int ntr, const double *tr_th, const double *tr_ph,
+                                          const double *ph_cbf, int iws, int ntr, const double *tr_th,
+                                          const double *tr_ph,
                                           int use_aca, int kmax, double eps_aca, double threshold, int mode,
                                           double *E, long lde, int *K_out, double *C_job, long ldc, int kcap,
                                           int sr, double res_sr, double *fsr_out, long *nnz_out,
@@ -332,6 +333,7 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
     MIDAS_CHECK(N > 0 && nb > 0 && blocks->block_start && blocks->ext_start, -1, "cbfm: empty cells or blocks");
     MIDAS_CHECK(ntr_cbf > 0 && ntr > 0, -1, "cbfm: ntr_cbf %d, ntr %d", ntr_cbf, ntr);
     MIDAS_CHECK(mode >= 0 && mode <= 5, -1, "cbfm: unknown solver mode %d", mode);
+    MIDAS_CHECK(iws >= 1 && iws <= 3, -1, "cbfm: CBF excitation type %d (1 plane, 2/3 spherical waves)", iws);
     if (use_aca)
         MIDAS_CHECK(kmax >= 1 && kmax <= ACA_KMAX_SUPPORTED && eps_aca > 0, -1, "cbfm: kmax %d (1..%d), eps %g", kmax,
                     ACA_KMAX_SUPPORTED, eps_aca);
@@ -457,8 +459,18 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                     hipLaunchKernelGGL(k_unit_diag, dim3((int)((mp - m + 255) / 256)), dim3(256), 0, st, Zq, mp, m,
                                        (int)mp);
                 MIDAS_HIP(hipGetLastError());
-                TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32, Xb(b), mp,
-                               st));
+                if (iws == 1) {
+                    TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32, Xb(b),
+                                   mp, st));
+                } else {
+                    // spherical waves around the extended block (type_iws 2 and 3 both call
+                    // Incident_Field_Spherical, Compute_EFields_CBFME.f90:363-369)
+                    double cen[3], rs;
+                    sph_source_frame(cells->x, cells->y, cells->z, allidx.data() + ioff[b], sb[b], k0, quirks, cen,
+                                     &rs);
+                    TRY(launch_rhs_sph(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32,
+                                       cen, rs, Xb(b), mp, st));
+                }
             }
             LuWork<double> w;
             TRY(w.alloc((int)mp, nb_lu, nc, mp * mp));

@@ -119,6 +119,10 @@ int launch_fill(const DevCells &cells, CellSet R, CellSet S, double k0, int quir
                 long ld64, float2 *W32, long ld32, hipStream_t st);
 int launch_rhs(const DevCells &cells, CellSet R, int ntr, const double *th, const double *ph, double k0, int quirks,
                bool ref32, double2 *V, long ld, hipStream_t st);
+void sph_source_frame(const double *x, const double *y, const double *z, const int *idx, int n, double k0, int quirks,
+                      double c[3], double *rs);
+int launch_rhs_sph(const DevCells &cells, CellSet R, int ntr, const double *th, const double *ph, double k0,
+                   int quirks, bool ref32, const double c[3], double rs, double2 *V, long ld, hipStream_t st);
 int far_field(const DevCells &cells, const double2 *E, long ld, int T, int nrx, const double *rth,
               const double *rph, int npol, const double *beta, double k0, int quirks, double2 *S, long lds,
               double2 *FF, double2 *E3, double2 *G, hipStream_t st);

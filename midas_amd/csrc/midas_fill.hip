@@ -16,6 +16,9 @@
 #include "midas_common.h"
 #include "midas_elem.h"
 
+#include <algorithm>
+#include <cmath>
+
 namespace midas {
 
 __global__ void k_cell_params(int n, const double *__restrict__ sc, const double *__restrict__ eps, double k0,
@@ -132,6 +135,58 @@ __global__ void k_rhs(DevCells cells, CellSet R, int ntr, const double *__restri
     }
 }
 
+// Spherical-wave excitation (Incident_Field_Spherical, Incident_Field.f90:65-166):
+// the source of direction t sits at (cx, cy, cz) + rs (cos th, sin th cos ph,
+// sin th sin ph), rs = 10 max(R_block, 2 pi / k0) around the centroid of the
+// extended block (computed by the host).  Field at a cell, with R, u the
+// distance and unit vector from the source, theta = acos(ux),
+// phi = atan2(uz, uy), phase = e^{i k0 R} / R:
+//   V-pol (-phase sin theta, phase cos theta cos phi, phase cos theta sin phi)
+//   H-pol (0, -phase sin phi, phase cos phi)
+// REF32: phase and every component are complex(4), as the reference declares
+// them; the products are formed in double first (mixed-kind promotion).
+template <bool REF32>
+__global__ void k_rhs_sph(DevCells cells, CellSet R, int ntr, const double *__restrict__ th_deg,
+                          const double *__restrict__ ph_deg, double k0, double pi, double cx, double cy, double cz,
+                          double rs, double2 *__restrict__ V, long ld)
+{
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (c >= R.n) return;
+    const int g = R.cell(c);
+    const double th = th_deg[t] * pi / 180.0, ph = ph_deg[t] * pi / 180.0;
+    const double xs = cx + rs * cos(th);
+    const double ys = cy + rs * sin(th) * cos(ph);
+    const double zs = cz + rs * sin(th) * sin(ph);
+    const double dx = cells.x[g] - xs, dy = cells.y[g] - ys, dz = cells.z[g] - zs;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double ux = dx / r, uy = dy / r, uz = dz / r;
+    double sk, ck;
+    sincos(k0 * r, &sk, &ck);
+    double pr = ck / r, pim = sk / r;
+    if (REF32) {
+        pr = (double)(float)pr;
+        pim = (double)(float)pim;
+    }
+    const double theta = acos(ux), phi = atan2(uz, uy);
+    const double st = sin(theta), ct = cos(theta), sp = sin(phi), cp = cos(phi);
+    z64 v[3] = {{-(pr * st), -(pim * st)}, {pr * ct * cp, pim * ct * cp}, {pr * ct * sp, pim * ct * sp}};
+    z64 h[3] = {{0.0, 0.0}, {-(pr * sp), -(pim * sp)}, {pr * cp, pim * cp}};
+    double2 *dv = V + (long)t * ld + 3L * c;
+    double2 *dh = V + (long)(t + ntr) * ld + 3L * c;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        if (REF32) {
+            dv[p] = make_double2((double)(float)v[p].r, (double)(float)v[p].i);
+            dh[p] = make_double2((double)(float)h[p].r, (double)(float)h[p].i);
+        } else {
+            dv[p] = make_double2(v[p].r, v[p].i);
+            dh[p] = make_double2(h[p].r, h[p].i);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -189,6 +244,49 @@ int launch_rhs(const DevCells &cells, CellSet R, int ntr, const double *th, cons
         hipLaunchKernelGGL(k_rhs<true>, grd, dim3(256), 0, st, cells, R, ntr, th, ph, k0, pi, V, ld);
     else
         hipLaunchKernelGGL(k_rhs<false>, grd, dim3(256), 0, st, cells, R, ntr, th, ph, k0, pi, V, ld);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+// Centre and source radius of Incident_Field_Spherical for the cells idx[0..n)
+// (an extended block, block cells first): centroid as sequential sums / n,
+// R_block = the largest distance to it, rs = 10 max(R_block, 2 pi / k0).
+void sph_source_frame(const double *x, const double *y, const double *z, const int *idx, int n, double k0, int quirks,
+                      double c[3], double *rs)
+{
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const int g = idx ? idx[i] : i;
+        sx += x[g];
+        sy += y[g];
+        sz += z[g];
+    }
+    c[0] = sx / n;
+    c[1] = sy / n;
+    c[2] = sz / n;
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const int g = idx ? idx[i] : i;
+        const double d = (x[g] - c[0]) * (x[g] - c[0]) + (y[g] - c[1]) * (y[g] - c[1]) + (z[g] - c[2]) * (z[g] - c[2]);
+        m = d > m ? d : m;
+    }
+    const double pi = host_pi(quirks);
+    *rs = 10.0 * std::max(std::sqrt(m), 2.0 * pi / k0);
+}
+
+int launch_rhs_sph(const DevCells &cells, CellSet R, int ntr, const double *th, const double *ph, double k0,
+                   int quirks, bool ref32, const double c[3], double rs, double2 *V, long ld, hipStream_t st)
+{
+    if (R.n <= 0 || ntr <= 0) return 0;
+    MIDAS_CHECK(ntr <= 65535, -1, "rhs_sph: too many directions in one launch");
+    double pi = host_pi(quirks);
+    dim3 grd((R.n + 255) / 256, ntr);
+    if (ref32)
+        hipLaunchKernelGGL(k_rhs_sph<true>, grd, dim3(256), 0, st, cells, R, ntr, th, ph, k0, pi, c[0], c[1], c[2], rs,
+                           V, ld);
+    else
+        hipLaunchKernelGGL(k_rhs_sph<false>, grd, dim3(256), 0, st, cells, R, ntr, th, ph, k0, pi, c[0], c[1], c[2],
+                           rs, V, ld);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }

@@ -160,7 +160,7 @@ SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,Transmitters,aca_
     integer(c_int), allocatable, target :: bstart(:), estart(:), ecells(:), kk(:)
     real(c_double), allocatable :: thc(:), phc(:), tth(:), tph(:)
     integer :: b, i, kcap
-    integer(c_int) :: r
+    integer(c_int) :: r, iws
 
     call midas_dropin_init()
     call midas_pack_cells(Cells, Nbc, ca, cs)
@@ -185,8 +185,12 @@ SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,Transmitters,aca_
     bl = midas_blocks_t(int(Nblocks, c_int), c_loc(bstart), c_loc(estart), c_loc(ecells))
     kcap = 2 * size(tcbf)
     allocate(E_total(3 * Nbc, 2 * NTr), C_b(3 * Nbc, kcap))
+    ! CBF excitation: type_iws in the dense variant (Compute_EFields_CBFME.f90:363-369);
+    ! the ACA variant always uses plane waves (Compute_EFields_CBFME_ACA.f90:312)
+    iws = int(type_iws, c_int)
+    if (aca_on /= 0) iws = 1
     r = midas_compute_efields_cbfm(midas_context, cs, bl, k_0, MIDAS_Q_ALL, int(size(tcbf), c_int), thc, phc, &
-                                   int(NTr, c_int), tth, tph, int(aca_on, c_int), int(Nb_it_max, c_int), &
+                                   iws, int(NTr, c_int), tth, tph, int(aca_on, c_int), int(Nb_it_max, c_int), &
                                    Epsilon_ACA, threshold, midas_mode, c_loc(E_total), int(3 * Nbc, c_long), kk, &
                                    c_loc(C_b), int(3 * Nbc, c_long), int(kcap, c_int), int(SR, c_int), res_SR, &
                                    c_null_ptr, c_null_ptr, inf)

@@ -61,7 +61,7 @@ module midas_gpu_c
         end function midas_compute_efields_mom
 
         integer(c_int) function midas_compute_efields_cbfm(ctx, cells, blocks, k0, quirks, ntr_cbf, th_cbf, ph_cbf, &
-                                                           ntr, tr_th, tr_ph, use_aca, kmax, eps_aca, threshold, &
+                                                           iws, ntr, tr_th, tr_ph, use_aca, kmax, eps_aca, threshold, &
                                                            mode, e, lde, k_out, c_job, ldc, kcap, sr, res_sr, &
                                                            fsr_out, nnz_out, info) &
             bind(c, name="midas_compute_efields_cbfm")
@@ -70,13 +70,25 @@ module midas_gpu_c
             type(midas_cells_t), intent(in) :: cells
             type(midas_blocks_t), intent(in) :: blocks
             real(c_double), value :: k0, eps_aca, threshold, res_sr
-            integer(c_int), value :: quirks, ntr_cbf, ntr, use_aca, kmax, mode, kcap, sr
+            integer(c_int), value :: quirks, ntr_cbf, iws, ntr, use_aca, kmax, mode, kcap, sr
             real(c_double), intent(in) :: th_cbf(*), ph_cbf(*), tr_th(*), tr_ph(*)
             type(c_ptr), value :: e, c_job, fsr_out, nnz_out
             integer(c_long), value :: lde, ldc
             integer(c_int), intent(out) :: k_out(*)
             type(midas_solve_info_t), intent(out) :: info
         end function midas_compute_efields_cbfm
+
+        integer(c_int) function midas_spherical_wave(ctx, cells, k0, quirks, ntr, th, ph, ref32, v, ldv) &
+            bind(c, name="midas_spherical_wave")
+            import :: c_int, c_long, c_ptr, c_double, midas_cells_t
+            type(c_ptr), value :: ctx
+            type(midas_cells_t), intent(in) :: cells
+            real(c_double), value :: k0
+            integer(c_int), value :: quirks, ntr, ref32
+            real(c_double), intent(in) :: th(*), ph(*)
+            type(c_ptr), value :: v
+            integer(c_long), value :: ldv
+        end function midas_spherical_wave
 
         integer(c_int) function midas_aca_block(ctx, cells, k0, quirks, r0, nr, c0, nc, kmax, eps, u, v, k_out) &
             bind(c, name="midas_aca_block")

@@ -146,6 +146,47 @@ def incident_field_f32(cells: Cells, th_deg, ph_deg, t0=0, t1=None):
     return E
 
 
+def incident_field_sph(cells: Cells, th_deg, ph_deg):
+    """Incident_Field_Spherical (Incident_Field.f90:65-166) over the cells given
+    (an extended block, block cells first).  Block centre = sequential sums / n
+    (:88-90), R_block = largest distance to it (:91), sources at
+    Rsource = 10 max(R_block, 2 Pi / k0) (:94) in the directions (:107-109);
+    phase = e^{i k0 R} / R and the components are complex(4) (:76, :126-148),
+    the products formed in double (mixed-kind promotion) before the rounding.
+    Pi is the reference's constant (single-precision acos(-1.) under quirk)."""
+    th, ph = _f64(th_deg), _f64(ph_deg)
+    n = cells.n
+    x, y, z = _f64(cells.x), _f64(cells.y), _f64(cells.z)
+    cx = float(np.cumsum(x)[-1]) / n     # cumsum: strictly sequential
+    cy = float(np.cumsum(y)[-1]) / n
+    cz = float(np.cumsum(z)[-1]) / n
+    rbl = np.sqrt(np.max((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2))
+    PI = pi(cells.quirk)
+    rs = 10.0 * max(rbl, 2.0 * PI / cells.k0)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    E = np.zeros((3 * n, 2 * len(th)), np.complex128, order="F")
+    for t in range(len(th)):
+        tr, pr_ = th[t] * PI / 180.0, ph[t] * PI / 180.0
+        xs = cx + rs * np.cos(tr)
+        ys = cy + rs * np.sin(tr) * np.cos(pr_)
+        zs = cz + rs * np.sin(tr) * np.sin(pr_)
+        dx, dy, dz = x - xs, y - ys, z - zs
+        R = np.sqrt(dx * dx + dy * dy + dz * dz)
+        ux, uy, uz = dx / R, dy / R, dz / R
+        kr = cells.k0 * R
+        ph_r, ph_i = f32(np.cos(kr) / R), f32(np.sin(kr) / R)
+        theta, phi = np.arccos(ux), np.arctan2(uz, uy)
+        st, ct, sp, cp = np.sin(theta), np.cos(theta), np.sin(phi), np.cos(phi)
+        cv = lambda re, im: f32(re) + 1j * f32(im)
+        E[0::3, t] = cv(-(ph_r * st), -(ph_i * st))
+        E[1::3, t] = cv(ph_r * ct * cp, ph_i * ct * cp)
+        E[2::3, t] = cv(ph_r * ct * sp, ph_i * ct * sp)
+        T = t + len(th)
+        E[1::3, T] = cv(-(ph_r * sp), -(ph_i * sp))
+        E[2::3, T] = cv(ph_r * cp, ph_i * cp)
+    return E
+
+
 def ff_coeff(cells: Cells, th, ph):
     ff = np.empty(cells.n, np.complex128)
     lib().or_ff_coeff(C.c_int(cells.n), _p(cells.x), _p(cells.y), _p(cells.z), _p(cells.kappa), _p(cells.che),
@@ -273,9 +314,11 @@ def cbf_threshold(eps_cells, aca_variant):
 
 
 def cbfm(cells: Cells, blocks, ext, th_cbf, ph_cbf, th, ph, use_aca=False, kmax=50, eps_aca=1e-4,
-         threshold=None, fp32_blocks=True, sr=False, res_sr=0.01, fsr_out=None):
+         threshold=None, fp32_blocks=True, sr=False, res_sr=0.01, fsr_out=None, iws=1):
     """Dense-block CBFM-E (Compute_EFields_CBFME.f90:348-449, 630-849, 896-1006)
     or its ACA variant (Compute_EFields_CBFME_ACA.f90:296-385, 569-636).
+    iws: CBF excitation, 1 plane waves, 2 / 3 spherical waves (type_iws,
+    Compute_EFields_CBFME.f90:363-369).
 
     blocks: list of 0-based cell-index arrays (contiguous ranges in block order);
     ext: list of extension cell-index arrays.  Returns (E, K per block, C list).
@@ -293,7 +336,7 @@ def cbfm(cells: Cells, blocks, ext, th_cbf, ph_cbf, th, ph, use_aca=False, kmax=
                 fsr_out.append(f)
         else:
             Zb = block_fill(cb, cb, fp32=fp32_blocks)
-        Eref = incident_field_f32(cb, th_cbf, ph_cbf)
+        Eref = incident_field_f32(cb, th_cbf, ph_cbf) if iws == 1 else incident_field_sph(cb, th_cbf, ph_cbf)
         Eb = np.linalg.solve(Zb, Eref)
         U, s, _ = np.linalg.svd(Eb, full_matrices=False)
         K = int(np.sum(s / s[0] >= threshold))
@@ -387,6 +430,17 @@ def ref_incident_field(c: Cells, th, ph, t0=1, t1=None):
     E = np.empty((3 * c.n, 2 * (t1 - t0 + 1)), np.complex128, order="F")
     r.ref_incident_field(C.c_int(c.n), _p(c.x), _p(c.y), _p(c.z), C.c_int(len(th)), _p(th), _p(ph), C.c_int(t0),
                          C.c_int(t1), _p(E))
+    return E
+
+
+def ref_incident_field_sph(c: Cells, th, ph, t0=1, t1=None):
+    r = ref()
+    th, ph = _f64(th), _f64(ph)
+    t1 = len(th) if t1 is None else t1
+    _ref_globals(r, c.k0, nbc=c.n, ntr=len(th))
+    E = np.empty((3 * c.n, 2 * (t1 - t0 + 1)), np.complex128, order="F")
+    r.ref_incident_field_sph(C.c_int(c.n), _p(c.x), _p(c.y), _p(c.z), C.c_int(len(th)), _p(th), _p(ph),
+                             C.c_int(t0), C.c_int(t1), _p(E))
     return E
 
 

@@ -181,6 +181,28 @@ contains
         call Incident_Field(1, n, cc, ntr, tr, t0, t1, E)
     end subroutine ref_incident_field
 
+    ! Incident_Field_Spherical (Incident_Field.f90:65-166): the CBF excitation
+    ! for type_iws = 2 / 3, over the n cells given (an extended block)
+    subroutine ref_incident_field_sph(n, x, y, z, ntr, th, ph, t0, t1, E) bind(c, name="ref_incident_field_sph")
+        integer(c_int), value :: n, ntr, t0, t1
+        real(c_double), intent(in) :: x(n), y(n), z(n), th(ntr), ph(ntr)
+        complex(c_double_complex), intent(out) :: E(3*n, 2*(t1-t0+1))
+        type(Cell), allocatable :: cc(:)
+        type(Dipole), allocatable :: tr(:)
+        integer(c_int), allocatable :: id(:)
+        real(c_double), allocatable :: zr(:)
+        complex(c_double_complex), allocatable :: zc(:)
+        integer :: c
+        allocate(cc(n), id(n), tr(ntr), zr(n), zc(n))
+        id = [(c, c = 1, n)]; zr = 0d0; zc = (0d0, 0d0)
+        call build_cells(n, id, x, y, z, zr, zr, zc, zc, cc)
+        do c = 1, ntr
+            tr(c)%theta = th(c)
+            tr(c)%phi = ph(c)
+        end do
+        call Incident_Field_Spherical(1, n, cc, ntr, tr, t0, t1, E)
+    end subroutine ref_incident_field_sph
+
     ! GetFFieldCoeff (Green_functions.f90:1399-1438)
     subroutine ref_ff_coeff(n, x, y, z, kappa, che, th, ph, ff) bind(c, name="ref_ff_coeff")
         integer(c_int), value :: n

@@ -17,6 +17,7 @@ Routines captured (reference file:line):
   SR_Green_s_tr_partial(_FN) Green_functions.f90:290-806 (CSR mask, norm)
   setfSR                    getParameters_CBFM.f90:113-168
   getTransmitters_CBFM      getParameters_CBFM.f90:1-110 (uniform grid)
+  Incident_Field_Spherical  Incident_Field.f90:65-166
 
 The fixture holds data only (inputs and expected outputs).
 """
@@ -129,6 +130,16 @@ def main():
     for ni in (94, 422):
         th, ph = o.ref_transmitters_un(ni)
         d[f"un_{ni}_th"], d[f"un_{ni}_ph"] = th, ph
+
+    # --- spherical-wave CBF excitation (Incident_Field_Spherical) -----------
+    # the 30-cell scatterer (Rsource = 10 wavelengths) and a block wider than a
+    # wavelength (Rsource = 10 block radii) on the 94-direction grid
+    d["sph1_E"] = o.ref_incident_field_sph(rc, d["g1_th"], d["g1_ph"])
+    pw = cluster(rng, 120, (1e-4, -2e-4, 3e-4), 1.5e-4)
+    kw = 2 * np.pi / 2.5e-4
+    cw = o.Cells(pw[:, 0], pw[:, 1], pw[:, 2], np.full(120, 1.5e-4), np.full(120, 1.5 + 0.1j), kw)
+    d.update(sph2_x=pw[:, 0].copy(), sph2_y=pw[:, 1].copy(), sph2_z=pw[:, 2].copy(), sph2_k0=np.float64(kw))
+    d["sph2_E"] = o.ref_incident_field_sph(cw, d["un_94_th"], d["un_94_ph"])
 
     np.savez_compressed(OUT, **d)
     print("wrote", OUT, os.path.getsize(OUT), "bytes;", {k_: v.shape for k_, v in d.items() if np.ndim(v) > 1})

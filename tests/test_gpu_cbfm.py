@@ -176,3 +176,46 @@ def test_cbfm_sr_vs_oracle(mid, oracle, dlca, use_aca):
     assert list(info["nnz"]) == nnz_o
     assert list(K) == list(Ko)
     assert _rel(E, Eo) < 1e-9
+
+
+@pytest.mark.parametrize("case", ["sph1", "sph2"])
+def test_spherical_wave_golden(mid, case):
+    """Incident_Field_Spherical on the GPU against the reference-compiled
+    routine's vectors: complex(4) values, equal or within one FP32 ulp
+    (device vs host libm in the double-precision steps before rounding)."""
+    gd = np.load(GOLD, allow_pickle=False)
+    if case == "sph1":
+        x, y, z, k0, th, ph = gd["g1_x"], gd["g1_y"], gd["g1_z"], float(gd["g1_k0"]), gd["g1_th"], gd["g1_ph"]
+    else:
+        x, y, z, k0 = gd["sph2_x"], gd["sph2_y"], gd["sph2_z"], float(gd["sph2_k0"])
+        th, ph = gd["un_94_th"], gd["un_94_ph"]
+    _case(f"spherical wave {case} n={len(x)} ntr={len(th)}")
+    n = len(x)
+    cells = api.CellTable(x, y, z, np.full(n, 1e-5), np.full(n, 1.5 + 0j))
+    V = mid.spherical_wave(cells, k0, th, ph)
+    ref = gd[case + "_E"]
+    ulp = np.spacing(np.maximum(np.abs(ref.real), np.abs(ref.imag)).astype(np.float32)).astype(np.float64)
+    d = np.maximum(np.abs(V.real - ref.real), np.abs(V.imag - ref.imag))
+    eq = np.mean((V == ref))
+    print("bit-equal fraction", eq, "max diff / ulp", (d / np.maximum(ulp, 1e-45)).max())
+    assert np.all(d <= ulp * 1.0001)
+    assert eq > 0.9
+
+
+def test_cbfm_spherical_vs_oracle(mid, oracle, dlca):
+    """CBFs from spherical-wave excitations (type_iws = 2 / 3): same K per block
+    and the same field as the oracle's CBFM with Incident_Field_Spherical."""
+    s = dlca
+    _case(f"cbfm spherical CBF excitation n={3 * s.n}")
+    oc = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    th_c, ph_c = oracle.transmitters_un(94)
+    tth, tph = np.array([0.0, 35.0]), np.array([0.0, 80.0])
+    E, K, info = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), iws=2)
+    Eo, Ko, _ = oracle.cbfm(oc, s.blocks, s.ext, th_c, ph_c, tth, tph, iws=2)
+    print("K", list(K), "oracle", Ko, "E err", _rel(E, Eo), info)
+    assert list(K) == list(Ko)
+    assert _rel(E, Eo) < 1e-9
+    Emom = np.linalg.solve(oracle.mom_fill(oc), oracle.mom_rhs(oc, tth, tph))
+    print("CBFM (spherical CBFs) vs MoM", _rel(E, Emom))
+    assert _rel(E, Emom) < 5e-2
+

@@ -86,6 +86,51 @@ def test_incident_field_golden(oracle, gold):
     assert np.abs(V - E).max() < 1e-6
 
 
+def _sph_cases(oracle, gold, quirk=1):
+    c1 = _cells(oracle, gold, "g1", quirk=quirk)
+    n2 = len(gold["sph2_x"])
+    c2 = oracle.Cells(gold["sph2_x"], gold["sph2_y"], gold["sph2_z"], np.full(n2, 1.5e-4), np.full(n2, 1.5 + 0.1j),
+                      float(gold["sph2_k0"]), quirk=quirk)
+    return [(c1, gold["g1_th"], gold["g1_ph"], gold["sph1_E"]), (c2, gold["un_94_th"], gold["un_94_ph"], gold["sph2_E"])]
+
+
+def test_incident_field_sph_golden(oracle, gold):
+    """Incident_Field_Spherical: source 10 wavelengths away (30-cell scatterer)
+    and 10 block radii away (a block wider than a wavelength), bit for bit."""
+    for c, th, ph, E_ref in _sph_cases(oracle, gold):
+        assert np.array_equal(oracle.incident_field_sph(c, th, ph), E_ref)
+
+
+def test_incident_field_sph_geometry(oracle, gold):
+    """What the formula fixes independently of the reference: V-pol and H-pol
+    are the unit theta- and phi-vectors of the local direction from the source,
+    so both are transverse to it, orthogonal to each other and of modulus
+    1/R; the source sits at Rsource = 10 max(R_block, lambda) from the
+    centroid in the given direction (checked through R at the centroid's
+    nearest cell)."""
+    for c, th, ph, _ in _sph_cases(oracle, gold):
+        E = oracle.incident_field_sph(c, th, ph)
+        x, y, z = c.x, c.y, c.z
+        cen = np.array([x.mean(), y.mean(), z.mean()])
+        rbl = np.sqrt(((x - cen[0]) ** 2 + (y - cen[1]) ** 2 + (z - cen[2]) ** 2).max())
+        rs = 10 * max(rbl, 2 * np.pi / c.k0)
+        nt = len(th)
+        for t in range(nt):
+            tr, pr = np.radians(th[t]), np.radians(ph[t])
+            src = cen + rs * np.array([np.cos(tr), np.sin(tr) * np.cos(pr), np.sin(tr) * np.sin(pr)])
+            d = np.stack([x, y, z], 1) - src
+            R = np.linalg.norm(d, axis=1)
+            u = d / R[:, None]
+            V = E[:, t].reshape(-1, 3)
+            H = E[:, t + nt].reshape(-1, 3)
+            tol = 1e-6 / R   # complex(4) rounding, pi to single precision
+            assert np.all(np.abs(np.abs(np.linalg.norm(V, axis=1)) - 1 / R) < 1e-5 / R)
+            assert np.all(np.abs(np.abs(np.linalg.norm(H, axis=1)) - 1 / R) < 1e-5 / R)
+            assert np.all(np.abs((V * u).sum(1)) < 10 * tol)
+            assert np.all(np.abs((H * u).sum(1)) < 10 * tol)
+            assert np.all(np.abs((V * H.conj()).sum(1)) < 10 * tol / R)
+
+
 def test_ff_coeff_golden(oracle, gold):
     c = _cells(oracle, gold, "g1")
     for i, (t, p) in enumerate(zip(gold["g1_th"], gold["g1_ph"])):
@@ -125,6 +170,8 @@ def test_golden_sensitive_to_pi_quirk(oracle, gold):
     c0 = oracle.Cells(np.zeros(16), np.zeros(16), np.zeros(16), gold["cp_sc"], gold["cp_eps"],
                       float(gold["cp_k0"]), quirk=0)
     assert not np.array_equal(c0.a, gold["cp_a"])
+    for c, th, ph, E_ref in _sph_cases(oracle, gold, quirk=0):
+        assert not np.array_equal(oracle.incident_field_sph(c, th, ph), E_ref)
 
 
 # ----------------------------------------------------------------------------
