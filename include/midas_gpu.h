@@ -201,6 +201,25 @@ int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const
                                int kcap, int sr, double res_sr, double *fsr_out, long *nnz_out,
                                midas_solve_info_t *info);
 
+/* ---------------- a17: block tables (host routines, no device work) ----------------
+ * Blocks are contiguous cell ranges block_start[b] .. block_start[b+1]-1.
+ * midas_block_frames: SetBlockCtrs (Division_blocks.f90:1616-1660); 13 doubles
+ * per block: BCubCont min x,y,z, max x,y,z, extent x,y,z, then BSphCont
+ * centre x,y,z (the box centre) and radius (largest distance to it).
+ * midas_extend_blocks: Extend_blocks (Extend_blocks.f90:1-162).  With sc the
+ * cell size (SimScatterer%Sc) and nc_extended layers: blocks whose bounding
+ * spheres lie within R_B + R_B' + nc_extended*sc are adjacent; the extension
+ * of block B is every cell of an adjacent block inside B's box grown by
+ * nc_extended*sc + sc/2, in cell order (0-based indices).  ext_start
+ * (nblocks + 1) always receives the offsets; ext_cells (capacity ext_cap) may
+ * be NULL to size it first.  adj_start / adj (optional) receive the adjacent
+ * blocks that gave cells (CBFM_Block%Badj).  No extension when nblocks == 1 or
+ * nc_extended == 0.  Fails (< 0) when an extension exceeds 10 times the
+ * largest block (Fact_Nbext_max, where the reference stops). */
+int midas_block_frames(const midas_cells_t *cells, int nblocks, const int *block_start, double *frames);
+int midas_extend_blocks(const midas_cells_t *cells, int nblocks, const int *block_start, double sc, int nc_extended,
+                        int *ext_start, int *ext_cells, long ext_cap, int *adj_start, int *adj, long adj_cap);
+
 /* ---------------- device-pointer variants (inputs resident in HBM) ----------------
  * Same semantics as above; every pointer is device memory; ordered on
  * midas_stream(ctx).  Z may be NULL in midas_dev_mom_solve when only the

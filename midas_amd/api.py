@@ -37,7 +37,8 @@ EXPORTS = (
     "midas_init", "midas_finalize", "midas_last_error", "midas_stream", "midas_cell_params", "midas_mom_fill",
     "midas_block_fill", "midas_plane_wave", "midas_spherical_wave", "midas_solve", "midas_scattering_matrix", "midas_cext_cabs",
     "midas_compute_efields_mom", "midas_aca_block", "midas_aca_matrix", "midas_compute_efields_cbfm",
-    "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_profile", "midas_profile_query",
+    "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_profile", "midas_profile_query", "midas_block_frames",
+    "midas_extend_blocks",
 )
 
 
@@ -127,6 +128,51 @@ class CellTable:
 
 def _cells(c):
     return c if isinstance(c, CellTable) else CellTable.from_scatterer(c)
+
+
+def _block_starts(blocks, n):
+    starts = np.zeros(len(blocks) + 1, np.int32)
+    for b, blk in enumerate(blocks):
+        blk = np.asarray(blk)
+        if len(blk) == 0 or np.any(np.diff(blk) != 1) or blk[0] != starts[b]:
+            raise MidasError("blocks must be non-empty contiguous cell ranges tiling the cells in order")
+        starts[b + 1] = blk[-1] + 1
+    if starts[-1] != n:
+        raise MidasError("blocks must cover every cell")
+    return starts
+
+
+def block_frames(cells, blocks):
+    """SetBlockCtrs for every block: (nblocks x 13) box min, max, extent,
+    sphere centre and radius.  Host routine, no GPU context needed."""
+    cells = _cells(cells)
+    starts = _block_starts(blocks, cells.n)
+    fr = np.empty((len(blocks), 13), np.float64)
+    cs = cells.struct()
+    _check(load().midas_block_frames(C.byref(cs), C.c_int(len(blocks)), _p(starts), _p(fr)), "block_frames")
+    return fr
+
+
+def extend_blocks(cells, blocks, sc, nc_extended=1):
+    """Extend_blocks: (extension cells per block, adjacent blocks per block),
+    0-based.  Host routine, no GPU context needed."""
+    cells = _cells(cells)
+    lib = load()
+    nb = len(blocks)
+    starts = _block_starts(blocks, cells.n)
+    es = np.zeros(nb + 1, np.int32)
+    cs = cells.struct()
+    _check(lib.midas_extend_blocks(C.byref(cs), C.c_int(nb), _p(starts), C.c_double(sc), C.c_int(nc_extended),
+                                   _p(es), None, C.c_long(0), None, None, C.c_long(0)), "extend_blocks")
+    ec = np.zeros(max(int(es[-1]), 1), np.int32)
+    ad = np.zeros(max(nb * nb, 1), np.int32)
+    ast = np.zeros(nb + 1, np.int32)
+    _check(lib.midas_extend_blocks(C.byref(cs), C.c_int(nb), _p(starts), C.c_double(sc), C.c_int(nc_extended),
+                                   _p(es), _p(ec), C.c_long(len(ec)), _p(ast), _p(ad), C.c_long(len(ad))),
+           "extend_blocks")
+    ext = [ec[es[b]:es[b + 1]].astype(np.int64) for b in range(nb)]
+    adj = [ad[ast[b]:ast[b + 1]].astype(np.int64) for b in range(nb)]
+    return ext, adj
 
 
 class Midas:

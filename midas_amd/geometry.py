@@ -343,15 +343,10 @@ def contiguous_blocks(n, nblocks):
 
 
 def extend_blocks(s: Scatterer, blocks, layers=1):
-    """Extension cells of each block: cells of other blocks whose centres lie in
-    the block's bounding box grown by `layers` cells (Extend_blocks.f90:1-162,
-    Nc_extended = 1)."""
-    p = np.stack([s.x, s.y, s.z], 1)
-    ext = []
-    for bi, b in enumerate(blocks):
-        lo = p[b].min(0) - layers * s.sc[b[0]] - s.sc[b[0]] / 2
-        hi = p[b].max(0) + layers * s.sc[b[0]] + s.sc[b[0]] / 2
-        inb = np.all((p >= lo) & (p <= hi), axis=1)
-        inb[b] = False
-        ext.append(np.nonzero(inb)[0])
+    """Extension cells of each block by the reference rule (Extend_blocks.f90:
+    1-162, Nc_extended = layers, SimScatterer%Sc = the lattice step), through
+    the library's midas_extend_blocks."""
+    from . import api
+    cells = api.CellTable(s.x, s.y, s.z, s.sc, s.eps)
+    ext, _ = api.extend_blocks(cells, blocks, float(s.sc[0]), layers)
     return ext

@@ -299,6 +299,48 @@ def transmitters_un(niws):
 # Dense solve and CBFM pipeline (numpy restatement)
 # --------------------------------------------------------------------------
 
+def block_frames(x, y, z, blocks):
+    """SetBlockCtrs (Division_blocks.f90:1631-1660): per block box min, max,
+    extent, the box centre and the largest distance to it (13 values)."""
+    out = []
+    for b in blocks:
+        p = np.stack([np.asarray(x)[b], np.asarray(y)[b], np.asarray(z)[b]], 1)
+        lo, hi = p.min(0), p.max(0)
+        c = (lo + hi) / 2.0
+        r = float(np.sqrt(((p - c) ** 2).sum(1)).max())
+        out.append(np.concatenate([lo, hi, hi - lo, c, [r]]))
+    return np.array(out)
+
+
+def extend_blocks(x, y, z, blocks, sc, nc_extended=1):
+    """Extend_blocks (Extend_blocks.f90:1-162), restated with numpy: adjacency
+    by bounding spheres (:51-79), the zone = box +- (nc_extended sc + sc/2)
+    (:83-88), cells of adjacent blocks inside it in block then cell order
+    (:97-133), adjacent blocks kept only if they gave cells (:130-134)."""
+    nb = len(blocks)
+    if nb == 1 or nc_extended == 0:
+        return [np.zeros(0, np.int64) for _ in blocks], [np.zeros(0, np.int64) for _ in blocks]
+    fr = block_frames(x, y, z, blocks)
+    P = np.stack([x, y, z], 1)
+    L = nc_extended * sc
+    ext, adj = [], []
+    for B in range(nb):
+        d = np.sqrt(((fr[:, 9:12] - fr[B, 9:12]) ** 2).sum(1))
+        near = [Bp for Bp in range(nb) if Bp != B and d[Bp] <= fr[B, 12] + fr[Bp, 12] + L]
+        lo = fr[B, 0:3] - L - sc / 2.0
+        hi = fr[B, 3:6] + L + sc / 2.0
+        e, a = [], []
+        for Bp in near:
+            idx = np.asarray(blocks[Bp])
+            inside = np.all((P[idx] >= lo) & (P[idx] <= hi), axis=1)
+            if inside.any():
+                e.append(idx[inside])
+                a.append(Bp)
+        ext.append(np.concatenate(e).astype(np.int64) if e else np.zeros(0, np.int64))
+        adj.append(np.array(a, np.int64))
+    return ext, adj
+
+
 def solve(Z, B):
     """PZGESV semantics (Compute_EFields_MoM.f90:173): LU with partial pivoting."""
     return np.linalg.solve(Z, B)
