@@ -141,6 +141,27 @@ int midas_scattering_matrix(midas_ctx *ctx, const midas_cells_t *cells, double k
 int midas_cext_cabs(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, const double *E, long lde,
                     int ntr, const double *th, const double *ph, double *C_ext, double *C_abs);
 
+/* ---------------- (f) orientation-averaged quantities ----------------
+ * Compute_Scattering_Quantities_2 (Compute_Scattering_Quantities.f90:593-1013)
+ * from S (nrx_tot x 4ntr, lds; Vv, Vh, Hv, Hh per transmitter, as
+ * midas_scattering_matrix writes it): rows 0..nrx-1 are the scattering
+ * receivers (rx_th, rx_ph), followed by the forward and backward directions
+ * as get_trans_Receiv lays them out (forward = receiver t when ntr == nrx and
+ * the receivers are sd / lb, else row nrx + t; backward = nrx + t when
+ * ntr == nrx, else nrx + ntr + t).  rx_type: MIDAS_RX_SD / MIDAS_RX_RF (equal
+ * weights 4 pi / nrx) or MIDAS_RX_LB (weights rx_w, Lebedev); tr_type the same
+ * for the average over transmitters (tr_w for lb).  dm is the scatterer's
+ * diameter (SimScatterer%dm): ap = dm/2, X = k0 dm/2.  C_ext, C_abs are the
+ * complex internal-field cross sections (midas_cext_cabs).  Q (5 x ntr):
+ * Q_ext (far field, or the internal-field value when cext_intfields),
+ * Q_abs (internal field), Q_sca, Q_bks, g; Q_av (5): their averages. */
+enum { MIDAS_RX_SD = 0, MIDAS_RX_RF = 1, MIDAS_RX_LB = 2 };
+int midas_scattering_quantities(midas_ctx *ctx, const double *S, long lds, int nrx_tot, int nrx, int ntr,
+                                const double *rx_th, const double *rx_ph, int rx_type, const double *rx_w,
+                                const double *tr_th, const double *tr_ph, int tr_type, const double *tr_w, double k0,
+                                double dm, int quirks, const double *C_ext, const double *C_abs, int cext_intfields,
+                                double *Q, double *Q_av);
+
 /* ---------------- Compute_EFields_MoM drop-in ----------------
  * Compute_EFields_MoM(Cells, Transmitters, Receivers, S_total, C_ext, C_abs)
  * (Compute_EFields_MoM.f90:1-19): fill, solve, S_total (nrx x 4ntr),

@@ -38,8 +38,10 @@ EXPORTS = (
     "midas_block_fill", "midas_plane_wave", "midas_spherical_wave", "midas_solve", "midas_scattering_matrix", "midas_cext_cabs",
     "midas_compute_efields_mom", "midas_aca_block", "midas_aca_matrix", "midas_compute_efields_cbfm",
     "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_profile", "midas_profile_query", "midas_block_frames",
-    "midas_extend_blocks",
+    "midas_extend_blocks", "midas_scattering_quantities",
 )
+
+RX_SD, RX_RF, RX_LB = 0, 1, 2
 
 
 class MidasError(RuntimeError):
@@ -251,6 +253,26 @@ class Midas:
                                              C.c_int(len(th)), _p(th), _p(ph), C.c_int(int(ref32)), _p(V),
                                              C.c_long(3 * cells.n)), "spherical_wave")
         return V
+
+    # -- (f) orientation-averaged quantities ---------------------------------
+    def scattering_quantities(self, S, nrx, ntr, rx_dirs, tr_dirs, k0, dm, cext, cabs, rx_type=RX_SD, rx_w=None,
+                              tr_type=RX_SD, tr_w=None, cext_intfields=False, quirks=Q_ALL):
+        """Q (ntr x 5: Q_ext, Q_abs, Q_sca, Q_bks, g) and their averages."""
+        S = np.asfortranarray(S, np.complex128)
+        f = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
+        rth, rph = (f(a) for a in rx_dirs)
+        tth, tph = (f(a) for a in tr_dirs)
+        rw, tw = f(rx_w), f(tr_w)
+        ce, ca = np.ascontiguousarray(cext, np.complex128), np.ascontiguousarray(cabs, np.complex128)
+        Q = np.zeros((ntr, 5), np.float64)
+        av = np.zeros(5, np.float64)
+        _check(self.lib.midas_scattering_quantities(self.ctx, _p(S), C.c_long(S.shape[0]), C.c_int(S.shape[0]),
+                                                    C.c_int(nrx), C.c_int(ntr), _p(rth), _p(rph), C.c_int(rx_type),
+                                                    _p(rw), _p(tth), _p(tph), C.c_int(tr_type), _p(tw),
+                                                    C.c_double(k0), C.c_double(dm), C.c_int(quirks), _p(ce), _p(ca),
+                                                    C.c_int(int(cext_intfields)), _p(Q), _p(av)),
+               "scattering_quantities")
+        return Q, av
 
     # -- a4 -----------------------------------------------------------------
     def solve(self, Z, B, mode=SOLVE_FP64, max_iter=0, tol=0.0, nb=0):

@@ -558,6 +558,95 @@ int midas_cext_cabs(midas_ctx *ctx, const midas_cells_t *cells, double k0, int q
     return 0;
 }
 
+int midas_scattering_quantities(midas_ctx *ctx, const double *S, long lds, int nrx_tot, int nrx, int ntr,
+                                const double *rx_th, const double *rx_ph, int rx_type, const double *rx_w,
+                                const double *tr_th, const double *tr_ph, int tr_type, const double *tr_w, double k0,
+                                double dm, int quirks, const double *C_ext, const double *C_abs, int cext_intfields,
+                                double *Q, double *Q_av)
+{
+    MIDAS_CHECK(ctx && S && rx_th && rx_ph && tr_th && tr_ph && C_ext && C_abs && Q && Q_av, -1,
+                "scattering_quantities: bad arguments");
+    MIDAS_CHECK(nrx > 0 && ntr > 0 && lds >= nrx_tot && k0 > 0 && dm > 0, -1,
+                "scattering_quantities: nrx %d, ntr %d, lds %ld, nrx_tot %d", nrx, ntr, lds, nrx_tot);
+    MIDAS_CHECK(rx_type >= MIDAS_RX_SD && rx_type <= MIDAS_RX_LB && tr_type >= MIDAS_RX_SD && tr_type <= MIDAS_RX_LB,
+                -1, "scattering_quantities: receiver / transmitter rule %d / %d", rx_type, tr_type);
+    MIDAS_CHECK(rx_type != MIDAS_RX_LB || rx_w, -1, "scattering_quantities: lb receivers need weights");
+    MIDAS_CHECK(tr_type != MIDAS_RX_LB || ntr == 1 || tr_w, -1, "scattering_quantities: lb transmitters need weights");
+    // forward / backward receivers (Compute_Scattering_Quantities.f90:773-777, 867-871)
+    const bool same = ntr == nrx;
+    auto fw = [&](int t) { return same && (rx_type == MIDAS_RX_SD || rx_type == MIDAS_RX_LB) ? t : nrx + t; };
+    auto bk = [&](int t) { return same ? nrx + t : nrx + ntr + t; };
+    MIDAS_CHECK(bk(ntr - 1) < nrx_tot && fw(ntr - 1) < nrx_tot, -1,
+                "scattering_quantities: nrx_tot %d has no forward / backward receivers", nrx_tot);
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    DBuf dS, drt, drp, dw, dtt, dtp, dout;
+    TRY(dS.alloc(sizeof(double2) * (size_t)nrx * 4 * ntr));
+    MIDAS_HIP(hipMemcpy2DAsync(dS.p, sizeof(double2) * nrx, S, sizeof(double2) * lds, sizeof(double2) * nrx, 4 * ntr,
+                               hipMemcpyHostToDevice, st));
+    TRY(upload(drt, rx_th, sizeof(double) * nrx, st));
+    TRY(upload(drp, rx_ph, sizeof(double) * nrx, st));
+    if (rx_type == MIDAS_RX_LB) TRY(upload(dw, rx_w, sizeof(double) * nrx, st));
+    TRY(upload(dtt, tr_th, sizeof(double) * ntr, st));
+    TRY(upload(dtp, tr_ph, sizeof(double) * ntr, st));
+    TRY(dout.alloc(sizeof(double) * 2 * ntr));
+    TRY(scatq_sums(dS.as<double2>(), nrx, nrx, ntr, drt.as<double>(), drp.as<double>(),
+                   rx_type == MIDAS_RX_LB ? dw.as<double>() : nullptr, dtt.as<double>(), dtp.as<double>(), quirks,
+                   dout.as<double>(), st));
+    std::vector<double> sums(2 * ntr);
+    TRY(copy_sync(sums.data(), dout.p, sizeof(double) * 2 * ntr, hipMemcpyDeviceToHost, st));
+    // per transmitter, the reference's expressions and literals
+    const double pi = host_pi(quirks), ap = dm / 2.0, X = k0 * dm / 2.0;
+    auto sq = [](double re, double im) {
+        const double m = std::hypot(re, im);
+        return m * m;
+    };
+    auto s_at = [&](int r, int c) { return S + 2 * ((long)r + (long)c * lds); };
+    for (int t = 0; t < ntr; ++t) {
+        const double *f = s_at(fw(t), 4 * t), *fh = s_at(fw(t), 4 * t + 3);
+        const double q_ext = (2 / (X * X)) * (std::fabs(fh[1]) + std::fabs(f[1]));
+        // C / (pi ap^2) taken to its real part, as the real(8) assignment does
+        const double q_ext_intf = C_ext[2 * t] / (pi * ap * ap);
+        const double q_abs_intf = C_abs[2 * t] / (pi * ap * ap);
+        double q_sca, g;
+        if (rx_type == MIDAS_RX_LB) {
+            const double q_sd = sums[2 * t];
+            const double c_sca = 1.0 / (2 * k0 * k0) * q_sd;
+            q_sca = (4 * pi) / (2 * pi * X * X) * q_sd;
+            g = 1.0 / (2.0 * k0 * k0 * c_sca) * sums[2 * t + 1];
+        } else {
+            const double q_sd = (4 * pi / nrx) * sums[2 * t];
+            const double c_sca = 1.0 / (2 * k0 * k0) * q_sd;
+            q_sca = 1 / (2 * pi * X * X) * q_sd;
+            g = 1.0 / (2.0 * k0 * k0 * c_sca) * (4 * pi / nrx) * sums[2 * t + 1];
+        }
+        const double *b1 = s_at(bk(t), 4 * t), *b2 = s_at(bk(t), 4 * t + 1), *b3 = s_at(bk(t), 4 * t + 2),
+                     *b4 = s_at(bk(t), 4 * t + 3);
+        const double q_bks = (1.0 / (2.0 * pi * X * X)) *
+                             (((sq(b1[0], b1[1]) + sq(b4[0], b4[1])) + sq(b2[0], b2[1])) + sq(b3[0], b3[1]));
+        double *q = Q + 5L * t;
+        q[0] = cext_intfields ? q_ext_intf : q_ext;
+        q[1] = q_abs_intf;   // Q_abs is always the internal-field value (:877)
+        q[2] = q_sca;
+        q[3] = q_bks;
+        q[4] = g;
+    }
+    // orientation averages (:896-918)
+    for (int j = 0; j < 5; ++j) {
+        double a = 0.0;
+        if (ntr == 1) {
+            a = Q[j];
+        } else if (tr_type != MIDAS_RX_LB) {
+            for (int t = 0; t < ntr; ++t) a += Q[5L * t + j];
+            a /= ntr;
+        } else {
+            for (int t = 0; t < ntr; ++t) a = a + tr_w[t] * Q[5L * t + j];
+        }
+        Q_av[j] = a;
+    }
+    return 0;
+}
+
 // Fill (FP64 Z, and the FP32 copy W for mixed modes) + RHS + solve, all on
 // the device; X (3n x 2ntr).  Shared by the drop-in and the device entry point.
 static int mom_solve_dev(midas_ctx *ctx, const DevCells &cells, double k0, int quirks, int ntr, const double *dth,

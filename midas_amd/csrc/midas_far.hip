@@ -151,6 +151,70 @@ __global__ __launch_bounds__(256) void k_cext_cabs(DevCells cells, const double2
 }
 
 // S (nrx x 4*T*npol, lds) from E (3N x 2T).  Workspace: FF (nrx x N), E3 (N x 6T), G (nrx x 6T).
+// Orientation-averaged quantities (Compute_Scattering_Quantities.f90:593-1013):
+// per transmitter t (one workgroup), over the NRx scattering receivers,
+//   f(r)  = w_r (|S_hh|^2 + |S_vv|^2 + |S_vh|^2 + |S_hv|^2)    (w_r = 1 unless weighted)
+//   out   = (sum_r f(r), sum_r f(r) k_i . k_s(r))
+// with k = -(sin th cos ph, sin th sin ph, cos th) for the incident and the
+// scattered direction (:784-788, :799-805).  The host turns the sums into
+// Q_sca and g with the reference's constants.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(256) void k_scatq_sums(const double2 *__restrict__ S, long lds, int nrx,
+                                                    const double *__restrict__ rth, const double *__restrict__ rph,
+                                                    const double *__restrict__ w, const double *__restrict__ tth,
+                                                    const double *__restrict__ tph, double pi,
+                                                    double *__restrict__ out)
+{
+    __shared__ double r1[256], r2[256];
+    const int t = blockIdx.x;
+    const double thi = tth[t] * pi / 180.0, phi = tph[t] * pi / 180.0;
+    const double kix = -(sin(thi) * cos(phi)), kiy = -(sin(thi) * sin(phi)), kiz = -cos(thi);
+    const double2 *vv = S + (long)(4 * t) * lds, *vh = vv + lds, *hv = vh + lds, *hh = hv + lds;
+    auto a2 = [](double2 z) {
+        const double m = hypot(z.x, z.y);
+        return m * m;
+    };
+    double s1 = 0.0, s2 = 0.0;
+    for (int r = threadIdx.x; r < nrx; r += 256) {
+        double f = ((a2(hh[r]) + a2(vv[r])) + a2(vh[r])) + a2(hv[r]);
+        if (WEIGHTED) f = w[r] * f;
+        const double ths = rth[r] * pi / 180.0, phs = rph[r] * pi / 180.0;
+        const double ksx = -(sin(ths) * cos(phs)), ksy = -(sin(ths) * sin(phs)), ksz = -cos(ths);
+        const double cg = kix * ksx + kiy * ksy + kiz * ksz;
+        s1 += f;
+        s2 += f * cg;
+    }
+    r1[threadIdx.x] = s1;
+    r2[threadIdx.x] = s2;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            r1[threadIdx.x] += r1[threadIdx.x + h];
+            r2[threadIdx.x] += r2[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * t] = r1[0];
+        out[2 * t + 1] = r2[0];
+    }
+}
+
+int scatq_sums(const double2 *S, long lds, int nrx, int ntr, const double *rth, const double *rph, const double *w,
+               const double *tth, const double *tph, int quirks, double *out, hipStream_t st)
+{
+    if (ntr <= 0) return 0;
+    const double pi = host_pi(quirks);
+    if (w)
+        hipLaunchKernelGGL(k_scatq_sums<true>, dim3(ntr), dim3(256), 0, st, S, lds, nrx, rth, rph, w, tth, tph, pi,
+                           out);
+    else
+        hipLaunchKernelGGL(k_scatq_sums<false>, dim3(ntr), dim3(256), 0, st, S, lds, nrx, rth, rph, w, tth, tph, pi,
+                           out);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
 int far_field(const DevCells &cells, const double2 *E, long ld, int T, int nrx, const double *rth,
               const double *rph, int npol, const double *beta, double k0, int quirks, double2 *S, long lds,
               double2 *FF, double2 *E3, double2 *G, hipStream_t st)

@@ -7,6 +7,7 @@ module midas_gpu_c
 
     integer(c_int), parameter :: MIDAS_Q_PI = 1, MIDAS_Q_LITERALS = 2, MIDAS_Q_CEXT_SUM = 4, &
                                  MIDAS_Q_FP32_BLOCKS = 8, MIDAS_Q_ALL = 15
+    integer(c_int), parameter :: MIDAS_RX_SD = 0, MIDAS_RX_RF = 1, MIDAS_RX_LB = 2
     integer(c_int), parameter :: MIDAS_SOLVE_FP64 = 0, MIDAS_SOLVE_C32_IR = 1, MIDAS_SOLVE_F16_IR = 2, &
                                  MIDAS_SOLVE_F16X3_IR = 3, MIDAS_SOLVE_BF16_IR = 4, MIDAS_SOLVE_BF16X3_IR = 5
 
@@ -89,6 +90,32 @@ module midas_gpu_c
             type(c_ptr), value :: v
             integer(c_long), value :: ldv
         end function midas_spherical_wave
+
+        integer(c_int) function midas_scattering_quantities(ctx, s, lds, nrx_tot, nrx, ntr, rx_th, rx_ph, rx_type, &
+                                                            rx_w, tr_th, tr_ph, tr_type, tr_w, k0, dm, quirks, &
+                                                            c_ext, c_abs, cext_intfields, q, q_av) &
+            bind(c, name="midas_scattering_quantities")
+            import :: c_int, c_long, c_ptr, c_double
+            type(c_ptr), value :: ctx, s, rx_w, tr_w
+            integer(c_long), value :: lds
+            integer(c_int), value :: nrx_tot, nrx, ntr, rx_type, tr_type, quirks, cext_intfields
+            real(c_double), intent(in) :: rx_th(*), rx_ph(*), tr_th(*), tr_ph(*), c_ext(*), c_abs(*)
+            real(c_double), value :: k0, dm
+            real(c_double), intent(out) :: q(*), q_av(*)
+        end function midas_scattering_quantities
+
+        integer(c_int) function midas_extend_blocks(cells, nblocks, block_start, sc, nc_extended, ext_start, &
+                                                    ext_cells, ext_cap, adj_start, adj, adj_cap) &
+            bind(c, name="midas_extend_blocks")
+            import :: c_int, c_long, c_ptr, c_double, midas_cells_t
+            type(midas_cells_t), intent(in) :: cells
+            integer(c_int), value :: nblocks, nc_extended
+            integer(c_int), intent(in) :: block_start(*)
+            real(c_double), value :: sc
+            integer(c_int), intent(out) :: ext_start(*)
+            type(c_ptr), value :: ext_cells, adj_start, adj
+            integer(c_long), value :: ext_cap, adj_cap
+        end function midas_extend_blocks
 
         integer(c_int) function midas_aca_block(ctx, cells, k0, quirks, r0, nr, c0, nc, kmax, eps, u, v, k_out) &
             bind(c, name="midas_aca_block")

@@ -216,6 +216,65 @@ def cext_cabs(cells: Cells, E, th_deg, ph_deg, q6=1):
     return ce, ca
 
 
+RX_SD, RX_RF, RX_LB = 0, 1, 2
+
+
+def scattering_quantities(S, nrx, ntr, rx_th, rx_ph, tr_th, tr_ph, k0, dm, cext, cabs, rx_type=RX_SD,
+                          rx_w=None, tr_type=RX_SD, tr_w=None, cext_intfields=False, quirk=1):
+    """Compute_Scattering_Quantities_2 (Compute_Scattering_Quantities.f90:593-1013),
+    receiver rules sd / rf (equal weights 4 pi / NRx, :812-831) and lb
+    (Lebedev weights, :790-811).  S: NRx_tot x 4 NTr (Vv, Vh, Hv, Hh per
+    transmitter).  Returns Q (ntr x 5: Q_ext, Q_abs, Q_sca, Q_bks, g) and the
+    orientation averages (5).  Sums are sequential, as Fortran's SUM."""
+    PI = pi(quirk)
+    ap = dm / 2.0
+    X = k0 * dm / 2.0
+    a2 = lambda z: np.hypot(z.real, z.imag) ** 2
+    rad = lambda d: d * PI / 180.0
+    Q = np.zeros((ntr, 5))
+    for t in range(ntr):
+        kix = -(np.sin(rad(tr_th[t])) * np.cos(rad(tr_ph[t])))
+        kiy = -(np.sin(rad(tr_th[t])) * np.sin(rad(tr_ph[t])))
+        kiz = -np.cos(rad(tr_th[t]))
+        fw = t if (ntr == nrx and rx_type in (RX_SD, RX_LB)) else nrx + t
+        bk = nrx + t if (ntr == nrx and rx_type in (RX_SD, RX_LB, RX_RF)) else nrx + ntr + t
+        q_ext = (2 / (X * X)) * (abs(S[fw, 4 * t + 3].imag) + abs(S[fw, 4 * t + 0].imag))
+        q_ext_intf = (cext[t] / (PI * ap ** 2)).real
+        Svv, Svh, Shv, Shh = (S[:nrx, 4 * t + j] for j in range(4))
+        f = ((a2(Shh) + a2(Svv)) + a2(Svh)) + a2(Shv)
+        if rx_type == RX_LB:
+            f = rx_w * f
+        ksx = -(np.sin(rad(rx_th)) * np.cos(rad(rx_ph)))
+        ksy = -(np.sin(rad(rx_th)) * np.sin(rad(rx_ph)))
+        ksz = -np.cos(rad(rx_th))
+        cg = kix * ksx + kiy * ksy + kiz * ksz
+        fg = f * cg
+        s1, s2 = float(np.cumsum(f)[-1]), float(np.cumsum(fg)[-1])
+        if rx_type == RX_LB:
+            q_sd = s1
+            c_sca = 1.0 / (2 * k0 ** 2) * q_sd
+            q_sca = (4 * PI) / (2 * PI * X * X) * q_sd
+            g = 1.0 / (2.0 * k0 ** 2.0 * c_sca) * s2
+        else:
+            q_sd = (4 * PI / nrx) * s1
+            c_sca = 1.0 / (2 * k0 ** 2) * q_sd
+            q_sca = 1 / (2 * PI * X * X) * q_sd
+            g = 1.0 / (2.0 * k0 ** 2.0 * c_sca) * (4 * PI / nrx) * s2
+        q_bks = (1.0 / (2.0 * PI * X * X)) * (((a2(S[bk, 4 * t + 0]) + a2(S[bk, 4 * t + 3])) + a2(S[bk, 4 * t + 1]))
+                                              + a2(S[bk, 4 * t + 2]))
+        q_abs_intf = (cabs[t] / (PI * ap ** 2)).real
+        Q[t] = [q_ext_intf if cext_intfields else q_ext, q_abs_intf, q_sca, q_bks, g]
+    if ntr == 1:
+        av = Q[0].copy()
+    elif tr_type in (RX_SD, RX_RF):
+        av = np.array([float(np.cumsum(Q[:, j])[-1]) / ntr for j in range(5)])
+    else:
+        av = np.zeros(5)
+        for i in range(ntr):
+            av = av + tr_w[i] * Q[i]
+    return Q, av
+
+
 def aca(cells: Cells, r0, nr, c0, nc, kmax=50, eps=1e-4, fp32=True):
     """ACA of the block rows cells[r0:r0+nr] x columns cells[c0:c0+nc] (0-based)."""
     U = np.zeros((3 * nr, kmax), np.complex128, order="F")

@@ -238,3 +238,35 @@ def test_compute_efields_mom_many_directions(mid, oracle, mode):
     ce0, _ = mid.cext_cabs(s, s.k0, E, th, ph, quirks=api.Q_ALL & ~api.Q_CEXT_SUM)
     print("Cext spread over directions", np.ptp(ce0.real) / abs(ce0.real).max())
     assert np.ptp(ce0.real) < 2e-2 * abs(ce0.real).max()
+
+
+@pytest.mark.parametrize("layout", ["separate", "shared"])
+@pytest.mark.parametrize("rx_type", [api.RX_SD, api.RX_RF, api.RX_LB])
+def test_scattering_quantities_vs_oracle(mid, oracle, rx_type, layout):
+    """Orientation-averaged quantities (Compute_Scattering_Quantities_2) from
+    the same S on the GPU and in the oracle: the receiver sums, the
+    reference's constants, both forward / backward layouts (ntr == nrx with
+    sd / lb receivers puts the forward direction among the receivers) and
+    the Lebedev-weighted transmitter average."""
+    rng = np.random.default_rng(5 + rx_type)
+    nrx = 40 if layout == "shared" else 57
+    ntr = 40 if layout == "shared" else 6
+    nrx_tot = nrx + (ntr if layout == "shared" else 2 * ntr)
+    _case(f"scattering quantities rx_type={rx_type} layout={layout} nrx={nrx} ntr={ntr}")
+    S = rng.normal(size=(nrx_tot, 4 * ntr)) + 1j * rng.normal(size=(nrx_tot, 4 * ntr))
+    rth, rph = rng.uniform(0, 180, nrx), rng.uniform(0, 360, nrx)
+    tth, tph = rng.uniform(0, 180, ntr), rng.uniform(0, 360, ntr)
+    rw = rng.uniform(0.5, 1.5, nrx) / nrx if rx_type == api.RX_LB else None
+    tw = rng.uniform(0.5, 1.5, ntr) / ntr
+    ce = rng.normal(size=ntr) + 1j * rng.normal(size=ntr)
+    ca = rng.normal(size=ntr) + 1j * rng.normal(size=ntr)
+    k0, dm = 2 * np.pi / 5e-4, 3e-4
+    for tr_type, w in ((api.RX_SD, None), (api.RX_LB, tw)):
+        Q, av = mid.scattering_quantities(S, nrx, ntr, (rth, rph), (tth, tph), k0, dm, ce, ca, rx_type=rx_type,
+                                          rx_w=rw, tr_type=tr_type, tr_w=w)
+        Qo, avo = oracle.scattering_quantities(S, nrx, ntr, rth, rph, tth, tph, k0, dm, ce, ca, rx_type=rx_type,
+                                               rx_w=rw, tr_type=tr_type, tr_w=w)
+        print("max rel", np.abs(Q - Qo).max() / np.abs(Qo).max(), np.abs(av - avo).max())
+        assert np.allclose(Q, Qo, rtol=1e-12, atol=0)
+        assert np.allclose(av, avo, rtol=1e-12, atol=0)
+

@@ -32,6 +32,29 @@ def mie_q(x, m):
     return qext, qsca
 
 
+def mie_g(x, m):
+    """Asymmetry parameter <cos> of a homogeneous sphere (Bohren & Huffman,
+    eq. 4.62, from the same a_n, b_n as mie_q)."""
+    nstop = int(x + 4 * x ** (1 / 3) + 2) + 1
+    n = np.arange(1, nstop + 1)
+    mx = m * x
+    nmx = int(max(nstop, abs(mx))) + 16
+    D = np.zeros(nmx + 1, complex)
+    for k in range(nmx, 0, -1):
+        D[k - 1] = k / mx - 1 / (D[k] + k / mx)
+    D = D[1:nstop + 1]
+    psi, psim1 = x * spherical_jn(n, x), x * spherical_jn(n - 1, x)
+    xi = psi + 1j * x * spherical_yn(n, x)
+    xim1 = psim1 + 1j * x * spherical_yn(n - 1, x)
+    a = ((D / m + n / x) * psi - psim1) / ((D / m + n / x) * xi - xim1)
+    b = ((D * m + n / x) * psi - psim1) / ((D * m + n / x) * xi - xim1)
+    qsca = 2 / x ** 2 * np.sum((2 * n + 1) * (abs(a) ** 2 + abs(b) ** 2))
+    nn = n[:-1]
+    t1 = nn * (nn + 2) / (nn + 1) * (a[:-1] * a[1:].conj() + b[:-1] * b[1:].conj()).real
+    t2 = (2 * nn + 1) / (nn * (nn + 1)) * (a[:-1] * b[:-1].conj()).real
+    return 4 / (x ** 2 * qsca) * np.sum(t1 + t2)
+
+
 def test_mie_series_small_particle_limit():
     """Rayleigh limit of the Mie routine itself: Qsca -> 8/3 x^4 |(m^2-1)/(m^2+2)|^2."""
     x, m = 0.01, 1.5 + 0.0j
@@ -80,3 +103,42 @@ def test_energy_balance_and_optical_theorem(oracle, sphere_solution):
     assert 4 * np.pi / s.k0 ** 2 * S0[0, 3].imag == pytest.approx(ce[0].real, rel=2e-2)
     # no cross-polarised forward scattering from a sphere
     assert abs(S0[0, 1]) < 1e-6 * abs(S0[0, 0])
+
+
+def _dirs(n):
+    """Quasi-uniform directions (Fibonacci lattice, theta from +x as MIDAS)."""
+    i = np.arange(n) + 0.5
+    ct = 1 - 2 * i / n
+    return np.degrees(np.arccos(ct)), np.degrees(np.mod(i * np.pi * (3 - np.sqrt(5)), 2 * np.pi))
+
+
+def test_scattering_quantities_vs_mie(oracle, sphere_solution):
+    """Compute_Scattering_Quantities_2 with equal-weight (sd) receivers on a
+    discretised sphere: Q_ext from the forward S (optical theorem) and from
+    the internal field, Q_sca and g against the Mie series, Q_abs = the
+    internal-field value; one orientation and the average of two (a sphere's
+    quantities do not depend on it)."""
+    s, c, E, th, ph = sphere_solution
+    ce, ca = oracle.cext_cabs(c, E, th, ph, q6=0)
+    nrx = 1200
+    rth, rph = _dirs(nrx)
+    # receivers, then forward (the incident direction) and backward
+    Srx = oracle.scattering_matrix(c, E, np.r_[rth, 0.0, 180.0], np.r_[rph, 0.0, 0.0])
+    a_eq = (3 * s.n * s.sc[0] ** 3 / (4 * np.pi)) ** (1 / 3)
+    dm = 2 * a_eq
+    Q, av = oracle.scattering_quantities(Srx, nrx, 1, rth, rph, th, ph, s.k0, dm, ce, ca)
+    qe, qs = mie_q(s.k0 * a_eq, np.sqrt(s.eps[0]))
+    g = mie_g(s.k0 * a_eq, np.sqrt(s.eps[0]))
+    print("Q", Q[0], "Mie qext qsca g", qe, qs, g)
+    assert Q[0, 0] == pytest.approx(qe, rel=2e-2)
+    assert Q[0, 1] == pytest.approx(qe - qs, rel=3e-2)
+    assert Q[0, 2] == pytest.approx(qs, rel=3e-2)
+    assert Q[0, 4] == pytest.approx(g, rel=3e-2)
+    Qi, _ = oracle.scattering_quantities(Srx, nrx, 1, rth, rph, th, ph, s.k0, dm, ce, ca, cext_intfields=True)
+    assert Qi[0, 0] == pytest.approx(Q[0, 0], rel=2e-2)
+    assert np.allclose(av, Q[0])
+    # Lebedev branch with equal weights 1/N is the sd rule
+    Ql, _ = oracle.scattering_quantities(Srx, nrx, 1, rth, rph, th, ph, s.k0, dm, ce, ca,
+                                         rx_type=oracle.RX_LB, rx_w=np.full(nrx, 1.0 / nrx))
+    assert np.allclose(Ql, Q, rtol=1e-12)
+
