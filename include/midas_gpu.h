@@ -141,6 +141,16 @@ int midas_scattering_matrix(midas_ctx *ctx, const midas_cells_t *cells, double k
 int midas_cext_cabs(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, const double *E, long lde,
                     int ntr, const double *th, const double *ph, double *C_ext, double *C_abs);
 
+/* ---------------- (f) near field ----------------
+ * Compute_Scattered_Fields (Compute_Scattered_Fields.f90:75-136) with
+ * Green_s_dt (Green_functions.f90:1335-1397), the path taken when the
+ * far-field approximation is off (FFA = 0): the scattered field of E
+ * (3n x 2ntr, lde) at receivers rso (cos th, sin th cos ph, sin th sin ph)
+ * (Rso), projected on the receivers' theta / phi directions.  Es (nrx x 4ntr,
+ * lds): Vv, Vh, Hv, Hh per transmitter.  FP64. */
+int midas_near_field(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, const double *E, long lde,
+                     int ntr, int nrx, const double *rth, const double *rph, double rso, double *Es, long lds);
+
 /* ---------------- (f) orientation-averaged quantities ----------------
  * Compute_Scattering_Quantities_2 (Compute_Scattering_Quantities.f90:593-1013)
  * from S (nrx_tot x 4ntr, lds; Vv, Vh, Hv, Hh per transmitter, as

@@ -38,7 +38,7 @@ EXPORTS = (
     "midas_block_fill", "midas_plane_wave", "midas_spherical_wave", "midas_solve", "midas_scattering_matrix", "midas_cext_cabs",
     "midas_compute_efields_mom", "midas_aca_block", "midas_aca_matrix", "midas_compute_efields_cbfm",
     "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_profile", "midas_profile_query", "midas_block_frames",
-    "midas_extend_blocks", "midas_scattering_quantities",
+    "midas_extend_blocks", "midas_scattering_quantities", "midas_near_field",
 )
 
 RX_SD, RX_RF, RX_LB = 0, 1, 2
@@ -253,6 +253,19 @@ class Midas:
                                              C.c_int(len(th)), _p(th), _p(ph), C.c_int(int(ref32)), _p(V),
                                              C.c_long(3 * cells.n)), "spherical_wave")
         return V
+
+    # -- (f) near field (FFA = 0) --------------------------------------------
+    def near_field(self, cells, k0, E, rth, rph, rso, quirks=Q_ALL):
+        cells = _cells(cells)
+        E = np.asfortranarray(E, np.complex128)
+        ntr = E.shape[1] // 2
+        rth, rph = np.ascontiguousarray(rth, np.float64), np.ascontiguousarray(rph, np.float64)
+        Es = np.empty((len(rth), 4 * ntr), np.complex128, order="F")
+        cs = cells.struct()
+        _check(self.lib.midas_near_field(self.ctx, C.byref(cs), C.c_double(k0), C.c_int(quirks), _p(E),
+                                         C.c_long(E.shape[0]), C.c_int(ntr), C.c_int(len(rth)), _p(rth), _p(rph),
+                                         C.c_double(rso), _p(Es), C.c_long(len(rth))), "near_field")
+        return Es
 
     # -- (f) orientation-averaged quantities ---------------------------------
     def scattering_quantities(self, S, nrx, ntr, rx_dirs, tr_dirs, k0, dm, cext, cabs, rx_type=RX_SD, rx_w=None,

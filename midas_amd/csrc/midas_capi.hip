@@ -558,6 +558,32 @@ int midas_cext_cabs(midas_ctx *ctx, const midas_cells_t *cells, double k0, int q
     return 0;
 }
 
+int midas_near_field(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, const double *E, long lde,
+                     int ntr, int nrx, const double *rth, const double *rph, double rso, double *Es, long lds)
+{
+    MIDAS_CHECK(ctx && cells && E && rth && rph && Es && ntr > 0 && nrx > 0 && rso > 0, -1,
+                "near_field: bad arguments");
+    MIDAS_CHECK(lde >= 3L * cells->n && lds >= nrx, -1, "near_field: lde / lds too small");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    const int n = cells->n;
+    DevCellsOwned dc;
+    TRY(dc.from_host(cells, k0, quirks, st));
+    DBuf dE, drt, drp, dS;
+    TRY(dE.alloc(sizeof(double2) * 3L * n * 2 * ntr));
+    MIDAS_HIP(hipMemcpy2DAsync(dE.p, sizeof(double2) * 3 * n, E, sizeof(double2) * lde, sizeof(double2) * 3 * n,
+                               2 * ntr, hipMemcpyHostToDevice, st));
+    TRY(upload(drt, rth, sizeof(double) * nrx, st));
+    TRY(upload(drp, rph, sizeof(double) * nrx, st));
+    TRY(dS.alloc(sizeof(double2) * (size_t)nrx * 4 * ntr));
+    TRY(near_field(dc.v, dE.as<double2>(), 3L * n, ntr, nrx, drt.as<double>(), drp.as<double>(), rso, k0, quirks,
+                   dS.as<double2>(), nrx, st));
+    MIDAS_HIP(hipMemcpy2DAsync(Es, sizeof(double2) * lds, dS.p, sizeof(double2) * nrx, sizeof(double2) * nrx, 4 * ntr,
+                               hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
 int midas_scattering_quantities(midas_ctx *ctx, const double *S, long lds, int nrx_tot, int nrx, int ntr,
                                 const double *rx_th, const double *rx_ph, int rx_type, const double *rx_w,
                                 const double *tr_th, const double *tr_ph, int tr_type, const double *tr_w, double k0,

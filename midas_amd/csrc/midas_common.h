@@ -123,6 +123,8 @@ void sph_source_frame(const double *x, const double *y, const double *z, const i
                       double c[3], double *rs);
 int launch_rhs_sph(const DevCells &cells, CellSet R, int ntr, const double *th, const double *ph, double k0,
                    int quirks, bool ref32, const double c[3], double rs, double2 *V, long ld, hipStream_t st);
+int near_field(const DevCells &cells, const double2 *E, long ld, int T, int nrx, const double *rth, const double *rph,
+               double rso, double k0, int quirks, double2 *S, long lds, hipStream_t st);
 int scatq_sums(const double2 *S, long lds, int nrx, int ntr, const double *rth, const double *rph, const double *w,
                const double *tth, const double *tph, int quirks, double *out, hipStream_t st);
 int far_field(const DevCells &cells, const double2 *E, long ld, int T, int nrx, const double *rth,

@@ -9,7 +9,10 @@
 // FF(nrx x N) * E3(N x 6T) (E3 holds the x, y, z components of all 2T
 // solutions side by side), followed by an elementwise projection kernel.
 #include "midas_common.h"
+#include "midas_host.h"
 #include "midas_lu.h"
+
+#include <algorithm>
 
 namespace midas {
 
@@ -151,6 +154,105 @@ __global__ __launch_bounds__(256) void k_cext_cabs(DevCells cells, const double2
 }
 
 // S (nrx x 4*T*npol, lds) from E (3N x 2T).  Workspace: FF (nrx x N), E3 (N x 6T), G (nrx x 6T).
+// Near-field dyad (Green_s_dt, Green_functions.f90:1335-1397), FP64: row
+// 3r+a, column 3n+q of G (3 nrx x 3N, ld) is +F_aq(r, n) kappa_n chi_n for a
+// receiver at rso (cos th, sin th cos ph, sin th sin ph), with
+// Gr = e^{ik r} / (4 Pi r^2), Tau = ik - 1/r,
+//   F_aa = Gr (Tau + k^2 (r - r_a^2 / r) - 3 r_a^2 / r^2 Tau)
+//   F_ab = Gr / r (r_a r_b) (-k^2 - 3 Tau / r).
+__global__ void k_green_dt(DevCells cells, int nrx, const double *__restrict__ rth, const double *__restrict__ rph,
+                           double rso, double k0, double pi, double2 *__restrict__ G, long ld)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (n >= cells.n) return;
+    const double t = rth[r] * pi / 180.0, p = rph[r] * pi / 180.0;
+    const double ra[3] = {rso * cos(t) - cells.x[n], rso * sin(t) * cos(p) - cells.y[n],
+                          rso * sin(t) * sin(p) - cells.z[n]};
+    const double rr = sqrt(ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
+    double s, c;
+    sincos(k0 * rr, &s, &c);
+    const double den = 4.0 * pi * (rr * rr);
+    const z64 gr = {c / den, s / den};
+    const z64 tau = {-1.0 / rr, k0};
+    const z64 kchi = {cells.kchi[2 * n], cells.kchi[2 * n + 1]};
+    const z64 grr = {gr.r / rr, gr.i / rr};
+    const z64 offc = {-k0 * k0 - 3.0 * tau.r / rr, -3.0 * tau.i / rr};
+    z64 F[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double q = 3.0 * (ra[a] * ra[a]) / (rr * rr);
+        const z64 b = {tau.r + k0 * k0 * (rr - ra[a] * ra[a] / rr) - q * tau.r, tau.i - q * tau.i};
+        F[a][a] = cmul(gr, b);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int q = a + 1; q < 3; ++q) F[a][q] = F[q][a] = cmul(cscale(grr, ra[a] * ra[q]), offc);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        double2 *col = G + (long)(3 * n + q) * ld + 3L * r;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const z64 v = cmul(F[a][q], kchi);
+            col[a] = make_double2(v.r, v.i);
+        }
+    }
+}
+
+// Es (nrx x 4T): the field Es3 (3 nrx x 2T, rows 3r+a) projected on the
+// receiver's theta / phi directions (Compute_Scattered_Fields.f90:100-128).
+__global__ void k_project_nf(const double2 *__restrict__ Es3, long ld3, int nrx, int T, int r0,
+                             const double *__restrict__ rth, const double *__restrict__ rph, double pi,
+                             double2 *__restrict__ S, long lds)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (r >= nrx) return;
+    const double th = rth[r0 + r] * pi / 180.0, ph = rph[r0 + r] * pi / 180.0;
+    const double st = sin(th), ct = cos(th), sp = sin(ph), cp = cos(ph);
+    const double2 *ev = Es3 + (long)t * ld3 + 3L * r, *eh = Es3 + (long)(t + T) * ld3 + 3L * r;
+    auto theta_part = [&](const double2 *e) {
+        return z64{-e[0].x * st + e[1].x * ct * cp + e[2].x * ct * sp, -e[0].y * st + e[1].y * ct * cp + e[2].y * ct * sp};
+    };
+    auto phi_part = [&](const double2 *e) { return z64{-e[1].x * sp + e[2].x * cp, -e[1].y * sp + e[2].y * cp}; };
+    const z64 vv = theta_part(ev), vh = phi_part(ev), hv = theta_part(eh), hh = phi_part(eh);
+    double2 *o = S + (long)(r0 + r);
+    o[(long)(4 * t) * lds] = make_double2(vv.r, vv.i);
+    o[(long)(4 * t + 1) * lds] = make_double2(vh.r, vh.i);
+    o[(long)(4 * t + 2) * lds] = make_double2(hv.r, hv.i);
+    o[(long)(4 * t + 3) * lds] = make_double2(hh.r, hh.i);
+}
+
+int near_field(const DevCells &cells, const double2 *E, long ld, int T, int nrx, const double *rth, const double *rph,
+               double rso, double k0, int quirks, double2 *S, long lds, hipStream_t st)
+{
+    if (T <= 0 || nrx <= 0) return 0;
+    MIDAS_CHECK(T <= 65535, -1, "near_field: too many transmitters for one launch");
+    const double pi = host_pi(quirks);
+    const int N = cells.n;
+    // receivers in chunks so that the dyads take at most ~4 GB
+    const long per_rx = 9L * N * (long)sizeof(double2);
+    const int rc = (int)std::max(1L, std::min((long)nrx, (long)(4e9 / (double)per_rx)));
+    DBuf G, Es3;
+    TRY(G.alloc(per_rx * rc));
+    TRY(Es3.alloc(sizeof(double2) * 3L * rc * 2 * T));
+    for (int r0 = 0; r0 < nrx; r0 += rc) {
+        const int nr = std::min(rc, nrx - r0);
+        hipLaunchKernelGGL(k_green_dt, dim3((N + 255) / 256, nr), dim3(256), 0, st, cells, nr, rth + r0, rph + r0, rso,
+                           k0, pi, G.as<double2>(), 3L * nr);
+        MIDAS_HIP(hipGetLastError());
+        int r = launch_cgemm<double>(3 * nr, 2 * T, 3 * N, z64{1.0, 0.0}, G.as<double2>(), 3L * nr, 0, E, ld, 0,
+                                     z64{0.0, 0.0}, Es3.as<double2>(), 3L * nr, st);
+        if (r) return r;
+        hipLaunchKernelGGL(k_project_nf, dim3((nr + 255) / 256, T), dim3(256), 0, st, Es3.as<double2>(), 3L * nr, nr, T,
+                           r0, rth, rph, pi, S, lds);
+        MIDAS_HIP(hipGetLastError());
+    }
+    MIDAS_HIP(hipStreamSynchronize(st));   // G and Es3 are released on return
+    return 0;
+}
+
 // Orientation-averaged quantities (Compute_Scattering_Quantities.f90:593-1013):
 // per transmitter t (one workgroup), over the NRx scattering receivers,
 //   f(r)  = w_r (|S_hh|^2 + |S_vv|^2 + |S_vh|^2 + |S_hv|^2)    (w_r = 1 unless weighted)

@@ -91,6 +91,17 @@ module midas_gpu_c
             integer(c_long), value :: ldv
         end function midas_spherical_wave
 
+        integer(c_int) function midas_near_field(ctx, cells, k0, quirks, e, lde, ntr, nrx, rth, rph, rso, es, lds) &
+            bind(c, name="midas_near_field")
+            import :: c_int, c_long, c_ptr, c_double, midas_cells_t
+            type(c_ptr), value :: ctx, e, es
+            type(midas_cells_t), intent(in) :: cells
+            real(c_double), value :: k0, rso
+            integer(c_int), value :: quirks, ntr, nrx
+            integer(c_long), value :: lde, lds
+            real(c_double), intent(in) :: rth(*), rph(*)
+        end function midas_near_field
+
         integer(c_int) function midas_scattering_quantities(ctx, s, lds, nrx_tot, nrx, ntr, rx_th, rx_ph, rx_type, &
                                                             rx_w, tr_th, tr_ph, tr_type, tr_w, k0, dm, quirks, &
                                                             c_ext, c_abs, cext_intfields, q, q_av) &

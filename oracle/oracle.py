@@ -194,6 +194,49 @@ def ff_coeff(cells: Cells, th, ph):
     return ff
 
 
+def green_s_dt(cells: Cells, rso, th, ph):
+    """Green_s_dt (Green_functions.f90:1335-1397), FP64: the 3 x 3n dyad
+    +F kappa chi from every cell to a receiver at rso (cos th, sin th cos ph,
+    sin th sin ph), Gr = e^{i k r} / (4 Pi r^2), Tau = i k - 1/r."""
+    PI = pi(cells.quirk)
+    k = cells.k0
+    t, p = th * PI / 180.0, ph * PI / 180.0
+    xc, yc, zc = rso * np.cos(t), rso * np.sin(t) * np.cos(p), rso * np.sin(t) * np.sin(p)
+    rx, ry, rz = xc - cells.x, yc - cells.y, zc - cells.z
+    r = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
+    gr = np.exp(1j * k * r) / (4 * PI * r ** 2)
+    tau = 1j * k - 1 / r
+    f = cells.kappa * cells.che
+    diag = lambda ra: gr * (tau + k ** 2 * (r - ra ** 2 / r) - 3.0 * ra ** 2 / r ** 2 * tau)
+    off = lambda ra, rb: gr / r * (ra * rb) * (-k ** 2 - 3.0 * tau / r)
+    F = [[diag(rx), off(rx, ry), off(rx, rz)], [off(rx, ry), diag(ry), off(ry, rz)],
+         [off(rx, rz), off(ry, rz), diag(rz)]]
+    G = np.zeros((3, 3 * cells.n), np.complex128)
+    for a in range(3):
+        for q in range(3):
+            G[a, q::3] = F[a][q] * f
+    return G
+
+
+def near_field(cells: Cells, E, rso, rth, rph):
+    """Compute_Scattered_Fields (Compute_Scattered_Fields.f90:75-136): scattered
+    field at the receivers from E (3n x 2T), projected on (theta, phi):
+    Es (nrx x 4T: Vv, Vh, Hv, Hh per transmitter)."""
+    PI = pi(cells.quirk)
+    T = E.shape[1] // 2
+    S = np.zeros((len(rth), 4 * T), np.complex128)
+    for r in range(len(rth)):
+        G = green_s_dt(cells, rso, rth[r], rph[r])
+        Ev, Eh = G @ E[:, :T], G @ E[:, T:]
+        t, p = rth[r] * PI / 180.0, rph[r] * PI / 180.0
+        st, ct, sp, cp = np.sin(t), np.cos(t), np.sin(p), np.cos(p)
+        S[r, 0::4] = -Ev[0] * st + Ev[1] * ct * cp + Ev[2] * ct * sp
+        S[r, 1::4] = -Ev[1] * sp + Ev[2] * cp
+        S[r, 2::4] = -Eh[0] * st + Eh[1] * ct * cp + Eh[2] * ct * sp
+        S[r, 3::4] = -Eh[1] * sp + Eh[2] * cp
+    return S
+
+
 def scattering_matrix(cells: Cells, E, rth, rph, beta_deg=(0.0,)):
     E = np.asfortranarray(E, dtype=np.complex128)
     ntr = E.shape[1] // 2
@@ -543,6 +586,15 @@ def ref_incident_field_sph(c: Cells, th, ph, t0=1, t1=None):
     r.ref_incident_field_sph(C.c_int(c.n), _p(c.x), _p(c.y), _p(c.z), C.c_int(len(th)), _p(th), _p(ph),
                              C.c_int(t0), C.c_int(t1), _p(E))
     return E
+
+
+def ref_green_s_dt(c: Cells, rso, th, ph):
+    r = ref()
+    _ref_globals(r, c.k0, nbc=c.n)
+    G = np.empty((3, 3 * c.n), np.complex128, order="F")
+    r.ref_green_s_dt(C.c_int(c.n), _p(c.x), _p(c.y), _p(c.z), _p(c.kappa), _p(c.che), C.c_double(rso),
+                     C.c_double(th), C.c_double(ph), _p(G))
+    return G
 
 
 def ref_ff_coeff(c: Cells, th, ph):

@@ -203,6 +203,24 @@ contains
         call Incident_Field_Spherical(1, n, cc, ntr, tr, t0, t1, E)
     end subroutine ref_incident_field_sph
 
+    ! Green_s_dt (Green_functions.f90:1335-1397): the 3 x 3n dyad from the
+    ! cells to a receiver at distance rso in direction (th, ph)
+    subroutine ref_green_s_dt(n, x, y, z, kappa, che, rso_in, th, ph, G) bind(c, name="ref_green_s_dt")
+        integer(c_int), value :: n
+        real(c_double), value :: rso_in, th, ph
+        real(c_double), intent(in) :: x(n), y(n), z(n), kappa(n)
+        complex(c_double_complex), intent(in) :: che(n)
+        complex(c_double_complex), intent(out) :: G(3, 3*n)
+        type(Cell), allocatable :: cc(:)
+        integer(c_int), allocatable :: id(:)
+        integer :: c
+        allocate(cc(n), id(n))
+        id = [(c, c = 1, n)]
+        call build_cells(n, id, x, y, z, kappa, kappa, che, che, cc)
+        Rso = rso_in
+        call Green_s_dt(n, cc, th, ph, G)
+    end subroutine ref_green_s_dt
+
     ! GetFFieldCoeff (Green_functions.f90:1399-1438)
     subroutine ref_ff_coeff(n, x, y, z, kappa, che, th, ph, ff) bind(c, name="ref_ff_coeff")
         integer(c_int), value :: n

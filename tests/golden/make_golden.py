@@ -18,6 +18,7 @@ Routines captured (reference file:line):
   setfSR                    getParameters_CBFM.f90:113-168
   getTransmitters_CBFM      getParameters_CBFM.f90:1-110 (uniform grid)
   Incident_Field_Spherical  Incident_Field.f90:65-166
+  Green_s_dt                Green_functions.f90:1335-1397
 
 The fixture holds data only (inputs and expected outputs).
 """
@@ -140,6 +141,10 @@ def main():
     cw = o.Cells(pw[:, 0], pw[:, 1], pw[:, 2], np.full(120, 1.5e-4), np.full(120, 1.5 + 0.1j), kw)
     d.update(sph2_x=pw[:, 0].copy(), sph2_y=pw[:, 1].copy(), sph2_z=pw[:, 2].copy(), sph2_k0=np.float64(kw))
     d["sph2_E"] = o.ref_incident_field_sph(cw, d["un_94_th"], d["un_94_ph"])
+
+    # --- near-field dyad Green_s_dt (receivers at Rso) ------------------------
+    d["gdt_args"] = np.array([[0.05, 0.0, 0.0], [0.05, 37.5, 12.0], [3e-4, 151.0, 333.3]])
+    d["gdt_G"] = np.stack([o.ref_green_s_dt(rc, *map(float, a_)) for a_ in d["gdt_args"]])
 
     np.savez_compressed(OUT, **d)
     print("wrote", OUT, os.path.getsize(OUT), "bytes;", {k_: v.shape for k_, v in d.items() if np.ndim(v) > 1})

@@ -270,3 +270,20 @@ def test_scattering_quantities_vs_oracle(mid, oracle, rx_type, layout):
         assert np.allclose(Q, Qo, rtol=1e-12, atol=0)
         assert np.allclose(av, avo, rtol=1e-12, atol=0)
 
+
+@pytest.mark.parametrize("rso_wavelengths", [0.6, 50.0])
+def test_near_field_vs_oracle(mid, small, oracle, rso_wavelengths):
+    """Scattered near field (Green_s_dt + projection, FP64) on the GPU against
+    the oracle, receivers close to the particle and far from it."""
+    s, oc = small
+    rng = np.random.default_rng(8)
+    E = rng.normal(size=(3 * s.n, 4)) + 1j * rng.normal(size=(3 * s.n, 4))
+    rth = np.linspace(0.0, 180.0, 23)
+    rph = np.linspace(0.0, 330.0, 23)
+    rso = rso_wavelengths * 2 * np.pi / s.k0
+    _case(f"near field rso={rso_wavelengths} wavelengths nrx={len(rth)}")
+    Es = mid.near_field(s, s.k0, E, rth, rph, rso)
+    Eo = oracle.near_field(oc, E, rso, rth, rph)
+    print("near field rel err", _rel(Es, Eo))
+    assert _rel(Es, Eo) < 1e-12
+

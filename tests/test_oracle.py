@@ -131,6 +131,15 @@ def test_incident_field_sph_geometry(oracle, gold):
             assert np.all(np.abs((V * H.conj()).sum(1)) < 10 * tol / R)
 
 
+def test_green_s_dt_golden(oracle, gold):
+    """Green_s_dt (near-field dyad, FP64) against the reference-compiled routine
+    at three receivers, one of them close to the cells (rso = 0.3 mm)."""
+    c = _cells(oracle, gold, "g1")
+    for (rso, th, ph), G in zip(gold["gdt_args"], gold["gdt_G"]):
+        Go = oracle.green_s_dt(c, rso, th, ph)
+        assert np.abs(Go - G).max() <= 1e-14 * np.abs(G).max()
+
+
 def test_ff_coeff_golden(oracle, gold):
     c = _cells(oracle, gold, "g1")
     for i, (t, p) in enumerate(zip(gold["g1_th"], gold["g1_ph"])):

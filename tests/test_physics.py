@@ -142,3 +142,21 @@ def test_scattering_quantities_vs_mie(oracle, sphere_solution):
                                          rx_type=oracle.RX_LB, rx_w=np.full(nrx, 1.0 / nrx))
     assert np.allclose(Ql, Q, rtol=1e-12)
 
+
+def test_near_field_far_limit(oracle, sphere_solution):
+    """The near field (Green_s_dt, FFA = 0) at distance R times k R e^{-ikR}
+    tends to the far-field S (GetFFieldCoeff) as 1/R: both paths share no code,
+    so a wrong factor, sign or projection in either shows here."""
+    s, c, E, th, ph = sphere_solution
+    rth, rph = np.array([10.0, 60.0, 120.0, 170.0]), np.array([0.0, 45.0, 200.0, 300.0])
+    S = oracle.scattering_matrix(c, E, rth, rph)
+    lam = 2 * np.pi / s.k0
+    errs = []
+    for R in (1e2 * lam, 1e3 * lam):
+        Es = oracle.near_field(c, E, R, rth, rph)
+        F = Es * s.k0 * R * np.exp(-1j * s.k0 * R)
+        errs.append(np.linalg.norm(F - S) / np.linalg.norm(S))
+    print("near -> far relative error at 100 and 1000 wavelengths", errs)
+    assert errs[1] < 1e-3
+    assert errs[0] / errs[1] == pytest.approx(10.0, rel=0.05)
+
