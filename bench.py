@@ -33,11 +33,11 @@ HBM_GBS = 8000.0
 MODES = {"fp64": 0, "c32": 1, "f16": 2, "f16x3": 3, "bf16": 4, "bf16x3": 5}
 MODE_DTYPE = {"fp64": "f64", "c32": "f32", "f16": "f16", "f16x3": "f16", "bf16": "bf16", "bf16x3": "bf16"}
 MODE_TERMS = {"fp64": 1, "c32": 1, "f16": 1, "f16x3": 3, "bf16": 1, "bf16x3": 3}
-# HBM bytes per k_trail16 launch measured with rocprofv3 --pmc FETCH_SIZE and
+# HBM bytes per k_trail16x launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors
 # a 1 GiB device copy reads on the same counters), keyed by (cells, mode, nb);
 # other configurations report null.  Source: profiles/lu_trail16_f16_grs20k_hbm_traffic.md
-TRAFFIC_PER_LAUNCH = {(20000, "f16", 0): 13.40e9}
+TRAFFIC_PER_LAUNCH = {(20000, "f16", 0): 14.95e9}
 
 
 def log(*a):
@@ -281,7 +281,7 @@ def main():
     hbm_bound = intensity < ridge
     traffic = TRAFFIC_PER_LAUNCH.get((args.cells, args.mode, args.nb))
     result["roofline"] = {
-        "kernel": "k_trail16 (LU trailing update, split-precision MFMA)" if dt in ("f16", "bf16")
+        "kernel": "k_trail16x (LU trailing update, split-precision MFMA)" if dt in ("f16", "bf16")
         else "k_cgemm (LU trailing update)",
         "bound": "hbm" if hbm_bound else "mfma",
         "achieved": round(trail_gbs if hbm_bound else trail_tf, 2),

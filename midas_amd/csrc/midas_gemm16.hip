@@ -566,7 +566,9 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
                                mt, ks, Ap);
             hipLaunchKernelGGL((k_split_b<BF, 1, true>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, ks, Bq,
                                sq);
-            ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 16.0 * m * n + 4.0 * ((double)m + 1.5 * n) * k);
+            // algorithmic bytes: C read and written, both operands once in FP16
+            // (the -Bi plane is an extra read of this kernel, seen in the counters)
+            ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 16.0 * m * n + 4.0 * ((double)m + n) * k);
             const long a_tile = (long)T_M * ks * T_K * 2 * 2;
             const int group = (int)std::max(2L, std::min(16L, (2L << 20) / a_tile));
             hipLaunchKernelGGL((k_trail16x<BF>), dim3(mt * nt), dim3(256), 0, st, m, n, ks, mt, nt, group, Ap, Bq, sq,
