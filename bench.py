@@ -165,6 +165,103 @@ def run_cbfm(args):
     print(json.dumps(res))
 
 
+def c4_cpu_baseline(s, nrx, nrhs, sample_cells=1800):
+    """CPU restatement on a sample of the same particle: OpenMP C fill, scipy
+    (OpenBLAS) LU and a solve with all right-hand sides, the far field for all
+    receivers; scaled to the full size by N^2 (fill), n^3 (LU), n^2 (solve) and
+    N (far field)."""
+    import numpy as np
+    import scipy.linalg as sla
+    from midas_amd import geometry as g
+    from oracle import oracle as o
+    ss = g.convex_hull_particle(size_param=5.0 * (sample_cells / s.n) ** (1 / 3), target_cells=sample_cells, seed=4)
+    c = o.Cells(ss.x, ss.y, ss.z, ss.sc, ss.eps, ss.k0)
+    tth, tph = g.directions(nrhs // 2)
+    t0 = time.perf_counter()
+    Z = o.mom_fill(c)
+    t_fill = time.perf_counter() - t0
+    B = o.mom_rhs(c, tth, tph)
+    t0 = time.perf_counter()
+    lu = sla.lu_factor(Z, check_finite=False)
+    t_lu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    E = sla.lu_solve(lu, B, check_finite=False)
+    t_solve = time.perf_counter() - t0
+    rth, rph = g.directions(nrx)
+    t0 = time.perf_counter()
+    o.scattering_matrix(c, E, rth, rph)
+    t_ff = time.perf_counter() - t0
+    rN, rn = s.n / ss.n, 3 * s.n / (3 * ss.n)
+    total = t_fill * rN ** 2 + t_lu * rn ** 3 + t_solve * rn ** 2 + t_ff * rN
+    import os
+    return {"value": total, "unit": "s", "cores": len(os.sched_getaffinity(0)), "kind": "port",
+            "sample": f"convex hull at {ss.n} cells (n={3 * ss.n}), {nrhs} RHS, {nrx} receivers: fill {t_fill:.2f} s, "
+                      f"LU {t_lu:.2f} s, solve {t_solve:.2f} s, far field {t_ff:.2f} s; scaled by N^2, n^3, n^2, N"}
+
+
+def run_c4(args):
+    """C4: convex-hull particle (hull of 40 random points on an ellipsoid
+    1:0.8:0.6, x = 5, ~12,000 cells, eps 1.33+0.001i); 500 incidence
+    directions x 2 polarisations = 1,000 right-hand sides in one mixed-precision
+    LU + FP64 refinement; the far field at the 500 directions (forward = the
+    incidence direction, the reference's sd layout), their backscatter
+    directions and a 0.5-degree theta grid at phi = 0 and 90; Cext/Cabs; the
+    orientation-averaged Q (equal weights).  One step = all of it, through the
+    host C-ABI (the S matrix and Cext/Cabs copied back)."""
+    import numpy as np
+    from midas_amd import api, geometry as g
+    s = g.convex_hull_particle(size_param=5.0, target_cells=12000, seed=4)
+    tth, tph = g.directions(500)
+    gth = np.arange(0.0, 180.25, 0.5)
+    rth = np.r_[tth, 180.0 - tth, np.tile(gth, 2)]
+    rph = np.r_[tph, np.mod(tph + 180.0, 360.0), np.repeat([0.0, 90.0], len(gth))]
+    nrx, ntr = len(tth), len(tth)
+    dm = 2.0 * (3.0 * s.n * s.sc[0] ** 3 / (4.0 * np.pi)) ** (1.0 / 3.0)   # volume-equivalent diameter
+    mode = MODES[args.mode]
+    m = api.Midas(0)
+    log(f"[bench] {s.name}: n = {3 * s.n}, {2 * ntr} RHS, {len(rth)} receivers")
+
+    def step(md, want_E=False):
+        S, ce, ca, E, info = m.compute_efields_mom(s, s.k0, (tth, tph), (rth, rph), mode=md, want_E=want_E)
+        Q, av = m.scattering_quantities(S, nrx, ntr, (tth, tph), (tth, tph), s.k0, dm, ce, ca)
+        return E, info, Q, av
+
+    for _ in range(args.warmup):
+        step(mode)
+    ts = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        _, info, Q, av = step(mode)
+        ts.append(time.perf_counter() - t0)
+        log(f"[bench] c4 step {k + 1}: {ts[-1]:.3f} s ({info})")
+    step_s = float(np.mean(ts))
+    res = {"metric": "multi-RHS fill+solve+far field+orientation averages wall-s (C4)", "value": round(step_s, 4),
+           "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(step_s * 1e3, 2), "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": f"f64 fill + {MODE_DTYPE[args.mode]} + f64 IR",
+           "data": "synthetic (seeded convex-hull particle)",
+           "config": {"workload": f"C4 convex hull x=5 axes 1:0.8:0.6, {s.n} cells, {3 * s.n} unknowns, "
+                                  f"{2 * ntr} RHS (500 directions x 2), {len(rth)} receivers (0.5-deg grid)",
+                      "solver": args.mode, "parallelism": "1 GPU"},
+           "factor_ms": round(info["t_factor_ms"], 2), "refine_ms": round(info["t_refine_ms"], 2),
+           "ir_iterations": info["iterations"], "ir_berr": info["berr"],
+           "Q_avg": {"Q_ext": av[0], "Q_abs": av[1], "Q_sca": av[2], "Q_bks": av[3], "g": av[4]}}
+    if not args.no_fp64:
+        log("[bench] FP64 path and field error")
+        t0 = time.perf_counter()
+        E64, _, Q64, _ = step(api.SOLVE_FP64, want_E=True)
+        res["fp64_path_s"] = round(time.perf_counter() - t0, 3)
+        Em, _, _, _ = step(mode, want_E=True)
+        res["speedup_vs_fp64"] = round(res["fp64_path_s"] / step_s, 3)
+        res["rel_field_err_vs_fp64"] = float(np.linalg.norm(Em - E64) / np.linalg.norm(E64))
+    if not args.no_cpu:
+        log("[bench] CPU baseline sample (C4)")
+        res["cpu_baseline"] = c4_cpu_baseline(s, len(rth), 2 * ntr)
+        res["speedup_vs_cpu"] = round(res["cpu_baseline"]["value"] / step_s, 1)
+    m.close()
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,8 +273,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample-cells", type=int, default=3500)
     ap.add_argument("--nb", type=int, default=0, help="LU panel width (0: library default)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
-                    help="c2: MoM fill + solve (default); c3: CBFM-E + ACA on a DLCA aggregate")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+                    help="c2: MoM fill + solve (default); c3: CBFM-E + ACA on a DLCA aggregate; "
+                         "c4: 1,000-RHS solve + far field + orientation averages on a convex-hull particle")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device to device (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -187,6 +285,8 @@ def main():
         sys.exit(2)
     if args.workload == "c3":
         return run_cbfm(args)
+    if args.workload == "c4":
+        return run_c4(args)
 
     import numpy as np
     import torch
