@@ -193,7 +193,7 @@ static void add_gemm_tiles(std::vector<GJob> &jobs, const double2 *A, int lda, i
 static int run_ggemm(const std::vector<GJob> &jobs, DBuf &djobs, hipStream_t st)
 {
     if (jobs.empty()) return 0;
-    TRY(djobs.alloc(sizeof(GJob) * jobs.size()));
+    TRY(djobs.reserve(sizeof(GJob) * jobs.size()));
     MIDAS_HIP(hipMemcpyAsync(djobs.p, jobs.data(), sizeof(GJob) * jobs.size(), hipMemcpyHostToDevice, st));
     for (size_t q0 = 0; q0 < jobs.size(); q0 += 1u << 30) {
         const unsigned cnt = (unsigned)std::min<size_t>(jobs.size() - q0, 1u << 30);
@@ -434,34 +434,59 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
         }
     }
     {
-        const long zbytes = (long)sizeof(double2) * mp * mp;
-        const int chunk = (int)std::max(1L, std::min((long)nb, (long)(24e9 / (double)zbytes)));
+        // Blocks sorted by extended size (largest first) and solved in batches,
+        // each padded only to its own largest block (uneven blocks would
+        // otherwise all pay for the largest), within a 24 GB budget for the
+        // batch's matrices.  Solutions are staged per batch (the batched solve
+        // needs one stride) and copied to X_b.
+        struct Chunk { int pos, nc; long mpc; };
+        std::vector<int> order(nb);
+        for (int b = 0; b < nb; ++b) order[b] = b;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return sb[a] > sb[c]; });
+        std::vector<Chunk> chunks;
+        long zmax = 0, smax_c = 0, tmax = 0, xmax = 0, pmax_c = 0;
         const int nb_lu = 128;
-        DBuf Zall, ipiv, perm, tmp;
-        TRY(Zall.alloc(zbytes * chunk));
-        TRY(ipiv.alloc(sizeof(int) * mp * chunk));
-        TRY(perm.alloc(sizeof(int) * mp * chunk));
-        TRY(tmp.alloc(sizeof(double2) * std::max<long>(mp, nb_lu) * T2c * chunk));
-        std::vector<int> hinfo(chunk);
-        for (int b0 = 0; b0 < nb; b0 += chunk) {
-            const int nc = std::min(chunk, nb - b0);
-            MIDAS_HIP(hipMemsetAsync(Zall.p, 0, zbytes * nc, st));
-            MIDAS_HIP(hipMemsetAsync(Xb(b0), 0, sizeof(double2) * mp * T2c * nc, st));
+        for (int pos = 0; pos < nb;) {
+            const long mpc = 3L * sb[order[pos]];
+            const long zb = (long)sizeof(double2) * mpc * mpc;
+            const int nc = (int)std::max(1L, std::min((long)(nb - pos), (long)(24e9 / (double)zb)));
+            chunks.push_back({pos, nc, mpc});
+            zmax = std::max(zmax, zb * nc);
+            pmax_c = std::max(pmax_c, mpc * nc);
+            tmax = std::max(tmax, std::max(mpc, (long)nb_lu) * T2c * nc);
+            xmax = std::max(xmax, mpc * T2c * nc);
+            smax_c = std::max(smax_c, (long)nc);
+            pos += nc;
+        }
+        DBuf Zall, ipiv, perm, tmp, Xc;
+        TRY(Zall.alloc(zmax));
+        TRY(ipiv.alloc(sizeof(int) * pmax_c));
+        TRY(perm.alloc(sizeof(int) * pmax_c));
+        TRY(tmp.alloc(sizeof(double2) * tmax));
+        TRY(Xc.alloc(sizeof(double2) * xmax));
+        MIDAS_HIP(hipMemsetAsync(Xall.p, 0, sizeof(double2) * mp * T2c * nb, st));
+        std::vector<int> hinfo(smax_c);
+        for (const Chunk &ch : chunks) {
+            const int nc = ch.nc;
+            const long mpc = ch.mpc;
+            auto Xq = [&](int q) { return Xc.as<double2>() + (long)q * mpc * T2c; };
+            MIDAS_HIP(hipMemsetAsync(Zall.p, 0, sizeof(double2) * mpc * mpc * nc, st));
+            MIDAS_HIP(hipMemsetAsync(Xc.p, 0, sizeof(double2) * mpc * T2c * nc, st));
             for (int q = 0; q < nc; ++q) {
-                const int b = b0 + q, m = 3 * sb[b];
-                double2 *Zq = Zall.as<double2>() + (long)q * mp * mp;
+                const int b = order[ch.pos + q], m = 3 * sb[b];
+                double2 *Zq = Zall.as<double2>() + (long)q * mpc * mpc;
                 CellSet S{sb[b], 0, didx.as<int>() + ioff[b]};
-                TRY(launch_fill(dc.v, S, S, k0, quirks, ref32, Zq, mp, nullptr, 0, st));
+                TRY(launch_fill(dc.v, S, S, k0, quirks, ref32, Zq, mpc, nullptr, 0, st));
                 if (sr)
-                    hipLaunchKernelGGL(k_sr_mask, dim3((m + 255) / 256, m), dim3(256), 0, st, Zq, mp, m, dc.v, S,
+                    hipLaunchKernelGGL(k_sr_mask, dim3((m + 255) / 256, m), dim3(256), 0, st, Zq, mpc, m, dc.v, S,
                                        fsr[b]);
-                if (m < mp)
-                    hipLaunchKernelGGL(k_unit_diag, dim3((int)((mp - m + 255) / 256)), dim3(256), 0, st, Zq, mp, m,
-                                       (int)mp);
+                if (m < mpc)
+                    hipLaunchKernelGGL(k_unit_diag, dim3((int)((mpc - m + 255) / 256)), dim3(256), 0, st, Zq, mpc, m,
+                                       (int)mpc);
                 MIDAS_HIP(hipGetLastError());
                 if (iws == 1) {
-                    TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32, Xb(b),
-                                   mp, st));
+                    TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32, Xq(q),
+                                   mpc, st));
                 } else {
                     // spherical waves around the extended block (type_iws 2 and 3 both call
                     // Incident_Field_Spherical, Compute_EFields_CBFME.f90:363-369)
@@ -469,24 +494,28 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                     sph_source_frame(cells->x, cells->y, cells->z, allidx.data() + ioff[b], sb[b], k0, quirks, cen,
                                      &rs);
                     TRY(launch_rhs_sph(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32,
-                                       cen, rs, Xb(b), mp, st));
+                                       cen, rs, Xq(q), mpc, st));
                 }
             }
             LuWork<double> w;
-            TRY(w.alloc((int)mp, nb_lu, nc, mp * mp));
+            TRY(w.alloc((int)mpc, nb_lu, nc, mpc * mpc));
             TrailingGemm<double> tg;
             tg.batch = nc;
-            tg.stride = mp * mp;
-            int r = getrf<double>(Zall.as<double2>(), mp, (int)mp, ipiv.as<int>(), w, st, tg);
-            if (!r) r = ipiv_to_perm_dev(ipiv.as<int>(), perm.as<int>(), (int)mp, nc, st);
+            tg.stride = mpc * mpc;
+            int r = getrf<double>(Zall.as<double2>(), mpc, (int)mpc, ipiv.as<int>(), w, st, tg);
+            if (!r) r = ipiv_to_perm_dev(ipiv.as<int>(), perm.as<int>(), (int)mpc, nc, st);
             if (!r)
-                r = getrs<double>(Zall.as<double2>(), mp, (int)mp, perm.as<int>(), w, Xb(b0), mp, T2c,
-                                  tmp.as<double2>(), st, mp * T2c);
+                r = getrs<double>(Zall.as<double2>(), mpc, (int)mpc, perm.as<int>(), w, Xq(0), mpc, T2c,
+                                  tmp.as<double2>(), st, mpc * T2c);
             if (!r) r = copy_sync(hinfo.data(), w.info, sizeof(int) * nc, hipMemcpyDeviceToHost, st);
             w.release();
             if (r) return r;
-            for (int q = 0; q < nc; ++q)
+            for (int q = 0; q < nc; ++q) {
+                const int b = order[ch.pos + q];
+                MIDAS_HIP(hipMemcpy2DAsync(Xb(b), sizeof(double2) * mp, Xq(q), sizeof(double2) * mpc,
+                                           sizeof(double2) * 3 * sb[b], T2c, hipMemcpyDeviceToDevice, st));
                 if (hinfo[q] > 0 && worst_info == 0) worst_info = hinfo[q];
+            }
         }
     }
     phase(sr ? "SR masks + block fills + LU solves" : "block fills + LU solves");
@@ -627,6 +656,9 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
         // the blocks Z_ij are filled side by side, then M = C_i^T Z_ij and
         // Zc_ij = M C_j for all of them in two grouped launches
         std::vector<std::pair<int, int>> dense;
+        // reused across batches (allocating multi-GB buffers per batch costs
+        // milliseconds each and serialises the stream)
+        DBuf Zd, Md, dj, Ub, Vb, dk, LR;
         auto dense_pairs = [&]() -> int {
             const long zbudget = 4L << 30;
             size_t q0 = 0;
@@ -644,9 +676,8 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                     mused += (long)K[i] * 3 * nbk[j];
                     ++q1;
                 }
-                DBuf Zd, Md;
-                TRY(Zd.alloc(sizeof(double2) * zused));
-                TRY(Md.alloc(sizeof(double2) * std::max(mused, 1L)));
+                TRY(Zd.reserve(sizeof(double2) * zused));
+                TRY(Md.reserve(sizeof(double2) * std::max(mused, 1L)));
                 jobs.clear();
                 for (size_t q = q0; q < q1; ++q) {
                     const int i = dense[q].first, j = dense[q].second;
@@ -704,11 +735,11 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                 ++j1;
             }
             if (!ajobs.empty()) {
-                DBuf dj, Ub, Vb, dk, LR;
-                TRY(upload(dj, ajobs.data(), sizeof(AcaJob) * ajobs.size(), st));
-                TRY(Ub.alloc(sizeof(double2) * std::max(uo, 1L)));
-                TRY(Vb.alloc(sizeof(double2) * std::max(vo, 1L)));
-                TRY(dk.alloc(sizeof(int) * ajobs.size()));
+                TRY(dj.reserve(sizeof(AcaJob) * ajobs.size()));
+                MIDAS_HIP(hipMemcpyAsync(dj.p, ajobs.data(), sizeof(AcaJob) * ajobs.size(), hipMemcpyHostToDevice, st));
+                TRY(Ub.reserve(sizeof(double2) * std::max(uo, 1L)));
+                TRY(Vb.reserve(sizeof(double2) * std::max(vo, 1L)));
+                TRY(dk.reserve(sizeof(int) * ajobs.size()));
                 TRY(launch_aca_cells(dc.v, ref32, k0, host_pi(quirks), dj.as<AcaJob>(), (int)ajobs.size(),
                                      Ub.as<double2>(), Vb.as<double2>(), kmax, eps_aca, dk.as<int>(), st));
                 std::vector<int> hk(ajobs.size());
@@ -724,7 +755,7 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                     ro[q] = lr;
                     lr += (long)kk * K[pj_[q]];
                 }
-                TRY(LR.alloc(sizeof(double2) * std::max(lr, 1L)));
+                TRY(LR.reserve(sizeof(double2) * std::max(lr, 1L)));
                 jobs.clear();
                 for (size_t q = 0; q < ajobs.size(); ++q) {
                     const int i = pi_[q], j = pj_[q], kk = std::min(hk[q], kmax);
@@ -748,7 +779,8 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                 for (size_t q = 0; q < ajobs.size(); ++q)
                     if (std::min(hk[q], kmax) >= kmax) dense.push_back({pi_[q], pj_[q]});
                 TRY(dense_pairs());
-                MIDAS_HIP(hipStreamSynchronize(st));   // U, V, L, R are freed with this batch
+                // the next batch overwrites U, V, L, R and the job list
+                MIDAS_HIP(hipStreamSynchronize(st));
             }
             j0 = j1;
         }

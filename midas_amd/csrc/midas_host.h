@@ -31,6 +31,13 @@ struct DBuf {
         bytes = b;
         return 0;
     }
+    // grow-only: keeps the allocation when it is large enough (contents are
+    // not preserved when it grows); hipFree waits for work still using it
+    int reserve(size_t b)
+    {
+        if (p && b <= bytes) return 0;
+        return alloc(b + b / 4);
+    }
     void release()
     {
         if (p) hipFree(p);
