@@ -79,11 +79,12 @@ def cpu_baseline(full_cells, sample_cells, full_s_n3):
             "fill_s": fill_s, "solve_s": solve_s}
 
 
-def cbfm_cpu_baseline(s, th_c, ph_c, nsample_blocks=3, nsample_pairs=6):
+def cbfm_cpu_baseline(s, th_c, ph_c, nsample_blocks=3, nsample_pairs=6, sr=True, nrhs=2, red_max=6000):
     """The oracle's CBFM-E + ACA on the host, timed per unit on a sample and
     scaled: extended-block solve + SVD (x number of blocks), ACA of an
     off-diagonal pair with its products (x number of pairs), and the reduced
-    dense solve at its full size (numpy/OpenBLAS zgesv)."""
+    dense solve (numpy/OpenBLAS zgesv) at its full size, or at red_max
+    unknowns scaled by K^3 when K is larger."""
     import numpy as np
     from oracle import oracle as o
     oc = o.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
@@ -93,8 +94,11 @@ def cbfm_cpu_baseline(s, th_c, ph_c, nsample_blocks=3, nsample_pairs=6):
     for b in range(nsample_blocks):
         idx = np.concatenate([s.blocks[b], s.ext[b]])
         cb = oc.subset(idx)
-        f, _ = o.setfsr(cb)
-        _, _, Zb = o.sr_block(cb, f)
+        if sr:
+            f, _ = o.setfsr(cb)
+            _, _, Zb = o.sr_block(cb, f)
+        else:
+            Zb = o.block_fill(cb, cb, fp32=True)
         Eb = np.linalg.solve(Zb, o.incident_field_f32(cb, th_c, ph_c))
         sv = np.linalg.svd(Eb, compute_uv=False)
         ks.append(int(np.sum(sv / sv[0] >= o.cbf_threshold(s.eps, True))))
@@ -111,15 +115,17 @@ def cbfm_cpu_baseline(s, th_c, ph_c, nsample_blocks=3, nsample_pairs=6):
         Ci.T @ (U @ (V @ Cj))
     t_pair = (time.perf_counter() - t0) / nsample_pairs
     kt = K * nb
-    A = np.random.default_rng(1).normal(size=(kt, kt)) + 1j * np.eye(kt)
+    ks_ = min(kt, red_max)
+    A = np.random.default_rng(1).normal(size=(ks_, ks_)) + 1j * np.eye(ks_)
     t0 = time.perf_counter()
-    np.linalg.solve(A, np.ones((kt, 2), complex))
-    t_red = time.perf_counter() - t0
+    np.linalg.solve(A, np.ones((ks_, nrhs), complex))
+    t_red = (time.perf_counter() - t0) * (kt / ks_) ** 3
     total = t_block * nb + t_pair * nb * (nb - 1) + t_red
     return {"value": total, "unit": "s", "cores": os.cpu_count(), "kind": "port",
-            "sample": f"{nsample_blocks} extended blocks (SR mask + zgesv + SVD, {t_block:.2f} s each) x {nb}; "
-                      f"{nsample_pairs} ACA pairs ({t_pair:.3f} s each) x {nb * (nb - 1)}; reduced zgesv at "
-                      f"K = {kt} ({t_red:.2f} s)"}
+            "sample": f"{nsample_blocks} extended blocks ({'SR mask + ' if sr else ''}zgesv + SVD, {t_block:.2f} s "
+                      f"each) x {nb}; {nsample_pairs} ACA pairs ({t_pair:.3f} s each) x {nb * (nb - 1)}; reduced "
+                      f"zgesv at K = {ks_} with {nrhs} RHS" + (f", scaled by (K/{ks_})^3 to K = {kt}" if kt > ks_ else "")
+                      + f" ({t_red:.2f} s)"}
 
 
 def run_cbfm(args):
@@ -160,6 +166,53 @@ def run_cbfm(args):
     if not args.no_cpu:
         log("[bench] CPU baseline sample (CBFM)")
         res["cpu_baseline"] = cbfm_cpu_baseline(s, th_c, ph_c)
+        res["speedup_vs_cpu"] = round(res["cpu_baseline"]["value"] / step_s, 1)
+    m.close()
+    print(json.dumps(res))
+
+
+def run_c5(args):
+    """C5: porous particle (x = 12, 30 % voxel porosity from a thresholded
+    Gaussian random field, eps 1.7+0.02i / 3.0+0.1i / 2.5+1.0i at 0.6/0.3/0.1),
+    ~150,000 cells in 400 k-means blocks with one extension layer; CBFM-E with
+    ACA (eps 1e-4, kmax 50) between blocks, CBFs from the reference's uniform
+    grid (94 requested: 100 directions), 200 incidence directions (400 RHS),
+    reduced system in mixed precision + FP64 refinement."""
+    import numpy as np
+    from midas_amd import api, geometry as g
+    s = g.porous_particle()
+    th_c, ph_c = g.cbf_directions_uniform(94)
+    tth, tph = g.directions(200)
+    mode = MODES[args.mode]
+    m = api.Midas(0)
+    log(f"[bench] {s.name}: n = {3 * s.n}, {len(th_c)} CBF directions, {2 * len(tth)} RHS")
+    run = lambda md: m.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True,
+                                             mode=md)
+    for _ in range(args.warmup):
+        run(mode)
+    ts = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        E, K, info = run(mode)
+        ts.append(time.perf_counter() - t0)
+        log(f"[bench] c5 step {k + 1}: {ts[-1]:.3f} s, K = {int(np.sum(K))}, {info}")
+    step_s = float(np.mean(ts))
+    res = {"metric": "CBFM-E+ACA wall-s (C5)", "value": round(step_s, 4), "unit": "s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": f"f32-quirk blocks, f64 LU/SVD/ACA, {MODE_DTYPE[args.mode]} reduced LU + f64 IR",
+           "data": "synthetic (seeded porous particle)",
+           "config": {"workload": f"C5 porous x=12, {s.n} cells, {3 * s.n} unknowns, {len(s.blocks)} blocks, ACA 1e-4, "
+                                  f"{len(th_c)} CBF directions, {len(tth)} incidence directions", "solver": args.mode,
+                      "parallelism": "1 GPU"},
+           "K_total": int(np.sum(K)), "reduced_iterations": info["iterations"],
+           "reduced_factor_ms": round(info["t_factor_ms"], 1), "reduced_refine_ms": round(info["t_refine_ms"], 1)}
+    if not args.no_fp64:
+        E64, _, _ = run(api.SOLVE_FP64)
+        res["rel_field_err_vs_fp64_reduced"] = float(np.linalg.norm(E - E64) / np.linalg.norm(E64))
+    if not args.no_cpu:
+        log("[bench] CPU baseline sample (C5)")
+        res["cpu_baseline"] = cbfm_cpu_baseline(s, th_c, ph_c, sr=False, nrhs=2 * len(tth))
         res["speedup_vs_cpu"] = round(res["cpu_baseline"]["value"] / step_s, 1)
     m.close()
     print(json.dumps(res))
@@ -268,14 +321,16 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=20000)
-    ap.add_argument("--mode", default="f16", choices=[m for m in MODES if m != "fp64"])
+    ap.add_argument("--mode", default=None, choices=[m for m in MODES if m != "fp64"],
+                    help="solver of the dense (reduced) system; default f16, f16x3 for c5")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample-cells", type=int, default=3500)
     ap.add_argument("--nb", type=int, default=0, help="LU panel width (0: library default)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: MoM fill + solve (default); c3: CBFM-E + ACA on a DLCA aggregate; "
-                         "c4: 1,000-RHS solve + far field + orientation averages on a convex-hull particle")
+                         "c4: 1,000-RHS solve + far field + orientation averages on a convex-hull particle; "
+                         "c5: CBFM-E + ACA on a 150,000-cell porous particle")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device to device (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -283,10 +338,14 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.mode is None:
+        args.mode = "f16x3" if args.workload == "c5" else "f16"
     if args.workload == "c3":
         return run_cbfm(args)
     if args.workload == "c4":
         return run_c4(args)
+    if args.workload == "c5":
+        return run_c5(args)
 
     import numpy as np
     import torch
