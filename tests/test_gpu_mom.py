@@ -215,6 +215,30 @@ def test_compute_efields_mom_sphere(mid, oracle, mode):
     assert _rel(ce, ceo) < 1e-9 and _rel(ca, cao) < 1e-9
 
 
+def test_compute_efields_mom_c1_full_size(mid, oracle):
+    """The reference's CPU-runnable case at its full size: sphere x = 2 at 16
+    cells per diameter (2,176 cells, 6,528 unknowns), eps 1.6+0.02i, one plane
+    wave along +x (MIDAS axes), FP64 fill + LU; internal field, S on the
+    1-degree theta grid at phi = 0 and 90, Cext/Cabs against the oracle's FP64
+    fill + LAPACK solve."""
+    s = g.sphere(size_param=2.0, nd=16)
+    assert s.n == 2176
+    _case(f"compute_efields_mom C1 full size n={3 * s.n}")
+    oc = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    tth, tph = np.array([0.0]), np.array([0.0])
+    rth = np.concatenate([np.arange(0.0, 181.0, 1.0)] * 2)
+    rph = np.concatenate([np.zeros(181), np.full(181, 90.0)])
+    S, ce, ca, E, info = mid.compute_efields_mom(s, s.k0, (tth, tph), (rth, rph), mode=api.SOLVE_FP64)
+    Eo = np.linalg.solve(oracle.mom_fill(oc), oracle.mom_rhs(oc, tth, tph))
+    So = oracle.scattering_matrix(oc, Eo, rth, rph)
+    ceo, cao = oracle.cext_cabs(oc, Eo, tth, tph)
+    print("C1: E err", _rel(E, Eo), "S err", _rel(S, So), "Cext", ce.real, ceo.real, "Cabs", ca.real, cao.real)
+    assert info["info"] == 0
+    assert _rel(E, Eo) < 1e-11
+    assert _rel(S, So) < 1e-11
+    assert _rel(ce, ceo) < 1e-11 and _rel(ca, cao) < 1e-11
+
+
 @pytest.mark.parametrize("mode", [api.SOLVE_FP64, api.SOLVE_F16_IR])
 def test_compute_efields_mom_many_directions(mid, oracle, mode):
     """Multi-RHS (config C4 style): 60 incidence directions (120 right-hand
