@@ -164,7 +164,7 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
         TRY(copy_sync(perm_d.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_copyz, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, X, ldx, n, nrhs);
         DBuf t2;
-        TRY(t2.alloc(sizeof(double2) * (size_t)n * nrhs));
+        TRY(t2.alloc(sizeof(double2) * (size_t)getrs_tmp_elems(n, nb, nrhs)));
         r = getrs<double>(A, lda, n, perm_d.as<int>(), w, X, ldx, nrhs, t2.as<double2>(), st);
         w.release();
         if (r) return r;
@@ -198,7 +198,7 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
         DBuf R, D, t2;
         TRY(R.alloc(sizeof(double2) * (size_t)n * nrhs));
         TRY(D.alloc(sizeof(float2) * (size_t)n * nrhs));
-        TRY(t2.alloc(sizeof(float2) * (size_t)n * nrhs));
+        TRY(t2.alloc(sizeof(float2) * (size_t)getrs_tmp_elems(n, nb, nrhs)));
         dim3 g2((n + 255) / 256, nrhs);
         hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R.as<double2>(), (long)n, n, nrhs);
         std::vector<double> hx(nrhs), hr(nrhs), hb(nrhs);

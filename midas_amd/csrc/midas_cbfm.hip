@@ -453,7 +453,7 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
             chunks.push_back({pos, nc, mpc});
             zmax = std::max(zmax, zb * nc);
             pmax_c = std::max(pmax_c, mpc * nc);
-            tmax = std::max(tmax, std::max(mpc, (long)nb_lu) * T2c * nc);
+            tmax = std::max(tmax, getrs_tmp_elems((int)mpc, nb_lu, T2c) * nc);
             xmax = std::max(xmax, mpc * T2c * nc);
             smax_c = std::max(smax_c, (long)nc);
             pos += nc;

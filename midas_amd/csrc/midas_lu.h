@@ -52,11 +52,12 @@ template <> struct cvec<float> { typedef float2 t; };
 template <typename T>
 int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipStream_t st, TrailingGemm<T> tg);
 // Solve with the factors of getrf (same LuWork: it holds the diagonal-block
-// inverses); tmp needs max(n, nb) x nrhs elements per matrix.  With
-// w.batch > 1 every matrix (stride w.sA) solves its own B (stride sB).
+// inverses); tmp needs getrs_tmp_elems(n, nb, nrhs) elements per matrix.
+// With w.batch > 1 every matrix (stride w.sA) solves its own B (stride sB).
 template <typename T>
 int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, const LuWork<T> &w, typename cvec<T>::t *B,
           long ldb, int nrhs, typename cvec<T>::t *tmp, hipStream_t st, long sB = 0);
+long getrs_tmp_elems(int n, int nb, int nrhs);
 void ipiv_to_perm(const int *ipiv, int n, int *perm);
 int ipiv_to_perm_dev(const int *ipiv, int *perm, int n, int batch, hipStream_t st);
 

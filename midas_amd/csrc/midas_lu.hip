@@ -873,15 +873,150 @@ int ipiv_to_perm_dev(const int *ipiv, int *perm, int n, int batch, hipStream_t s
     return 0;
 }
 
+// Few right-hand sides: C(0:m, :) (+)= A(0:m, 0:kd) X, a row strip streamed
+// once.
+//   sub == 0:  C = A X      (a diagonal-block inverse times B_k)
+//   sub == 1:  C -= A X     (the update of the rows below/above)
+// X (kd x nrhs, nrhs <= R) is staged in LDS; workgroup 0 also writes it to Y
+// when Y is given (the solved block B_k).  64 rows per workgroup, one row per
+// lane; the NW waves take an equal share of the kd columns each, and their
+// partial sums are added in wave order (deterministic).  NW = 4 for the tall
+// updates; NW = 8 or 16 for the square diagonal blocks, where only kd / 64
+// workgroups exist and each wave must cover few columns.  The bound is HBM:
+// each element of A is read once, 8 (c64) or 16 (c128) bytes.
+constexpr int STRIP_ROWS = 64;
+template <typename T, int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_strip_mv(int m, int kd, const typename cvec<T>::t *A, long lda, long sA,
+                                                      const typename cvec<T>::t *X, long ldx, long sX,
+                                                      typename cvec<T>::t *C, long ldc, long sC, int nrhs, int sub,
+                                                      typename cvec<T>::t *Y, long ldy, long sY)
+{
+    typedef typename cvec<T>::t V;
+    extern __shared__ double strip_smem[];
+    cplx<T> *xs = (cplx<T> *)strip_smem;      // kd x R, row-major
+    cplx<T> *part = xs + (long)kd * R;        // waves 1..NW-1: 64 x R each
+    const int zb = blockIdx.z;
+    A += zb * sA;
+    X += zb * sX;
+    C += zb * sC;
+    for (int e = threadIdx.x; e < kd * R; e += 64 * NW) {
+        const int c = e / R, q = e % R;
+        V v = V{0, 0};
+        if (q < nrhs) v = X[c + (long)q * ldx];
+        xs[e] = cplx<T>{v.x, v.y};
+    }
+    __syncthreads();
+    if (Y != nullptr && blockIdx.x == 0) {
+        Y += zb * sY;
+        for (int e = threadIdx.x; e < kd * R; e += 64 * NW) {
+            const int c = e / R, q = e % R;
+            if (q < nrhs) Y[c + (long)q * ldy] = V{xs[e].r, xs[e].i};
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = blockIdx.x * STRIP_ROWS + lane;
+    const int per = (kd + NW - 1) / NW;
+    const int c0 = min(kd, wv * per), c1 = min(kd, c0 + per);
+    cplx<T> acc[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) acc[q] = cplx<T>{(T)0, (T)0};
+    if (r < m) {
+        constexpr int U = sizeof(T) == 4 ? 16 : 8;   // loads in flight per lane
+        const V *col = A + (long)c0 * lda + r;
+        for (int c = c0; c < c1; c += U) {
+            V a[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) a[u] = c + u < c1 ? col[(long)(c - c0 + u) * lda] : V{0, 0};
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (c + u >= c1) continue;
+                const cplx<T> av = {a[u].x, a[u].y};
+#pragma unroll
+                for (int q = 0; q < R; ++q) acc[q] = cadd(acc[q], cmul(av, xs[(c + u) * R + q]));
+            }
+        }
+    }
+    if (wv > 0) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) part[((wv - 1) * STRIP_ROWS + lane) * R + q] = acc[q];
+    }
+    __syncthreads();
+    if (wv == 0 && r < m) {
+        for (int w2 = 0; w2 < NW - 1; ++w2)
+#pragma unroll
+            for (int q = 0; q < R; ++q) acc[q] = cadd(acc[q], part[(w2 * STRIP_ROWS + lane) * R + q]);
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (q < nrhs) {
+                V *o = C + r + (long)q * ldc;
+                if (sub) {
+                    const V v = *o;
+                    *o = V{v.x - acc[q].r, v.y - acc[q].i};
+                } else {
+                    *o = V{acc[q].r, acc[q].i};
+                }
+            }
+    }
+}
+
+template <typename T, int R, int NW>
+static int strip_mv(int m, int kd, const typename cvec<T>::t *A, long lda, long sA, const typename cvec<T>::t *X,
+                    long ldx, long sX, typename cvec<T>::t *C, long ldc, long sC, int nrhs, int sub,
+                    typename cvec<T>::t *Y, long ldy, long sY, int batch, hipStream_t st)
+{
+    const size_t lds = sizeof(cplx<T>) * ((size_t)kd * R + (size_t)(NW - 1) * STRIP_ROWS * R);
+    const int gx = max(1, (m + STRIP_ROWS - 1) / STRIP_ROWS);
+    hipLaunchKernelGGL((k_strip_mv<T, R, NW>), dim3(gx, 1, batch), dim3(64 * NW), lds, st, m, kd, A, lda, sA, X, ldx,
+                       sX, C, ldc, sC, nrhs, sub, Y, ldy, sY);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+// Blocked substitution for at most R right-hand sides: per diagonal block,
+// Y_k = inv(D_k) B_k into tmp, then one streaming pass over the block column
+// that updates the remaining rows and writes Y_k back to B_k.  Every factor
+// element is read once per solve.
+template <typename T, int R>
+static int getrs_few(const typename cvec<T>::t *A, long lda, int n, const LuWork<T> &w, typename cvec<T>::t *B,
+                     long ldb, int nrhs, typename cvec<T>::t *tmp, long sT, hipStream_t st, long sB)
+{
+    typedef typename cvec<T>::t V;
+    const int nb = w.nb, bt = w.batch;
+    const long sA = w.sA, sI = w.inv_stride();
+    constexpr int DW = R <= 2 ? 16 : 8;   // waves per diagonal-block workgroup (no spills at 8 RHS)
+    int r;
+    for (int k = 0; k < n; k += nb) {
+        const int b = min(nb, n - k);
+        const V *Li = (const V *)w.linv + (long)(k / nb) * nb * nb;
+        if ((r = strip_mv<T, R, DW>(b, b, Li, nb, sI, B + k, ldb, sB, tmp, nb, sT, nrhs, 0, nullptr, 0, 0, bt, st)))
+            return r;
+        if ((r = strip_mv<T, R, 4>(n - k - b, b, A + (k + b) + (long)k * lda, lda, sA, tmp, nb, sT, B + k + b, ldb,
+                                   sB, nrhs, 1, B + k, ldb, sB, bt, st)))
+            return r;
+    }
+    for (int k = ((n - 1) / nb) * nb; k >= 0; k -= nb) {
+        const int b = min(nb, n - k);
+        const V *Ui = (const V *)w.uinv + (long)(k / nb) * nb * nb;
+        if ((r = strip_mv<T, R, DW>(b, b, Ui, nb, sI, B + k, ldb, sB, tmp, nb, sT, nrhs, 0, nullptr, 0, 0, bt, st)))
+            return r;
+        if ((r = strip_mv<T, R, 4>(k, b, A + (long)k * lda, lda, sA, tmp, nb, sT, B, ldb, sB, nrhs, 1, B + k, ldb, sB,
+                                   bt, st)))
+            return r;
+    }
+    return 0;
+}
+
+long getrs_tmp_elems(int n, int nb, int nrhs) { return (long)max(n, nb) * nrhs; }
+
 // Solve with the factors of getrf for every matrix of the batch: B (n x nrhs,
-// ldb, stride sB per matrix); tmp needs max(n, nb) x nrhs per matrix.
+// ldb, stride sB per matrix); tmp needs getrs_tmp_elems(n, nb, nrhs) per matrix.
 template <typename T>
 int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, const LuWork<T> &w,
           typename cvec<T>::t *B, long ldb, int nrhs, typename cvec<T>::t *tmp, hipStream_t st, long sB)
 {
     typedef typename cvec<T>::t V;
     const int nb = w.nb, bt = w.batch;
-    const long sA = w.sA, sI = w.inv_stride(), sT = (long)max(n, nb) * nrhs;
+    const long sA = w.sA, sI = w.inv_stride(), sT = getrs_tmp_elems(n, nb, nrhs);
     if (n <= 0 || nrhs <= 0) return 0;
     MIDAS_CHECK(nrhs <= 65535, -1, "getrs: nrhs too large for one launch");
     hipLaunchKernelGGL(k_permute_rows<T>, dim3((n + 255) / 256, nrhs, bt), dim3(256), 0, st, B, ldb, sB, tmp, (long)n,
@@ -889,6 +1024,12 @@ int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, const 
     hipLaunchKernelGGL(k_copy2d<T>, dim3((n + 255) / 256, nrhs, bt), dim3(256), 0, st, tmp, (long)n, B, ldb, n, nrhs,
                        sT, sB);
     MIDAS_HIP(hipGetLastError());
+    // few right-hand sides: streaming passes instead of thin GEMMs
+    if (nb <= 512 && !getenv("MIDAS_GETRS_GEMM")) {
+        if (nrhs <= 2) return getrs_few<T, 2>(A, lda, n, w, B, ldb, nrhs, tmp, sT, st, sB);
+        if constexpr (sizeof(T) == 4)
+            if (nrhs <= 8) return getrs_few<T, 8>(A, lda, n, w, B, ldb, nrhs, tmp, sT, st, sB);
+    }
     // forward: B_k <- Linv_k B_k ; B_{>k} -= L_{>k,k} B_k
     for (int k = 0; k < n; k += nb) {
         int b = min(nb, n - k);

@@ -127,6 +127,21 @@ def test_cbfm_vs_oracle(mid, oracle, dlca, use_aca):
     assert _rel(E, Emom) < 5e-2
 
 
+def test_cbfm_one_cbf_direction(mid, oracle, dlca):
+    """One CBF excitation direction: two right-hand sides per block, so the
+    batched block solves take the streaming few-RHS substitution."""
+    s = dlca
+    _case(f"cbfm one cbf direction n={3 * s.n}")
+    oc = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    th_c, ph_c = np.array([20.0]), np.array([30.0])
+    tth, tph = np.array([0.0]), np.array([0.0])
+    E, K, info = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph))
+    Eo, Ko, _ = oracle.cbfm(oc, s.blocks, s.ext, th_c, ph_c, tth, tph)
+    print("K", list(K), "oracle", Ko, "E err", _rel(E, Eo))
+    assert list(K) == list(Ko)
+    assert _rel(E, Eo) < 1e-9
+
+
 def test_cbfm_mixed_reduced_solve(mid, oracle, dlca):
     s = dlca
     _case("cbfm f16x3+ir reduced solve")

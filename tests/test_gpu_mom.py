@@ -144,6 +144,30 @@ def test_lu_panel_paths_agree(mid, mode, monkeypatch):
     assert _rel(X1, Xo) < 1e-10
 
 
+@pytest.mark.parametrize("mode", [api.SOLVE_FP64, api.SOLVE_C32_IR])
+@pytest.mark.parametrize("nrhs", [1, 2, 3, 8])
+def test_getrs_few_rhs_paths_agree(mid, mode, nrhs, monkeypatch):
+    """Few right-hand sides: the streaming substitution (one pass over each
+    block column of L and U) and the GEMM-based substitution give the same
+    solution, and both solve the system; n over two panels and not a multiple
+    of the panel."""
+    rng = np.random.default_rng(13 + nrhs)
+    n = 1301
+    _case(f"getrs few rhs {api.MODE_NAMES[mode]} n={n} nrhs={nrhs}")
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 40.0 * np.eye(n)
+    B = rng.normal(size=(n, nrhs)) + 1j * rng.normal(size=(n, nrhs))
+    X1, i1 = mid.solve(A, B, mode)
+    monkeypatch.setenv("MIDAS_GETRS_GEMM", "1")
+    X2, i2 = mid.solve(A, B, mode)
+    monkeypatch.delenv("MIDAS_GETRS_GEMM")
+    Xo = np.linalg.solve(A, B)
+    print("streaming vs gemm", _rel(X1, X2), "vs numpy", _rel(X1, Xo), i1, i2)
+    assert i1["info"] == 0 and i2["info"] == 0
+    assert _rel(X1, Xo) < 1e-11 and _rel(X2, Xo) < 1e-11
+    assert _rel(X1, X2) < 1e-11
+    assert np.linalg.norm(A @ X1 - B) / (np.linalg.norm(A) * np.linalg.norm(X1)) < 1e-14
+
+
 @pytest.mark.parametrize("column_steps", ["0", "1"])
 def test_lu_exactly_singular(mid, column_steps, monkeypatch):
     """An exactly zero column makes U(j, j) exactly zero: the factorisation
