@@ -248,6 +248,29 @@ int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const
  * nc_extended == 0.  Fails (< 0) when an extension exceeds 10 times the
  * largest block (Fact_Nbext_max, where the reference stops). */
 int midas_block_frames(const midas_cells_t *cells, int nblocks, const int *block_start, double *frames);
+
+/* Division of the cells into CBFM blocks (host routines, no context).  Both
+ * reorder the cells so that every block is a contiguous range, as the
+ * reference leaves Cells: order[i] (n entries) is the input index of the cell
+ * at position i; block_size (capacity cap; n always suffices) receives the
+ * sizes in block order and *nblocks their count; *hblock (optional) = 2 x the
+ * largest block radius, the reference's final hBlock.  Cells keep their input
+ * order inside a block.
+ * midas_partition_sph: div_type 'SPH' (Division_blocks.f90:95-132 with
+ * SPH_v = 2, Division_blocks_sph_H :1252-1401, HierarchicalOctree_subdivision
+ * :1403-1537): octree splits at 1/2 of the centre extent; a block of N >
+ * navg_cells cells with |N - navg_cells| > |nint(N/8.) - navg_cells| is split
+ * again.  best_combin = 1 is SPH_v = 3 (9 split fractions tried per block).
+ * midas_partition_csh: div_type 'CSH' (Division_blocks.f90:133-180,
+ * Division_blocks_csh :190-652): slabs along the longest side of the box
+ * (centre extremes -+ sc/2, sc = SimScatterer%Sc), then halving cuts with the
+ * reference's balancing search, over up to 8 trial hBlock values;
+ * type_s = 3 (cylinder) stops after the slabs.  Fails (< 0) where the
+ * reference reports a failed division. */
+int midas_partition_sph(const midas_cells_t *cells, int navg_cells, int best_combin, int *order, int *block_size,
+                        int cap, int *nblocks, double *hblock);
+int midas_partition_csh(const midas_cells_t *cells, double sc, int navg_cells, int type_s, int *order,
+                        int *block_size, int cap, int *nblocks, double *hblock);
 int midas_extend_blocks(const midas_cells_t *cells, int nblocks, const int *block_start, double sc, int nc_extended,
                         int *ext_start, int *ext_cells, long ext_cap, int *adj_start, int *adj, long adj_cap);
 

@@ -38,7 +38,8 @@ EXPORTS = (
     "midas_block_fill", "midas_plane_wave", "midas_spherical_wave", "midas_solve", "midas_scattering_matrix", "midas_cext_cabs",
     "midas_compute_efields_mom", "midas_aca_block", "midas_aca_matrix", "midas_compute_efields_cbfm",
     "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_profile", "midas_profile_query", "midas_block_frames",
-    "midas_extend_blocks", "midas_scattering_quantities", "midas_near_field",
+    "midas_extend_blocks", "midas_scattering_quantities", "midas_near_field", "midas_partition_sph",
+    "midas_partition_csh",
 )
 
 RX_SD, RX_RF, RX_LB = 0, 1, 2
@@ -175,6 +176,30 @@ def extend_blocks(cells, blocks, sc, nc_extended=1):
     ext = [ec[es[b]:es[b + 1]].astype(np.int64) for b in range(nb)]
     adj = [ad[ast[b]:ast[b + 1]].astype(np.int64) for b in range(nb)]
     return ext, adj
+
+
+def _partition(fn, cells, *args):
+    cells = _cells(cells)
+    order = np.zeros(cells.n, np.int32)
+    sizes = np.zeros(cells.n, np.int32)
+    nb = C.c_int(0)
+    hb = C.c_double(0.0)
+    cs = cells.struct()
+    _check(fn(C.byref(cs), *args, _p(order), _p(sizes), C.c_int(cells.n), C.byref(nb), C.byref(hb)), "partition")
+    return order.astype(np.int64), sizes[:nb.value].astype(np.int64), hb.value
+
+
+def partition_sph(cells, navg_cells, best_combin=0):
+    """Division_blocks, div_type SPH (octree): (order, block sizes, hBlock).
+    order[i] = input index of the cell at position i of the reordered cells.
+    Host routine, no GPU context needed."""
+    return _partition(load().midas_partition_sph, cells, C.c_int(navg_cells), C.c_int(best_combin))
+
+
+def partition_csh(cells, sc, navg_cells, type_s=2):
+    """Division_blocks, div_type CSH (slabs along the longest side, then
+    balanced halvings): (order, block sizes, hBlock).  Host routine."""
+    return _partition(load().midas_partition_csh, cells, C.c_double(sc), C.c_int(navg_cells), C.c_int(type_s))
 
 
 class Midas:

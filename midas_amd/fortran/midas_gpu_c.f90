@@ -128,6 +128,25 @@ module midas_gpu_c
             integer(c_long), value :: ext_cap, adj_cap
         end function midas_extend_blocks
 
+        integer(c_int) function midas_partition_sph(cells, navg_cells, best_combin, order, block_size, cap, &
+                                                    nblocks, hblock) bind(c, name="midas_partition_sph")
+            import :: c_int, c_double, midas_cells_t
+            type(midas_cells_t), intent(in) :: cells
+            integer(c_int), value :: navg_cells, best_combin, cap
+            integer(c_int), intent(out) :: order(*), block_size(*), nblocks
+            real(c_double), intent(out) :: hblock
+        end function midas_partition_sph
+
+        integer(c_int) function midas_partition_csh(cells, sc, navg_cells, type_s, order, block_size, cap, &
+                                                    nblocks, hblock) bind(c, name="midas_partition_csh")
+            import :: c_int, c_double, midas_cells_t
+            type(midas_cells_t), intent(in) :: cells
+            real(c_double), value :: sc
+            integer(c_int), value :: navg_cells, type_s, cap
+            integer(c_int), intent(out) :: order(*), block_size(*), nblocks
+            real(c_double), intent(out) :: hblock
+        end function midas_partition_csh
+
         integer(c_int) function midas_aca_block(ctx, cells, k0, quirks, r0, nr, c0, nc, kmax, eps, u, v, k_out) &
             bind(c, name="midas_aca_block")
             import :: c_int, c_ptr, c_double, midas_cells_t

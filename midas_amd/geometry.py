@@ -350,3 +350,25 @@ def extend_blocks(s: Scatterer, blocks, layers=1):
     cells = api.CellTable(s.x, s.y, s.z, s.sc, s.eps)
     ext, _ = api.extend_blocks(cells, blocks, float(s.sc[0]), layers)
     return ext
+
+
+def divide_blocks(s: Scatterer, div_type="SPH", navg_cells=300, best_combin=0, type_s=2, layers=1):
+    """Division_blocks (Division_blocks.f90:1-183) then Extend_blocks: a copy of
+    the scatterer with its cells reordered block by block (as the reference
+    reorders Cells), its blocks (contiguous ranges) and their extensions.
+    div_type 'SPH' (octree) or 'CSH' (slabs and halvings); navg_cells is the
+    reference's NBlock (avgNcells) input.  Returns (scatterer, hBlock)."""
+    from . import api
+    cells = api.CellTable(s.x, s.y, s.z, s.sc, s.eps)
+    if div_type.upper() == "SPH":
+        order, sizes, hblock = api.partition_sph(cells, navg_cells, best_combin)
+    elif div_type.upper() == "CSH":
+        order, sizes, hblock = api.partition_csh(cells, float(np.min(s.sc)), navg_cells, type_s)
+    else:
+        raise ValueError("div_type must be SPH or CSH")
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    blocks = [np.arange(starts[b], starts[b + 1]) for b in range(len(sizes))]
+    t = Scatterer(s.x[order].copy(), s.y[order].copy(), s.z[order].copy(), s.sc[order].copy(), s.eps[order].copy(),
+                  s.k0, blocks=blocks, name=f"{s.name} {div_type.upper()} blocks={len(blocks)}")
+    t.ext = extend_blocks(t, blocks, layers=layers)
+    return t, hblock

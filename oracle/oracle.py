@@ -443,6 +443,204 @@ def extend_blocks(x, y, z, blocks, sc, nc_extended=1):
     return ext, adj
 
 
+def _f_nint(v):
+    """Fortran NINT: nearest integer, halves away from zero."""
+    return int(np.floor(abs(v) + 0.5)) * (1 if v >= 0 else -1)
+
+
+def _f_ceiling(v):
+    """Fortran CEILING to a default integer; a value outside the integer range
+    (an infinite quotient) gives the x86 conversion's -2**31."""
+    if not np.isfinite(v) or abs(v) >= 2.0 ** 31:
+        return -(2 ** 31)
+    return int(np.ceil(v))
+
+
+def _octree_split(P, cells, fr):
+    """HierarchicalOctree_subdivision (Division_blocks.f90:1403-1537): split the
+    cells (positions P[cells]) at fr of their centre extent per axis; octant
+    number 1 + ix + 2 iy + 4 iz; empty octants dropped; cells stay in input
+    order inside an octant.  Returns the list of octant cell lists."""
+    q = P[cells]
+    lo = q.min(0)
+    h = (q.max(0) - lo) * np.asarray(fr)
+    octs = [[] for _ in range(8)]
+    for c, p in zip(cells, q):
+        i = [0 if p[a] - lo[a] <= h[a] else 1 for a in range(3)]
+        octs[i[0] + 2 * i[1] + 4 * i[2]].append(c)
+    return [o for o in octs if o]
+
+
+_THIRD, _TWO_THIRDS = float(np.float32(1.0 / 3.0)), float(np.float32(2.0 / 3.0))
+_SPH_FRACTIONS = [(0.5, 0.5, 0.5)] + [(a, b, c) for a in (_THIRD, _TWO_THIRDS) for b in (_THIRD, _TWO_THIRDS)
+                                      for c in (_THIRD, _TWO_THIRDS)]
+
+
+def partition_sph(x, y, z, navg, best_combin=0):
+    """Division_blocks with div_type SPH (Division_blocks.f90:95-132, SPH_v = 2
+    -> best_combin 0, SPH_v = 3 -> 1) = Division_blocks_sph_H (:1252-1401).
+    First octree split of all cells, then the block list is scanned: a block of
+    N > navg cells with |N - navg| > |nint(N/8.) - navg| is split again in
+    place and the scan resumes at its first piece.  With best_combin the split
+    fractions are the one of 9 (1/2 everywhere, or 1/3. / 2/3. per axis) whose
+    pieces deviate least from their mean.  Returns (order, sizes, hblock):
+    order = input indices in the new cell order, sizes per block,
+    hblock = 2 max block radius (:160)."""
+    P = np.stack([np.asarray(x, float), np.asarray(y, float), np.asarray(z, float)], 1)
+    blocks = _octree_split(P, list(range(len(P))), _SPH_FRACTIONS[0])
+    ii = 0
+    while ii < len(blocks):
+        N = len(blocks[ii])
+        if N > navg and abs(N - navg) > abs(_f_nint(float(np.float32(N) / np.float32(8.0))) - navg):
+            fr = _SPH_FRACTIONS[0]
+            if best_combin:
+                best = None
+                for f in _SPH_FRACTIONS:
+                    sub = _octree_split(P, blocks[ii], f)
+                    if len(sub) > 1:
+                        mean = N // len(sub)
+                        dev = max(abs(len(s) - mean) for s in sub)
+                        if best is None or dev < best[0]:
+                            best = (dev, f)
+                if best is not None:
+                    fr = best[1]
+            sub = _octree_split(P, blocks[ii], fr)
+            if len(sub) < 2:
+                raise ValueError("partition_sph: a block of coincident cells cannot be split")
+            blocks[ii:ii + 1] = sub
+        else:
+            ii += 1
+    order = np.array([c for b in blocks for c in b], np.int64)
+    sizes = np.array([len(b) for b in blocks], np.int64)
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    fr = block_frames(P[order, 0], P[order, 1], P[order, 2],
+                      [np.arange(starts[i], starts[i + 1]) for i in range(len(sizes))])
+    return order, sizes, 2.0 * float(fr[:, 12].max())
+
+
+def _distribute(pos, cells, nbl, dmin, dstep):
+    """distributeCells (Division_blocks.f90:1668-1709): slab ceiling((c - dmin)
+    / dstep) clamped to [1, nbl], cells in input order per slab."""
+    out = [[] for _ in range(nbl)]
+    for c in cells:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            q = np.float64(pos[c] - dmin) / np.float64(dstep)
+        out[min(max(_f_ceiling(q), 1), nbl) - 1].append(c)
+    return out
+
+
+NB_BLOCK_S_MAX = 10000   # NbBlock_s_max (Initialization.f90:54)
+
+
+def _csh_once(P, order, sc, navg, h_block, box_min, dims, type_s):
+    """Division_blocks_csh (Division_blocks.f90:190-652) on the cells in the
+    given order.  Returns the new order and block sizes, or None where the
+    reference sets error_division."""
+    dirs = []
+    d = list(dims)
+    for _ in range(3):
+        m = max(d)
+        k = d.index(m)
+        dirs.append(k)
+        d[k] = 0.0
+    dir0, dir1, dir2 = dirs
+    hbox = dims[dir0]
+    div = int(hbox / h_block + 1)
+    h0 = hbox / div
+    if div > 1:
+        dstep = _f_nint(h0 / sc) * sc
+        blocks = _distribute(P[:, dir0], order, div, box_min[dir0], dstep)
+    else:
+        blocks = [list(order)]
+    if type_s == 3:
+        blocks = [b for b in blocks if b]
+        return [c for b in blocks for c in b], [len(b) for b in blocks]
+    nbmax = min(min(len(b) for b in blocks), navg)
+    possible = True
+    while possible:
+        possible = False
+        new = []
+        for blk in blocks:
+            n = len(blk)
+            if n > nbmax and abs(nbmax - n) > abs(nbmax - n // 2):
+                possible = True
+                q = P[blk]
+                ext = q.max(0) - q.min(0)
+                dd = dir1 if ext[dir1] > ext[dir2] else dir2
+                h = ext[dd] / 2
+                dmin = q[:, dd].min() - sc / 2.0
+                pos = P[:, dd]
+                sub = _distribute(pos, blk, 2, dmin, _f_nint(h / sc) * sc)
+                ref = n // 2
+                f32 = np.float32
+                err1 = float(f32(abs(len(sub[0]) - ref)) / f32(ref)) if ref else float("nan")
+                prev_err, prev_h = 2 * err1, None
+                step, tr = h / 2.0, 1
+                while err1 <= prev_err and tr <= 20:
+                    prev_err, prev_h = err1, h
+                    h = h - step if len(sub[0]) > len(sub[1]) else h + step
+                    sub = _distribute(pos, blk, 2, dmin, _f_nint(h / sc) * sc)
+                    err1 = float(f32(abs(len(sub[0]) - ref)) / f32(ref))
+                    tr += 1
+                    step = step / 2.0
+                if prev_h is None:
+                    return None
+                sub = _distribute(pos, blk, 2, dmin, _f_nint(prev_h / sc) * sc)
+                new.extend(sub)
+            else:
+                new.append(blk)
+            if len(new) > NB_BLOCK_S_MAX:
+                return None
+        blocks = new
+    return [c for b in blocks for c in b], [len(b) for b in blocks]
+
+
+def partition_csh(x, y, z, sc, navg, type_s=2):
+    """Division_blocks with div_type CSH (Division_blocks.f90:133-180): up to 8
+    trial divisions (hBlock halved after each, the cells reordered by every
+    trial), then the division with the hBlock whose mean block size came
+    closest to navg (or hBlock = the largest box side when the scatterer has
+    at most 2 navg cells).  The box is the scatterer's (Discretization.f90:
+    510-519): centre extremes -+ sc/2.  type_s 3 (cylinder): slabs only.
+    Returns (order, sizes, hblock)."""
+    P = np.stack([np.asarray(x, float), np.asarray(y, float), np.asarray(z, float)], 1)
+    nbc = len(P)
+    box_min = P.min(0) - sc / 2.0
+    dims = list((P.max(0) + sc / 2.0) - box_min)
+    dmax = max(dims)
+    h_block = dmax
+    order = list(range(nbc))
+    diff = [nbc] * 9
+    htest = [None] * 9
+    i, N = 0, nbc
+    while i < 8 and N > navg:
+        r = _csh_once(P, order, sc, navg, h_block, box_min, dims, type_s)
+        if r is None:
+            if i == 0:
+                raise ValueError("partition_csh: the division failed")
+            break
+        order, sizes = r
+        htest[i] = h_block
+        N = nbc // len(sizes)
+        diff[i] = abs(N - navg)
+        h_block = h_block * 0.5
+        i += 1
+    if nbc <= 2 * navg:
+        h_block = dmax
+    else:
+        h_block = htest[int(np.argmin(diff))]
+    r = _csh_once(P, order, sc, navg, h_block, box_min, dims, type_s)
+    if r is None:
+        raise ValueError("partition_csh: the division failed")
+    order, sizes = r
+    order = np.array(order, np.int64)
+    sizes = np.array(sizes, np.int64)
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    fr = block_frames(P[order, 0], P[order, 1], P[order, 2],
+                      [np.arange(starts[i], starts[i + 1]) for i in range(len(sizes))])
+    return order, sizes, 2.0 * float(fr[:, 12].max())
+
+
 def solve(Z, B):
     """PZGESV semantics (Compute_EFields_MoM.f90:173): LU with partial pivoting."""
     return np.linalg.solve(Z, B)

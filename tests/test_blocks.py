@@ -82,3 +82,111 @@ def test_extend_blocks_nbext_max_stop():
     blocks = [np.arange(i, i + 1) for i in range(12)]
     with pytest.raises(api.MidasError):
         api.extend_blocks(_cells(p), blocks, 1.0, 1)
+
+
+# ---------------------------------------------------------------------------
+# Division into blocks: Division_blocks (Division_blocks.f90), SPH and CSH.
+# The file uses the MPI module, so there is no reference-compiled vector.
+# Parity is pinned by (1) the oracle's independent restatement on several
+# geometries, exact integer orders and sizes; (2) lattices whose division has
+# a closed form; (3) invariants (a permutation, contiguous blocks, hBlock =
+# 2 x the largest block radius); (4) the reference's failure case.
+# ---------------------------------------------------------------------------
+
+def _lattice(nx, ny, nz, sc=1.0):
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ijk = np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
+    return ijk, (ijk + 0.5) * sc
+
+
+@pytest.mark.parametrize("geom", ["sphere", "dlca", "hull"])
+@pytest.mark.parametrize("navg", [40, 150, 600])
+def test_partition_matches_oracle(oracle, geom, navg):
+    s = {"sphere": lambda: g.sphere(2.0, 14), "dlca": lambda: g.dlca_aggregate(nmono=8, cells_per_mono=120, seed=3),
+         "hull": lambda: g.convex_hull_particle(5.0, target_cells=2500)}[geom]()
+    c = _cells(np.stack([s.x, s.y, s.z], 1))
+    for best in (0, 1):
+        o1, n1, h1 = api.partition_sph(c, navg, best)
+        o2, n2, h2 = oracle.partition_sph(s.x, s.y, s.z, navg, best)
+        assert np.array_equal(o1, o2) and np.array_equal(n1, n2) and h1 == h2, (geom, navg, best)
+    for type_s in (2, 3):
+        o1, n1, h1 = api.partition_csh(c, float(s.sc[0]), navg, type_s)
+        o2, n2, h2 = oracle.partition_csh(s.x, s.y, s.z, float(s.sc[0]), navg, type_s)
+        assert np.array_equal(o1, o2) and np.array_equal(n1, n2) and h1 == h2, (geom, navg, type_s)
+
+
+def _invariants(p, order, sizes, hblock, input_order=True):
+    n = len(p)
+    assert np.array_equal(np.sort(order), np.arange(n))
+    assert sizes.sum() == n and np.all(sizes > 0)
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    q = p[order]
+    fr = api.block_frames(_cells(q), [np.arange(starts[b], starts[b + 1]) for b in range(len(sizes))])
+    assert hblock == 2.0 * fr[:, 12].max()
+    # SPH: cells keep their input order inside a block (CSH: the order the
+    # previous trial division left)
+    for b in range(len(sizes) if input_order else 0):
+        assert np.all(np.diff(order[starts[b]:starts[b + 1]]) > 0)
+
+
+@pytest.mark.parametrize("navg,nblocks", [(512, 8), (64, 64), (600, 8), (4096, 8)])
+def test_partition_sph_lattice_cube(navg, nblocks):
+    """16^3 lattice cube: the octree cuts at the lattice midplanes.  N = 4096
+    splits (|4096 - navg| > |512 - navg|) into 8 octants of 512; an octant
+    splits again only when |512 - navg| > |64 - navg|.  Blocks are ordered by
+    octant 1 + ix + 2 iy + 4 iz, then sub-octant, then input index."""
+    ijk, p = _lattice(16, 16, 16)
+    order, sizes, hb = api.partition_sph(_cells(p), navg)
+    top = (ijk[:, 0] >= 8) + 2 * (ijk[:, 1] >= 8) + 4 * (ijk[:, 2] >= 8)
+    sub = ((ijk % 8)[:, 0] >= 4) + 2 * ((ijk % 8)[:, 1] >= 4) + 4 * ((ijk % 8)[:, 2] >= 4)
+    key = top * 8 + (sub if nblocks == 64 else 0)
+    want = np.lexsort((np.arange(len(p)), key))
+    assert len(sizes) == nblocks and np.all(sizes == 4096 // nblocks)
+    assert np.array_equal(order, want)
+    _invariants(p, order, sizes, hb)
+
+
+def test_partition_csh_box():
+    """40 x 20 x 10 lattice box (sc = 1): with navg = 4000 (at most 2 navg
+    cells) the box is cut once across its longest side into two 20 x 20 x 10
+    halves; with navg = 2000 each half (4000 > 2000, |2000 - 4000| >
+    |2000 - 2000|) is cut across its next longest side, y, at y = 10 (the
+    balancing search keeps the first, exact cut): 4 blocks of 2000.  type_s
+    = 3 (cylinder) keeps the slabs only, so the trials go on: hBlock 40, 20,
+    10 give 2, 3 and 5 slabs (widths nint(40 / div) = 20, 13, 8) and mean
+    sizes 4000, 2666, 1600; the last is closest to 2000: 5 slabs of 8."""
+    ijk, p = _lattice(40, 20, 10)
+    idx = np.arange(len(p))
+    order, sizes, hb = api.partition_csh(_cells(p), 1.0, 4000)
+    assert list(sizes) == [4000, 4000]
+    assert np.array_equal(order, np.lexsort((idx, ijk[:, 0] >= 20)))
+    _invariants(p, order, sizes, hb)
+    order, sizes, hb = api.partition_csh(_cells(p), 1.0, 2000)
+    assert list(sizes) == [2000] * 4
+    assert np.array_equal(order, np.lexsort((idx, ijk[:, 1] >= 10, ijk[:, 0] >= 20)))
+    _invariants(p, order, sizes, hb)
+    order, sizes, _ = api.partition_csh(_cells(p), 1.0, 2000, type_s=3)
+    assert list(sizes) == [1600] * 5
+    assert np.array_equal(order, np.lexsort((idx, ijk[:, 0] // 8)))
+
+
+@pytest.mark.parametrize("navg", [30, 200])
+def test_partition_invariants_dlca(navg):
+    s = g.dlca_aggregate(nmono=6, cells_per_mono=60, seed=9)
+    p = np.stack([s.x, s.y, s.z], 1)
+    _invariants(p, *api.partition_sph(_cells(p), navg))
+    _invariants(p, *api.partition_sph(_cells(p), navg, 1))
+    _invariants(p, *api.partition_csh(_cells(p), float(s.sc[0]), navg), input_order=False)
+
+
+def test_partition_csh_single_cell_fails(oracle):
+    """One cell: the box gives two slabs, one empty, so the reference's
+    smallest block size is 0 and its one-cell block has no valid cut (its
+    balancing search starts from 0/0): the division fails."""
+    p = np.array([[0.5, 0.5, 0.5]])
+    with pytest.raises(api.MidasError):
+        api.partition_csh(_cells(p), 1.0, 10)
+    with pytest.raises(ValueError):
+        oracle.partition_csh(p[:, 0], p[:, 1], p[:, 2], 1.0, 10)
+    order, sizes, hb = api.partition_sph(_cells(p), 10)
+    assert list(order) == [0] and list(sizes) == [1] and hb == 0.0
