@@ -221,6 +221,15 @@ template <typename X> __device__ inline void st_coh(X *p, X x)
 // visible to every workgroup)
 __device__ inline void stores_done() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// tid mod 32, recomputed where it is used: an address formed from it is not
+// hoisted out of the column loop and kept live (and spilled) across it
+__device__ inline int lane32()
+{
+    int x = threadIdx.x & 31;
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 template <typename T> __device__ inline bool pivot_better(T v, int i, T bv, int bi)
 {
     return v > bv || (v == bv && i < bi);
@@ -249,6 +258,7 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
 {
     typedef typename cvec<T>::t V;
     __shared__ V urow[LU_IB];
+    __shared__ V urowj[LU_IB];   // FP32: row j's values for the thread holding row p
     __shared__ T rv[BLK_T / 64];
     __shared__ int ri[BLK_T / 64];
     const int tid = threadIdx.x, b = blockIdx.x, nwg = gridDim.x, zb = blockIdx.y;
@@ -256,6 +266,9 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
     ipiv += (long)zb * n;
     V *cand = (V *)w.cand + (long)zb * 2 * w.maxblk * LU_IB;
     V *rowj = (V *)w.rowj + (long)zb * 2 * LU_IB;
+    // FP32: the same buffers as tagged 64-bit words, two per complex value
+    unsigned long long *candw = (unsigned long long *)w.cand + (long)zb * 2 * w.maxblk * LU_IB * 2;
+    unsigned long long *rowjw = (unsigned long long *)w.rowj + (long)zb * 2 * LU_IB * 2;
     T *bmax = w.pmax + (long)zb * 2 * w.maxblk;
     int *bidx = w.pidx + (long)zb * 2 * w.maxblk;
     unsigned *tag = w.tag + (long)zb * w.maxblk;
@@ -287,20 +300,35 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
             }
             argmax_block(best, bi, rv, ri);
             bi = __builtin_amdgcn_readfirstlane(bi);   // uniform: keeps address math scalar
-            if (bi == i) {
-#pragma unroll
-                for (int t = 0; t < LU_IB; ++t) st_coh(cand + ((long)par * w.maxblk + b) * LU_IB + t, a[t]);
-            }
-            // row j lies in workgroup 0, thread c
-            if (b == 0 && tid == c) {
-#pragma unroll
-                for (int t = 0; t < LU_IB; ++t) st_coh(rowj + par * LU_IB + t, a[t]);
-            }
-            stores_done();
-            __syncthreads();
             T gv = (T)-1;
             int gi = 0x7fffffff;
             if constexpr (sizeof(T) == 4) {
+                // the candidate row and row j as tagged words (step tag | float
+                // bits): a reader checks every word's tag, so these stores need
+                // no completion wait before the record is published
+                // (staged through LDS so that one wave stores each row, a word
+                // per lane; row j lies in workgroup 0, thread c)
+                if (bi == i) {
+#pragma unroll
+                    for (int t = 0; t < LU_IB; ++t) urow[t] = a[t];
+                }
+                if (b == 0 && tid == c) {
+#pragma unroll
+                    for (int t = 0; t < LU_IB; ++t) urowj[t] = a[t];
+                }
+                __syncthreads();
+                const unsigned long long tg = (unsigned long long)want << 32;
+                // wave 0: the candidate's 32 words; wave 1 of workgroup 0: row j's
+                if (tid < 2 * LU_IB) {
+                    unsigned long long *dst = candw + ((long)par * w.maxblk + b) * LU_IB * 2;
+                    const int l = lane32();
+                    st_coh(dst + l, tg | __float_as_uint(((float *)urow)[l]));
+                } else if (b == 0 && tid >= 64 && tid < 64 + 2 * LU_IB) {
+                    unsigned long long *dst = rowjw + par * LU_IB * 2;
+                    const int l = lane32();
+                    st_coh(dst + l, tg | __float_as_uint(((float *)urowj)[l]));
+                }
+                __syncthreads();   // urow, urowj are reused below
                 // one 64-bit record: step tag (12 bits) | row (20 bits) | max (float bits)
                 unsigned long long *rec = (unsigned long long *)w.pmax + (long)zb * 2 * w.maxblk;
                 if (tid == 0)
@@ -326,6 +354,17 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
                     if (pivot_better(v, vi, gv, gi)) { gv = v; gi = vi; }
                 }
             } else {
+                if (bi == i) {
+#pragma unroll
+                    for (int t = 0; t < LU_IB; ++t) st_coh(cand + ((long)par * w.maxblk + b) * LU_IB + t, a[t]);
+                }
+                // row j lies in workgroup 0, thread c
+                if (b == 0 && tid == c) {
+#pragma unroll
+                    for (int t = 0; t < LU_IB; ++t) st_coh(rowj + par * LU_IB + t, a[t]);
+                }
+                stores_done();
+                __syncthreads();
                 if (tid == 0) {
                     st_coh(bmax + par * w.maxblk + b, best);
                     st_coh(bidx + par * w.maxblk + b, bi);
@@ -350,15 +389,48 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
             // ---- global pivot -------------------------------------------------
             argmax_block(gv, gi, rv, ri);
             const int p = __builtin_amdgcn_readfirstlane(gi == 0x7fffffff ? j : gi);
-            if (tid < LU_IB)
-                urow[tid] = p == j ? ld_coh(rowj + par * LU_IB + tid)
-                                   : ld_coh(cand + ((long)par * w.maxblk + (p - jb) / BLK_T) * LU_IB + tid);
-            // row p takes row j's old values (loaded alongside the pivot row)
-            if (p != j && i == p) {
+            if constexpr (sizeof(T) == 4) {
+                // wave 0: the pivot row's 32 words; wave 1 of the workgroup
+                // holding row p: row j's (row p takes them)
+                const bool holds_p = p != j && (p - jb) / BLK_T == b;
+                auto fetch = [&](const unsigned long long *src, float *dst) {
+                    int spins = 0;
+                    unsigned long long v = ld_coh(src);
+                    while (!waited_out && (unsigned)(v >> 32) != want) {
+                        if (++spins > BLK_SPIN) {
+                            st_coh(w.fault, 1);
+                            waited_out = true;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        v = ld_coh(src);
+                    }
+                    *dst = __uint_as_float((unsigned)v);
+                };
+                if (tid < 2 * LU_IB) {
+                    const unsigned long long *src =
+                        p == j ? rowjw + par * LU_IB * 2 : candw + ((long)par * w.maxblk + (p - jb) / BLK_T) * LU_IB * 2;
+                    const int l = lane32();
+                    fetch(src + l, (float *)urow + l);
+                } else if (holds_p && tid >= 64 && tid < 64 + 2 * LU_IB) {
+                    const int l = lane32();
+                    fetch(rowjw + par * LU_IB * 2 + l, (float *)urowj + l);
+                }
+                __syncthreads();
+                if (holds_p && i == p) {
 #pragma unroll
-                for (int t = 0; t < LU_IB; ++t) a[t] = ld_coh(rowj + par * LU_IB + t);
+                    for (int t = 0; t < LU_IB; ++t) a[t] = urowj[t];
+                }
+            } else {
+                if (tid < LU_IB)
+                    urow[tid] = p == j ? ld_coh(rowj + par * LU_IB + tid)
+                                       : ld_coh(cand + ((long)par * w.maxblk + (p - jb) / BLK_T) * LU_IB + tid);
+                // row p takes row j's old values (loaded alongside the pivot row)
+                if (p != j && i == p) {
+#pragma unroll
+                    for (int t = 0; t < LU_IB; ++t) a[t] = ld_coh(rowj + par * LU_IB + t);
+                }
+                __syncthreads();
             }
-            __syncthreads();
             if (b == 0 && tid == 0) ipiv[j] = p;
             // row j takes the pivot row
             if (p != j && i == j) {
@@ -408,14 +480,18 @@ __global__ __launch_bounds__(64) void k_swap_trsm(typename cvec<T>::t *A, long l
                                                   int jb, int jend, int k, int k1)
 {
     typedef typename cvec<T>::t V;
-    __shared__ int pos[2 * LU_IB], src[2 * LU_IB];
+    __shared__ int pos[2 * LU_IB], src[2 * LU_IB], piv[LU_IB];
     __shared__ int npair;
     A += blockIdx.y * sA;
     ipiv += (long)blockIdx.y * n;
+    // the block's pivots in one parallel load (a serial loop would wait on
+    // each global load in turn)
+    if (threadIdx.x < jend - jb) piv[threadIdx.x] = ipiv[jb + threadIdx.x];
+    __syncthreads();
     if (threadIdx.x == 0) {
         int m = 0;
         for (int j = jb; j < jend; ++j) {
-            const int p = ipiv[j];
+            const int p = piv[j - jb];
             if (p == j) continue;
             int sj = -1, sp = -1;
             for (int s = 0; s < m; ++s) {
@@ -776,6 +852,8 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
     // last-workgroup counters, info, the fault flag and the step tags
     MIDAS_HIP(hipMemsetAsync(w.counter, 0, sizeof(int) * (2 * w.batch + 1 + w.maxblk * w.batch), st));
     MIDAS_HIP(hipMemsetAsync(w.pmax, 0, 8 * 2 * w.maxblk * w.batch, st));   // tagged records
+    MIDAS_HIP(hipMemsetAsync(w.cand, 0, 16 * 2 * w.maxblk * LU_IB * w.batch, st));   // tagged words
+    MIDAS_HIP(hipMemsetAsync(w.rowj, 0, 16 * 2 * LU_IB * w.batch, st));
     w.tag_base = 0;
     MIDAS_HIP(hipEventRecord(w.ev_next, st));
     MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
@@ -1094,8 +1172,9 @@ int LuWork<T>::alloc(int n_, int nb_, int batch_, long sA_)
     info = counter + batch;
     fault = counter + 2 * batch;
     tag = (unsigned *)(counter + 2 * batch + 1);
-    MIDAS_HIP(hipMalloc(&cand, sizeof(cplx<T>) * 2 * maxblk * LU_IB * batch));
-    MIDAS_HIP(hipMalloc(&rowj, sizeof(cplx<T>) * 2 * LU_IB * batch));
+    // 16 bytes per element: complex FP64, or complex FP32 as two tagged words
+    MIDAS_HIP(hipMalloc(&cand, 16 * 2 * maxblk * LU_IB * batch));
+    MIDAS_HIP(hipMalloc(&rowj, 16 * 2 * LU_IB * batch));
     MIDAS_HIP(hipMalloc(&pinv, sizeof(cplx<T>) * batch));
     MIDAS_HIP(hipMalloc(&linv, sizeof(cplx<T>) * inv_stride() * batch));
     MIDAS_HIP(hipMalloc(&uinv, sizeof(cplx<T>) * inv_stride() * batch));
