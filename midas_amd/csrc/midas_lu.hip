@@ -550,8 +550,10 @@ __global__ __launch_bounds__(64) void k_swap_trsm(typename cvec<T>::t *A, long l
 // once): thread = row, its IB multipliers in registers, the workgroup's
 // UPD_COLS columns of U in LDS (read as broadcasts).  FP32: kept under 104
 // registers (5 waves per SIMD) so its waves fit beside the trailing GEMM's.
-constexpr int UPD_COLS = 64;
-template <typename T, int WPE>
+// UPD_COLS = 64 amortises each row's multipliers over many columns; for few
+// rows (the last panels) 16 columns per workgroup give four times the
+// workgroups and a quarter of the serial column batches.
+template <typename T, int WPE, int UPD_COLS>
 __global__ __launch_bounds__(256, WPE) void k_panel_update(typename cvec<T>::t *A, long lda, long sA, int n, int jb,
                                                            int ib, int jend, int c1)
 {
@@ -655,12 +657,13 @@ __global__ void k_laswp(typename cvec<T>::t *A, long lda, long sA, const int *ip
 // x = U^-1 e_c.  Inv is ldi x bs.
 template <typename T, bool LOWER, int R>
 __global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D, long lda, long sD, int bs,
-                                                    typename cvec<T>::t *Inv, long ldi, long sI)
+                                                    typename cvec<T>::t *Inv, long ldi, long sI, long sD2 = 0,
+                                                    long sI2 = 0)
 {
     typedef typename cvec<T>::t V;
     constexpr int TRINV_QC = R >= 8 ? 2 : (R >= 4 ? 4 : 8);
-    D += blockIdx.y * sD;
-    Inv += blockIdx.y * sI;
+    D += blockIdx.y * sD + blockIdx.z * sD2;
+    Inv += blockIdx.y * sI + blockIdx.z * sI2;
     const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (c >= bs) return;   // wave-uniform
@@ -728,18 +731,21 @@ __global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D
     }
 }
 
+// Inverses of bs x bs triangular blocks: grid y over the matrices of a batch
+// (strides sD, sI), grid z over nblk blocks of each (strides sD2, sI2).
 template <typename T, bool LOWER>
 static int trinv(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T>::t *Inv, long ldi, hipStream_t st,
-                 int batch = 1, long sD = 0, long sI = 0)
+                 int batch = 1, long sD = 0, long sI = 0, int nblk = 1, long sD2 = 0, long sI2 = 0)
 {
     MIDAS_CHECK(bs <= 512, -1, "trinv: block %d above 512", bs);
-    const dim3 g((bs + 3) / 4, batch), b(256);
+    MIDAS_CHECK(nblk >= 1 && nblk <= 65535 && batch <= 65535, -1, "trinv: %d x %d blocks", batch, nblk);
+    const dim3 g((bs + 3) / 4, batch, nblk), b(256);
     if (bs <= 128)
-        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 2>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI);
+        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 2>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI, sD2, sI2);
     else if (bs <= 256)
-        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 4>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI);
+        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 4>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI, sD2, sI2);
     else
-        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 8>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI);
+        hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 8>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI, sD2, sI2);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
@@ -806,9 +812,13 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
         // rest of the outer panel: columns [jend, k+nb)
         if (rc > 0 && n - jend > 0) {
             constexpr int wpe = sizeof(T) == 4 ? 5 : 4;
-            hipLaunchKernelGGL((k_panel_update<T, wpe>),
-                               dim3((n - jend + 255) / 256, (rc + UPD_COLS - 1) / UPD_COLS, w.batch), dim3(256), 0, st,
-                               A, lda, w.sA, n, jb, jend - jb, jend, k + nb);
+            const int rb = (n - jend + 255) / 256;
+            if (rb * w.batch >= 64)
+                hipLaunchKernelGGL((k_panel_update<T, wpe, 64>), dim3(rb, (rc + 63) / 64, w.batch), dim3(256), 0, st, A,
+                                   lda, w.sA, n, jb, jend - jb, jend, k + nb);
+            else
+                hipLaunchKernelGGL((k_panel_update<T, wpe, 16>), dim3(rb, (rc + 15) / 16, w.batch), dim3(256), 0, st, A,
+                                   lda, w.sA, n, jb, jend - jb, jend, k + nb);
             MIDAS_HIP(hipGetLastError());
         }
     }
@@ -897,10 +907,19 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
         MIDAS_HIP(hipEventRecord(w.ev_panel, ps));
     }
     MIDAS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
-    // inverses of the diagonal blocks of U, used by getrs (those of L come from the panels)
-    for (int k = 0; k < n; k += w.nb) {
-        r = trinv<T, false>(A + k + (long)k * lda, lda, min(w.nb, n - k), (V *)w.uinv + (long)(k / w.nb) * w.nb * w.nb,
-                            w.nb, st, w.batch, w.sA, w.inv_stride());
+    // inverses of the diagonal blocks of U, used by getrs (those of L come from
+    // the panels): the full blocks in one launch (they are independent), then
+    // the last, partial one
+    const int nfull = n / w.nb;
+    if (nfull > 0) {
+        r = trinv<T, false>(A, lda, w.nb, (V *)w.uinv, w.nb, st, w.batch, w.sA, w.inv_stride(), nfull,
+                            (long)w.nb * (lda + 1), (long)w.nb * w.nb);
+        if (r) return r;
+    }
+    if (n % w.nb) {
+        const int k = nfull * w.nb;
+        r = trinv<T, false>(A + k + (long)k * lda, lda, n - k, (V *)w.uinv + (long)nfull * w.nb * w.nb, w.nb, st,
+                            w.batch, w.sA, w.inv_stride());
         if (r) return r;
     }
     int fault = 0;
