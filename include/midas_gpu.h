@@ -172,6 +172,33 @@ int midas_scattering_quantities(midas_ctx *ctx, const double *S, long lds, int n
                                 double dm, int quirks, const double *C_ext, const double *C_abs, int cext_intfields,
                                 double *Q, double *Q_av);
 
+/* Compute_Scattering_Quantities_1 (Compute_Scattering_Quantities.f90:1-589),
+ * the rule when the transmitters are integrated by tr_type = MIDAS_INT_UN
+ * (the reference's single-precision QUADPACK qag, 61-point Gauss-Kronrod,
+ * reading the integrand at the nearest grid node), _GL (Gauss-Legendre),
+ * _TR (trapezoid cubature; the reference passes the lower phi bound twice,
+ * so its average is 0) or _SM (no rule: 0).  Host routine, no context.
+ * Transmitters (tr_th, tr_ph, degrees) form a theta x phi grid, theta
+ * fastest; receivers rows 0..nrx-1 of S a grid of the same kind for rx_type
+ * _UN / _GL / _TR / _SM, or a list for _SD (equal weights) / _LB (weights
+ * rx_w).  The forward and backward directions are looked up among all nrx_tot
+ * receivers (rx_th, rx_ph) within 1e-2 degree, as getTxRxIndex.  Other
+ * arguments and outputs as midas_scattering_quantities. */
+enum { MIDAS_INT_SD = 0, MIDAS_INT_LB = 1, MIDAS_INT_UN = 2, MIDAS_INT_GL = 3, MIDAS_INT_TR = 4, MIDAS_INT_SM = 5 };
+int midas_scattering_quantities_grid(const double *S, long lds, int nrx_tot, const double *rx_th, const double *rx_ph,
+                                     int nrx, int rx_type, const double *rx_w, int ntr, const double *tr_th,
+                                     const double *tr_ph, int tr_type, double k0, double dm, int quirks,
+                                     const double *C_ext, const double *C_abs, int cext_intfields, double *Q,
+                                     double *Q_av);
+/* The quadrature rules of the above, exposed for callers that integrate their
+ * own grids: getAdaptiveQuadPts2 (Numerical_Integration.f90:213-245; ph = 0 for
+ * 'th', 1 for 'ph'; f is d1 x d2, column jj 1-based), trap_2Dc (:251-305) and
+ * the Gauss-Legendre rule of getGaussLegendreQuadPts3 (:88-145). */
+int midas_quad_adaptive(int nth, int nph, int d1, int d2, const double *f, int ph, int jj, double a0, double b0,
+                        double *I);
+int midas_quad_trap2d(int nx, int ny, const double *f, double a, double b, double c, double d, double *I);
+int midas_quad_gauss_legendre(int n, double a, double b, double *x, double *w);
+
 /* ---------------- Compute_EFields_MoM drop-in ----------------
  * Compute_EFields_MoM(Cells, Transmitters, Receivers, S_total, C_ext, C_abs)
  * (Compute_EFields_MoM.f90:1-19): fill, solve, S_total (nrx x 4ntr),

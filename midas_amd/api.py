@@ -39,10 +39,12 @@ EXPORTS = (
     "midas_compute_efields_mom", "midas_aca_block", "midas_aca_matrix", "midas_compute_efields_cbfm",
     "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_profile", "midas_profile_query", "midas_block_frames",
     "midas_extend_blocks", "midas_scattering_quantities", "midas_near_field", "midas_partition_sph",
-    "midas_partition_csh",
+    "midas_partition_csh", "midas_scattering_quantities_grid", "midas_quad_adaptive", "midas_quad_trap2d",
+    "midas_quad_gauss_legendre",
 )
 
 RX_SD, RX_RF, RX_LB = 0, 1, 2
+INT_SD, INT_LB, INT_UN, INT_GL, INT_TR, INT_SM = range(6)   # Compute_Scattering_Quantities_1 rules
 
 
 class MidasError(RuntimeError):
@@ -200,6 +202,55 @@ def partition_csh(cells, sc, navg_cells, type_s=2):
     """Division_blocks, div_type CSH (slabs along the longest side, then
     balanced halvings): (order, block sizes, hBlock).  Host routine."""
     return _partition(load().midas_partition_csh, cells, C.c_double(sc), C.c_int(navg_cells), C.c_int(type_s))
+
+
+def scattering_quantities_grid(S, nrx, rx_dirs, rx_type, tr_dirs, tr_type, k0, dm, cext, cabs, rx_w=None,
+                               cext_intfields=False, quirks=Q_ALL):
+    """Compute_Scattering_Quantities_1 on the host: S (NRx_tot x 4 ntr),
+    rx_dirs = (theta, phi) of all NRx_tot receivers (the first nrx are the
+    scattering grid or list), tr_dirs of the ntr transmitters (a theta x phi
+    grid, theta fastest).  Returns (Q (ntr x 5): Q_ext, Q_abs, Q_sca, Q_bks,
+    g; Q_av (5))."""
+    S = np.asfortranarray(S, np.complex128)
+    rth, rph = (np.ascontiguousarray(a, np.float64) for a in rx_dirs)
+    tth, tph = (np.ascontiguousarray(a, np.float64) for a in tr_dirs)
+    ntr = len(tth)
+    ce = np.ascontiguousarray(cext, np.complex128)
+    ca = np.ascontiguousarray(cabs, np.complex128)
+    w = None if rx_w is None else np.ascontiguousarray(rx_w, np.float64)
+    Q = np.zeros((ntr, 5))
+    Qav = np.zeros(5)
+    _check(load().midas_scattering_quantities_grid(_p(S), C.c_long(S.shape[0]), C.c_int(len(rth)), _p(rth), _p(rph),
+                                                   C.c_int(nrx), C.c_int(rx_type), _p(w), C.c_int(ntr), _p(tth),
+                                                   _p(tph), C.c_int(tr_type), C.c_double(k0), C.c_double(dm),
+                                                   C.c_int(quirks), _p(ce), _p(ca), C.c_int(int(cext_intfields)),
+                                                   _p(Q), _p(Qav)), "scattering_quantities_grid")
+    return Q, Qav
+
+
+def quad_adaptive(f, nth, nph, ph, jj, a0, b0):
+    """getAdaptiveQuadPts2 of the library (column jj, 1-based, of f)."""
+    f = np.asfortranarray(f, np.float64)
+    out = C.c_double(0)
+    _check(load().midas_quad_adaptive(C.c_int(nth), C.c_int(nph), C.c_int(f.shape[0]), C.c_int(f.shape[1]), _p(f),
+                                      C.c_int(1 if ph else 0), C.c_int(jj), C.c_double(a0), C.c_double(b0),
+                                      C.byref(out)), "quad_adaptive")
+    return out.value
+
+
+def quad_trap2d(f, a, b, c, d):
+    f = np.asfortranarray(f, np.float64)
+    out = C.c_double(0)
+    _check(load().midas_quad_trap2d(C.c_int(f.shape[0]), C.c_int(f.shape[1]), _p(f), C.c_double(a), C.c_double(b),
+                                    C.c_double(c), C.c_double(d), C.byref(out)), "quad_trap2d")
+    return out.value
+
+
+def quad_gauss_legendre(n, a, b):
+    x, w = np.zeros(n), np.zeros(n)
+    _check(load().midas_quad_gauss_legendre(C.c_int(n), C.c_double(a), C.c_double(b), _p(x), _p(w)),
+           "quad_gauss_legendre")
+    return x, w
 
 
 class Midas:

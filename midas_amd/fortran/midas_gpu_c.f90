@@ -8,6 +8,8 @@ module midas_gpu_c
     integer(c_int), parameter :: MIDAS_Q_PI = 1, MIDAS_Q_LITERALS = 2, MIDAS_Q_CEXT_SUM = 4, &
                                  MIDAS_Q_FP32_BLOCKS = 8, MIDAS_Q_ALL = 15
     integer(c_int), parameter :: MIDAS_RX_SD = 0, MIDAS_RX_RF = 1, MIDAS_RX_LB = 2
+    integer(c_int), parameter :: MIDAS_INT_SD = 0, MIDAS_INT_LB = 1, MIDAS_INT_UN = 2, MIDAS_INT_GL = 3, &
+                                 MIDAS_INT_TR = 4, MIDAS_INT_SM = 5
     integer(c_int), parameter :: MIDAS_SOLVE_FP64 = 0, MIDAS_SOLVE_C32_IR = 1, MIDAS_SOLVE_F16_IR = 2, &
                                  MIDAS_SOLVE_F16X3_IR = 3, MIDAS_SOLVE_BF16_IR = 4, MIDAS_SOLVE_BF16X3_IR = 5
 
@@ -114,6 +116,43 @@ module midas_gpu_c
             real(c_double), value :: k0, dm
             real(c_double), intent(out) :: q(*), q_av(*)
         end function midas_scattering_quantities
+
+        integer(c_int) function midas_scattering_quantities_grid(s, lds, nrx_tot, rx_th, rx_ph, nrx, rx_type, rx_w, &
+                                                                 ntr, tr_th, tr_ph, tr_type, k0, dm, quirks, c_ext, &
+                                                                 c_abs, cext_intfields, q, q_av) &
+            bind(c, name="midas_scattering_quantities_grid")
+            import :: c_int, c_long, c_ptr, c_double
+            type(c_ptr), value :: s, rx_w
+            integer(c_long), value :: lds
+            integer(c_int), value :: nrx_tot, nrx, rx_type, ntr, tr_type, quirks, cext_intfields
+            real(c_double), intent(in) :: rx_th(*), rx_ph(*), tr_th(*), tr_ph(*), c_ext(*), c_abs(*)
+            real(c_double), value :: k0, dm
+            real(c_double), intent(out) :: q(*), q_av(*)
+        end function midas_scattering_quantities_grid
+
+        integer(c_int) function midas_quad_adaptive(nth, nph, d1, d2, f, ph, jj, a0, b0, res) &
+            bind(c, name="midas_quad_adaptive")
+            import :: c_int, c_double
+            integer(c_int), value :: nth, nph, d1, d2, ph, jj
+            real(c_double), intent(in) :: f(*)
+            real(c_double), value :: a0, b0
+            real(c_double), intent(out) :: res
+        end function midas_quad_adaptive
+
+        integer(c_int) function midas_quad_trap2d(nx, ny, f, a, b, c, d, res) bind(c, name="midas_quad_trap2d")
+            import :: c_int, c_double
+            integer(c_int), value :: nx, ny
+            real(c_double), intent(in) :: f(*)
+            real(c_double), value :: a, b, c, d
+            real(c_double), intent(out) :: res
+        end function midas_quad_trap2d
+
+        integer(c_int) function midas_quad_gauss_legendre(n, a, b, x, w) bind(c, name="midas_quad_gauss_legendre")
+            import :: c_int, c_double
+            integer(c_int), value :: n
+            real(c_double), value :: a, b
+            real(c_double), intent(out) :: x(*), w(*)
+        end function midas_quad_gauss_legendre
 
         integer(c_int) function midas_extend_blocks(cells, nblocks, block_start, sc, nc_extended, ext_start, &
                                                     ext_cells, ext_cap, adj_start, adj, adj_cap) &

@@ -12,6 +12,7 @@ import this module; the product never does.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -875,3 +876,394 @@ def ref_transmitters_un(niws, nmax=100000):
     n = C.c_int(0)
     r.ref_transmitters_un(C.c_int(niws), C.c_int(nmax), _p(th), _p(ph), C.byref(n))
     return th[:n.value].copy(), ph[:n.value].copy()
+
+
+def ref_adaptive_quad(f, nth, nph, ph, jj, a0, b0):
+    """The reference's getAdaptiveQuadPts2 (compiled): f (d1 x d2) grid values."""
+    r = ref()
+    f = np.asfortranarray(f, dtype=np.float64)
+    out = C.c_double(0)
+    r.ref_adaptive_quad(C.c_int(nth), C.c_int(nph), C.c_int(f.shape[0]), C.c_int(f.shape[1]), _p(f),
+                        C.c_int(1 if ph else 0), C.c_int(jj), C.c_double(a0), C.c_double(b0), C.byref(out))
+    return out.value
+
+
+def ref_trap2d(f, a, b, c, d):
+    r = ref()
+    f = np.asfortranarray(f, dtype=np.float64)
+    out = C.c_double(0)
+    r.ref_trap2d(C.c_int(f.shape[0]), C.c_int(f.shape[1]), _p(f), C.c_double(a), C.c_double(b), C.c_double(c),
+                 C.c_double(d), C.byref(out))
+    return out.value
+
+
+def ref_gauss_legendre(n, a, b):
+    r = ref()
+    x, w = np.zeros(n), np.zeros(n)
+    r.ref_gauss_legendre(C.c_int(n), C.c_double(a), C.c_double(b), _p(x), _p(w))
+    return x, w
+
+
+# ---------------------------------------------------------------------------
+# Compute_Scattering_Quantities_1 (Compute_Scattering_Quantities.f90:1-589) and
+# its quadrature rules, restated.  QUADPACK qag / qk61 (quadpack.f90; the
+# published algorithm of Piessens et al. 1983, 61-point Gauss-Kronrod rule) in
+# single precision, reading the integrand at the nearest grid node
+# (getAdaptiveQuadPts2, Numerical_Integration.f90:213-245).
+# ---------------------------------------------------------------------------
+_F = np.float32
+_XGK = np.array([0.999484410050490637571325895705811, 0.996893484074649540271630050918695,
+                 0.991630996870404594858628366109486, 0.983668123279747209970032581605663,
+                 0.973116322501126268374693868423707, 0.960021864968307512216871025581798,
+                 0.944374444748559979415831324037439, 0.926200047429274325879324277080474,
+                 0.905573307699907798546522558925958, 0.882560535792052681543116462530226,
+                 0.857205233546061098958658510658944, 0.829565762382768397442898119732502,
+                 0.799727835821839083013668942322683, 0.767777432104826194917977340974503,
+                 0.733790062453226804726171131369528, 0.697850494793315796932292388026640,
+                 0.660061064126626961370053668149271, 0.620526182989242861140477556431189,
+                 0.579345235826361691756024932172540, 0.536624148142019899264169793311073,
+                 0.492480467861778574993693061207709, 0.447033769538089176780609900322854,
+                 0.400401254830394392535476211542661, 0.352704725530878113471037207089374,
+                 0.304073202273625077372677107199257, 0.254636926167889846439805129817805,
+                 0.204525116682309891438957671002025, 0.153869913608583546963794672743256,
+                 0.102806937966737030147096751318001, 0.051471842555317695833025213166723, 0.0], _F)
+_WGK = np.array([0.001389013698677007624551591226760, 0.003890461127099884051267201844516,
+                 0.006630703915931292173319826369750, 0.009273279659517763428441146892024,
+                 0.011823015253496341742232898853251, 0.014369729507045804812451432443580,
+                 0.016920889189053272627572289420322, 0.019414141193942381173408951050128,
+                 0.021828035821609192297167485738339, 0.024191162078080601365686370725232,
+                 0.026509954882333101610601709335075, 0.028754048765041292843978785354334,
+                 0.030907257562387762472884252943092, 0.032981447057483726031814191016854,
+                 0.034979338028060024137499670731468, 0.036882364651821229223911065617136,
+                 0.038678945624727592950348651532281, 0.040374538951535959111995279752468,
+                 0.041969810215164246147147541285970, 0.043452539701356069316831728117073,
+                 0.044814800133162663192355551616723, 0.046059238271006988116271735559374,
+                 0.047185546569299153945261478181099, 0.048185861757087129140779492298305,
+                 0.049055434555029778887528165367238, 0.049795683427074206357811569379942,
+                 0.050405921402782346840893085653585, 0.050881795898749606492297473049805,
+                 0.051221547849258772170656282604944, 0.051426128537459025933862879215781,
+                 0.051494729429451567558340433647099], _F)
+_WG = np.array([0.007968192496166605615465883474674, 0.018466468311090959142302131912047,
+                0.028784707883323369349719179611292, 0.038799192569627049596801936446348,
+                0.048402672830594052902938140422808, 0.057493156217619066481721689402056,
+                0.065974229882180495128128515115962, 0.073755974737705206268243850022191,
+                0.080755895229420215354694938460530, 0.086899787201082979802387530715126,
+                0.092122522237786128717632707087619, 0.096368737174644259639468626351810,
+                0.099593420586795267062780282103569, 0.101762389748405504596428952168554,
+                0.102852652893558840341285636705415], _F)
+_FEPS, _FTINY = _F(np.finfo(np.float32).eps), _F(np.finfo(np.float32).tiny)
+
+
+def _nint32(v):
+    v = float(v)
+    return int(np.floor(abs(v) + 0.5)) * (1 if v >= 0 else -1)
+
+
+def _qk61(fcol, n, a0, b0, a, b):
+    """qk61 (quadpack.f90:486-725) on the grid column fcol (float64 values,
+    read as float32 at node nint(x (n-1) / (b0 - a0)) + 1)."""
+    def at(x):
+        return fcol[_nint32(_F(_F(x) * _F(n - 1)) / _F(b0 - a0))]
+    centr, hlgth = _F(_F(0.5) * _F(b + a)), _F(_F(0.5) * _F(b - a))
+    dhlgth = abs(hlgth)
+    resg = _F(0)
+    fc = at(centr)
+    resk = _F(_WGK[30] * fc)
+    resabs = abs(resk)
+    fv1, fv2 = [_F(0)] * 30, [_F(0)] * 30
+    for j in range(1, 16):
+        jtw = 2 * j
+        absc = _F(hlgth * _XGK[jtw - 1])
+        f1, f2 = at(_F(centr - absc)), at(_F(centr + absc))
+        fv1[jtw - 1], fv2[jtw - 1] = f1, f2
+        fsum = _F(f1 + f2)
+        resg = _F(resg + _F(_WG[j - 1] * fsum))
+        resk = _F(resk + _F(_WGK[jtw - 1] * fsum))
+        resabs = _F(resabs + _F(_WGK[jtw - 1] * _F(abs(f1) + abs(f2))))
+    for j in range(1, 16):
+        jm = 2 * j - 1
+        absc = _F(hlgth * _XGK[jm - 1])
+        f1, f2 = at(_F(centr - absc)), at(_F(centr + absc))
+        fv1[jm - 1], fv2[jm - 1] = f1, f2
+        fsum = _F(f1 + f2)
+        resk = _F(resk + _F(_WGK[jm - 1] * fsum))
+        resabs = _F(resabs + _F(_WGK[jm - 1] * _F(abs(f1) + abs(f2))))
+    reskh = _F(resk * _F(0.5))
+    resasc = _F(_WGK[30] * abs(_F(fc - reskh)))
+    for j in range(30):
+        resasc = _F(resasc + _F(_WGK[j] * _F(abs(_F(fv1[j] - reskh)) + abs(_F(fv2[j] - reskh)))))
+    result = _F(resk * hlgth)
+    resabs = _F(resabs * dhlgth)
+    resasc = _F(resasc * dhlgth)
+    abserr = abs(_F(_F(resk - resg) * hlgth))
+    if resasc != 0 and abserr != 0:
+        abserr = _F(resasc * min(_F(1), _F(np.power(_F(_F(_F(200) * abserr) / resasc), _F(1.5)))))
+    if resabs > _F(_FTINY / _F(_F(50) * _FEPS)):
+        abserr = max(_F(_F(_FEPS * _F(50)) * resabs), abserr)
+    return result, abserr, resabs, resasc
+
+
+def _qpsrt(limit, last, maxerr, elist, iord, nrmax):
+    """qsort of quadpack.f90 (QUADPACK qpsrt), 1-based indices."""
+    E = lambda i: elist[i - 1]   # noqa: E731
+    if last <= 2:
+        iord[0], iord[1] = 1, 2
+    else:
+        errmax = E(maxerr)
+        for _ in range(1, nrmax):
+            isucc = iord[nrmax - 2]
+            if errmax <= E(isucc):
+                break
+            iord[nrmax - 1] = isucc
+            nrmax -= 1
+        jupbn = last
+        if limit // 2 + 2 < last:
+            jupbn = limit + 3 - last
+        errmin = E(last)
+        jbnd, ibeg = jupbn - 1, nrmax + 1
+        i, found = ibeg, False
+        while i <= jbnd:
+            isucc = iord[i - 1]
+            if E(isucc) <= errmax:
+                found = True
+                break
+            iord[i - 2] = isucc
+            i += 1
+        if not found:
+            iord[jbnd - 1] = maxerr
+            iord[jupbn - 1] = last
+        else:
+            iord[i - 2] = maxerr
+            k, placed = jbnd, False
+            for _ in range(i, jbnd + 1):
+                isucc = iord[k - 1]
+                if errmin < E(isucc):
+                    iord[k] = last
+                    placed = True
+                    break
+                iord[k] = isucc
+                k -= 1
+            if not placed:
+                iord[i - 1] = last
+    maxerr = iord[nrmax - 1]
+    return maxerr, E(maxerr), nrmax
+
+
+def adaptive_quad(f, nth, nph, ph, jj, a0, b0):
+    """getAdaptiveQuadPts2: qag (key 6, epsabs 0, epsrel 1e-3, limit 500) of
+    column jj (1-based) of f over [a0, b0], single precision."""
+    fcol = np.asarray(f, np.float64)[:, jj - 1].astype(np.float32)   # read as single precision
+    n = nph if ph else nth
+    a, b = _F(a0), _F(b0)
+    A0, B0 = a, b
+    limit, epsabs, epsrel = 500, _F(0), _F(0.001)
+    alist, blist, rlist, elist = [_F(0)] * limit, [_F(0)] * limit, [_F(0)] * limit, [_F(0)] * limit
+    iord = [0] * limit
+    ier = 0
+    alist[0], blist[0] = a, b
+    result, abserr, defabs, resabs = _qk61(fcol, n, A0, B0, a, b)
+    rlist[0], elist[0], iord[0] = result, abserr, 1
+    errbnd = max(epsabs, _F(epsrel * abs(result)))
+    if abserr <= _F(_F(_F(50) * _FEPS) * defabs) and errbnd < abserr:
+        ier = 2
+    if ier != 0 or (abserr <= errbnd and abserr != resabs) or abserr == 0:
+        return float(result)
+    errmax, maxerr, area, errsum, nrmax, iroff1, iroff2 = abserr, 1, result, abserr, 1, 0, 0
+    last = 1
+    for last in range(2, limit + 1):
+        a1 = alist[maxerr - 1]
+        b1 = _F(_F(0.5) * _F(alist[maxerr - 1] + blist[maxerr - 1]))
+        a2, b2 = b1, blist[maxerr - 1]
+        area1, error1, _, defab1 = _qk61(fcol, n, A0, B0, a1, b1)
+        area2, error2, _, defab2 = _qk61(fcol, n, A0, B0, a2, b2)
+        area12, erro12 = _F(area1 + area2), _F(error1 + error2)
+        errsum = _F(_F(errsum + erro12) - errmax)
+        area = _F(_F(area + area12) - rlist[maxerr - 1])
+        if defab1 != error1 and defab2 != error2:
+            if abs(_F(rlist[maxerr - 1] - area12)) <= _F(_F(1e-5) * abs(area12)) and _F(_F(0.99) * errmax) <= erro12:
+                iroff1 += 1
+            if 10 < last and errmax < erro12:
+                iroff2 += 1
+        rlist[maxerr - 1], rlist[last - 1] = area1, area2
+        errbnd = max(epsabs, _F(epsrel * abs(area)))
+        if errbnd < errsum:
+            if iroff1 >= 6 or iroff2 >= 20:
+                ier = 2
+            if last == limit:
+                ier = 1
+            if max(abs(a1), abs(b2)) <= _F(_F(_F(1) + _F(_F(_F(6) * _F(1e3)) * _FEPS)) *
+                                           _F(abs(a2) + _F(_F(1e4) * _FTINY))):
+                ier = 3
+        if error2 <= error1:
+            alist[last - 1], blist[maxerr - 1], blist[last - 1] = a2, b1, b2
+            elist[maxerr - 1], elist[last - 1] = error1, error2
+        else:
+            alist[maxerr - 1], alist[last - 1], blist[last - 1] = a2, a1, b1
+            rlist[maxerr - 1], rlist[last - 1] = area2, area1
+            elist[maxerr - 1], elist[last - 1] = error2, error1
+        maxerr, errmax, nrmax = _qpsrt(limit, last, maxerr, elist, iord, nrmax)
+        if ier != 0 or errsum <= errbnd:
+            break
+    s = _F(0)
+    for v in rlist[:last]:
+        s = _F(s + v)
+    return float(s)
+
+
+def trap2d(f, a, b, c, d):
+    """trap_2Dc (Numerical_Integration.f90:251-305)."""
+    f = np.asarray(f, np.float64)
+    nx, ny = f.shape
+    with np.errstate(divide="ignore", invalid="ignore"):
+        hx, hy = (b - a) / np.float64(nx - 1), (d - c) / np.float64(ny - 1)
+    s = f[0, 0] + f[0, ny - 1] + f[nx - 1, 0] + f[nx - 1, ny - 1]
+    for i in range(1, nx - 1):
+        s = s + 2.0 * (f[i, 0] + f[i, ny - 1])
+    for j in range(1, ny - 1):
+        s = s + 2.0 * (f[0, j] + f[nx - 1, j])
+    for i in range(1, nx - 1):
+        for j in range(1, ny - 1):
+            s = s + 4.0 * f[i, j]
+    with np.errstate(invalid="ignore"):
+        return 0.25 * hx * hy * s
+
+
+def gauss_legendre(n, a, b):
+    """Gauss-Legendre rule on [a, b], nodes descending (getGaussLegendreQuadPts3)."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    return ((a * (1 - x) + b * (1 + x)) / 2)[::-1], (w * (b - a) / 2)[::-1]
+
+
+INT_SD, INT_LB, INT_UN, INT_GL, INT_TR, INT_SM = range(6)
+
+
+def _unique_first(v):
+    out = []
+    for x in v:
+        if x not in out:
+            out.append(x)
+    return out
+
+
+def _txrx(th_all, ph_all, th, ph):
+    """getTxRxIndex (get_trans_Receiv.f90:690-713), 0-based, -1 if none."""
+    eps = float(np.float32(1e-2))
+    for i, (t, p) in enumerate(zip(th_all, ph_all)):
+        if abs(t - th) <= eps and abs(p - ph) <= eps:
+            return i
+    return -1
+
+
+def scattering_quantities_grid(S, nrx, rx_th, rx_ph, rx_type, tr_th, tr_ph, tr_type, k0, dm, cext, cabs,
+                               rx_w=None, cext_intfields=False, quirks=15):
+    """Compute_Scattering_Quantities_1 (Compute_Scattering_Quantities.f90:1-589):
+    per transmitter Q_ext (forward S, or internal field), Q_abs (internal
+    field), Q_sca and g (receiver integral by the rule rx_type), Q_bks
+    (backward S); averages over the transmitter grid by tr_type.  rx_th,
+    rx_ph hold all NRx_tot receivers (the forward / backward ones are looked
+    up there).  Returns (Q (ntr x 5), Q_av (5))."""
+    pi = float(np.float32(np.arccos(np.float32(-1)))) if quirks & 1 else np.pi
+    ap, X = dm / 2.0, k0 * dm / 2.0
+    S = np.asarray(S)
+    ntr = len(tr_th)
+    sq = lambda z: abs(z) ** 2   # noqa: E731
+    tth, tph = _unique_first(list(tr_th)), _unique_first(list(tr_ph))
+    ntt, ntp = len(tth), len(tph)
+    trth_min, trth_max = tth[0] * pi / 180., tth[-1] * pi / 180.
+    trph_min, trph_max = tph[0] * pi / 180., tph[-1] * pi / 180.
+    one_d = rx_type in (INT_SD, INT_LB)
+    if not one_d:
+        rth, rph = _unique_first(list(rx_th[:nrx])), _unique_first(list(rx_ph[:nrx]))
+        nrt, nrp = len(rth), len(rph)
+        rth_min, rth_max = rth[0] * pi / 180., rth[-1] * pi / 180.
+        rph_min, rph_max = rph[0] * pi / 180., rph[-1] * pi / 180.
+        sn_r = max(rph_max - rph_min, 1.0) * (math.cos(rth_min) - math.cos(rth_max))
+    Q = np.zeros((ntr, 5))
+    for t in range(ntr):
+        thi, phi = tr_th[t], tr_ph[t]
+        ki = np.array([-math.sin(thi * pi / 180.) * math.cos(phi * pi / 180.),
+                       -math.sin(thi * pi / 180.) * math.sin(phi * pi / 180.), -math.cos(thi * pi / 180.)])
+
+        def cg(ts, ps):
+            ks = (-math.sin(ts * pi / 180.) * math.cos(ps * pi / 180.), -math.sin(ts * pi / 180.) * math.sin(ps * pi / 180.),
+                  -math.cos(ts * pi / 180.))
+            return ki[0] * ks[0] + ki[1] * ks[1] + ki[2] * ks[2]
+
+        def s_it(r):
+            return ((sq(S[r, 4 * t + 3]) + sq(S[r, 4 * t])) + sq(S[r, 4 * t + 1])) + sq(S[r, 4 * t + 2])
+        ifw = _txrx(rx_th, rx_ph, thi, phi)
+        q_ext = (2 / (X * X)) * (abs(S[ifw, 4 * t + 3].imag) + abs(S[ifw, 4 * t].imag))
+        q_sca = g = 0.0
+        if one_d:
+            f = [(rx_w[r] if rx_type == INT_LB else 1.0) * s_it(r) for r in range(nrx)]
+            sf = sum(f)
+            sg = sum(fr * cg(rx_th[r], rx_ph[r]) for r, fr in enumerate(f))
+            if rx_type == INT_LB:
+                c_sca = 1. / (2 * k0 ** 2) * sf
+                q_sca = (4 * pi) / (2 * pi * X * X) * sf
+                g = 1. / (2. * k0 ** 2. * c_sca) * sg
+            else:
+                q_sd = (4 * pi / nrx) * sf
+                c_sca = 1. / (2 * k0 ** 2) * q_sd
+                q_sca = 1 / (2 * pi * X * X) * q_sd
+                g = 1. / (2. * k0 ** 2. * c_sca) * (4 * pi / nrx) * sg
+        else:
+            fxy, fxyg = np.zeros((nrt, nrp)), np.zeros((nrt, nrp))
+            for ii in range(nrt):
+                s = math.sin(rth[ii] * pi / 180.)
+                for jj in range(nrp):
+                    v = s_it(jj * nrt + ii)
+                    fxy[ii, jj] = v * s
+                    fxyg[ii, jj] = v * s * cg(rth[ii], rph[jj])
+            qi = qg = 0.0
+            if rx_type == INT_GL:
+                _, wt = gauss_legendre(nrt, rth_min, rth_max)
+                wp = gauss_legendre(nrp, rph_min, rph_max)[1] if nrp > 1 else np.ones(1)
+                wt, wp = wt[::-1], wp[::-1]
+                for ii in range(nrt):
+                    for jj in range(nrp):
+                        qi = qi + wp[jj] * wt[ii] * fxy[ii, jj]
+                for ii in range(nrt):
+                    for jj in range(nrp):
+                        qg = qg + wp[jj] * wt[ii] * fxyg[ii, jj]
+            elif rx_type == INT_UN:
+                fy = np.array([[adaptive_quad(fxy, nrt, nrp, False, jj, rth_min, rth_max)] for jj in range(1, nrp + 1)])
+                qi = adaptive_quad(fy, nrt, nrp, True, 1, rph_min, rph_max) if nrp > 1 else fy[0, 0]
+                fy = np.array([[adaptive_quad(fxyg, nrt, nrp, False, jj, rth_min, rth_max)] for jj in range(1, nrp + 1)])
+                qg = adaptive_quad(fy, nrt, nrp, True, 1, rph_min, rph_max) if nrp > 1 else fy[0, 0]
+            elif rx_type == INT_TR:
+                qi = trap2d(fxy, rth_min, rth_max, rph_min, rph_max)
+                qg = trap2d(fxyg, rth_min, rth_max, rph_min, rph_max)
+            if rx_type != INT_SM:
+                c_sca = 1. / (2 * k0 ** 2) * qi
+                q_sca = 2. / (sn_r * X * X) * qi
+                g = 1. / (2. * k0 ** 2. * c_sca) * qg
+        ibk = _txrx(rx_th, rx_ph, 180. - thi, np.fmod(phi + 180., 360.))
+        q_bks = (1. / (2. * pi * X * X)) * (((sq(S[ibk, 4 * t]) + sq(S[ibk, 4 * t + 3])) + sq(S[ibk, 4 * t + 1])) +
+                                          sq(S[ibk, 4 * t + 2]))
+        Q[t] = [(cext[t].real / (pi * ap * ap)) if cext_intfields else q_ext, cabs[t].real / (pi * ap * ap), q_sca,
+                q_bks, g]
+    if ntr == 1:
+        return Q, Q[0].copy()
+    sn_t = max(trph_max - trph_min, 1.0) * (math.cos(trth_min) - math.cos(trth_max))
+    G = np.zeros((5, ntt, ntp))
+    for ii in range(ntt):
+        s = math.sin(tr_th[ii] * pi / 180.)
+        for jj in range(ntp):
+            G[:, ii, jj] = Q[jj * ntt + ii] * s
+    integ = np.zeros(5)
+    if tr_type == INT_GL:
+        _, w1 = gauss_legendre(ntt, trth_min, trth_max)
+        w2 = gauss_legendre(ntp, trph_min, trph_max)[1] if ntp > 1 else np.ones(1)
+        w1, w2 = w1[::-1], w2[::-1]
+        for ii in range(ntt):
+            for jj in range(ntp):
+                integ = integ + w2[jj] * w1[ii] * G[:, ii, jj]
+    elif tr_type == INT_UN:
+        for j in range(5):
+            fy = np.array([[adaptive_quad(G[j], ntt, ntp, False, jj, trth_min, trth_max)] for jj in range(1, ntp + 1)])
+            integ[j] = adaptive_quad(fy, ntt, ntp, True, 1, trph_min, trph_max) if ntp > 1 else fy[0, 0]
+    elif tr_type == INT_TR:
+        for j in range(5):
+            integ[j] = trap2d(G[j], trth_min, trth_max, trph_min, trph_min)
+    return Q, 1. / sn_t * integ

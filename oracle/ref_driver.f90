@@ -355,4 +355,39 @@ contains
         ph(1:m) = t(1:m)%phi
     end subroutine ref_transmitters_un
 
+    ! getAdaptiveQuadPts2 (Numerical_Integration.f90:213-245): QUADPACK qag with the
+    ! reference's 61-point Gauss-Kronrod rule reading the integrand from a grid
+    ! (quadpack.f90); tf = 0 for 'th', 1 for 'ph'
+    subroutine ref_adaptive_quad(nth, nph, d1, d2, f, tf, jj, a0, b0, res) bind(c, name="ref_adaptive_quad")
+        integer(c_int), value :: nth, nph, d1, d2, tf, jj
+        real(c_double), intent(in) :: f(d1, d2)
+        real(c_double), value :: a0, b0
+        real(c_double), intent(out) :: res
+        character(len=2) :: t
+        external getAdaptiveQuadPts2
+        t = 'th'
+        if (tf /= 0) t = 'ph'
+        res = 0.0d0
+        call getAdaptiveQuadPts2(nth, nph, d1, d2, f, t, jj, a0, b0, res)
+    end subroutine ref_adaptive_quad
+
+    ! trap_2Dc (Numerical_Integration.f90:251-305)
+    subroutine ref_trap2d(nx, ny, f, a, b, c, d, res) bind(c, name="ref_trap2d")
+        integer(c_int), value :: nx, ny
+        real(c_double), intent(in) :: f(nx, ny)
+        real(c_double), value :: a, b, c, d
+        real(c_double), intent(out) :: res
+        external trap_2Dc
+        call trap_2Dc(f, a, b, c, d, res, nx, ny)
+    end subroutine ref_trap2d
+
+    ! getGaussLegendreQuadPts3 (Numerical_Integration.f90:88-145, LegendreRuleFast.f90)
+    subroutine ref_gauss_legendre(n, a, b, x, w) bind(c, name="ref_gauss_legendre")
+        integer(c_int), value :: n
+        real(c_double), value :: a, b
+        real(c_double), intent(out) :: x(n), w(n)
+        external getGaussLegendreQuadPts3
+        call getGaussLegendreQuadPts3(n, a, b, x, w)
+    end subroutine ref_gauss_legendre
+
 end module midas_ref_driver
