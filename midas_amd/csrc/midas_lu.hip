@@ -164,9 +164,11 @@ __global__ __launch_bounds__(STEP_THREADS) void k_getf2_step(typename cvec<T>::t
 // workgroup b holds row jb + 256 b + t of the block's columns in registers for
 // all IB column steps.  Per column j:
 //   local argmax of |Re|+|Im| over rows >= j  ->  publish the candidate
-//   row's IB values and, in workgroup 0, row j; wait until those stores are
-//   done, then publish (max, row) with the step tag (FP32: one 64-bit record
-//   tag | row | max; FP64: max and row, then a separate tag)
+//   row's IB values and, in workgroup 0, row j, and (max, row) with the step
+//   tag.  FP32: the rows as 64-bit words (step tag | float bits), so no store
+//   needs to complete before the record (tag | row | max, one 64-bit word)
+//   goes out; a reader checks the tag of every word it takes.  FP64: the
+//   rows, a wait for those stores, max and row, then a separate tag.
 //   wait until every workgroup's record of this step is there
 //   every workgroup reduces the candidates (first row on ties, as i?amax),
 //   takes the pivot row from the winner's published copy, applies the
@@ -661,7 +663,7 @@ __global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D
                                                     long sI2 = 0)
 {
     typedef typename cvec<T>::t V;
-    constexpr int TRINV_QC = R >= 8 ? 2 : (R >= 4 ? 4 : 8);
+    constexpr int TRINV_QC = R >= 8 ? (sizeof(T) == 4 ? 3 : 2) : (R >= 4 ? 4 : 8);
     D += blockIdx.y * sD + blockIdx.z * sD2;
     Inv += blockIdx.y * sI + blockIdx.z * sI2;
     const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
