@@ -8,11 +8,13 @@
 //  k_rhs         : Incident_Field_single_element (Compute_EFields_MoM.f90:529-598)
 //                  and Incident_Field (Incident_Field.f90:1-63, FP32 locals).
 //
-// One thread per cell pair writes the whole 3x3 block: the pair's distance,
+// One thread per cell pair computes the whole 3x3 block: the pair's distance,
 // e^{ikr} and dyad factors are computed once instead of nine times as in the
 // reference's per-entry routine.  A wavefront runs along receiver cells, so
-// each of the 3 columns a wave touches is written as one contiguous
-// 64 x 48-byte run; the source cell's constants are wave-uniform.
+// each of the 3 columns a wave touches is one contiguous run of 192 entries;
+// the wave stages it in LDS and writes it with three fully coalesced stores
+// (64 consecutive elements each).  The source cell's constants are
+// wave-uniform.
 #include "midas_common.h"
 #include "midas_elem.h"
 
@@ -54,10 +56,17 @@ __global__ __launch_bounds__(256) void k_fill(DevCells cells, CellSet R, CellSet
                                               double2 *__restrict__ Z64, long ld64, float2 *__restrict__ W32,
                                               long ld32)
 {
-    const int o = blockIdx.x * 64 + threadIdx.x;
-    const int sl = blockIdx.y * 4 + threadIdx.y;
-    if (o >= R.n || sl >= S.n) return;
-    const int go = R.cell(o), gs = S.cell(sl);
+    // per wave: one column's 192 entries, staged so that each store
+    // instruction writes 64 consecutive elements
+    __shared__ double2 st64[4][3 * 64];
+    __shared__ float2 st32[4][3 * 64];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int o = blockIdx.x * 64 + tx;
+    const int sl = blockIdx.y * 4 + ty;
+    if (sl >= S.n) return;   // wave-uniform
+    const bool valid = o < R.n;
+    const int rows = 3 * min(64, R.n - (int)blockIdx.x * 64);
+    const int go = R.cell(valid ? o : blockIdx.x * 64), gs = S.cell(sl);
     z64 e[9];
     if (cells.id[go] == cells.id[gs]) {
         z64 d = {1.0 - cells.znnpp[2 * gs], -cells.znnpp[2 * gs + 1]};
@@ -80,14 +89,29 @@ __global__ __launch_bounds__(256) void k_fill(DevCells cells, CellSet R, CellSet
     for (int ps = 0; ps < 3; ++ps) {
         const long col = 3L * sl + ps;
         if (Z64) {
-            double2 *dst = Z64 + col * ld64 + 3L * o;
+            if (valid) {
 #pragma unroll
-            for (int po = 0; po < 3; ++po) dst[po] = make_double2(e[3 * ps + po].r, e[3 * ps + po].i);
+                for (int po = 0; po < 3; ++po) st64[ty][3 * tx + po] = make_double2(e[3 * ps + po].r, e[3 * ps + po].i);
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            double2 *dst = Z64 + col * ld64 + 3L * blockIdx.x * 64;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (64 * k + tx < rows) dst[64 * k + tx] = st64[ty][64 * k + tx];
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         }
         if (W32) {
-            float2 *dst = W32 + col * ld32 + 3L * o;
+            if (valid) {
 #pragma unroll
-            for (int po = 0; po < 3; ++po) dst[po] = make_float2((float)e[3 * ps + po].r, (float)e[3 * ps + po].i);
+                for (int po = 0; po < 3; ++po)
+                    st32[ty][3 * tx + po] = make_float2((float)e[3 * ps + po].r, (float)e[3 * ps + po].i);
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            float2 *dst = W32 + col * ld32 + 3L * blockIdx.x * 64;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (64 * k + tx < rows) dst[64 * k + tx] = st32[ty][64 * k + tx];
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         }
     }
 }
