@@ -218,6 +218,62 @@ def run_c5(args):
     print(json.dumps(res))
 
 
+def run_a0006(args):
+    """The reference's own input deck examples/Run_Sims_a0006 (DDSCAT
+    shape.dat of 140,896 dipoles, a = 1.614 mm at 94 GHz, m = 1.7861 +
+    0.00133i; fixture tests/golden/deck_a0006.npz): cells from the lattice,
+    the deck's CSH division (NBlock (avgNcells) = 2000, 75 blocks) and one
+    extension layer, CBFM-E with ACA (eps 1e-4, kmax 50) and its 422 CBF
+    directions, 482 incidence directions (964 RHS), reduced system in mixed
+    precision + FP64 refinement.  The deck's sd t-designs are replaced by the
+    build's own quasi-uniform directions of the same counts."""
+    import numpy as np
+    from midas_amd import api, geometry as g, shape_io as sio
+    d = np.load(os.path.join(ROOT, "tests", "golden", "deck_a0006.npz"), allow_pickle=False)
+    x, y, z, sc, _ = sio.cells_from_shape(d["lattice"], float(d["a_eff_m"]))
+    eps = np.full(len(x), complex(d["m"]) ** 2)
+    k0 = 2 * np.pi * float(d["freq_hz"]) / 299792458.0
+    t0 = time.perf_counter()
+    s, _ = g.divide_blocks(g.Scatterer(x, y, z, sc, eps, k0, name="deck a0006"), "CSH", int(d["navg_cells"]))
+    t_div = time.perf_counter() - t0
+    th_c, ph_c = g.directions(int(d["n_iws"]))
+    tth, tph = g.directions(int(d["n_inc"]))
+    mode = MODES[args.mode]
+    m = api.Midas(0)
+    log(f"[bench] {s.name}: n = {3 * s.n}, {len(s.blocks)} blocks (division + extension {t_div:.2f} s), "
+        f"{len(th_c)} CBF directions, {2 * len(tth)} RHS")
+    run = lambda md: m.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True,
+                                             mode=md)
+    for _ in range(args.warmup):
+        run(mode)
+    ts = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        E, K, info = run(mode)
+        ts.append(time.perf_counter() - t0)
+        log(f"[bench] a0006 step {k + 1}: {ts[-1]:.3f} s, K = {int(np.sum(K))}, {info}")
+    step_s = float(np.mean(ts))
+    res = {"metric": "CBFM-E+ACA wall-s (reference deck a0006)", "value": round(step_s, 4), "unit": "s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": f"f32-quirk blocks, f64 LU/SVD/ACA, {MODE_DTYPE[args.mode]} reduced LU + f64 IR",
+           "data": "the reference's examples/Run_Sims_a0006 shape.dat (fixture)",
+           "config": {"workload": f"deck a0006, {s.n} cells, {3 * s.n} unknowns, CSH {len(s.blocks)} blocks, "
+                                  f"ACA 1e-4, {len(th_c)} CBF directions, {len(tth)} incidence directions",
+                      "solver": args.mode, "parallelism": "1 GPU"},
+           "division_s": round(t_div, 3), "K_total": int(np.sum(K)), "reduced_iterations": info["iterations"],
+           "reduced_factor_ms": round(info["t_factor_ms"], 1), "reduced_refine_ms": round(info["t_refine_ms"], 1)}
+    if not args.no_fp64:
+        E64, _, _ = run(api.SOLVE_FP64)
+        res["rel_field_err_vs_fp64_reduced"] = float(np.linalg.norm(E - E64) / np.linalg.norm(E64))
+    if not args.no_cpu:
+        log("[bench] CPU baseline sample (deck a0006)")
+        res["cpu_baseline"] = cbfm_cpu_baseline(s, th_c, ph_c, sr=False, nrhs=2 * len(tth))
+        res["speedup_vs_cpu"] = round(res["cpu_baseline"]["value"] / step_s, 1)
+    m.close()
+    print(json.dumps(res))
+
+
 def c4_cpu_baseline(s, nrx, nrhs, sample_cells=1800):
     """CPU restatement on a sample of the same particle: OpenMP C fill, scipy
     (OpenBLAS) LU and a solve with all right-hand sides, the far field for all
@@ -327,7 +383,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample-cells", type=int, default=3500)
     ap.add_argument("--nb", type=int, default=0, help="LU panel width (0: library default)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "a0006"],
                     help="c2: MoM fill + solve (default); c3: CBFM-E + ACA on a DLCA aggregate; "
                          "c4: 1,000-RHS solve + far field + orientation averages on a convex-hull particle; "
                          "c5: CBFM-E + ACA on a 150,000-cell porous particle")
@@ -339,13 +395,15 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.mode is None:
-        args.mode = "f16x3" if args.workload == "c5" else "f16"
+        args.mode = "f16x3" if args.workload in ("c5", "a0006") else "f16"
     if args.workload == "c3":
         return run_cbfm(args)
     if args.workload == "c4":
         return run_c4(args)
     if args.workload == "c5":
         return run_c5(args)
+    if args.workload == "a0006":
+        return run_a0006(args)
 
     import numpy as np
     import torch
