@@ -518,6 +518,9 @@ def main():
         "peak": HBM_GBS, "unit": "GB/s",
         "frac": round(fill_bytes / (fill_ms * 1e-3) / 1e9 / HBM_GBS, 4) if fill_ms else 0,
         "alg_bytes_per_launch": fill_bytes / max(fill_launches, 1),
+        # FP64 rate: 125 flops per cell pair (distance, e^{ikr}, the dyad's 9 entries)
+        "fp64_tflops": round(125.0 * s.n * s.n * fill_launches / (fill_ms * 1e-3) / 1e12, 2) if fill_ms else 0,
+        "fp64_peak_tflops": 78.6,
     }
 
     if not args.no_fp64:
