@@ -131,13 +131,19 @@ struct GJob {
 };
 
 constexpr int GT = 32, GK = 16;
+typedef double gd4 __attribute__((ext_vector_type(4)));
+// One 32 x 32 tile per workgroup on the FP64 matrix cores: wave w owns the
+// 16 x 16 quadrant (16 (w & 1), 16 (w >> 1)) and runs v_mfma_f64_16x16x4_f64
+// (lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16); it receives rows
+// l / 16 + 4 q of column l % 16).  The four real products of the complex
+// multiply-add accumulate in four independent chains, summed at the end.
 __global__ __launch_bounds__(256) void k_ggemm(const GJob *__restrict__ jobs)
 {
     const GJob jb = jobs[blockIdx.x];
     __shared__ double2 sA[GK][GT + 1], sB[GK][GT + 1];
-    const int t = threadIdx.x;
-    const int tx = t & 15, ty = t >> 4;   // outputs (i0 + tx + 16 p, j0 + ty + 16 q)
-    z64 acc[2][2] = {};
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = (w & 1) * 16, wn = (w >> 1) * 16;
+    gd4 rr = {0, 0, 0, 0}, ii = {0, 0, 0, 0}, ri = {0, 0, 0, 0}, ir = {0, 0, 0, 0};
     for (int k0 = 0; k0 < jb.k; k0 += GK) {
         // op(A) tile: GT rows x GK k
 #pragma unroll
@@ -162,24 +168,22 @@ __global__ __launch_bounds__(256) void k_ggemm(const GJob *__restrict__ jobs)
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < GK; ++kk)
-#pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const double2 x = sA[kk][tx + 16 * p], y = sB[kk][ty + 16 * q];
-                    acc[p][q].r += x.x * y.x - x.y * y.y;
-                    acc[p][q].i += x.x * y.y + x.y * y.x;
-                }
+        for (int kk = 0; kk < GK; kk += 4) {
+            const double2 x = sA[kk + (lane >> 4)][wm + (lane & 15)];
+            const double2 y = sB[kk + (lane >> 4)][wn + (lane & 15)];
+            rr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, rr, 0, 0, 0);
+            ii = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.y, ii, 0, 0, 0);
+            ri = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, ri, 0, 0, 0);
+            ir = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.x, ir, 0, 0, 0);
+        }
         __syncthreads();
     }
+    const int b = jb.j0 + wn + (lane & 15);
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int a = jb.i0 + tx + 16 * p, b = jb.j0 + ty + 16 * q;
-            if (a < jb.m && b < jb.n) jb.C[(long)b * jb.ldc + a] = make_double2(acc[p][q].r, acc[p][q].i);
-        }
+    for (int q = 0; q < 4; ++q) {
+        const int a = jb.i0 + wm + (lane >> 4) + 4 * q;
+        if (a < jb.m && b < jb.n) jb.C[(long)b * jb.ldc + a] = make_double2(rr[q] - ii[q], ri[q] + ir[q]);
+    }
 }
 
 // All tiles of C = op(A) B (m x n) appended to a job list.
@@ -536,13 +540,22 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
         TRY(dinfo.alloc(sizeof(int) * nb));
         TRY(Wk.alloc(sizeof(double2) * (size_t)T2c * T2c));
         MIDAS_HIP(hipMemsetAsync(G.p, 0, G.bytes, st));
-        for (int b = 0; b < nb; ++b) {
-            const int m = 3 * sb[b];
-            double2 *Gb = G.as<double2>() + b * gs;
-            if (T2c <= m)
-                TRY(launch_cgemm<double>(T2c, T2c, m, ONE, Xb(b), mp, 2, Xb(b), mp, 0, ZERO, Gb, rpad, st));
-            else
-                TRY(launch_cgemm<double>(m, m, T2c, ONE, Xb(b), mp, 0, Xb(b), mp, 2, ZERO, Gb, rpad, st));
+        bool all_tall = true;
+        for (int b = 0; b < nb; ++b) all_tall = all_tall && T2c <= 3 * sb[b];
+        if (all_tall) {
+            // X_b^H X_b of every block in one batched launch: the rows of X_b
+            // past 3 s_b are zero, so the sum over all mp rows adds exact zeros
+            TRY(launch_cgemm<double>(T2c, T2c, (int)mp, ONE, Xb(0), mp, 2, Xb(0), mp, 0, ZERO, G.p, rpad, st, nb,
+                                     mp * T2c, mp * T2c, gs));
+        } else {
+            for (int b = 0; b < nb; ++b) {
+                const int m = 3 * sb[b];
+                double2 *Gb = G.as<double2>() + b * gs;
+                if (T2c <= m)
+                    TRY(launch_cgemm<double>(T2c, T2c, m, ONE, Xb(b), mp, 2, Xb(b), mp, 0, ZERO, Gb, rpad, st));
+                else
+                    TRY(launch_cgemm<double>(m, m, T2c, ONE, Xb(b), mp, 0, Xb(b), mp, 2, ZERO, Gb, rpad, st));
+            }
         }
         MIDAS_ROC(rocsolver_zheevd_strided_batched(rh.h, rocblas_evect_original, rocblas_fill_upper, rpad,
                                                    (rocblas_double_complex *)G.p, rpad, gs, D.as<double>(), rpad,
@@ -793,10 +806,13 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
     TRY(Al.alloc(sizeof(double2) * (size_t)Kt * T2));
     TRY(launch_rhs(dc.v, ALL, ntr, dtt.as<double>(), dtp.as<double>(), k0, quirks, ref32, Einc.as<double2>(), N3,
                    st));
+    // reduced excitation V_i = C_i^T E_i of every block as one grouped launch
+    jobs.clear();
     for (int i = 0; i < nb; ++i)
         if (K[i])
-            TRY(launch_cgemm<double>(K[i], T2, 3 * nbk[i], ONE, Cb(i), 3L * nbk[i], 1, Einc.as<double2>() + 3L * bs[i],
-                                     N3, 0, ZERO, Vc.as<double2>() + offs[i], Kt, st));
+            add_gemm_tiles(jobs, Cb(i), 3 * nbk[i], 1, Einc.as<double2>() + 3L * bs[i], (int)N3,
+                           Vc.as<double2>() + offs[i], (int)Kt, K[i], T2, 3 * nbk[i]);
+    TRY(run_ggemm(jobs, djobs, st));
     midas_solve_info_t inf{};
     int r = solve_dev(ctx, (int)Kt, T2, zc_, Kt, nullptr, 0, Vc.as<double2>(), Kt, Al.as<double2>(), Kt, mode, nullptr,
                       &inf, true);
@@ -806,10 +822,13 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
     Vc.release();
     TRY(dE.alloc(sizeof(double2) * N3 * T2));
     MIDAS_HIP(hipMemsetAsync(dE.p, 0, dE.bytes, st));
+    // E_i = C_i alpha_i of every block as one grouped launch
+    jobs.clear();
     for (int i = 0; i < nb; ++i)
         if (K[i])
-            TRY(launch_cgemm<double>(3 * nbk[i], T2, K[i], ONE, Cb(i), 3L * nbk[i], 0, Al.as<double2>() + offs[i], Kt,
-                                     0, ZERO, dE.as<double2>() + 3L * bs[i], N3, st));
+            add_gemm_tiles(jobs, Cb(i), 3 * nbk[i], 0, Al.as<double2>() + offs[i], (int)Kt,
+                           dE.as<double2>() + 3L * bs[i], (int)N3, 3 * nbk[i], T2, K[i]);
+    TRY(run_ggemm(jobs, djobs, st));
     MIDAS_HIP(hipMemcpy2DAsync(E, sizeof(double2) * lde, dE.p, sizeof(double2) * N3, sizeof(double2) * N3, T2,
                                hipMemcpyDeviceToHost, st));
     if (C_job) {
