@@ -147,7 +147,7 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
 
     __shared__ int Ir[ACA_KMAX + 1], Jc[ACA_KMAX + 1];
     __shared__ double2 coef[ACA_KMAX];
-    __shared__ double2 dpart[ACA_KMAX][ACA_W], epart[ACA_KMAX][ACA_W];
+    __shared__ double2 dpart[ACA_KMAX], epart[ACA_KMAX];
     __shared__ double sred[4][ACA_W];
     __shared__ int ired[2][ACA_W];
     __shared__ double s_cross;
@@ -249,40 +249,52 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
             ired[1][w] = ri;
         }
         __syncthreads();
-        // ---- (4) cross terms sum_l |u_l . u_k| |v_k . v_l| (non-conjugated)
-        for (int l = 0; l < kk; ++l) {
-            z64 du = {0.0, 0.0}, dv = {0.0, 0.0};
-            for (int i = t; i < m; i += ACA_T) {
-                double2 a = u[(long)l * m + i], b = u[(long)kk * m + i];
-                du.r += a.x * b.x - a.y * b.y;
-                du.i += a.x * b.y + a.y * b.x;
+        // ---- (4) cross terms sum_l |u_l . u_k| |v_k . v_l| (non-conjugated).
+        // Wave w owns l = w, w + ACA_W, ...; it takes two of them per pass so
+        // that their eight wave reductions run interleaved.
+        for (int l0 = w; l0 < kk; l0 += 2 * ACA_W) {
+            const int l1 = l0 + ACA_W;
+            const bool two = l1 < kk;   // uniform across the wave
+            double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // du0, dv0, du1, dv1 (re, im)
+            for (int i = lane; i < m; i += 64) {
+                const double2 b = u[(long)kk * m + i], a = u[(long)l0 * m + i];
+                x[0] += a.x * b.x - a.y * b.y;
+                x[1] += a.x * b.y + a.y * b.x;
+                if (two) {
+                    const double2 a1 = u[(long)l1 * m + i];
+                    x[4] += a1.x * b.x - a1.y * b.y;
+                    x[5] += a1.x * b.y + a1.y * b.x;
+                }
             }
-            for (int j = t; j < n; j += ACA_T) {
-                double2 a = v[(long)kk * n + j], b = v[(long)l * n + j];
-                dv.r += a.x * b.x - a.y * b.y;
-                dv.i += a.x * b.y + a.y * b.x;
+            for (int j = lane; j < n; j += 64) {
+                const double2 a = v[(long)kk * n + j], b = v[(long)l0 * n + j];
+                x[2] += a.x * b.x - a.y * b.y;
+                x[3] += a.x * b.y + a.y * b.x;
+                if (two) {
+                    const double2 b1 = v[(long)l1 * n + j];
+                    x[6] += a.x * b1.x - a.y * b1.y;
+                    x[7] += a.x * b1.y + a.y * b1.x;
+                }
             }
-            du.r = wave_sum(du.r);
-            du.i = wave_sum(du.i);
-            dv.r = wave_sum(dv.r);
-            dv.i = wave_sum(dv.i);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) x[q] += __shfl_xor(x[q], o, 64);
             if (lane == 0) {
-                dpart[l][w] = make_double2(du.r, du.i);
-                epart[l][w] = make_double2(dv.r, dv.i);
+                dpart[l0] = make_double2(x[0], x[1]);
+                epart[l0] = make_double2(x[2], x[3]);
+                if (two) {
+                    dpart[l1] = make_double2(x[4], x[5]);
+                    epart[l1] = make_double2(x[6], x[7]);
+                }
             }
         }
         __syncthreads();
         if (w == 0) {
             double term = 0.0;
             if (lane < kk) {
-                z64 du = {0.0, 0.0}, dv = {0.0, 0.0};
-                for (int q = 0; q < ACA_W; ++q) {
-                    du.r += dpart[lane][q].x;
-                    du.i += dpart[lane][q].y;
-                    dv.r += epart[lane][q].x;
-                    dv.i += epart[lane][q].y;
-                }
-                term = zabs(du) * zabs(dv);
+                const double2 du = dpart[lane], dv = epart[lane];
+                term = zabs(z64{du.x, du.y}) * zabs(z64{dv.x, dv.y});
             }
             term = wave_sum(term);
             if (lane == 0) s_cross = term;
