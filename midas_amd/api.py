@@ -37,7 +37,7 @@ EXPORTS = (
     "midas_init", "midas_finalize", "midas_last_error", "midas_stream", "midas_cell_params", "midas_mom_fill",
     "midas_block_fill", "midas_plane_wave", "midas_spherical_wave", "midas_solve", "midas_scattering_matrix", "midas_cext_cabs",
     "midas_compute_efields_mom", "midas_aca_block", "midas_aca_matrix", "midas_compute_efields_cbfm",
-    "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_profile", "midas_profile_query", "midas_block_frames",
+    "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_dev_compute_efields_cbfm", "midas_profile", "midas_profile_query", "midas_block_frames",
     "midas_extend_blocks", "midas_scattering_quantities", "midas_near_field", "midas_partition_sph",
     "midas_partition_csh", "midas_scattering_quantities_grid", "midas_quad_adaptive", "midas_quad_trap2d",
     "midas_quad_gauss_legendre",
@@ -454,7 +454,10 @@ class Midas:
     # -- Compute_EFields_CBFME(_ACA) -----------------------------------------
     def compute_efields_cbfme(self, cells, k0, blocks, ext, cbf_dirs, transmitters, use_aca=False, kmax=50,
                               eps_aca=1e-4, threshold=0.0, mode=SOLVE_FP64, quirks=Q_ALL,
-                              return_cbfs=False, sr=False, res_sr=0.01, iws=1):
+                              return_cbfs=False, sr=False, res_sr=0.01, iws=1, e_dev=None):
+        """e_dev: optional device address of a (3n x 2T) complex128 column-major
+        buffer (ld 3n); the fields are written there and stay on the GPU
+        (midas_dev_compute_efields_cbfm), and E is returned as None."""
         cells = _cells(cells)
         nb = len(blocks)
         starts = np.zeros(nb + 1, np.int32)
@@ -475,7 +478,7 @@ class Midas:
         bt = blocks_t(nb, _p(starts), _p(es), _p(ec))
         cth, cph = (np.ascontiguousarray(a, np.float64) for a in cbf_dirs)
         tth, tph = (np.ascontiguousarray(a, np.float64) for a in transmitters)
-        E = np.empty((3 * cells.n, 2 * len(tth)), np.complex128, order="F")
+        E = None if e_dev is not None else np.empty((3 * cells.n, 2 * len(tth)), np.complex128, order="F")
         K = np.zeros(nb, np.int32)
         kcap = 2 * len(cth)
         Cj = np.zeros((3 * cells.n, kcap), np.complex128, order="F") if return_cbfs else None
@@ -483,12 +486,14 @@ class Midas:
         nnz = np.zeros(nb, np.int64)
         info = solve_info_t()
         cs = cells.struct()
-        r = _check(self.lib.midas_compute_efields_cbfm(self.ctx, C.byref(cs), C.byref(bt), C.c_double(k0),
+        entry = self.lib.midas_compute_efields_cbfm if e_dev is None else self.lib.midas_dev_compute_efields_cbfm
+        r = _check(entry(self.ctx, C.byref(cs), C.byref(bt), C.c_double(k0),
                                                        C.c_int(quirks), C.c_int(len(cth)), _p(cth), _p(cph),
                                                        C.c_int(iws), C.c_int(len(tth)), _p(tth), _p(tph),
                                                        C.c_int(int(use_aca)),
                                                        C.c_int(kmax), C.c_double(eps_aca), C.c_double(threshold),
-                                                       C.c_int(mode), _p(E), C.c_long(3 * cells.n), _p(K),
+                                                       C.c_int(mode), _p(E) if E is not None else C.c_void_p(e_dev),
+                                                       C.c_long(3 * cells.n), _p(K),
                                                        None if Cj is None else _p(Cj), C.c_long(3 * cells.n),
                                                        C.c_int(kcap), C.c_int(int(sr)), C.c_double(res_sr),
                                                        _p(fsr), _p(nnz), C.byref(info)),

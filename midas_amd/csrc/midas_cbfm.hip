@@ -321,14 +321,13 @@ __global__ void k_sr_mask(double2 *Z, long ld, int m, DevCells cells, CellSet S,
 
 }  // namespace midas
 
-extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks,
-                                          double k0, int quirks, int ntr_cbf, const double *th_cbf,
-                                          const double *ph_cbf, int iws, int ntr, const double *tr_th,
-                                          const double *tr_ph,
-                                          int use_aca, int kmax, double eps_aca, double threshold, int mode,
-                                          double *E, long lde, int *K_out, double *C_job, long ldc, int kcap,
-                                          int sr, double res_sr, double *fsr_out, long *nnz_out,
-                                          midas_solve_info_t *info)
+// E_dev: E is device memory (the fields stay in HBM) rather than host memory.
+static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks, double k0,
+                                int quirks, int ntr_cbf, const double *th_cbf, const double *ph_cbf, int iws, int ntr,
+                                const double *tr_th, const double *tr_ph, int use_aca, int kmax, double eps_aca,
+                                double threshold, int mode, double *E, long lde, bool E_dev, int *K_out,
+                                double *C_job, long ldc, int kcap, int sr, double res_sr, double *fsr_out,
+                                long *nnz_out, midas_solve_info_t *info)
 {
     MIDAS_CHECK(ctx && cells && blocks && E && th_cbf && ph_cbf && tr_th && tr_ph, -1, "cbfm: bad arguments");
     const int N = cells->n, nb = blocks->nblocks;
@@ -830,7 +829,7 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
                            dE.as<double2>() + 3L * bs[i], (int)N3, 3 * nbk[i], T2, K[i]);
     TRY(run_ggemm(jobs, djobs, st));
     MIDAS_HIP(hipMemcpy2DAsync(E, sizeof(double2) * lde, dE.p, sizeof(double2) * N3, sizeof(double2) * N3, T2,
-                               hipMemcpyDeviceToHost, st));
+                               E_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     if (C_job) {
         MIDAS_CHECK(kbmax <= kcap, -1, "cbfm: a block has %d CBFs, more than kcap = %d", kbmax, kcap);
         std::vector<double2> zero((size_t)ldc, make_double2(0.0, 0.0));
@@ -848,4 +847,30 @@ extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *c
         for (int b = 0; b < nb; ++b) K_out[b] = K[b];
     if (info) *info = inf;
     return inf.info > 0 ? inf.info : worst_info;
+}
+
+extern "C" int midas_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks,
+                                          double k0, int quirks, int ntr_cbf, const double *th_cbf,
+                                          const double *ph_cbf, int iws, int ntr, const double *tr_th,
+                                          const double *tr_ph, int use_aca, int kmax, double eps_aca,
+                                          double threshold, int mode, double *E, long lde, int *K_out,
+                                          double *C_job, long ldc, int kcap, int sr, double res_sr,
+                                          double *fsr_out, long *nnz_out, midas_solve_info_t *info)
+{
+    return compute_efields_cbfm(ctx, cells, blocks, k0, quirks, ntr_cbf, th_cbf, ph_cbf, iws, ntr, tr_th, tr_ph,
+                                use_aca, kmax, eps_aca, threshold, mode, E, lde, false, K_out, C_job, ldc, kcap, sr,
+                                res_sr, fsr_out, nnz_out, info);
+}
+
+extern "C" int midas_dev_compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells,
+                                              const midas_blocks_t *blocks, double k0, int quirks, int ntr_cbf,
+                                              const double *th_cbf, const double *ph_cbf, int iws, int ntr,
+                                              const double *tr_th, const double *tr_ph, int use_aca, int kmax,
+                                              double eps_aca, double threshold, int mode, double *E_dev, long lde,
+                                              int *K_out, double *C_job, long ldc, int kcap, int sr, double res_sr,
+                                              double *fsr_out, long *nnz_out, midas_solve_info_t *info)
+{
+    return compute_efields_cbfm(ctx, cells, blocks, k0, quirks, ntr_cbf, th_cbf, ph_cbf, iws, ntr, tr_th, tr_ph,
+                                use_aca, kmax, eps_aca, threshold, mode, E_dev, lde, true, K_out, C_job, ldc, kcap,
+                                sr, res_sr, fsr_out, nnz_out, info);
 }

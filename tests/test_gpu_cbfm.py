@@ -155,6 +155,34 @@ def test_cbfm_mixed_reduced_solve(mid, oracle, dlca):
     assert _rel(E16, E64) < 1e-10
 
 
+def test_cbfm_fields_on_device(mid, dlca):
+    """midas_dev_compute_efields_cbfm leaves the fields in HBM: the same values,
+    bit for bit, as the host-output entry point (ACA path, two plane waves)."""
+    import ctypes as C
+    s = dlca
+    _case(f"cbfm device-resident fields n={3 * s.n}")
+    th_c, ph_c = _cbf_dirs(30)
+    tth, tph = np.array([0.0, 40.0]), np.array([0.0, 120.0])
+    E, K, _ = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True)
+    # device memory from the HIP runtime the library itself runs on
+    hip = C.CDLL("libamdhip64.so.7")
+    nbytes = C.c_size_t(E.size * 16)
+    p = C.c_void_p()
+    assert hip.hipMalloc(C.byref(p), nbytes) == 0
+    try:
+        assert hip.hipMemset(p, C.c_int(0xFF), nbytes) == 0   # NaN everywhere until written
+        assert hip.hipDeviceSynchronize() == 0
+        none, Kd, _ = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True,
+                                                e_dev=p.value)
+        out = np.empty((E.shape[1], E.shape[0]), np.complex128)
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), p, nbytes, C.c_int(2)) == 0   # device to host
+    finally:
+        hip.hipFree(p)
+    assert none is None
+    assert list(Kd) == list(K)
+    np.testing.assert_array_equal(out.T, E)
+
+
 def test_cbfm_without_truncation_is_mom(mid, oracle):
     s = g.dlca_aggregate(nmono=3, cells_per_mono=12, seed=4)
     _case(f"cbfm no truncation n={3 * s.n}")
