@@ -64,9 +64,9 @@ template <bool REF32> struct GenCells {
     {
         z64 e[9];
         dyad(j.r0 + i / 3, j.c0 + g, e);
-        const int po = i % 3;
+        const int po = i % 3;   // picked by selects: a runtime index would put e[] in scratch
 #pragma unroll
-        for (int ps = 0; ps < 3; ++ps) out[ps] = e[3 * ps + po];
+        for (int ps = 0; ps < 3; ++ps) out[ps] = po == 0 ? e[3 * ps] : po == 1 ? e[3 * ps + 1] : e[3 * ps + 2];
     }
     // column jc of the job, entries 3g..3g+2 (receiver cell r0 + g)
     __device__ void col3(const AcaJob &j, int jc, int g, z64 out[3]) const
@@ -75,7 +75,7 @@ template <bool REF32> struct GenCells {
         dyad(j.r0 + g, j.c0 + jc / 3, e);
         const int ps = jc % 3;
 #pragma unroll
-        for (int po = 0; po < 3; ++po) out[po] = e[3 * ps + po];
+        for (int po = 0; po < 3; ++po) out[po] = ps == 0 ? e[po] : ps == 1 ? e[3 + po] : e[6 + po];
     }
 };
 
@@ -134,6 +134,40 @@ __device__ inline z64 zdiv(z64 a, z64 b)
     return {(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
 }
 
+// acc[q] = sum_l coef[l] * x[l * ld + q] for q < nq (nq <= 3 entries at
+// unit stride): one pass over l with the three sums as independent chains,
+// each summed in ascending l.
+__device__ inline int dot3(const double2 *coef, const double2 *x, int ld, int kk, int nq, z64 acc[3])
+{
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc[q] = {0.0, 0.0};
+    if (nq == 3) {
+        for (int l = 0; l < kk; ++l) {
+            const double2 c = coef[l];
+            const double2 *row = x + (long)l * ld;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double2 y = row[q];
+                acc[q].r += c.x * y.x - c.y * y.y;
+                acc[q].i += c.x * y.y + c.y * y.x;
+            }
+        }
+    } else {
+        for (int l = 0; l < kk; ++l) {
+            const double2 c = coef[l];
+            const double2 *row = x + (long)l * ld;
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                if (q < nq) {
+                    const double2 y = row[q];
+                    acc[q].r += c.x * y.x - c.y * y.y;
+                    acc[q].i += c.x * y.y + c.y * y.x;
+                }
+        }
+    }
+    return nq;
+}
+
 template <class G>
 __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__ jobs, double2 *__restrict__ U,
                                                double2 *__restrict__ Vt, int kmax, double eps, int *__restrict__ kout)
@@ -170,22 +204,18 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
         for (int g = t; g < gn; g += ACA_T) {
             z64 e[3];
             gen.row3(jb, ik, g, e);
+            z64 acc[3];
+            const int nq = dot3(coef, v + 3 * g, n, kk, min(3, n - 3 * g), acc);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const int j = 3 * g + q;
-                if (j >= n) break;
-                z64 acc = {0.0, 0.0};
-                for (int l = 0; l < kk; ++l) {
-                    double2 c = coef[l], y = v[(long)l * n + j];
-                    acc.r += c.x * y.x - c.y * y.y;
-                    acc.i += c.x * y.y + c.y * y.x;
+            for (int q = 0; q < 3; ++q)
+                if (q < nq) {
+                    const int j = 3 * g + q;
+                    z64 r = {e[q].r - acc[q].r, e[q].i - acc[q].i};
+                    v[(long)kk * n + j] = make_double2(r.r, r.i);
+                    bool used = false;
+                    for (int l = 0; l < kk; ++l) used |= (Jc[l] == j);
+                    amax_merge(bv, bi, used ? 0.0 : zabs(r), j);
                 }
-                z64 r = {e[q].r - acc.r, e[q].i - acc.i};
-                v[(long)kk * n + j] = make_double2(r.r, r.i);
-                bool used = false;
-                for (int l = 0; l < kk; ++l) used |= (Jc[l] == j);
-                amax_merge(bv, bi, used ? 0.0 : zabs(r), j);
-            }
         }
         wave_amax(bv, bi);
         if (lane == 0) {
@@ -221,17 +251,13 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
         for (int g = t; g < gm; g += ACA_T) {
             z64 e[3];
             gen.col3(jb, jk, g, e);
+            z64 acc[3];
+            const int nq = dot3(coef, u + 3 * g, m, kk, min(3, m - 3 * g), acc);
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 const int i = 3 * g + q;
-                if (i >= m) break;
-                z64 acc = {0.0, 0.0};
-                for (int l = 0; l < kk; ++l) {
-                    double2 c = coef[l], y = u[(long)l * m + i];
-                    acc.r += c.x * y.x - c.y * y.y;
-                    acc.i += c.x * y.y + c.y * y.x;
-                }
-                z64 r = {e[q].r - acc.r, e[q].i - acc.i};
+                if (q >= nq) continue;
+                z64 r = {e[q].r - acc[q].r, e[q].i - acc[q].i};
                 u[(long)kk * m + i] = make_double2(r.r, r.i);
                 nu2 += r.r * r.r + r.i * r.i;
                 bool used = false;
