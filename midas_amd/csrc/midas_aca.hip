@@ -136,14 +136,19 @@ __device__ inline z64 zdiv(z64 a, z64 b)
 
 // acc[q] = sum_l coef[l] * x[l * ld + q] for q < nq (nq <= 3 entries at
 // unit stride): one pass over l with the three sums as independent chains,
-// each summed in ascending l.
-__device__ inline int dot3(const double2 *coef, const double2 *x, int ld, int kk, int nq, z64 acc[3])
+// each summed in ascending l.  The same pass marks bit q of the result when
+// entry j0 + q is among the previous pivots piv[0 .. kk-1].
+__device__ inline int dot3(const double2 *coef, const int *piv, const double2 *x, int ld, int kk, int j0, int nq,
+                           z64 acc[3])
 {
+    int used = 0;
 #pragma unroll
     for (int q = 0; q < 3; ++q) acc[q] = {0.0, 0.0};
     if (nq == 3) {
         for (int l = 0; l < kk; ++l) {
             const double2 c = coef[l];
+            const int d = piv[l] - j0;
+            used |= (unsigned)d < 3u ? 1 << d : 0;
             const double2 *row = x + (long)l * ld;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
@@ -155,6 +160,8 @@ __device__ inline int dot3(const double2 *coef, const double2 *x, int ld, int kk
     } else {
         for (int l = 0; l < kk; ++l) {
             const double2 c = coef[l];
+            const int d = piv[l] - j0;
+            used |= (unsigned)d < 3u ? 1 << d : 0;
             const double2 *row = x + (long)l * ld;
 #pragma unroll
             for (int q = 0; q < 3; ++q)
@@ -165,7 +172,7 @@ __device__ inline int dot3(const double2 *coef, const double2 *x, int ld, int kk
                 }
         }
     }
-    return nq;
+    return used;
 }
 
 template <class G>
@@ -205,16 +212,15 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
             z64 e[3];
             gen.row3(jb, ik, g, e);
             z64 acc[3];
-            const int nq = dot3(coef, v + 3 * g, n, kk, min(3, n - 3 * g), acc);
+            const int nq = min(3, n - 3 * g);
+            const int used = dot3(coef, Jc, v + 3 * g, n, kk, 3 * g, nq, acc);
 #pragma unroll
             for (int q = 0; q < 3; ++q)
                 if (q < nq) {
                     const int j = 3 * g + q;
                     z64 r = {e[q].r - acc[q].r, e[q].i - acc[q].i};
                     v[(long)kk * n + j] = make_double2(r.r, r.i);
-                    bool used = false;
-                    for (int l = 0; l < kk; ++l) used |= (Jc[l] == j);
-                    amax_merge(bv, bi, used ? 0.0 : zabs(r), j);
+                    amax_merge(bv, bi, (used >> q) & 1 ? 0.0 : zabs(r), j);
                 }
         }
         wave_amax(bv, bi);
@@ -252,7 +258,10 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
             z64 e[3];
             gen.col3(jb, jk, g, e);
             z64 acc[3];
-            const int nq = dot3(coef, u + 3 * g, m, kk, min(3, m - 3 * g), acc);
+            const int nq = min(3, m - 3 * g);
+            int used = dot3(coef, Ir, u + 3 * g, m, kk, 3 * g, nq, acc);
+            const int dk = Ir[kk] - 3 * g;   // the current pivot row counts as used too
+            used |= (unsigned)dk < 3u ? 1 << dk : 0;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 const int i = 3 * g + q;
@@ -260,9 +269,7 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
                 z64 r = {e[q].r - acc[q].r, e[q].i - acc[q].i};
                 u[(long)kk * m + i] = make_double2(r.r, r.i);
                 nu2 += r.r * r.r + r.i * r.i;
-                bool used = false;
-                for (int l = 0; l <= kk; ++l) used |= (Ir[l] == i);
-                amax_merge(rv, ri, used ? 0.0 : zabs(r), i);
+                amax_merge(rv, ri, (used >> q) & 1 ? 0.0 : zabs(r), i);
             }
         }
         nu2 = wave_sum(nu2);
