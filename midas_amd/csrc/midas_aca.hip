@@ -144,7 +144,10 @@ __device__ inline int dot3(const double2 *coef, const int *piv, const double2 *x
     int used = 0;
 #pragma unroll
     for (int q = 0; q < 3; ++q) acc[q] = {0.0, 0.0};
+    // unrolled so that the loads of several l are in flight at once (the sums
+    // still run in ascending l)
     if (nq == 3) {
+#pragma unroll 2
         for (int l = 0; l < kk; ++l) {
             const double2 c = coef[l];
             const int d = piv[l] - j0;
@@ -158,6 +161,7 @@ __device__ inline int dot3(const double2 *coef, const int *piv, const double2 *x
             }
         }
     } else {
+#pragma unroll 2
         for (int l = 0; l < kk; ++l) {
             const double2 c = coef[l];
             const int d = piv[l] - j0;
@@ -240,16 +244,19 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
         __syncthreads();
         // ---- (2) scale v(:, kk) by the pivot; ||v_k||^2
         double nv2 = 0.0;
-        for (int g = t; g < gn; g += ACA_T)
+        for (int g = t; g < gn; g += ACA_T) {
+            const int nq = min(3, n - 3 * g);
+            double2 x[3];   // the three loads issued before the divisions
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const int j = 3 * g + q;
-                if (j >= n) break;
-                double2 x = v[(long)kk * n + j];
-                z64 y = zdiv(z64{x.x, x.y}, piv);
-                v[(long)kk * n + j] = make_double2(y.r, y.i);
-                nv2 += y.r * y.r + y.i * y.i;
-            }
+            for (int q = 0; q < 3; ++q) x[q] = q < nq ? v[(long)kk * n + 3 * g + q] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                if (q < nq) {
+                    z64 y = zdiv(z64{x[q].x, x[q].y}, piv);
+                    v[(long)kk * n + 3 * g + q] = make_double2(y.r, y.i);
+                    nv2 += y.r * y.r + y.i * y.i;
+                }
+        }
         // ---- (3) residual column jk into u(:, kk); ||u_k||^2; arg max over unused rows
         double nu2 = 0.0;
         double rv = -1.0;
@@ -289,6 +296,7 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
             const int l1 = l0 + ACA_W;
             const bool two = l1 < kk;   // uniform across the wave
             double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // du0, dv0, du1, dv1 (re, im)
+#pragma unroll 2
             for (int i = lane; i < m; i += 64) {
                 const double2 b = u[(long)kk * m + i], a = u[(long)l0 * m + i];
                 x[0] += a.x * b.x - a.y * b.y;
@@ -299,6 +307,7 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
                     x[5] += a1.x * b.y + a1.y * b.x;
                 }
             }
+#pragma unroll 2
             for (int j = lane; j < n; j += 64) {
                 const double2 a = v[(long)kk * n + j], b = v[(long)l0 * n + j];
                 x[2] += a.x * b.x - a.y * b.y;
