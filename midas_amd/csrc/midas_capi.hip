@@ -215,7 +215,11 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
                 r = launch_residual_few(Z, ldz, n, X, ldx, nrhs, B, ldb, R.as<double2>(), n, zx.p, st);
             } else {
                 hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R.as<double2>(), (long)n, n, nrhs);
-                r = launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R.p, n, st);
+                // many right-hand sides: rocBLAS (33k x 448 x 33k: 60 ms against 74 ms for k_cgemm<double>)
+                r = nrhs >= 32 ? blas_zgemm_nn(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R.p, n,
+                                               0, 1, st)
+                               : launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0},
+                                                      R.p, n, st);
             }
             if (r) { w.release(); return r; }
             hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, X, ldx, n, colx);

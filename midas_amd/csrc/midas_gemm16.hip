@@ -622,6 +622,25 @@ static rocblas_handle blas_handle()
     return h[d];
 }
 
+int blas_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, long sA, const void *B, long ldb, long sB,
+                  z64 beta, void *C, long ldc, long sC, int batch, hipStream_t st)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || batch <= 0) return 0;
+    rocblas_handle h = blas_handle();
+    MIDAS_CHECK(h, -1, "zgemm: no rocBLAS handle for this device");
+    rocblas_status s = rocblas_set_stream(h, st);
+    MIDAS_CHECK(s == rocblas_status_success, -1, "zgemm: rocblas_set_stream status %d", (int)s);
+    const rocblas_double_complex al(alpha.r, alpha.i), be(beta.r, beta.i);
+    const rocblas_operation nt = rocblas_operation_none;
+    const auto *a = (const rocblas_double_complex *)A, *b = (const rocblas_double_complex *)B;
+    auto *c = (rocblas_double_complex *)C;
+    s = batch == 1 ? rocblas_zgemm(h, nt, nt, m, n, k, &al, a, lda, b, ldb, &be, c, ldc)
+                   : rocblas_zgemm_strided_batched(h, nt, nt, m, n, k, &al, a, lda, sA, b, ldb, sB, &be, c, ldc, sC,
+                                                   batch);
+    MIDAS_CHECK(s == rocblas_status_success, -1, "zgemm: rocblas status %d", (int)s);
+    return 0;
+}
+
 // FP64 trailing update C -= A B: a plain (strided-batched) ZGEMM, handed to
 // rocBLAS, which sustains 74 TF/s on the batched k = 128 shapes of the block
 // LUs against 52 TF/s for k_cgemm<double>.
@@ -631,19 +650,8 @@ int TrailingGemm<double>::operator()(int m, int n, int k, const void *A, long ld
 {
     if (m <= 0 || n <= 0 || k <= 0 || batch <= 0) return 0;
     ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k * batch, 16.0 * (2.0 * m * n + (double)m * k + (double)k * n) * batch);
-    rocblas_handle h = blas_handle();
-    MIDAS_CHECK(h, -1, "trailing update: no rocBLAS handle for this device");
-    rocblas_status s = rocblas_set_stream(h, st);
-    MIDAS_CHECK(s == rocblas_status_success, -1, "trailing update: rocblas_set_stream status %d", (int)s);
-    const rocblas_double_complex alpha(-1.0, 0.0), beta(1.0, 0.0);
-    const rocblas_operation nt = rocblas_operation_none;
-    const auto *a = (const rocblas_double_complex *)A, *b = (const rocblas_double_complex *)B;
-    auto *c = (rocblas_double_complex *)C;
-    s = batch == 1 ? rocblas_zgemm(h, nt, nt, m, n, k, &alpha, a, lda, b, ldb, &beta, c, ldc)
-                   : rocblas_zgemm_strided_batched(h, nt, nt, m, n, k, &alpha, a, lda, stride, b, ldb, stride, &beta,
-                                                   c, ldc, stride, batch);
-    MIDAS_CHECK(s == rocblas_status_success, -1, "trailing update: rocblas_zgemm status %d", (int)s);
-    return 0;
+    return blas_zgemm_nn(m, n, k, z64{-1.0, 0.0}, A, lda, stride, B, ldb, stride, z64{1.0, 0.0}, C, ldc, stride, batch,
+                         st);
 }
 
 template <>

@@ -61,6 +61,11 @@ long getrs_tmp_elems(int n, int nb, int nrhs);
 void ipiv_to_perm(const int *ipiv, int n, int *perm);
 int ipiv_to_perm_dev(const int *ipiv, int *perm, int n, int batch, hipStream_t st);
 
+// C = alpha A B + beta C through rocBLAS ZGEMM (strided batch when batch > 1):
+// the plain FP64 products where the library is faster than k_cgemm<double>.
+int blas_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, long sA, const void *B, long ldb, long sB,
+                  z64 beta, void *C, long ldc, long sC, int batch, hipStream_t st);
+
 // C = alpha op(A) op(B) + beta C; op: 0 none, 1 transpose, 2 conjugate transpose.
 template <typename T>
 int launch_cgemm(int m, int n, int k, cplx<T> alpha, const void *A, long lda, int tA, const void *B, long ldb,
