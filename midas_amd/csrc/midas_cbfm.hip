@@ -144,29 +144,38 @@ __global__ __launch_bounds__(256) void k_ggemm(const GJob *__restrict__ jobs)
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = (w & 1) * 16, wn = (w >> 1) * 16;
     gd4 rr = {0, 0, 0, 0}, ii = {0, 0, 0, 0}, ri = {0, 0, 0, 0}, ir = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < jb.k; k0 += GK) {
-        // op(A) tile: GT rows x GK k
+    // this thread's two elements of each stage's op(A) tile (GT rows x GK k)
+    // and B tile (GK k x GT cols); the next stage's are loaded into registers
+    // while the current one computes
+    int ak[2], aa[2], bk[2], bb[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int idx = t + 256 * e;
+        if (jb.tA) { ak[e] = idx & 15; aa[e] = idx >> 4; }   // A stored k x m: unit stride along k
+        else { aa[e] = idx & 31; ak[e] = idx >> 5; }          // A stored m x k: unit stride along m
+        bk[e] = idx & 15;
+        bb[e] = idx >> 4;
+    }
+    double2 ra[2], rb[2];
+    auto gload = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const int idx = t + 256 * e;
-            int kk, aa;
-            if (jb.tA) { kk = idx & 15; aa = idx >> 4; }   // A stored k x m: unit stride along k
-            else { aa = idx & 31; kk = idx >> 5; }          // A stored m x k: unit stride along m
-            const int a = jb.i0 + aa, kg = k0 + kk;
-            double2 v = make_double2(0.0, 0.0);
-            if (a < jb.m && kg < jb.k)
-                v = jb.tA ? jb.A[(long)a * jb.lda + kg] : jb.A[(long)kg * jb.lda + a];
-            sA[kk][aa] = v;
+            const int a = jb.i0 + aa[e], kg = k0 + ak[e];
+            ra[e] = make_double2(0.0, 0.0);
+            if (a < jb.m && kg < jb.k) ra[e] = jb.tA ? jb.A[(long)a * jb.lda + kg] : jb.A[(long)kg * jb.lda + a];
+            const int b = jb.j0 + bb[e], kb = k0 + bk[e];
+            rb[e] = (b < jb.n && kb < jb.k) ? jb.B[(long)b * jb.ldb + kb] : make_double2(0.0, 0.0);
         }
-        // B tile: GK k x GT cols
+    };
+    gload(0);
+    for (int k0 = 0; k0 < jb.k; k0 += GK) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const int idx = t + 256 * e;
-            const int kk = idx & 15, bb = idx >> 4;
-            const int b = jb.j0 + bb, kg = k0 + kk;
-            sB[kk][bb] = (b < jb.n && kg < jb.k) ? jb.B[(long)b * jb.ldb + kg] : make_double2(0.0, 0.0);
+            sA[ak[e]][aa[e]] = ra[e];
+            sB[bk[e]][bb[e]] = rb[e];
         }
         __syncthreads();
+        if (k0 + GK < jb.k) gload(k0 + GK);
 #pragma unroll
         for (int kk = 0; kk < GK; kk += 4) {
             const double2 x = sA[kk + (lane >> 4)][wm + (lane & 15)];
