@@ -58,23 +58,50 @@ typedef struct {
     const int *id;            /* n_cell identifiers (optional)       */
 } midas_cells_t;
 
+/* Mixed-precision solves refine X in FP64 until the normwise backward error
+ * max_j ||b_j - Z x_j||_inf / (||Z||_inf ||x_j||_inf + ||b_j||_inf) is <= tol.
+ * When it is not reached (the sweeps stop gaining, or max_iter is spent), the
+ * solve falls back to the FP64 LU of Z, as LAPACK's ZCGESV does
+ * (status MIDAS_STATUS_FP64_FALLBACK), unless no_fallback is set: then X holds
+ * the iterate with the smallest backward error, status is
+ * MIDAS_STATUS_NOT_CONVERGED and the call returns MIDAS_ERR_NOT_CONVERGED. */
 typedef struct {
     int max_iter;      /* refinement sweeps (default 30)                                      */
-    double tol;        /* stop when the normwise backward error is <= tol (default 1e-14)    */
+    double tol;        /* backward-error target (default max(1e-14, sqrt(n) * DBL_EPSILON))  */
     int nb;            /* LU panel width (0: default)                                        */
+    int no_fallback;   /* 1: report a refinement that misses tol instead of the FP64 fallback */
 } midas_solve_opts_t;
+
+#define MIDAS_STATUS_CONVERGED 0       /* FP64 LU, or refinement reached tol                   */
+#define MIDAS_STATUS_NOT_CONVERGED 1   /* refinement missed tol; X = best iterate (no_fallback) */
+#define MIDAS_STATUS_FP64_FALLBACK 2   /* refinement missed tol; X from the FP64 LU of Z        */
+#define MIDAS_ERR_NOT_CONVERGED (-4)
 
 typedef struct {
     int info;          /* 0, or first exactly-zero pivot (1-based)                            */
     int iterations;    /* refinement sweeps done (0 for MIDAS_SOLVE_FP64)                     */
-    double berr;       /* final max_j ||b_j - Z x_j||_inf / (||Z||_inf ||x_j||_inf + ||b_j||_inf) */
+    double berr;       /* backward error of the returned X (definition above)                 */
     double anorm;      /* ||Z||_inf (PZLANGE 'I', Compute_EFields_MoM.f90:168)                */
     double t_factor_ms, t_solve_ms, t_refine_ms;
+    int status;        /* MIDAS_STATUS_*                                                      */
 } midas_solve_info_t;
 
 /* ---------------- context ---------------- */
 int midas_init(int device, midas_ctx **ctx);
 int midas_finalize(midas_ctx *ctx);
+/* The solvers keep their device workspace (the FP32 copy of the matrix, LU
+ * work buffers, refinement vectors) in the context and grow it on demand, so
+ * that repeated solves of one size allocate nothing; this frees it.  No
+ * reference counterpart (ScaLAPACK workspaces are the caller's). */
+int midas_release_workspace(midas_ctx *ctx);
+/* Memory budgets (bytes) of the CBFM batching: the padded block matrices of one
+ * batched block-LU chunk, the column strips of the dense reduced-matrix
+ * assembly, the dense block pairs filled side by side, and the ACA factors of
+ * one batch of block columns.  0 keeps the default (24e9, 8 GiB, 4 GiB, 4e9).
+ * Smaller budgets give more, smaller batches with the same results.  No
+ * reference counterpart (the reference distributes blocks over MPI ranks). */
+int midas_set_cbfm_budgets(midas_ctx *ctx, double lu_bytes, double strip_bytes, double dense_bytes,
+                           double aca_bytes);
 const char *midas_last_error(void);
 void *midas_stream(midas_ctx *ctx);           /* hipStream_t of the context */
 

@@ -30,6 +30,9 @@ Q_PI, Q_LITERALS, Q_CEXT_SUM, Q_FP32_BLOCKS = 1, 2, 4, 8
 Q_ALL = 15
 
 SOLVE_FP64, SOLVE_C32_IR, SOLVE_F16_IR, SOLVE_F16X3_IR, SOLVE_BF16_IR, SOLVE_BF16X3_IR = range(6)
+# refinement outcome (midas_solve_info_t.status) and the no_fallback error code
+STATUS_CONVERGED, STATUS_NOT_CONVERGED, STATUS_FP64_FALLBACK = 0, 1, 2
+ERR_NOT_CONVERGED = -4
 MODE_NAMES = {SOLVE_FP64: "fp64", SOLVE_C32_IR: "c32+ir", SOLVE_F16_IR: "f16+ir", SOLVE_F16X3_IR: "f16x3+ir",
               SOLVE_BF16_IR: "bf16+ir", SOLVE_BF16X3_IR: "bf16x3+ir"}
 
@@ -40,7 +43,7 @@ EXPORTS = (
     "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_dev_compute_efields_cbfm", "midas_profile", "midas_profile_query", "midas_block_frames",
     "midas_extend_blocks", "midas_scattering_quantities", "midas_near_field", "midas_partition_sph",
     "midas_partition_csh", "midas_scattering_quantities_grid", "midas_quad_adaptive", "midas_quad_trap2d",
-    "midas_quad_gauss_legendre",
+    "midas_quad_gauss_legendre", "midas_release_workspace", "midas_set_cbfm_budgets",
 )
 
 RX_SD, RX_RF, RX_LB = 0, 1, 2
@@ -51,6 +54,15 @@ class MidasError(RuntimeError):
     pass
 
 
+class NotConverged(MidasError):
+    """A mixed-precision solve with no_fallback missed its backward-error
+    target; .result holds what the call returned (the best iterate)."""
+
+    def __init__(self, msg, result):
+        super().__init__(msg)
+        self.result = result
+
+
 class cells_t(C.Structure):
     _fields_ = [("n", C.c_int), ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("sc", C.c_void_p),
                 ("eps", C.c_void_p), ("kappa", C.c_void_p), ("che", C.c_void_p), ("znnpp", C.c_void_p),
@@ -58,12 +70,13 @@ class cells_t(C.Structure):
 
 
 class solve_opts_t(C.Structure):
-    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("nb", C.c_int)]
+    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("nb", C.c_int), ("no_fallback", C.c_int)]
 
 
 class solve_info_t(C.Structure):
     _fields_ = [("info", C.c_int), ("iterations", C.c_int), ("berr", C.c_double), ("anorm", C.c_double),
-                ("t_factor_ms", C.c_double), ("t_solve_ms", C.c_double), ("t_refine_ms", C.c_double)]
+                ("t_factor_ms", C.c_double), ("t_solve_ms", C.c_double), ("t_refine_ms", C.c_double),
+                ("status", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -276,6 +289,15 @@ class Midas:
     def stream(self):
         return self.lib.midas_stream(self.ctx)
 
+    def release_workspace(self):
+        """Free the solver workspace the context keeps between calls."""
+        _check(self.lib.midas_release_workspace(self.ctx), "release_workspace")
+
+    def set_cbfm_budgets(self, lu=0.0, strip=0.0, dense=0.0, aca=0.0):
+        """Batching budgets of the CBFM path in bytes (0 = default)."""
+        _check(self.lib.midas_set_cbfm_budgets(self.ctx, C.c_double(lu), C.c_double(strip), C.c_double(dense),
+                                               C.c_double(aca)), "set_cbfm_budgets")
+
     # -- a1 ---------------------------------------------------------------
     def set_cells_params(self, sc, eps, k0, quirks=Q_ALL):
         sc = np.ascontiguousarray(sc, np.float64)
@@ -364,18 +386,23 @@ class Midas:
         return Q, av
 
     # -- a4 -----------------------------------------------------------------
-    def solve(self, Z, B, mode=SOLVE_FP64, max_iter=0, tol=0.0, nb=0):
+    def solve(self, Z, B, mode=SOLVE_FP64, max_iter=0, tol=0.0, nb=0, no_fallback=False):
+        """X = Z^-1 B and the solve report.  With no_fallback a refinement that
+        misses tol raises NotConverged (its .result = (X, report))."""
         Z = np.asfortranarray(Z, np.complex128)
         X = np.array(B, np.complex128, order="F", copy=True)
         if X.ndim == 1:
             X = X.reshape(-1, 1, order="F")
         n, nrhs = X.shape
-        opts = solve_opts_t(max_iter, tol, nb)
+        opts = solve_opts_t(max_iter, tol, nb, int(no_fallback))
         info = solve_info_t()
-        r = _check(self.lib.midas_solve(self.ctx, C.c_int(n), C.c_int(nrhs), _p(Z), C.c_long(Z.shape[0]), _p(X),
-                                        C.c_long(n), C.c_int(mode), C.byref(opts), C.byref(info)), "solve")
+        r = self.lib.midas_solve(self.ctx, C.c_int(n), C.c_int(nrhs), _p(Z), C.c_long(Z.shape[0]), _p(X),
+                                 C.c_long(n), C.c_int(mode), C.byref(opts), C.byref(info))
         d = info.as_dict()
         d["info"] = r
+        if r == ERR_NOT_CONVERGED:
+            raise NotConverged(self.lib.midas_last_error().decode(), (X, d))
+        _check(r, "solve")
         return X, d
 
     # -- a5 / a6 ------------------------------------------------------------
@@ -407,7 +434,7 @@ class Midas:
 
     # -- Compute_EFields_MoM -------------------------------------------------
     def compute_efields_mom(self, cells, k0, transmitters, receivers, mode=SOLVE_FP64, quirks=Q_ALL, want_E=True,
-                            max_iter=0, tol=0.0, nb=0):
+                            max_iter=0, tol=0.0, nb=0, no_fallback=False):
         cells = _cells(cells)
         tth, tph = (np.ascontiguousarray(a, np.float64) for a in transmitters)
         rth, rph = (np.ascontiguousarray(a, np.float64) for a in receivers)
@@ -416,15 +443,17 @@ class Midas:
         ce = np.empty(ntr, np.complex128)
         ca = np.empty(ntr, np.complex128)
         E = np.empty((3 * cells.n, 2 * ntr), np.complex128, order="F") if want_E else None
-        opts = solve_opts_t(max_iter, tol, nb)
+        opts = solve_opts_t(max_iter, tol, nb, int(no_fallback))
         info = solve_info_t()
         cs = cells.struct()
-        r = _check(self.lib.midas_compute_efields_mom(self.ctx, C.byref(cs), C.c_double(k0), C.c_int(quirks),
-                                                      C.c_int(ntr), _p(tth), _p(tph), C.c_int(nrx), _p(rth), _p(rph),
-                                                      C.c_int(mode), C.byref(opts), _p(S), _p(ce), _p(ca), _p(E),
-                                                      C.byref(info)), "compute_efields_mom")
+        r = self.lib.midas_compute_efields_mom(self.ctx, C.byref(cs), C.c_double(k0), C.c_int(quirks), C.c_int(ntr),
+                                               _p(tth), _p(tph), C.c_int(nrx), _p(rth), _p(rph), C.c_int(mode),
+                                               C.byref(opts), _p(S), _p(ce), _p(ca), _p(E), C.byref(info))
         d = info.as_dict()
         d["info"] = r
+        if r == ERR_NOT_CONVERGED:
+            raise NotConverged(self.lib.midas_last_error().decode(), (S, ce, ca, E, d))
+        _check(r, "compute_efields_mom")
         return S, ce, ca, E, d
 
     # -- a14 / a16 ----------------------------------------------------------

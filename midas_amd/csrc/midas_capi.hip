@@ -5,7 +5,9 @@
 #include "../../include/midas_gpu.h"
 #include "../../include/midas_gpu_test.h"
 
+#include <cfloat>
 #include <chrono>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -108,143 +110,248 @@ static int trail_mode(int mode)
 
 static int default_nb(int n, int mode) { return mode == MIDAS_SOLVE_FP64 ? 128 : 512; }
 
-// Solve Z X = B on the device.  Z (n x n, ldz) FP64; if W32 is non-NULL it
-// already holds the FP32 copy of Z (mixed modes).  For MIDAS_SOLVE_FP64 Z is
-// factored in place when z_inplace, else copied.  X (n x nrhs, ldx) receives
-// the solution; B (n x nrhs, ldb) is read only.
-int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32, long ldw, const double2 *B,
-                     long ldb, double2 *X, long ldx, int mode, const midas_solve_opts_t *opts, midas_solve_info_t *info,
-                     bool z_inplace)
+// perm from LAPACK interchanges for one matrix, on the device: one workgroup,
+// the permutation held in LDS (n <= PERM_LDS_MAX), one thread walks the swaps.
+constexpr int PERM_LDS_MAX = 16384;
+__global__ __launch_bounds__(256) void k_ipiv_to_perm_lds(const int *ipiv, int *perm, int n)
+{
+    __shared__ int p[PERM_LDS_MAX];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < n; ++i) {
+            const int q = ipiv[i];
+            if (q != i) {
+                const int t = p[i];
+                p[i] = p[q];
+                p[q] = t;
+            }
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = p[i];
+}
+
+// ipiv (device) -> perm (device) for one matrix of order n
+static int make_perm(const int *ipiv_d, int *perm_d, int n, hipStream_t st)
+{
+    if (n <= PERM_LDS_MAX) {
+        hipLaunchKernelGGL(k_ipiv_to_perm_lds, dim3(1), dim3(256), 0, st, ipiv_d, perm_d, n);
+        MIDAS_HIP(hipGetLastError());
+        return 0;
+    }
+    std::vector<int> ipiv(n), perm(n);
+    TRY(copy_sync(ipiv.data(), ipiv_d, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    ipiv_to_perm(ipiv.data(), n, perm.data());
+    TRY(copy_sync(perm_d, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    return 0;
+}
+
+// FP64 LU of A (in place) and X = A^-1 B with it (PZGESV semantics).
+static int fp64_lu_solve(midas_ctx *ctx, int n, int nrhs, double2 *A, long lda, const double2 *B, long ldb, double2 *X,
+                         long ldx, int nb, midas_solve_info_t &inf)
 {
     hipStream_t st = ctx->st;
+    SolveWs &ws = ctx->ws;
+    LuWork<double> &w = ws.lud;
+    TRY(w.alloc(n, nb));
+    TrailingGemm<double> tg;
+    int r = getrf<double>(A, lda, n, ws.ipiv.as<int>(), w, st, tg);
+    if (r) return r;
+    MIDAS_HIP(hipMemcpyAsync(&inf.info, w.info, sizeof(int), hipMemcpyDeviceToHost, st));
+    TRY(make_perm(ws.ipiv.as<int>(), ws.perm.as<int>(), n, st));
+    hipLaunchKernelGGL(k_copyz, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, X, ldx, n, nrhs);
+    TRY(ws.t2.reserve(sizeof(double2) * (size_t)getrs_tmp_elems(n, nb, nrhs)));
+    r = getrs<double>(A, lda, n, ws.perm.as<int>(), w, X, ldx, nrhs, ws.t2.as<double2>(), st);
+    if (r) return r;
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+// R = B - Z X, the column maxima of X and R to the host, and the normwise
+// backward error max_j ||r_j|| / (||Z|| ||x_j|| + ||b_j||) (infinity norms).
+static int residual_berr(midas_ctx *ctx, int n, int nrhs, const double2 *Z, long ldz, const double2 *B, long ldb,
+                         const double2 *X, long ldx, double anorm, const std::vector<double> &hb, double *berr)
+{
+    hipStream_t st = ctx->st;
+    SolveWs &ws = ctx->ws;
+    double *colx = ws.norms.as<double>() + n, *colr = colx + nrhs;
+    double2 *R = ws.R.as<double2>();
+    const dim3 g2((n + 255) / 256, nrhs);
+    int r;
+    if (nrhs <= 4) {
+        r = launch_residual_few(Z, ldz, n, X, ldx, nrhs, B, ldb, R, n, ws.zx.p, st);
+    } else {
+        hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R, (long)n, n, nrhs);
+        // many right-hand sides: rocBLAS (33k x 448 x 33k: 60 ms against 74 ms for k_cgemm<double>)
+        r = nrhs >= 32 ? blas_zgemm_nn(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R, n, 0, 1, st)
+                       : launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R, n,
+                                              st);
+    }
+    if (r) return r;
+    hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, X, ldx, n, colx);
+    hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, R, (long)n, n, colr);
+    MIDAS_HIP(hipGetLastError());
+    std::vector<double> hx(nrhs), hr(nrhs);
+    TRY(copy_sync(hx.data(), colx, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    TRY(copy_sync(hr.data(), colr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    double b = 0.0;
+    for (int c = 0; c < nrhs; ++c) {
+        const double den = anorm * hx[c] + hb[c];
+        const double e = den > 0 ? hr[c] / den : (hr[c] > 0 ? INFINITY : 0.0);
+        b = (e > b || std::isnan(e)) ? e : b;
+    }
+    *berr = b;
+    return 0;
+}
+
+// Solve Z X = B on the device.  Z (n x n, ldz) FP64; if W32 is non-NULL it
+// already holds the FP32 copy of Z (mixed modes).  For MIDAS_SOLVE_FP64 Z is
+// factored in place when z_inplace, else copied; mixed modes factor Z in place
+// only for the FP64 fallback, which z_inplace allows.  X (n x nrhs, ldx)
+// receives the solution; B (n x nrhs, ldb) is read only.  Work buffers live in
+// ctx->ws and only grow.
+int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32, long ldw, const double2 *B,
+              long ldb, double2 *X, long ldx, int mode, const midas_solve_opts_t *opts, midas_solve_info_t *info,
+              bool z_inplace)
+{
+    hipStream_t st = ctx->st;
+    SolveWs &ws = ctx->ws;
     midas_solve_info_t inf{};
-    int max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 30;
-    double tol = (opts && opts->tol > 0) ? opts->tol : 1e-14;
+    const int max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 30;
+    const double tol = (opts && opts->tol > 0) ? opts->tol : std::max(1e-14, std::sqrt((double)n) * DBL_EPSILON);
+    const bool fallback = !(opts && opts->no_fallback) && z_inplace;
     int nb = (opts && opts->nb > 0) ? opts->nb : default_nb(n, mode);
     nb = nb > n ? n : nb;
     MIDAS_CHECK(n > 0 && nrhs > 0, -1, "solve: n=%d nrhs=%d", n, nrhs);
     MIDAS_CHECK(mode >= 0 && mode <= 5, -1, "solve: unknown mode %d", mode);
-    DBuf ipiv_d, perm_d, norms, tmpn;
-    TRY(ipiv_d.alloc(sizeof(int) * n));
-    TRY(perm_d.alloc(sizeof(int) * n));
-    TRY(norms.alloc(sizeof(double) * (n + nrhs * 2 + 4)));
-    std::vector<int> ipiv(n), perm(n);
-    double *rowsum = norms.as<double>();
-    double *colx = rowsum + n, *colr = colx + nrhs, *scal = colr + nrhs;
+    TRY(ws.ipiv.reserve(sizeof(int) * n));
+    TRY(ws.perm.reserve(sizeof(int) * n));
+    TRY(ws.norms.reserve(sizeof(double) * (n + nrhs * 2 + 4)));
+    double *rowsum = ws.norms.as<double>();
+    double *scal = rowsum + n + 2 * nrhs;
 
     // ||Z||_inf before factoring (PZLANGE 'I'); the scratch also serves the
     // few-RHS residuals of refinement
-    DBuf zx;
-    TRY(zx.alloc(zx_scratch_bytes(n, nrhs <= 4 ? nrhs : 1)));
-    TRY(launch_rowabs(Z, ldz, n, rowsum, zx.p, st));
+    TRY(ws.zx.reserve(zx_scratch_bytes(n, nrhs <= 4 ? nrhs : 1)));
+    TRY(launch_rowabs(Z, ldz, n, rowsum, ws.zx.p, st));
     hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, st, rowsum, n, scal);
     MIDAS_HIP(hipGetLastError());
-    MIDAS_HIP(hipMemcpyAsync(&inf.anorm, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+    TRY(copy_sync(&inf.anorm, scal, sizeof(double), hipMemcpyDeviceToHost, st));
 
     Timer tm(st);
     if (mode == MIDAS_SOLVE_FP64) {
-        DBuf zc;
         double2 *A = Z;
         long lda = ldz;
         if (!z_inplace) {
-            TRY(zc.alloc(sizeof(double2) * (size_t)n * n));
-            A = zc.as<double2>();
+            TRY(ws.zcopy.reserve(sizeof(double2) * (size_t)n * n));
+            A = ws.zcopy.as<double2>();
             lda = n;
             hipLaunchKernelGGL(k_copyz, dim3((n + 255) / 256, n), dim3(256), 0, st, Z, ldz, A, lda, n, n);
         }
-        LuWork<double> w;
-        TRY(w.alloc(n, nb));
-        TrailingGemm<double> tg;
-        int r = getrf<double>(A, lda, n, ipiv_d.as<int>(), w, st, tg);
-        if (r) { w.release(); return r; }
-        MIDAS_HIP(hipMemcpyAsync(&inf.info, w.info, sizeof(int), hipMemcpyDeviceToHost, st));
-        inf.t_factor_ms = tm.ms();
-        TRY(copy_sync(ipiv.data(), ipiv_d.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
-        ipiv_to_perm(ipiv.data(), n, perm.data());
-        TRY(copy_sync(perm_d.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_copyz, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, X, ldx, n, nrhs);
-        DBuf t2;
-        TRY(t2.alloc(sizeof(double2) * (size_t)getrs_tmp_elems(n, nb, nrhs)));
-        r = getrs<double>(A, lda, n, perm_d.as<int>(), w, X, ldx, nrhs, t2.as<double2>(), st);
-        w.release();
+        int r = fp64_lu_solve(ctx, n, nrhs, A, lda, B, ldb, X, ldx, nb, inf);
         if (r) return r;
-        inf.t_solve_ms = tm.ms();
-    } else {
-        DBuf wc;
-        float2 *A = W32;
-        long lda = ldw;
-        if (!A) {
-            TRY(wc.alloc(sizeof(float2) * (size_t)n * n));
-            A = wc.as<float2>();
-            lda = n;
-            hipLaunchKernelGGL(k_z2c, dim3((n + 255) / 256, n), dim3(256), 0, st, Z, ldz, A, lda, n, n);
-            MIDAS_HIP(hipGetLastError());
-        }
-        LuWork<float> w;
-        TRY(w.alloc(n, nb));
-        DBuf scale;
-        TRY(scale.alloc(trail16_scratch_bytes(n, n, nb)));
-        TrailingGemm<float> tg;
-        tg.mode = trail_mode(mode);
-        tg.scratch = scale.p;
-        int r = getrf<float>(A, lda, n, ipiv_d.as<int>(), w, st, tg);
-        if (r) { w.release(); return r; }
-        MIDAS_HIP(hipMemcpyAsync(&inf.info, w.info, sizeof(int), hipMemcpyDeviceToHost, st));
         inf.t_factor_ms = tm.ms();
-        TRY(copy_sync(ipiv.data(), ipiv_d.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
-        ipiv_to_perm(ipiv.data(), n, perm.data());
-        TRY(copy_sync(perm_d.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-        // refinement: X = 0, R = B; repeat D = LU^-1 R (FP32), X += D, R = B - Z X (FP64)
-        DBuf R, D, t2;
-        TRY(R.alloc(sizeof(double2) * (size_t)n * nrhs));
-        TRY(D.alloc(sizeof(float2) * (size_t)n * nrhs));
-        TRY(t2.alloc(sizeof(float2) * (size_t)getrs_tmp_elems(n, nb, nrhs)));
-        dim3 g2((n + 255) / 256, nrhs);
-        hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R.as<double2>(), (long)n, n, nrhs);
-        std::vector<double> hx(nrhs), hr(nrhs), hb(nrhs);
-        hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, B, ldb, n, colr);
-        TRY(copy_sync(hb.data(), colr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
-        double berr = 1e300;
-        int it = 0;
-        for (; it < max_iter; ++it) {
-            hipLaunchKernelGGL(k_z2c, g2, dim3(256), 0, st, R.as<double2>(), (long)n, D.as<float2>(), (long)n, n, nrhs);
-            r = getrs<float>(A, lda, n, perm_d.as<int>(), w, D.as<float2>(), (long)n, nrhs, t2.as<float2>(), st);
-            if (r) { w.release(); return r; }
-            hipLaunchKernelGGL(k_c2z_add, g2, dim3(256), 0, st, D.as<float2>(), (long)n, X, ldx, n, nrhs, it > 0);
-            if (nrhs <= 4) {
-                r = launch_residual_few(Z, ldz, n, X, ldx, nrhs, B, ldb, R.as<double2>(), n, zx.p, st);
-            } else {
-                hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R.as<double2>(), (long)n, n, nrhs);
-                // many right-hand sides: rocBLAS (33k x 448 x 33k: 60 ms against 74 ms for k_cgemm<double>)
-                r = nrhs >= 32 ? blas_zgemm_nn(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R.p, n,
-                                               0, 1, st)
-                               : launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0},
-                                                      R.p, n, st);
-            }
-            if (r) { w.release(); return r; }
-            hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, X, ldx, n, colx);
-            hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, R.as<double2>(), (long)n, n, colr);
-            MIDAS_HIP(hipGetLastError());
-            TRY(copy_sync(hx.data(), colx, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
-            TRY(copy_sync(hr.data(), colr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
-            double b = 0.0;
-            for (int c = 0; c < nrhs; ++c) {
-                double den = inf.anorm * hx[c] + hb[c];
-                double e = den > 0 ? hr[c] / den : 0.0;
-                b = e > b ? e : b;
-            }
-            if (it == 0) inf.t_solve_ms = tm.ms();
-            bool stalled = (it > 2 && b > 0.5 * berr);
-            berr = b;
-            if (berr <= tol || stalled) { ++it; break; }
+        inf.status = MIDAS_STATUS_CONVERGED;
+        if (info) *info = inf;
+        return inf.info;
+    }
+
+    float2 *A = W32;
+    long lda = ldw;
+    if (!A) {
+        TRY(ws.W.reserve(sizeof(float2) * (size_t)n * n));
+        A = ws.W.as<float2>();
+        lda = n;
+        hipLaunchKernelGGL(k_z2c, dim3((n + 255) / 256, n), dim3(256), 0, st, Z, ldz, A, lda, n, n);
+        MIDAS_HIP(hipGetLastError());
+    }
+    LuWork<float> &w = ws.luf;
+    TRY(w.alloc(n, nb));
+    TRY(ws.trail.reserve(trail16_scratch_bytes(n, n, nb)));
+    TrailingGemm<float> tg;
+    tg.mode = trail_mode(mode);
+    tg.scratch = ws.trail.p;
+    int r = getrf<float>(A, lda, n, ws.ipiv.as<int>(), w, st, tg);
+    if (r) return r;
+    MIDAS_HIP(hipMemcpyAsync(&inf.info, w.info, sizeof(int), hipMemcpyDeviceToHost, st));
+    TRY(make_perm(ws.ipiv.as<int>(), ws.perm.as<int>(), n, st));
+    inf.t_factor_ms = tm.ms();
+    // refinement: X = 0, R = B; repeat D = LU^-1 R (FP32), X += D, R = B - Z X (FP64)
+    TRY(ws.R.reserve(sizeof(double2) * (size_t)n * nrhs));
+    TRY(ws.D.reserve(sizeof(float2) * (size_t)n * nrhs));
+    TRY(ws.t2.reserve(sizeof(float2) * (size_t)getrs_tmp_elems(n, nb, nrhs)));
+    TRY(ws.xbest.reserve(sizeof(double2) * (size_t)n * nrhs));
+    double2 *R = ws.R.as<double2>(), *Xb = ws.xbest.as<double2>();
+    float2 *D = ws.D.as<float2>();
+    const dim3 g2((n + 255) / 256, nrhs);
+    hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R, (long)n, n, nrhs);
+    std::vector<double> hb(nrhs);
+    double *colr = rowsum + n + nrhs;
+    hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, B, ldb, n, colr);
+    TRY(copy_sync(hb.data(), colr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    double berr = INFINITY, best = INFINITY, prev = INFINITY;
+    int it = 0;
+    bool best_saved = false;   // Xb holds an iterate better than the current X
+    for (; it < max_iter; ++it) {
+        hipLaunchKernelGGL(k_z2c, g2, dim3(256), 0, st, R, (long)n, D, (long)n, n, nrhs);
+        r = getrs<float>(A, lda, n, ws.perm.as<int>(), w, D, (long)n, nrhs, ws.t2.as<float2>(), st);
+        if (r) return r;
+        hipLaunchKernelGGL(k_c2z_add, g2, dim3(256), 0, st, D, (long)n, X, ldx, n, nrhs, it > 0);
+        if (it == 0) inf.t_solve_ms = tm.ms();
+        TRY(residual_berr(ctx, n, nrhs, Z, ldz, B, ldb, X, ldx, inf.anorm, hb, &berr));
+        if (std::isnan(berr)) {
+            best_saved = best < INFINITY;
+            ++it;
+            break;
         }
-        inf.iterations = it;
-        inf.berr = berr;
-        w.release();
-        inf.t_refine_ms = tm.ms();
+        if (berr <= tol) {
+            best = berr;
+            best_saved = false;
+            ++it;
+            break;
+        }
+        if (berr < best) {
+            // keep this iterate: a later sweep may be worse
+            best = berr;
+            hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, X, ldx, Xb, (long)n, n, nrhs);
+            best_saved = false;
+        } else {
+            best_saved = true;
+        }
+        // no progress: the sweeps stop gaining at least a factor of two
+        const bool stalled = it >= 2 && berr > 0.5 * prev;
+        prev = berr;
+        if (stalled) {
+            ++it;
+            break;
+        }
+    }
+    inf.iterations = it;
+    if (best_saved) {
+        // the last sweep made X worse than an earlier iterate: return that one
+        hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, Xb, (long)n, X, ldx, n, nrhs);
+        MIDAS_HIP(hipGetLastError());
+    }
+    inf.berr = best;
+    inf.t_refine_ms = tm.ms();
+    if (best <= tol) {
+        inf.status = MIDAS_STATUS_CONVERGED;
+    } else if (fallback && inf.info == 0) {
+        // refinement cannot reach tol with this factorisation (ZCGESV's rule):
+        // the FP64 LU of Z, in place
+        r = fp64_lu_solve(ctx, n, nrhs, Z, ldz, B, ldb, X, ldx, default_nb(n, MIDAS_SOLVE_FP64), inf);
+        if (r) return r;
+        inf.status = MIDAS_STATUS_FP64_FALLBACK;
+        inf.t_refine_ms += tm.ms();
+    } else {
+        inf.status = MIDAS_STATUS_NOT_CONVERGED;
     }
     if (info) *info = inf;
-    return inf.info;
+    if (inf.info) return inf.info;
+    MIDAS_CHECK(inf.status != MIDAS_STATUS_NOT_CONVERGED, MIDAS_ERR_NOT_CONVERGED,
+                "solve: refinement stopped at backward error %.3g above tol %.3g after %d sweeps (mode %d); X is the "
+                "best iterate", inf.berr, tol, it, mode);
+    return 0;
 }
 
 namespace midas {
@@ -282,9 +389,34 @@ int midas_init(int device, midas_ctx **ctx)
 int midas_finalize(midas_ctx *ctx)
 {
     if (!ctx) return 0;
+    hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->st);
+    ctx->ws.release();
     hipStreamDestroy(ctx->st);
     delete ctx;
+    return 0;
+}
+
+int midas_release_workspace(midas_ctx *ctx)
+{
+    MIDAS_CHECK(ctx, -1, "release_workspace: ctx is NULL");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    ctx->ws.release();
+    DBuf *b[] = {&ctx->cells.x, &ctx->cells.y, &ctx->cells.z, &ctx->cells.sc, &ctx->cells.eps, &ctx->cells.kappa,
+                 &ctx->cells.che, &ctx->cells.znnpp, &ctx->cells.kchi, &ctx->cells.id, &ctx->cells.a};
+    for (DBuf *x : b) x->release();
+    return 0;
+}
+
+int midas_set_cbfm_budgets(midas_ctx *ctx, double lu_bytes, double strip_bytes, double dense_bytes, double aca_bytes)
+{
+    MIDAS_CHECK(ctx, -1, "set_cbfm_budgets: ctx is NULL");
+    const double v[4] = {lu_bytes, strip_bytes, dense_bytes, aca_bytes};
+    for (int i = 0; i < 4; ++i) {
+        MIDAS_CHECK(v[i] >= 0 && std::isfinite(v[i]), -1, "set_cbfm_budgets: budget %d = %g", i, v[i]);
+        ctx->cbfm_budget[i] = v[i];
+    }
     return 0;
 }
 
@@ -496,7 +628,8 @@ int midas_solve(midas_ctx *ctx, int n, int nrhs, const double *Z, long ldz, doub
                                hipMemcpyHostToDevice, st));
     int r = solve_dev(ctx, n, nrhs, dz.as<double2>(), n, nullptr, 0, db.as<double2>(), n, dx.as<double2>(), n, mode,
                       opts, info, true);
-    if (r < 0) return r;
+    // not converged (no_fallback): the best iterate is still returned in B
+    if (r < 0 && r != MIDAS_ERR_NOT_CONVERGED) return r;
     MIDAS_HIP(hipMemcpy2DAsync(B, sizeof(double2) * ldb, dx.p, sizeof(double2) * n, sizeof(double2) * n, nrhs,
                                hipMemcpyDeviceToHost, st));
     MIDAS_HIP(hipStreamSynchronize(st));
@@ -684,20 +817,20 @@ static int mom_solve_dev(midas_ctx *ctx, const DevCells &cells, double k0, int q
                          double2 *X, long ldx, midas_solve_info_t *info, double *t_fill_ms)
 {
     hipStream_t st = ctx->st;
+    SolveWs &ws = ctx->ws;
     const int n = cells.n, n3 = 3 * n;
-    DBuf W, V;
     float2 *w32 = nullptr;
     if (mode != MIDAS_SOLVE_FP64) {
-        TRY(W.alloc(sizeof(float2) * (size_t)n3 * n3));
-        w32 = W.as<float2>();
+        TRY(ws.W.reserve(sizeof(float2) * (size_t)n3 * n3));
+        w32 = ws.W.as<float2>();
     }
-    TRY(V.alloc(sizeof(double2) * (size_t)n3 * 2 * ntr));
+    TRY(ws.V.reserve(sizeof(double2) * (size_t)n3 * 2 * ntr));
     Timer tm(st);
     TRY(launch_fill(cells, CellSet{n, 0, nullptr}, CellSet{n, 0, nullptr}, k0, quirks, false, Z, ldz, w32, n3, st));
-    TRY(launch_rhs(cells, CellSet{n, 0, nullptr}, ntr, dth, dph, k0, quirks, false, V.as<double2>(), n3, st));
+    TRY(launch_rhs(cells, CellSet{n, 0, nullptr}, ntr, dth, dph, k0, quirks, false, ws.V.as<double2>(), n3, st));
     double tf = tm.ms();
     if (t_fill_ms) *t_fill_ms = tf;
-    return solve_dev(ctx, n3, 2 * ntr, Z, ldz, w32, n3, V.as<double2>(), n3, X, ldx, mode, opts, info, true);
+    return solve_dev(ctx, n3, 2 * ntr, Z, ldz, w32, n3, ws.V.as<double2>(), n3, X, ldx, mode, opts, info, true);
 }
 
 int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int ntr,
@@ -718,8 +851,11 @@ int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double
     TRY(upload(dph, tr_ph, sizeof(double) * ntr, st));
     int r = mom_solve_dev(ctx, dc.v, k0, quirks, ntr, dth.as<double>(), dph.as<double>(), mode, opts, Z.as<double2>(),
                           n3, X.as<double2>(), n3, info, nullptr);
-    if (r < 0) return r;
+    // a refinement that missed its tolerance (no_fallback) still delivers the
+    // outputs of its best iterate, then reports
+    if (r < 0 && r != MIDAS_ERR_NOT_CONVERGED) return r;
     int singular = r;
+    std::string nc_msg = r == MIDAS_ERR_NOT_CONVERGED ? midas::last_error() : "";
     Z.release();
     if (S_total && nrx > 0) {
         DBuf drt, drp, db, dS, FF, E3, G;
@@ -747,6 +883,7 @@ int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double
     }
     if (E) MIDAS_HIP(hipMemcpyAsync(E, X.p, sizeof(double2) * (size_t)n3 * 2 * ntr, hipMemcpyDeviceToHost, st));
     MIDAS_HIP(hipStreamSynchronize(st));
+    if (singular == MIDAS_ERR_NOT_CONVERGED) midas::set_error("%s", nc_msg.c_str());
     return singular;
 }
 
@@ -771,7 +908,7 @@ int midas_dev_mom_solve(midas_ctx *ctx, const midas_cells_t *dcells, double k0, 
     MIDAS_CHECK(ctx && dcells && dth && dph && Z && X && ntr > 0, -1, "dev_mom_solve: bad arguments");
     MIDAS_CHECK(ldz >= 3L * dcells->n && ldx >= 3L * dcells->n, -1, "dev_mom_solve: leading dimension too small");
     MIDAS_HIP(hipSetDevice(ctx->device));
-    DevCellsOwned dc;
+    DevCellsOwned &dc = ctx->cells;
     TRY(dc.from_device(dcells, k0, quirks, ctx->st));
     int r = mom_solve_dev(ctx, dc.v, k0, quirks, ntr, dth, dph, mode, opts, (double2 *)Z, ldz, (double2 *)X, ldx, info,
                           t_fill_ms);

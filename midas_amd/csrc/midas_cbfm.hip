@@ -366,6 +366,10 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
     const double thr = threshold > 0 ? threshold : cbf_threshold_rule(cells, use_aca);
     const long N3 = 3L * N;
     const int T2c = 2 * ntr_cbf, T2 = 2 * ntr;
+    // batching budgets (midas_set_cbfm_budgets; 0 = default)
+    auto pick_budget = [&](int i, double dflt) { return ctx->cbfm_budget[i] > 0 ? ctx->cbfm_budget[i] : dflt; };
+    const double lu_budget = pick_budget(0, 24e9), strip_budget = pick_budget(1, (double)(8L << 30)),
+                 dense_budget = pick_budget(2, (double)(4L << 30)), aca_budget = pick_budget(3, 4.0e9);
 
     // MIDAS_VERBOSE=1 prints the time of each phase (synchronising between them)
     const bool verbose = std::getenv("MIDAS_VERBOSE") && std::atoi(std::getenv("MIDAS_VERBOSE")) > 0;
@@ -448,7 +452,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
     {
         // Blocks sorted by extended size (largest first) and solved in batches,
         // each padded only to its own largest block (uneven blocks would
-        // otherwise all pay for the largest), within a 24 GB budget for the
+        // otherwise all pay for the largest), within the LU budget (24 GB unless set) for the
         // batch's matrices.  Solutions are staged per batch (the batched solve
         // needs one stride) and copied to X_b.
         struct Chunk { int pos, nc; long mpc; };
@@ -461,7 +465,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         for (int pos = 0; pos < nb;) {
             const long mpc = 3L * sb[order[pos]];
             const long zb = (long)sizeof(double2) * mpc * mpc;
-            const int nc = (int)std::max(1L, std::min((long)(nb - pos), (long)(24e9 / (double)zb)));
+            const int nc = (int)std::max(1L, std::min((long)(nb - pos), (long)(lu_budget / (double)zb)));
             chunks.push_back({pos, nc, mpc});
             zmax = std::max(zmax, zb * nc);
             pmax_c = std::max(pmax_c, mpc * nc);
@@ -642,7 +646,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         // T(:, block j) = Z(:, block j) C_j for a chunk of block columns (one
         // strip fill and one GEMM each), then every C_i^T T(rows of i, j) as
         // one grouped launch
-        const long tbudget = 8L << 30;
+        const long tbudget = (long)strip_budget;
         const int tcols = (int)std::max<long>(kbmax, std::min<long>(Kt, tbudget / ((long)sizeof(double2) * N3)));
         DBuf Zs, Tt;
         TRY(Zs.alloc(sizeof(double2) * N3 * 3 * nmax));
@@ -681,7 +685,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         // milliseconds each and serialises the stream)
         DBuf Zd, Md, dj, Ub, Vb, dk, LR;
         auto dense_pairs = [&]() -> int {
-            const long zbudget = 4L << 30;
+            const long zbudget = (long)dense_budget;
             size_t q0 = 0;
             while (q0 < dense.size()) {
                 size_t q1 = q0;
@@ -726,7 +730,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         TRY(dense_pairs());
         // off-diagonal pairs by ACA, a batch of block columns per launch; then
         // Zc_ij = (C_i^T U)(V C_j) through two grouped launches
-        const double budget = 4.0e9;   // bytes of U, Vt and the small factors per batch
+        const double budget = aca_budget;   // bytes of U, Vt and the small factors per batch
         int j0 = 0;
         while (j0 < nb) {
             std::vector<AcaJob> ajobs;

@@ -2,8 +2,10 @@
 // the context, the device cell table and the dense solver entry.
 #pragma once
 #include "midas_common.h"
+#include "midas_lu.h"
 #include "../../include/midas_gpu.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace midas {
@@ -32,11 +34,12 @@ struct DBuf {
         return 0;
     }
     // grow-only: keeps the allocation when it is large enough (contents are
-    // not preserved when it grows); hipFree waits for work still using it
+    // not preserved when it grows; the slack is a quarter, at most 256 MiB);
+    // hipFree waits for work still using it
     int reserve(size_t b)
     {
         if (p && b <= bytes) return 0;
-        return alloc(b + b / 4);
+        return alloc(b + std::min(b / 4, (size_t)256 << 20));
     }
     void release()
     {
@@ -68,14 +71,11 @@ inline int upload(DBuf &b, const void *h, size_t bytes, hipStream_t st)
     return 0;
 }
 
+int launch_iota1(int *id, int n, hipStream_t st);   // id[i] = i + 1 (midas_resid.hip)
+
 }  // namespace midas
 
 using namespace midas;
-
-struct midas_ctx {
-    int device = 0;
-    hipStream_t st = nullptr;
-};
 
 // Device-resident cell table with derived constants.
 struct DevCellsOwned {
@@ -129,17 +129,16 @@ struct DevCellsOwned {
         MIDAS_CHECK(c && c->n > 0 && c->x && c->y && c->z && c->sc && c->eps, -1, "dev cells: missing arrays");
         const int n = c->n;
         const size_t d = sizeof(double) * n, z2 = 2 * d;
-        TRY(kappa.alloc(d));
-        TRY(che.alloc(z2));
-        TRY(znnpp.alloc(z2));
-        TRY(kchi.alloc(z2));
-        TRY(a.alloc(d));
+        // grow-only: a DevCellsOwned kept by the context serves repeated calls
+        TRY(kappa.reserve(d));
+        TRY(che.reserve(z2));
+        TRY(znnpp.reserve(z2));
+        TRY(kchi.reserve(z2));
+        TRY(a.reserve(d));
         const int *ids = c->id;
         if (!ids) {
-            std::vector<int> h(n);
-            for (int i = 0; i < n; ++i) h[i] = i + 1;
-            TRY(upload(id, h.data(), sizeof(int) * n, st));
-            MIDAS_HIP(hipStreamSynchronize(st));
+            TRY(id.reserve(sizeof(int) * n));
+            TRY(launch_iota1(id.as<int>(), n, st));
             ids = id.as<int>();
         }
         TRY(launch_cell_params(n, c->sc, c->eps, k0, quirks, a.as<double>(), che.as<double>(), znnpp.as<double>(),
@@ -148,6 +147,36 @@ struct DevCellsOwned {
                      kchi.as<double>(), ids};
         return 0;
     }
+};
+
+// Device workspace of a context, grow-only, reused by every solve on it.
+struct SolveWs {
+    DBuf W;            // complex-FP32 copy of the matrix (mixed modes)
+    DBuf trail;        // split-precision operand planes of the trailing update
+    DBuf ipiv, perm;   // interchanges and the final permutation
+    DBuf norms;        // row sums, column maxima, scalars
+    DBuf zx;           // one-pass residual / row-sum scratch
+    DBuf R, D, t2;     // refinement: FP64 residual, FP32 correction, substitution scratch
+    DBuf xbest;        // refinement: iterate with the smallest backward error
+    DBuf zcopy;        // FP64 copy of Z when Z must be kept
+    DBuf V;            // excitation of the MoM solve
+    LuWork<float> luf;
+    LuWork<double> lud;
+    void release()
+    {
+        DBuf *b[] = {&W, &trail, &ipiv, &perm, &norms, &zx, &R, &D, &t2, &xbest, &zcopy, &V};
+        for (DBuf *x : b) x->release();
+        luf.release();
+        lud.release();
+    }
+};
+
+struct midas_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    SolveWs ws;
+    DevCellsOwned cells;          // cell constants of the device-pointer entry points
+    double cbfm_budget[4] = {};   // bytes; 0 = default (midas_set_cbfm_budgets)
 };
 
 struct Timer {

@@ -26,6 +26,10 @@ template <typename T> struct LuWork {
     cplx<T> *u12 = nullptr;          // nb x n staging for U12 = L11^-1 A12
     hipStream_t pstream = nullptr;   // panel stream (look-ahead), default priority
     hipEvent_t ev_panel = nullptr, ev_next = nullptr;
+    size_t cap[9] = {};              // bytes held by each buffer above (grow-only)
+    // Sizes the workspace for (n, nb, batch, sA): buffers grow when too small
+    // and are kept otherwise, so a LuWork reused for one size allocates once;
+    // the panel stream and its events are created on first use.
     int alloc(int n, int nb, int batch = 1, long sA = 0);
     void release();
     long inv_stride() const { return (long)((n + nb - 1) / nb) * nb * nb; }   // diagonal-block inverses per matrix

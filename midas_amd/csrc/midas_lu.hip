@@ -1187,28 +1187,44 @@ int LuWork<T>::alloc(int n_, int nb_, int batch_, long sA_)
     batch = batch_;
     sA = sA_;
     maxblk = (n + BLK_T - 1) / BLK_T + 1;   // workgroups of either panel kernel
-    MIDAS_HIP(hipMalloc(&pmax, 8 * 2 * maxblk * batch));   // FP32: 64-bit records of k_getf2_block
-    MIDAS_HIP(hipMalloc(&pidx, sizeof(int) * 2 * maxblk * batch));
-    MIDAS_HIP(hipMalloc(&counter, sizeof(int) * (2 * batch + 1 + maxblk * batch)));
+    // 16 bytes per candidate element: complex FP64, or complex FP32 as two tagged words
+    const size_t need[9] = {
+        (size_t)8 * 2 * maxblk * batch,                         // pmax (FP32: 64-bit records)
+        sizeof(int) * 2 * (size_t)maxblk * batch,               // pidx
+        sizeof(int) * (2 * (size_t)batch + 1 + (size_t)maxblk * batch),   // counter, info, fault, tags
+        (size_t)16 * 2 * maxblk * LU_IB * batch,                // cand
+        (size_t)16 * 2 * LU_IB * batch,                         // rowj
+        sizeof(cplx<T>) * batch,                                // pinv
+        sizeof(cplx<T>) * (size_t)inv_stride() * batch,         // linv
+        sizeof(cplx<T>) * (size_t)inv_stride() * batch,         // uinv
+        sizeof(cplx<T>) * (size_t)nb * n * batch,               // u12
+    };
+    void **buf[9] = {(void **)&pmax, (void **)&pidx, (void **)&counter, (void **)&cand, (void **)&rowj,
+                     (void **)&pinv, (void **)&linv, (void **)&uinv, (void **)&u12};
+    for (int i = 0; i < 9; ++i) {
+        if (*buf[i] && need[i] <= cap[i]) continue;
+        if (*buf[i]) MIDAS_HIP(hipFree(*buf[i]));
+        *buf[i] = nullptr;
+        cap[i] = 0;
+        MIDAS_HIP(hipMalloc(buf[i], need[i]));
+        cap[i] = need[i];
+    }
     info = counter + batch;
     fault = counter + 2 * batch;
     tag = (unsigned *)(counter + 2 * batch + 1);
-    // 16 bytes per element: complex FP64, or complex FP32 as two tagged words
-    MIDAS_HIP(hipMalloc(&cand, 16 * 2 * maxblk * LU_IB * batch));
-    MIDAS_HIP(hipMalloc(&rowj, 16 * 2 * LU_IB * batch));
-    MIDAS_HIP(hipMalloc(&pinv, sizeof(cplx<T>) * batch));
-    MIDAS_HIP(hipMalloc(&linv, sizeof(cplx<T>) * inv_stride() * batch));
-    MIDAS_HIP(hipMalloc(&uinv, sizeof(cplx<T>) * inv_stride() * batch));
-    MIDAS_HIP(hipMalloc(&u12, sizeof(cplx<T>) * (size_t)nb * n * batch));
-    MIDAS_HIP(hipStreamCreate(&pstream));
-    MIDAS_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
-    MIDAS_HIP(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming));
+    if (!pstream) MIDAS_HIP(hipStreamCreate(&pstream));
+    if (!ev_panel) MIDAS_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+    if (!ev_next) MIDAS_HIP(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming));
     return 0;
 }
 
 template <typename T>
 void LuWork<T>::release()
 {
+    if (pstream) {
+        hipStreamSynchronize(pstream);
+        hipStreamDestroy(pstream);
+    }
     hipFree(pmax);
     hipFree(pidx);
     hipFree(counter);
@@ -1218,10 +1234,6 @@ void LuWork<T>::release()
     hipFree(u12);
     hipFree(cand);
     hipFree(rowj);
-    if (pstream) {
-        hipStreamSynchronize(pstream);
-        hipStreamDestroy(pstream);
-    }
     if (ev_panel) hipEventDestroy(ev_panel);
     if (ev_next) hipEventDestroy(ev_next);
     pstream = nullptr;
@@ -1231,6 +1243,7 @@ void LuWork<T>::release()
     tag = nullptr;
     pinv = nullptr;
     linv = uinv = u12 = cand = rowj = nullptr;
+    for (size_t &c : cap) c = 0;
 }
 
 template struct LuWork<double>;

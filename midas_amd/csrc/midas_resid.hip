@@ -156,4 +156,17 @@ int launch_rowabs(const double2 *Z, long ldz, int n, double *rowsum, void *scrat
     return 0;
 }
 
+__global__ void k_iota1(int *id, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) id[i] = i + 1;
+}
+
+int launch_iota1(int *id, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_iota1, dim3((n + 255) / 256), dim3(256), 0, st, id, n);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
 }  // namespace midas

@@ -15,8 +15,12 @@
 !
 ! Environment: MIDAS_DEVICE (GPU ordinal, default 0) and MIDAS_SOLVER (fp64,
 ! c32, f16, f16x3, bf16, bf16x3; default fp64 = the reference's PZGESV
-! semantics).  Each MPI rank computes the whole system on its own GPU; the
-! intended use is one rank per GPU with the problem fitting one GPU's HBM.
+! semantics).  Each MPI rank computes the whole system on its own GPU and
+! returns the rows the reference's callers expect of that rank (the CBFM
+! outputs: its blocks MPI_CBFM_Blocks(rank+1, 2:), in that order); the intended
+! use is one rank per GPU with the problem fitting one GPU's HBM.  A
+! mixed-precision solve whose refinement misses its tolerance falls back to the
+! FP64 LU and says so.
 module midas_dropin_state
     use iso_c_binding
     use midas_gpu_c
@@ -59,6 +63,15 @@ contains
             error stop 1
         end if
     end subroutine midas_check
+
+    ! report a mixed-precision solve that needed the FP64 fallback
+    subroutine midas_report(inf, what)
+        type(midas_solve_info_t), intent(in) :: inf
+        character(len=*), intent(in) :: what
+        if (inf%status == MIDAS_STATUS_FP64_FALLBACK) &
+            write (*, '(a,a,a,es10.3,a)') 'MIDAS GPU: ', what, ': refinement stopped at backward error ', &
+                inf%berr, '; solved by the FP64 LU instead'
+    end subroutine midas_report
 end module midas_dropin_state
 
 
@@ -124,12 +137,14 @@ SUBROUTINE Compute_EFields_MoM(Cells,Transmitters,Receivers,S_total,C_ext,C_abs)
                                   int(NRx_tot, c_int), rth, rph, midas_mode, c_null_ptr, c_loc(S_total), &
                                   c_loc(C_ext), c_loc(C_abs), c_null_ptr, inf)
     call midas_check(r, 'Compute_EFields_MoM')
+    if (rank == 0) call midas_report(inf, 'Compute_EFields_MoM')
     ! the reference prints INFO / "SINGULAR MATRIX" and continues (Compute_EFields_MoM.f90:174-181)
     if (r > 0 .and. rank == 0) write (*, '(a,i10)') ' SINGULAR MATRIX, INFO = ', r
 END SUBROUTINE Compute_EFields_MoM
 
 
-SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,Transmitters,aca_on,threshold,E_total,K_b,C_b)
+SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,MPI_CBFM_Blocks,Transmitters,aca_on,threshold, &
+                             E_total,K_b,C_b)
     USE Initialization
     USE common_variables
     USE midas_dropin_state
@@ -138,6 +153,7 @@ SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,Transmitters,aca_
     type (Cell), Dimension(Nbc), INTENT(IN) :: Cells
     type (CBFM_Block), Dimension(Nblocks), INTENT(IN) :: CBFM_Blocks
     Integer, Dimension(Nblocks,Nbc_ext), INTENT(IN) :: CBFM_Blocks_Ext
+    Integer, Dimension(nber_procs,Nblk_proc_max+1), INTENT(IN):: MPI_CBFM_Blocks
     type (Dipole), Dimension(NTr), INTENT(IN) :: Transmitters
     integer, intent(in) :: aca_on
     real(real64), intent(in) :: threshold
@@ -159,7 +175,8 @@ SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,Transmitters,aca_
     type (Dipole), Dimension(:), allocatable :: tcbf
     integer(c_int), allocatable, target :: bstart(:), estart(:), ecells(:), kk(:)
     real(c_double), allocatable :: thc(:), phc(:), tth(:), tph(:)
-    integer :: b, i, kcap
+    COMPLEX(real64), Dimension(:,:), allocatable, TARGET :: E_all, C_all
+    integer :: b, i, kcap, job, r0, nrow, row
     integer(c_int) :: r, iws
 
     call midas_dropin_init()
@@ -184,19 +201,38 @@ SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,Transmitters,aca_
     end do
     bl = midas_blocks_t(int(Nblocks, c_int), c_loc(bstart), c_loc(estart), c_loc(ecells))
     kcap = 2 * size(tcbf)
-    allocate(E_total(3 * Nbc, 2 * NTr), C_b(3 * Nbc, kcap))
+    allocate(E_all(3 * Nbc, 2 * NTr), C_all(3 * Nbc, kcap))
     ! CBF excitation: type_iws in the dense variant (Compute_EFields_CBFME.f90:363-369);
     ! the ACA variant always uses plane waves (Compute_EFields_CBFME_ACA.f90:312)
     iws = int(type_iws, c_int)
     if (aca_on /= 0) iws = 1
     r = midas_compute_efields_cbfm(midas_context, cs, bl, k_0, MIDAS_Q_ALL, int(size(tcbf), c_int), thc, phc, &
                                    iws, int(NTr, c_int), tth, tph, int(aca_on, c_int), int(Nb_it_max, c_int), &
-                                   Epsilon_ACA, threshold, midas_mode, c_loc(E_total), int(3 * Nbc, c_long), kk, &
-                                   c_loc(C_b), int(3 * Nbc, c_long), int(kcap, c_int), int(SR, c_int), res_SR, &
+                                   Epsilon_ACA, threshold, midas_mode, c_loc(E_all), int(3 * Nbc, c_long), kk, &
+                                   c_loc(C_all), int(3 * Nbc, c_long), int(kcap, c_int), int(SR, c_int), res_SR, &
                                    c_null_ptr, c_null_ptr, inf)
     call midas_check(r, 'Compute_EFields_CBFME')
+    if (rank == 0) call midas_report(inf, 'Compute_EFields_CBFME')
     if (r > 0 .and. rank == 0) write (*, '(a,i10)') ' SINGULAR MATRIX, INFO = ', r
-    K_b = kk
+    ! this rank's rows: its blocks MPI_CBFM_Blocks(rank+1, 2:) in that order, as
+    ! E_total(3*Nbc_proc, 2*NTr), K_patchs(MyNBlocks) and C_job_patchs(3*Nbc_proc, :)
+    ! (Compute_EFields_CBFME.f90:154-163, 443-444, 990-1006)
+    allocate(E_total(3 * Nbc_proc, 2 * NTr), C_b(3 * Nbc_proc, kcap), K_b(MPI_CBFM_Blocks(rank + 1, 1)))
+    C_b = (0d0, 0d0)
+    row = 1
+    do job = 1, MPI_CBFM_Blocks(rank + 1, 1)
+        b = MPI_CBFM_Blocks(rank + 1, 1 + job)
+        r0 = 3 * bstart(b) + 1
+        nrow = 3 * CBFM_Blocks(b)%Nbc_b
+        E_total(row:row + nrow - 1, :) = E_all(r0:r0 + nrow - 1, :)
+        C_b(row:row + nrow - 1, :) = C_all(r0:r0 + nrow - 1, :)
+        K_b(job) = kk(b)
+        row = row + nrow
+    end do
+    if (row /= 3 * Nbc_proc + 1) then
+        write (*, '(a)') 'MIDAS GPU: MPI_CBFM_Blocks of this rank do not hold Nbc_proc cells'
+        error stop 1
+    end if
 END SUBROUTINE midas_cbfm_common
 
 
@@ -215,12 +251,14 @@ SUBROUTINE Compute_EFields_CBFME(Cells,CBFM_Blocks,CBFM_Blocks_Ext,MPI_CBFM_Bloc
     COMPLEX(real64), Dimension(:,:), allocatable, INTENT(OUT):: E_total
 
     interface
-        SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,Transmitters,aca_on,threshold,E_total,K_b,C_b)
+        SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,MPI_CBFM_Blocks,Transmitters,aca_on,threshold, &
+                             E_total,K_b,C_b)
             USE Initialization
             USE common_variables
             type (Cell), Dimension(Nbc), INTENT(IN) :: Cells
             type (CBFM_Block), Dimension(Nblocks), INTENT(IN) :: CBFM_Blocks
             Integer, Dimension(Nblocks,Nbc_ext), INTENT(IN) :: CBFM_Blocks_Ext
+            Integer, Dimension(nber_procs,Nblk_proc_max+1), INTENT(IN):: MPI_CBFM_Blocks
             type (Dipole), Dimension(NTr), INTENT(IN) :: Transmitters
             integer, intent(in) :: aca_on
             real(real64), intent(in) :: threshold
@@ -239,7 +277,8 @@ SUBROUTINE Compute_EFields_CBFME(Cells,CBFM_Blocks,CBFM_Blocks_Ext,MPI_CBFM_Bloc
     vals = abs(Cells(1:Nbc)%m_n)
     m_max_relative = ceiling(maxval(vals)) - 2.0
     Threshold_CBFM = max(10.0**(-3.0-m_max_relative),1e-10)
-    call midas_cbfm_common(Cells, CBFM_Blocks, CBFM_Blocks_Ext, Transmitters, 0, Threshold_CBFM, E_total, K_b, C_b)
+    call midas_cbfm_common(Cells, CBFM_Blocks, CBFM_Blocks_Ext, MPI_CBFM_Blocks, Transmitters, 0, Threshold_CBFM, &
+                           E_total, K_b, C_b)
     if (allocated(K_patchs)) deallocate(K_patchs)
     if (allocated(C_job_patchs)) deallocate(C_job_patchs)
     K_patchs = K_b
@@ -259,12 +298,14 @@ SUBROUTINE Compute_EFields_CBFME_ACA(Cells,CBFM_Blocks,CBFM_Blocks_Ext,MPI_CBFM_
     COMPLEX(real64), Dimension(:,:), allocatable, INTENT(OUT):: E_total
 
     interface
-        SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,Transmitters,aca_on,threshold,E_total,K_b,C_b)
+        SUBROUTINE midas_cbfm_common(Cells,CBFM_Blocks,CBFM_Blocks_Ext,MPI_CBFM_Blocks,Transmitters,aca_on,threshold, &
+                             E_total,K_b,C_b)
             USE Initialization
             USE common_variables
             type (Cell), Dimension(Nbc), INTENT(IN) :: Cells
             type (CBFM_Block), Dimension(Nblocks), INTENT(IN) :: CBFM_Blocks
             Integer, Dimension(Nblocks,Nbc_ext), INTENT(IN) :: CBFM_Blocks_Ext
+            Integer, Dimension(nber_procs,Nblk_proc_max+1), INTENT(IN):: MPI_CBFM_Blocks
             type (Dipole), Dimension(NTr), INTENT(IN) :: Transmitters
             integer, intent(in) :: aca_on
             real(real64), intent(in) :: threshold
@@ -279,7 +320,8 @@ SUBROUTINE Compute_EFields_CBFME_ACA(Cells,CBFM_Blocks,CBFM_Blocks_Ext,MPI_CBFM_
     Integer, Dimension(:), allocatable :: K_b
 
     Threshold_CBFM = 1e-3      ! Compute_EFields_CBFME_ACA.f90:124 (a single literal)
-    call midas_cbfm_common(Cells, CBFM_Blocks, CBFM_Blocks_Ext, Transmitters, 1, Threshold_CBFM, E_total, K_b, C_b)
+    call midas_cbfm_common(Cells, CBFM_Blocks, CBFM_Blocks_Ext, MPI_CBFM_Blocks, Transmitters, 1, Threshold_CBFM, &
+                           E_total, K_b, C_b)
 END SUBROUTINE Compute_EFields_CBFME_ACA
 
 

@@ -18,15 +18,20 @@ module midas_gpu_c
         type(c_ptr) :: x, y, z, sc, eps, kappa, che, znnpp, id
     end type midas_cells_t
 
+    integer(c_int), parameter :: MIDAS_STATUS_CONVERGED = 0, MIDAS_STATUS_NOT_CONVERGED = 1, &
+                                 MIDAS_STATUS_FP64_FALLBACK = 2, MIDAS_ERR_NOT_CONVERGED = -4
+
     type, bind(c) :: midas_solve_opts_t
         integer(c_int) :: max_iter
         real(c_double) :: tol
         integer(c_int) :: nb
+        integer(c_int) :: no_fallback
     end type midas_solve_opts_t
 
     type, bind(c) :: midas_solve_info_t
         integer(c_int) :: info, iterations
         real(c_double) :: berr, anorm, t_factor_ms, t_solve_ms, t_refine_ms
+        integer(c_int) :: status
     end type midas_solve_info_t
 
     type, bind(c) :: midas_blocks_t

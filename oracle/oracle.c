@@ -29,8 +29,12 @@
  *   or_ff_coeff           : reference GetFFieldCoeff in oracle/_ref.
  *   or_scattering_matrix  : beta=0 reduces to the MoM projection; Mie series
  *                           for the sphere (physics, ~1e-2); optical theorem.
- *   or_cext_cabs          : hand-computed 1-cell case; optical theorem for the
- *                           corrected (dot-product) variant.
+ *   or_cext_cabs          : dot-product variant (q6 = 0): Mie series, energy
+ *                           balance and optical theorem; component-sum variant
+ *                           (q6 = 1, the reference's formula): identities it
+ *                           fixes against the dot product (E = (a, a, a) gives
+ *                           3x Cext and 3x Cabs, E = (a, -a, 0) gives 0), and a
+ *                           one-cell case at the origin worked by hand.
  *   or_aca                : reference Calcul_MatrixZij_ACA in oracle/_ref
  *                           (pivots, rank, U, V).
  *   or_aca_gen            : reference Calcul_GenMatrix_ACA in oracle/_ref.

@@ -657,11 +657,14 @@ def cbf_threshold(eps_cells, aca_variant):
 
 
 def cbfm(cells: Cells, blocks, ext, th_cbf, ph_cbf, th, ph, use_aca=False, kmax=50, eps_aca=1e-4,
-         threshold=None, fp32_blocks=True, sr=False, res_sr=0.01, fsr_out=None, iws=1):
+         threshold=None, fp32_blocks=True, sr=False, res_sr=0.01, fsr_out=None, iws=1, transpose="T"):
     """Dense-block CBFM-E (Compute_EFields_CBFME.f90:348-449, 630-849, 896-1006)
     or its ACA variant (Compute_EFields_CBFME_ACA.f90:296-385, 569-636).
     iws: CBF excitation, 1 plane waves, 2 / 3 spherical waves (type_iws,
     Compute_EFields_CBFME.f90:363-369).
+    transpose: "T" is the reference's plain transpose C_i^T in Zc and Vc
+    (ZGEMM 'T', Compute_EFields_CBFME.f90:666-669, 844-847); "H" (conjugate
+    transpose) exists only so that tests can show their checks tell the two apart.
 
     blocks: list of 0-based cell-index arrays (contiguous ranges in block order);
     ext: list of extension cell-index arrays.  Returns (E, K per block, C list).
@@ -685,6 +688,9 @@ def cbfm(cells: Cells, blocks, ext, th_cbf, ph_cbf, th, ph, use_aca=False, kmax=
         K = int(np.sum(s / s[0] >= threshold))
         Cs.append(np.asfortranarray(U[: 3 * len(b), :K]))
         Ks.append(K)
+    if transpose not in ("T", "H"):
+        raise ValueError(transpose)
+    tr = (lambda M: M.T) if transpose == "T" else (lambda M: M.conj().T)
     offs = np.concatenate([[0], np.cumsum(Ks)])
     Kt = offs[-1]
     Zc = np.zeros((Kt, Kt), np.complex128)
@@ -695,13 +701,13 @@ def cbfm(cells: Cells, blocks, ext, th_cbf, ph_cbf, th, ph, use_aca=False, kmax=
             if use_aca and i != j:
                 k, U, V, _, _ = aca(cells, int(bi[0]), len(bi), int(bj[0]), len(bj), kmax, eps_aca)
                 if k < kmax:
-                    Zc[offs[i]:offs[i + 1], offs[j]:offs[j + 1]] = Cs[i].T @ (U @ (V @ Cs[j]))
+                    Zc[offs[i]:offs[i + 1], offs[j]:offs[j + 1]] = tr(Cs[i]) @ (U @ (V @ Cs[j]))
                     continue
             Zij = block_fill(ci, cj, fp32=fp32_blocks)
-            Zc[offs[i]:offs[i + 1], offs[j]:offs[j + 1]] = Cs[i].T @ Zij @ Cs[j]
+            Zc[offs[i]:offs[i + 1], offs[j]:offs[j + 1]] = tr(Cs[i]) @ Zij @ Cs[j]
     Vc = np.zeros((Kt, 2 * len(th)), np.complex128)
     for i, bi in enumerate(blocks):
-        Vc[offs[i]:offs[i + 1]] = Cs[i].T @ incident_field_f32(cells.subset(bi), th, ph)
+        Vc[offs[i]:offs[i + 1]] = tr(Cs[i]) @ incident_field_f32(cells.subset(bi), th, ph)
     alpha = np.linalg.solve(Zc, Vc)
     E = np.zeros((3 * cells.n, 2 * len(th)), np.complex128)
     for i, bi in enumerate(blocks):

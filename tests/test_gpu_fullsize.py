@@ -1,0 +1,169 @@
+"""GPU parity at the full sizes of the four GPU workloads.
+
+  C2  Gaussian random sphere, x = 6, 19,983 cells (59,949 unknowns, a 57.5 GB
+      complex128 matrix): the complex-FP32 LU + FP64 refinement and the FP16
+      split LU + FP64 refinement against the FP64 LU on the GPU;
+  C3  DLCA aggregate of 64 monomers, 19,160 cells, CBFM-E with SR blocks and
+      ACA: against the oracle's solution of the same workload
+      (tests/golden/c3_dlca64.npz, written by make_c3_golden.py);
+  C4  convex hull, x = 5, 12,016 cells, 1,000 right-hand sides: mixed vs FP64
+      for the fields, the S matrix and the orientation-averaged Q;
+  C5  porous particle, x = 12, 150,088 cells in 400 blocks, CBFM-E + ACA (the
+      ACA runs in several batches at this size), 400 right-hand sides: mixed
+      vs FP64 reduced solve.
+
+At these sizes the oracle cannot solve the MoM systems on the host, so
+besides mixed-vs-FP64 the MoM cases check what does not depend on size: the
+backward error of the returned field on a sample of matrix rows built on the
+host side of the library (block fill, then numpy), and Cext from the internal
+field against the forward-scattering amplitude (optical theorem, a few %)."""
+import os
+
+import numpy as np
+import pytest
+
+from midas_amd import api, geometry as g
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def mid():
+    m = api.Midas(0)
+    yield m
+    m.close()
+
+
+def _case(name):
+    print(f"\n[case] {name}", flush=True)
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def _sampled_berr(mid, s, E, tth, tph, anorm, ncells=64, seed=0):
+    """max_j ||(V - Z E)_rows||_inf / (||Z||_inf ||E_j||_inf + ||V_j||_inf) over
+    the rows of `ncells` random cells: Z's rows from the block fill (FP64
+    elements, the MoM fill's), V from the FP64 plane wave, the product in numpy."""
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(s.n, ncells, replace=False))
+    cells = api.CellTable.from_scatterer(s)
+    q = api.Q_ALL & ~api.Q_FP32_BLOCKS
+    Zs = mid.green_s_tr_partial(cells.subset(idx), cells, s.k0, quirks=q)
+    V = mid.plane_wave(cells.subset(idx), s.k0, tth, tph, quirks=q)
+    R = V - Zs @ E
+    b = 0.0
+    for j in range(E.shape[1]):
+        b = max(b, np.abs(R[:, j]).max() / (anorm * np.abs(E[:, j]).max() + np.abs(V[:, j]).max()))
+    return b
+
+
+def test_c2_mixed_vs_fp64_full_size(mid):
+    _case("C2 building the Gaussian random sphere (~20,000 cells)")
+    s = g.gaussian_random_sphere(size_param=6.0, target_cells=20000, eps=1.5 + 0.01j, seed=2)
+    n3 = 3 * s.n
+    tth, tph = np.array([0.0]), np.array([0.0])
+    rth, rph = np.array([0.0, 90.0, 180.0]), np.array([0.0, 0.0, 0.0])
+    res = {}
+    for mode in (api.SOLVE_F16_IR, api.SOLVE_C32_IR, api.SOLVE_FP64):
+        _case(f"C2 {api.MODE_NAMES[mode]} n={n3}")
+        S, ce, ca, E, info = mid.compute_efields_mom(s, s.k0, (tth, tph), (rth, rph), mode=mode)
+        print(info, flush=True)
+        res[mode] = (S, ce, ca, E, info)
+    mid.release_workspace()
+    E64 = res[api.SOLVE_FP64][3]
+    for mode in (api.SOLVE_F16_IR, api.SOLVE_C32_IR):
+        S, ce, ca, E, info = res[mode]
+        err = _rel(E, E64)
+        print(api.MODE_NAMES[mode], "field err vs FP64 LU", err, "S err", _rel(S, res[api.SOLVE_FP64][0]),
+              flush=True)
+        assert info["info"] == 0 and info["status"] == api.STATUS_CONVERGED
+        assert info["berr"] <= 2.7e-14          # max(1e-14, sqrt(n) eps) at n = 59,949
+        assert err <= 1e-6                      # the bar; expected ~1e-14
+        assert err <= 1e-12
+        assert _rel(S, res[api.SOLVE_FP64][0]) <= 1e-12
+        assert abs(ce[0] - res[api.SOLVE_FP64][1][0]) <= 1e-12 * abs(ce[0])
+    anorm = res[api.SOLVE_F16_IR][4]["anorm"]
+    for mode in (api.SOLVE_F16_IR, api.SOLVE_FP64):
+        b = _sampled_berr(mid, s, res[mode][3], tth, tph, anorm)
+        print(api.MODE_NAMES[mode], "sampled backward error", b, flush=True)
+        assert b <= 1e-14
+    # optical theorem: Cext (internal field) = (4 pi / k^2) Im S_Vv(forward) (MIDAS's S
+    # convention, the same relation the oracle meets on the Mie sphere)
+    S, ce = res[api.SOLVE_F16_IR][0], res[api.SOLVE_F16_IR][1]
+    ot = 4 * np.pi / s.k0 ** 2 * S[0, 0].imag
+    print("Cext", ce[0].real, "optical theorem", ot, flush=True)
+    assert ot == pytest.approx(ce[0].real, rel=5e-2)
+
+
+def test_c4_multi_rhs_full_size(mid):
+    s = g.convex_hull_particle(size_param=5.0, target_cells=12000, seed=4)
+    tth, tph = g.directions(500)
+    gth = np.arange(0.0, 180.25, 0.5)
+    rth = np.r_[tth, 180.0 - tth, np.tile(gth, 2)]
+    rph = np.r_[tph, np.mod(tph + 180.0, 360.0), np.repeat([0.0, 90.0], len(gth))]
+    dm = 2.0 * (3.0 * s.n * s.sc[0] ** 3 / (4.0 * np.pi)) ** (1.0 / 3.0)
+    out = {}
+    for mode in (api.SOLVE_F16_IR, api.SOLVE_FP64):
+        _case(f"C4 {api.MODE_NAMES[mode]} n={3 * s.n}, {2 * len(tth)} RHS, {len(rth)} receivers")
+        S, ce, ca, E, info = mid.compute_efields_mom(s, s.k0, (tth, tph), (rth, rph), mode=mode)
+        Q, av = mid.scattering_quantities(S, len(tth), len(tth), (tth, tph), (tth, tph), s.k0, dm, ce, ca)
+        print(info, "Q_av", av, flush=True)
+        out[mode] = (S, ce, ca, E, info, av)
+    mid.release_workspace()
+    S, ce, ca, E, info, av = out[api.SOLVE_F16_IR]
+    S64, ce64, ca64, E64, _, av64 = out[api.SOLVE_FP64]
+    print("field err", _rel(E, E64), "S err", _rel(S, S64), "Q_av err", _rel(av, av64), flush=True)
+    assert info["status"] == api.STATUS_CONVERGED and info["berr"] <= 1e-14
+    assert _rel(E, E64) <= 1e-12
+    assert _rel(S, S64) <= 1e-12
+    assert _rel(ce, ce64) <= 1e-12 and _rel(ca, ca64) <= 1e-12
+    assert _rel(av, av64) <= 1e-12
+    b = _sampled_berr(mid, s, E, tth, tph, info["anorm"], ncells=32)
+    print("sampled backward error", b, flush=True)
+    assert b <= 1e-14
+    # averaged extinction = scattering + absorption (the averaged Q of this build)
+    assert av[0] == pytest.approx(av[2] + av[1], rel=3e-2)
+
+
+def test_c3_dlca64_vs_oracle_fixture(mid):
+    d = np.load(os.path.join(GOLD, "c3_dlca64.npz"), allow_pickle=False)
+    s = g.dlca_aggregate(nmono=64, cells_per_mono=300, seed=3)
+    th_c, ph_c = g.cbf_directions_uniform(422)
+    assert s.n == int(d["n_cells"]) and len(th_c) == int(d["n_cbf_dirs"])
+    tth, tph = np.array([0.0]), np.array([0.0])
+    for mode in (api.SOLVE_FP64, api.SOLVE_F16_IR):
+        _case(f"C3 DLCA 64 monomers n={3 * s.n} SR + ACA, {api.MODE_NAMES[mode]} reduced solve")
+        E, K, info = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True,
+                                               sr=True, res_sr=0.01, mode=mode)
+        print("K total", int(np.sum(K)), "oracle", int(np.sum(d["K"])), "E err", _rel(E, d["E"]), info, flush=True)
+        assert list(info["fsr"]) == list(d["fsr"])
+        assert list(K) == list(d["K"])
+        assert _rel(E, d["E"]) <= 1e-9
+
+
+def test_c5_porous_full_size(mid):
+    _case("C5 building the porous particle (150,088 cells, 400 blocks)")
+    s = g.porous_particle()
+    th_c, ph_c = g.cbf_directions_uniform(94)
+    tth, tph = g.directions(200)
+    out = {}
+    for mode in (api.SOLVE_F16X3_IR, api.SOLVE_FP64):
+        _case(f"C5 CBFM-E + ACA n={3 * s.n}, {len(s.blocks)} blocks, {2 * len(tth)} RHS, "
+              f"{api.MODE_NAMES[mode]} reduced solve")
+        E, K, info = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=True,
+                                               mode=mode)
+        print("K total", int(np.sum(K)), info, flush=True)
+        out[mode] = (E, K, info)
+        mid.release_workspace()
+    E, K, info = out[api.SOLVE_F16X3_IR]
+    E64, K64, _ = out[api.SOLVE_FP64]
+    err = _rel(E, E64)
+    print("field err vs FP64 reduced solve", err, flush=True)
+    assert list(K) == list(K64)
+    assert info["status"] == api.STATUS_CONVERGED
+    assert err <= 1e-6
+    assert err <= 1e-11

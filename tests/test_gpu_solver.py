@@ -1,0 +1,148 @@
+"""GPU checks of the solver's control paths: the outcome of iterative
+refinement (converged, FP64 fallback, or reported non-convergence with the
+best iterate), the context's reusable workspace across sizes, and the CBFM
+batching budgets forced down to one item per batch (every batched branch then
+runs many batches) against the oracle."""
+import numpy as np
+import pytest
+
+from midas_amd import api, geometry as g
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mid():
+    m = api.Midas(0)
+    yield m
+    m.close()
+
+
+def _case(name):
+    print(f"\n[case] {name}", flush=True)
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def _berr(A, X, B):
+    """max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)"""
+    R = B - A @ X
+    an = np.abs(A).sum(1).max()
+    return max(np.abs(R[:, j]).max() / (an * np.abs(X[:, j]).max() + np.abs(B[:, j]).max())
+               for j in range(B.shape[1]))
+
+
+def _ill_conditioned(n, cond, seed):
+    rng = np.random.default_rng(seed)
+    U, _ = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    V, _ = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    A = (U * np.logspace(0, -np.log10(cond), n)) @ V.conj().T
+    B = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    return A, B
+
+
+def test_refinement_converges_well_conditioned(mid):
+    rng = np.random.default_rng(3)
+    n = 700
+    _case(f"refinement status converged, f16, n={n}")
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 30 * np.eye(n)
+    B = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    X, info = mid.solve(A, B, api.SOLVE_F16_IR, no_fallback=True)
+    print(info)
+    assert info["status"] == api.STATUS_CONVERGED
+    assert info["berr"] <= 1e-14
+    assert _berr(A, X, B) <= 1e-14
+
+
+def test_refinement_falls_back_to_fp64(mid):
+    """cond(A) = 1e7 with BF16 trailing updates (unit roundoff 2^-8) and 64-wide
+    panels: refinement cannot converge (cond x u >> 1).  By default the solve
+    then takes the FP64 LU of A, as LAPACK's ZCGESV does, and says so."""
+    n = 600
+    A, B = _ill_conditioned(n, 1e7, 5)
+    _case(f"refinement fallback, bf16 nb=64, cond 1e7, n={n}")
+    X, info = mid.solve(A, B, api.SOLVE_BF16_IR, nb=64)
+    print(info)
+    Xo = np.linalg.solve(A, B)
+    assert info["status"] == api.STATUS_FP64_FALLBACK
+    assert info["info"] == 0
+    assert _berr(A, X, B) < 1e-14
+    assert _rel(X, Xo) < 1e-6          # cond x eps ~ 1e-9
+
+
+def test_refinement_reports_non_convergence(mid):
+    """The same system with no_fallback: the call reports MIDAS_ERR_NOT_CONVERGED
+    (NotConverged here) and returns the iterate with the smallest backward
+    error, whose backward error it reports."""
+    n = 600
+    A, B = _ill_conditioned(n, 1e7, 5)
+    _case(f"refinement not converged, bf16 nb=64, cond 1e7, n={n}, no_fallback")
+    with pytest.raises(api.NotConverged) as e:
+        mid.solve(A, B, api.SOLVE_BF16_IR, nb=64, no_fallback=True)
+    X, info = e.value.result
+    print(str(e.value), info)
+    assert info["status"] == api.STATUS_NOT_CONVERGED
+    assert info["info"] == api.ERR_NOT_CONVERGED
+    assert info["berr"] > 1e-14
+    assert np.all(np.isfinite(X))
+    # the reported backward error is the returned iterate's
+    assert _berr(A, X, B) == pytest.approx(info["berr"], rel=1e-6)
+
+
+def test_workspace_reused_across_sizes(mid):
+    """One context solving n = 1301, then 701, then 1301 again (the workspace
+    grows once and is reused smaller) gives what fresh contexts give."""
+    rng = np.random.default_rng(21)
+    sys = {}
+    for n in (1301, 701):
+        A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 40 * np.eye(n)
+        sys[n] = (A, rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3)))
+    _case("workspace reuse f16: 1301, 701, 1301")
+    X1, _ = mid.solve(*sys[1301], api.SOLVE_F16_IR)
+    X2, _ = mid.solve(*sys[701], api.SOLVE_F16_IR)
+    X3, _ = mid.solve(*sys[1301], api.SOLVE_F16_IR)
+    mid.release_workspace()
+    X4, _ = mid.solve(*sys[701], api.SOLVE_F16_IR)
+    fresh = api.Midas(0)
+    try:
+        F1, _ = fresh.solve(*sys[1301], api.SOLVE_F16_IR)
+        F2, _ = fresh.solve(*sys[701], api.SOLVE_F16_IR)
+    finally:
+        fresh.close()
+    print("1301 again vs first", _rel(X3, X1), "701 vs fresh", _rel(X2, F2))
+    assert _rel(X1, F1) < 1e-15 and _rel(X3, F1) < 1e-15
+    assert _rel(X2, F2) < 1e-15 and _rel(X4, F2) < 1e-15
+    assert _rel(X1, np.linalg.solve(*sys[1301])) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def dlca6():
+    return g.dlca_aggregate(nmono=6, cells_per_mono=40, seed=5)
+
+
+@pytest.mark.parametrize("use_aca", [False, True])
+def test_cbfm_budgets_force_batches(mid, oracle, dlca6, use_aca):
+    """Every batching budget of the CBFM set to one byte: each block LU chunk
+    holds one block, each column strip one block column, each dense batch one
+    block pair and each ACA batch one block column, so every batched branch
+    runs as many batches as it has items.  Same K and field as the oracle and
+    as the default budgets (one batch each at this size)."""
+    s = dlca6
+    _case(f"cbfm budgets 1 byte use_aca={use_aca} n={3 * s.n} blocks={len(s.blocks)} "
+          f"sizes={[len(b) for b in s.blocks]}")
+    oc = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    th_c, ph_c = g.directions(30)
+    tth, tph = np.array([0.0, 35.0]), np.array([0.0, 80.0])
+    E0, K0, _ = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=use_aca)
+    mid.set_cbfm_budgets(1, 1, 1, 1)
+    try:
+        E1, K1, _ = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), use_aca=use_aca)
+    finally:
+        mid.set_cbfm_budgets(0, 0, 0, 0)
+    Eo, Ko, _ = oracle.cbfm(oc, s.blocks, s.ext, th_c, ph_c, tth, tph, use_aca=use_aca)
+    print("K", list(K1), "oracle", Ko, "batched vs default", _rel(E1, E0), "vs oracle", _rel(E1, Eo))
+    assert list(K1) == list(K0) == list(Ko)
+    assert _rel(E1, E0) < 1e-12
+    assert _rel(E1, Eo) < 1e-9
