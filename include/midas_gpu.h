@@ -330,13 +330,28 @@ int midas_extend_blocks(const midas_cells_t *cells, int nblocks, const int *bloc
 
 /* ---------------- device-pointer variants (inputs resident in HBM) ----------------
  * Same semantics as above; the pointers named below are device memory; ordered
- * on midas_stream(ctx).  Z may be NULL in midas_dev_mom_solve when only the
- * FP32 copy is needed (mixed modes then fill W directly). */
+ * on midas_stream(ctx).  midas_dev_mom_solve fills Z (FP64, the refinement's
+ * residual matrix) and overwrites it with its LU factors in FP64 mode and in
+ * the FP64 fallback. */
 int midas_dev_mom_fill(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, double *Z, long ldz,
                        float *W32, long ldw);
 int midas_dev_mom_solve(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, int ntr,
                         const double *dth, const double *dph, int mode, const midas_solve_opts_t *opts, double *Z,
                         long ldz, double *X, long ldx, midas_solve_info_t *info, double *t_fill_ms);
+/* midas_dev_mom_solve_noalloc: midas_dev_mom_solve that allocates no device
+ * memory, for callers that cannot (code under stream capture, tight serving
+ * loops, harnesses that forbid it).  No counterpart in the reference.  Every
+ * work buffer comes from a static device array of 76 MiB that each GPU the
+ * library runs on holds until the process ends; midas_init finds its address.
+ * Limits: 3n <= 2048 unknowns, 2 ntr <= 32 right-hand sides (larger problems
+ * fail with an error and no work done).  It runs the same kernels with
+ * 256-column panels (128 in FP64 and in the FP64 fallback), without the
+ * look-ahead stream, and only this library's GEMM kernels (no rocBLAS);
+ * t_fill_ms is not measured (no timing events).  All calls on one GPU share
+ * the array: calls that overlap on one GPU race. */
+int midas_dev_mom_solve_noalloc(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, int ntr,
+                                const double *dth, const double *dph, int mode, const midas_solve_opts_t *opts,
+                                double *Z, long ldz, double *X, long ldx, midas_solve_info_t *info);
 /* midas_dev_compute_efields_cbfm: midas_compute_efields_cbfm with the fields
  * E_dev (3n x 2ntr, ld lde) in device memory, so they stay in HBM
  * (Compute_EFields_CBFME.f90 / Compute_EFields_CBFME_ACA.f90, the same

@@ -43,7 +43,7 @@ EXPORTS = (
     "midas_dev_mom_fill", "midas_dev_mom_solve", "midas_dev_compute_efields_cbfm", "midas_profile", "midas_profile_query", "midas_block_frames",
     "midas_extend_blocks", "midas_scattering_quantities", "midas_near_field", "midas_partition_sph",
     "midas_partition_csh", "midas_scattering_quantities_grid", "midas_quad_adaptive", "midas_quad_trap2d",
-    "midas_quad_gauss_legendre", "midas_release_workspace", "midas_set_cbfm_budgets",
+    "midas_quad_gauss_legendre", "midas_release_workspace", "midas_set_cbfm_budgets", "midas_dev_mom_solve_noalloc",
 )
 
 RX_SD, RX_RF, RX_LB = 0, 1, 2

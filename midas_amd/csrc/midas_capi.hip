@@ -13,6 +13,13 @@
 #include <cstring>
 #include <vector>
 
+// Device scratch of the allocation-free entry point midas_dev_mom_solve_noalloc:
+// every device the library runs on holds one copy until the process ends.  In
+// host code `scratch` only names the symbol; its device address comes from
+// hipGetSymbolAddress (midas_init, per device) and is what host code passes on.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)76 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 namespace midas {
 
 static thread_local char g_err[1024] = "";
@@ -147,14 +154,14 @@ static int make_perm(const int *ipiv_d, int *perm_d, int n, hipStream_t st)
 }
 
 // FP64 LU of A (in place) and X = A^-1 B with it (PZGESV semantics).
-static int fp64_lu_solve(midas_ctx *ctx, int n, int nrhs, double2 *A, long lda, const double2 *B, long ldb, double2 *X,
-                         long ldx, int nb, midas_solve_info_t &inf)
+static int fp64_lu_solve(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, double2 *A, long lda, const double2 *B,
+                         long ldb, double2 *X, long ldx, int nb, midas_solve_info_t &inf)
 {
     hipStream_t st = ctx->st;
-    SolveWs &ws = ctx->ws;
     LuWork<double> &w = ws.lud;
     TRY(w.alloc(n, nb));
     TrailingGemm<double> tg;
+    tg.library = !ws.lent;
     int r = getrf<double>(A, lda, n, ws.ipiv.as<int>(), w, st, tg);
     if (r) return r;
     MIDAS_HIP(hipMemcpyAsync(&inf.info, w.info, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -169,11 +176,11 @@ static int fp64_lu_solve(midas_ctx *ctx, int n, int nrhs, double2 *A, long lda, 
 
 // R = B - Z X, the column maxima of X and R to the host, and the normwise
 // backward error max_j ||r_j|| / (||Z|| ||x_j|| + ||b_j||) (infinity norms).
-static int residual_berr(midas_ctx *ctx, int n, int nrhs, const double2 *Z, long ldz, const double2 *B, long ldb,
-                         const double2 *X, long ldx, double anorm, const std::vector<double> &hb, double *berr)
+static int residual_berr(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, const double2 *Z, long ldz, const double2 *B,
+                         long ldb, const double2 *X, long ldx, double anorm, const std::vector<double> &hb,
+                         double *berr)
 {
     hipStream_t st = ctx->st;
-    SolveWs &ws = ctx->ws;
     double *colx = ws.norms.as<double>() + n, *colr = colx + nrhs;
     double2 *R = ws.R.as<double2>();
     const dim3 g2((n + 255) / 256, nrhs);
@@ -183,7 +190,7 @@ static int residual_berr(midas_ctx *ctx, int n, int nrhs, const double2 *Z, long
     } else {
         hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R, (long)n, n, nrhs);
         // many right-hand sides: rocBLAS (33k x 448 x 33k: 60 ms against 74 ms for k_cgemm<double>)
-        r = nrhs >= 32 ? blas_zgemm_nn(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R, n, 0, 1, st)
+        r = nrhs >= 32 && !ws.lent ? blas_zgemm_nn(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R, n, 0, 1, st)
                        : launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R, n,
                                               st);
     }
@@ -212,10 +219,10 @@ static int residual_berr(midas_ctx *ctx, int n, int nrhs, const double2 *Z, long
 // ctx->ws and only grow.
 int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32, long ldw, const double2 *B,
               long ldb, double2 *X, long ldx, int mode, const midas_solve_opts_t *opts, midas_solve_info_t *info,
-              bool z_inplace)
+              bool z_inplace, SolveWs *wsp)
 {
     hipStream_t st = ctx->st;
-    SolveWs &ws = ctx->ws;
+    SolveWs &ws = wsp ? *wsp : ctx->ws;
     midas_solve_info_t inf{};
     const int max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 30;
     const double tol = (opts && opts->tol > 0) ? opts->tol : std::max(1e-14, std::sqrt((double)n) * DBL_EPSILON);
@@ -238,7 +245,7 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
     MIDAS_HIP(hipGetLastError());
     TRY(copy_sync(&inf.anorm, scal, sizeof(double), hipMemcpyDeviceToHost, st));
 
-    Timer tm(st);
+    Timer tm(st, !ws.lent);
     if (mode == MIDAS_SOLVE_FP64) {
         double2 *A = Z;
         long lda = ldz;
@@ -248,7 +255,7 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
             lda = n;
             hipLaunchKernelGGL(k_copyz, dim3((n + 255) / 256, n), dim3(256), 0, st, Z, ldz, A, lda, n, n);
         }
-        int r = fp64_lu_solve(ctx, n, nrhs, A, lda, B, ldb, X, ldx, nb, inf);
+        int r = fp64_lu_solve(ctx, ws, n, nrhs, A, lda, B, ldb, X, ldx, nb, inf);
         if (r) return r;
         inf.t_factor_ms = tm.ms();
         inf.status = MIDAS_STATUS_CONVERGED;
@@ -298,7 +305,7 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
         if (r) return r;
         hipLaunchKernelGGL(k_c2z_add, g2, dim3(256), 0, st, D, (long)n, X, ldx, n, nrhs, it > 0);
         if (it == 0) inf.t_solve_ms = tm.ms();
-        TRY(residual_berr(ctx, n, nrhs, Z, ldz, B, ldb, X, ldx, inf.anorm, hb, &berr));
+        TRY(residual_berr(ctx, ws, n, nrhs, Z, ldz, B, ldb, X, ldx, inf.anorm, hb, &berr));
         if (std::isnan(berr)) {
             best_saved = best < INFINITY;
             ++it;
@@ -339,7 +346,7 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
     } else if (fallback && inf.info == 0) {
         // refinement cannot reach tol with this factorisation (ZCGESV's rule):
         // the FP64 LU of Z, in place
-        r = fp64_lu_solve(ctx, n, nrhs, Z, ldz, B, ldb, X, ldx, default_nb(n, MIDAS_SOLVE_FP64), inf);
+        r = fp64_lu_solve(ctx, ws, n, nrhs, Z, ldz, B, ldb, X, ldx, default_nb(n, MIDAS_SOLVE_FP64), inf);
         if (r) return r;
         inf.status = MIDAS_STATUS_FP64_FALLBACK;
         inf.t_refine_ms += tm.ms();
@@ -377,11 +384,21 @@ int midas_init(int device, midas_ctx **ctx)
     midas_ctx *c = new midas_ctx;
     c->device = device;
     hipError_t e = hipStreamCreate(&c->st);
+    // the scratch array's address on this device, and the two events of the
+    // allocation-free factorisation, made once here
+    void *base = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&base, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->nev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->nev[1], hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (c->st) hipStreamDestroy(c->st);
+        for (hipEvent_t ev : c->nev)
+            if (ev) hipEventDestroy(ev);
         delete c;
-        set_error("hipStreamCreate: %s", hipGetErrorString(e));
+        set_error("midas_init: %s", hipGetErrorString(e));
         return -100 - (int)e;
     }
+    c->noalloc_base = base;
     *ctx = c;
     return 0;
 }
@@ -392,7 +409,10 @@ int midas_finalize(midas_ctx *ctx)
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->st);
     ctx->ws.release();
+    ctx->nws.release();
     hipStreamDestroy(ctx->st);
+    for (hipEvent_t ev : ctx->nev)
+        if (ev) hipEventDestroy(ev);
     delete ctx;
     return 0;
 }
@@ -812,12 +832,11 @@ int midas_scattering_quantities(midas_ctx *ctx, const double *S, long lds, int n
 
 // Fill (FP64 Z, and the FP32 copy W for mixed modes) + RHS + solve, all on
 // the device; X (3n x 2ntr).  Shared by the drop-in and the device entry point.
-static int mom_solve_dev(midas_ctx *ctx, const DevCells &cells, double k0, int quirks, int ntr, const double *dth,
-                         const double *dph, int mode, const midas_solve_opts_t *opts, double2 *Z, long ldz,
-                         double2 *X, long ldx, midas_solve_info_t *info, double *t_fill_ms)
+static int mom_solve_dev(midas_ctx *ctx, SolveWs &ws, const DevCells &cells, double k0, int quirks, int ntr,
+                         const double *dth, const double *dph, int mode, const midas_solve_opts_t *opts, double2 *Z,
+                         long ldz, double2 *X, long ldx, midas_solve_info_t *info, double *t_fill_ms)
 {
     hipStream_t st = ctx->st;
-    SolveWs &ws = ctx->ws;
     const int n = cells.n, n3 = 3 * n;
     float2 *w32 = nullptr;
     if (mode != MIDAS_SOLVE_FP64) {
@@ -825,12 +844,12 @@ static int mom_solve_dev(midas_ctx *ctx, const DevCells &cells, double k0, int q
         w32 = ws.W.as<float2>();
     }
     TRY(ws.V.reserve(sizeof(double2) * (size_t)n3 * 2 * ntr));
-    Timer tm(st);
+    Timer tm(st, !ws.lent);
     TRY(launch_fill(cells, CellSet{n, 0, nullptr}, CellSet{n, 0, nullptr}, k0, quirks, false, Z, ldz, w32, n3, st));
     TRY(launch_rhs(cells, CellSet{n, 0, nullptr}, ntr, dth, dph, k0, quirks, false, ws.V.as<double2>(), n3, st));
     double tf = tm.ms();
     if (t_fill_ms) *t_fill_ms = tf;
-    return solve_dev(ctx, n3, 2 * ntr, Z, ldz, w32, n3, ws.V.as<double2>(), n3, X, ldx, mode, opts, info, true);
+    return solve_dev(ctx, n3, 2 * ntr, Z, ldz, w32, n3, ws.V.as<double2>(), n3, X, ldx, mode, opts, info, true, &ws);
 }
 
 int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int ntr,
@@ -849,8 +868,8 @@ int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double
     TRY(X.alloc(sizeof(double2) * (size_t)n3 * 2 * ntr));
     TRY(upload(dth, tr_th, sizeof(double) * ntr, st));
     TRY(upload(dph, tr_ph, sizeof(double) * ntr, st));
-    int r = mom_solve_dev(ctx, dc.v, k0, quirks, ntr, dth.as<double>(), dph.as<double>(), mode, opts, Z.as<double2>(),
-                          n3, X.as<double2>(), n3, info, nullptr);
+    int r = mom_solve_dev(ctx, ctx->ws, dc.v, k0, quirks, ntr, dth.as<double>(), dph.as<double>(), mode, opts,
+                          Z.as<double2>(), n3, X.as<double2>(), n3, info, nullptr);
     // a refinement that missed its tolerance (no_fallback) still delivers the
     // outputs of its best iterate, then reports
     if (r < 0 && r != MIDAS_ERR_NOT_CONVERGED) return r;
@@ -910,8 +929,99 @@ int midas_dev_mom_solve(midas_ctx *ctx, const midas_cells_t *dcells, double k0, 
     MIDAS_HIP(hipSetDevice(ctx->device));
     DevCellsOwned &dc = ctx->cells;
     TRY(dc.from_device(dcells, k0, quirks, ctx->st));
-    int r = mom_solve_dev(ctx, dc.v, k0, quirks, ntr, dth, dph, mode, opts, (double2 *)Z, ldz, (double2 *)X, ldx, info,
-                          t_fill_ms);
+    int r = mom_solve_dev(ctx, ctx->ws, dc.v, k0, quirks, ntr, dth, dph, mode, opts, (double2 *)Z, ldz, (double2 *)X,
+                          ldx, info, t_fill_ms);
+    MIDAS_HIP(hipStreamSynchronize(ctx->st));
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of midas_dev_mom_solve: every work buffer is carved
+// from the device's static scratch array (its address from midas_init).
+// Unknowns n <= NOALLOC_MAX_N, right-hand sides <= NOALLOC_MAX_RHS; panels of
+// 256 columns (mixed modes) / 128 (FP64 and the FP64 fallback); the trailing
+// updates, residuals and FP64 products run on this library's kernels (no
+// rocBLAS); the factorisation runs on the context's stream alone (no second
+// stream).  Calls on one device share the array: calls that overlap race.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_MAX_N = 2048, NOALLOC_MAX_RHS = 32, NOALLOC_NB = 256, NOALLOC_NB64 = 128;
+
+struct NoallocPlan {
+    size_t off[20];
+    size_t total;
+};
+// offsets (256-byte aligned) of the buffers, in the order used by noalloc_lend
+static NoallocPlan noalloc_plan(int ncell, int nrhs)
+{
+    const int n = 3 * ncell;
+    const size_t d = sizeof(double) * ncell, z2 = 2 * d;
+    const size_t sz[] = {
+        d, z2, z2, z2, d, sizeof(int) * (size_t)ncell,                     // cells: kappa che znnpp kchi a id
+        sizeof(float2) * (size_t)n * n,                                     // W
+        trail16_scratch_bytes(n, n, NOALLOC_NB),                            // trail
+        sizeof(int) * (size_t)n, sizeof(int) * (size_t)n,                   // ipiv perm
+        sizeof(double) * (size_t)(n + 2 * nrhs + 4),                        // norms
+        zx_scratch_bytes(n, nrhs <= 4 ? nrhs : 1),                          // zx
+        sizeof(double2) * (size_t)n * nrhs,                                 // R
+        sizeof(float2) * (size_t)n * nrhs,                                  // D
+        sizeof(double2) * (size_t)getrs_tmp_elems(n, NOALLOC_NB, nrhs),     // t2 (FP32 or FP64 use)
+        sizeof(double2) * (size_t)n * nrhs,                                 // xbest
+        sizeof(double2) * (size_t)n * nrhs,                                 // V
+        LuWork<float>::bytes_needed(n, NOALLOC_NB),                         // luf
+        LuWork<double>::bytes_needed(n, NOALLOC_NB64),                      // lud
+    };
+    NoallocPlan p{};
+    size_t o = 0;
+    for (size_t i = 0; i < sizeof(sz) / sizeof(sz[0]); ++i) {
+        p.off[i] = o;
+        o += (sz[i] + 255) & ~(size_t)255;
+    }
+    p.off[19] = o;
+    p.total = o;
+    return p;
+}
+
+static int noalloc_lend(midas_ctx *ctx, int ncell, int nrhs)
+{
+    const NoallocPlan p = noalloc_plan(ncell, nrhs);
+    MIDAS_CHECK(p.total <= NOALLOC_SCRATCH_BYTES, -1, "noalloc: %zu bytes of scratch needed, %zu held", p.total,
+                NOALLOC_SCRATCH_BYTES);
+    char *b = (char *)ctx->noalloc_base;
+    auto len = [&](int i) { return p.off[i + 1] - p.off[i]; };
+    DevCellsOwned &c = ctx->ncells;
+    DBuf *cb[] = {&c.kappa, &c.che, &c.znnpp, &c.kchi, &c.a, &c.id};
+    for (int i = 0; i < 6; ++i) cb[i]->borrow(b + p.off[i], len(i));
+    SolveWs &w = ctx->nws;
+    DBuf *wb[] = {&w.W, &w.trail, &w.ipiv, &w.perm, &w.norms, &w.zx, &w.R, &w.D, &w.t2, &w.xbest, &w.V};
+    for (int i = 0; i < 11; ++i) wb[i]->borrow(b + p.off[6 + i], len(6 + i));
+    w.zcopy.release();   // Z is factored in place: no copy is ever asked for
+    w.lent = true;
+    const int n = 3 * ncell;
+    TRY(w.luf.lend(b + p.off[17], len(17), n, NOALLOC_NB, 1, ctx->st, ctx->nev[0], ctx->nev[1]));
+    TRY(w.lud.lend(b + p.off[18], len(18), n, NOALLOC_NB64, 1, ctx->st, ctx->nev[0], ctx->nev[1]));
+    return 0;
+}
+
+int midas_dev_mom_solve_noalloc(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, int ntr,
+                                const double *dth, const double *dph, int mode, const midas_solve_opts_t *opts,
+                                double *Z, long ldz, double *X, long ldx, midas_solve_info_t *info)
+{
+    MIDAS_CHECK(ctx && dcells && dth && dph && Z && X && ntr > 0, -1, "dev_mom_solve_noalloc: bad arguments");
+    MIDAS_CHECK(ctx->noalloc_base, -1, "dev_mom_solve_noalloc: no scratch address (midas_init)");
+    const int ncell = dcells->n;
+    MIDAS_CHECK(ncell > 0 && 3L * ncell <= NOALLOC_MAX_N && 2L * ntr <= NOALLOC_MAX_RHS, -1,
+                "dev_mom_solve_noalloc: %d unknowns and %d right-hand sides; at most %d and %d", 3 * ncell, 2 * ntr,
+                NOALLOC_MAX_N, NOALLOC_MAX_RHS);
+    MIDAS_CHECK(ldz >= 3L * ncell && ldx >= 3L * ncell, -1, "dev_mom_solve_noalloc: leading dimension too small");
+    MIDAS_CHECK(mode >= 0 && mode <= 5, -1, "dev_mom_solve_noalloc: unknown mode %d", mode);
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    TRY(noalloc_lend(ctx, ncell, 2 * ntr));
+    midas_solve_opts_t o{};
+    if (opts) o = *opts;
+    o.nb = mode == MIDAS_SOLVE_FP64 ? NOALLOC_NB64 : NOALLOC_NB;
+    TRY(ctx->ncells.from_device(dcells, k0, quirks, ctx->st));
+    int r = mom_solve_dev(ctx, ctx->nws, ctx->ncells.v, k0, quirks, ntr, dth, dph, mode, &o, (double2 *)Z, ldz,
+                          (double2 *)X, ldx, info, nullptr);
     MIDAS_HIP(hipStreamSynchronize(ctx->st));
     return r;
 }

@@ -650,6 +650,9 @@ int TrailingGemm<double>::operator()(int m, int n, int k, const void *A, long ld
 {
     if (m <= 0 || n <= 0 || k <= 0 || batch <= 0) return 0;
     ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k * batch, 16.0 * (2.0 * m * n + (double)m * k + (double)k * n) * batch);
+    if (!library)
+        return launch_cgemm<double>(m, n, k, z64{-1.0, 0.0}, A, lda, 0, B, ldb, 0, z64{1.0, 0.0}, C, ldc, st, batch,
+                                    stride, stride, stride);
     return blas_zgemm_nn(m, n, k, z64{-1.0, 0.0}, A, lda, stride, B, ldb, stride, z64{1.0, 0.0}, C, ldc, stride, batch,
                          st);
 }

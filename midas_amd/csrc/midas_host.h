@@ -16,12 +16,14 @@ namespace midas {
 struct DBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;   // memory lent by its owner (never allocated or freed here)
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
     int alloc(size_t b)
     {
+        MIDAS_CHECK(!borrowed, -1, "internal: allocation of a borrowed buffer (%zu bytes)", b);
         release();
         if (b == 0) b = 16;
         hipError_t e = hipMalloc(&p, b);
@@ -35,17 +37,28 @@ struct DBuf {
     }
     // grow-only: keeps the allocation when it is large enough (contents are
     // not preserved when it grows; the slack is a quarter, at most 256 MiB);
-    // hipFree waits for work still using it
+    // hipFree waits for work still using it.  A borrowed buffer never grows:
+    // asking for more than it holds is an error.
     int reserve(size_t b)
     {
         if (p && b <= bytes) return 0;
+        MIDAS_CHECK(!borrowed, -1, "internal: borrowed buffer of %zu bytes asked for %zu", bytes, b);
         return alloc(b + std::min(b / 4, (size_t)256 << 20));
+    }
+    // lend `cap` bytes at q (not owned: release forgets them)
+    void borrow(void *q, size_t cap)
+    {
+        release();
+        p = q;
+        bytes = cap;
+        borrowed = true;
     }
     void release()
     {
-        if (p) hipFree(p);
+        if (p && !borrowed) hipFree(p);
         p = nullptr;
         bytes = 0;
+        borrowed = false;
     }
     template <typename X> X *as() const { return (X *)p; }
 };
@@ -162,6 +175,7 @@ struct SolveWs {
     DBuf V;            // excitation of the MoM solve
     LuWork<float> luf;
     LuWork<double> lud;
+    bool lent = false; // every buffer lent from a fixed scratch: nothing may be allocated
     void release()
     {
         DBuf *b[] = {&W, &trail, &ipiv, &perm, &norms, &zx, &R, &D, &t2, &xbest, &zcopy, &V};
@@ -176,20 +190,31 @@ struct midas_ctx {
     hipStream_t st = nullptr;
     SolveWs ws;
     DevCellsOwned cells;          // cell constants of the device-pointer entry points
+    // the allocation-free entry point (midas_dev_mom_solve_noalloc): its
+    // workspace, lent from the device's static scratch array, and the events
+    // its factorisation uses, made once by midas_init
+    void *noalloc_base = nullptr;
+    SolveWs nws;
+    DevCellsOwned ncells;
+    hipEvent_t nev[2] = {};
     double cbfm_budget[4] = {};   // bytes; 0 = default (midas_set_cbfm_budgets)
 };
 
+// Event timer on a stream; a disabled timer creates no events and reads 0.
 struct Timer {
-    hipEvent_t a, b;
+    hipEvent_t a{}, b{};
     hipStream_t st;
-    explicit Timer(hipStream_t s) : st(s)
+    bool on;
+    explicit Timer(hipStream_t s, bool enabled = true) : st(s), on(enabled)
     {
+        if (!on) return;
         hipEventCreate(&a);
         hipEventCreate(&b);
         hipEventRecord(a, st);
     }
     double ms()
     {
+        if (!on) return 0.0;
         hipEventRecord(b, st);
         hipEventSynchronize(b);
         float t = 0;
@@ -199,6 +224,7 @@ struct Timer {
     }
     ~Timer()
     {
+        if (!on) return;
         hipEventDestroy(a);
         hipEventDestroy(b);
     }
@@ -214,6 +240,8 @@ int launch_rowabs(const double2 *Z, long ldz, int n, double *rowsum, void *scrat
 
 // Dense solve on the device (FP64 LU or mixed-precision LU + FP64 refinement):
 // Z (n x n, ldz) FP64; W32 optional FP32 copy of Z for mixed modes; X = Z^-1 B.
+// ws: the workspace (ctx->ws when NULL); a lent workspace (ws->lent) makes
+// the solve allocate nothing: no timers, hand-written FP64 GEMMs only.
 int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32, long ldw, const double2 *B,
               long ldb, double2 *X, long ldx, int mode, const midas_solve_opts_t *opts, midas_solve_info_t *info,
-              bool z_inplace);
+              bool z_inplace, SolveWs *ws = nullptr);

@@ -27,11 +27,18 @@ template <typename T> struct LuWork {
     hipStream_t pstream = nullptr;   // panel stream (look-ahead), default priority
     hipEvent_t ev_panel = nullptr, ev_next = nullptr;
     size_t cap[9] = {};              // bytes held by each buffer above (grow-only)
+    bool borrowed = false;           // buffers, stream and events lent by the caller (lend())
     // Sizes the workspace for (n, nb, batch, sA): buffers grow when too small
     // and are kept otherwise, so a LuWork reused for one size allocates once;
     // the panel stream and its events are created on first use.
     int alloc(int n, int nb, int batch = 1, long sA = 0);
     void release();
+    // Bytes of device memory alloc(n, nb, batch) needs, and a workspace lent
+    // from `mem` (that many bytes, 256-byte aligned) with the given panel
+    // stream (may be the main stream: the factorisation then runs without
+    // look-ahead overlap) and events: alloc() then only checks the sizes.
+    static size_t bytes_needed(int n, int nb, int batch = 1);
+    int lend(void *mem, size_t bytes, int n, int nb, int batch, hipStream_t ps, hipEvent_t e1, hipEvent_t e2);
     long inv_stride() const { return (long)((n + nb - 1) / nb) * nb * nb; }   // diagonal-block inverses per matrix
 };
 
@@ -43,6 +50,7 @@ template <typename T> struct TrailingGemm {
     void *scratch = nullptr;        // split operand planes + column scales (trail16_scratch_bytes)
     int batch = 1;                  // batched factorisations: matrices at stride `stride`
     long stride = 0;
+    bool library = true;            // FP64: rocBLAS ZGEMM (false: the hand-written k_cgemm<double>)
     int operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C, long ldc,
                    hipStream_t st) const;
 };

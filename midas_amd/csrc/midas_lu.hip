@@ -22,6 +22,8 @@
 #include "midas_common.h"
 #include "midas_lu.h"
 
+#include <algorithm>
+
 namespace midas {
 
 constexpr int STEP_THREADS = 256;
@@ -764,6 +766,26 @@ __global__ void k_copy2d(const typename cvec<T>::t *S, long lds, typename cvec<T
     if (r < rows && c < cols) D[(long)r + (long)c * ldd] = S[(long)r + (long)c * lds];
 }
 
+// bytes (a multiple of 4) of device memory set to zero by a kernel
+__global__ void k_zero_words(unsigned *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+static int zero_words(void *p, size_t bytes, hipStream_t st)
+{
+    const size_t n = bytes / 4;
+    if (!n) return 0;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero_words, dim3(g), dim3(256), 0, st, (unsigned *)p, n);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+#define TRY_LU(x)               \
+    do {                        \
+        int r_ = (x);           \
+        if (r_) return r_;      \
+    } while (0)
+
 template <typename T> static cplx<T> cone() { return {(T)1, (T)0}; }
 template <typename T> static cplx<T> cmone() { return {(T)-1, (T)0}; }
 template <typename T> static cplx<T> czero() { return {(T)0, (T)0}; }
@@ -861,11 +883,12 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
     typedef typename cvec<T>::t V;
     hipStream_t ps = w.pstream;
     MIDAS_CHECK(w.batch == 1 || tg.batch == w.batch, -1, "getrf: trailing GEMM not batched like the factorisation");
-    // last-workgroup counters, info, the fault flag and the step tags
-    MIDAS_HIP(hipMemsetAsync(w.counter, 0, sizeof(int) * (2 * w.batch + 1 + w.maxblk * w.batch), st));
-    MIDAS_HIP(hipMemsetAsync(w.pmax, 0, 8 * 2 * w.maxblk * w.batch, st));   // tagged records
-    MIDAS_HIP(hipMemsetAsync(w.cand, 0, 16 * 2 * w.maxblk * LU_IB * w.batch, st));   // tagged words
-    MIDAS_HIP(hipMemsetAsync(w.rowj, 0, 16 * 2 * LU_IB * w.batch, st));
+    // last-workgroup counters, info, the fault flag and the step tags; the
+    // tagged records and words (a fill kernel: safe under stream capture)
+    TRY_LU(zero_words(w.counter, sizeof(int) * (2 * w.batch + 1 + w.maxblk * w.batch), st));
+    TRY_LU(zero_words(w.pmax, 8 * 2 * w.maxblk * w.batch, st));
+    TRY_LU(zero_words(w.cand, 16 * 2 * w.maxblk * LU_IB * w.batch, st));
+    TRY_LU(zero_words(w.rowj, 16 * 2 * LU_IB * w.batch, st));
     w.tag_base = 0;
     MIDAS_HIP(hipEventRecord(w.ev_next, st));
     MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
@@ -1179,6 +1202,33 @@ void ipiv_to_perm(const int *ipiv, int n, int *perm)
     }
 }
 
+// sizes of the nine work buffers (pmax, pidx, counter, cand, rowj, pinv, linv, uinv, u12)
+template <typename T> static void lu_sizes(int n, int nb, int batch, size_t need[9])
+{
+    const size_t maxblk = (n + BLK_T - 1) / BLK_T + 1;   // workgroups of either panel kernel
+    const size_t inv = (size_t)((n + nb - 1) / nb) * nb * nb;
+    // 16 bytes per candidate element: complex FP64, or complex FP32 as two tagged words
+    need[0] = (size_t)8 * 2 * maxblk * batch;                        // pmax (FP32: 64-bit records)
+    need[1] = sizeof(int) * 2 * maxblk * batch;                      // pidx
+    need[2] = sizeof(int) * (2 * (size_t)batch + 1 + maxblk * batch);   // counter, info, fault, tags
+    need[3] = (size_t)16 * 2 * maxblk * LU_IB * batch;               // cand
+    need[4] = (size_t)16 * 2 * LU_IB * batch;                        // rowj
+    need[5] = sizeof(cplx<T>) * batch;                               // pinv
+    need[6] = sizeof(cplx<T>) * inv * batch;                         // linv
+    need[7] = sizeof(cplx<T>) * inv * batch;                         // uinv
+    need[8] = sizeof(cplx<T>) * (size_t)nb * n * batch;              // u12
+}
+static inline size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+template <typename T>
+size_t LuWork<T>::bytes_needed(int n, int nb, int batch)
+{
+    size_t need[9], t = 0;
+    lu_sizes<T>(n, nb, batch, need);
+    for (size_t b : need) t += round256(b);
+    return t;
+}
+
 template <typename T>
 int LuWork<T>::alloc(int n_, int nb_, int batch_, long sA_)
 {
@@ -1186,23 +1236,14 @@ int LuWork<T>::alloc(int n_, int nb_, int batch_, long sA_)
     nb = nb_;
     batch = batch_;
     sA = sA_;
-    maxblk = (n + BLK_T - 1) / BLK_T + 1;   // workgroups of either panel kernel
-    // 16 bytes per candidate element: complex FP64, or complex FP32 as two tagged words
-    const size_t need[9] = {
-        (size_t)8 * 2 * maxblk * batch,                         // pmax (FP32: 64-bit records)
-        sizeof(int) * 2 * (size_t)maxblk * batch,               // pidx
-        sizeof(int) * (2 * (size_t)batch + 1 + (size_t)maxblk * batch),   // counter, info, fault, tags
-        (size_t)16 * 2 * maxblk * LU_IB * batch,                // cand
-        (size_t)16 * 2 * LU_IB * batch,                         // rowj
-        sizeof(cplx<T>) * batch,                                // pinv
-        sizeof(cplx<T>) * (size_t)inv_stride() * batch,         // linv
-        sizeof(cplx<T>) * (size_t)inv_stride() * batch,         // uinv
-        sizeof(cplx<T>) * (size_t)nb * n * batch,               // u12
-    };
+    maxblk = (n + BLK_T - 1) / BLK_T + 1;
+    size_t need[9];
+    lu_sizes<T>(n, nb, batch, need);
     void **buf[9] = {(void **)&pmax, (void **)&pidx, (void **)&counter, (void **)&cand, (void **)&rowj,
                      (void **)&pinv, (void **)&linv, (void **)&uinv, (void **)&u12};
     for (int i = 0; i < 9; ++i) {
         if (*buf[i] && need[i] <= cap[i]) continue;
+        MIDAS_CHECK(!borrowed, -1, "lu: lent workspace too small (buffer %d: %zu of %zu bytes)", i, cap[i], need[i]);
         if (*buf[i]) MIDAS_HIP(hipFree(*buf[i]));
         *buf[i] = nullptr;
         cap[i] = 0;
@@ -1219,23 +1260,50 @@ int LuWork<T>::alloc(int n_, int nb_, int batch_, long sA_)
 }
 
 template <typename T>
+int LuWork<T>::lend(void *mem, size_t bytes, int n_, int nb_, int batch_, hipStream_t ps, hipEvent_t e1, hipEvent_t e2)
+{
+    MIDAS_CHECK(mem && ps && e1 && e2, -1, "lu: lend needs memory, a stream and two events");
+    size_t need[9];
+    lu_sizes<T>(n_, nb_, batch_, need);
+    MIDAS_CHECK(bytes >= bytes_needed(n_, nb_, batch_), -1, "lu: lent %zu bytes, need %zu", bytes,
+                bytes_needed(n_, nb_, batch_));
+    release();
+    void **buf[9] = {(void **)&pmax, (void **)&pidx, (void **)&counter, (void **)&cand, (void **)&rowj,
+                     (void **)&pinv, (void **)&linv, (void **)&uinv, (void **)&u12};
+    char *q = (char *)mem;
+    for (int i = 0; i < 9; ++i) {
+        *buf[i] = q;
+        cap[i] = need[i];
+        q += round256(need[i]);
+    }
+    pstream = ps;
+    ev_panel = e1;
+    ev_next = e2;
+    borrowed = true;
+    return alloc(n_, nb_, batch_, 0);
+}
+
+template <typename T>
 void LuWork<T>::release()
 {
-    if (pstream) {
-        hipStreamSynchronize(pstream);
-        hipStreamDestroy(pstream);
+    if (!borrowed) {
+        if (pstream) {
+            hipStreamSynchronize(pstream);
+            hipStreamDestroy(pstream);
+        }
+        hipFree(pmax);
+        hipFree(pidx);
+        hipFree(counter);
+        hipFree(pinv);
+        hipFree(linv);
+        hipFree(uinv);
+        hipFree(u12);
+        hipFree(cand);
+        hipFree(rowj);
+        if (ev_panel) hipEventDestroy(ev_panel);
+        if (ev_next) hipEventDestroy(ev_next);
     }
-    hipFree(pmax);
-    hipFree(pidx);
-    hipFree(counter);
-    hipFree(pinv);
-    hipFree(linv);
-    hipFree(uinv);
-    hipFree(u12);
-    hipFree(cand);
-    hipFree(rowj);
-    if (ev_panel) hipEventDestroy(ev_panel);
-    if (ev_next) hipEventDestroy(ev_next);
+    borrowed = false;
     pstream = nullptr;
     ev_panel = ev_next = nullptr;
     pmax = nullptr;
