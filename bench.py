@@ -36,8 +36,8 @@ MODE_TERMS = {"fp64": 1, "c32": 1, "f16": 1, "f16x3": 3, "bf16": 1, "bf16x3": 3}
 # HBM bytes per k_trail16x launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors
 # a 1 GiB device copy reads on the same counters), keyed by (cells, mode, nb);
-# other configurations report null.  Source: profiles/lu_trail16_f16_grs20k_hbm_traffic.md
-TRAFFIC_PER_LAUNCH = {(20000, "f16", 0): 14.99e9}
+# other configurations report null.
+TRAFFIC_PER_LAUNCH = {}
 
 
 def log(*a):

@@ -115,7 +115,15 @@ static int trail_mode(int mode)
     return -1;
 }
 
-static int default_nb(int n, int mode) { return mode == MIDAS_SOLVE_FP64 ? 128 : 512; }
+// Outer step width of the LU: 128 for FP64; 512 for the mixed modes, and 1024
+// (two 512-wide panels per step: trailing updates of 1024 columns at twice
+// the arithmetic per byte of C) for the single-term split modes on large n.
+static int default_nb(int n, int mode)
+{
+    if (mode == MIDAS_SOLVE_FP64) return 128;
+    if ((mode == MIDAS_SOLVE_F16_IR || mode == MIDAS_SOLVE_BF16_IR) && n >= 16384) return 1024;
+    return 512;
+}
 
 // perm from LAPACK interchanges for one matrix, on the device: one workgroup,
 // the permutation held in LDS (n <= PERM_LDS_MAX), one thread walks the swaps.
@@ -272,13 +280,24 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
         hipLaunchKernelGGL(k_z2c, dim3((n + 255) / 256, n), dim3(256), 0, st, Z, ldz, A, lda, n, n);
         MIDAS_HIP(hipGetLastError());
     }
+    // an outer step wider than 512 is two inner panels of half its width
+    // (single-term split modes only: their GEMM takes k up to 2048)
+    const bool two = nb > 512 && (mode == MIDAS_SOLVE_F16_IR || mode == MIDAS_SOLVE_BF16_IR);
+    MIDAS_CHECK(nb <= 512 || two, -1, "solve: panel width %d above 512 needs a single-term split mode", nb);
+    const int nbi = two ? (nb + 1) / 2 : nb;
     LuWork<float> &w = ws.luf;
-    TRY(w.alloc(n, nb));
+    TRY(w.alloc(n, nbi));
+    w.outer = two ? 2 : 1;
     TRY(ws.trail.reserve(trail16_scratch_bytes(n, n, nb)));
-    TrailingGemm<float> tg;
-    tg.mode = trail_mode(mode);
+    TrailingGemm<float> tg, tgp;
+    tg.mode = tgp.mode = trail_mode(mode);
     tg.scratch = ws.trail.p;
-    int r = getrf<float>(A, lda, n, ws.ipiv.as<int>(), w, st, tg);
+    if (two) {
+        TRY(ws.trailp.reserve(trail16_scratch_bytes(n, nbi, nbi)));
+        tgp.scratch = ws.trailp.p;
+    }
+    int r = getrf<float>(A, lda, n, ws.ipiv.as<int>(), w, st, tg, tgp);
+    nb = nbi;   // the diagonal blocks getrs works with
     if (r) return r;
     MIDAS_HIP(hipMemcpyAsync(&inf.info, w.info, sizeof(int), hipMemcpyDeviceToHost, st));
     TRY(make_perm(ws.ipiv.as<int>(), ws.perm.as<int>(), n, st));

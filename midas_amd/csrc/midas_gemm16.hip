@@ -96,57 +96,57 @@ __global__ void k_split_a(const float2 *__restrict__ A, long lda, int m, int K, 
         }
 }
 
-// B (K x n, ldb) -> per-column power-of-two scale + planes; one wave per column.
-// NEG (single term only): a third plane holds -Bi, so the real part of the
-// product is Br Ar + (-Bi) Ai in one accumulator.
-template <bool BF, int P, bool NEG = false>
+// B (K x n, ldb) -> per-column power-of-two scale + planes; one wave per
+// column, each lane 8 consecutive k of every 512 (K <= SPLIT_B_KMAX).
+constexpr int SPLIT_B_KMAX = 2048;
+template <bool BF, int P>
 __global__ __launch_bounds__(64) void k_split_b(const float2 *__restrict__ B, long ldb, int K, int n, int rtiles,
                                                 int kst, u16 *__restrict__ out, float *__restrict__ scale)
 {
+    constexpr int G = SPLIT_B_KMAX / 512;
     const int col = blockIdx.x;
     const int t = threadIdx.x;
-    const int k0 = 8 * t;
     const int Kp = kst * T_K;
-    float2 v[8];
+    float2 v[G][8];
     float mx = 0.f;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        v[q] = (col < n && k0 + q < K) ? B[(long)(k0 + q) + (long)col * ldb] : float2{0.f, 0.f};
-        mx = fmaxf(mx, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
-    }
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int kk = 512 * g + 8 * t + q;
+            v[g][q] = (col < n && kk < K) ? B[(long)kk + (long)col * ldb] : float2{0.f, 0.f};
+            mx = fmaxf(mx, fmaxf(fabsf(v[g][q].x), fabsf(v[g][q].y)));
+        }
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
     int e = 0;
     if (mx > 0.f && isfinite(mx)) frexpf(mx, &e);
     const float s = ldexpf(1.0f, -e);
     if (t == 0 && col < n) scale[col] = s;
-    if (k0 >= Kp) return;
-    u16 h[2][2][8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        float xr = v[q].x * s, xi = v[q].y * s;
-        u16 hr = half_t<BF>::cvt(xr), hi = half_t<BF>::cvt(xi);
-        h[0][0][q] = hr;
-        h[0][1][q] = hi;
-        if (P == 2) {
-            h[1][0][q] = half_t<BF>::cvt(xr - half_t<BF>::back(hr));
-            h[1][1][q] = half_t<BF>::cvt(xi - half_t<BF>::back(hi));
+    for (int g = 0; g < G; ++g) {
+        const int k0 = 512 * g + 8 * t;
+        if (k0 >= Kp) break;
+        u16 h[2][2][8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            float xr = v[g][q].x * s, xi = v[g][q].y * s;
+            u16 hr = half_t<BF>::cvt(xr), hi = half_t<BF>::cvt(xi);
+            h[0][0][q] = hr;
+            h[0][1][q] = hi;
+            if (P == 2) {
+                h[1][0][q] = half_t<BF>::cvt(xr - half_t<BF>::back(hr));
+                h[1][1][q] = half_t<BF>::cvt(xi - half_t<BF>::back(hi));
+            }
         }
-    }
 #pragma unroll
-    for (int p = 0; p < P; ++p)
+        for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            s8 w;
+            for (int c = 0; c < 2; ++c) {
+                s8 w;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) w[q] = (short)h[p][c][q];
-            *(s8 *)(out + plane_off(p * 2 + c, col, k0, rtiles, kst)) = w;
-        }
-    if constexpr (NEG) {
-        static_assert(P == 1, "negated plane: single term only");
-        s8 w;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) w[q] = (short)(h[0][1][q] ^ 0x8000);   // sign bit of the half
-        *(s8 *)(out + plane_off(2, col, k0, rtiles, kst)) = w;
+                for (int q = 0; q < 8; ++q) w[q] = (short)h[p][c][q];
+                *(s8 *)(out + plane_off(p * 2 + c, col, k0, rtiles, kst)) = w;
+            }
     }
 }
 
@@ -385,13 +385,15 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
 // workgroup fills those gaps.  To fit two: the real part accumulates
 // Br Ar + (-Bi) Ai in one set (B carries a negated imaginary plane), so a wave
 // holds 2 x 4 accumulator tiles; stages are 16 k deep in a ring of three 20 KB
-// slots (60 KB per workgroup); registers stay within 224 per lane, which
+// slots (48 KB per workgroup); registers stay within 224 per lane, which
 // leaves room on each SIMD for a 64-register wave of the LU panel kernel.
-// A planes: Ar, Ai.  B planes: Br, Bi, -Bi.
+// A planes: Ar, Ai.  B planes: Br, Bi; the -Bi fragment is made in registers
+// (a sign-bit flip of each half), which costs two VALU operations per
+// fragment instead of a fifth plane through L2 and LDS.
 // ---------------------------------------------------------------------------
 struct Stage2 {
     u16 a[2][T_M][T_K];
-    u16 b[3][T_N][T_K];
+    u16 b[2][T_N][T_K];
 };
 __shared__ __attribute__((aligned(16))) unsigned char g2_slot0[sizeof(Stage2)];
 __shared__ __attribute__((aligned(16))) unsigned char g2_slot1[sizeof(Stage2)];
@@ -402,7 +404,7 @@ __global__ __launch_bounds__(256, 2) void k_trail16x(int m, int n, int kst, int 
                                                      const u16 *__restrict__ Ap, const u16 *__restrict__ Bp,
                                                      const float *__restrict__ bscale, float2 *__restrict__ C, long ldc)
 {
-    constexpr int IPS = 5;   // copy instructions per wave per stage (one 1 KB part of each of 5 plane tiles)
+    constexpr int IPS = 4;   // copy instructions per wave per stage (one 1 KB part of each of 4 plane tiles)
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(256, 2) void k_trail16x(int m, int n, int kst, int 
             copy16(Ap + ((((long)pc * mt + tm) * kst + ks) * T_M) * T_K + wave * 512 + lane * 8,
                    &dst->a[pc][0][0] + wave * 512);
 #pragma unroll
-        for (int pc = 0; pc < 3; ++pc)
+        for (int pc = 0; pc < 2; ++pc)
             copy16(Bp + ((((long)pc * nt + tn) * kst + ks) * T_N) * T_K + wave * 512 + lane * 8,
                    &dst->b[pc][0][0] + wave * 512);
     };
@@ -455,9 +457,15 @@ __global__ __launch_bounds__(256, 2) void k_trail16x(int m, int n, int kst, int 
         const int r32 = lane & 31, kc = (lane >> 5) * 8;
         s8 fb[3][2];
 #pragma unroll
-        for (int pc = 0; pc < 3; ++pc)
+        for (int pc = 0; pc < 2; ++pc)
 #pragma unroll
             for (int t = 0; t < 2; ++t) fb[pc][t] = *(const s8 *)&src->b[pc][wn + 32 * t + r32][kc];
+        // -Bi: the sign bit of every half flipped
+        typedef unsigned u4v __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+            fb[2][t] = __builtin_bit_cast(s8, __builtin_bit_cast(u4v, fb[1][t]) ^
+                                                  (u4v){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u});
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             s8 fa[2];
@@ -556,20 +564,18 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
     u16 *Ap = (u16 *)scratch;
     u16 *Bp = Ap + (size_t)2 * P * mt * T_M * kst * T_K;
     float *sc = (float *)(Bp + (size_t)2 * P * nt * T_N * kst * T_K);
-    MIDAS_CHECK(kst * T_K <= 512, -1, "trail16: k = %d above 512", k);
     if constexpr (P == 1) {
         // two workgroups per CU (MIDAS_TRAIL_1WG=1 selects the one-workgroup kernel)
         const char *e = getenv("MIDAS_TRAIL_1WG");
         if (!(e && atoi(e) != 0)) {
+            MIDAS_CHECK(k <= SPLIT_B_KMAX, -1, "trail16: k = %d above %d", k, SPLIT_B_KMAX);
             const int ks = (k + T_K - 1) / T_K;   // 16-k stages, no ring padding
             u16 *Bq = Ap + (size_t)2 * mt * T_M * ks * T_K;
-            float *sq = (float *)(Bq + (size_t)3 * nt * T_N * ks * T_K);
+            float *sq = (float *)(Bq + (size_t)2 * nt * T_N * ks * T_K);
             hipLaunchKernelGGL((k_split_a<BF, 1>), dim3(mt * T_M / 256 + 1, ks * 2), dim3(256), 0, st, A, lda, m, k,
                                mt, ks, Ap);
-            hipLaunchKernelGGL((k_split_b<BF, 1, true>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, ks, Bq,
-                               sq);
+            hipLaunchKernelGGL((k_split_b<BF, 1>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, ks, Bq, sq);
             // algorithmic bytes: C read and written, both operands once in FP16
-            // (the -Bi plane is an extra read of this kernel, seen in the counters)
             ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 16.0 * m * n + 4.0 * ((double)m + n) * k);
             const long a_tile = (long)T_M * ks * T_K * 2 * 2;
             const int group = (int)std::max(2L, std::min(16L, (2L << 20) / a_tile));
@@ -579,6 +585,7 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
             return 0;
         }
     }
+    MIDAS_CHECK(kst * T_K <= 512, -1, "trail16: k = %d above 512", k);
     hipLaunchKernelGGL((k_split_a<BF, P>), dim3(mt * T_M / 256 + 1, kst * 2), dim3(256), 0, st, A, lda, m, k, mt, kst,
                        Ap);
     hipLaunchKernelGGL((k_split_b<BF, P>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, kst, Bp, sc);

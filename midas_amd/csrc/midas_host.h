@@ -166,6 +166,7 @@ struct DevCellsOwned {
 struct SolveWs {
     DBuf W;            // complex-FP32 copy of the matrix (mixed modes)
     DBuf trail;        // split-precision operand planes of the trailing update
+    DBuf trailp;       // the same for the panel stream's update (two-panel steps)
     DBuf ipiv, perm;   // interchanges and the final permutation
     DBuf norms;        // row sums, column maxima, scalars
     DBuf zx;           // one-pass residual / row-sum scratch
@@ -178,7 +179,7 @@ struct SolveWs {
     bool lent = false; // every buffer lent from a fixed scratch: nothing may be allocated
     void release()
     {
-        DBuf *b[] = {&W, &trail, &ipiv, &perm, &norms, &zx, &R, &D, &t2, &xbest, &zcopy, &V};
+        DBuf *b[] = {&W, &trail, &trailp, &ipiv, &perm, &norms, &zx, &R, &D, &t2, &xbest, &zcopy, &V};
         for (DBuf *x : b) x->release();
         luf.release();
         lud.release();

@@ -24,9 +24,11 @@ template <typename T> struct LuWork {
     cplx<T> *linv = nullptr;         // inverses of the diagonal blocks of L (nb x nb each)
     cplx<T> *uinv = nullptr;         // inverses of the diagonal blocks of U
     cplx<T> *u12 = nullptr;          // nb x n staging for U12 = L11^-1 A12
+    cplx<T> *u12p = nullptr;         // nb x nb staging of the panel stream (two-panel steps)
+    int outer = 1;                   // inner panels per outer step: 1, or 2 (trailing updates of 2 nb columns)
     hipStream_t pstream = nullptr;   // panel stream (look-ahead), default priority
     hipEvent_t ev_panel = nullptr, ev_next = nullptr;
-    size_t cap[9] = {};              // bytes held by each buffer above (grow-only)
+    size_t cap[10] = {};              // bytes held by each buffer above (grow-only)
     bool borrowed = false;           // buffers, stream and events lent by the caller (lend())
     // Sizes the workspace for (n, nb, batch, sA): buffers grow when too small
     // and are kept otherwise, so a LuWork reused for one size allocates once;
@@ -61,8 +63,13 @@ template <typename T> struct cvec;
 template <> struct cvec<double> { typedef double2 t; };
 template <> struct cvec<float> { typedef float2 t; };
 
+// With w.outer == 2 the panel stream factors two nb-wide panels per outer step
+// (updating the second with the first through tg_panel, which must own a
+// scratch of its own) and the main stream's trailing updates take 2 nb columns
+// at once.
 template <typename T>
-int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipStream_t st, TrailingGemm<T> tg);
+int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipStream_t st, TrailingGemm<T> tg,
+          TrailingGemm<T> tg_panel = TrailingGemm<T>());
 // Solve with the factors of getrf (same LuWork: it holds the diagonal-block
 // inverses); tmp needs getrs_tmp_elems(n, nb, nrhs) elements per matrix.
 // With w.batch > 1 every matrix (stride w.sA) solves its own B (stride sB).

@@ -852,16 +852,18 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
                           w.nb, st, w.batch, w.sA, w.inv_stride());
 }
 
-// U12(:, c0:c1) = L11^-1 A12(:, c0:c1) through the workspace, for panel k.
+// U12(:, c0:c1) = L11^-1 A12(:, c0:c1) through a staging buffer (w.u12, or
+// stage: nb x (c1 - c0) per matrix at stride sst), for the panel at k.
 template <typename T>
-static int u12_solve(typename cvec<T>::t *A, long lda, int k, int nb, int c0, int c1, LuWork<T> &w, hipStream_t st)
+static int u12_solve(typename cvec<T>::t *A, long lda, int k, int nb, int c0, int c1, LuWork<T> &w, hipStream_t st,
+                     cplx<T> *stage = nullptr, long sst = 0)
 {
     typedef typename cvec<T>::t V;
     const int nc = c1 - c0;
     if (nc <= 0) return 0;
     const V *Li = (const V *)w.linv + (long)(k / w.nb) * w.nb * w.nb;
-    V *tmp = (V *)w.u12;
-    const long st12 = (long)w.nb * w.n;
+    V *tmp = (V *)(stage ? stage : w.u12);
+    const long st12 = stage ? sst : (long)w.nb * w.n;
     int r = launch_cgemm<T>(nb, nc, nb, cone<T>(), Li, w.nb, 0, A + k + (long)c0 * lda, lda, 0, czero<T>(), tmp, nb,
                             st, w.batch, w.inv_stride(), w.sA, st12);
     if (r) return r;
@@ -871,6 +873,57 @@ static int u12_solve(typename cvec<T>::t *A, long lda, int k, int nb, int c0, in
     return 0;
 }
 
+// The panel of an outer step at column k, W columns, on stream ps.  One
+// inner panel when W <= nb; else two: factor the first nb columns, carry their
+// interchanges and U12 over the second, update the second's rows below
+// (tgp), factor it, and carry its interchanges back over the first.
+template <typename T>
+static int outer_panel(typename cvec<T>::t *A, long lda, int n, int k, int W, int *ipiv, LuWork<T> &w, hipStream_t ps,
+                       const TrailingGemm<T> &tgp)
+{
+    const int h1 = min(w.nb, W), h2 = W - h1;
+    int r = panel<T>(A, lda, n, k, h1, ipiv, w, ps);
+    if (r || h2 <= 0) return r;
+    hipLaunchKernelGGL(k_laswp<T>, dim3((h2 + 255) / 256, w.batch), dim3(256), 0, ps, A, lda, w.sA, ipiv, n, k,
+                       k + h1, k + h1, k + W, 0, 0);
+    MIDAS_HIP(hipGetLastError());
+    r = u12_solve<T>(A, lda, k, h1, k + h1, k + W, w, ps, w.u12p, (long)w.nb * w.nb);
+    if (r) return r;
+    r = tgp(n - k - h1, h2, h1, A + (k + h1) + (long)k * lda, lda, A + k + (long)(k + h1) * lda, lda,
+            A + (k + h1) + (long)(k + h1) * lda, lda, ps);
+    if (r) return r;
+    r = panel<T>(A, lda, n, k + h1, h2, ipiv, w, ps);
+    if (r) return r;
+    hipLaunchKernelGGL(k_laswp<T>, dim3((h1 + 255) / 256, w.batch), dim3(256), 0, ps, A, lda, w.sA, ipiv, n, k + h1,
+                       k + W, k, k + h1, 0, 0);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+// U12 of an outer step (rows [k, k + W)) over columns [c0, c1) and the trailing
+// update of those columns: with two inner panels, the second's rows get the
+// first's update before their own triangular solve.
+template <typename T>
+static int outer_update(typename cvec<T>::t *A, long lda, int n, int k, int W, int c0, int c1, LuWork<T> &w,
+                        hipStream_t st, const TrailingGemm<T> &tg)
+{
+    if (c1 <= c0) return 0;
+    const int h1 = min(w.nb, W), h2 = W - h1;
+    int r = u12_solve<T>(A, lda, k, h1, c0, c1, w, st);
+    if (r) return r;
+    if (h2 > 0) {
+        r = tg(h2, c1 - c0, h1, A + (k + h1) + (long)k * lda, lda, A + k + (long)c0 * lda, lda,
+               A + (k + h1) + (long)c0 * lda, lda, st);
+        if (r) return r;
+        r = u12_solve<T>(A, lda, k + h1, h2, c0, c1, w, st);
+        if (r) return r;
+    }
+    const int nr = n - k - W;
+    if (nr <= 0) return 0;
+    return tg(nr, c1 - c0, W, A + (k + W) + (long)k * lda, lda, A + k + (long)c0 * lda, lda,
+              A + (k + W) + (long)c0 * lda, lda, st);
+}
+
 // Look-ahead: the panel of step k+1 is factored on a second stream while the
 // main stream applies the rest of step k's trailing update.  Per step, on the
 // main stream: wait for panel k, row swaps outside it, U12 = L11^-1 A12,
@@ -878,11 +931,15 @@ static int u12_solve(typename cvec<T>::t *A, long lda, int k, int nb, int c0, in
 // of the remaining columns.  The panel stream only touches the next panel's
 // columns, which the remaining update never reads or writes.
 template <typename T>
-int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipStream_t st, TrailingGemm<T> tg)
+int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipStream_t st, TrailingGemm<T> tg,
+          TrailingGemm<T> tgp)
 {
     typedef typename cvec<T>::t V;
     hipStream_t ps = w.pstream;
     MIDAS_CHECK(w.batch == 1 || tg.batch == w.batch, -1, "getrf: trailing GEMM not batched like the factorisation");
+    MIDAS_CHECK(w.outer == 1 || (w.outer == 2 && tgp.scratch != tg.scratch) || sizeof(T) == 8 || tg.mode == TRAIL_NATIVE,
+                -1, "getrf: two-panel steps need a panel-stream GEMM scratch of its own");
+    const int nbo = w.nb * (w.outer == 2 ? 2 : 1);
     // last-workgroup counters, info, the fault flag and the step tags; the
     // tagged records and words (a fill kernel: safe under stream capture)
     TRY_LU(zero_words(w.counter, sizeof(int) * (2 * w.batch + 1 + w.maxblk * w.batch), st));
@@ -892,42 +949,34 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
     w.tag_base = 0;
     MIDAS_HIP(hipEventRecord(w.ev_next, st));
     MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
-    int r = panel<T>(A, lda, n, 0, min(w.nb, n), ipiv, w, ps);
+    int r = outer_panel<T>(A, lda, n, 0, min(nbo, n), ipiv, w, ps, tgp);
     if (r) return r;
     MIDAS_HIP(hipEventRecord(w.ev_panel, ps));
-    for (int k = 0; k < n; k += w.nb) {
-        int nb = min(w.nb, n - k);
+    for (int k = 0; k < n; k += nbo) {
+        const int W = min(nbo, n - k);
         MIDAS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
-        // swaps outside the panel: columns [0, k) and [k+nb, n)
-        int ncols = k + (n - k - nb);
+        // swaps outside the outer panel: columns [0, k) and [k+W, n)
+        int ncols = k + (n - k - W);
         if (ncols > 0) {
             hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + 255) / 256, w.batch), dim3(256), 0, st, A, lda, w.sA, ipiv, n,
-                               k, k + nb, 0, k, k + nb, n);
+                               k, k + W, 0, k, k + W, n);
             MIDAS_HIP(hipGetLastError());
         }
-        int nr = n - k - nb;
+        int nr = n - k - W;
         if (nr <= 0) continue;
-        // The next panel's columns first (U12 = L11^-1 A12, then A22 -= L21 U12),
-        // so that the panel stream can start on them; then the rest.
-        const int nb1 = min(w.nb, nr);
-        r = u12_solve<T>(A, lda, k, nb, k + nb, k + nb + nb1, w, st);
-        if (r) return r;
-        r = tg(nr, nb1, nb, A + (k + nb) + (long)k * lda, lda, A + k + (long)(k + nb) * lda, lda,
-               A + (k + nb) + (long)(k + nb) * lda, lda, st);
+        // The next panel's columns first (U12, then their trailing update), so
+        // that the panel stream can start on them; then the rest.
+        const int W1 = min(nbo, nr);
+        r = outer_update<T>(A, lda, n, k, W, k + W, k + W + W1, w, st, tg);
         if (r) return r;
         MIDAS_HIP(hipEventRecord(w.ev_next, st));
         // The rest of the update is enqueued before the panel: a panel is
         // hundreds of small launches, and the host takes milliseconds to issue
         // them; issued first, the big update runs while they are issued.
-        if (nr > nb1) {
-            r = u12_solve<T>(A, lda, k, nb, k + nb + nb1, n, w, st);
-            if (r) return r;
-            r = tg(nr, nr - nb1, nb, A + (k + nb) + (long)k * lda, lda, A + k + (long)(k + nb + nb1) * lda, lda,
-                   A + (k + nb) + (long)(k + nb + nb1) * lda, lda, st);
-            if (r) return r;
-        }
+        r = outer_update<T>(A, lda, n, k, W, k + W + W1, n, w, st, tg);
+        if (r) return r;
         MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
-        r = panel<T>(A, lda, n, k + nb, nb1, ipiv, w, ps);
+        r = outer_panel<T>(A, lda, n, k + W, W1, ipiv, w, ps, tgp);
         if (r) return r;
         MIDAS_HIP(hipEventRecord(w.ev_panel, ps));
     }
@@ -1202,8 +1251,8 @@ void ipiv_to_perm(const int *ipiv, int n, int *perm)
     }
 }
 
-// sizes of the nine work buffers (pmax, pidx, counter, cand, rowj, pinv, linv, uinv, u12)
-template <typename T> static void lu_sizes(int n, int nb, int batch, size_t need[9])
+// sizes of the ten work buffers (pmax, pidx, counter, cand, rowj, pinv, linv, uinv, u12, u12p)
+template <typename T> static void lu_sizes(int n, int nb, int batch, size_t need[10])
 {
     const size_t maxblk = (n + BLK_T - 1) / BLK_T + 1;   // workgroups of either panel kernel
     const size_t inv = (size_t)((n + nb - 1) / nb) * nb * nb;
@@ -1217,13 +1266,14 @@ template <typename T> static void lu_sizes(int n, int nb, int batch, size_t need
     need[6] = sizeof(cplx<T>) * inv * batch;                         // linv
     need[7] = sizeof(cplx<T>) * inv * batch;                         // uinv
     need[8] = sizeof(cplx<T>) * (size_t)nb * n * batch;              // u12
+    need[9] = sizeof(cplx<T>) * (size_t)nb * nb * batch;             // u12p
 }
 static inline size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 template <typename T>
 size_t LuWork<T>::bytes_needed(int n, int nb, int batch)
 {
-    size_t need[9], t = 0;
+    size_t need[10], t = 0;
     lu_sizes<T>(n, nb, batch, need);
     for (size_t b : need) t += round256(b);
     return t;
@@ -1237,11 +1287,11 @@ int LuWork<T>::alloc(int n_, int nb_, int batch_, long sA_)
     batch = batch_;
     sA = sA_;
     maxblk = (n + BLK_T - 1) / BLK_T + 1;
-    size_t need[9];
+    size_t need[10];
     lu_sizes<T>(n, nb, batch, need);
-    void **buf[9] = {(void **)&pmax, (void **)&pidx, (void **)&counter, (void **)&cand, (void **)&rowj,
-                     (void **)&pinv, (void **)&linv, (void **)&uinv, (void **)&u12};
-    for (int i = 0; i < 9; ++i) {
+    void **buf[10] = {(void **)&pmax, (void **)&pidx, (void **)&counter, (void **)&cand, (void **)&rowj,
+                      (void **)&pinv, (void **)&linv, (void **)&uinv, (void **)&u12, (void **)&u12p};
+    for (int i = 0; i < 10; ++i) {
         if (*buf[i] && need[i] <= cap[i]) continue;
         MIDAS_CHECK(!borrowed, -1, "lu: lent workspace too small (buffer %d: %zu of %zu bytes)", i, cap[i], need[i]);
         if (*buf[i]) MIDAS_HIP(hipFree(*buf[i]));
@@ -1263,15 +1313,15 @@ template <typename T>
 int LuWork<T>::lend(void *mem, size_t bytes, int n_, int nb_, int batch_, hipStream_t ps, hipEvent_t e1, hipEvent_t e2)
 {
     MIDAS_CHECK(mem && ps && e1 && e2, -1, "lu: lend needs memory, a stream and two events");
-    size_t need[9];
+    size_t need[10];
     lu_sizes<T>(n_, nb_, batch_, need);
     MIDAS_CHECK(bytes >= bytes_needed(n_, nb_, batch_), -1, "lu: lent %zu bytes, need %zu", bytes,
                 bytes_needed(n_, nb_, batch_));
     release();
-    void **buf[9] = {(void **)&pmax, (void **)&pidx, (void **)&counter, (void **)&cand, (void **)&rowj,
-                     (void **)&pinv, (void **)&linv, (void **)&uinv, (void **)&u12};
+    void **buf[10] = {(void **)&pmax, (void **)&pidx, (void **)&counter, (void **)&cand, (void **)&rowj,
+                      (void **)&pinv, (void **)&linv, (void **)&uinv, (void **)&u12, (void **)&u12p};
     char *q = (char *)mem;
-    for (int i = 0; i < 9; ++i) {
+    for (int i = 0; i < 10; ++i) {
         *buf[i] = q;
         cap[i] = need[i];
         q += round256(need[i]);
@@ -1298,6 +1348,7 @@ void LuWork<T>::release()
         hipFree(linv);
         hipFree(uinv);
         hipFree(u12);
+        hipFree(u12p);
         hipFree(cand);
         hipFree(rowj);
         if (ev_panel) hipEventDestroy(ev_panel);
@@ -1310,14 +1361,16 @@ void LuWork<T>::release()
     pidx = counter = info = fault = nullptr;
     tag = nullptr;
     pinv = nullptr;
-    linv = uinv = u12 = cand = rowj = nullptr;
+    linv = uinv = u12 = u12p = cand = rowj = nullptr;
     for (size_t &c : cap) c = 0;
 }
 
 template struct LuWork<double>;
 template struct LuWork<float>;
-template int getrf<double>(double2 *, long, int, int *, LuWork<double> &, hipStream_t, TrailingGemm<double>);
-template int getrf<float>(float2 *, long, int, int *, LuWork<float> &, hipStream_t, TrailingGemm<float>);
+template int getrf<double>(double2 *, long, int, int *, LuWork<double> &, hipStream_t, TrailingGemm<double>,
+                            TrailingGemm<double>);
+template int getrf<float>(float2 *, long, int, int *, LuWork<float> &, hipStream_t, TrailingGemm<float>,
+                           TrailingGemm<float>);
 template int getrs<double>(const double2 *, long, int, const int *, const LuWork<double> &, double2 *, long, int,
                            double2 *, hipStream_t, long);
 template int getrs<float>(const float2 *, long, int, const int *, const LuWork<float> &, float2 *, long, int,

@@ -146,3 +146,28 @@ def test_cbfm_budgets_force_batches(mid, oracle, dlca6, use_aca):
     assert list(K1) == list(K0) == list(Ko)
     assert _rel(E1, E0) < 1e-12
     assert _rel(E1, Eo) < 1e-9
+
+
+@pytest.mark.parametrize("mode", [api.SOLVE_F16_IR, api.SOLVE_BF16_IR])
+@pytest.mark.parametrize("n", [1300, 2600])
+def test_two_panel_steps(mid, mode, n):
+    """Outer steps of 1024 columns (two 512-wide panels factored on the panel
+    stream, the second updated by the first; trailing updates with k = 1024):
+    n = 1300 has one such step and a last single panel, n = 2600 two steps and
+    a last step whose second panel is 40 columns wide.  Same solution as the
+    one-panel steps and as LAPACK.  The matrix is diagonally weighted
+    (cond ~ 5): on a plain random one (cond ~ 1e3-1e4) single-term FP16 / BF16
+    factors are too coarse for refinement, with one- or two-panel steps alike,
+    and the solve falls back to FP64."""
+    rng = np.random.default_rng(30 + n)
+    _case(f"two-panel LU steps {api.MODE_NAMES[mode]} n={n}")
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 2.5 * np.sqrt(2 * n) * np.eye(n)
+    B = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    X2, i2 = mid.solve(A, B, mode, nb=1024)
+    X1, i1 = mid.solve(A, B, mode, nb=512)
+    Xo = np.linalg.solve(A, B)
+    print("two-panel", i2, "one-panel", i1, "vs numpy", _rel(X2, Xo))
+    assert i2["status"] == api.STATUS_CONVERGED and i2["info"] == 0
+    assert i2["berr"] <= 1e-14
+    assert _rel(X2, Xo) < 1e-10
+    assert _rel(X2, X1) < 1e-10
