@@ -790,9 +790,11 @@ template <typename T> static cplx<T> cone() { return {(T)1, (T)0}; }
 template <typename T> static cplx<T> cmone() { return {(T)-1, (T)0}; }
 template <typename T> static cplx<T> czero() { return {(T)0, (T)0}; }
 
-// Panel [k, k+nb) of an n x n matrix.
+// Columns [k, k+nb) of an n x n matrix factored right-looking in inner
+// blocks of LU_IB columns: each block's pivots and rank-IB update reach only
+// these columns.
 template <typename T>
-static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ipiv, LuWork<T> &w, hipStream_t st)
+static int panel_leaf(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ipiv, LuWork<T> &w, hipStream_t st)
 {
     // one launch per inner block while the workgroups of a matrix fit on the
     // device together, else one launch per column (MIDAS_LU_COLUMN_STEPS=1
@@ -846,8 +848,63 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
             MIDAS_HIP(hipGetLastError());
         }
     }
-    // inverse of the unit-lower diagonal block L11 of this panel: U12 = L11^-1 A12
-    // becomes one GEMM on the main stream, and getrs reuses it
+    return 0;
+}
+
+// Columns [k, k+nb) factored recursively: the left half, its interchanges and
+// L11^-1 carried over the right half, the right half's rows below updated by
+// one GEMM (complex FP32 / FP64 on the matrix cores), the right half, and its
+// interchanges carried back over the left half; halves of at most
+// PANEL_LEAF columns go to panel_leaf.  Against one right-looking pass over the
+// whole panel, the columns right of an inner block are read and written once
+// per recursion level instead of once per inner block.  Scratch: w.u12p
+// (L11^-1 of the left half, then L11^-1 A12).
+constexpr int PANEL_LEAF = 64;
+template <typename T>
+static int panel_rec(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ipiv, LuWork<T> &w, hipStream_t st)
+{
+    typedef typename cvec<T>::t V;
+    if (nb <= PANEL_LEAF) return panel_leaf<T>(A, lda, n, k, nb, ipiv, w, st);
+    const int h = ((nb / 2 + LU_IB - 1) / LU_IB) * LU_IB, h2 = nb - h;
+    int r = panel_rec<T>(A, lda, n, k, h, ipiv, w, st);
+    if (r) return r;
+    hipLaunchKernelGGL(k_laswp<T>, dim3((h2 + 255) / 256, w.batch), dim3(256), 0, st, A, lda, w.sA, ipiv, n, k, k + h,
+                       k + h, k + nb, 0, 0);
+    MIDAS_HIP(hipGetLastError());
+    // A12 <- L11^-1 A12 through the staging buffer (nb x nb per matrix)
+    V *Li = (V *)w.u12p, *P = Li + (long)h * h;
+    const long sU = (long)w.nb * w.nb;
+    r = trinv<T, true>(A + k + (long)k * lda, lda, h, Li, h, st, w.batch, w.sA, sU);
+    if (r) return r;
+    r = launch_cgemm<T>(h, h2, h, cone<T>(), Li, h, 0, A + k + (long)(k + h) * lda, lda, 0, czero<T>(), P, h, st,
+                        w.batch, sU, w.sA, sU);
+    if (r) return r;
+    hipLaunchKernelGGL(k_copy2d<T>, dim3((h + 255) / 256, h2, w.batch), dim3(256), 0, st, P, (long)h,
+                       A + k + (long)(k + h) * lda, lda, h, h2, sU, w.sA);
+    MIDAS_HIP(hipGetLastError());
+    // A22 -= A21 A12 for the rows below
+    if (n - k - h > 0) {
+        r = launch_cgemm<T>(n - k - h, h2, h, cmone<T>(), A + (k + h) + (long)k * lda, lda, 0,
+                            A + k + (long)(k + h) * lda, lda, 0, cone<T>(), A + (k + h) + (long)(k + h) * lda, lda,
+                            st, w.batch, w.sA, w.sA, w.sA);
+        if (r) return r;
+    }
+    r = panel_rec<T>(A, lda, n, k + h, h2, ipiv, w, st);
+    if (r) return r;
+    hipLaunchKernelGGL(k_laswp<T>, dim3((h + 255) / 256, w.batch), dim3(256), 0, st, A, lda, w.sA, ipiv, n, k + h,
+                       k + nb, k, k + h, 0, 0);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+// Panel [k, k+nb) of an n x n matrix, then the inverse of its unit-lower
+// diagonal block L11: U12 = L11^-1 A12 becomes one GEMM on the main stream,
+// and getrs reuses it.
+template <typename T>
+static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ipiv, LuWork<T> &w, hipStream_t st)
+{
+    int r = panel_rec<T>(A, lda, n, k, nb, ipiv, w, st);
+    if (r) return r;
     return trinv<T, true>(A + k + (long)k * lda, lda, nb, (typename cvec<T>::t *)w.linv + (long)(k / w.nb) * w.nb * w.nb,
                           w.nb, st, w.batch, w.sA, w.inv_stride());
 }

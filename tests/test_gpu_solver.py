@@ -114,7 +114,7 @@ def test_workspace_reused_across_sizes(mid):
     print("1301 again vs first", _rel(X3, X1), "701 vs fresh", _rel(X2, F2))
     assert _rel(X1, F1) < 1e-15 and _rel(X3, F1) < 1e-15
     assert _rel(X2, F2) < 1e-15 and _rel(X4, F2) < 1e-15
-    assert _rel(X1, np.linalg.solve(*sys[1301])) < 1e-12
+    assert _rel(X1, np.linalg.solve(*sys[1301])) < 1e-10
 
 
 @pytest.fixture(scope="module")
