@@ -68,8 +68,11 @@ __host__ __device__ inline long plane_off(int pc, int row, int k, int rtiles, in
 
 // A (m x K, lda) -> planes; thread per (row, group of 8 k)
 template <bool BF, int P>
-__global__ void k_split_a(const float2 *__restrict__ A, long lda, int m, int K, int rtiles, int kst, u16 *__restrict__ out)
+__global__ void k_split_a(const float2 *__restrict__ A, long lda, int m, int K, int rtiles, int kst, u16 *__restrict__ out,
+                          long sA = 0, long sOut = 0)
 {
+    A += blockIdx.z * sA;      // matrix of a batch
+    out += blockIdx.z * sOut;
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     const int k0 = blockIdx.y * 8;
     if (row >= rtiles * T_M) return;
@@ -101,8 +104,12 @@ __global__ void k_split_a(const float2 *__restrict__ A, long lda, int m, int K, 
 constexpr int SPLIT_B_KMAX = 2048;
 template <bool BF, int P>
 __global__ __launch_bounds__(64) void k_split_b(const float2 *__restrict__ B, long ldb, int K, int n, int rtiles,
-                                                int kst, u16 *__restrict__ out, float *__restrict__ scale)
+                                                int kst, u16 *__restrict__ out, float *__restrict__ scale,
+                                                long sB = 0, long sOut = 0, long sScale = 0)
 {
+    B += blockIdx.y * sB;      // matrix of a batch
+    out += blockIdx.y * sOut;
+    scale += blockIdx.y * sScale;
     constexpr int G = SPLIT_B_KMAX / 512;
     const int col = blockIdx.x;
     const int t = threadIdx.x;
@@ -402,8 +409,15 @@ __shared__ __attribute__((aligned(16))) unsigned char g2_slot2[sizeof(Stage2)];
 template <bool BF>
 __global__ __launch_bounds__(256, 2) void k_trail16x(int m, int n, int kst, int mt, int nt, int group,
                                                      const u16 *__restrict__ Ap, const u16 *__restrict__ Bp,
-                                                     const float *__restrict__ bscale, float2 *__restrict__ C, long ldc)
+                                                     const float *__restrict__ bscale, float2 *__restrict__ C, long ldc,
+                                                     long sP = 0, long sScale = 0, long sC = 0)
 {
+    // matrix of a batch (grid y): its planes at sP halves from the previous
+    // one's, its scales at sScale, its C at sC
+    Ap += blockIdx.y * sP;
+    Bp += blockIdx.y * sP;
+    bscale += blockIdx.y * sScale;
+    C += blockIdx.y * sC;
     constexpr int IPS = 4;   // copy instructions per wave per stage (one 1 KB part of each of 4 plane tiles)
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -556,31 +570,44 @@ size_t trail16_scratch_bytes(int mmax, int nmax, int kmax)
     return 2 * 4 * (rt_m * T_M + rt_n * T_N) * kst * T_K + 4 * rt_n * T_N + 256;
 }
 
+// Split-operand planes of one matrix for an (m x n x k) update, in bytes
+// (a multiple of 256): the matrices of a batch sit this far apart in scratch.
+static size_t trail16_matrix_bytes(int m, int n, int k)
+{
+    return (trail16_scratch_bytes(m, n, k) + 255) & ~(size_t)255;
+}
+
 template <bool BF, int P, int TERMS>
 static int run_trail(int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C, long ldc,
-                     char *scratch, hipStream_t st)
+                     char *scratch, hipStream_t st, int batch, long stride)
 {
     const int mt = (m + T_M - 1) / T_M, nt = (n + T_N - 1) / T_N, kst = kstages(k);
     u16 *Ap = (u16 *)scratch;
     u16 *Bp = Ap + (size_t)2 * P * mt * T_M * kst * T_K;
     float *sc = (float *)(Bp + (size_t)2 * P * nt * T_N * kst * T_K);
+    MIDAS_CHECK(batch >= 1 && batch <= 65535 && (batch == 1 || P == 1), -1, "trail16: batch %d (P %d)", batch, P);
     if constexpr (P == 1) {
         // two workgroups per CU (MIDAS_TRAIL_1WG=1 selects the one-workgroup kernel)
         const char *e = getenv("MIDAS_TRAIL_1WG");
-        if (!(e && atoi(e) != 0)) {
+        if (!(e && atoi(e) != 0) || batch > 1) {
             MIDAS_CHECK(k <= SPLIT_B_KMAX, -1, "trail16: k = %d above %d", k, SPLIT_B_KMAX);
             const int ks = (k + T_K - 1) / T_K;   // 16-k stages, no ring padding
             u16 *Bq = Ap + (size_t)2 * mt * T_M * ks * T_K;
             float *sq = (float *)(Bq + (size_t)2 * nt * T_N * ks * T_K);
-            hipLaunchKernelGGL((k_split_a<BF, 1>), dim3(mt * T_M / 256 + 1, ks * 2), dim3(256), 0, st, A, lda, m, k,
-                               mt, ks, Ap);
-            hipLaunchKernelGGL((k_split_b<BF, 1>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, ks, Bq, sq);
+            // per matrix of a batch: planes (A's, then B's, then the scales)
+            // of trail16_matrix_bytes each
+            const long mb = (long)trail16_matrix_bytes(m, n, k);
+            const long sP = mb / 2, sS = mb / 4;
+            hipLaunchKernelGGL((k_split_a<BF, 1>), dim3(mt * T_M / 256 + 1, ks * 2, batch), dim3(256), 0, st, A, lda, m,
+                               k, mt, ks, Ap, stride, sP);
+            hipLaunchKernelGGL((k_split_b<BF, 1>), dim3(nt * T_N, batch), dim3(64), 0, st, B, ldb, k, n, nt, ks, Bq, sq,
+                               stride, sP, sS);
             // algorithmic bytes: C read and written, both operands once in FP16
-            ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 16.0 * m * n + 4.0 * ((double)m + n) * k);
+            ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k * batch, (16.0 * m * n + 4.0 * ((double)m + n) * k) * batch);
             const long a_tile = (long)T_M * ks * T_K * 2 * 2;
             const int group = (int)std::max(2L, std::min(16L, (2L << 20) / a_tile));
-            hipLaunchKernelGGL((k_trail16x<BF>), dim3(mt * nt), dim3(256), 0, st, m, n, ks, mt, nt, group, Ap, Bq, sq,
-                               C, ldc);
+            hipLaunchKernelGGL((k_trail16x<BF>), dim3(mt * nt, batch), dim3(256), 0, st, m, n, ks, mt, nt, group, Ap,
+                               Bq, sq, C, ldc, sP, sS, stride);
             MIDAS_HIP(hipGetLastError());
             return 0;
         }
@@ -603,18 +630,23 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
 }
 
 int launch_trail16(int mode, int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
-                   long ldc, void *scratch, hipStream_t st)
+                   long ldc, void *scratch, hipStream_t st, int batch, long stride)
 {
     if (m <= 0 || n <= 0 || k <= 0) return 0;
     MIDAS_CHECK(scratch, -1, "trail16: no scratch");
     char *s = (char *)scratch;
     switch (mode) {
-    case TRAIL_F16_1: return run_trail<false, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st);
-    case TRAIL_F16_3: return run_trail<false, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st);
-    case TRAIL_BF16_1: return run_trail<true, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st);
-    case TRAIL_BF16_3: return run_trail<true, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st);
+    case TRAIL_F16_1: return run_trail<false, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride);
+    case TRAIL_F16_3: return run_trail<false, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride);
+    case TRAIL_BF16_1: return run_trail<true, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride);
+    case TRAIL_BF16_3: return run_trail<true, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride);
     }
     MIDAS_CHECK(false, -1, "trail16: bad mode %d", mode);
+}
+
+size_t trail16_batch_bytes(int mmax, int nmax, int kmax, int batch)
+{
+    return trail16_matrix_bytes(mmax, nmax, kmax) * (size_t)batch;
 }
 
 // One rocBLAS handle per (host thread, device), created on first use and kept
@@ -673,9 +705,10 @@ int TrailingGemm<float>::operator()(int m, int n, int k, const void *A, long lda
         return launch_cgemm<float>(m, n, k, c32{-1.f, 0.f}, A, lda, 0, B, ldb, 0, c32{1.f, 0.f}, C, ldc, st, batch,
                                    stride, stride, stride);
     }
-    MIDAS_CHECK(batch == 1, -1, "trail16: batched split-precision updates are not supported");
+    MIDAS_CHECK(batch == 1 || mode == TRAIL_F16_1 || mode == TRAIL_BF16_1, -1,
+                "trail16: batched split-precision updates are single-term only");
     return launch_trail16(mode, m, n, k, (const float2 *)A, lda, (const float2 *)B, ldb, (float2 *)C, ldc, scratch,
-                          st);
+                          st, batch, stride);
 }
 
 }  // namespace midas

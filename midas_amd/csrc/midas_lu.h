@@ -58,6 +58,8 @@ template <typename T> struct TrailingGemm {
 };
 
 size_t trail16_scratch_bytes(int mmax, int nmax, int kmax);
+// scratch of a batched single-term split update (matrices of up to mmax x nmax, k <= kmax)
+size_t trail16_batch_bytes(int mmax, int nmax, int kmax, int batch);
 
 template <typename T> struct cvec;
 template <> struct cvec<double> { typedef double2 t; };
