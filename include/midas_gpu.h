@@ -104,6 +104,15 @@ int midas_set_cbfm_budgets(midas_ctx *ctx, double lu_bytes, double strip_bytes, 
                            double aca_bytes);
 const char *midas_last_error(void);
 void *midas_stream(midas_ctx *ctx);           /* hipStream_t of the context */
+/* Schedule parameters (process-wide; defaults chosen by the project's
+ * autotuner, midas_amd/autotune.py): "lu_nb_split", "lu_two_panel_min_n",
+ * "lu_nb_mixed", "lu_nb_fp64" (LU panel widths), "panel_leaf" (recursive panel
+ * leaves), "trail_group_kib" (L2 share of a row-tile group in the split GEMM),
+ * "refine_graph" (1: refinement sweeps replayed from a captured hipGraph).
+ * They change speed, not results beyond rounding.  No reference counterpart
+ * (the reference fixes its 32 x 32 ScaLAPACK blocks, Initialization.f90:66-67). */
+int midas_tune_set(const char *name, int value);
+int midas_tune_get(const char *name, int *value);
 
 /* Live kernel accounting: midas_profile(ctx, 1) starts recording HIP events
  * around the trailing-update GEMM launches of the LU (kind 0) and the matrix

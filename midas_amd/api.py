@@ -44,6 +44,7 @@ EXPORTS = (
     "midas_extend_blocks", "midas_scattering_quantities", "midas_near_field", "midas_partition_sph",
     "midas_partition_csh", "midas_scattering_quantities_grid", "midas_quad_adaptive", "midas_quad_trap2d",
     "midas_quad_gauss_legendre", "midas_release_workspace", "midas_set_cbfm_budgets", "midas_dev_mom_solve_noalloc",
+    "midas_tune_set", "midas_tune_get",
 )
 
 RX_SD, RX_RF, RX_LB = 0, 1, 2
@@ -105,6 +106,24 @@ def load():
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+TUNE_NAMES = ("lu_nb_split", "lu_two_panel_min_n", "lu_nb_mixed", "lu_nb_fp64", "panel_leaf", "trail_group_kib",
+              "refine_graph")
+
+
+def tune_get(name):
+    """A schedule parameter of the library (process-wide)."""
+    v = C.c_int(0)
+    _check(load().midas_tune_get(name.encode(), C.byref(v)), "tune_get")
+    return v.value
+
+
+def tune_set(name, value):
+    """Set a schedule parameter (process-wide); returns the previous value."""
+    old = tune_get(name)
+    _check(load().midas_tune_set(name.encode(), C.c_int(int(value))), "tune_set")
+    return old
 
 
 def _check(r, what):

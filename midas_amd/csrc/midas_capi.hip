@@ -2,6 +2,7 @@
 #include "midas_common.h"
 #include "midas_lu.h"
 #include "midas_host.h"
+#include "midas_tune.h"
 #include "../../include/midas_gpu.h"
 #include "../../include/midas_gpu_test.h"
 
@@ -115,14 +116,23 @@ static int trail_mode(int mode)
     return -1;
 }
 
-// Outer step width of the LU: 128 for FP64; 512 for the mixed modes, and 1024
-// (two 512-wide panels per step: trailing updates of 1024 columns at twice
-// the arithmetic per byte of C) for the single-term split modes on large n.
+// Outer step width of the LU (tuned defaults, midas_tune_set): FP64; the
+// mixed modes; and for the single-term split modes on large n a wider step
+// (above 512: two panels of half its width per step, so the trailing updates
+// carry twice the arithmetic per byte of C).
 static int default_nb(int n, int mode)
 {
-    if (mode == MIDAS_SOLVE_FP64) return 128;
-    if ((mode == MIDAS_SOLVE_F16_IR || mode == MIDAS_SOLVE_BF16_IR) && n >= 16384) return 1024;
-    return 512;
+    if (mode == MIDAS_SOLVE_FP64) return tune(TUNE_LU_NB_FP64);
+    if ((mode == MIDAS_SOLVE_F16_IR || mode == MIDAS_SOLVE_BF16_IR) && n >= tune(TUNE_LU_TWO_PANEL_MIN_N))
+        return tune(TUNE_LU_NB_SPLIT);
+    return tune(TUNE_LU_NB_MIXED);
+}
+
+__global__ void k_zeroz(double2 *X, long ldx, int m, int n)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (r < m && c < n) X[(long)r + (long)c * ldx] = make_double2(0.0, 0.0);
 }
 
 // perm from LAPACK interchanges for one matrix, on the device: one workgroup,
@@ -182,11 +192,10 @@ static int fp64_lu_solve(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, double2 *
     return 0;
 }
 
-// R = B - Z X, the column maxima of X and R to the host, and the normwise
-// backward error max_j ||r_j|| / (||Z|| ||x_j|| + ||b_j||) (infinity norms).
-static int residual_berr(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, const double2 *Z, long ldz, const double2 *B,
-                         long ldb, const double2 *X, long ldx, double anorm, const std::vector<double> &hb,
-                         double *berr)
+// R = B - Z X and the column maxima of X and R (device work only; a stream
+// capture may record it).
+static int residual_launch(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, const double2 *Z, long ldz,
+                           const double2 *B, long ldb, const double2 *X, long ldx)
 {
     hipStream_t st = ctx->st;
     double *colx = ws.norms.as<double>() + n, *colr = colx + nrhs;
@@ -198,14 +207,25 @@ static int residual_berr(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, const dou
     } else {
         hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R, (long)n, n, nrhs);
         // many right-hand sides: rocBLAS (33k x 448 x 33k: 60 ms against 74 ms for k_cgemm<double>)
-        r = nrhs >= 32 && !ws.lent ? blas_zgemm_nn(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R, n, 0, 1, st)
-                       : launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0}, R, n,
-                                              st);
+        r = nrhs >= 32 && !ws.lent ? blas_zgemm_nn(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0},
+                                                   R, n, 0, 1, st)
+                                   : launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0,
+                                                          z64{1.0, 0.0}, R, n, st);
     }
     if (r) return r;
     hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, X, ldx, n, colx);
     hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, st, R, (long)n, n, colr);
     MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+// The column maxima to the host and the normwise backward error
+// max_j ||r_j|| / (||Z|| ||x_j|| + ||b_j||) (infinity norms).
+static int berr_read(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, double anorm, const std::vector<double> &hb,
+                     double *berr)
+{
+    hipStream_t st = ctx->st;
+    const double *colx = ws.norms.as<double>() + n, *colr = colx + nrhs;
     std::vector<double> hx(nrhs), hr(nrhs);
     TRY(copy_sync(hx.data(), colx, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
     TRY(copy_sync(hr.data(), colr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
@@ -216,6 +236,66 @@ static int residual_berr(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, const dou
         b = (e > b || std::isnan(e)) ? e : b;
     }
     *berr = b;
+    return 0;
+}
+
+static int residual_berr(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, const double2 *Z, long ldz, const double2 *B,
+                         long ldb, const double2 *X, long ldx, double anorm, const std::vector<double> &hb,
+                         double *berr)
+{
+    TRY(residual_launch(ctx, ws, n, nrhs, Z, ldz, B, ldb, X, ldx));
+    return berr_read(ctx, ws, n, nrhs, anorm, hb, berr);
+}
+
+// One refinement sweep's device work: D = LU^-1 R (FP32), X += D, R = B - Z X,
+// column maxima.  The same launches every sweep, so they can be captured once.
+struct SweepArgs {
+    int n, nrhs;
+    const float2 *A;
+    long lda;
+    const LuWork<float> *w;
+    double2 *Z;
+    long ldz;
+    const double2 *B;
+    long ldb;
+    double2 *X;
+    long ldx;
+};
+static int sweep_launch(midas_ctx *ctx, SolveWs &ws, const SweepArgs &a)
+{
+    hipStream_t st = ctx->st;
+    const dim3 g2((a.n + 255) / 256, a.nrhs);
+    float2 *D = ws.D.as<float2>();
+    hipLaunchKernelGGL(k_z2c, g2, dim3(256), 0, st, ws.R.as<double2>(), (long)a.n, D, (long)a.n, a.n, a.nrhs);
+    TRY(getrs<float>(a.A, a.lda, a.n, ws.perm.as<int>(), *a.w, D, (long)a.n, a.nrhs, ws.t2.as<float2>(), st));
+    hipLaunchKernelGGL(k_c2z_add, g2, dim3(256), 0, st, D, (long)a.n, a.X, a.ldx, a.n, a.nrhs, 1);
+    return residual_launch(ctx, ws, a.n, a.nrhs, a.Z, a.ldz, a.B, a.ldb, a.X, a.ldx);
+}
+
+// A captured sweep (hipGraph), destroyed with its owner.
+struct SweepGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    ~SweepGraph()
+    {
+        if (e) hipGraphExecDestroy(e);
+        if (g) hipGraphDestroy(g);
+    }
+};
+static int sweep_capture(midas_ctx *ctx, SolveWs &ws, const SweepArgs &a, SweepGraph &sg)
+{
+    hipStream_t st = ctx->st;
+    MIDAS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int r = sweep_launch(ctx, ws, a);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (r) {
+        if (g) hipGraphDestroy(g);
+        return r;
+    }
+    MIDAS_HIP(e);
+    sg.g = g;
+    MIDAS_HIP(hipGraphInstantiate(&sg.e, sg.g, nullptr, nullptr, 0));
     return 0;
 }
 
@@ -283,7 +363,8 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
     // an outer step wider than 512 is two inner panels of half its width
     // (single-term split modes only: their GEMM takes k up to 2048)
     const bool two = nb > 512 && (mode == MIDAS_SOLVE_F16_IR || mode == MIDAS_SOLVE_BF16_IR);
-    MIDAS_CHECK(nb <= 512 || two, -1, "solve: panel width %d above 512 needs a single-term split mode", nb);
+    MIDAS_CHECK(nb <= 512 || (two && nb <= 1024), -1,
+                "solve: panel width %d: above 512 needs a single-term split mode, at most 1024", nb);
     const int nbi = two ? (nb + 1) / 2 : nb;
     LuWork<float> &w = ws.luf;
     TRY(w.alloc(n, nbi));
@@ -308,7 +389,6 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
     TRY(ws.t2.reserve(sizeof(float2) * (size_t)getrs_tmp_elems(n, nb, nrhs)));
     TRY(ws.xbest.reserve(sizeof(double2) * (size_t)n * nrhs));
     double2 *R = ws.R.as<double2>(), *Xb = ws.xbest.as<double2>();
-    float2 *D = ws.D.as<float2>();
     const dim3 g2((n + 255) / 256, nrhs);
     hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R, (long)n, n, nrhs);
     std::vector<double> hb(nrhs);
@@ -318,13 +398,22 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
     double berr = INFINITY, best = INFINITY, prev = INFINITY;
     int it = 0;
     bool best_saved = false;   // Xb holds an iterate better than the current X
+    // X = 0, then every sweep adds its correction.  The sweeps repeat the same
+    // hundreds of small launches (the few-RHS substitution is one per block
+    // column): captured once into a hipGraph and replayed, the host issues one
+    // launch per sweep (not under no-allocation use, and not around the
+    // library GEMM of the many-RHS residual).
+    hipLaunchKernelGGL(k_zeroz, g2, dim3(256), 0, st, X, ldx, n, nrhs);
+    MIDAS_HIP(hipGetLastError());
+    const SweepArgs sa{n, nrhs, A, lda, &w, Z, ldz, B, ldb, X, ldx};
+    const bool use_graph = tune(TUNE_REFINE_GRAPH) && !ws.lent && nrhs < 32 && !profile().on;
+    SweepGraph sg;
+    if (use_graph) TRY(sweep_capture(ctx, ws, sa, sg));
     for (; it < max_iter; ++it) {
-        hipLaunchKernelGGL(k_z2c, g2, dim3(256), 0, st, R, (long)n, D, (long)n, n, nrhs);
-        r = getrs<float>(A, lda, n, ws.perm.as<int>(), w, D, (long)n, nrhs, ws.t2.as<float2>(), st);
-        if (r) return r;
-        hipLaunchKernelGGL(k_c2z_add, g2, dim3(256), 0, st, D, (long)n, X, ldx, n, nrhs, it > 0);
+        if (use_graph) MIDAS_HIP(hipGraphLaunch(sg.e, st));
+        else TRY(sweep_launch(ctx, ws, sa));
         if (it == 0) inf.t_solve_ms = tm.ms();
-        TRY(residual_berr(ctx, ws, n, nrhs, Z, ldz, B, ldb, X, ldx, inf.anorm, hb, &berr));
+        TRY(berr_read(ctx, ws, n, nrhs, inf.anorm, hb, &berr));
         if (std::isnan(berr)) {
             best_saved = best < INFINITY;
             ++it;

@@ -35,6 +35,7 @@
 #include "midas_elem.h"
 #include "midas_host.h"
 #include "midas_lu.h"
+#include "midas_tune.h"
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -461,7 +462,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return sb[a] > sb[c]; });
         std::vector<Chunk> chunks;
         long zmax = 0, smax_c = 0, tmax = 0, xmax = 0, pmax_c = 0;
-        const int nb_lu = 128;
+        const int nb_lu = tune(TUNE_LU_NB_FP64);
         for (int pos = 0; pos < nb;) {
             const long mpc = 3L * sb[order[pos]];
             const long zb = (long)sizeof(double2) * mpc * mpc;

@@ -23,6 +23,7 @@
 // workgroups of one XCD share a column block of B (L2 reuse).
 #include "midas_common.h"
 #include "midas_lu.h"
+#include "midas_tune.h"
 
 #include <rocblas/rocblas.h>
 
@@ -605,7 +606,7 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
             // algorithmic bytes: C read and written, both operands once in FP16
             ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k * batch, (16.0 * m * n + 4.0 * ((double)m + n) * k) * batch);
             const long a_tile = (long)T_M * ks * T_K * 2 * 2;
-            const int group = (int)std::max(2L, std::min(16L, (2L << 20) / a_tile));
+            const int group = (int)std::max(2L, std::min(16L, ((long)tune(TUNE_TRAIL_GROUP_KIB) << 10) / a_tile));
             hipLaunchKernelGGL((k_trail16x<BF>), dim3(mt * nt, batch), dim3(256), 0, st, m, n, ks, mt, nt, group, Ap,
                                Bq, sq, C, ldc, sP, sS, stride);
             MIDAS_HIP(hipGetLastError());
@@ -622,7 +623,7 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
     ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k, 16.0 * m * n + 4.0 * P * ((double)m + n) * k);
     // row tiles per group: A tiles of the group in ~2 MB of the XCD's L2
     const long a_tile = (long)T_M * kst * T_K * 2 * P * 2;
-    const int group = (int)std::max(2L, std::min(16L, (2L << 20) / a_tile));
+    const int group = (int)std::max(2L, std::min(16L, ((long)tune(TUNE_TRAIL_GROUP_KIB) << 10) / a_tile));
     hipLaunchKernelGGL((k_trail16<BF, P, TERMS>), dim3(mt * nt), dim3(256), 0, st, m, n, kst, mt, nt, group, Ap, Bp,
                        sc, C, ldc);
     MIDAS_HIP(hipGetLastError());

@@ -21,6 +21,7 @@
 // on the FP16/BF16 split kernels (midas_gemm16.hip).
 #include "midas_common.h"
 #include "midas_lu.h"
+#include "midas_tune.h"
 
 #include <algorithm>
 
@@ -855,16 +856,15 @@ static int panel_leaf(typename cvec<T>::t *A, long lda, int n, int k, int nb, in
 // L11^-1 carried over the right half, the right half's rows below updated by
 // one GEMM (complex FP32 / FP64 on the matrix cores), the right half, and its
 // interchanges carried back over the left half; halves of at most
-// PANEL_LEAF columns go to panel_leaf.  Against one right-looking pass over the
+// tune(TUNE_PANEL_LEAF) columns go to panel_leaf.  Against one right-looking pass over the
 // whole panel, the columns right of an inner block are read and written once
 // per recursion level instead of once per inner block.  Scratch: w.u12p
 // (L11^-1 of the left half, then L11^-1 A12).
-constexpr int PANEL_LEAF = 64;
 template <typename T>
 static int panel_rec(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ipiv, LuWork<T> &w, hipStream_t st)
 {
     typedef typename cvec<T>::t V;
-    if (nb <= PANEL_LEAF) return panel_leaf<T>(A, lda, n, k, nb, ipiv, w, st);
+    if (nb <= tune(TUNE_PANEL_LEAF)) return panel_leaf<T>(A, lda, n, k, nb, ipiv, w, st);
     const int h = ((nb / 2 + LU_IB - 1) / LU_IB) * LU_IB, h2 = nb - h;
     int r = panel_rec<T>(A, lda, n, k, h, ipiv, w, st);
     if (r) return r;

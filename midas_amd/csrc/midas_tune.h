@@ -1,0 +1,17 @@
+// Run-time schedule parameters (midas_tune.hip).
+#pragma once
+#include "midas_common.h"
+
+namespace midas {
+enum {
+    TUNE_LU_NB_SPLIT,
+    TUNE_LU_TWO_PANEL_MIN_N,
+    TUNE_LU_NB_MIXED,
+    TUNE_LU_NB_FP64,
+    TUNE_PANEL_LEAF,
+    TUNE_TRAIL_GROUP_KIB,
+    TUNE_REFINE_GRAPH,
+    TUNE_COUNT
+};
+int tune(int which);
+}  // namespace midas

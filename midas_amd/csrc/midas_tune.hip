@@ -1,0 +1,54 @@
+// Run-time schedule parameters of the library (defaults in midas_tuned.h, set
+// by midas_amd/autotune.py), readable and settable through the C-ABI so that
+// the autotuner can time alternatives.
+#include "midas_tune.h"
+#include "midas_tuned.h"
+#include "../../include/midas_gpu.h"
+
+#include <cstring>
+
+namespace midas {
+
+struct TuneEntry {
+    const char *name;
+    int value, lo, hi;
+};
+static TuneEntry g_tune[TUNE_COUNT] = {
+    {"lu_nb_split", MIDAS_TUNED_LU_NB_SPLIT, 16, 1024},
+    {"lu_two_panel_min_n", MIDAS_TUNED_LU_TWO_PANEL_MIN_N, 0, 1 << 30},
+    {"lu_nb_mixed", MIDAS_TUNED_LU_NB_MIXED, 16, 512},
+    {"lu_nb_fp64", MIDAS_TUNED_LU_NB_FP64, 16, 512},
+    {"panel_leaf", MIDAS_TUNED_PANEL_LEAF, 16, 512},
+    {"trail_group_kib", MIDAS_TUNED_TRAIL_GROUP_KIB, 64, 65536},
+    {"refine_graph", MIDAS_TUNED_REFINE_GRAPH, 0, 1},
+};
+
+int tune(int which) { return g_tune[which].value; }
+
+}  // namespace midas
+
+using namespace midas;
+
+extern "C" int midas_tune_set(const char *name, int value)
+{
+    MIDAS_CHECK(name, -1, "tune_set: no name");
+    for (TuneEntry &e : g_tune)
+        if (!std::strcmp(e.name, name)) {
+            MIDAS_CHECK(value >= e.lo && value <= e.hi, -1, "tune_set: %s = %d outside [%d, %d]", name, value, e.lo,
+                        e.hi);
+            e.value = value;
+            return 0;
+        }
+    MIDAS_CHECK(false, -1, "tune_set: unknown parameter %s", name);
+}
+
+extern "C" int midas_tune_get(const char *name, int *value)
+{
+    MIDAS_CHECK(name && value, -1, "tune_get: bad arguments");
+    for (const TuneEntry &e : g_tune)
+        if (!std::strcmp(e.name, name)) {
+            *value = e.value;
+            return 0;
+        }
+    MIDAS_CHECK(false, -1, "tune_get: unknown parameter %s", name);
+}

@@ -1,0 +1,12 @@
+// Tuned defaults of the library's schedule parameters, written by
+// midas_amd/autotune.py from timings on one MI355X (its measurements are in
+// midas_amd/tuning.json).  Each can be changed at run time with
+// midas_tune_set (include/midas_gpu.h).
+#pragma once
+#define MIDAS_TUNED_LU_NB_SPLIT            1024     // LU outer step, single-term split modes, n >= lu_two_panel_min_n
+#define MIDAS_TUNED_LU_TWO_PANEL_MIN_N     24000    // from this order on the split modes take two-panel steps
+#define MIDAS_TUNED_LU_NB_MIXED            512      // LU panel, the other mixed modes (and smaller n)
+#define MIDAS_TUNED_LU_NB_FP64             256      // LU panel, FP64
+#define MIDAS_TUNED_PANEL_LEAF             128      // columns of a recursive panel's leaves
+#define MIDAS_TUNED_TRAIL_GROUP_KIB        2048     // L2 share (KiB) of a row-tile group of the split GEMM
+#define MIDAS_TUNED_REFINE_GRAPH           0        // refinement sweeps replayed from one captured hipGraph

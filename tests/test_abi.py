@@ -132,3 +132,19 @@ def test_cbf_direction_grid_matches_reference(oracle):
         tho, pho = oracle.transmitters_un(ni)
         assert len(th) == len(tho)
         assert np.abs(th - tho).max() < 1e-12 and np.abs(ph - pho).max() < 1e-12
+
+
+def test_tune_parameters_roundtrip(lib):
+    """The schedule parameters the autotuner varies: readable, settable within
+    their ranges, refused outside them and for unknown names (no GPU needed)."""
+    from midas_amd import api
+    for name in api.TUNE_NAMES:
+        v = api.tune_get(name)
+        assert v == api.tune_set(name, v)
+    old = api.tune_set("panel_leaf", 32)
+    assert api.tune_get("panel_leaf") == 32
+    api.tune_set("panel_leaf", old)
+    with pytest.raises(api.MidasError):
+        api.tune_set("panel_leaf", 4)
+    with pytest.raises(api.MidasError):
+        api.tune_get("no_such_parameter")

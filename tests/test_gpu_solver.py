@@ -171,3 +171,50 @@ def test_two_panel_steps(mid, mode, n):
     assert i2["berr"] <= 1e-14
     assert _rel(X2, Xo) < 1e-10
     assert _rel(X2, X1) < 1e-10
+
+
+@pytest.mark.parametrize("nrhs", [2, 8])
+def test_refinement_graph_replays_match_eager(mid, oracle, nrhs):
+    """The refinement sweeps replayed from one captured hipGraph (several
+    replays, with the host's backward-error readback between them) give the
+    eager launches' solution bit for bit, with the same sweep count."""
+    s = g.sphere(size_param=2.0, nd=10)
+    oc = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    Z = oracle.mom_fill(oc)
+    th = np.linspace(0.0, 70.0, nrhs // 2)
+    B = oracle.mom_rhs(oc, th, np.zeros(nrhs // 2))
+    _case(f"refinement graph vs eager f16 n={3 * s.n} nrhs={nrhs}")
+    old = api.tune_set("refine_graph", 1)
+    try:
+        Xg, ig = mid.solve(Z, B, api.SOLVE_F16_IR)
+        api.tune_set("refine_graph", 0)
+        Xe, ie = mid.solve(Z, B, api.SOLVE_F16_IR)
+    finally:
+        api.tune_set("refine_graph", old)
+    print("graph", ig, "eager", ie)
+    assert ig["iterations"] == ie["iterations"] >= 2
+    np.testing.assert_array_equal(Xg, Xe)
+    assert _rel(Xg, np.linalg.solve(Z, B)) < 1e-10
+
+
+@pytest.mark.parametrize("name,value", [("panel_leaf", 32), ("panel_leaf", 128), ("trail_group_kib", 512),
+                                        ("lu_nb_split", 768), ("lu_nb_fp64", 256)])
+def test_tuned_parameters_keep_results(mid, name, value):
+    """Every schedule parameter the autotuner varies changes the time, not the
+    solution: FP64 for lu_nb_fp64, the FP16 split mode (with two-panel steps
+    forced on) for the others."""
+    rng = np.random.default_rng(40)
+    n = 2600
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 2.5 * np.sqrt(2 * n) * np.eye(n)
+    B = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    mode = api.SOLVE_FP64 if name == "lu_nb_fp64" else api.SOLVE_F16_IR
+    _case(f"tuning {name} = {value}, {api.MODE_NAMES[mode]} n={n}")
+    olds = {name: api.tune_set(name, value), "lu_two_panel_min_n": api.tune_set("lu_two_panel_min_n", 0)}
+    try:
+        X, info = mid.solve(A, B, mode)
+    finally:
+        for k, v in olds.items():
+            api.tune_set(k, v)
+    print(info)
+    assert info["status"] == api.STATUS_CONVERGED and info["info"] == 0
+    assert _rel(X, np.linalg.solve(A, B)) < 1e-10
