@@ -36,8 +36,8 @@ MODE_TERMS = {"fp64": 1, "c32": 1, "f16": 1, "f16x3": 3, "bf16": 1, "bf16x3": 3}
 # HBM bytes per k_trail16x launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors
 # a 1 GiB device copy reads on the same counters), keyed by (cells, mode, nb);
-# other configurations report null.
-TRAFFIC_PER_LAUNCH = {}
+# other configurations report null.  Source: profiles/lu_trail16_f16_grs20k_hbm_traffic.md
+TRAFFIC_PER_LAUNCH = {(20000, "f16", 0): 7.548e9}
 
 
 def log(*a):
@@ -65,6 +65,40 @@ def fields_to_host(Ed):
     t0 = time.perf_counter()
     E = Ed.cpu().numpy().T
     return E, time.perf_counter() - t0
+
+
+def copy_probe():
+    """A 1 GiB device-to-device copy (its counters calibrate rocprofv3's
+    FETCH_SIZE / WRITE_SIZE in bytes)."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("[bench] copy probe: 1 GiB read + 1 GiB written")
+    del a, b
+
+
+def kernel_profile(m, kind):
+    """(launches, total ms, algorithmic flops, algorithmic bytes) of one kind of
+    kernel recorded by the library's live HIP events (midas_profile)."""
+    v = [C.c_int(0), C.c_double(0), C.c_double(0), C.c_double(0)]
+    m.lib.midas_profile_query(m.ctx, C.c_int(kind), *[C.byref(x) for x in v])
+    return v[0].value, v[1].value, v[2].value, v[3].value
+
+
+def aca_roofline(m):
+    """k_aca against HBM: algorithmic bytes per rank step ~16 (m + n) (s + 2)
+    (complex FP64; the s - 1 previous columns of U and rows of V read, the new
+    pair written), summed over the pairs and steps of each launch, over the
+    launch's duration on the library stream."""
+    n, ms, _, by = kernel_profile(m, 3)
+    if not n or ms <= 0:
+        return None
+    gbs = by / (ms * 1e-3) / 1e9
+    return {"kernel": "k_aca (batched ACA of block pairs)", "bound": "hbm", "achieved": round(gbs, 1),
+            "peak": HBM_GBS, "unit": "GB/s", "frac": round(gbs / HBM_GBS, 4), "launches": n,
+            "avg_launch_ms": round(ms / n, 3), "alg_bytes_per_launch": by / n}
 
 
 def make_workload(cells):
@@ -206,6 +240,8 @@ def run_c5(args):
     grid (94 requested: 100 directions), 200 incidence directions (400 RHS),
     reduced system in mixed precision + FP64 refinement."""
     torch_first()
+    if args.copy_probe:
+        copy_probe()
     import numpy as np
     from midas_amd import api, geometry as g
     s = g.porous_particle()
@@ -220,11 +256,14 @@ def run_c5(args):
     for _ in range(args.warmup):
         run(mode, Ed.data_ptr())
     ts = []
+    m.lib.midas_profile(m.ctx, C.c_int(1))
     for k in range(args.steps):
         t0 = time.perf_counter()
         _, K, info = run(mode, Ed.data_ptr())
         ts.append(time.perf_counter() - t0)
         log(f"[bench] c5 step {k + 1}: {ts[-1]:.3f} s, K = {int(np.sum(K))}, {info}")
+    aca = aca_roofline(m)
+    m.lib.midas_profile(m.ctx, C.c_int(0))
     E, t_d2h = fields_to_host(Ed)
     del Ed
     step_s = float(np.mean(ts))
@@ -238,7 +277,7 @@ def run_c5(args):
                       "parallelism": "1 GPU"},
            "K_total": int(np.sum(K)), "reduced_iterations": info["iterations"],
            "reduced_factor_ms": round(info["t_factor_ms"], 1), "reduced_refine_ms": round(info["t_refine_ms"], 1),
-           "fields_in": "HBM", "fields_d2h_s": round(t_d2h, 4)}
+           "fields_in": "HBM", "fields_d2h_s": round(t_d2h, 4), "aca_roofline": aca}
     if not args.no_fp64:
         E64, _, _ = run(api.SOLVE_FP64)
         res["rel_field_err_vs_fp64_reduced"] = float(np.linalg.norm(E - E64) / np.linalg.norm(E64))
@@ -453,12 +492,7 @@ def main():
     log(f"[bench] {s.name}: n = {n3} unknowns, Z = {n3 * n3 * 16 / 1e9:.1f} GB")
     dev = torch.device("cuda:0")
     if args.copy_probe:
-        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log("[bench] copy probe: 1 GiB read + 1 GiB written")
-        del a, b
+        copy_probe()
     t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
     dx, dy, dz, dsc = t(s.x), t(s.y), t(s.z), t(s.sc)
     deps = t(np.ascontiguousarray(s.eps).view(np.float64))

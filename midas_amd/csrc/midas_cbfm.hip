@@ -766,10 +766,26 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                 TRY(Ub.reserve(sizeof(double2) * std::max(uo, 1L)));
                 TRY(Vb.reserve(sizeof(double2) * std::max(vo, 1L)));
                 TRY(dk.reserve(sizeof(int) * ajobs.size()));
-                TRY(launch_aca_cells(dc.v, ref32, k0, host_pi(quirks), dj.as<AcaJob>(), (int)ajobs.size(),
-                                     Ub.as<double2>(), Vb.as<double2>(), kmax, eps_aca, dk.as<int>(), st));
+                const size_t prof_at = profile().recs.size();
+                {
+                    ProfScope ps(st, PROF_ACA, 0.0, 0.0);
+                    TRY(launch_aca_cells(dc.v, ref32, k0, host_pi(quirks), dj.as<AcaJob>(), (int)ajobs.size(),
+                                         Ub.as<double2>(), Vb.as<double2>(), kmax, eps_aca, dk.as<int>(), st));
+                }
                 std::vector<int> hk(ajobs.size());
                 TRY(copy_sync(hk.data(), dk.p, sizeof(int) * hk.size(), hipMemcpyDeviceToHost, st));
+                if (profile().recs.size() > prof_at) {
+                    // algorithmic bytes of the launch, known once the ranks are:
+                    // rank step s of an m x n pair reads the s - 1 previous
+                    // columns of U and rows of V and writes the new pair,
+                    // ~16 (m + n) (s + 2) bytes (complex FP64)
+                    double by = 0.0;
+                    for (size_t q = 0; q < ajobs.size(); ++q) {
+                        const double steps = std::min(hk[q], kmax);
+                        by += 16.0 * (ajobs[q].m + ajobs[q].n) * (steps * (steps + 1) / 2 + 2 * steps);
+                    }
+                    profile().recs[prof_at].bytes = by;
+                }
                 // L_ij = C_i^T U (K_i x k) and R_ij = V C_j (k x K_j) in one scratch
                 std::vector<long> lo(ajobs.size()), ro(ajobs.size());
                 long lr = 0;

@@ -80,7 +80,7 @@ struct KernelProfile {
 };
 KernelProfile &profile();
 // kinds
-enum { PROF_TRAIL = 0, PROF_FILL = 1, PROF_PANEL = 2 };
+enum { PROF_TRAIL = 0, PROF_FILL = 1, PROF_PANEL = 2, PROF_ACA = 3 };
 struct ProfScope {
     hipEvent_t a{}, b{};
     hipStream_t st;
