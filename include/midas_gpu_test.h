@@ -12,7 +12,11 @@ extern "C" {
 #endif
 
 /* C = alpha op(A) op(B) + beta C, complex, column-major; prec 64 (ZGEMM on
- * FP64 MFMA) or 32 (CGEMM on FP32 MFMA, operands given as complex128 and
+ * FP64 MFMA, 64 x 64 tiles; without transposes the kernel of 65), 65 (ZGEMM,
+ * the kernel of the FP64 LU and the residuals, its tile shape the zgemm_tile
+ * parameter; no transposes), 66 (that kernel as a strided batch of
+ * three: A, B and C each hold three matrices side by side, m x k, k x n and
+ * m x n) or 32 (CGEMM on FP32 MFMA, operands given as complex128 and
  * rounded); transA/transB: 0 = N, 1 = T. */
 int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double *alpha, const double *A, long lda,
                     int transA, const double *B, long ldb, int transB, const double *beta, double *C, long ldc);

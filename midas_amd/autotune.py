@@ -47,6 +47,8 @@ COMMENTS = {
     "panel_leaf": "columns of a recursive panel's leaves",
     "trail_group_kib": "L2 share (KiB) of a row-tile group of the split GEMM",
     "refine_graph": "refinement sweeps replayed from one captured hipGraph",
+    "zgemm_tile": "tile shape of the FP64 ZGEMM (0: 128x128 ... 4: 64x32)",
+    "fp64_gemm_library": "1: rocBLAS for the FP64 LU updates and many-RHS residuals",
 }
 
 

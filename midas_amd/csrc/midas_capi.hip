@@ -206,8 +206,9 @@ static int residual_launch(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, const d
         r = launch_residual_few(Z, ldz, n, X, ldx, nrhs, B, ldb, R, n, ws.zx.p, st);
     } else {
         hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R, (long)n, n, nrhs);
-        // many right-hand sides: rocBLAS (33k x 448 x 33k: 60 ms against 74 ms for k_cgemm<double>)
-        r = nrhs >= 32 && !ws.lent ? blas_zgemm_nn(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0},
+        // many right-hand sides: k_zgemm_nn (through launch_cgemm), or rocBLAS
+        // when fp64_gemm_library = 1
+        r = nrhs >= 32 && !ws.lent && tune(TUNE_FP64_GEMM_LIBRARY) ? blas_zgemm_nn(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0, z64{1.0, 0.0},
                                                    R, n, 0, 1, st)
                                    : launch_cgemm<double>(n, nrhs, n, z64{-1.0, 0.0}, Z, ldz, 0, X, ldx, 0,
                                                           z64{1.0, 0.0}, R, n, st);
@@ -1151,6 +1152,7 @@ __global__ void k_z2c1(const double2 *S, float2 *D, long n)
 static int upload_mat(DBuf &b, const double *h, int rows, int cols, long ld, hipStream_t st)
 {
     TRY(b.alloc(sizeof(double2) * (size_t)rows * cols));
+    if ((size_t)rows * cols == 0) return 0;
     MIDAS_HIP(hipMemcpy2DAsync(b.p, sizeof(double2) * rows, h, sizeof(double2) * ld, sizeof(double2) * rows, cols,
                                hipMemcpyHostToDevice, st));
     return 0;
@@ -1166,15 +1168,21 @@ static int to_c64(DBuf &d32, const DBuf &d64, long cnt, hipStream_t st)
 int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double *alpha, const double *A, long lda,
                     int transA, const double *B, long ldb, int transB, const double *beta, double *C, long ldc)
 {
-    MIDAS_CHECK(ctx && A && B && C && alpha && beta && (prec == 64 || prec == 32), -1, "test_gemm: bad arguments");
+    MIDAS_CHECK(ctx && A && B && C && alpha && beta && (prec == 64 || prec == 32 || prec == 65 || prec == 66), -1,
+                "test_gemm: bad arguments");
+    MIDAS_CHECK(prec < 65 || (!transA && !transB), -1, "test_gemm: prec 65 and 66 take no transposes");
     MIDAS_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->st;
+    const int nbat = prec == 66 ? 3 : 1;
     int ar = transA ? k : m, ac = transA ? m : k, br = transB ? n : k, bc = transB ? k : n;
     DBuf dA, dB, dC;
-    TRY(upload_mat(dA, A, ar, ac, lda, st));
-    TRY(upload_mat(dB, B, br, bc, ldb, st));
-    TRY(upload_mat(dC, C, m, n, ldc, st));
-    if (prec == 64) {
+    TRY(upload_mat(dA, A, ar, ac * nbat, lda, st));
+    TRY(upload_mat(dB, B, br, bc * nbat, ldb, st));
+    TRY(upload_mat(dC, C, m, n * nbat, ldc, st));
+    if (prec >= 65) {
+        TRY(launch_zgemm_nn(m, n, k, z64{alpha[0], alpha[1]}, dA.p, ar, (long)ar * ac, dB.p, br, (long)br * bc,
+                            z64{beta[0], beta[1]}, dC.p, m, (long)m * n, nbat, st, tune(TUNE_ZGEMM_TILE)));
+    } else if (prec == 64) {
         TRY(launch_cgemm<double>(m, n, k, z64{alpha[0], alpha[1]}, dA.p, ar, transA, dB.p, br, transB,
                                  z64{beta[0], beta[1]}, dC.p, m, st));
     } else {
@@ -1187,7 +1195,7 @@ int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double 
         hipLaunchKernelGGL(k_c2z, dim3(((long)m * n + 255) / 256), dim3(256), 0, st, c32.as<float2>(),
                            dC.as<double2>(), (long)m * n);
     }
-    MIDAS_HIP(hipMemcpy2DAsync(C, sizeof(double2) * ldc, dC.p, sizeof(double2) * m, sizeof(double2) * m, n,
+    MIDAS_HIP(hipMemcpy2DAsync(C, sizeof(double2) * ldc, dC.p, sizeof(double2) * m, sizeof(double2) * m, n * nbat,
                                hipMemcpyDeviceToHost, st));
     MIDAS_HIP(hipStreamSynchronize(st));
     return 0;

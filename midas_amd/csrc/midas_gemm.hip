@@ -6,7 +6,9 @@
 // Interleaved complex operands, column-major.  The product is formed from four
 // real MFMA products per complex multiply-add (4M): Cr += Ar Br - Ai Bi,
 // Ci += Ar Bi + Ai Br, with planar (re / im) tiles staged through LDS.
-//   double: v_mfma_f64_16x16x4_f64   (ZGEMM; FP64 baseline LU, residuals, CBFM)
+//   double: v_mfma_f64_16x16x4_f64   (ZGEMM with a transposed operand: the CBFM
+//           Gram matrices and projections; products without transposes go to
+//           k_zgemm_nn, midas_zgemm.hip)
 //   float : v_mfma_f32_16x16x4_f32   (CGEMM; complex-FP32 LU trailing update)
 // A 256-thread workgroup computes a 64x64 tile; each wave owns a 32x32
 // quadrant as 2x2 MFMA 16x16 tiles.  K advances 16 complex values per stage;
@@ -14,6 +16,7 @@
 // A batch of equal-shape products runs on blockIdx.z with element strides.
 #include "midas_common.h"
 #include "midas_lu.h"
+#include "midas_tune.h"
 
 namespace midas {
 
@@ -170,6 +173,11 @@ int launch_cgemm(int m, int n, int k, cplx<T> alpha, const void *A, long lda, in
 {
     typedef typename cvec<T>::t V;
     if (m <= 0 || n <= 0 || batch <= 0) return 0;
+    // complex FP64 without transposes: the faster k_zgemm_nn
+    if constexpr (sizeof(T) == 8)
+        if (tA == 0 && tB == 0)
+            return launch_zgemm_nn(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, st,
+                                   tune(TUNE_ZGEMM_TILE));
     MIDAS_CHECK((n + GB_N - 1) / GB_N <= 65535, -1, "cgemm: n too large for one launch");
     if (k <= 0) {
         // C = beta C only: run with k=0 (loop body skipped)

@@ -681,18 +681,17 @@ int blas_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, long 
     return 0;
 }
 
-// FP64 trailing update C -= A B: a plain (strided-batched) ZGEMM, handed to
-// rocBLAS, which sustains 74 TF/s on the batched k = 128 shapes of the block
-// LUs against 52 TF/s for k_cgemm<double>.
+// FP64 trailing update C -= A B: a plain (strided-batched) ZGEMM on the
+// hand-written k_zgemm_nn, or on rocBLAS when fp64_gemm_library = 1.
 template <>
 int TrailingGemm<double>::operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C,
                                      long ldc, hipStream_t st) const
 {
     if (m <= 0 || n <= 0 || k <= 0 || batch <= 0) return 0;
     ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k * batch, 16.0 * (2.0 * m * n + (double)m * k + (double)k * n) * batch);
-    if (!library)
-        return launch_cgemm<double>(m, n, k, z64{-1.0, 0.0}, A, lda, 0, B, ldb, 0, z64{1.0, 0.0}, C, ldc, st, batch,
-                                    stride, stride, stride);
+    if (!library || !tune(TUNE_FP64_GEMM_LIBRARY))
+        return launch_zgemm_nn(m, n, k, z64{-1.0, 0.0}, A, lda, stride, B, ldb, stride, z64{1.0, 0.0}, C, ldc, stride,
+                               batch, st, tune(TUNE_ZGEMM_TILE));
     return blas_zgemm_nn(m, n, k, z64{-1.0, 0.0}, A, lda, stride, B, ldb, stride, z64{1.0, 0.0}, C, ldc, stride, batch,
                          st);
 }

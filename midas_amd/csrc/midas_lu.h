@@ -52,7 +52,7 @@ template <typename T> struct TrailingGemm {
     void *scratch = nullptr;        // split operand planes + column scales (trail16_scratch_bytes)
     int batch = 1;                  // batched factorisations: matrices at stride `stride`
     long stride = 0;
-    bool library = true;            // FP64: rocBLAS ZGEMM (false: the hand-written k_cgemm<double>)
+    bool library = true;            // FP64: rocBLAS ZGEMM allowed (where fp64_gemm_library asks for it)
     int operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C, long ldc,
                    hipStream_t st) const;
 };
@@ -83,9 +83,16 @@ void ipiv_to_perm(const int *ipiv, int n, int *perm);
 int ipiv_to_perm_dev(const int *ipiv, int *perm, int n, int batch, hipStream_t st);
 
 // C = alpha A B + beta C through rocBLAS ZGEMM (strided batch when batch > 1):
-// the plain FP64 products where the library is faster than k_cgemm<double>.
+// the FP64 products when the fp64_gemm_library parameter asks for the vendor
+// library (a comparison point; the default is k_zgemm_nn).
 int blas_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, long sA, const void *B, long ldb, long sB,
                   z64 beta, void *C, long ldc, long sC, int batch, hipStream_t st);
+
+// C = alpha A B + beta C, complex FP64 on FP64 MFMA (midas_zgemm.hip), strided
+// batch when batch > 1; shape picks the tile (0: 128 x 128, 1: 128 x 64 of
+// 64 x 32 waves, 2: 64 x 64, 3: 128 x 64 of 32 x 32 waves, 4: 64 x 32).
+int launch_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, long sA, const void *B, long ldb, long sB,
+                    z64 beta, void *C, long ldc, long sC, int batch, hipStream_t st, int shape);
 
 // C = alpha op(A) op(B) + beta C; op: 0 none, 1 transpose, 2 conjugate transpose.
 template <typename T>

@@ -11,6 +11,8 @@ enum {
     TUNE_PANEL_LEAF,
     TUNE_TRAIL_GROUP_KIB,
     TUNE_REFINE_GRAPH,
+    TUNE_ZGEMM_TILE,
+    TUNE_FP64_GEMM_LIBRARY,
     TUNE_COUNT
 };
 int tune(int which);

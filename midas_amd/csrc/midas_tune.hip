@@ -21,6 +21,8 @@ static TuneEntry g_tune[TUNE_COUNT] = {
     {"panel_leaf", MIDAS_TUNED_PANEL_LEAF, 16, 512},
     {"trail_group_kib", MIDAS_TUNED_TRAIL_GROUP_KIB, 64, 65536},
     {"refine_graph", MIDAS_TUNED_REFINE_GRAPH, 0, 1},
+    {"zgemm_tile", MIDAS_TUNED_ZGEMM_TILE, 0, 4},
+    {"fp64_gemm_library", MIDAS_TUNED_FP64_GEMM_LIBRARY, 0, 1},
 };
 
 int tune(int which) { return g_tune[which].value; }

@@ -10,3 +10,5 @@
 #define MIDAS_TUNED_PANEL_LEAF             128      // columns of a recursive panel's leaves
 #define MIDAS_TUNED_TRAIL_GROUP_KIB        2048     // L2 share (KiB) of a row-tile group of the split GEMM
 #define MIDAS_TUNED_REFINE_GRAPH           0        // refinement sweeps replayed from one captured hipGraph
+#define MIDAS_TUNED_ZGEMM_TILE             2        // tile shape of the FP64 ZGEMM (0: 128x128 ... 4: 64x32)
+#define MIDAS_TUNED_FP64_GEMM_LIBRARY      0        // 1: rocBLAS for the FP64 LU updates and many-RHS residuals

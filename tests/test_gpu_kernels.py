@@ -47,9 +47,10 @@ def test_mfma_lane_layout(mid, which):
     assert np.array_equal(D, E)
 
 
-def _gemm(mid, prec, A, B, C0, alpha, beta, tA, tB):
-    m, n = C0.shape
-    k = A.shape[0] if tA else A.shape[1]
+def _gemm(mid, prec, A, B, C0, alpha, beta, tA, tB, k=None, n=None):
+    m = C0.shape[0]
+    n = n or C0.shape[1]
+    k = k if k is not None else (A.shape[0] if tA else A.shape[1])
     Cm = np.asfortranarray(C0.copy())
     a = np.array([alpha.real, alpha.imag])
     b = np.array([beta.real, beta.imag])
@@ -80,6 +81,52 @@ def test_gemm(mid, prec, shape, tA, tB):
     err = np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * k + np.abs(C0).max())
     print("max scaled err", err)
     assert err < (1e-15 if prec == 64 else 3e-7)
+
+
+@pytest.fixture(params=[0, 1, 2, 3, 4])
+def zgemm_tile(request):
+    api.tune_set("zgemm_tile", request.param)
+    yield request.param
+    api.tune_set("zgemm_tile", 2)
+
+
+@pytest.mark.parametrize("shape", [(128, 128, 8), (64, 64, 16), (100, 37, 53), (1, 5, 200), (130, 2, 300),
+                                   (257, 300, 129), (384, 256, 1024), (129, 131, 7), (5, 3, 0)])
+def test_zgemm_nn(mid, zgemm_tile, shape):
+    """The FP64 ZGEMM of the LU trailing updates and many-RHS residuals, every
+    tile shape: edges of every kind (partial tiles in m, n and k, k below one
+    stage, k = 0) against numpy, to FP64 rounding."""
+    m, n, k = shape
+    print(f"\n[case] zgemm_nn tile {zgemm_tile} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(m * 1000 + n + k)
+    cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    A, B, C0 = cr(m, max(k, 1)), cr(max(k, 1), n), cr(m, n)
+    alpha, beta = 0.7 - 0.2j, -1.1 + 0.4j
+    got = _gemm(mid, 65, A, B, C0, alpha, beta, 0, 0, k=k)
+    ref = alpha * (A[:, :k] @ B[:k, :]) + beta * C0
+    err = np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * max(k, 1) + np.abs(C0).max())
+    print("max scaled err", err)
+    assert err < 1e-15
+    # the trailing update's form: C - A B
+    got = _gemm(mid, 65, A, B, C0, -1.0 + 0j, 1.0 + 0j, 0, 0, k=k)
+    assert np.abs(got - (C0 - A[:, :k] @ B[:k, :])).max() < 1e-15 * (np.abs(A).max() * np.abs(B).max() * max(k, 1)
+                                                                     + np.abs(C0).max())
+
+
+@pytest.mark.parametrize("shape", [(200, 150, 40), (128, 256, 300)])
+def test_zgemm_nn_batched(mid, zgemm_tile, shape):
+    """A strided batch of three products (the CBFM block LUs' trailing updates)."""
+    m, n, k = shape
+    print(f"\n[case] zgemm_nn tile {zgemm_tile} batch 3 m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(7 + m + n + k)
+    cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    A, B, C0 = cr(m, 3 * k), cr(k, 3 * n), cr(m, 3 * n)
+    alpha, beta = -1.0 + 0j, 1.0 + 0j
+    got = _gemm(mid, 66, A, B, C0, alpha, beta, 0, 0, k=k, n=n)
+    for q in range(3):
+        ref = C0[:, q * n:(q + 1) * n] - A[:, q * k:(q + 1) * k] @ B[:, q * n:(q + 1) * n]
+        err = np.abs(got[:, q * n:(q + 1) * n] - ref).max() / (np.abs(A).max() * np.abs(B).max() * k)
+        assert err < 1e-15, (q, err)
 
 
 @pytest.mark.parametrize("mode,tol", [(1, 4e-3), (3, 1e-5), (11, 3e-2), (13, 2e-4)])
