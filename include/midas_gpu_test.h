@@ -21,6 +21,12 @@ extern "C" {
 int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double *alpha, const double *A, long lda,
                     int transA, const double *B, long ldb, int transB, const double *beta, double *C, long ldc);
 
+/* Eigen-decomposition of a batch of Hermitian n x n complex matrices G
+ * (batch matrices one after another, column-major) by the CBF subspaces'
+ * batched eigensolver: D (n per matrix) ascending, V (n x n per matrix) the
+ * eigenvectors, column c for D[c]. */
+int midas_test_heev(midas_ctx *ctx, int n, int batch, const double *G, double *D, double *V);
+
 /* C -= A B with the split FP16/BF16 trailing-update kernel (mode as
  * TrailingMode: 1 f16, 3 f16x3, 11 bf16, 13 bf16x3); operands complex128,
  * rounded to complex FP32 before the call. */

@@ -1201,6 +1201,28 @@ int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double 
     return 0;
 }
 
+int midas_test_heev(midas_ctx *ctx, int n, int batch, const double *G, double *D, double *V)
+{
+    MIDAS_CHECK(ctx && G && D && V && n > 0 && batch > 0, -1, "test_heev: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    const size_t nn = (size_t)n * n;
+    DBuf dG, dD, dI, W;
+    TRY(dG.alloc(sizeof(double2) * nn * batch));
+    TRY(dD.alloc(sizeof(double) * (size_t)n * batch));
+    TRY(dI.alloc(sizeof(int) * batch));
+    TRY(W.alloc(heev_work_bytes(n, batch)));
+    MIDAS_HIP(hipMemcpyAsync(dG.p, G, sizeof(double2) * nn * batch, hipMemcpyHostToDevice, st));
+    TRY(launch_heev_batched(n, dG.as<double2>(), n, (long)nn, dD.as<double>(), n, dI.as<int>(), batch, W.p, st));
+    std::vector<int> info(batch);
+    MIDAS_HIP(hipMemcpyAsync(info.data(), dI.p, sizeof(int) * batch, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipMemcpyAsync(D, dD.p, sizeof(double) * (size_t)n * batch, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipMemcpyAsync(V, dG.p, sizeof(double2) * nn * batch, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    for (int b = 0; b < batch; ++b) MIDAS_CHECK(info[b] == 0, -2, "test_heev: matrix %d did not converge", b);
+    return 0;
+}
+
 int midas_test_trailing(midas_ctx *ctx, int mode, int m, int n, int k, const double *A, long lda, const double *B,
                         long ldb, double *C, long ldc)
 {

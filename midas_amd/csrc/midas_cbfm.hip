@@ -13,8 +13,8 @@
 //      library.  The reference takes the SVD X_b = U S W^H (GESVD, JOB='U');
 //      only sigma and the span of the leading left vectors enter the result,
 //      so here they come from the Gram matrix X^H X (or X X^H when X is wide):
-//      one GEMM per block and ONE batched Hermitian eigensolver call
-//      (rocSOLVER heevd, strided batched) for all blocks, U = X W S^-1.  For
+//      one GEMM per block and ONE launch of this library's batched Hermitian
+//      eigensolver (midas_heev.hip) for all blocks, U = X W S^-1.  For
 //      thresholds below 1e-5 the squared spectrum is not resolved well enough
 //      and each block takes rocSOLVER gesvd instead.
 //      K_b = #{sigma_d / sigma_1 >= threshold}; C_b = U(1:3 n_b, 1:K_b).
@@ -538,19 +538,17 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
     phase(sr ? "SR masks + block fills + LU solves" : "block fills + LU solves");
     // 1b. left singular vectors of X_b with sigma_d / sigma_1 >= threshold.
     std::vector<int> K(nb);
-    RocHandle rh;
-    MIDAS_ROC(rocblas_create_handle(&rh.h));
-    MIDAS_ROC(rocblas_set_stream(rh.h, st));
     if (thr >= CBF_GRAM_MIN_THRESHOLD) {
         // Gram route: G_b = X^H X (T2c <= m_b, then U = X W S^-1) or X X^H
         // (T2c > m_b, then U = W); every block's G zero-padded to rpad and
-        // decomposed by one batched Hermitian eigensolver call.
+        // decomposed by one launch of the batched Hermitian eigensolver
+        // (midas_heev.hip).
         const long gs = (long)rpad * rpad;
-        DBuf G, D, Eo, dinfo, Wk;
+        DBuf G, D, dinfo, Wk, HW;
         TRY(G.alloc(sizeof(double2) * gs * nb));
         TRY(D.alloc(sizeof(double) * (size_t)rpad * nb));
-        TRY(Eo.alloc(sizeof(double) * (size_t)rpad * nb));
         TRY(dinfo.alloc(sizeof(int) * nb));
+        TRY(HW.alloc(heev_work_bytes(rpad, nb)));
         TRY(Wk.alloc(sizeof(double2) * (size_t)T2c * T2c));
         MIDAS_HIP(hipMemsetAsync(G.p, 0, G.bytes, st));
         bool all_tall = true;
@@ -570,9 +568,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                     TRY(launch_cgemm<double>(m, m, T2c, ONE, Xb(b), mp, 0, Xb(b), mp, 2, ZERO, Gb, rpad, st));
             }
         }
-        MIDAS_ROC(rocsolver_zheevd_strided_batched(rh.h, rocblas_evect_original, rocblas_fill_upper, rpad,
-                                                   (rocblas_double_complex *)G.p, rpad, gs, D.as<double>(), rpad,
-                                                   Eo.as<double>(), rpad, dinfo.as<int>(), nb));
+        TRY(launch_heev_batched(rpad, G.as<double2>(), rpad, gs, D.as<double>(), rpad, dinfo.as<int>(), nb, HW.p, st));
         std::vector<double> hd((size_t)rpad * nb);
         std::vector<int> hi(nb);
         MIDAS_HIP(hipMemcpyAsync(hi.data(), dinfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
@@ -600,6 +596,10 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         }
     } else {
         // thresholds below what the Gram route resolves: one-by-one SVD
+        // (rocSOLVER gesvd; not on the default path)
+        RocHandle rh;
+        MIDAS_ROC(rocblas_create_handle(&rh.h));
+        MIDAS_ROC(rocblas_set_stream(rh.h, st));
         DBuf Usv, Sv, Esup, dinfo;
         const long rmax = std::min<long>(mmax, T2c);
         TRY(Usv.alloc(sizeof(double2) * mmax * rmax));

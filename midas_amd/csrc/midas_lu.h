@@ -94,6 +94,14 @@ int blas_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, long 
 int launch_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, long sA, const void *B, long ldb, long sB,
                     z64 beta, void *C, long ldc, long sC, int batch, hipStream_t st, int shape);
 
+// Eigen-decomposition of a strided batch of Hermitian n x n matrices G
+// (midas_heev.hip): eigenvalues ascending into D (stride dstride), the
+// eigenvectors over G (column c for D[c]); info[b] = 1 where the QL iteration
+// did not converge.  work: heev_work_bytes(n, batch) of device memory.
+size_t heev_work_bytes(int n, int batch);
+int launch_heev_batched(int n, double2 *G, long ld, long stride, double *D, long dstride, int *info, int batch,
+                        void *work, hipStream_t st);
+
 // C = alpha op(A) op(B) + beta C; op: 0 none, 1 transpose, 2 conjugate transpose.
 template <typename T>
 int launch_cgemm(int m, int n, int k, cplx<T> alpha, const void *A, long lda, int tA, const void *B, long ldb,

@@ -180,3 +180,68 @@ def test_trailing_split_kernels(mid, mode, tol, shape, one_wg, monkeypatch):
     print("scaled err", err)
     assert err < tol
 
+
+
+def _heev(mid, Gs):
+    b, n, _ = Gs.shape
+    G = np.ascontiguousarray(np.stack([np.asfortranarray(g) for g in Gs]).transpose(0, 2, 1))  # column-major each
+    D = np.zeros(b * n)
+    V = np.zeros(b * n * n, dtype=complex)
+    r = mid.lib.midas_test_heev(mid.ctx, C.c_int(n), C.c_int(b), _p(G), _p(D), _p(V))
+    assert r == 0, mid.lib.midas_last_error()
+    return D.reshape(b, n), V.reshape(b, n, n).transpose(0, 2, 1)
+
+
+def _heev_cases(n, rng):
+    cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    A = cr(n, n)
+    out = {"random": A + A.conj().T, "real": (A + A.T).real.astype(complex),
+           "diagonal": np.diag(rng.normal(size=n)).astype(complex)}
+    # the CBF Gram matrices: X^H X of a rank-deficient X, zero-padded
+    r = max(1, n // 3)
+    X = cr(2 * n, r) @ cr(r, n - n // 5) * np.logspace(0, -8, n - n // 5)
+    Gm = np.zeros((n, n), dtype=complex)
+    Gm[:n - n // 5, :n - n // 5] = X.conj().T @ X
+    out["gram"] = Gm
+    # rank-deficient without padding: n - r eigenvalues at the rounding level
+    # of the large ones (the tail that a deflation test relative to the
+    # neighbouring diagonal entries never converges on)
+    Xr = cr(2 * n, r) @ cr(r, n)
+    out["rankdef"] = Xr.conj().T @ Xr
+    # clusters: repeated eigenvalues under a random unitary
+    Q, _ = np.linalg.qr(cr(n, n))
+    lam = np.repeat(np.arange(1, n // 4 + 2, dtype=float), 4)[:n]
+    out["clusters"] = (Q * lam) @ Q.conj().T
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 182, 300])
+def test_heev_batched(mid, n):
+    """The batched Hermitian eigensolver of the CBF subspaces against LAPACK
+    (numpy eigh): eigenvalues to n eps |G|, residual G V - V diag(D) and
+    orthonormality of V to rounding, and the invariant subspace of a gap-
+    separated leading group (what the CBF basis keeps) equal to LAPACK's."""
+    rng = np.random.default_rng(100 + n)
+    cases = _heev_cases(n, rng)
+    names = list(cases)
+    Gs = np.stack([0.5 * (cases[k] + cases[k].conj().T) for k in names])
+    print(f"\n[case] heev n={n} batch {len(names)}: {names}", flush=True)
+    D, V = _heev(mid, Gs)
+    for q, name in enumerate(names):
+        G = Gs[q]
+        nrm = max(np.abs(G).max(), 1e-300)
+        lam = np.linalg.eigvalsh(G)
+        assert np.all(np.diff(D[q]) >= 0), name
+        eerr = np.abs(D[q] - lam).max() / nrm
+        res = np.abs(G @ V[q] - V[q] * D[q]).max() / nrm
+        orth = np.abs(V[q].conj().T @ V[q] - np.eye(n)).max()
+        print(f"  {name:9s} eig {eerr:.1e} res {res:.1e} orth {orth:.1e}")
+        assert eerr < 1e-14 * max(n, 8), name
+        assert res < 1e-14 * max(n, 8), name
+        assert orth < 1e-14 * max(n, 8), name
+        if name in ("gram", "rankdef") and n >= 7:
+            # leading eigenvectors above a relative gap: same span as LAPACK's
+            w, U = np.linalg.eigh(G)
+            k = int(np.sum(w > 1e-6 * w[-1]))
+            P, Pr = V[q][:, -k:] @ V[q][:, -k:].conj().T, U[:, -k:] @ U[:, -k:].conj().T
+            assert np.abs(P - Pr).max() < 1e-10, name
