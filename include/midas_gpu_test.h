@@ -24,8 +24,10 @@ int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double 
 /* Eigen-decomposition of a batch of Hermitian n x n complex matrices G
  * (batch matrices one after another, column-major) by the CBF subspaces'
  * batched eigensolver: D (n per matrix) ascending, V (n x n per matrix) the
- * eigenvectors, column c for D[c]. */
-int midas_test_heev(midas_ctx *ctx, int n, int batch, const double *G, double *D, double *V);
+ * eigenvectors, column c for D[c].  path: 0 by order, 1 the one-stage kernel
+ * (all pairs), 2 the large-order kernel, which with cut > 0 computes only the
+ * pairs with lambda >= cut lambda_max (the others: 0 and zero columns). */
+int midas_test_heev(midas_ctx *ctx, int n, int batch, const double *G, double *D, double *V, double cut, int path);
 
 /* C -= A B with the split FP16/BF16 trailing-update kernel (mode as
  * TrailingMode: 1 f16, 3 f16x3, 11 bf16, 13 bf16x3); operands complex128,

@@ -1201,7 +1201,7 @@ int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double 
     return 0;
 }
 
-int midas_test_heev(midas_ctx *ctx, int n, int batch, const double *G, double *D, double *V)
+int midas_test_heev(midas_ctx *ctx, int n, int batch, const double *G, double *D, double *V, double cut, int path)
 {
     MIDAS_CHECK(ctx && G && D && V && n > 0 && batch > 0, -1, "test_heev: bad arguments");
     MIDAS_HIP(hipSetDevice(ctx->device));
@@ -1213,7 +1213,8 @@ int midas_test_heev(midas_ctx *ctx, int n, int batch, const double *G, double *D
     TRY(dI.alloc(sizeof(int) * batch));
     TRY(W.alloc(heev_work_bytes(n, batch)));
     MIDAS_HIP(hipMemcpyAsync(dG.p, G, sizeof(double2) * nn * batch, hipMemcpyHostToDevice, st));
-    TRY(launch_heev_batched(n, dG.as<double2>(), n, (long)nn, dD.as<double>(), n, dI.as<int>(), batch, W.p, st));
+    TRY(launch_heev_batched(n, dG.as<double2>(), n, (long)nn, dD.as<double>(), n, dI.as<int>(), batch, W.p, st, cut,
+                            path));
     std::vector<int> info(batch);
     MIDAS_HIP(hipMemcpyAsync(info.data(), dI.p, sizeof(int) * batch, hipMemcpyDeviceToHost, st));
     MIDAS_HIP(hipMemcpyAsync(D, dD.p, sizeof(double) * (size_t)n * batch, hipMemcpyDeviceToHost, st));

@@ -568,7 +568,10 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                     TRY(launch_cgemm<double>(m, m, T2c, ONE, Xb(b), mp, 0, Xb(b), mp, 2, ZERO, Gb, rpad, st));
             }
         }
-        TRY(launch_heev_batched(rpad, G.as<double2>(), rpad, gs, D.as<double>(), rpad, dinfo.as<int>(), nb, HW.p, st));
+        // (large orders compute only the pairs at or above a quarter of the
+        // threshold's eigenvalue ratio, a margin over the count below)
+        TRY(launch_heev_batched(rpad, G.as<double2>(), rpad, gs, D.as<double>(), rpad, dinfo.as<int>(), nb, HW.p, st,
+                                0.25 * thr * thr));
         std::vector<double> hd((size_t)rpad * nb);
         std::vector<int> hi(nb);
         MIDAS_HIP(hipMemcpyAsync(hi.data(), dinfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));

@@ -98,9 +98,12 @@ int launch_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, lon
 // (midas_heev.hip): eigenvalues ascending into D (stride dstride), the
 // eigenvectors over G (column c for D[c]); info[b] = 1 where the QL iteration
 // did not converge.  work: heev_work_bytes(n, batch) of device memory.
+// path 0 picks by order (1: the one-stage kernel, all pairs; 2: the
+// large-order kernel, which computes only the eigenpairs with
+// lambda >= cut lambda_max when cut > 0 and leaves the others 0).
 size_t heev_work_bytes(int n, int batch);
 int launch_heev_batched(int n, double2 *G, long ld, long stride, double *D, long dstride, int *info, int batch,
-                        void *work, hipStream_t st);
+                        void *work, hipStream_t st, double cut = 0.0, int path = 0);
 
 // C = alpha op(A) op(B) + beta C; op: 0 none, 1 transpose, 2 conjugate transpose.
 template <typename T>

@@ -182,12 +182,12 @@ def test_trailing_split_kernels(mid, mode, tol, shape, one_wg, monkeypatch):
 
 
 
-def _heev(mid, Gs):
+def _heev(mid, Gs, cut=0.0, path=0):
     b, n, _ = Gs.shape
     G = np.ascontiguousarray(np.stack([np.asfortranarray(g) for g in Gs]).transpose(0, 2, 1))  # column-major each
     D = np.zeros(b * n)
     V = np.zeros(b * n * n, dtype=complex)
-    r = mid.lib.midas_test_heev(mid.ctx, C.c_int(n), C.c_int(b), _p(G), _p(D), _p(V))
+    r = mid.lib.midas_test_heev(mid.ctx, C.c_int(n), C.c_int(b), _p(G), _p(D), _p(V), C.c_double(cut), C.c_int(path))
     assert r == 0, mid.lib.midas_last_error()
     return D.reshape(b, n), V.reshape(b, n, n).transpose(0, 2, 1)
 
@@ -215,18 +215,20 @@ def _heev_cases(n, rng):
     return out
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 64, 182, 300])
-def test_heev_batched(mid, n):
-    """The batched Hermitian eigensolver of the CBF subspaces against LAPACK
-    (numpy eigh): eigenvalues to n eps |G|, residual G V - V diag(D) and
-    orthonormality of V to rounding, and the invariant subspace of a gap-
-    separated leading group (what the CBF basis keeps) equal to LAPACK's."""
+@pytest.mark.parametrize("n,path", [(1, 1), (2, 1), (7, 1), (64, 1), (182, 1), (300, 1),
+                                    (1, 2), (2, 2), (7, 2), (64, 2), (182, 2), (300, 2), (870, 2)])
+def test_heev_batched(mid, n, path):
+    """The batched Hermitian eigensolver of the CBF subspaces (both kernels)
+    against LAPACK (numpy eigh): eigenvalues to n eps |G|, residual
+    G V - V diag(D) and orthonormality of V to rounding, and the invariant
+    subspace of a gap-separated leading group (what the CBF basis keeps) equal
+    to LAPACK's."""
     rng = np.random.default_rng(100 + n)
     cases = _heev_cases(n, rng)
     names = list(cases)
     Gs = np.stack([0.5 * (cases[k] + cases[k].conj().T) for k in names])
-    print(f"\n[case] heev n={n} batch {len(names)}: {names}", flush=True)
-    D, V = _heev(mid, Gs)
+    print(f"\n[case] heev path {path} n={n} batch {len(names)}: {names}", flush=True)
+    D, V = _heev(mid, Gs, path=path)
     for q, name in enumerate(names):
         G = Gs[q]
         nrm = max(np.abs(G).max(), 1e-300)
@@ -245,3 +247,35 @@ def test_heev_batched(mid, n):
             k = int(np.sum(w > 1e-6 * w[-1]))
             P, Pr = V[q][:, -k:] @ V[q][:, -k:].conj().T, U[:, -k:] @ U[:, -k:].conj().T
             assert np.abs(P - Pr).max() < 1e-10, name
+
+
+@pytest.mark.parametrize("n", [300, 844])
+def test_heev_large_leading_pairs(mid, n):
+    """The large-order kernel as the CBF subspaces call it: only the pairs with
+    lambda >= cut lambda_max (cut = thr^2 / 4 for thr = 1e-3).  The count of
+    eigenvalues above thr^2 lambda_max (the CBF count K_b) and those
+    eigenvalues equal LAPACK's, the uncomputed ones are 0, and the kept
+    eigenvectors span LAPACK's leading invariant subspace."""
+    rng = np.random.default_rng(7 * n)
+    cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    Gs = []
+    for decay in (1e-3, 1e-5, 1e-7):   # CBF-like spectra: smooth decay into a rank-deficient tail
+        X = cr(2 * n, n) * np.logspace(0, np.log10(decay), n)
+        Gs.append(X.conj().T @ X)
+    Gs = np.stack(Gs)
+    thr = 1e-3
+    print(f"\n[case] heev large leading pairs n={n}", flush=True)
+    D, V = _heev(mid, Gs, cut=0.25 * thr * thr, path=2)
+    for q in range(len(Gs)):
+        w, U = np.linalg.eigh(Gs[q])
+        K = int(np.sum(np.sqrt(np.maximum(w, 0) / w[-1]) >= thr))
+        Kg = int(np.sum(np.sqrt(np.maximum(D[q], 0) / D[q][-1]) >= thr))
+        kv = int(np.sum(D[q] != 0))
+        print(f"  decay case {q}: K {K} got {Kg}, computed {kv}")
+        assert Kg == K and kv >= K
+        assert np.abs(D[q][-kv:] - w[-kv:]).max() < 1e-13 * w[-1]
+        assert np.all(D[q][:-kv] == 0) and np.all(V[q][:, :-kv] == 0)
+        Vk = V[q][:, -K:]
+        assert np.abs(Gs[q] @ Vk - Vk * D[q][-K:]).max() < 1e-12 * w[-1]
+        P, Pr = Vk @ Vk.conj().T, U[:, -K:] @ U[:, -K:].conj().T
+        assert np.abs(P - Pr).max() < 1e-8
