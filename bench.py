@@ -613,9 +613,13 @@ def main():
             torch.cuda.synchronize()
             c32.append((time.perf_counter() - ta, ic))
             log(f"[bench] c32 run {k + 1}: {c32[-1][0]:.3f} s, factor {ic.t_factor_ms:.1f} ms, iters {ic.iterations}")
-        # the plain FP64 path (fill + FP64 LU, rocBLAS trailing update): one
-        # warm-up (rocBLAS handle, kernel loading), then the mean of two runs
-        log("[bench] plain FP64 path (fill + FP64 LU): warm-up")
+        # the plain FP64 path (fill + FP64 LU) with its fastest trailing update,
+        # rocBLAS ZGEMM (fp64_gemm_library = 1): one warm-up (rocBLAS handle,
+        # kernel loading), then the mean of two runs; then one run with the
+        # library's own k_zgemm_nn (its default) for comparison
+        lib0 = api.tune_get("fp64_gemm_library")
+        api.tune_set("fp64_gemm_library", 1)
+        log("[bench] plain FP64 path (fill + FP64 LU, rocBLAS trailing update): warm-up")
         run(0, Xref)
         f64 = []
         for k in range(2):
@@ -625,12 +629,21 @@ def main():
             torch.cuda.synchronize()
             f64.append((time.perf_counter() - ta, info64))
             log(f"[bench] fp64 run {k + 1}: {f64[-1][0]:.3f} s")
+        api.tune_set("fp64_gemm_library", 0)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        run(0, Xref)
+        torch.cuda.synchronize()
+        fp64_own_s = time.perf_counter() - ta
+        log(f"[bench] fp64 run with k_zgemm_nn: {fp64_own_s:.3f} s")
+        api.tune_set("fp64_gemm_library", lib0)
         fp64_s = float(np.mean([t for t, _ in f64]))
         c32_s = float(np.mean([t for t, _ in c32]))
         err = float(torch.linalg.norm(X - Xref) / torch.linalg.norm(Xref))
         err32 = float(torch.linalg.norm(Xc - Xref) / torch.linalg.norm(Xref))
         result["fp64_path_s"] = round(fp64_s, 3)
         result["fp64_path_runs_s"] = [round(t, 3) for t, _ in f64]
+        result["fp64_path_own_zgemm_s"] = round(fp64_own_s, 3)
         result["fp64_factor_ms"] = round(float(np.mean([i.t_factor_ms for _, i in f64])), 1)
         result["speedup_vs_fp64"] = round(fp64_s / step_s, 3)
         result["rel_field_err_vs_fp64"] = err

@@ -29,6 +29,12 @@ int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double 
  * pairs with lambda >= cut lambda_max (the others: 0 and zero columns). */
 int midas_test_heev(midas_ctx *ctx, int n, int batch, const double *G, double *D, double *V, double cut, int path);
 
+/* Pivoted Cholesky G ~= L L^H of a batch of Hermitian positive semi-definite
+ * n x n matrices (the CBF Gram matrices' rank reduction), stopped where the
+ * largest remaining diagonal is <= tolrel max diag(G): L n x n per matrix
+ * (columns past the rank zero), rank[b] its number of columns. */
+int midas_test_pchol(midas_ctx *ctx, int n, int batch, const double *G, double *L, int *rank, double tolrel);
+
 /* C -= A B with the split FP16/BF16 trailing-update kernel (mode as
  * TrailingMode: 1 f16, 3 f16x3, 11 bf16, 13 bf16x3); operands complex128,
  * rounded to complex FP32 before the call. */

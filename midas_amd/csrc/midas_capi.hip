@@ -1224,6 +1224,25 @@ int midas_test_heev(midas_ctx *ctx, int n, int batch, const double *G, double *D
     return 0;
 }
 
+int midas_test_pchol(midas_ctx *ctx, int n, int batch, const double *G, double *L, int *rank, double tolrel)
+{
+    MIDAS_CHECK(ctx && G && L && rank && n > 0 && batch > 0, -1, "test_pchol: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    const size_t nn = (size_t)n * n;
+    DBuf dG, dL, dR;
+    TRY(dG.alloc(sizeof(double2) * nn * batch));
+    TRY(dL.alloc(sizeof(double2) * nn * batch));
+    TRY(dR.alloc(sizeof(int) * batch));
+    MIDAS_HIP(hipMemcpyAsync(dG.p, G, sizeof(double2) * nn * batch, hipMemcpyHostToDevice, st));
+    TRY(launch_pchol_batched(n, dG.as<double2>(), n, (long)nn, dL.as<double2>(), n, (long)nn, dR.as<int>(), batch,
+                             tolrel, st));
+    MIDAS_HIP(hipMemcpyAsync(L, dL.p, sizeof(double2) * nn * batch, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipMemcpyAsync(rank, dR.p, sizeof(int) * batch, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
 int midas_test_trailing(midas_ctx *ctx, int mode, int m, int n, int k, const double *A, long lda, const double *B,
                         long ldb, double *C, long ldc)
 {

@@ -13,8 +13,9 @@
 //      library.  The reference takes the SVD X_b = U S W^H (GESVD, JOB='U');
 //      only sigma and the span of the leading left vectors enter the result,
 //      so here they come from the Gram matrix X^H X (or X X^H when X is wide):
-//      one GEMM per block and ONE launch of this library's batched Hermitian
-//      eigensolver (midas_heev.hip) for all blocks, U = X W S^-1.  For
+//      one GEMM per block, pivoted Cholesky G = L L^H to the numerical rank r,
+//      and ONE launch of this library's batched Hermitian eigensolver on the
+//      r x r matrices L^H L (midas_heev.hip) for all blocks, U = X W S^-1.  For
 //      thresholds below 1e-5 the squared spectrum is not resolved well enough
 //      and each block takes rocSOLVER gesvd instead.
 //      K_b = #{sigma_d / sigma_1 >= threshold}; C_b = U(1:3 n_b, 1:K_b).
@@ -92,26 +93,32 @@ static const zc ONE = {1.0, 0.0}, ZERO = {0.0, 0.0};
 // to a few digits down to ~1e-6 (eigenvalue ratio 1e-12 against an absolute
 // eigenvalue error of ~n eps lambda_1); smaller thresholds take the SVD.
 constexpr double CBF_GRAM_MIN_THRESHOLD = 1e-5;
+// The Gram matrices are reduced to their numerical rank by pivoted Cholesky,
+// stopped where the largest remaining diagonal is <= this times the largest
+// diagonal: the dropped semi-definite remainder is at the level of the
+// eigensolver's own backward error (a few hundred eps |G|), far below the
+// smallest kept eigenvalue (threshold^2 |G| >= 1e-10 |G|).
+constexpr double CBF_GRAM_RANK_TOL = 100.0 * 2.220446049250313e-16;
 
-// out(:, c) = W(0:rows, rpad-1-c) * (scale ? 1/sqrt(lambda[rpad-1-c]) : 1),
-// c < K: the eigenvectors of the K largest eigenvalues (ascending storage),
-// largest first.
-__global__ void k_pick_eigvecs(const double2 *W, long ldw, const double *lam, int rpad, int rows, int K, int scale,
+// out(:, c) = W(0:rows, rpad-1-c) * lambda[rpad-1-c]^(-power/2), c < K: the
+// eigenvectors of the K largest eigenvalues (ascending storage), largest
+// first, scaled by lambda^-1/2 (power 1) or lambda^-1 (power 2).
+__global__ void k_pick_eigvecs(const double2 *W, long ldw, const double *lam, int rpad, int rows, int K, int power,
                                double2 *out, long ldo)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     if (r >= rows || c >= K) return;
     const int src = rpad - 1 - c;
-    const double s = scale ? 1.0 / sqrt(lam[src]) : 1.0;
+    const double s = power == 2 ? 1.0 / lam[src] : power == 1 ? 1.0 / sqrt(lam[src]) : 1.0;
     double2 v = W[(long)src * ldw + r];
     out[(long)c * ldo + r] = make_double2(v.x * s, v.y * s);
 }
-static int launch_pick_eigvecs(const double2 *W, long ldw, const double *lam, int rpad, int rows, int K, bool scale,
+static int launch_pick_eigvecs(const double2 *W, long ldw, const double *lam, int rpad, int rows, int K, int power,
                                double2 *out, long ldo, hipStream_t st)
 {
     hipLaunchKernelGGL(k_pick_eigvecs, dim3((rows + 255) / 256, K), dim3(256), 0, st, W, ldw, lam, rpad, rows, K,
-                       (int)scale, out, ldo);
+                       power, out, ldo);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
@@ -540,15 +547,17 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
     std::vector<int> K(nb);
     if (thr >= CBF_GRAM_MIN_THRESHOLD) {
         // Gram route: G_b = X^H X (T2c <= m_b, then U = X W S^-1) or X X^H
-        // (T2c > m_b, then U = W); every block's G zero-padded to rpad and
-        // decomposed by one launch of the batched Hermitian eigensolver
-        // (midas_heev.hip).
+        // (T2c > m_b, then U = W); every block's G zero-padded to rpad,
+        // reduced to its numerical rank r_b by pivoted Cholesky G = L L^H,
+        // and B = L^H L (r x r, r = max r_b, zero-padded) decomposed by one
+        // launch of the batched Hermitian eigensolver (midas_heev.hip):
+        // B w = lambda w gives G (L w) = lambda (L w), |L w|^2 = lambda.
         const long gs = (long)rpad * rpad;
-        DBuf G, D, dinfo, Wk, HW;
+        DBuf G, Lc, dr, D, dinfo, Wk, Wr, HW, Bm;
         TRY(G.alloc(sizeof(double2) * gs * nb));
-        TRY(D.alloc(sizeof(double) * (size_t)rpad * nb));
+        TRY(Lc.alloc(sizeof(double2) * gs * nb));
+        TRY(dr.alloc(sizeof(int) * nb));
         TRY(dinfo.alloc(sizeof(int) * nb));
-        TRY(HW.alloc(heev_work_bytes(rpad, nb)));
         TRY(Wk.alloc(sizeof(double2) * (size_t)T2c * T2c));
         MIDAS_HIP(hipMemsetAsync(G.p, 0, G.bytes, st));
         bool all_tall = true;
@@ -568,33 +577,54 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                     TRY(launch_cgemm<double>(m, m, T2c, ONE, Xb(b), mp, 0, Xb(b), mp, 2, ZERO, Gb, rpad, st));
             }
         }
-        // (large orders compute only the pairs at or above a quarter of the
-        // threshold's eigenvalue ratio, a margin over the count below)
-        TRY(launch_heev_batched(rpad, G.as<double2>(), rpad, gs, D.as<double>(), rpad, dinfo.as<int>(), nb, HW.p, st,
-                                0.25 * thr * thr));
-        std::vector<double> hd((size_t)rpad * nb);
-        std::vector<int> hi(nb);
-        MIDAS_HIP(hipMemcpyAsync(hi.data(), dinfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
-        TRY(copy_sync(hd.data(), D.p, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, st));
+        TRY(launch_pchol_batched(rpad, G.as<double2>(), rpad, gs, Lc.as<double2>(), rpad, gs, dr.as<int>(), nb,
+                                 CBF_GRAM_RANK_TOL, st));
+        std::vector<int> rk(nb);
+        TRY(copy_sync(rk.data(), dr.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+        const int rr = *std::max_element(rk.begin(), rk.end());
+        G.release();
+        std::vector<double> hd((size_t)std::max(rr, 1) * nb, 0.0);
+        if (rr > 0) {
+            const long bs2 = (long)rr * rr;
+            TRY(Bm.alloc(sizeof(double2) * bs2 * nb));
+            TRY(D.alloc(sizeof(double) * (size_t)rr * nb));
+            TRY(HW.alloc(heev_work_bytes(rr, nb)));
+            TRY(Wr.alloc(sizeof(double2) * (size_t)rr * T2c));
+            TRY(launch_cgemm<double>(rr, rr, rpad, ONE, Lc.p, rpad, 2, Lc.p, rpad, 0, ZERO, Bm.p, rr, st, nb, gs, gs,
+                                     bs2));
+            // (large orders compute only the pairs at or above a quarter of
+            // the threshold's eigenvalue ratio, a margin over the count below)
+            TRY(launch_heev_batched(rr, Bm.as<double2>(), rr, bs2, D.as<double>(), rr, dinfo.as<int>(), nb, HW.p, st,
+                                    0.25 * thr * thr));
+            std::vector<int> hi(nb);
+            MIDAS_HIP(hipMemcpyAsync(hi.data(), dinfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+            TRY(copy_sync(hd.data(), D.p, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, st));
+            for (int b = 0; b < nb; ++b)
+                MIDAS_CHECK(hi[b] == 0, -2, "cbfm: eigensolver failed on block %d (info %d)", b, hi[b]);
+        }
         for (int b = 0; b < nb; ++b) {
-            MIDAS_CHECK(hi[b] == 0, -2, "cbfm: eigensolver failed on block %d (info %d)", b, hi[b]);
-            const int m = 3 * sb[b], r = std::min(m, T2c);
-            const double *lam = hd.data() + (size_t)b * rpad;   // ascending
-            const double s1 = std::sqrt(std::max(lam[rpad - 1], 0.0));
+            const int m = 3 * sb[b], r = std::min(std::min(m, T2c), rr);
+            const double *lam = hd.data() + (size_t)b * std::max(rr, 1);   // ascending
+            const double s1 = rr > 0 ? std::sqrt(std::max(lam[rr - 1], 0.0)) : 0.0;
             int kb = 0;
             for (int d = 0; d < r; ++d)
-                if (s1 > 0 && std::sqrt(std::max(lam[rpad - 1 - d], 0.0)) / s1 >= thr) ++kb;
+                if (s1 > 0 && std::sqrt(std::max(lam[rr - 1 - d], 0.0)) / s1 >= thr) ++kb;
             K[b] = kb;
             if (!kb) continue;
-            const double2 *Wb = G.as<double2>() + b * gs;
+            const double2 *Wb = Bm.as<double2>() + (long)b * rr * rr;
+            const double *Db = D.as<double>() + (size_t)b * rr;
+            const double2 *Lb = Lc.as<double2>() + b * gs;
             if (T2c <= m) {
-                TRY(launch_pick_eigvecs(Wb, rpad, D.as<double>() + (size_t)b * rpad, rpad, T2c, kb, true,
-                                        Wk.as<double2>(), T2c, st));
+                // W_G S^-1 = L W Lambda^-1, then U = X (W_G S^-1)
+                TRY(launch_pick_eigvecs(Wb, rr, Db, rr, rr, kb, 2, Wr.as<double2>(), rr, st));
+                TRY(launch_cgemm<double>(T2c, kb, rr, ONE, Lb, rpad, 0, Wr.p, rr, 0, ZERO, Wk.p, T2c, st));
                 TRY(launch_cgemm<double>(3 * nbk[b], kb, T2c, ONE, Xb(b), mp, 0, Wk.p, T2c, 0, ZERO,
                                          Call.as<double2>() + coff[b], 3L * nbk[b], st));
             } else {
-                TRY(launch_pick_eigvecs(Wb, rpad, D.as<double>() + (size_t)b * rpad, rpad, 3 * nbk[b], kb, false,
-                                        Call.as<double2>() + coff[b], 3L * nbk[b], st));
+                // U = L W Lambda^-1/2 (rows of the block's own cells)
+                TRY(launch_pick_eigvecs(Wb, rr, Db, rr, rr, kb, 1, Wr.as<double2>(), rr, st));
+                TRY(launch_cgemm<double>(3 * nbk[b], kb, rr, ONE, Lb, rpad, 0, Wr.p, rr, 0, ZERO,
+                                         Call.as<double2>() + coff[b], 3L * nbk[b], st));
             }
         }
     } else {

@@ -742,6 +742,123 @@ __global__ __launch_bounds__(HL_TH) void k_heev_large(int n, double2 *Gall, long
     if (tid == 0) info[b] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Pivoted Cholesky factorisation of a batch of Hermitian positive
+// semi-definite matrices, stopped at numerical rank: G ~= L L^H with L n x r,
+// r the first step whose largest remaining diagonal is <= tolrel times G's
+// largest diagonal (the remainder S = G - L L^H is semi-definite with
+// |S| <= (n - r) * that diagonal; Higham, Accuracy and Stability of Numerical
+// Algorithms, 2nd ed., 10.3).  The CBF Gram matrices have a numerical rank
+// well below their order, so their eigenpairs come from the r x r matrix
+// L^H L instead (same nonzero eigenvalues; eigenvectors L w / sqrt(lambda)).
+// One 1024-thread workgroup per matrix, lane i owning row i (n <= 1024);
+// columns r .. n-1 of L are zeroed; rank[b] = r.
+constexpr int PC_TH = 1024;
+
+__global__ __launch_bounds__(PC_TH) void k_pchol(int n, const double2 *Gall, long ld, long gstride, double2 *Lall,
+                                                 long ldl, long lstride, int *rank, double tolrel)
+{
+    __shared__ double2 lp[PC_TH];   // row p of L, columns 0 .. k-1
+    __shared__ double wv_d[PC_TH / 64];
+    __shared__ int wv_i[PC_TH / 64];
+    __shared__ double s_d;
+    __shared__ int s_p;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const long b = blockIdx.x;
+    const double2 *G = Gall + b * gstride;
+    double2 *L = Lall + b * lstride;
+    const bool row = tid < n;
+    double d = row ? G[tid + (long)tid * ld].x : 0.0;
+    bool piv = false;
+    // largest remaining diagonal (ties: the lower row) over the workgroup
+    auto argmax = [&](double v, int &p) -> double {
+        int idx = tid;
+        for (int o = 32; o > 0; o >>= 1) {
+            const double v2 = __shfl_xor(v, o);
+            const int i2 = __shfl_xor(idx, o);
+            if (v2 > v || (v2 == v && i2 < idx)) {
+                v = v2;
+                idx = i2;
+            }
+        }
+        if (lane == 0) {
+            wv_d[w] = v;
+            wv_i[w] = idx;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double bv = wv_d[0];
+            int bi = wv_i[0];
+            for (int q = 1; q < PC_TH / 64; ++q)
+                if (wv_d[q] > bv || (wv_d[q] == bv && wv_i[q] < bi)) {
+                    bv = wv_d[q];
+                    bi = wv_i[q];
+                }
+            s_d = bv;
+            s_p = bi;
+        }
+        __syncthreads();
+        p = s_p;
+        return s_d;
+    };
+    int p0;
+    const double tol = tolrel * fmax(argmax(row ? d : -1.0, p0), 0.0);
+    int k = 0;
+    for (; k < n; ++k) {
+        int p;
+        const double dp = argmax(row && !piv ? d : -1.0, p);
+        if (!(dp > tol)) break;
+        for (int j = tid; j < k; j += PC_TH) lp[j] = L[p + (long)j * ldl];
+        __syncthreads();
+        if (row) {
+            double2 l;
+            if (tid == p) {
+                l = make_double2(sqrt(dp), 0.0);
+                d = 0.0;
+                piv = true;
+            } else if (piv) {
+                l = make_double2(0.0, 0.0);
+            } else {
+                double2 a = G[tid + (long)p * ld], a2 = make_double2(0.0, 0.0);
+                const double2 *lr = L + tid;
+                int j = 0;
+                for (; j + 1 < k; j += 2) {
+                    const double2 t = zcmul2(lp[j], lr[(long)j * ldl]);
+                    const double2 t2 = zcmul2(lp[j + 1], lr[(long)(j + 1) * ldl]);
+                    a.x -= t.x;
+                    a.y -= t.y;
+                    a2.x -= t2.x;
+                    a2.y -= t2.y;
+                }
+                if (j < k) {
+                    const double2 t = zcmul2(lp[j], lr[(long)j * ldl]);
+                    a.x -= t.x;
+                    a.y -= t.y;
+                }
+                const double is = 1.0 / sqrt(dp);
+                l = make_double2((a.x + a2.x) * is, (a.y + a2.y) * is);
+                d -= l.x * l.x + l.y * l.y;
+            }
+            L[tid + (long)k * ldl] = l;
+        }
+        __syncthreads();
+    }
+    for (long idx = tid; idx < (long)(n - k) * n; idx += PC_TH)
+        L[(idx % n) + (k + idx / n) * ldl] = make_double2(0.0, 0.0);
+    if (tid == 0) rank[b] = k;
+}
+
+int launch_pchol_batched(int n, const double2 *G, long ld, long stride, double2 *L, long ldl, long lstride, int *rank,
+                         int batch, double tolrel, hipStream_t st)
+{
+    if (n <= 0 || batch <= 0) return 0;
+    MIDAS_CHECK(n <= PC_TH && ld >= n && ldl >= n && stride >= ld * n && lstride >= ldl * n, -1,
+                "pchol: bad layout (n %d, ld %ld, ldl %ld)", n, ld, ldl);
+    hipLaunchKernelGGL(k_pchol, dim3(batch), dim3(PC_TH), 0, st, n, G, ld, stride, L, ldl, lstride, rank, tolrel);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
 constexpr int HEEV_SMALL_MAX = 256;   // orders the one-stage kernel takes (path 0)
 
 static size_t heev_large_bytes(int n) { return (size_t)n * n * (sizeof(double2) + sizeof(double)) + (size_t)n * HL_NB * sizeof(double2) + (size_t)HL_WAVES * 6 * n * sizeof(double); }

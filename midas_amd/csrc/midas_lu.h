@@ -102,6 +102,12 @@ int launch_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, lon
 // large-order kernel, which computes only the eigenpairs with
 // lambda >= cut lambda_max when cut > 0 and leaves the others 0).
 size_t heev_work_bytes(int n, int batch);
+// Pivoted Cholesky G ~= L L^H of a strided batch of Hermitian positive
+// semi-definite n x n matrices (n <= 1024), stopped where the largest remaining
+// diagonal is <= tolrel max diag(G); L is n x n per matrix with columns past
+// the rank zeroed; rank[b] receives the number of columns (midas_heev.hip).
+int launch_pchol_batched(int n, const double2 *G, long ld, long stride, double2 *L, long ldl, long lstride, int *rank,
+                         int batch, double tolrel, hipStream_t st);
 int launch_heev_batched(int n, double2 *G, long ld, long stride, double *D, long dstride, int *info, int batch,
                         void *work, hipStream_t st, double cut = 0.0, int path = 0);
 

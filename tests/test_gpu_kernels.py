@@ -279,3 +279,35 @@ def test_heev_large_leading_pairs(mid, n):
         assert np.abs(Gs[q] @ Vk - Vk * D[q][-K:]).max() < 1e-12 * w[-1]
         P, Pr = Vk @ Vk.conj().T, U[:, -K:] @ U[:, -K:].conj().T
         assert np.abs(P - Pr).max() < 1e-8
+
+
+@pytest.mark.parametrize("n", [1, 7, 182, 870])
+def test_pchol_rank_reduction(mid, n):
+    """Pivoted Cholesky of the CBF Gram matrices: G = L L^H to the stopping
+    tolerance, the exact rank of an exactly rank-deficient X^H X, a smoothly
+    decaying spectrum cut where its remaining diagonal falls below 100 eps,
+    and the zero matrix (rank 0)."""
+    rng = np.random.default_rng(11 + n)
+    cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    r0 = max(1, n // 4)
+    X1 = cr(2 * n, r0) @ cr(r0, n)
+    X2 = cr(2 * n, n) * np.logspace(0, -9, n)
+    Gs = np.stack([X1.conj().T @ X1, X2.conj().T @ X2, np.zeros((n, n), complex)])
+    G = np.ascontiguousarray(Gs.transpose(0, 2, 1))
+    L = np.zeros(3 * n * n, dtype=complex)
+    rk = np.zeros(3, dtype=np.int32)
+    tol = 100 * np.finfo(float).eps
+    print(f"\n[case] pchol n={n}", flush=True)
+    r = mid.lib.midas_test_pchol(mid.ctx, C.c_int(n), C.c_int(3), _p(G), _p(L), rk.ctypes.data_as(C.c_void_p),
+                                 C.c_double(tol))
+    assert r == 0, mid.lib.midas_last_error()
+    L = L.reshape(3, n, n).transpose(0, 2, 1)
+    print("ranks", list(rk))
+    assert rk[0] == min(r0, n) and rk[2] == 0 and np.all(L[2] == 0)
+    for q in range(2):
+        Gq, Lq = Gs[q], L[q]
+        assert np.all(Lq[:, rk[q]:] == 0)
+        nrm = np.abs(np.diag(Gq)).max()
+        err = np.abs(Gq - Lq @ Lq.conj().T).max() / nrm
+        print(f"  case {q}: rank {rk[q]}, |G - L L^H| / max diag {err:.1e}")
+        assert err < 100 * n * np.finfo(float).eps
