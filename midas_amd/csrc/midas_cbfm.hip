@@ -124,6 +124,180 @@ static int launch_pick_eigvecs(const double2 *W, long ldw, const double *lam, in
 }
 
 // ---------------------------------------------------------------------------
+// Mixed-precision block solves: the batch of extended-block systems
+// Z_b X_b = E_b factored in complex FP32 (trailing updates on the split FP16 /
+// BF16 or FP32 matrix cores, one batched LU) and refined in FP64 against the
+// FP64 Z_b until every block's normwise backward error
+// max_j |e_j - Z x_j|_inf / (|Z|_inf |x_j|_inf + |e_j|_inf) is <= tol; a
+// block that does not get there is solved again by the FP64 LU.  The
+// reference's sparse block solver refines its direct solution the same way
+// (pardiso_solver.f90:123-142, iparm(8) refinement steps).
+// ---------------------------------------------------------------------------
+__global__ void k_flat_z2c(const double2 *S, float2 *D, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        D[i] = make_float2((float)S[i].x, (float)S[i].y);
+}
+__global__ void k_flat_c2z_add(const float2 *S, double2 *D, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        D[i] = make_double2(D[i].x + (double)S[i].x, D[i].y + (double)S[i].y);
+}
+// row sums of |Z_b| (row r of block blockIdx.y), then their maximum per block
+__global__ void k_blk_rowsum(const double2 *Z, long ld, long zs, int m, double *rs)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const double2 *z = Z + blockIdx.y * zs + r;
+    double s = 0.0;
+    for (int c = 0; c < m; ++c) {
+        const double2 v = z[(long)c * ld];
+        s += sqrt(v.x * v.x + v.y * v.y);
+    }
+    rs[(long)blockIdx.y * m + r] = s;
+}
+__global__ void k_blk_max(const double *v, int m, double *out)
+{
+    __shared__ double red[4];
+    double x = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) x = fmax(x, v[(long)blockIdx.x * m + i]);
+    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+// column maxima of |R|, |X|, |E| for column blockIdx.x of block blockIdx.y
+__global__ void k_blk_colmax3(const double2 *R, const double2 *X, const double2 *E, int m, long ld, long xs, int ncol,
+                              double *out)
+{
+    __shared__ double red[3][4];
+    const long off = blockIdx.y * xs + (long)blockIdx.x * ld;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        const double2 r = R[off + i], x = X[off + i], e = E[off + i];
+        a = fmax(a, fmax(fabs(r.x), fabs(r.y)));
+        b = fmax(b, fmax(fabs(x.x), fabs(x.y)));
+        c = fmax(c, fmax(fabs(e.x), fabs(e.y)));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a = fmax(a, __shfl_xor(a, o, 64));
+        b = fmax(b, __shfl_xor(b, o, 64));
+        c = fmax(c, __shfl_xor(c, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+        red[2][threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int q = threadIdx.x;
+        out[((long)blockIdx.y * ncol + blockIdx.x) * 3 + q] =
+            fmax(fmax(red[q][0], red[q][1]), fmax(red[q][2], red[q][3]));
+    }
+}
+
+struct BlockMixedWs {
+    DBuf W, E, R, D, tmp, scr, rs, anorm, cm;
+};
+
+// X (nc blocks of mpc x T2c at stride mpc T2c, ld mpc) holds E on entry and
+// X on return; Z (nc blocks of mpc x mpc, ld mpc) stays unmodified.
+static int block_solve_mixed(int mode, int nc, int mpc, int T2c, const double2 *Z, double2 *X, int *ipiv, int *perm,
+                             std::vector<int> &hinfo, BlockMixedWs &bw, int nb, int nb64, hipStream_t st)
+{
+    const long zs = (long)mpc * mpc, xs = (long)mpc * T2c, nx = xs * nc;
+    const dim3 fg(2048), fb(256);
+    TRY(bw.W.reserve(sizeof(float2) * (size_t)zs * nc));
+    hipLaunchKernelGGL(k_flat_z2c, fg, fb, 0, st, Z, bw.W.as<float2>(), zs * nc);
+    TRY(bw.E.reserve(sizeof(double2) * (size_t)nx));
+    TRY(bw.R.reserve(sizeof(double2) * (size_t)nx));
+    TRY(bw.D.reserve(sizeof(float2) * (size_t)nx));
+    MIDAS_HIP(hipMemcpyAsync(bw.E.p, X, sizeof(double2) * nx, hipMemcpyDeviceToDevice, st));
+    // ||Z_b||_inf
+    TRY(bw.rs.reserve(sizeof(double) * (size_t)mpc * nc));
+    TRY(bw.anorm.reserve(sizeof(double) * nc));
+    hipLaunchKernelGGL(k_blk_rowsum, dim3((mpc + 255) / 256, nc), dim3(256), 0, st, Z, (long)mpc, zs, mpc,
+                       bw.rs.as<double>());
+    hipLaunchKernelGGL(k_blk_max, dim3(nc), dim3(256), 0, st, bw.rs.as<double>(), mpc, bw.anorm.as<double>());
+    MIDAS_HIP(hipGetLastError());
+    // batched complex-FP32 LU
+    LuWork<float> w;
+    TRY(w.alloc(mpc, nb, nc, zs));
+    TrailingGemm<float> tg;
+    tg.mode = mode == MIDAS_SOLVE_C32_IR ? TRAIL_NATIVE : mode == MIDAS_SOLVE_BF16_IR ? TRAIL_BF16_1 : TRAIL_F16_1;
+    tg.batch = nc;
+    tg.stride = zs;
+    if (tg.mode != TRAIL_NATIVE) {
+        TRY(bw.scr.reserve(trail16_batch_bytes(mpc, mpc, nb, nc)));
+        tg.scratch = bw.scr.p;
+    }
+    TRY(getrf<float>(bw.W.as<float2>(), mpc, mpc, ipiv, w, st, tg));
+    TRY(ipiv_to_perm_dev(ipiv, perm, mpc, nc, st));
+    TRY(copy_sync(hinfo.data(), w.info, sizeof(int) * nc, hipMemcpyDeviceToHost, st));
+    // refinement: X = 0, R = E; D = LU^-1 R (FP32), X += D, R = E - Z X (FP64)
+    TRY(bw.tmp.reserve(sizeof(float2) * (size_t)getrs_tmp_elems(mpc, nb, T2c) * nc));
+    TRY(bw.cm.reserve(sizeof(double) * 3 * (size_t)T2c * nc));
+    MIDAS_HIP(hipMemsetAsync(X, 0, sizeof(double2) * nx, st));
+    MIDAS_HIP(hipMemcpyAsync(bw.R.p, bw.E.p, sizeof(double2) * nx, hipMemcpyDeviceToDevice, st));
+    const double tol = std::max(1e-14, std::sqrt((double)mpc) * DBL_EPSILON);
+    std::vector<double> cm(3 * (size_t)T2c * nc), an(nc), berr(nc, INFINITY), prev(nc, INFINITY);
+    TRY(copy_sync(an.data(), bw.anorm.p, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
+    std::vector<char> open_(nc, 1);   // still refining
+    for (int it = 0; it < 10; ++it) {
+        hipLaunchKernelGGL(k_flat_z2c, fg, fb, 0, st, bw.R.as<double2>(), bw.D.as<float2>(), nx);
+        TRY(getrs<float>(bw.W.as<float2>(), mpc, mpc, perm, w, bw.D.as<float2>(), mpc, T2c, bw.tmp.as<float2>(), st,
+                         xs));
+        hipLaunchKernelGGL(k_flat_c2z_add, fg, fb, 0, st, bw.D.as<float2>(), X, nx);
+        MIDAS_HIP(hipMemcpyAsync(bw.R.p, bw.E.p, sizeof(double2) * nx, hipMemcpyDeviceToDevice, st));
+        TRY(launch_zgemm_nn(mpc, T2c, mpc, z64{-1.0, 0.0}, Z, mpc, zs, X, mpc, xs, z64{1.0, 0.0}, bw.R.p, mpc, xs, nc,
+                            st, tune(TUNE_ZGEMM_TILE)));
+        hipLaunchKernelGGL(k_blk_colmax3, dim3(T2c, nc), dim3(256), 0, st, bw.R.as<double2>(), X, bw.E.as<double2>(),
+                           mpc, (long)mpc, xs, T2c, bw.cm.as<double>());
+        MIDAS_HIP(hipGetLastError());
+        TRY(copy_sync(cm.data(), bw.cm.p, sizeof(double) * cm.size(), hipMemcpyDeviceToHost, st));
+        bool any = false;
+        for (int q = 0; q < nc; ++q) {
+            double e = 0.0;
+            for (int j = 0; j < T2c; ++j) {
+                const double *c = &cm[((size_t)q * T2c + j) * 3];
+                const double den = an[q] * c[1] + c[2];
+                e = std::max(e, den > 0.0 ? c[0] / den : (c[0] > 0.0 ? INFINITY : 0.0));
+            }
+            prev[q] = berr[q];
+            berr[q] = e;
+            // a block keeps refining while above tol and still gaining
+            open_[q] = std::isfinite(e) && e > tol && e < 0.5 * prev[q];
+            any = any || open_[q];
+        }
+        if (!any) break;
+    }
+    w.release();
+    // blocks above tol: FP64 LU of their Z (unmodified) and E
+    for (int q = 0; q < nc; ++q) {
+        if (berr[q] <= tol) continue;
+        LuWork<double> w64;
+        TRY(w64.alloc(mpc, nb64));
+        TrailingGemm<double> tg64;
+        DBuf Ab;
+        TRY(Ab.alloc(sizeof(double2) * (size_t)zs));
+        double2 *A = Ab.as<double2>();
+        MIDAS_HIP(hipMemcpyAsync(A, Z + q * zs, sizeof(double2) * zs, hipMemcpyDeviceToDevice, st));
+        TRY(getrf<double>(A, mpc, mpc, ipiv, w64, st, tg64));
+        TRY(ipiv_to_perm_dev(ipiv, perm, mpc, 1, st));
+        MIDAS_HIP(hipMemcpyAsync(X + q * xs, bw.E.as<double2>() + q * xs, sizeof(double2) * xs, hipMemcpyDeviceToDevice,
+                                 st));
+        DBuf t64;
+        TRY(t64.alloc(sizeof(double2) * (size_t)getrs_tmp_elems(mpc, nb64, T2c)));
+        TRY(getrs<double>(A, mpc, mpc, perm, w64, X + q * xs, mpc, T2c, t64.as<double2>(), st));
+        int inf1 = 0;
+        TRY(copy_sync(&inf1, w64.info, sizeof(int), hipMemcpyDeviceToHost, st));
+        hinfo[q] = inf1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Grouped small GEMMs: every job is one 32 x 32 tile of C = op(A) B, complex
 // FP64, op = none or plain transpose.  The reduced-matrix assembly has
 // thousands of independent products of a few tens of columns (C_i^T T, C_i^T U,
@@ -470,6 +644,16 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         std::vector<Chunk> chunks;
         long zmax = 0, smax_c = 0, tmax = 0, xmax = 0, pmax_c = 0;
         const int nb_lu = tune(TUNE_LU_NB_FP64);
+        // the block systems: FP64 LU, or a mixed-precision LU + FP64 refinement
+        // (single-term split modes and c32: the batched trailing updates).
+        // Each refinement sweep costs a residual of 8 m^2 T2c FP64 flops, so
+        // the mixed LU (about 5x cheaper than the FP64 one, 8/3 m^3) pays
+        // only when the blocks are much taller than the excitation count:
+        // mode 5 takes f16 for m >= 16 T2c (C5: m / T2c ~ 18.5 gained 7 % of
+        // the block phase; a0006 at ~9 lost 60 %), else the FP64 LU.
+        const int bm_set = tune(TUNE_CBFM_BLOCK_MODE);
+        const int nb_mixed = 256;
+        BlockMixedWs bmw;
         for (int pos = 0; pos < nb;) {
             const long mpc = 3L * sb[order[pos]];
             const long zb = (long)sizeof(double2) * mpc * mpc;
@@ -521,19 +705,26 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                                        cen, rs, Xq(q), mpc, st));
                 }
             }
-            LuWork<double> w;
-            TRY(w.alloc((int)mpc, nb_lu, nc, mpc * mpc));
-            TrailingGemm<double> tg;
-            tg.batch = nc;
-            tg.stride = mpc * mpc;
-            int r = getrf<double>(Zall.as<double2>(), mpc, (int)mpc, ipiv.as<int>(), w, st, tg);
-            if (!r) r = ipiv_to_perm_dev(ipiv.as<int>(), perm.as<int>(), (int)mpc, nc, st);
-            if (!r)
-                r = getrs<double>(Zall.as<double2>(), mpc, (int)mpc, perm.as<int>(), w, Xq(0), mpc, T2c,
-                                  tmp.as<double2>(), st, mpc * T2c);
-            if (!r) r = copy_sync(hinfo.data(), w.info, sizeof(int) * nc, hipMemcpyDeviceToHost, st);
-            w.release();
-            if (r) return r;
+            const int block_mode = bm_set == 5 ? (mpc >= 16L * T2c ? MIDAS_SOLVE_F16_IR : MIDAS_SOLVE_FP64)
+                                   : bm_set == 3 ? MIDAS_SOLVE_F16_IR : bm_set;
+            if (block_mode != MIDAS_SOLVE_FP64) {
+                TRY(block_solve_mixed(block_mode, nc, (int)mpc, T2c, Zall.as<double2>(), Xq(0), ipiv.as<int>(),
+                                      perm.as<int>(), hinfo, bmw, nb_mixed, nb_lu, st));
+            } else {
+                LuWork<double> w;
+                TRY(w.alloc((int)mpc, nb_lu, nc, mpc * mpc));
+                TrailingGemm<double> tg;
+                tg.batch = nc;
+                tg.stride = mpc * mpc;
+                int r = getrf<double>(Zall.as<double2>(), mpc, (int)mpc, ipiv.as<int>(), w, st, tg);
+                if (!r) r = ipiv_to_perm_dev(ipiv.as<int>(), perm.as<int>(), (int)mpc, nc, st);
+                if (!r)
+                    r = getrs<double>(Zall.as<double2>(), mpc, (int)mpc, perm.as<int>(), w, Xq(0), mpc, T2c,
+                                      tmp.as<double2>(), st, mpc * T2c);
+                if (!r) r = copy_sync(hinfo.data(), w.info, sizeof(int) * nc, hipMemcpyDeviceToHost, st);
+                w.release();
+                if (r) return r;
+            }
             for (int q = 0; q < nc; ++q) {
                 const int b = order[ch.pos + q];
                 MIDAS_HIP(hipMemcpy2DAsync(Xb(b), sizeof(double2) * mp, Xq(q), sizeof(double2) * mpc,

@@ -13,6 +13,7 @@ enum {
     TUNE_REFINE_GRAPH,
     TUNE_ZGEMM_TILE,
     TUNE_FP64_GEMM_LIBRARY,
+    TUNE_CBFM_BLOCK_MODE,
     TUNE_COUNT
 };
 int tune(int which);

@@ -148,6 +148,29 @@ def test_cbfm_budgets_force_batches(mid, oracle, dlca6, use_aca):
     assert _rel(E1, Eo) < 1e-9
 
 
+@pytest.mark.parametrize("block_mode", [0, 1, 2, 4, 5])
+@pytest.mark.parametrize("sr", [False, True])
+def test_cbfm_block_solve_modes(mid, oracle, dlca6, block_mode, sr):
+    """The CBF block systems solved by the batched FP64 LU (0) or by a batched
+    complex-FP32 LU with c32 / f16 / bf16 trailing updates (1 / 2 / 4) refined
+    in FP64, or by the size rule (5): the same K per block and the same field
+    as the oracle (dense and SR-masked blocks)."""
+    s = dlca6
+    _case(f"cbfm block mode {block_mode} sr={sr} n={3 * s.n} blocks={len(s.blocks)}")
+    oc = oracle.Cells(s.x, s.y, s.z, s.sc, s.eps, s.k0)
+    th_c, ph_c = g.directions(30)
+    tth, tph = np.array([0.0, 35.0]), np.array([0.0, 80.0])
+    old = api.tune_set("cbfm_block_mode", block_mode)
+    try:
+        E, K, _ = mid.compute_efields_cbfme(s, s.k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), sr=sr, res_sr=0.01)
+    finally:
+        api.tune_set("cbfm_block_mode", old)
+    Eo, Ko, _ = oracle.cbfm(oc, s.blocks, s.ext, th_c, ph_c, tth, tph, sr=sr, res_sr=0.01)
+    print("K", list(K), "oracle", Ko, "vs oracle", _rel(E, Eo))
+    assert list(K) == list(Ko)
+    assert _rel(E, Eo) < 1e-9
+
+
 @pytest.mark.parametrize("mode", [api.SOLVE_F16_IR, api.SOLVE_BF16_IR])
 @pytest.mark.parametrize("n", [1300, 2600])
 def test_two_panel_steps(mid, mode, n):
