@@ -134,66 +134,34 @@ __device__ inline z64 zdiv(z64 a, z64 b)
     return {(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
 }
 
-// acc[q] = sum_l coef[l] * x[l * ld + q] for q < nq (nq <= 3 entries at
-// unit stride): one pass over l with the three sums as independent chains,
-// each summed in ascending l.  The same pass marks bit q of the result when
-// entry j0 + q is among the previous pivots piv[0 .. kk-1].
-__device__ inline int dot3(const double2 *coef, const int *piv, const double2 *x, int ld, int kk, int j0, int nq,
-                           z64 acc[3])
-{
-    int used = 0;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) acc[q] = {0.0, 0.0};
-    // unrolled so that the loads of several l are in flight at once (the sums
-    // still run in ascending l)
-    if (nq == 3) {
-#pragma unroll 2
-        for (int l = 0; l < kk; ++l) {
-            const double2 c = coef[l];
-            const int d = piv[l] - j0;
-            used |= (unsigned)d < 3u ? 1 << d : 0;
-            const double2 *row = x + (long)l * ld;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const double2 y = row[q];
-                acc[q].r += c.x * y.x - c.y * y.y;
-                acc[q].i += c.x * y.y + c.y * y.x;
-            }
-        }
-    } else {
-#pragma unroll 2
-        for (int l = 0; l < kk; ++l) {
-            const double2 c = coef[l];
-            const int d = piv[l] - j0;
-            used |= (unsigned)d < 3u ? 1 << d : 0;
-            const double2 *row = x + (long)l * ld;
-#pragma unroll
-            for (int q = 0; q < 3; ++q)
-                if (q < nq) {
-                    const double2 y = row[q];
-                    acc[q].r += c.x * y.x - c.y * y.y;
-                    acc[q].i += c.x * y.y + c.y * y.x;
-                }
-        }
-    }
-    return used;
-}
-
+// The cross terms of step k need u_l . u_k and v_k . v_l for every l < k.
+// They come from the non-conjugated Gram matrices Gu(l, l') = u_l . u_l',
+// Gv(l, l') = v_l . v_l' (per job in global memory, kmax x kmax each) and the
+// products of the generated row / column with the previous terms, summed in
+// the same passes that read those terms for the residuals:
+//   u_l . u_k = u_l . col - sum_l' v_l'(jk) Gu(l, l'),
+//   v_k . v_l = (row . v_l - sum_l' u_l'(ik) Gv(l, l')) / pivot,
+// so no step reads the factors a second time.  Per-wave partial sums are
+// added in a fixed order, so the result does not depend on scheduling.
 template <class G>
 __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__ jobs, double2 *__restrict__ U,
-                                               double2 *__restrict__ Vt, int kmax, double eps, int *__restrict__ kout)
+                                               double2 *__restrict__ Vt, int kmax, double eps, int *__restrict__ kout,
+                                               double2 *__restrict__ gram)
 {
     const AcaJob jb = jobs[blockIdx.x];
     const int m = jb.m, n = jb.n;
     const int gm = (m + 2) / 3, gn = (n + 2) / 3;
     double2 *u = U + jb.u_off;    // u(i, l) = u[l*m + i]
     double2 *v = Vt + jb.v_off;   // v(j, l) = v[l*n + j]
+    double2 *Gu = gram + (long)blockIdx.x * 2 * kmax * kmax, *Gv = Gu + (long)kmax * kmax;
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
 
     __shared__ int Ir[ACA_KMAX + 1], Jc[ACA_KMAX + 1];
-    __shared__ double2 coef[ACA_KMAX];
-    __shared__ double2 dpart[ACA_KMAX], epart[ACA_KMAX];
+    __shared__ double2 cu[ACA_KMAX], cv[ACA_KMAX];   // u_l(ik), v_l(jk)
+    __shared__ double2 part[ACA_W][ACA_KMAX];       // per-wave row . v_l / col . u_l
+    __shared__ double2 brow[ACA_KMAX];              // row . v_l
     __shared__ double sred[4][ACA_W];
+    __shared__ double2 zred[2][ACA_W];              // u_k . u_k, v_k . v_k (non-conjugated)
     __shared__ int ired[2][ACA_W];
     __shared__ double s_cross;
 
@@ -204,20 +172,41 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
     for (int k = 1; k <= kmax; ++k) {
         const int kk = k - 1;             // 0-based index of the new term
         const int ik = Ir[kk];
-        // ---- (1) residual row ik into v(:, kk); arg max over unused columns
-        if (t < kk) {
-            double2 x = u[(long)t * m + ik];
-            coef[t] = x;
-        }
+        // ---- (1) residual row ik into v(:, kk); arg max over unused columns;
+        // row . v_l for l < kk
+        if (t < kk) cu[t] = u[(long)t * m + ik];
         __syncthreads();
         double bv = -1.0;
         int bi = INT_MAX;
-        for (int g = t; g < gn; g += ACA_T) {
-            z64 e[3];
-            gen.row3(jb, ik, g, e);
-            z64 acc[3];
-            const int nq = min(3, n - 3 * g);
-            const int used = dot3(coef, Jc, v + 3 * g, n, kk, 3 * g, nq, acc);
+        for (int g0 = 0; g0 < gn; g0 += ACA_T) {   // uniform trip count: the wave sums below
+            const int g = g0 + t;
+            const bool ok = g < gn;
+            const int nq = ok ? min(3, n - 3 * g) : 0;
+            z64 e[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+            if (ok) gen.row3(jb, ik, g, e);
+            z64 acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+            int used = 0;
+#pragma unroll 2
+            for (int l = 0; l < kk; ++l) {
+                const double2 c = cu[l];
+                const int d = Jc[l] - 3 * g;
+                used |= (unsigned)d < 3u ? 1 << d : 0;
+                double px = 0.0, py = 0.0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double2 y = q < nq ? v[(long)l * n + 3 * g + q] : make_double2(0.0, 0.0);
+                    acc[q].r += c.x * y.x - c.y * y.y;
+                    acc[q].i += c.x * y.y + c.y * y.x;
+                    px += e[q].r * y.x - e[q].i * y.y;
+                    py += e[q].r * y.y + e[q].i * y.x;
+                }
+                px = wave_sum(px);
+                py = wave_sum(py);
+                if (lane == 0) {
+                    const double2 o = g0 ? part[w][l] : make_double2(0.0, 0.0);
+                    part[w][l] = make_double2(o.x + px, o.y + py);
+                }
+            }
 #pragma unroll
             for (int q = 0; q < 3; ++q)
                 if (q < nq) {
@@ -240,10 +229,15 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
         const double2 pv = v[(long)kk * n + jk];
         const z64 piv = {pv.x, pv.y};
         if (t == 0) Jc[kk] = jk;
-        if (t < kk) coef[t] = v[(long)t * n + jk];
+        if (t < kk) {
+            cv[t] = v[(long)t * n + jk];
+            double2 b = part[0][t];
+            for (int q = 1; q < ACA_W; ++q) b = make_double2(b.x + part[q][t].x, b.y + part[q][t].y);
+            brow[t] = b;
+        }
         __syncthreads();
-        // ---- (2) scale v(:, kk) by the pivot; ||v_k||^2
-        double nv2 = 0.0;
+        // ---- (2) scale v(:, kk) by the pivot; ||v_k||^2 and v_k . v_k
+        double nv2 = 0.0, vvx = 0.0, vvy = 0.0;
         for (int g = t; g < gn; g += ACA_T) {
             const int nq = min(3, n - 3 * g);
             double2 x[3];   // the three loads issued before the divisions
@@ -255,18 +249,44 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
                     z64 y = zdiv(z64{x[q].x, x[q].y}, piv);
                     v[(long)kk * n + 3 * g + q] = make_double2(y.r, y.i);
                     nv2 += y.r * y.r + y.i * y.i;
+                    vvx += y.r * y.r - y.i * y.i;
+                    vvy += 2.0 * y.r * y.i;
                 }
         }
-        // ---- (3) residual column jk into u(:, kk); ||u_k||^2; arg max over unused rows
-        double nu2 = 0.0;
+        // ---- (3) residual column jk into u(:, kk); ||u_k||^2, u_k . u_k; arg max
+        // over unused rows; col . u_l for l < kk
+        double nu2 = 0.0, uux = 0.0, uuy = 0.0;
         double rv = -1.0;
         int ri = INT_MAX;
-        for (int g = t; g < gm; g += ACA_T) {
-            z64 e[3];
-            gen.col3(jb, jk, g, e);
-            z64 acc[3];
-            const int nq = min(3, m - 3 * g);
-            int used = dot3(coef, Ir, u + 3 * g, m, kk, 3 * g, nq, acc);
+        for (int g0 = 0; g0 < gm; g0 += ACA_T) {
+            const int g = g0 + t;
+            const bool ok = g < gm;
+            const int nq = ok ? min(3, m - 3 * g) : 0;
+            z64 e[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+            if (ok) gen.col3(jb, jk, g, e);
+            z64 acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+            int used = 0;
+#pragma unroll 2
+            for (int l = 0; l < kk; ++l) {
+                const double2 c = cv[l];
+                const int d = Ir[l] - 3 * g;
+                used |= (unsigned)d < 3u ? 1 << d : 0;
+                double px = 0.0, py = 0.0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double2 y = q < nq ? u[(long)l * m + 3 * g + q] : make_double2(0.0, 0.0);
+                    acc[q].r += c.x * y.x - c.y * y.y;
+                    acc[q].i += c.x * y.y + c.y * y.x;
+                    px += e[q].r * y.x - e[q].i * y.y;
+                    py += e[q].r * y.y + e[q].i * y.x;
+                }
+                px = wave_sum(px);
+                py = wave_sum(py);
+                if (lane == 0) {
+                    const double2 o = g0 ? part[w][l] : make_double2(0.0, 0.0);
+                    part[w][l] = make_double2(o.x + px, o.y + py);
+                }
+            }
             const int dk = Ir[kk] - 3 * g;   // the current pivot row counts as used too
             used |= (unsigned)dk < 3u ? 1 << dk : 0;
 #pragma unroll
@@ -276,70 +296,63 @@ __global__ __launch_bounds__(ACA_T) void k_aca(G gen, const AcaJob *__restrict__
                 z64 r = {e[q].r - acc[q].r, e[q].i - acc[q].i};
                 u[(long)kk * m + i] = make_double2(r.r, r.i);
                 nu2 += r.r * r.r + r.i * r.i;
+                uux += r.r * r.r - r.i * r.i;
+                uuy += 2.0 * r.r * r.i;
                 amax_merge(rv, ri, (used >> q) & 1 ? 0.0 : zabs(r), i);
             }
         }
         nu2 = wave_sum(nu2);
         nv2 = wave_sum(nv2);
+        uux = wave_sum(uux);
+        uuy = wave_sum(uuy);
+        vvx = wave_sum(vvx);
+        vvy = wave_sum(vvy);
         wave_amax(rv, ri);
         if (lane == 0) {
             sred[1][w] = nu2;
             sred[2][w] = nv2;
             sred[3][w] = rv;
             ired[1][w] = ri;
+            zred[0][w] = make_double2(uux, uuy);
+            zred[1][w] = make_double2(vvx, vvy);
         }
         __syncthreads();
-        // ---- (4) cross terms sum_l |u_l . u_k| |v_k . v_l| (non-conjugated).
-        // Wave w owns l = w, w + ACA_W, ...; it takes two of them per pass so
-        // that their eight wave reductions run interleaved.
-        for (int l0 = w; l0 < kk; l0 += 2 * ACA_W) {
-            const int l1 = l0 + ACA_W;
-            const bool two = l1 < kk;   // uniform across the wave
-            double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // du0, dv0, du1, dv1 (re, im)
-#pragma unroll 2
-            for (int i = lane; i < m; i += 64) {
-                const double2 b = u[(long)kk * m + i], a = u[(long)l0 * m + i];
-                x[0] += a.x * b.x - a.y * b.y;
-                x[1] += a.x * b.y + a.y * b.x;
-                if (two) {
-                    const double2 a1 = u[(long)l1 * m + i];
-                    x[4] += a1.x * b.x - a1.y * b.y;
-                    x[5] += a1.x * b.y + a1.y * b.x;
-                }
-            }
-#pragma unroll 2
-            for (int j = lane; j < n; j += 64) {
-                const double2 a = v[(long)kk * n + j], b = v[(long)l0 * n + j];
-                x[2] += a.x * b.x - a.y * b.y;
-                x[3] += a.x * b.y + a.y * b.x;
-                if (two) {
-                    const double2 b1 = v[(long)l1 * n + j];
-                    x[6] += a.x * b1.x - a.y * b1.y;
-                    x[7] += a.x * b1.y + a.y * b1.x;
-                }
-            }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-                for (int q = 0; q < 8; ++q) x[q] += __shfl_xor(x[q], o, 64);
-            if (lane == 0) {
-                dpart[l0] = make_double2(x[0], x[1]);
-                epart[l0] = make_double2(x[2], x[3]);
-                if (two) {
-                    dpart[l1] = make_double2(x[4], x[5]);
-                    epart[l1] = make_double2(x[6], x[7]);
-                }
-            }
-        }
-        __syncthreads();
+        // ---- (4) cross terms sum_l |u_l . u_k| |v_k . v_l| from the Gram rows
+        // (kk <= 63 < 64: wave 0 holds them all); the Gram matrices take row
+        // and column kk
         if (w == 0) {
             double term = 0.0;
             if (lane < kk) {
-                const double2 du = dpart[lane], dv = epart[lane];
-                term = zabs(z64{du.x, du.y}) * zabs(z64{dv.x, dv.y});
+                const int l = lane;
+                double2 a = part[0][l];
+                for (int q = 1; q < ACA_W; ++q) a = make_double2(a.x + part[q][l].x, a.y + part[q][l].y);
+                double2 b = brow[l];
+                const double2 *gu = Gu + (long)l * kmax, *gv = Gv + (long)l * kmax;
+                for (int l2 = 0; l2 < kk; ++l2) {
+                    const double2 x = gu[l2], y = gv[l2], c = cv[l2], d = cu[l2];
+                    a.x -= c.x * x.x - c.y * x.y;
+                    a.y -= c.x * x.y + c.y * x.x;
+                    b.x -= d.x * y.x - d.y * y.y;
+                    b.y -= d.x * y.y + d.y * y.x;
+                }
+                const z64 sv = zdiv(z64{b.x, b.y}, piv);
+                Gu[(long)kk * kmax + l] = a;
+                Gu[(long)l * kmax + kk] = a;
+                Gv[(long)kk * kmax + l] = make_double2(sv.r, sv.i);
+                Gv[(long)l * kmax + kk] = make_double2(sv.r, sv.i);
+                term = zabs(z64{a.x, a.y}) * zabs(sv);
             }
             term = wave_sum(term);
-            if (lane == 0) s_cross = term;
+            if (lane == 0) {
+                s_cross = term;
+                double2 su = zred[0][0], svv = zred[1][0];
+                for (int q = 1; q < ACA_W; ++q) {
+                    su = make_double2(su.x + zred[0][q].x, su.y + zred[0][q].y);
+                    svv = make_double2(svv.x + zred[1][q].x, svv.y + zred[1][q].y);
+                }
+                Gu[(long)kk * kmax + kk] = su;
+                Gv[(long)kk * kmax + kk] = svv;
+            }
         }
         __syncthreads();
         double snu = 0.0, snv = 0.0;
@@ -372,26 +385,28 @@ __global__ void k_vt_to_v(const double2 *Vt, int n, int kmax, double2 *V, long l
 }
 
 int launch_aca_cells(const DevCells &cells, bool ref32, double k0, double pi, const AcaJob *jobs, int njobs,
-                     double2 *U, double2 *Vt, int kmax, double eps, int *kout, hipStream_t st)
+                     double2 *U, double2 *Vt, int kmax, double eps, int *kout, double2 *gram, hipStream_t st)
 {
-    MIDAS_CHECK(kmax >= 1 && kmax <= ACA_KMAX, -1, "aca: kmax %d outside 1..%d", kmax, ACA_KMAX);
+    MIDAS_CHECK(kmax >= 1 && kmax <= ACA_KMAX && gram, -1, "aca: kmax %d outside 1..%d or no Gram buffer", kmax,
+                ACA_KMAX);
     if (njobs <= 0) return 0;
     if (ref32)
         hipLaunchKernelGGL(k_aca<GenCells<true>>, dim3(njobs), dim3(ACA_T), 0, st, GenCells<true>{cells, k0, pi},
-                           jobs, U, Vt, kmax, eps, kout);
+                           jobs, U, Vt, kmax, eps, kout, gram);
     else
         hipLaunchKernelGGL(k_aca<GenCells<false>>, dim3(njobs), dim3(ACA_T), 0, st, GenCells<false>{cells, k0, pi},
-                           jobs, U, Vt, kmax, eps, kout);
+                           jobs, U, Vt, kmax, eps, kout, gram);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
 
 int launch_aca_matrix(const double2 *A, long lda, const AcaJob *jobs, int njobs, double2 *U, double2 *Vt, int kmax,
-                      double eps, int *kout, hipStream_t st)
+                      double eps, int *kout, double2 *gram, hipStream_t st)
 {
-    MIDAS_CHECK(kmax >= 1 && kmax <= ACA_KMAX, -1, "aca: kmax %d outside 1..%d", kmax, ACA_KMAX);
+    MIDAS_CHECK(kmax >= 1 && kmax <= ACA_KMAX && gram, -1, "aca: kmax %d outside 1..%d or no Gram buffer", kmax,
+                ACA_KMAX);
     hipLaunchKernelGGL(k_aca<GenMatrix>, dim3(njobs), dim3(ACA_T), 0, st, GenMatrix{A, lda}, jobs, U, Vt, kmax, eps,
-                       kout);
+                       kout, gram);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
@@ -425,16 +440,17 @@ int midas_aca_block(midas_ctx *ctx, const midas_cells_t *cells, double k0, int q
     TRY(dc.from_host(cells, k0, quirks, st));
     const int m = 3 * nr, n = 3 * nc;
     AcaJob job{m, n, r0, c0, 0, 0};
-    DBuf dj, du, dvt, dv, dk;
+    DBuf dj, du, dvt, dv, dk, dg;
     TRY(upload(dj, &job, sizeof(job), st));
     TRY(du.alloc(sizeof(double2) * (size_t)m * kmax));
     TRY(dvt.alloc(sizeof(double2) * (size_t)n * kmax));
     TRY(dv.alloc(sizeof(double2) * (size_t)n * kmax));
     TRY(dk.alloc(sizeof(int)));
+    TRY(dg.alloc(aca_gram_bytes(kmax, 1)));
     MIDAS_HIP(hipMemsetAsync(du.p, 0, du.bytes, st));
     MIDAS_HIP(hipMemsetAsync(dvt.p, 0, dvt.bytes, st));
     TRY(launch_aca_cells(dc.v, (quirks & MIDAS_Q_FP32_BLOCKS) != 0, k0, host_pi(quirks), dj.as<AcaJob>(), 1,
-                         du.as<double2>(), dvt.as<double2>(), kmax, eps, dk.as<int>(), st));
+                         du.as<double2>(), dvt.as<double2>(), kmax, eps, dk.as<int>(), dg.as<double2>(), st));
     TRY(launch_vt_to_v(dvt.as<double2>(), n, kmax, dv.as<double2>(), kmax, st));
     MIDAS_HIP(hipMemcpyAsync(U, du.p, du.bytes, hipMemcpyDeviceToHost, st));
     MIDAS_HIP(hipMemcpyAsync(V, dv.p, dv.bytes, hipMemcpyDeviceToHost, st));
@@ -451,7 +467,8 @@ int midas_aca_matrix(midas_ctx *ctx, int m, int n, const double *A, long lda, in
     MIDAS_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->st;
     AcaJob job{m, n, 0, 0, 0, 0};
-    DBuf da, dj, du, dvt, dv, dk;
+    DBuf da, dj, du, dvt, dv, dk, dg;
+    TRY(dg.alloc(aca_gram_bytes(kmax, 1)));
     TRY(da.alloc(sizeof(double2) * (size_t)m * n));
     MIDAS_HIP(hipMemcpy2DAsync(da.p, sizeof(double2) * m, A, sizeof(double2) * lda, sizeof(double2) * m, n,
                                hipMemcpyHostToDevice, st));
@@ -463,7 +480,7 @@ int midas_aca_matrix(midas_ctx *ctx, int m, int n, const double *A, long lda, in
     MIDAS_HIP(hipMemsetAsync(du.p, 0, du.bytes, st));
     MIDAS_HIP(hipMemsetAsync(dvt.p, 0, dvt.bytes, st));
     TRY(launch_aca_matrix(da.as<double2>(), m, dj.as<AcaJob>(), 1, du.as<double2>(), dvt.as<double2>(), kmax, eps,
-                          dk.as<int>(), st));
+                          dk.as<int>(), dg.as<double2>(), st));
     TRY(launch_vt_to_v(dvt.as<double2>(), n, kmax, dv.as<double2>(), kmax, st));
     MIDAS_HIP(hipMemcpyAsync(U, du.p, du.bytes, hipMemcpyDeviceToHost, st));
     MIDAS_HIP(hipMemcpyAsync(V, dv.p, dv.bytes, hipMemcpyDeviceToHost, st));

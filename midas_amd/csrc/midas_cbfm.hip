@@ -908,7 +908,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         std::vector<std::pair<int, int>> dense;
         // reused across batches (allocating multi-GB buffers per batch costs
         // milliseconds each and serialises the stream)
-        DBuf Zd, Md, dj, Ub, Vb, dk, LR;
+        DBuf Zd, Md, dj, Ub, Vb, dk, LR, Gb;
         auto dense_pairs = [&]() -> int {
             const long zbudget = (long)dense_budget;
             size_t q0 = 0;
@@ -990,11 +990,13 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                 TRY(Ub.reserve(sizeof(double2) * std::max(uo, 1L)));
                 TRY(Vb.reserve(sizeof(double2) * std::max(vo, 1L)));
                 TRY(dk.reserve(sizeof(int) * ajobs.size()));
+                TRY(Gb.reserve(aca_gram_bytes(kmax, (int)ajobs.size())));
                 const size_t prof_at = profile().recs.size();
                 {
                     ProfScope ps(st, PROF_ACA, 0.0, 0.0);
                     TRY(launch_aca_cells(dc.v, ref32, k0, host_pi(quirks), dj.as<AcaJob>(), (int)ajobs.size(),
-                                         Ub.as<double2>(), Vb.as<double2>(), kmax, eps_aca, dk.as<int>(), st));
+                                         Ub.as<double2>(), Vb.as<double2>(), kmax, eps_aca, dk.as<int>(),
+                                         Gb.as<double2>(), st));
                 }
                 std::vector<int> hk(ajobs.size());
                 TRY(copy_sync(hk.data(), dk.p, sizeof(int) * hk.size(), hipMemcpyDeviceToHost, st));
