@@ -908,8 +908,8 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         std::vector<std::pair<int, int>> dense;
         // reused across batches (allocating multi-GB buffers per batch costs
         // milliseconds each and serialises the stream)
-        DBuf Zd, Md, dj, Ub, Vb, dk, LR, Gb;
-        auto dense_pairs = [&]() -> int {
+        DBuf Zd, Md;
+        auto dense_pairs = [&](hipStream_t sp) -> int {
             const long zbudget = (long)dense_budget;
             size_t q0 = 0;
             while (q0 < dense.size()) {
@@ -933,18 +933,18 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                     const int i = dense[q].first, j = dense[q].second;
                     double2 *Zq = Zd.as<double2>() + zo[q - q0];
                     TRY(launch_fill(dc.v, CellSet{nbk[i], bs[i], nullptr}, CellSet{nbk[j], bs[j], nullptr}, k0, quirks,
-                                    ref32, Zq, 3L * nbk[i], nullptr, 0, st));
+                                    ref32, Zq, 3L * nbk[i], nullptr, 0, sp));
                     add_gemm_tiles(jobs, Cb(i), 3 * nbk[i], 1, Zq, 3 * nbk[i], Md.as<double2>() + mo[q - q0], K[i],
                                    K[i], 3 * nbk[j], 3 * nbk[i]);
                 }
-                TRY(run_ggemm(jobs, djobs, st));
+                TRY(run_ggemm(jobs, djobs, sp));
                 jobs.clear();
                 for (size_t q = q0; q < q1; ++q) {
                     const int i = dense[q].first, j = dense[q].second;
                     add_gemm_tiles(jobs, Md.as<double2>() + mo[q - q0], K[i], 0, Cb(j), 3 * nbk[j],
                                    zc_ + offs[i] + offs[j] * Kt, (int)Kt, K[i], K[j], 3 * nbk[j]);
                 }
-                TRY(run_ggemm(jobs, djobs, st));
+                TRY(run_ggemm(jobs, djobs, sp));
                 q0 = q1;
             }
             dense.clear();
@@ -952,105 +952,167 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         };
         for (int j = 0; j < nb; ++j)
             if (K[j]) dense.push_back({j, j});
-        TRY(dense_pairs());
+        TRY(dense_pairs(st));
         // off-diagonal pairs by ACA, a batch of block columns per launch; then
-        // Zc_ij = (C_i^T U)(V C_j) through two grouped launches
+        // Zc_ij = (C_i^T U)(V C_j) through two grouped launches.  Two batches
+        // are in flight: the ACA of batch b runs on the context stream while
+        // the products of batch b - 1 run on a second stream (their buffers
+        // alternate), so the latency-bound ACA and the MFMA products overlap.
         const double budget = aca_budget;   // bytes of U, Vt and the small factors per batch
-        int j0 = 0;
-        while (j0 < nb) {
-            std::vector<AcaJob> ajobs;
-            std::vector<int> pi_, pj_;
+        struct AcaBatch {
+            std::vector<AcaJob> jobs;
+            std::vector<int> pi, pj;
             long uo = 0, vo = 0;
-            int j1 = j0;
+        };
+        std::vector<AcaBatch> batches;
+        for (int j0 = 0, j1; j0 < nb; j0 = j1) {
+            AcaBatch bt;
+            j1 = j0;
             while (j1 < nb) {
                 long add_u = 0, add_v = 0;
                 std::vector<AcaJob> col;
                 std::vector<int> ci;
                 for (int i = 0; i < nb; ++i) {
                     if (i == j1 || !K[i] || !K[j1]) continue;
-                    AcaJob jb{3 * nbk[i], 3 * nbk[j1], bs[i], bs[j1], uo + add_u, vo + add_v};
+                    AcaJob jb{3 * nbk[i], 3 * nbk[j1], bs[i], bs[j1], bt.uo + add_u, bt.vo + add_v};
                     add_u += 3L * nbk[i] * kmax;
                     add_v += 3L * nbk[j1] * kmax;
                     col.push_back(jb);
                     ci.push_back(i);
                 }
-                if (j1 > j0 && (double)(uo + add_u + vo + add_v) * sizeof(double2) > budget) break;
-                ajobs.insert(ajobs.end(), col.begin(), col.end());
+                if (j1 > j0 && (double)(bt.uo + add_u + bt.vo + add_v) * sizeof(double2) > budget) break;
+                bt.jobs.insert(bt.jobs.end(), col.begin(), col.end());
                 for (int i : ci) {
-                    pi_.push_back(i);
-                    pj_.push_back(j1);
+                    bt.pi.push_back(i);
+                    bt.pj.push_back(j1);
                 }
-                uo += add_u;
-                vo += add_v;
+                bt.uo += add_u;
+                bt.vo += add_v;
                 ++j1;
             }
-            if (!ajobs.empty()) {
-                TRY(dj.reserve(sizeof(AcaJob) * ajobs.size()));
-                MIDAS_HIP(hipMemcpyAsync(dj.p, ajobs.data(), sizeof(AcaJob) * ajobs.size(), hipMemcpyHostToDevice, st));
-                TRY(Ub.reserve(sizeof(double2) * std::max(uo, 1L)));
-                TRY(Vb.reserve(sizeof(double2) * std::max(vo, 1L)));
-                TRY(dk.reserve(sizeof(int) * ajobs.size()));
-                TRY(Gb.reserve(aca_gram_bytes(kmax, (int)ajobs.size())));
-                const size_t prof_at = profile().recs.size();
+            if (!bt.jobs.empty()) batches.push_back(std::move(bt));
+        }
+        // buffers of the two parities, sized once for the largest batch (a
+        // buffer that grew while a kernel of the other stream read it would be
+        // freed under it)
+        size_t nj_max = 1;
+        long uo_max = 1, vo_max = 1;
+        for (const AcaBatch &bt : batches) {
+            nj_max = std::max(nj_max, bt.jobs.size());
+            uo_max = std::max(uo_max, bt.uo);
+            vo_max = std::max(vo_max, bt.vo);
+        }
+        struct Pipe {
+            hipStream_t s2 = nullptr;
+            hipEvent_t ev[5] = {};   // start, rank[2], products[2]
+            int *hk[2] = {};         // pinned: the ranks of a batch
+            ~Pipe()
+            {
+                if (s2) hipStreamSynchronize(s2);
+                for (hipEvent_t e : ev)
+                    if (e) hipEventDestroy(e);
+                for (int *h : hk)
+                    if (h) hipHostFree(h);
+                if (s2) hipStreamDestroy(s2);
+            }
+        } pp;
+        DBuf dj[2], Ub[2], Vb[2], dk[2], Gb[2], LR[2], dgj[2];
+        if (!batches.empty()) {
+            MIDAS_HIP(hipStreamCreate(&pp.s2));
+            for (hipEvent_t &e : pp.ev) MIDAS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            for (int q = 0; q < 2; ++q) {
+                MIDAS_HIP(hipHostMalloc((void **)&pp.hk[q], sizeof(int) * nj_max));
+                TRY(dj[q].reserve(sizeof(AcaJob) * nj_max));
+                TRY(Ub[q].reserve(sizeof(double2) * uo_max));
+                TRY(Vb[q].reserve(sizeof(double2) * vo_max));
+                TRY(dk[q].reserve(sizeof(int) * nj_max));
+                TRY(Gb[q].reserve(aca_gram_bytes(kmax, (int)nj_max)));
+            }
+            // the second stream starts after the diagonal products
+            MIDAS_HIP(hipEventRecord(pp.ev[0], st));
+            MIDAS_HIP(hipStreamWaitEvent(pp.s2, pp.ev[0], 0));
+        }
+        size_t prof_at[2] = {0, 0};
+        // the products of batch b (parity q) on the second stream
+        auto products = [&](int b) -> int {
+            const int q = b & 1;
+            const AcaBatch &bt = batches[b];
+            MIDAS_HIP(hipEventSynchronize(pp.ev[1 + q]));
+            const int *hk = pp.hk[q];
+            if (profile().recs.size() > prof_at[q]) {
+                // algorithmic bytes of the launch, known once the ranks are:
+                // rank step s of an m x n pair reads the s - 1 previous
+                // columns of U and rows of V and writes the new pair,
+                // ~16 (m + n) (s + 2) bytes (complex FP64)
+                double by = 0.0;
+                for (size_t r = 0; r < bt.jobs.size(); ++r) {
+                    const double steps = std::min(hk[r], kmax);
+                    by += 16.0 * (bt.jobs[r].m + bt.jobs[r].n) * (steps * (steps + 1) / 2 + 2 * steps);
+                }
+                profile().recs[prof_at[q]].bytes = by;
+            }
+            // L_ij = C_i^T U (K_i x k) and R_ij = V C_j (k x K_j) in one scratch
+            std::vector<long> lo(bt.jobs.size()), ro(bt.jobs.size());
+            long lr = 0;
+            for (size_t r = 0; r < bt.jobs.size(); ++r) {
+                const int kk = std::min(hk[r], kmax);
+                if (kk >= kmax) continue;
+                lo[r] = lr;
+                lr += (long)K[bt.pi[r]] * kk;
+                ro[r] = lr;
+                lr += (long)kk * K[bt.pj[r]];
+            }
+            TRY(LR[q].reserve(sizeof(double2) * std::max(lr, 1L)));
+            jobs.clear();
+            for (size_t r = 0; r < bt.jobs.size(); ++r) {
+                const int i = bt.pi[r], j = bt.pj[r], kk = std::min(hk[r], kmax);
+                if (kk >= kmax) continue;
+                const double2 *U = Ub[q].as<double2>() + bt.jobs[r].u_off, *Vt = Vb[q].as<double2>() + bt.jobs[r].v_off;
+                add_gemm_tiles(jobs, Cb(i), 3 * nbk[i], 1, U, 3 * nbk[i], LR[q].as<double2>() + lo[r], K[i], K[i], kk,
+                               3 * nbk[i]);
+                add_gemm_tiles(jobs, Vt, 3 * nbk[j], 1, Cb(j), 3 * nbk[j], LR[q].as<double2>() + ro[r], kk, kk, K[j],
+                               3 * nbk[j]);
+            }
+            TRY(run_ggemm(jobs, dgj[q], pp.s2));
+            jobs.clear();
+            for (size_t r = 0; r < bt.jobs.size(); ++r) {
+                const int i = bt.pi[r], j = bt.pj[r], kk = std::min(hk[r], kmax);
+                if (kk >= kmax) continue;
+                add_gemm_tiles(jobs, LR[q].as<double2>() + lo[r], K[i], 0, LR[q].as<double2>() + ro[r], kk,
+                               zc_ + offs[i] + offs[j] * Kt, (int)Kt, K[i], K[j], kk);
+            }
+            TRY(run_ggemm(jobs, dgj[q], pp.s2));
+            // not rank-deficient enough: dense blocks (Compute_EFields_CBFME_ACA.f90:609-620)
+            for (size_t r = 0; r < bt.jobs.size(); ++r)
+                if (std::min(hk[r], kmax) >= kmax) dense.push_back({bt.pi[r], bt.pj[r]});
+            TRY(dense_pairs(pp.s2));
+            MIDAS_HIP(hipEventRecord(pp.ev[3 + q], pp.s2));
+            return 0;
+        };
+        for (size_t b = 0; b <= batches.size(); ++b) {
+            if (b < batches.size()) {
+                const int q = b & 1;
+                const AcaBatch &bt = batches[b];
+                // parity q's buffers were last read by the products of batch b - 2
+                if (b >= 2) MIDAS_HIP(hipStreamWaitEvent(st, pp.ev[3 + q], 0));
+                MIDAS_HIP(hipMemcpyAsync(dj[q].p, bt.jobs.data(), sizeof(AcaJob) * bt.jobs.size(),
+                                         hipMemcpyHostToDevice, st));
+                prof_at[q] = profile().recs.size();
                 {
                     ProfScope ps(st, PROF_ACA, 0.0, 0.0);
-                    TRY(launch_aca_cells(dc.v, ref32, k0, host_pi(quirks), dj.as<AcaJob>(), (int)ajobs.size(),
-                                         Ub.as<double2>(), Vb.as<double2>(), kmax, eps_aca, dk.as<int>(),
-                                         Gb.as<double2>(), st));
+                    TRY(launch_aca_cells(dc.v, ref32, k0, host_pi(quirks), dj[q].as<AcaJob>(), (int)bt.jobs.size(),
+                                         Ub[q].as<double2>(), Vb[q].as<double2>(), kmax, eps_aca, dk[q].as<int>(),
+                                         Gb[q].as<double2>(), st));
                 }
-                std::vector<int> hk(ajobs.size());
-                TRY(copy_sync(hk.data(), dk.p, sizeof(int) * hk.size(), hipMemcpyDeviceToHost, st));
-                if (profile().recs.size() > prof_at) {
-                    // algorithmic bytes of the launch, known once the ranks are:
-                    // rank step s of an m x n pair reads the s - 1 previous
-                    // columns of U and rows of V and writes the new pair,
-                    // ~16 (m + n) (s + 2) bytes (complex FP64)
-                    double by = 0.0;
-                    for (size_t q = 0; q < ajobs.size(); ++q) {
-                        const double steps = std::min(hk[q], kmax);
-                        by += 16.0 * (ajobs[q].m + ajobs[q].n) * (steps * (steps + 1) / 2 + 2 * steps);
-                    }
-                    profile().recs[prof_at].bytes = by;
-                }
-                // L_ij = C_i^T U (K_i x k) and R_ij = V C_j (k x K_j) in one scratch
-                std::vector<long> lo(ajobs.size()), ro(ajobs.size());
-                long lr = 0;
-                for (size_t q = 0; q < ajobs.size(); ++q) {
-                    const int kk = std::min(hk[q], kmax);
-                    if (kk >= kmax) continue;
-                    lo[q] = lr;
-                    lr += (long)K[pi_[q]] * kk;
-                    ro[q] = lr;
-                    lr += (long)kk * K[pj_[q]];
-                }
-                TRY(LR.reserve(sizeof(double2) * std::max(lr, 1L)));
-                jobs.clear();
-                for (size_t q = 0; q < ajobs.size(); ++q) {
-                    const int i = pi_[q], j = pj_[q], kk = std::min(hk[q], kmax);
-                    if (kk >= kmax) continue;
-                    const double2 *U = Ub.as<double2>() + ajobs[q].u_off, *Vt = Vb.as<double2>() + ajobs[q].v_off;
-                    add_gemm_tiles(jobs, Cb(i), 3 * nbk[i], 1, U, 3 * nbk[i], LR.as<double2>() + lo[q], K[i], K[i], kk,
-                                   3 * nbk[i]);
-                    add_gemm_tiles(jobs, Vt, 3 * nbk[j], 1, Cb(j), 3 * nbk[j], LR.as<double2>() + ro[q], kk, kk, K[j],
-                                   3 * nbk[j]);
-                }
-                TRY(run_ggemm(jobs, djobs, st));
-                jobs.clear();
-                for (size_t q = 0; q < ajobs.size(); ++q) {
-                    const int i = pi_[q], j = pj_[q], kk = std::min(hk[q], kmax);
-                    if (kk >= kmax) continue;
-                    add_gemm_tiles(jobs, LR.as<double2>() + lo[q], K[i], 0, LR.as<double2>() + ro[q], kk,
-                                   zc_ + offs[i] + offs[j] * Kt, (int)Kt, K[i], K[j], kk);
-                }
-                TRY(run_ggemm(jobs, djobs, st));
-                // not rank-deficient enough: dense blocks (Compute_EFields_CBFME_ACA.f90:609-620)
-                for (size_t q = 0; q < ajobs.size(); ++q)
-                    if (std::min(hk[q], kmax) >= kmax) dense.push_back({pi_[q], pj_[q]});
-                TRY(dense_pairs());
-                // the next batch overwrites U, V, L, R and the job list
-                MIDAS_HIP(hipStreamSynchronize(st));
+                MIDAS_HIP(hipMemcpyAsync(pp.hk[q], dk[q].p, sizeof(int) * bt.jobs.size(), hipMemcpyDeviceToHost, st));
+                MIDAS_HIP(hipEventRecord(pp.ev[1 + q], st));
             }
-            j0 = j1;
+            if (b >= 1) TRY(products((int)b - 1));
+        }
+        if (!batches.empty()) {
+            // the reduced matrix is complete when the second stream is
+            MIDAS_HIP(hipEventRecord(pp.ev[0], pp.s2));
+            MIDAS_HIP(hipStreamWaitEvent(st, pp.ev[0], 0));
         }
     }
     phase(use_aca ? "reduced matrix (ACA)" : "reduced matrix (dense)");
