@@ -50,6 +50,7 @@ COMMENTS = {
     "zgemm_tile": "tile shape of the FP64 ZGEMM (0: 128x128 ... 4: 64x32)",
     "fp64_gemm_library": "1: rocBLAS for the FP64 LU updates and many-RHS residuals",
     "cbfm_block_mode": "CBF block solves: 0 FP64 LU, 1 c32 / 2 f16 / 4 bf16 LU + FP64 refinement, 5 by size",
+    "cgemm_tile": "tile shape of the complex-FP32 GEMM without transposes (5: the 64x64 k_cgemm)",
 }
 
 

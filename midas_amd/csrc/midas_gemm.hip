@@ -173,11 +173,16 @@ int launch_cgemm(int m, int n, int k, cplx<T> alpha, const void *A, long lda, in
 {
     typedef typename cvec<T>::t V;
     if (m <= 0 || n <= 0 || batch <= 0) return 0;
-    // complex FP64 without transposes: the faster k_zgemm_nn
-    if constexpr (sizeof(T) == 8)
-        if (tA == 0 && tB == 0)
+    // without transposes: the faster k_zgemm_nn kernels (FP64, and FP32 unless
+    // cgemm_tile selects this kernel)
+    if (tA == 0 && tB == 0) {
+        if constexpr (sizeof(T) == 8)
             return launch_zgemm_nn(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, st,
                                    tune(TUNE_ZGEMM_TILE));
+        else if (tune(TUNE_CGEMM_TILE) < 5)
+            return launch_cgemm_nn(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, st,
+                                   tune(TUNE_CGEMM_TILE));
+    }
     MIDAS_CHECK((n + GB_N - 1) / GB_N <= 65535, -1, "cgemm: n too large for one launch");
     if (k <= 0) {
         // C = beta C only: run with k=0 (loop body skipped)

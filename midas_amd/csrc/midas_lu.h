@@ -93,6 +93,9 @@ int blas_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, long 
 // 64 x 32 waves, 2: 64 x 64, 3: 128 x 64 of 32 x 32 waves, 4: 64 x 32).
 int launch_zgemm_nn(int m, int n, int k, z64 alpha, const void *A, long lda, long sA, const void *B, long ldb, long sB,
                     z64 beta, void *C, long ldc, long sC, int batch, hipStream_t st, int shape);
+// The same kernel in complex FP32 (v_mfma_f32_16x16x4_f32).
+int launch_cgemm_nn(int m, int n, int k, c32 alpha, const void *A, long lda, long sA, const void *B, long ldb, long sB,
+                    c32 beta, void *C, long ldc, long sC, int batch, hipStream_t st, int shape);
 
 // Eigen-decomposition of a strided batch of Hermitian n x n matrices G
 // (midas_heev.hip): eigenvalues ascending into D (stride dstride), the

@@ -14,6 +14,7 @@ enum {
     TUNE_ZGEMM_TILE,
     TUNE_FP64_GEMM_LIBRARY,
     TUNE_CBFM_BLOCK_MODE,
+    TUNE_CGEMM_TILE,
     TUNE_COUNT
 };
 int tune(int which);

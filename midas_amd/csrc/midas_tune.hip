@@ -24,6 +24,7 @@ static TuneEntry g_tune[TUNE_COUNT] = {
     {"zgemm_tile", MIDAS_TUNED_ZGEMM_TILE, 0, 4},
     {"fp64_gemm_library", MIDAS_TUNED_FP64_GEMM_LIBRARY, 0, 1},
     {"cbfm_block_mode", MIDAS_TUNED_CBFM_BLOCK_MODE, 0, 5},
+    {"cgemm_tile", MIDAS_TUNED_CGEMM_TILE, 0, 5},
 };
 
 int tune(int which) { return g_tune[which].value; }

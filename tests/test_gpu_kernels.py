@@ -113,6 +113,32 @@ def test_zgemm_nn(mid, zgemm_tile, shape):
                                                                      + np.abs(C0).max())
 
 
+@pytest.fixture(params=[0, 1, 2, 3, 4, 5])
+def cgemm_tile(request):
+    old = api.tune_set("cgemm_tile", request.param)
+    yield request.param
+    api.tune_set("cgemm_tile", old)
+
+
+@pytest.mark.parametrize("shape", [(128, 128, 8), (100, 37, 53), (1, 5, 200), (257, 300, 129), (129, 131, 7)])
+def test_cgemm_nn(mid, cgemm_tile, shape):
+    """The complex-FP32 products without transposes in every tile shape of the
+    FP32 kernel (5: the 64 x 64 k_cgemm): partial tiles in m, n and k, against
+    numpy on the operands rounded to complex64, to FP32 rounding."""
+    m, n, k = shape
+    print(f"\n[case] cgemm_nn tile {cgemm_tile} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(5 * m + n + k)
+    cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    A, B, C0 = cr(m, k), cr(k, n), cr(m, n)
+    alpha, beta = 0.7 - 0.2j, -1.1 + 0.4j
+    got = _gemm(mid, 32, A, B, C0, alpha, beta, 0, 0)
+    r = lambda X: X.astype(np.complex64).astype(np.complex128)
+    ref = alpha * (r(A) @ r(B)) + beta * r(C0)
+    err = np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * k + np.abs(C0).max())
+    print("max scaled err", err)
+    assert err < 3e-7
+
+
 @pytest.mark.parametrize("shape", [(200, 150, 40), (128, 256, 300)])
 def test_zgemm_nn_batched(mid, zgemm_tile, shape):
     """A strided batch of three products (the CBFM block LUs' trailing updates)."""
