@@ -37,7 +37,9 @@ int midas_test_pchol(midas_ctx *ctx, int n, int batch, const double *G, double *
 
 /* C -= A B with the split FP16/BF16 trailing-update kernel (mode as
  * TrailingMode: 1 f16, 3 f16x3, 11 bf16, 13 bf16x3); operands complex128,
- * rounded to complex FP32 before the call. */
+ * rounded to complex FP32 before the call.  A (lda x k) and C (ldc x n) keep
+ * their leading dimensions on the device: the kernel works on the m-row
+ * blocks inside them, and rows m .. ld-1 come back rounded to FP32. */
 int midas_test_trailing(midas_ctx *ctx, int mode, int m, int n, int k, const double *A, long lda, const double *B,
                         long ldb, double *C, long ldc);
 

@@ -1250,18 +1250,20 @@ int midas_test_trailing(midas_ctx *ctx, int mode, int m, int n, int k, const dou
     MIDAS_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->st;
     DBuf dA, dB, dC, a32, b32, c32, sc;
-    TRY(upload_mat(dA, A, m, k, lda, st));
+    MIDAS_CHECK(lda >= m && ldc >= m && ldb >= k, -1, "test_trailing: leading dimensions below the block");
+    // A and C keep their leading dimensions on the device (the rows past m
+    // travel along, rounded to FP32), so the kernel sees lda, ldc > m
+    TRY(upload_mat(dA, A, (int)lda, k, lda, st));
     TRY(upload_mat(dB, B, k, n, ldb, st));
-    TRY(upload_mat(dC, C, m, n, ldc, st));
-    TRY(to_c64(a32, dA, (long)m * k, st));
+    TRY(upload_mat(dC, C, (int)ldc, n, ldc, st));
+    TRY(to_c64(a32, dA, lda * k, st));
     TRY(to_c64(b32, dB, (long)k * n, st));
-    TRY(to_c64(c32, dC, (long)m * n, st));
+    TRY(to_c64(c32, dC, ldc * n, st));
     TRY(sc.alloc(trail16_scratch_bytes(m, n, k)));
-    TRY(launch_trail16(mode, m, n, k, a32.as<float2>(), m, b32.as<float2>(), k, c32.as<float2>(), m, sc.p, st));
-    hipLaunchKernelGGL(k_c2z, dim3(((long)m * n + 255) / 256), dim3(256), 0, st, c32.as<float2>(), dC.as<double2>(),
-                       (long)m * n);
-    MIDAS_HIP(hipMemcpy2DAsync(C, sizeof(double2) * ldc, dC.p, sizeof(double2) * m, sizeof(double2) * m, n,
-                               hipMemcpyDeviceToHost, st));
+    TRY(launch_trail16(mode, m, n, k, a32.as<float2>(), lda, b32.as<float2>(), k, c32.as<float2>(), ldc, sc.p, st));
+    hipLaunchKernelGGL(k_c2z, dim3((ldc * n + 255) / 256), dim3(256), 0, st, c32.as<float2>(), dC.as<double2>(),
+                       ldc * n);
+    MIDAS_HIP(hipMemcpyAsync(C, dC.p, sizeof(double2) * ldc * n, hipMemcpyDeviceToHost, st));
     MIDAS_HIP(hipStreamSynchronize(st));
     return 0;
 }

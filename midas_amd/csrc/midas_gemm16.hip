@@ -392,20 +392,30 @@ __global__ __launch_bounds__(256, 1) void k_trail16(int m, int n, int kst, int m
 // waits of the main loop leave the matrix cores idle; a second resident
 // workgroup fills those gaps.  To fit two: the real part accumulates
 // Br Ar + (-Bi) Ai in one set (B carries a negated imaginary plane), so a wave
-// holds 2 x 4 accumulator tiles; stages are 16 k deep in a ring of three 20 KB
-// slots (48 KB per workgroup); registers stay within 224 per lane, which
-// leaves room on each SIMD for a 64-register wave of the LU panel kernel.
-// A planes: Ar, Ai.  B planes: Br, Bi; the -Bi fragment is made in registers
-// (a sign-bit flip of each half), which costs two VALU operations per
-// fragment instead of a fifth plane through L2 and LDS.
+// holds 2 x 4 accumulator tiles; the -Bi fragment is made in registers (a
+// sign-bit flip of each half) rather than read as a fifth plane.
+//   * Stages are 16 k deep in a ring of four 16 KB slots (64 KB per
+//     workgroup).  At step s the wave holds stage s's fragments in registers
+//     (read during step s-1), reads stage s+1's from LDS, and the slot that
+//     held stage s is refilled with stage s+4: the LDS reads overlap the
+//     matrix-core work instead of preceding it.
+//   * LDS rows are 32 B (two 16-byte halves).  The half holding logical half c
+//     of row r sits at c ^ x(r), x(r) = bit 2 ^ bit 3 of r, so the 16 lanes of
+//     a fragment read that share a bank cycle land on distinct banks (the
+//     unswizzled layout measured 2-way conflicts on half of the LDS cycles).
+//     The copies to LDS keep their fixed destinations (M0 + 16 lane); each
+//     lane fetches the global half that belongs at its destination.
+//   * The epilogue of a tile that lies fully inside C uses buffer loads and
+//     stores (a scalar descriptor per 32 x 32 block, a scalar offset per
+//     column, one 32-bit lane offset), so all loads of a block are in flight
+//     together and nothing spills; per-element bounds checks (tiles on the
+//     edge of C) made every access a branch and serialised the waits.
 // ---------------------------------------------------------------------------
 struct Stage2 {
     u16 a[2][T_M][T_K];
     u16 b[2][T_N][T_K];
 };
-__shared__ __attribute__((aligned(16))) unsigned char g2_slot0[sizeof(Stage2)];
-__shared__ __attribute__((aligned(16))) unsigned char g2_slot1[sizeof(Stage2)];
-__shared__ __attribute__((aligned(16))) unsigned char g2_slot2[sizeof(Stage2)];
+constexpr int TRAIL_NS = 4;   // ring slots; the stage count is padded to a multiple of 4
 
 template <bool BF>
 __global__ __launch_bounds__(256, 2) void k_trail16x(int m, int n, int kst, int mt, int nt, int group,
@@ -420,10 +430,12 @@ __global__ __launch_bounds__(256, 2) void k_trail16x(int m, int n, int kst, int 
     bscale += blockIdx.y * sScale;
     C += blockIdx.y * sC;
     constexpr int IPS = 4;   // copy instructions per wave per stage (one 1 KB part of each of 4 plane tiles)
+    constexpr int NS = TRAIL_NS;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
     __shared__ float sinv[T_N];
+    __shared__ __attribute__((aligned(16))) Stage2 ring[NS];
 
     // tile order as in k_trail16
     const int total = mt * nt;
@@ -452,70 +464,145 @@ __global__ __launch_bounds__(256, 2) void k_trail16x(int m, int n, int kst, int 
                 acci[a][b][q] = 0.f;
             }
 
-    // copies straight to LDS, as in k_trail16 (one asm statement sets M0)
+    // copies straight to LDS, as in k_trail16 (one asm statement sets M0).
+    // Lane l writes bytes [16 l, 16 l + 16) of the wave's 1 KB part: row
+    // srow, physical half l & 1, which holds logical half (l & 1) ^ x(srow).
     typedef __attribute__((address_space(3))) void lds_void;
     auto copy16 = [&](const u16 *gsrc, u16 *ldst) {
         const unsigned lds = (unsigned)(size_t)(lds_void *)ldst;
         asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(gsrc), "s"(lds) : "memory");
     };
+    const int srow = wave * 32 + (lane >> 1);
+    const int soff = (lane >> 1) * T_K + (((lane & 1) ^ ((srow >> 2) ^ (srow >> 3))) & 1) * 8;
     auto dma = [&](Stage2 *dst, int ks) {
 #pragma unroll
         for (int pc = 0; pc < 2; ++pc)
-            copy16(Ap + ((((long)pc * mt + tm) * kst + ks) * T_M) * T_K + wave * 512 + lane * 8,
+            copy16(Ap + ((((long)pc * mt + tm) * kst + ks) * T_M) * T_K + wave * 512 + soff,
                    &dst->a[pc][0][0] + wave * 512);
 #pragma unroll
         for (int pc = 0; pc < 2; ++pc)
-            copy16(Bp + ((((long)pc * nt + tn) * kst + ks) * T_N) * T_K + wave * 512 + lane * 8,
+            copy16(Bp + ((((long)pc * nt + tn) * kst + ks) * T_N) * T_K + wave * 512 + soff,
                    &dst->b[pc][0][0] + wave * 512);
     };
-    auto compute = [&](const Stage2 *src) {
-        const int r32 = lane & 31, kc = (lane >> 5) * 8;
-        s8 fb[3][2];
+    // a stage's fragments: lane l reads row l % 32 of each 32-row tile,
+    // logical half l / 32 (physical half (l / 32) ^ x(row))
+    struct Frag {
+        s8 b[2][2];   // [plane][column tile]
+        s8 a[2][2];   // [row tile][plane]
+    };
+    auto fetch = [&](const Stage2 *src, Frag &f) {
+        const int r32 = lane & 31, kc = (((lane >> 5) ^ ((r32 >> 2) ^ (r32 >> 3))) & 1) * 8;
 #pragma unroll
         for (int pc = 0; pc < 2; ++pc)
 #pragma unroll
-            for (int t = 0; t < 2; ++t) fb[pc][t] = *(const s8 *)&src->b[pc][wn + 32 * t + r32][kc];
+            for (int t = 0; t < 2; ++t) f.b[pc][t] = *(const s8 *)&src->b[pc][wn + 32 * t + r32][kc];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int pc = 0; pc < 2; ++pc) f.a[a][pc] = *(const s8 *)&src->a[pc][wm + 32 * a + r32][kc];
+    };
+    auto compute = [&](const Frag &f) {
         // -Bi: the sign bit of every half flipped
         typedef unsigned u4v __attribute__((ext_vector_type(4)));
+        s8 nb[2];
 #pragma unroll
         for (int t = 0; t < 2; ++t)
-            fb[2][t] = __builtin_bit_cast(s8, __builtin_bit_cast(u4v, fb[1][t]) ^
-                                                  (u4v){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u});
+            nb[t] = __builtin_bit_cast(s8, __builtin_bit_cast(u4v, f.b[1][t]) ^
+                                               (u4v){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u});
+        // C^T: Dr = Br^T Ar^T + (-Bi)^T Ai^T, Di = Br^T Ai^T + Bi^T Ar^T
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            s8 fa[2];
-#pragma unroll
-            for (int pc = 0; pc < 2; ++pc) fa[pc] = *(const s8 *)&src->a[pc][wm + 32 * a + r32][kc];
-            // C^T: Dr = Br^T Ar^T + (-Bi)^T Ai^T, Di = Br^T Ai^T + Bi^T Ar^T
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                accr[a][b] = mma32<BF>(fb[0][b], fa[0], accr[a][b]);
-                acci[a][b] = mma32<BF>(fb[0][b], fa[1], acci[a][b]);
+                accr[a][b] = mma32<BF>(f.b[0][b], f.a[a][0], accr[a][b]);
+                acci[a][b] = mma32<BF>(f.b[0][b], f.a[a][1], acci[a][b]);
             }
 #pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
             for (int b = 0; b < 2; ++b) {
-                accr[a][b] = mma32<BF>(fb[2][b], fa[1], accr[a][b]);
-                acci[a][b] = mma32<BF>(fb[1][b], fa[0], acci[a][b]);
+                accr[a][b] = mma32<BF>(nb[b], f.a[a][1], accr[a][b]);
+                acci[a][b] = mma32<BF>(f.b[1][b], f.a[a][0], acci[a][b]);
             }
-        }
     };
-    // ring of three: at step s, stage s+1 is in flight; refill the slot of
-    // stage s-1 (computed by every wave before the barrier) with stage s+2
-    auto step = [&](int s, const Stage2 *cur, Stage2 *refill) {
-        if (s + 1 < kst) MIDAS_WAIT_VM(IPS);
+
+    // prologue: stages 0 .. NS-1 in flight; stage 0 landed, its fragments read
+    Frag F[2];
+#pragma unroll
+    for (int u = 0; u < NS; ++u) dma(ring + u, u);
+    MIDAS_WAIT_VM(IPS * (NS - 1));
+    asm volatile("s_barrier" ::: "memory");
+    fetch(ring, F[0]);
+    // Step s: wait for this wave's copies of stage s+1; then, with every
+    // wave's LDS reads of stage s complete (lgkmcnt(0) before the plain
+    // barrier), refill stage s's slot with stage s+NS, read stage s+1's
+    // fragments (the last step reads a stale slot, unused) and run stage s.
+    auto step = [&](int s, int fc, int snext, int sfree) {
+        const int later = min(kst - 1, s + NS - 1) - (s + 1);   // copies issued after stage s+1's
+        if (later >= 2) MIDAS_WAIT_VM(IPS * 2);
+        else if (later == 1) MIDAS_WAIT_VM(IPS);
         else MIDAS_WAIT_VM(0);
-        asm volatile("s_barrier" ::: "memory");
-        if (s + 2 < kst) dma(refill, s + 2);
-        compute(cur);
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        if (s + NS < kst) dma(ring + sfree, s + NS);
+        fetch(ring + snext, F[fc ^ 1]);
+        // keep the fragment reads ahead of the matrix-core work
+        __builtin_amdgcn_sched_barrier(0);
+        compute(F[fc]);
     };
-    Stage2 *s0 = (Stage2 *)g2_slot0, *s1 = (Stage2 *)g2_slot1, *s2 = (Stage2 *)g2_slot2;
+    for (int s = 0; s < kst; s += NS) {
+#pragma unroll
+        for (int u = 0; u < NS; ++u) step(s + u, u & 1, (u + 1) % NS, u);
+    }
 
     // Epilogue: four 32 x 32 blocks e (a = e & 1, b = e >> 1).  Lane holds
     // column lane%32 of D (a row of C); register q holds D row
-    // 8*(q/4) + 4*(lane/32) + q%4 (a column of C).  The column's q part is
-    // uniform (scalar bases), the lane part a 32-bit offset.  C is streamed
-    // non-temporally; two blocks are in flight: each block's loads are issued
-    // before the previous block is written.
+    // 8*(q/4) + 4*(lane/32) + q%4 (a column of C).  C is streamed; two blocks
+    // are in flight: each block's loads are issued before the previous block
+    // is written.
+    if (m0 + T_M <= m && n0 + T_N <= n) {
+        const unsigned lob = (unsigned)((lane & 31) * 8) + (unsigned)(lane >> 5) * (unsigned)(4 * ldc * 8);
+        const int nrec = (int)(32 * ldc * 8 + 256);
+        auto rsrc = [&](int e) {
+            return __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(C + (long)(n0 + wn + 32 * (e >> 1)) * ldc + m0 + wm + 32 * (e & 1)), 0, nrec, 0x00020000);
+        };
+        typedef decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc(0), 0, 0, 0)) b64t;
+        auto fload = [&](int e, double *cv) {
+            const auto r = rsrc(e);
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                cv[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                                       r, lob, (8 * (q >> 2) + (q & 3)) * (int)ldc * 8, 0));
+        };
+        auto fstore = [&](int e, const double *cv, const f16v &dr, const f16v &di) {
+            const auto r = rsrc(e);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const float inv = sinv[wn + 32 * (e >> 1) + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
+                const float2 c0 = __builtin_bit_cast(float2, cv[q]);
+                const float2 c1 = float2{c0.x - dr[q] * inv, c0.y - di[q] * inv};
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(b64t, c1), r, lob,
+                                                      (8 * (q >> 2) + (q & 3)) * (int)ldc * 8, 0);
+            }
+        };
+        double cv0[16], cv1[16];
+        fload(0, cv0);
+        fload(1, cv1);
+        __builtin_amdgcn_sched_barrier(0);
+        fstore(0, cv0, accr[0][0], acci[0][0]);
+        __builtin_amdgcn_sched_barrier(0);
+        fload(2, cv0);
+        __builtin_amdgcn_sched_barrier(0);
+        fstore(1, cv1, accr[1][0], acci[1][0]);
+        __builtin_amdgcn_sched_barrier(0);
+        fload(3, cv1);
+        __builtin_amdgcn_sched_barrier(0);
+        fstore(2, cv0, accr[0][1], acci[0][1]);
+        __builtin_amdgcn_sched_barrier(0);
+        fstore(3, cv1, accr[1][1], acci[1][1]);
+        return;
+    }
+    // a tile on the edge of C: bounds-checked accesses
     const int loff = (lane & 31) + 4 * (lane >> 5) * (int)ldc;
     auto cload = [&](int e, double *cv) {
         const int row = m0 + wm + 32 * (e & 1) + (lane & 31);
@@ -542,14 +629,6 @@ __global__ __launch_bounds__(256, 2) void k_trail16x(int m, int n, int kst, int 
             }
         }
     };
-
-    dma(s0, 0);
-    if (kst > 1) dma(s1, 1);
-    for (int s = 0; s < kst; s += 3) {
-        step(s, s0, s2);
-        if (s + 1 < kst) step(s + 1, s1, s0);
-        if (s + 2 < kst) step(s + 2, s2, s1);
-    }
     double cv0[16], cv1[16];
     cload(0, cv0);
     cload(1, cv1);
@@ -592,7 +671,8 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
         const char *e = getenv("MIDAS_TRAIL_1WG");
         if (!(e && atoi(e) != 0) || batch > 1) {
             MIDAS_CHECK(k <= SPLIT_B_KMAX, -1, "trail16: k = %d above %d", k, SPLIT_B_KMAX);
-            const int ks = (k + T_K - 1) / T_K;   // 16-k stages, no ring padding
+            // 16-k stages, padded to a multiple of the ring's slots (zero planes)
+            const int ks = ((k + T_K - 1) / T_K + TRAIL_NS - 1) / TRAIL_NS * TRAIL_NS;
             u16 *Bq = Ap + (size_t)2 * mt * T_M * ks * T_K;
             float *sq = (float *)(Bq + (size_t)2 * nt * T_N * ks * T_K);
             // per matrix of a batch: planes (A's, then B's, then the scales)

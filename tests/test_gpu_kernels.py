@@ -207,6 +207,31 @@ def test_trailing_split_kernels(mid, mode, tol, shape, one_wg, monkeypatch):
     assert err < tol
 
 
+@pytest.mark.parametrize("mode,tol", [(1, 4e-3), (11, 3e-2)])
+def test_trailing_split_in_larger_matrix(mid, mode, tol):
+    """Single-term split update of a block inside larger arrays (lda, ldc
+    above m), k = 1024 as in the two-panel LU steps, full and edge tiles: the
+    rows of C below the block and the columns right of it stay untouched."""
+    m, n, k, ld, ncol = 520, 390, 1024, 611, 400
+    print(f"\n[case] trailing split in a larger matrix mode={mode}", flush=True)
+    rng = np.random.default_rng(7 + mode)
+    cr = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    f32 = lambda x: x.astype(np.complex64).astype(complex)   # FP32-exact: the round trip keeps them
+    Abig = np.asfortranarray(f32(cr(ld, k) / 3.0))
+    B = np.asfortranarray(f32(cr(k, n) * np.exp(rng.normal(size=(1, n)))))
+    Cbig = np.asfortranarray(f32(cr(ld, ncol)))
+    C0 = Cbig.copy()
+    r = mid.lib.midas_test_trailing(mid.ctx, C.c_int(mode), C.c_int(m), C.c_int(n), C.c_int(k), _p(Abig), C.c_long(ld),
+                                    _p(B), C.c_long(k), _p(Cbig), C.c_long(ld))
+    assert r == 0
+    A = Abig[:m]
+    ref = C0[:m, :n] - A @ B
+    colscale = (np.abs(A).max() * np.abs(B).max(0) * k)[None, :]
+    err = (np.abs(Cbig[:m, :n] - ref) / (colscale + np.abs(C0).max() * 1e-7)).max()
+    print("scaled err", err)
+    assert err < tol
+    assert np.array_equal(Cbig[m:, :], C0[m:, :]) and np.array_equal(Cbig[:, n:], C0[:, n:])
+
 
 def _heev(mid, Gs, cut=0.0, path=0):
     b, n, _ = Gs.shape
