@@ -465,7 +465,15 @@ def main():
                          "c5: CBFM-E + ACA on a 150,000-cell porous particle")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device to device (calibrates HBM byte counters under rocprofv3 --pmc)")
+    ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
+                    help="set a library schedule parameter (midas_tune_set) for the run; repeatable")
     args = ap.parse_args()
+    if args.tune:
+        import torch  # noqa: F401  (the GPU runtime torch brings is the one the library must bind to)
+        from midas_amd import api
+    for kv in args.tune:
+        name, _, value = kv.partition("=")
+        api.tune_set(name, int(value))
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
