@@ -108,7 +108,11 @@ void *midas_stream(midas_ctx *ctx);           /* hipStream_t of the context */
  * autotuner, midas_amd/autotune.py): "lu_nb_split", "lu_two_panel_min_n",
  * "lu_nb_mixed", "lu_nb_fp64" (LU panel widths), "panel_leaf" (recursive panel
  * leaves), "trail_group_kib" (L2 share of a row-tile group in the split GEMM),
- * "refine_graph" (1: refinement sweeps replayed from a captured hipGraph).
+ * "refine_graph" (1: refinement sweeps replayed from a captured hipGraph),
+ * "cbfm_graph" (1: each batch of CBF block systems -- grouped fills, SR masks,
+ * excitations, batched FP64 LU and solve -- recorded into a hipGraph and
+ * launched as one, the executable graph kept and updated in place by the next
+ * call), "zgemm_tile", "cgemm_tile", "fp64_gemm_library", "cbfm_block_mode".
  * They change speed, not results beyond rounding.  No reference counterpart
  * (the reference fixes its 32 x 32 ScaLAPACK blocks, Initialization.f90:66-67). */
 int midas_tune_set(const char *name, int value);

@@ -51,6 +51,7 @@ COMMENTS = {
     "fp64_gemm_library": "1: rocBLAS for the FP64 LU updates and many-RHS residuals",
     "cbfm_block_mode": "CBF block solves: 0 FP64 LU, 1 c32 / 2 f16 / 4 bf16 LU + FP64 refinement, 5 by size",
     "cgemm_tile": "tile shape of the complex-FP32 GEMM without transposes (5: the 64x64 k_cgemm)",
+    "cbfm_graph": "CBFM block batches (fills, excitations, FP64 LU, solve) launched as one hipGraph each",
 }
 
 

@@ -519,6 +519,8 @@ int midas_finalize(midas_ctx *ctx)
     hipStreamSynchronize(ctx->st);
     ctx->ws.release();
     ctx->nws.release();
+    for (hipGraphExec_t g : ctx->cbfm_graphs)
+        if (g) hipGraphExecDestroy(g);
     hipStreamDestroy(ctx->st);
     for (hipEvent_t ev : ctx->nev)
         if (ev) hipEventDestroy(ev);

@@ -489,30 +489,101 @@ __global__ __launch_bounds__(64) void k_sr_stats(DevCells cells, CellSet S, doub
     }
 }
 
-// Unit diagonal on the padding rows/columns [m, mp) of an mp x mp matrix.
-__global__ void k_unit_diag(double2 *Z, long ld, int m, int mp)
-{
-    const int i = m + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < mp) Z[(long)i * ld + i] = make_double2(1.0, 0.0);
-}
-
 // Zero the entries of a filled (m x m) block that the SR mask drops: off the
 // self pairs, |g| (single precision) <= |1 - Znnpp(first cell)| / fsr.
-__global__ void k_sr_mask(double2 *Z, long ld, int m, DevCells cells, CellSet S, double fsr)
+__device__ __forceinline__ void sr_mask_entry(double2 *Z, long ld, int i, int j, const DevCells &cells, int g0,
+                                              double fsr)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (i >= m || i / 3 == j / 3) return;
-    const int g0 = S.cell(0);
     const double thr = hypot(1.0 - cells.znnpp[2 * g0], -cells.znnpp[2 * g0 + 1]) / fsr;
     double2 *p = Z + (long)j * ld + i;
     const double2 v = *p;
     if (!((double)hypotf((float)v.x, (float)v.y) > thr)) *p = make_double2(0.0, 0.0);
 }
+// Every matrix of a grouped batch (blockIdx.z), each with its own fSR.
+__global__ void k_sr_mask_group(const BlockJob *__restrict__ jobs, const int *__restrict__ idx, DevCells cells,
+                                double2 *Z, long ld, long zstride)
+{
+    const BlockJob jb = jobs[blockIdx.z];
+    const int m = 3 * jb.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= m || j >= m || i / 3 == j / 3) return;
+    sr_mask_entry(Z + blockIdx.z * zstride, ld, i, j, cells, idx[jb.off], jb.fsr);
+}
+
+// Padding of a grouped batch of block systems in mp x mp slots (matrix q holds
+// m = 3 jobs[q].n rows): zeros and a unit diagonal outside the block in Z, and
+// zero rows past m in the ncx columns of X, so the padded unknowns solve to
+// zero.  Columns [0, mp) of grid y are Z's, the next ncx are X's.
+__global__ void k_pad_group(const BlockJob *__restrict__ jobs, double2 *Z, long ldz, long zstride, int mp,
+                            double2 *X, long ldx, long xstride, int ncx)
+{
+    const int m = 3 * jobs[blockIdx.z].n;
+    const int j = blockIdx.y;
+    if (j < mp) {
+        double2 *c = Z + blockIdx.z * zstride + (long)j * ldz;
+        for (int i = (j < m ? m : 0) + threadIdx.x; i < mp; i += blockDim.x)
+            c[i] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    } else if (j - mp < ncx) {
+        double2 *c = X + blockIdx.z * xstride + (long)(j - mp) * ldx;
+        for (int i = m + threadIdx.x; i < mp; i += blockDim.x) c[i] = make_double2(0.0, 0.0);
+    }
+}
+
+// Solutions of a grouped batch back to their blocks: rows [0, 3 n) of the
+// ncol columns of matrix q go to X_b, b = jobs[q].dst.
+__global__ void k_copy_out_group(const BlockJob *__restrict__ jobs, const double2 *__restrict__ S, long lds,
+                                 long sstride, double2 *D, long ldd, long dstride)
+{
+    const BlockJob jb = jobs[blockIdx.z];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * jb.n) return;
+    D[jb.dst * dstride + (long)blockIdx.y * ldd + i] = S[blockIdx.z * sstride + (long)blockIdx.y * lds + i];
+}
 
 }  // namespace midas
 
 // E_dev: E is device memory (the fields stay in HBM) rather than host memory.
+// Releases a LuWork when its scope ends.
+template <typename T> struct LuRelease {
+    LuWork<T> &w;
+    ~LuRelease() { w.release(); }
+};
+
+// Records the launches of enqueue() on st into a hipGraph and launches it as
+// one.  The executable graph of batch `slot` stays in the context: the next
+// call whose batch records the same launches (other buffers and arguments)
+// updates it in place instead of instantiating a new one.
+template <typename F> static int run_as_graph(midas_ctx *ctx, size_t slot, F &&enqueue, hipStream_t st)
+{
+    MIDAS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int r = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (r || e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        if (r) return r;
+        MIDAS_HIP(e);
+    }
+    hipGraphExec_t &x = ctx->cbfm_graphs[slot];
+    if (x) {
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        hipGraphNode_t bad = nullptr;
+        if (hipGraphExecUpdate(x, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            hipGraphExecDestroy(x);
+            x = nullptr;
+        }
+    }
+    hipError_t ie = hipSuccess;
+    if (!x) ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ie != hipSuccess) x = nullptr;
+    MIDAS_HIP(ie);
+    MIDAS_HIP(hipGraphLaunch(x, st));
+    return 0;
+}
+
 static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, const midas_blocks_t *blocks, double k0,
                                 int quirks, int ntr_cbf, const double *th_cbf, const double *ph_cbf, int iws, int ntr,
                                 const double *tr_th, const double *tr_ph, int use_aca, int kmax, double eps_aca,
@@ -674,63 +745,89 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         TRY(Xc.alloc(sizeof(double2) * xmax));
         MIDAS_HIP(hipMemsetAsync(Xall.p, 0, sizeof(double2) * mp * T2c * nb, st));
         std::vector<int> hinfo(smax_c);
-        for (const Chunk &ch : chunks) {
+        DBuf djob;
+        TRY(djob.alloc(sizeof(BlockJob) * smax_c));
+        const BlockJob *jd = djob.as<BlockJob>();
+        const int *idx = didx.as<int>();
+        const bool use_graph = tune(TUNE_CBFM_GRAPH) != 0;
+        if (ctx->cbfm_graphs.size() < chunks.size()) ctx->cbfm_graphs.resize(chunks.size(), nullptr);
+        for (size_t ci = 0; ci < chunks.size(); ++ci) {
+            const Chunk &ch = chunks[ci];
             const int nc = ch.nc;
             const long mpc = ch.mpc;
-            auto Xq = [&](int q) { return Xc.as<double2>() + (long)q * mpc * T2c; };
-            MIDAS_HIP(hipMemsetAsync(Zall.p, 0, sizeof(double2) * mpc * mpc * nc, st));
-            MIDAS_HIP(hipMemsetAsync(Xc.p, 0, sizeof(double2) * mpc * T2c * nc, st));
-            for (int q = 0; q < nc; ++q) {
-                const int b = order[ch.pos + q], m = 3 * sb[b];
-                double2 *Zq = Zall.as<double2>() + (long)q * mpc * mpc;
-                CellSet S{sb[b], 0, didx.as<int>() + ioff[b]};
-                TRY(launch_fill(dc.v, S, S, k0, quirks, ref32, Zq, mpc, nullptr, 0, st));
-                if (sr)
-                    hipLaunchKernelGGL(k_sr_mask, dim3((m + 255) / 256, m), dim3(256), 0, st, Zq, mpc, m, dc.v, S,
-                                       fsr[b]);
-                if (m < mpc)
-                    hipLaunchKernelGGL(k_unit_diag, dim3((int)((mpc - m + 255) / 256)), dim3(256), 0, st, Zq, mpc, m,
-                                       (int)mpc);
-                MIDAS_HIP(hipGetLastError());
-                if (iws == 1) {
-                    TRY(launch_rhs(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32, Xq(q),
-                                   mpc, st));
-                } else {
-                    // spherical waves around the extended block (type_iws 2 and 3 both call
-                    // Incident_Field_Spherical, Compute_EFields_CBFME.f90:363-369)
-                    double cen[3], rs;
-                    sph_source_frame(cells->x, cells->y, cells->z, allidx.data() + ioff[b], sb[b], k0, quirks, cen,
-                                     &rs);
-                    TRY(launch_rhs_sph(dc.v, S, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0, quirks, ref32,
-                                       cen, rs, Xq(q), mpc, st));
-                }
-            }
-            const int block_mode = bm_set == 5 ? (mpc >= 16L * T2c ? MIDAS_SOLVE_F16_IR : MIDAS_SOLVE_FP64)
-                                   : bm_set == 3 ? MIDAS_SOLVE_F16_IR : bm_set;
-            if (block_mode != MIDAS_SOLVE_FP64) {
-                TRY(block_solve_mixed(block_mode, nc, (int)mpc, T2c, Zall.as<double2>(), Xq(0), ipiv.as<int>(),
-                                      perm.as<int>(), hinfo, bmw, nb_mixed, nb_lu, st));
-            } else {
-                LuWork<double> w;
-                TRY(w.alloc((int)mpc, nb_lu, nc, mpc * mpc));
-                TrailingGemm<double> tg;
-                tg.batch = nc;
-                tg.stride = mpc * mpc;
-                int r = getrf<double>(Zall.as<double2>(), mpc, (int)mpc, ipiv.as<int>(), w, st, tg);
-                if (!r) r = ipiv_to_perm_dev(ipiv.as<int>(), perm.as<int>(), (int)mpc, nc, st);
-                if (!r)
-                    r = getrs<double>(Zall.as<double2>(), mpc, (int)mpc, perm.as<int>(), w, Xq(0), mpc, T2c,
-                                      tmp.as<double2>(), st, mpc * T2c);
-                if (!r) r = copy_sync(hinfo.data(), w.info, sizeof(int) * nc, hipMemcpyDeviceToHost, st);
-                w.release();
-                if (r) return r;
-            }
+            MIDAS_CHECK(mpc + T2c <= 65535, -1, "cbfm: block order %ld + %d excitations too large for one launch", mpc,
+                        T2c);
+            // the batch's jobs: cells, SR factor, spherical-wave frame
+            std::vector<BlockJob> hj(nc);
+            int ncell = 0;
             for (int q = 0; q < nc; ++q) {
                 const int b = order[ch.pos + q];
-                MIDAS_HIP(hipMemcpy2DAsync(Xb(b), sizeof(double2) * mp, Xq(q), sizeof(double2) * mpc,
-                                           sizeof(double2) * 3 * sb[b], T2c, hipMemcpyDeviceToDevice, st));
-                if (hinfo[q] > 0 && worst_info == 0) worst_info = hinfo[q];
+                BlockJob &j = hj[q];
+                j = BlockJob{sb[b], ioff[b], b, 0, sr ? fsr[b] : 1.0, 0.0, 0.0, 0.0, 0.0};
+                if (iws != 1) {
+                    // spherical waves around the extended block (type_iws 2 and 3 both call
+                    // Incident_Field_Spherical, Compute_EFields_CBFME.f90:363-369)
+                    double cen[3];
+                    sph_source_frame(cells->x, cells->y, cells->z, allidx.data() + ioff[b], sb[b], k0, quirks, cen,
+                                     &j.rs);
+                    j.cx = cen[0];
+                    j.cy = cen[1];
+                    j.cz = cen[2];
+                }
+                ncell = std::max(ncell, sb[b]);
             }
+            TRY(copy_sync(djob.p, hj.data(), sizeof(BlockJob) * nc, hipMemcpyHostToDevice, st));
+            const int block_mode = bm_set == 5 ? (mpc >= 16L * T2c ? MIDAS_SOLVE_F16_IR : MIDAS_SOLVE_FP64)
+                                   : bm_set == 3 ? MIDAS_SOLVE_F16_IR : bm_set;
+            const bool fp64 = block_mode == MIDAS_SOLVE_FP64;
+            LuWork<double> w;
+            LuRelease<double> wguard{w};
+            TrailingGemm<double> tg;
+            if (fp64) {
+                TRY(w.alloc((int)mpc, nb_lu, nc, mpc * mpc));
+                w.defer_fault_check = true;   // checked after the batch (no synchronisation inside a graph)
+                tg.batch = nc;
+                tg.stride = mpc * mpc;
+            }
+            double2 *Z = Zall.as<double2>(), *X = Xc.as<double2>();
+            // One batch: grouped block fills, SR masks, padding, excitations;
+            // with the FP64 LU also the batched factorisation, its solve and
+            // the copy of the solutions to their blocks.
+            auto enqueue = [&]() -> int {
+                TRY(launch_fill_group(dc.v, jd, idx, nc, ncell, k0, quirks, ref32, Z, mpc, mpc * mpc, st));
+                if (sr)
+                    hipLaunchKernelGGL(k_sr_mask_group, dim3((3 * ncell + 255) / 256, 3 * ncell, nc), dim3(256), 0, st,
+                                       jd, idx, dc.v, Z, mpc, mpc * mpc);
+                hipLaunchKernelGGL(k_pad_group, dim3(1, (int)mpc + T2c, nc), dim3(256), 0, st, jd, Z, mpc, mpc * mpc,
+                                   (int)mpc, X, mpc, mpc * T2c, T2c);
+                MIDAS_HIP(hipGetLastError());
+                TRY(launch_rhs_group(dc.v, jd, idx, nc, ncell, ntr_cbf, dthc.as<double>(), dphc.as<double>(), k0,
+                                     quirks, ref32, iws != 1, X, mpc, mpc * T2c, st));
+                if (!fp64) return 0;
+                TRY(getrf<double>(Z, mpc, (int)mpc, ipiv.as<int>(), w, st, tg));
+                TRY(ipiv_to_perm_dev(ipiv.as<int>(), perm.as<int>(), (int)mpc, nc, st));
+                TRY(getrs<double>(Z, mpc, (int)mpc, perm.as<int>(), w, X, mpc, T2c, tmp.as<double2>(), st, mpc * T2c));
+                hipLaunchKernelGGL(k_copy_out_group, dim3((int)((mpc + 255) / 256), T2c, nc), dim3(256), 0, st, jd, X,
+                                   mpc, mpc * T2c, Xall.as<double2>(), mp, mp * T2c);
+                MIDAS_HIP(hipGetLastError());
+                return 0;
+            };
+            if (use_graph)
+                TRY(run_as_graph(ctx, ci, enqueue, st));
+            else
+                TRY(enqueue());
+            if (fp64) {
+                TRY(lu_fault_check<double>(w, st));
+                TRY(copy_sync(hinfo.data(), w.info, sizeof(int) * nc, hipMemcpyDeviceToHost, st));
+            } else {
+                TRY(block_solve_mixed(block_mode, nc, (int)mpc, T2c, Z, X, ipiv.as<int>(), perm.as<int>(), hinfo, bmw,
+                                      nb_mixed, nb_lu, st));
+                hipLaunchKernelGGL(k_copy_out_group, dim3((int)((mpc + 255) / 256), T2c, nc), dim3(256), 0, st, jd, X,
+                                   mpc, mpc * T2c, Xall.as<double2>(), mp, mp * T2c);
+                MIDAS_HIP(hipGetLastError());
+            }
+            for (int q = 0; q < nc; ++q)
+                if (hinfo[q] > 0 && worst_info == 0) worst_info = hinfo[q];
         }
     }
     phase(sr ? "SR masks + block fills + LU solves" : "block fills + LU solves");

@@ -89,6 +89,10 @@ struct ProfScope {
     bool on;
     ProfScope(hipStream_t s, int k, double f, double by) : st(s), flops(f), bytes(by), kind(k), on(profile().on)
     {
+        // launches recorded into a graph are not timed (a timing event cannot
+        // be recorded on a capturing stream)
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (on && hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) on = false;
         if (on) {
             hipEventCreate(&a);
             hipEventCreate(&b);
@@ -112,6 +116,14 @@ struct CellSet {
     __host__ __device__ int cell(int k) const { return idx ? idx[k] : start + k; }
 };
 
+// One matrix of a grouped block launch (blockIdx.z = matrix of the batch):
+// the extended block's cells are idx[off, off + n); dst is the block's number,
+// fsr its SR mask factor, (cx, cy, cz, rs) its spherical-wave source frame.
+struct BlockJob {
+    int n, off, dst, pad_;
+    double fsr, cx, cy, cz, rs;
+};
+
 // launchers (kernel files)
 int launch_cell_params(int n, const double *sc, const double *eps, double k0, int quirks, double *a, double *che,
                        double *znnpp, double *kappa, double *kchi, hipStream_t st);
@@ -123,6 +135,15 @@ void sph_source_frame(const double *x, const double *y, const double *z, const i
                       double c[3], double *rs);
 int launch_rhs_sph(const DevCells &cells, CellSet R, int ntr, const double *th, const double *ph, double k0,
                    int quirks, bool ref32, const double c[3], double rs, double2 *V, long ld, hipStream_t st);
+// Grouped forms for a batch of nq diagonal blocks (jobs and idx in device
+// memory; nmax = the largest jobs[q].n): matrix q is the (3n x 3n) block of its
+// cell set at Z + q zstride (ld), its excitations the 3n x 2 ntr columns at
+// V + q vstride (ld); sph picks the spherical waves of each job's frame.
+int launch_fill_group(const DevCells &cells, const BlockJob *jobs, const int *idx, int nq, int nmax, double k0,
+                      int quirks, bool ref32, double2 *Z, long ld, long zstride, hipStream_t st);
+int launch_rhs_group(const DevCells &cells, const BlockJob *jobs, const int *idx, int nq, int nmax, int ntr,
+                     const double *th, const double *ph, double k0, int quirks, bool ref32, bool sph, double2 *V,
+                     long ld, long vstride, hipStream_t st);
 int near_field(const DevCells &cells, const double2 *E, long ld, int T, int nrx, const double *rth, const double *rph,
                double rso, double k0, int quirks, double2 *S, long lds, hipStream_t st);
 int scatq_sums(const double2 *S, long lds, int nrx, int ntr, const double *rth, const double *rph, const double *w,

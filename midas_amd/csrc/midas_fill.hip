@@ -51,22 +51,23 @@ __global__ void k_cell_params(int n, const double *__restrict__ sc, const double
 
 // Z64 (ld64) and/or W32 (interleaved complex float, ld32) receive the block
 // (3*R.n x 3*S.n).  Diagonal cell pairs are detected by identifier equality.
+// Tile (bx, by) of a fill: 64 receiver cells x 4 source cells (one per wave).
 template <bool REF32>
-__global__ __launch_bounds__(256) void k_fill(DevCells cells, CellSet R, CellSet S, double k0, double pi,
-                                              double2 *__restrict__ Z64, long ld64, float2 *__restrict__ W32,
-                                              long ld32)
+__device__ __forceinline__ void fill_tile(const DevCells &cells, const CellSet &R, const CellSet &S, double k0,
+                                          double pi, double2 *__restrict__ Z64, long ld64, float2 *__restrict__ W32,
+                                          long ld32, int bx, int by)
 {
     // per wave: one column's 192 entries, staged so that each store
     // instruction writes 64 consecutive elements
     __shared__ double2 st64[4][3 * 64];
     __shared__ float2 st32[4][3 * 64];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int o = blockIdx.x * 64 + tx;
-    const int sl = blockIdx.y * 4 + ty;
+    const int o = bx * 64 + tx;
+    const int sl = by * 4 + ty;
     if (sl >= S.n) return;   // wave-uniform
     const bool valid = o < R.n;
-    const int rows = 3 * min(64, R.n - (int)blockIdx.x * 64);
-    const int go = R.cell(valid ? o : blockIdx.x * 64), gs = S.cell(sl);
+    const int rows = 3 * min(64, R.n - bx * 64);
+    const int go = R.cell(valid ? o : bx * 64), gs = S.cell(sl);
     z64 e[9];
     if (cells.id[go] == cells.id[gs]) {
         z64 d = {1.0 - cells.znnpp[2 * gs], -cells.znnpp[2 * gs + 1]};
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void k_fill(DevCells cells, CellSet R, CellSet
                 for (int po = 0; po < 3; ++po) st64[ty][3 * tx + po] = make_double2(e[3 * ps + po].r, e[3 * ps + po].i);
             }
             asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-            double2 *dst = Z64 + col * ld64 + 3L * blockIdx.x * 64;
+            double2 *dst = Z64 + col * ld64 + 3L * bx * 64;
 #pragma unroll
             for (int k = 0; k < 3; ++k)
                 if (64 * k + tx < rows) dst[64 * k + tx] = st64[ty][64 * k + tx];
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(256) void k_fill(DevCells cells, CellSet R, CellSet
                     st32[ty][3 * tx + po] = make_float2((float)e[3 * ps + po].r, (float)e[3 * ps + po].i);
             }
             asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-            float2 *dst = W32 + col * ld32 + 3L * blockIdx.x * 64;
+            float2 *dst = W32 + col * ld32 + 3L * bx * 64;
 #pragma unroll
             for (int k = 0; k < 3; ++k)
                 if (64 * k + tx < rows) dst[64 * k + tx] = st32[ty][64 * k + tx];
@@ -116,16 +117,36 @@ __global__ __launch_bounds__(256) void k_fill(DevCells cells, CellSet R, CellSet
     }
 }
 
+template <bool REF32>
+__global__ __launch_bounds__(256) void k_fill(DevCells cells, CellSet R, CellSet S, double k0, double pi,
+                                              double2 *__restrict__ Z64, long ld64, float2 *__restrict__ W32,
+                                              long ld32)
+{
+    fill_tile<REF32>(cells, R, S, k0, pi, Z64, ld64, W32, ld32, blockIdx.x, blockIdx.y);
+}
+
+// Grouped diagonal-block fill: matrix blockIdx.z of a batch is the block of
+// the cell set jobs[z] (Green_s_tr_partial of an extended block,
+// Compute_EFields_CBFME.f90:375-377), one launch for all blocks of the batch.
+template <bool REF32>
+__global__ __launch_bounds__(256) void k_fill_group(DevCells cells, const BlockJob *__restrict__ jobs,
+                                                    const int *__restrict__ idx, double k0, double pi,
+                                                    double2 *__restrict__ Z, long ld, long zstride)
+{
+    const BlockJob j = jobs[blockIdx.z];
+    if ((int)blockIdx.x * 64 >= j.n) return;   // workgroup-uniform
+    const CellSet S{j.n, 0, idx + j.off};
+    fill_tile<REF32>(cells, S, S, k0, pi, Z + blockIdx.z * zstride, ld, nullptr, 0, blockIdx.x, blockIdx.y);
+}
+
 // Plane-wave excitation: column t (V-pol) and t+ntr (H-pol) of V (3*R.n x 2*ntr).
 // ref32: K11 and the field components are single precision (Incident_Field.f90:17).
 template <bool REF32>
-__global__ void k_rhs(DevCells cells, CellSet R, int ntr, const double *__restrict__ th_deg,
-                      const double *__restrict__ ph_deg, double k0, double pi, double2 *__restrict__ V, long ld)
+__device__ __forceinline__ void rhs_plane_at(const DevCells &cells, const CellSet &R, int c, int t, int ntr,
+                                             const double *__restrict__ th_deg, const double *__restrict__ ph_deg,
+                                             double k0, double pi, double2 *__restrict__ V, long ld)
 {
 #pragma clang fp contract(off)
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    int t = blockIdx.y;
-    if (c >= R.n) return;
     int g = R.cell(c);
     double th = th_deg[t] * pi / 180.0, ph = ph_deg[t] * pi / 180.0;
     double sth, cth, sph, cph;
@@ -170,14 +191,12 @@ __global__ void k_rhs(DevCells cells, CellSet R, int ntr, const double *__restri
 // REF32: phase and every component are complex(4), as the reference declares
 // them; the products are formed in double first (mixed-kind promotion).
 template <bool REF32>
-__global__ void k_rhs_sph(DevCells cells, CellSet R, int ntr, const double *__restrict__ th_deg,
-                          const double *__restrict__ ph_deg, double k0, double pi, double cx, double cy, double cz,
-                          double rs, double2 *__restrict__ V, long ld)
+__device__ __forceinline__ void rhs_sph_at(const DevCells &cells, const CellSet &R, int c, int t, int ntr,
+                                           const double *__restrict__ th_deg, const double *__restrict__ ph_deg,
+                                           double k0, double pi, double cx, double cy, double cz, double rs,
+                                           double2 *__restrict__ V, long ld)
 {
 #pragma clang fp contract(off)
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = blockIdx.y;
-    if (c >= R.n) return;
     const int g = R.cell(c);
     const double th = th_deg[t] * pi / 180.0, ph = ph_deg[t] * pi / 180.0;
     const double xs = cx + rs * cos(th);
@@ -209,6 +228,42 @@ __global__ void k_rhs_sph(DevCells cells, CellSet R, int ntr, const double *__re
             dh[p] = make_double2(h[p].r, h[p].i);
         }
     }
+}
+
+template <bool REF32>
+__global__ void k_rhs(DevCells cells, CellSet R, int ntr, const double *__restrict__ th_deg,
+                      const double *__restrict__ ph_deg, double k0, double pi, double2 *__restrict__ V, long ld)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < R.n) rhs_plane_at<REF32>(cells, R, c, blockIdx.y, ntr, th_deg, ph_deg, k0, pi, V, ld);
+}
+
+template <bool REF32>
+__global__ void k_rhs_sph(DevCells cells, CellSet R, int ntr, const double *__restrict__ th_deg,
+                          const double *__restrict__ ph_deg, double k0, double pi, double cx, double cy, double cz,
+                          double rs, double2 *__restrict__ V, long ld)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < R.n) rhs_sph_at<REF32>(cells, R, c, blockIdx.y, ntr, th_deg, ph_deg, k0, pi, cx, cy, cz, rs, V, ld);
+}
+
+// Grouped excitations of a batch of extended blocks (blockIdx.z = matrix):
+// plane waves (Incident_Field) or each block's spherical waves
+// (Incident_Field_Spherical), Compute_EFields_CBFME.f90:363-369.
+template <bool REF32, bool SPH>
+__global__ void k_rhs_group(DevCells cells, const BlockJob *__restrict__ jobs, const int *__restrict__ idx, int ntr,
+                            const double *__restrict__ th_deg, const double *__restrict__ ph_deg, double k0, double pi,
+                            double2 *__restrict__ V, long ld, long vstride)
+{
+    const BlockJob j = jobs[blockIdx.z];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= j.n) return;
+    const CellSet R{j.n, 0, idx + j.off};
+    V += blockIdx.z * vstride;
+    if (SPH)
+        rhs_sph_at<REF32>(cells, R, c, blockIdx.y, ntr, th_deg, ph_deg, k0, pi, j.cx, j.cy, j.cz, j.rs, V, ld);
+    else
+        rhs_plane_at<REF32>(cells, R, c, blockIdx.y, ntr, th_deg, ph_deg, k0, pi, V, ld);
 }
 
 // ---------------------------------------------------------------------------
@@ -311,6 +366,45 @@ int launch_rhs_sph(const DevCells &cells, CellSet R, int ntr, const double *th, 
     else
         hipLaunchKernelGGL(k_rhs_sph<false>, grd, dim3(256), 0, st, cells, R, ntr, th, ph, k0, pi, c[0], c[1], c[2],
                            rs, V, ld);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_fill_group(const DevCells &cells, const BlockJob *jobs, const int *idx, int nq, int nmax, double k0,
+                      int quirks, bool ref32, double2 *Z, long ld, long zstride, hipStream_t st)
+{
+    if (nq <= 0 || nmax <= 0) return 0;
+    MIDAS_CHECK(ld >= 3L * nmax && zstride >= ld * 3L * nmax && nq <= 65535 && (nmax + 3) / 4 <= 65535, -1,
+                "fill_group: %d blocks of up to %d cells do not fit ld %ld, stride %ld", nq, nmax, ld, zstride);
+    const double pi = host_pi(quirks);
+    const dim3 grd((nmax + 63) / 64, (nmax + 3) / 4, nq), blk(64, 4);
+    if (ref32)
+        hipLaunchKernelGGL(k_fill_group<true>, grd, blk, 0, st, cells, jobs, idx, k0, pi, Z, ld, zstride);
+    else
+        hipLaunchKernelGGL(k_fill_group<false>, grd, blk, 0, st, cells, jobs, idx, k0, pi, Z, ld, zstride);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_rhs_group(const DevCells &cells, const BlockJob *jobs, const int *idx, int nq, int nmax, int ntr,
+                     const double *th, const double *ph, double k0, int quirks, bool ref32, bool sph, double2 *V,
+                     long ld, long vstride, hipStream_t st)
+{
+    if (nq <= 0 || nmax <= 0 || ntr <= 0) return 0;
+    MIDAS_CHECK(ntr <= 65535 && nq <= 65535 && ld >= 3L * nmax && vstride >= ld * 2L * ntr, -1,
+                "rhs_group: %d blocks of up to %d cells, %d directions do not fit ld %ld, stride %ld", nq, nmax, ntr,
+                ld, vstride);
+    const double pi = host_pi(quirks);
+    const dim3 grd((nmax + 255) / 256, ntr, nq);
+#define RHS_GROUP(R32, SP)                                                                                        \
+    hipLaunchKernelGGL((k_rhs_group<R32, SP>), grd, dim3(256), 0, st, cells, jobs, idx, ntr, th, ph, k0, pi, V, ld, \
+                       vstride)
+    if (ref32) {
+        if (sph) RHS_GROUP(true, true); else RHS_GROUP(true, false);
+    } else {
+        if (sph) RHS_GROUP(false, true); else RHS_GROUP(false, false);
+    }
+#undef RHS_GROUP
     MIDAS_HIP(hipGetLastError());
     return 0;
 }

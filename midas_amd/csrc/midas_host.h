@@ -199,6 +199,9 @@ struct midas_ctx {
     DevCellsOwned ncells;
     hipEvent_t nev[2] = {};
     double cbfm_budget[4] = {};   // bytes; 0 = default (midas_set_cbfm_budgets)
+    // executable graphs of the CBFM block batches (one per batch position),
+    // updated in place by the next call whose batch has the same launches
+    std::vector<hipGraphExec_t> cbfm_graphs;
 };
 
 // Event timer on a stream; a disabled timer creates no events and reads 0.

@@ -30,6 +30,7 @@ template <typename T> struct LuWork {
     hipEvent_t ev_panel = nullptr, ev_next = nullptr;
     size_t cap[10] = {};              // bytes held by each buffer above (grow-only)
     bool borrowed = false;           // buffers, stream and events lent by the caller (lend())
+    bool defer_fault_check = false;  // getrf leaves the wait-fault check to lu_fault_check (stream capture)
     // Sizes the workspace for (n, nb, batch, sA): buffers grow when too small
     // and are kept otherwise, so a LuWork reused for one size allocates once;
     // the panel stream and its events are created on first use.
@@ -79,6 +80,12 @@ template <typename T>
 int getrs(const typename cvec<T>::t *A, long lda, int n, const int *perm, const LuWork<T> &w, typename cvec<T>::t *B,
           long ldb, int nrhs, typename cvec<T>::t *tmp, hipStream_t st, long sB = 0);
 long getrs_tmp_elems(int n, int nb, int nrhs);
+// The check getrf makes at its end (a panel wait that timed out), for
+// factorisations enqueued with w.defer_fault_check (synchronises st).
+template <typename T> int lu_fault_check(const LuWork<T> &w, hipStream_t st);
+// bytes (a multiple of 4) of device memory set to zero by a kernel (safe under
+// stream capture, unlike hipMemsetAsync)
+int zero_device(void *p, size_t bytes, hipStream_t st);
 void ipiv_to_perm(const int *ipiv, int n, int *perm);
 int ipiv_to_perm_dev(const int *ipiv, int *perm, int n, int batch, hipStream_t st);
 

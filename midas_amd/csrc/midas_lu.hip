@@ -781,6 +781,7 @@ static int zero_words(void *p, size_t bytes, hipStream_t st)
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
+int zero_device(void *p, size_t bytes, hipStream_t st) { return zero_words(p, bytes, st); }
 #define TRY_LU(x)               \
     do {                        \
         int r_ = (x);           \
@@ -1053,12 +1054,19 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
                             w.batch, w.sA, w.inv_stride());
         if (r) return r;
     }
+    return w.defer_fault_check ? 0 : lu_fault_check<T>(w, st);
+}
+
+template <typename T> int lu_fault_check(const LuWork<T> &w, hipStream_t st)
+{
     int fault = 0;
     MIDAS_HIP(hipMemcpyAsync(&fault, w.fault, sizeof(int), hipMemcpyDeviceToHost, st));
     MIDAS_HIP(hipStreamSynchronize(st));
     MIDAS_CHECK(fault == 0, -3, "lu: a panel wait timed out (workgroups not co-resident)");
     return 0;
 }
+template int lu_fault_check<double>(const LuWork<double> &, hipStream_t);
+template int lu_fault_check<float>(const LuWork<float> &, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Triangular solves with the factors (getrs), nrhs columns of B (ldb).

@@ -15,6 +15,7 @@ enum {
     TUNE_FP64_GEMM_LIBRARY,
     TUNE_CBFM_BLOCK_MODE,
     TUNE_CGEMM_TILE,
+    TUNE_CBFM_GRAPH,
     TUNE_COUNT
 };
 int tune(int which);

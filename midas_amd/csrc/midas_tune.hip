@@ -25,6 +25,7 @@ static TuneEntry g_tune[TUNE_COUNT] = {
     {"fp64_gemm_library", MIDAS_TUNED_FP64_GEMM_LIBRARY, 0, 1},
     {"cbfm_block_mode", MIDAS_TUNED_CBFM_BLOCK_MODE, 0, 5},
     {"cgemm_tile", MIDAS_TUNED_CGEMM_TILE, 0, 5},
+    {"cbfm_graph", MIDAS_TUNED_CBFM_GRAPH, 0, 1},
 };
 
 int tune(int which) { return g_tune[which].value; }

@@ -220,6 +220,49 @@ def test_refinement_graph_replays_match_eager(mid, oracle, nrhs):
     assert _rel(Xg, np.linalg.solve(Z, B)) < 1e-10
 
 
+@pytest.mark.parametrize("sr,iws,one_per_batch", [(False, 1, False), (True, 2, False), (False, 1, True)])
+def test_cbfm_graph_matches_eager(mid, dlca6, sr, iws, one_per_batch):
+    """The CBF block batches (grouped fills, SR masks, excitations, batched
+    FP64 LU and solve) launched as hipGraphs give the same K and fields, bit
+    for bit, as the same launches issued one by one: on the first call (graphs
+    instantiated), on a repeat (the kept executable graphs updated in place)
+    and with another wavenumber (same launches, other arguments), with eager
+    calls in between.  one_per_batch sets the LU budget to one byte, so every
+    block is a batch with a graph of its own."""
+    s = dlca6
+    _case(f"cbfm graph vs eager sr={sr} iws={iws} one_per_batch={one_per_batch} n={3 * s.n} "
+          f"blocks={len(s.blocks)}")
+    th_c, ph_c = g.directions(30)
+    tth, tph = np.array([0.0, 35.0]), np.array([0.0, 80.0])
+
+    def run(k0, graph):
+        old = api.tune_set("cbfm_graph", graph)
+        if one_per_batch:
+            mid.set_cbfm_budgets(1, 0, 0, 0)
+        try:
+            E, K, _ = mid.compute_efields_cbfme(s, k0, s.blocks, s.ext, (th_c, ph_c), (tth, tph), sr=sr,
+                                                res_sr=0.01, iws=iws)
+        finally:
+            api.tune_set("cbfm_graph", old)
+            if one_per_batch:
+                mid.set_cbfm_budgets(0, 0, 0, 0)
+        return E, list(K)
+
+    k1 = 1.03 * s.k0
+    ref = {s.k0: run(s.k0, 0), k1: run(k1, 0)}
+    for k0 in (s.k0, s.k0, k1, s.k0):
+        Eg, Kg = run(k0, 1)
+        Ee, Ke = ref[k0]
+        print("k0", k0, "K", Kg, "graph vs eager", _rel(Eg, Ee))
+        assert Kg == Ke
+        np.testing.assert_array_equal(Eg, Ee)
+        # eager work between the graph launches
+        Ex, Kx = run(k0, 0)
+        assert Kx == Ke
+        np.testing.assert_array_equal(Ex, Ee)
+    assert not np.array_equal(ref[s.k0][0], ref[k1][0])
+
+
 @pytest.mark.parametrize("name,value", [("panel_leaf", 32), ("panel_leaf", 128), ("trail_group_kib", 512),
                                         ("lu_nb_split", 768), ("lu_nb_fp64", 256)])
 def test_tuned_parameters_keep_results(mid, name, value):
