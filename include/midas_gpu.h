@@ -84,6 +84,12 @@ typedef struct {
     double anorm;      /* ||Z||_inf (PZLANGE 'I', Compute_EFields_MoM.f90:168)                */
     double t_factor_ms, t_solve_ms, t_refine_ms;
     int status;        /* MIDAS_STATUS_*                                                      */
+    int pivot_search;  /* mixed LU: 0 partial pivoting over each whole column; 1 pivot search
+                        * restricted to each panel's diagonal block (schedule parameter
+                        * "lu_pivot_search": 1 for the MoM solves, 2 for every solve)         */
+    double max_l21;    /* pivot_search 1: largest |l| below the diagonal blocks; <= 1/sqrt(2)
+                        * certifies that partial pivoting over the whole columns would have
+                        * chosen the same pivots                                              */
 } midas_solve_info_t;
 
 /* ---------------- context ---------------- */

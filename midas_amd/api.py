@@ -77,7 +77,7 @@ class solve_opts_t(C.Structure):
 class solve_info_t(C.Structure):
     _fields_ = [("info", C.c_int), ("iterations", C.c_int), ("berr", C.c_double), ("anorm", C.c_double),
                 ("t_factor_ms", C.c_double), ("t_solve_ms", C.c_double), ("t_refine_ms", C.c_double),
-                ("status", C.c_int)]
+                ("status", C.c_int), ("pivot_search", C.c_int), ("max_l21", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -109,7 +109,8 @@ def _p(a):
 
 
 TUNE_NAMES = ("lu_nb_split", "lu_two_panel_min_n", "lu_nb_mixed", "lu_nb_fp64", "panel_leaf", "trail_group_kib",
-              "refine_graph", "zgemm_tile", "fp64_gemm_library", "cbfm_block_mode", "cgemm_tile", "cbfm_graph")
+              "refine_graph", "zgemm_tile", "fp64_gemm_library", "cbfm_block_mode", "cgemm_tile", "cbfm_graph",
+              "lu_pivot_search")
 
 
 def tune_get(name):

@@ -52,6 +52,7 @@ COMMENTS = {
     "cbfm_block_mode": "CBF block solves: 0 FP64 LU, 1 c32 / 2 f16 / 4 bf16 LU + FP64 refinement, 5 by size",
     "cgemm_tile": "tile shape of the complex-FP32 GEMM without transposes (5: the 64x64 k_cgemm)",
     "cbfm_graph": "CBFM block batches (fills, excitations, FP64 LU, solve) launched as one hipGraph each",
+    "lu_pivot_search": "mixed LU pivot search in each panel's diagonal block (certified): 1 MoM solves, 2 all; 0 whole columns",
 }
 
 

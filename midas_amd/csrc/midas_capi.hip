@@ -378,10 +378,26 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
         TRY(ws.trailp.reserve(trail16_scratch_bytes(n, nbi, nbi)));
         tgp.scratch = ws.trailp.p;
     }
+    // pivot search in the diagonal blocks (MoM solves; not with lent buffers)
+    w.diag_pivot = ws.pivot_search && !ws.lent;
+    if (w.diag_pivot) {
+        const size_t sl = sizeof(float2) * (size_t)n * nbi;
+        TRY(ws.l21.reserve(sl + 256));
+        w.l21 = ws.l21.as<c32>();
+        w.lmax = (unsigned *)((char *)ws.l21.p + sl);
+    }
     int r = getrf<float>(A, lda, n, ws.ipiv.as<int>(), w, st, tg, tgp);
     nb = nbi;   // the diagonal blocks getrs works with
+    inf.pivot_search = w.diag_pivot ? 1 : 0;
+    w.diag_pivot = false;
     if (r) return r;
     MIDAS_HIP(hipMemcpyAsync(&inf.info, w.info, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (inf.pivot_search) {
+        unsigned lm = 0;
+        MIDAS_HIP(hipMemcpyAsync(&lm, w.lmax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        MIDAS_HIP(hipStreamSynchronize(st));
+        inf.max_l21 = (double)__builtin_bit_cast(float, lm);
+    }
     TRY(make_perm(ws.ipiv.as<int>(), ws.perm.as<int>(), n, st));
     inf.t_factor_ms = tm.ms();
     // refinement: X = 0, R = B; repeat D = LU^-1 R (FP32), X += D, R = B - Z X (FP64)
@@ -757,8 +773,12 @@ int midas_solve(midas_ctx *ctx, int n, int nrhs, const double *Z, long ldz, doub
                                hipMemcpyHostToDevice, st));
     MIDAS_HIP(hipMemcpy2DAsync(db.p, sizeof(double2) * n, B, sizeof(double2) * ldb, sizeof(double2) * n, nrhs,
                                hipMemcpyHostToDevice, st));
+    // lu_pivot_search = 2 extends the diagonal-block pivot search to these
+    // general solves (refinement and its FP64 fallback keep the answer)
+    ctx->ws.pivot_search = tune(TUNE_LU_PIVOT_SEARCH) == 2;
     int r = solve_dev(ctx, n, nrhs, dz.as<double2>(), n, nullptr, 0, db.as<double2>(), n, dx.as<double2>(), n, mode,
                       opts, info, true);
+    ctx->ws.pivot_search = 0;
     // not converged (no_fallback): the best iterate is still returned in B
     if (r < 0 && r != MIDAS_ERR_NOT_CONVERGED) return r;
     MIDAS_HIP(hipMemcpy2DAsync(B, sizeof(double2) * ldb, dx.p, sizeof(double2) * n, sizeof(double2) * n, nrhs,
@@ -960,7 +980,17 @@ static int mom_solve_dev(midas_ctx *ctx, SolveWs &ws, const DevCells &cells, dou
     TRY(launch_rhs(cells, CellSet{n, 0, nullptr}, ntr, dth, dph, k0, quirks, false, ws.V.as<double2>(), n3, st));
     double tf = tm.ms();
     if (t_fill_ms) *t_fill_ms = tf;
-    return solve_dev(ctx, n3, 2 * ntr, Z, ldz, w32, n3, ws.V.as<double2>(), n3, X, ldx, mode, opts, info, true, &ws);
+    // The MoM matrix (1 - Znnpp on the diagonal, Green's couplings scaled by
+    // the cells' contrast off it) takes its pivots on the diagonal under
+    // partial pivoting (LAPACK's zgetrf makes no interchange on the dielectric
+    // scatterers here): its mixed LU searches each panel's diagonal block only
+    // and reports the certificate (max |L21|); refinement and its FP64
+    // fallback keep the answer whatever the pivots.
+    ws.pivot_search = tune(TUNE_LU_PIVOT_SEARCH);
+    const int r = solve_dev(ctx, n3, 2 * ntr, Z, ldz, w32, n3, ws.V.as<double2>(), n3, X, ldx, mode, opts, info, true,
+                            &ws);
+    ws.pivot_search = 0;
+    return r;
 }
 
 int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double k0, int quirks, int ntr,

@@ -174,12 +174,14 @@ struct SolveWs {
     DBuf xbest;        // refinement: iterate with the smallest backward error
     DBuf zcopy;        // FP64 copy of Z when Z must be kept
     DBuf V;            // excitation of the MoM solve
+    DBuf l21;          // diagonal-block pivot search: L21 staging (n x nb) and max |L21|
+    int pivot_search = 0;   // the next mixed solve restricts its pivot search to diagonal blocks (MoM)
     LuWork<float> luf;
     LuWork<double> lud;
     bool lent = false; // every buffer lent from a fixed scratch: nothing may be allocated
     void release()
     {
-        DBuf *b[] = {&W, &trail, &trailp, &ipiv, &perm, &norms, &zx, &R, &D, &t2, &xbest, &zcopy, &V};
+        DBuf *b[] = {&W, &trail, &trailp, &ipiv, &perm, &norms, &zx, &R, &D, &t2, &xbest, &zcopy, &V, &l21};
         for (DBuf *x : b) x->release();
         luf.release();
         lud.release();

@@ -240,8 +240,8 @@ template <typename T> __device__ inline bool pivot_better(T v, int i, T bv, int 
     return v > bv || (v == bv && i < bi);
 }
 
-// argmax over the workgroup; every thread gets the result.
-template <typename T> __device__ inline void argmax_block(T &v, int &i, T *rv, int *ri)
+// argmax over the workgroup of NW waves; every thread gets the result.
+template <typename T, int NW = BLK_T / 64> __device__ inline void argmax_block(T &v, int &i, T *rv, int *ri)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -252,7 +252,7 @@ template <typename T> __device__ inline void argmax_block(T &v, int &i, T *rv, i
     if ((threadIdx.x & 63) == 0) { rv[threadIdx.x >> 6] = v; ri[threadIdx.x >> 6] = i; }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < BLK_T / 64; ++q)
+    for (int q = 0; q < NW; ++q)
         if (pivot_better(rv[q], ri[q], v, i)) { v = rv[q]; i = ri[q]; }
     __syncthreads();
 }
@@ -475,6 +475,147 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
     }
 }
 
+// Inner block whose rows [jb, rend) fit one workgroup (rend - jb <= LOC_T),
+// one workgroup per matrix of a batch (grid y): the same column steps as
+// k_getf2_block (pivot = largest |re| + |im|, first row on ties, as izamax;
+// the same scaling and rank-1 update), with one barrier per column: each
+// wave reduces its candidates, and its winning lane leaves (value, row, the
+// row's entries) in an LDS slot, next to row j's entries; after the barrier
+// every thread picks the winner among the waves' slots and reads the pivot
+// row from it.  Slots alternate by column parity, so a slot is rewritten only
+// after every thread has passed the next column's barrier.  It takes the inner
+// blocks of the last rows of a factorisation and the diagonal blocks of
+// panels whose pivot search is restricted to them.
+constexpr int LOC_T = 512;
+template <typename T>
+__global__ __launch_bounds__(LOC_T) void k_getf2_local(typename cvec<T>::t *A, long lda, long sA, int rend, int jb,
+                                                       int ncols, int *ipiv, long sP, int *info)
+{
+    typedef typename cvec<T>::t V;
+    constexpr int NW = LOC_T / 64;
+    __shared__ V crow[2][NW][LU_IB];   // each wave's candidate row
+    __shared__ V jrow[2][LU_IB];       // row j
+    __shared__ T cval[2][NW];
+    __shared__ int cidx[2][NW];
+    const int tid = threadIdx.x, zb = blockIdx.y, wv = tid >> 6, lane = tid & 63;
+    A += zb * sA;
+    ipiv += zb * sP;
+    const int i = jb + tid;
+    const bool mine = i < rend;
+    __builtin_amdgcn_s_setprio(3);
+    V *Ab = A + jb + (long)jb * lda;
+    V a[LU_IB];
+#pragma unroll
+    for (int t = 0; t < LU_IB; ++t) a[t] = (mine && t < ncols) ? Ab[(long)t * lda + tid] : V{0, 0};
+    for (int c = 0; c < LU_IB; ++c) {
+        if (c < ncols) {
+            const int j = jb + c, par = c & 1;
+            T best = (T)-1;
+            int bi = 0x7fffffff;
+            if (mine && i >= j) {
+                best = fabs(a[0].x) + fabs(a[0].y);
+                bi = i;
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const T ov = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (pivot_better(ov, oi, best, bi)) {
+                    best = ov;
+                    bi = oi;
+                }
+            }
+            if (i == bi) {
+                cval[par][wv] = best;
+                cidx[par][wv] = bi;
+#pragma unroll
+                for (int t = 0; t < LU_IB; ++t) crow[par][wv][t] = a[t];
+            } else if (lane == 0 && bi == 0x7fffffff) {
+                cval[par][wv] = best;
+                cidx[par][wv] = bi;
+            }
+            if (i == j) {
+#pragma unroll
+                for (int t = 0; t < LU_IB; ++t) jrow[par][t] = a[t];
+            }
+            __syncthreads();
+            T gv = cval[par][0];
+            int gi = cidx[par][0], gw = 0;
+#pragma unroll
+            for (int q = 1; q < NW; ++q)
+                if (pivot_better(cval[par][q], cidx[par][q], gv, gi)) {
+                    gv = cval[par][q];
+                    gi = cidx[par][q];
+                    gw = q;
+                }
+            const int p = __builtin_amdgcn_readfirstlane(gi == 0x7fffffff ? j : gi);
+            const V *urow = gi == 0x7fffffff ? jrow[par] : crow[par][__builtin_amdgcn_readfirstlane(gw)];
+            if (tid == 0) ipiv[j] = p;
+            if (p != j) {
+                if (i == j) {
+#pragma unroll
+                    for (int t = 0; t < LU_IB; ++t) a[t] = urow[t];
+                } else if (i == p) {
+#pragma unroll
+                    for (int t = 0; t < LU_IB; ++t) a[t] = jrow[par][t];
+                }
+            }
+            const V d = urow[0];
+            if (mine && i > j) {
+                cplx<T> pinv = {(T)0, (T)0};
+                if (d.x != (T)0 || d.y != (T)0) pinv = cinv(cplx<T>{d.x, d.y});
+                const cplx<T> l = cmul(cplx<T>{a[0].x, a[0].y}, pinv);
+                a[0] = V{l.r, l.i};
+                const int nright = ncols - c;
+#pragma unroll
+                for (int t = 1; t < LU_IB; ++t)
+                    if (t < nright) {
+                        const cplx<T> r = csub(cplx<T>{a[t].x, a[t].y}, cmul(l, cplx<T>{urow[t].x, urow[t].y}));
+                        a[t] = V{r.r, r.i};
+                    }
+            }
+            if (tid == 0 && d.x == (T)0 && d.y == (T)0) atomicCAS(info + zb, 0, j + 1);
+        }
+        const V a0 = a[0];
+#pragma unroll
+        for (int t = 0; t < LU_IB - 1; ++t) a[t] = a[t + 1];
+        a[LU_IB - 1] = a0;
+    }
+    if (mine) {
+#pragma unroll
+        for (int t = 0; t < LU_IB; ++t)
+            if (t < ncols) Ab[(long)t * lda + tid] = a[t];
+    }
+}
+
+// L21 from the staging buffer S (rows x cols, lds) back into the matrix, and
+// the largest modulus of its entries into *amax (float bits, atomic max: the
+// order of non-negative floats is that of their bits).
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_amax(const typename cvec<T>::t *S, long lds,
+                                                   typename cvec<T>::t *D, long ldd, int rows, int cols,
+                                                   unsigned *amax)
+{
+    // a few thousand workgroups, each over a share of rows and columns, so
+    // that few atomics meet on *amax
+    __shared__ float wm[4];
+    float m = 0.0f;
+    for (int c = blockIdx.y; c < cols; c += gridDim.y)
+        for (int r = blockIdx.x * 256 + threadIdx.x; r < rows; r += gridDim.x * 256) {
+            const typename cvec<T>::t v = S[(long)r + (long)c * lds];
+            D[(long)r + (long)c * ldd] = v;
+            m = fmaxf(m, (float)sqrt((double)v.x * v.x + (double)v.y * v.y));
+        }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
+        if (m > 0.0f) atomicMax(amax, __float_as_uint(m));
+    }
+}
+
 // Interchanges ipiv[jb, jend) on panel columns [k, jb) and [jend, k1); then,
 // on [jend, k1), forward substitution with the unit-lower block
 // L(jb:jend, jb:jend) on rows [jb, jend).  One thread per column.  The
@@ -628,21 +769,33 @@ __global__ void k_trsm_lunit_small(typename cvec<T>::t *A, long lda, long sA, in
     }
 }
 
-// Apply the row interchanges ipiv[k0..k1) to columns [c0, c1) (and [c2, c3)).
+// Apply the row interchanges ipiv[k0..k1) to columns [c0, c1) (and [c2, c3)),
+// one thread per column (256 per workgroup).  The workgroup stages the
+// interchanges in LDS first and returns at once when all are identities (the
+// rule with a pivot search that keeps the diagonal).
+constexpr int LASWP_LDS = 1024;
 template <typename T>
-__global__ void k_laswp(typename cvec<T>::t *A, long lda, long sA, const int *ipiv, int n, int k0, int k1, int c0,
-                        int c1, int c2, int c3)
+__global__ __launch_bounds__(256) void k_laswp(typename cvec<T>::t *A, long lda, long sA, const int *ipiv, int n,
+                                               int k0, int k1, int c0, int c1, int c2, int c3)
 {
     typedef typename cvec<T>::t V;
+    __shared__ int pv[LASWP_LDS];
     A += blockIdx.y * sA;
     ipiv += (long)blockIdx.y * n;
+    int moved = 0;
+    for (int j = threadIdx.x; j < k1 - k0; j += blockDim.x) {
+        const int p = ipiv[k0 + j];
+        if (j < LASWP_LDS) pv[j] = p;
+        moved |= p != k0 + j;
+    }
+    if (!__syncthreads_or(moved)) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int w0 = c1 - c0;
     int c = t < w0 ? c0 + t : c2 + (t - w0);
     if (t >= w0 + (c3 - c2)) return;
     V *col = A + (long)c * lda;
     for (int j = k0; j < k1; ++j) {
-        int p = ipiv[j];
+        const int p = j - k0 < LASWP_LDS ? pv[j - k0] : ipiv[j];
         if (p != j) {
             V a = col[j];
             col[j] = col[p];
@@ -661,14 +814,11 @@ __global__ void k_laswp(typename cvec<T>::t *A, long lda, long sA, const int *ip
 // LOWER: unit lower L, x = L^-1 e_c; otherwise upper U with its diagonal,
 // x = U^-1 e_c.  Inv is ldi x bs.
 template <typename T, bool LOWER, int R>
-__global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D, long lda, long sD, int bs,
-                                                    typename cvec<T>::t *Inv, long ldi, long sI, long sD2 = 0,
-                                                    long sI2 = 0)
+__device__ __forceinline__ void trinv_wave(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T>::t *Inv,
+                                           long ldi)
 {
     typedef typename cvec<T>::t V;
     constexpr int TRINV_QC = R >= 8 ? (sizeof(T) == 4 ? 3 : 2) : (R >= 4 ? 4 : 8);
-    D += blockIdx.y * sD + blockIdx.z * sD2;
-    Inv += blockIdx.y * sI + blockIdx.z * sI2;
     const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (c >= bs) return;   // wave-uniform
@@ -735,6 +885,23 @@ __global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D
         if (r < bs) Inv[(long)r + (long)c * ldi] = V{x[t].r, x[t].i};
     }
 }
+template <typename T, bool LOWER, int R>
+__global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D, long lda, long sD, int bs,
+                                                    typename cvec<T>::t *Inv, long ldi, long sI, long sD2 = 0,
+                                                    long sI2 = 0)
+{
+    trinv_wave<T, LOWER, R>(D + blockIdx.y * sD + blockIdx.z * sD2, lda, bs, Inv + blockIdx.y * sI + blockIdx.z * sI2,
+                            ldi);
+}
+// Both inverses of one LU-factored diagonal block in one launch: grid z = 0
+// the unit-lower L^-1 into Li, z = 1 the upper U^-1 into Ui.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_trinv_lu(const typename cvec<T>::t *D, long lda, int bs,
+                                                  typename cvec<T>::t *Li, typename cvec<T>::t *Ui, long ldi)
+{
+    if (blockIdx.z == 0) trinv_wave<T, true, R>(D, lda, bs, Li, ldi);
+    else trinv_wave<T, false, R>(D, lda, bs, Ui, ldi);
+}
 
 // Inverses of bs x bs triangular blocks: grid y over the matrices of a batch
 // (strides sD, sI), grid z over nblk blocks of each (strides sD2, sI2).
@@ -751,6 +918,23 @@ static int trinv(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T
         hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 4>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI, sD2, sI2);
     else
         hipLaunchKernelGGL((k_trinv_wave<T, LOWER, 8>), g, b, 0, st, D, lda, sD, bs, Inv, ldi, sI, sD2, sI2);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
+}
+
+// L^-1 and U^-1 of an LU-factored bs x bs block (one matrix), one launch.
+template <typename T>
+static int trinv_lu(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T>::t *Li, typename cvec<T>::t *Ui,
+                    long ldi, hipStream_t st)
+{
+    MIDAS_CHECK(bs <= 512, -1, "trinv: block %d above 512", bs);
+    const dim3 g((bs + 3) / 4, 1, 2), b(256);
+    if (bs <= 128)
+        hipLaunchKernelGGL((k_trinv_lu<T, 2>), g, b, 0, st, D, lda, bs, Li, Ui, ldi);
+    else if (bs <= 256)
+        hipLaunchKernelGGL((k_trinv_lu<T, 4>), g, b, 0, st, D, lda, bs, Li, Ui, ldi);
+    else
+        hipLaunchKernelGGL((k_trinv_lu<T, 8>), g, b, 0, st, D, lda, bs, Li, Ui, ldi);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
@@ -808,7 +992,15 @@ static int panel_leaf(typename cvec<T>::t *A, long lda, int n, int k, int nb, in
     for (int jb = k; jb < k + nb; jb += LU_IB) {
         int jend = min(jb + LU_IB, k + nb);
         int rc = k + nb - jend;
-        if (fused) {
+        if (!column_steps && n - jb <= LOC_T) {
+            hipLaunchKernelGGL(k_getf2_local<T>, dim3(1, w.batch), dim3(LOC_T), 0, st, A, lda, w.sA, n, jb, jend - jb,
+                               ipiv, (long)n, w.info);
+            const int nc = (jb - k) + rc;
+            if (nc > 0)
+                hipLaunchKernelGGL(k_swap_trsm<T>, dim3((nc + 63) / 64, w.batch), dim3(64), 0, st, A, lda, w.sA, ipiv,
+                                   n, jb, jend, k, k + nb);
+            MIDAS_HIP(hipGetLastError());
+        } else if (fused) {
             const int nblk = (n - jb + BLK_T - 1) / BLK_T;
             MIDAS_CHECK(nblk <= w.maxblk, -2, "lu: step workspace too small");
             // FP32: at most 64 VGPRs (8 waves per SIMD), see k_getf2_block
@@ -904,10 +1096,35 @@ static int panel_rec(typename cvec<T>::t *A, long lda, int n, int k, int nb, int
 template <typename T>
 static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ipiv, LuWork<T> &w, hipStream_t st)
 {
-    int r = panel_rec<T>(A, lda, n, k, nb, ipiv, w, st);
+    typedef typename cvec<T>::t V;
+    if (!w.diag_pivot) {
+        int r = panel_rec<T>(A, lda, n, k, nb, ipiv, w, st);
+        if (r) return r;
+        return trinv<T, true>(A + k + (long)k * lda, lda, nb, (V *)w.linv + (long)(k / w.nb) * w.nb * w.nb, w.nb, st,
+                              w.batch, w.sA, w.inv_stride());
+    }
+    // Pivot search restricted to the diagonal block: rows [k, k + nb) factored
+    // with partial pivoting among themselves (P11 A11 = L11 U11: the recursive
+    // panel of a matrix of order k + nb), the inverses of L11 (for U12) and of
+    // U11 formed, and L21 = A21 U11^-1 as one GEMM.  The multiplier of a row
+    // below the block is its candidate value over the pivot, so where every
+    // |l| <= 1/sqrt(2) (|re| + |im| is within sqrt(2) of the modulus) no row
+    // below the block would have beaten a pivot the block chose: partial
+    // pivoting over the whole column takes the same pivots.  The largest |l|
+    // goes to w.lmax for the caller to report.
+    int r = panel_rec<T>(A, lda, k + nb, k, nb, ipiv, w, st);
     if (r) return r;
-    return trinv<T, true>(A + k + (long)k * lda, lda, nb, (typename cvec<T>::t *)w.linv + (long)(k / w.nb) * w.nb * w.nb,
-                          w.nb, st, w.batch, w.sA, w.inv_stride());
+    V *D = A + k + (long)k * lda;
+    V *Ui = (V *)w.uinv + (long)(k / w.nb) * w.nb * w.nb;
+    if ((r = trinv_lu<T>(D, lda, nb, (V *)w.linv + (long)(k / w.nb) * w.nb * w.nb, Ui, w.nb, st))) return r;
+    const int mr = n - k - nb;
+    if (mr <= 0) return 0;
+    V *A21 = A + (k + nb) + (long)k * lda, *S = (V *)w.l21;
+    if ((r = launch_cgemm<T>(mr, nb, nb, cone<T>(), A21, lda, 0, Ui, w.nb, 0, czero<T>(), S, mr, st))) return r;
+    hipLaunchKernelGGL(k_copy_amax<T>, dim3(min((mr + 255) / 256, 64), min(nb, 32)), dim3(256), 0, st, S, (long)mr,
+                       A21, lda, mr, nb, w.lmax);
+    MIDAS_HIP(hipGetLastError());
+    return 0;
 }
 
 // U12(:, c0:c1) = L11^-1 A12(:, c0:c1) through a staging buffer (w.u12, or
@@ -997,6 +1214,8 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
     MIDAS_CHECK(w.batch == 1 || tg.batch == w.batch, -1, "getrf: trailing GEMM not batched like the factorisation");
     MIDAS_CHECK(w.outer == 1 || (w.outer == 2 && tgp.scratch != tg.scratch) || sizeof(T) == 8 || tg.mode == TRAIL_NATIVE,
                 -1, "getrf: two-panel steps need a panel-stream GEMM scratch of its own");
+    MIDAS_CHECK(!w.diag_pivot || (w.batch == 1 && w.l21 && w.lmax), -1,
+                "getrf: the diagonal-block pivot search needs batch 1 and its staging buffers");
     const int nbo = w.nb * (w.outer == 2 ? 2 : 1);
     // last-workgroup counters, info, the fault flag and the step tags; the
     // tagged records and words (a fill kernel: safe under stream capture)
@@ -1004,6 +1223,7 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
     TRY_LU(zero_words(w.pmax, 8 * 2 * w.maxblk * w.batch, st));
     TRY_LU(zero_words(w.cand, 16 * 2 * w.maxblk * LU_IB * w.batch, st));
     TRY_LU(zero_words(w.rowj, 16 * 2 * LU_IB * w.batch, st));
+    if (w.diag_pivot) TRY_LU(zero_words(w.lmax, sizeof(unsigned), st));
     w.tag_base = 0;
     MIDAS_HIP(hipEventRecord(w.ev_next, st));
     MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
@@ -1040,15 +1260,16 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
     }
     MIDAS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
     // inverses of the diagonal blocks of U, used by getrs (those of L come from
-    // the panels): the full blocks in one launch (they are independent), then
-    // the last, partial one
-    const int nfull = n / w.nb;
+    // the panels, and with the diagonal-block pivot search those of U too):
+    // the full blocks in one launch (they are independent), then the last,
+    // partial one
+    const int nfull = w.diag_pivot ? 0 : n / w.nb;
     if (nfull > 0) {
         r = trinv<T, false>(A, lda, w.nb, (V *)w.uinv, w.nb, st, w.batch, w.sA, w.inv_stride(), nfull,
                             (long)w.nb * (lda + 1), (long)w.nb * w.nb);
         if (r) return r;
     }
-    if (n % w.nb) {
+    if (n % w.nb && !w.diag_pivot) {
         const int k = nfull * w.nb;
         r = trinv<T, false>(A + k + (long)k * lda, lda, n - k, (V *)w.uinv + (long)nfull * w.nb * w.nb, w.nb, st,
                             w.batch, w.sA, w.inv_stride());

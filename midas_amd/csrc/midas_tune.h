@@ -16,6 +16,7 @@ enum {
     TUNE_CBFM_BLOCK_MODE,
     TUNE_CGEMM_TILE,
     TUNE_CBFM_GRAPH,
+    TUNE_LU_PIVOT_SEARCH,
     TUNE_COUNT
 };
 int tune(int which);

@@ -26,6 +26,7 @@ static TuneEntry g_tune[TUNE_COUNT] = {
     {"cbfm_block_mode", MIDAS_TUNED_CBFM_BLOCK_MODE, 0, 5},
     {"cgemm_tile", MIDAS_TUNED_CGEMM_TILE, 0, 5},
     {"cbfm_graph", MIDAS_TUNED_CBFM_GRAPH, 0, 1},
+    {"lu_pivot_search", MIDAS_TUNED_LU_PIVOT_SEARCH, 0, 2},
 };
 
 int tune(int which) { return g_tune[which].value; }

@@ -31,7 +31,8 @@ module midas_gpu_c
     type, bind(c) :: midas_solve_info_t
         integer(c_int) :: info, iterations
         real(c_double) :: berr, anorm, t_factor_ms, t_solve_ms, t_refine_ms
-        integer(c_int) :: status
+        integer(c_int) :: status, pivot_search
+        real(c_double) :: max_l21
     end type midas_solve_info_t
 
     type, bind(c) :: midas_blocks_t

@@ -196,6 +196,87 @@ def test_two_panel_steps(mid, mode, n):
     assert _rel(X2, X1) < 1e-10
 
 
+@pytest.mark.parametrize("mode", [api.SOLVE_F16_IR, api.SOLVE_C32_IR, api.SOLVE_BF16X3_IR])
+@pytest.mark.parametrize("n", [1300, 2600])
+def test_diag_block_pivot_search(mid, mode, n):
+    """lu_pivot_search = 2: every panel searches its pivots in its diagonal
+    block only and forms L21 = A21 U11^-1 by a GEMM.  On a diagonally weighted
+    matrix, where LAPACK's partial pivoting makes no interchange, the reported
+    max |L21| certifies the same pivots (<= 1/sqrt 2) and the solution equals
+    the whole-column search's and LAPACK's (one-panel steps, and for the
+    single-term split mode two-panel steps with a ragged last step)."""
+    import scipy.linalg as sla
+    rng = np.random.default_rng(50 + n)
+    _case(f"diagonal-block pivot search {api.MODE_NAMES[mode]} n={n}")
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 2.5 * np.sqrt(2 * n) * np.eye(n)
+    B = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    _, piv = sla.lu_factor(A)
+    assert np.array_equal(piv, np.arange(n))   # partial pivoting keeps the diagonal
+    nb = 1024 if mode == api.SOLVE_F16_IR else 512
+    olds = {"lu_pivot_search": api.tune_set("lu_pivot_search", 2),
+            "lu_two_panel_min_n": api.tune_set("lu_two_panel_min_n", 0)}
+    try:
+        X2, i2 = mid.solve(A, B, mode, nb=nb)
+        api.tune_set("lu_pivot_search", 0)
+        X0, i0 = mid.solve(A, B, mode, nb=nb)
+    finally:
+        for k, v in olds.items():
+            api.tune_set(k, v)
+    Xo = np.linalg.solve(A, B)
+    print("diagonal-block search", i2, "whole columns", i0, "vs numpy", _rel(X2, Xo))
+    assert i2["pivot_search"] == 1 and i0["pivot_search"] == 0
+    assert 0.0 < i2["max_l21"] <= 2 ** -0.5
+    assert i2["status"] == api.STATUS_CONVERGED and i2["info"] == 0 and i2["berr"] <= 1e-14
+    assert _rel(X2, Xo) < 1e-10
+    assert _rel(X2, X0) < 1e-10
+
+
+def test_diag_block_pivot_search_uncertified(mid):
+    """A matrix that needs interchanges across diagonal blocks (plain random):
+    the diagonal-block search reports max |L21| above 1/sqrt 2, and the solve
+    still returns LAPACK's solution (refinement, or its FP64 fallback)."""
+    rng = np.random.default_rng(61)
+    n = 1500
+    _case(f"diagonal-block pivot search, uncertified, c32 n={n}")
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    B = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    old = api.tune_set("lu_pivot_search", 2)
+    try:
+        X, info = mid.solve(A, B, api.SOLVE_C32_IR, nb=256)
+    finally:
+        api.tune_set("lu_pivot_search", old)
+    print(info, "vs numpy", _rel(X, np.linalg.solve(A, B)))
+    assert info["pivot_search"] == 1 and info["max_l21"] > 2 ** -0.5
+    assert info["status"] in (api.STATUS_CONVERGED, api.STATUS_FP64_FALLBACK)
+    assert _rel(X, np.linalg.solve(A, B)) < 1e-9
+
+
+@pytest.mark.parametrize("mode", [api.SOLVE_F16_IR, api.SOLVE_C32_IR])
+def test_mom_diag_block_pivot_search(mid, mode):
+    """The MoM solve's default (lu_pivot_search = 1): the internal field of a
+    Gaussian random sphere equals the whole-column search's, the certificate
+    holds (the MoM matrix keeps its pivots on the diagonal), with two-panel
+    steps forced on."""
+    s = g.gaussian_random_sphere(size_param=3.0, target_cells=1400, eps=1.5 + 0.01j, seed=7)
+    _case(f"MoM diagonal-block pivot search {api.MODE_NAMES[mode]} cells={s.n}")
+    tth, tph = np.array([0.0, 40.0]), np.array([0.0, 30.0])
+    rx = (np.array([0.0, 90.0]), np.array([0.0, 0.0]))
+    olds = {"lu_pivot_search": api.tune_set("lu_pivot_search", 1),
+            "lu_two_panel_min_n": api.tune_set("lu_two_panel_min_n", 0)}
+    try:
+        _, _, _, E1, i1 = mid.compute_efields_mom(s, s.k0, (tth, tph), rx, mode=mode)
+        api.tune_set("lu_pivot_search", 0)
+        _, _, _, E0, i0 = mid.compute_efields_mom(s, s.k0, (tth, tph), rx, mode=mode)
+    finally:
+        for k, v in olds.items():
+            api.tune_set(k, v)
+    print("diagonal-block", i1, "whole columns", i0, "field diff", _rel(E1, E0))
+    assert i1["pivot_search"] == 1 and i0["pivot_search"] == 0
+    assert 0.0 < i1["max_l21"] <= 2 ** -0.5
+    assert i1["status"] == api.STATUS_CONVERGED and i1["berr"] <= 1e-14
+    assert _rel(E1, E0) < 1e-12
+
+
 @pytest.mark.parametrize("nrhs", [2, 8])
 def test_refinement_graph_replays_match_eager(mid, oracle, nrhs):
     """The refinement sweeps replayed from one captured hipGraph (several
