@@ -483,13 +483,16 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
 // row's entries) in an LDS slot, next to row j's entries; after the barrier
 // every thread picks the winner among the waves' slots and reads the pivot
 // row from it.  Slots alternate by column parity, so a slot is rewritten only
-// after every thread has passed the next column's barrier.  It takes the inner
-// blocks of the last rows of a factorisation and the diagonal blocks of
-// panels whose pivot search is restricted to them.
+// after every thread has passed the next column's barrier.  Then, as
+// k_swap_trsm would (the same workgroup, no launch): the block's interchanges
+// on the panel's columns [k, jb) and [jend, k1), and the unit-lower solve with
+// L11 (from LDS) on [jend, k1).  It takes the inner blocks of the last rows of
+// a factorisation and the diagonal blocks of panels whose pivot search is
+// restricted to them.
 constexpr int LOC_T = 512;
 template <typename T>
 __global__ __launch_bounds__(LOC_T) void k_getf2_local(typename cvec<T>::t *A, long lda, long sA, int rend, int jb,
-                                                       int ncols, int *ipiv, long sP, int *info)
+                                                       int ncols, int *ipiv, long sP, int *info, int k, int k1)
 {
     typedef typename cvec<T>::t V;
     constexpr int NW = LOC_T / 64;
@@ -497,6 +500,9 @@ __global__ __launch_bounds__(LOC_T) void k_getf2_local(typename cvec<T>::t *A, l
     __shared__ V jrow[2][LU_IB];       // row j
     __shared__ T cval[2][NW];
     __shared__ int cidx[2][NW];
+    __shared__ V l11[LU_IB][LU_IB];    // the block's diagonal 16 x 16, [row][column]
+    __shared__ int piv[LU_IB], pos[2 * LU_IB], src[2 * LU_IB];
+    __shared__ int npair;
     const int tid = threadIdx.x, zb = blockIdx.y, wv = tid >> 6, lane = tid & 63;
     A += zb * sA;
     ipiv += zb * sP;
@@ -550,7 +556,10 @@ __global__ __launch_bounds__(LOC_T) void k_getf2_local(typename cvec<T>::t *A, l
                 }
             const int p = __builtin_amdgcn_readfirstlane(gi == 0x7fffffff ? j : gi);
             const V *urow = gi == 0x7fffffff ? jrow[par] : crow[par][__builtin_amdgcn_readfirstlane(gw)];
-            if (tid == 0) ipiv[j] = p;
+            if (tid == 0) {
+                ipiv[j] = p;
+                piv[c] = p;
+            }
             if (p != j) {
                 if (i == j) {
 #pragma unroll
@@ -585,6 +594,66 @@ __global__ __launch_bounds__(LOC_T) void k_getf2_local(typename cvec<T>::t *A, l
 #pragma unroll
         for (int t = 0; t < LU_IB; ++t)
             if (t < ncols) Ab[(long)t * lda + tid] = a[t];
+    }
+    const int jend = jb + ncols, nl = jb - k, nr = k1 - jend;
+    if (nl + nr <= 0) return;   // workgroup-uniform
+    if (tid < ncols) {
+#pragma unroll
+        for (int t = 0; t < LU_IB; ++t) l11[tid][t] = a[t];
+    }
+    stores_done();
+    __syncthreads();
+    // the interchanges composed into at most 2 LU_IB (position, source) moves
+    if (tid == 0) {
+        int m = 0;
+        for (int j = jb; j < jend; ++j) {
+            const int p = piv[j - jb];
+            if (p == j) continue;
+            int sj = -1, sp = -1;
+            for (int s = 0; s < m; ++s) {
+                if (pos[s] == j) sj = s;
+                if (pos[s] == p) sp = s;
+            }
+            if (sj < 0) { pos[m] = j; src[m] = j; sj = m++; }
+            if (sp < 0) { pos[m] = p; src[m] = p; sp = m++; }
+            const int t = src[sj];
+            src[sj] = src[sp];
+            src[sp] = t;
+        }
+        npair = m;
+    }
+    __syncthreads();
+    const int m = npair;
+    for (int t = tid; t < nl + nr; t += LOC_T) {
+        const int c = t < nl ? k + t : jend + (t - nl);
+        V *col = A + (long)c * lda;
+        if (m > 0) {
+            cplx<T> v[2 * LU_IB];
+#pragma unroll
+            for (int s = 0; s < 2 * LU_IB; ++s)
+                if (s < m) {
+                    const V x = col[src[s]];
+                    v[s] = cplx<T>{x.x, x.y};
+                }
+#pragma unroll
+            for (int s = 0; s < 2 * LU_IB; ++s)
+                if (s < m) col[pos[s]] = V{v[s].r, v[s].i};
+            stores_done();
+        }
+        if (c < jend) continue;
+        cplx<T> x[LU_IB];
+#pragma unroll
+        for (int r = 0; r < LU_IB; ++r)
+            if (r < ncols) x[r] = cplx<T>{col[jb + r].x, col[jb + r].y};
+#pragma unroll
+        for (int r = 1; r < LU_IB; ++r) {
+            if (r >= ncols) continue;
+#pragma unroll
+            for (int q = 0; q < r; ++q) x[r] = csub(x[r], cmul(cplx<T>{l11[r][q].x, l11[r][q].y}, x[q]));
+        }
+#pragma unroll
+        for (int r = 1; r < LU_IB; ++r)
+            if (r < ncols) col[jb + r] = V{x[r].r, x[r].i};
     }
 }
 
@@ -993,12 +1062,9 @@ static int panel_leaf(typename cvec<T>::t *A, long lda, int n, int k, int nb, in
         int jend = min(jb + LU_IB, k + nb);
         int rc = k + nb - jend;
         if (!column_steps && n - jb <= LOC_T) {
+            // (its interchanges and U12 solve on the other panel columns included)
             hipLaunchKernelGGL(k_getf2_local<T>, dim3(1, w.batch), dim3(LOC_T), 0, st, A, lda, w.sA, n, jb, jend - jb,
-                               ipiv, (long)n, w.info);
-            const int nc = (jb - k) + rc;
-            if (nc > 0)
-                hipLaunchKernelGGL(k_swap_trsm<T>, dim3((nc + 63) / 64, w.batch), dim3(64), 0, st, A, lda, w.sA, ipiv,
-                                   n, jb, jend, k, k + nb);
+                               ipiv, (long)n, w.info, k, k + nb);
             MIDAS_HIP(hipGetLastError());
         } else if (fused) {
             const int nblk = (n - jb + BLK_T - 1) / BLK_T;
