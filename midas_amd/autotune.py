@@ -53,6 +53,7 @@ COMMENTS = {
     "cgemm_tile": "tile shape of the complex-FP32 GEMM without transposes (5: the 64x64 k_cgemm)",
     "cbfm_graph": "CBFM block batches (fills, excitations, FP64 LU, solve) launched as one hipGraph each",
     "lu_pivot_search": "mixed LU pivot search in each panel's diagonal block (certified): 1 MoM solves, 2 all; 0 whole columns",
+    "mom_matrix_free": "MoM mixed solves (<= 2 directions): FP64 residuals from the cells, the FP64 matrix not stored",
 }
 
 

@@ -202,7 +202,9 @@ static int residual_launch(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, const d
     double2 *R = ws.R.as<double2>();
     const dim3 g2((n + 255) / 256, nrhs);
     int r;
-    if (nrhs <= 4) {
+    if (ws.mf_active && nrhs <= 4) {
+        r = launch_residual_free(*ws.mf_cells, n / 3, ws.mf_k0, ws.mf_pi, X, ldx, nrhs, B, ldb, R, n, ws.mfs.p, st);
+    } else if (nrhs <= 4) {
         r = launch_residual_few(Z, ldz, n, X, ldx, nrhs, B, ldb, R, n, ws.zx.p, st);
     } else {
         hipLaunchKernelGGL(k_copyz, g2, dim3(256), 0, st, B, ldb, R, (long)n, n, nrhs);
@@ -306,6 +308,12 @@ static int sweep_capture(midas_ctx *ctx, SolveWs &ws, const SweepArgs &a, SweepG
 // only for the FP64 fallback, which z_inplace allows.  X (n x nrhs, ldx)
 // receives the solution; B (n x nrhs, ldb) is read only.  Work buffers live in
 // ctx->ws and only grow.
+// Clears the matrix-free flag when a solve ends, however it ends.
+struct MfOff {
+    SolveWs &ws;
+    ~MfOff() { ws.mf_active = false; }
+};
+
 int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32, long ldw, const double2 *B,
               long ldb, double2 *X, long ldx, int mode, const midas_solve_opts_t *opts, midas_solve_info_t *info,
               bool z_inplace, SolveWs *wsp)
@@ -327,9 +335,18 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
     double *scal = rowsum + n + 2 * nrhs;
 
     // ||Z||_inf before factoring (PZLANGE 'I'); the scratch also serves the
-    // few-RHS residuals of refinement
-    TRY(ws.zx.reserve(zx_scratch_bytes(n, nrhs <= 4 ? nrhs : 1)));
-    TRY(launch_rowabs(Z, ldz, n, rowsum, ws.zx.p, st));
+    // few-RHS residuals of refinement.  Matrix-free (MoM mixed solves): both
+    // from the cells, Z untouched until a fallback fills it.
+    ws.mf_active = ws.mf_cells && mode != MIDAS_SOLVE_FP64 && nrhs <= 4 && !ws.lent && 3 * ws.mf_cells->n == n;
+    MfOff mf_off{ws};
+    if (ws.mf_active) {
+        TRY(ws.mfs.reserve(mf_scratch_bytes(n / 3, nrhs)));
+        TRY(launch_rowabs_free(*ws.mf_cells, n / 3, ws.mf_k0, ws.mf_pi, rowsum, ws.mfs.p, st));
+    } else {
+        MIDAS_CHECK(Z, -1, "solve: no FP64 matrix");
+        TRY(ws.zx.reserve(zx_scratch_bytes(n, nrhs <= 4 ? nrhs : 1)));
+        TRY(launch_rowabs(Z, ldz, n, rowsum, ws.zx.p, st));
+    }
     hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, st, rowsum, n, scal);
     MIDAS_HIP(hipGetLastError());
     TRY(copy_sync(&inf.anorm, scal, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -470,7 +487,17 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
         inf.status = MIDAS_STATUS_CONVERGED;
     } else if (fallback && inf.info == 0) {
         // refinement cannot reach tol with this factorisation (ZCGESV's rule):
-        // the FP64 LU of Z, in place
+        // the FP64 LU of Z, in place (matrix-free: Z filled first, into the
+        // context's buffer when the caller gave none)
+        if (ws.mf_active) {
+            if (!Z) {
+                TRY(ws.zcopy.reserve(sizeof(double2) * (size_t)n * n));
+                Z = ws.zcopy.as<double2>();
+                ldz = n;
+            }
+            const CellSet all{n / 3, 0, nullptr};
+            TRY(launch_fill(*ws.mf_cells, all, all, ws.mf_k0, ws.mf_quirks, false, Z, ldz, nullptr, 0, st));
+        }
         r = fp64_lu_solve(ctx, ws, n, nrhs, Z, ldz, B, ldb, X, ldx, default_nb(n, MIDAS_SOLVE_FP64), inf);
         if (r) return r;
         inf.status = MIDAS_STATUS_FP64_FALLBACK;
@@ -963,12 +990,21 @@ int midas_scattering_quantities(midas_ctx *ctx, const double *S, long lds, int n
 
 // Fill (FP64 Z, and the FP32 copy W for mixed modes) + RHS + solve, all on
 // the device; X (3n x 2ntr).  Shared by the drop-in and the device entry point.
+// The MoM mixed solves with few right-hand sides take their FP64 residuals
+// from the cells (the FP64 matrix is not stored, only its FP32 copy).
+static bool mom_matrix_free(const SolveWs &ws, int mode, int ntr)
+{
+    return mode != MIDAS_SOLVE_FP64 && 2 * ntr <= 4 && !ws.lent && tune(TUNE_MOM_MATRIX_FREE);
+}
+
 static int mom_solve_dev(midas_ctx *ctx, SolveWs &ws, const DevCells &cells, double k0, int quirks, int ntr,
                          const double *dth, const double *dph, int mode, const midas_solve_opts_t *opts, double2 *Z,
                          long ldz, double2 *X, long ldx, midas_solve_info_t *info, double *t_fill_ms)
 {
     hipStream_t st = ctx->st;
     const int n = cells.n, n3 = 3 * n;
+    const bool mf = mom_matrix_free(ws, mode, ntr);
+    MIDAS_CHECK(Z || mf, -1, "mom_solve: no FP64 matrix buffer");
     float2 *w32 = nullptr;
     if (mode != MIDAS_SOLVE_FP64) {
         TRY(ws.W.reserve(sizeof(float2) * (size_t)n3 * n3));
@@ -976,10 +1012,17 @@ static int mom_solve_dev(midas_ctx *ctx, SolveWs &ws, const DevCells &cells, dou
     }
     TRY(ws.V.reserve(sizeof(double2) * (size_t)n3 * 2 * ntr));
     Timer tm(st, !ws.lent);
-    TRY(launch_fill(cells, CellSet{n, 0, nullptr}, CellSet{n, 0, nullptr}, k0, quirks, false, Z, ldz, w32, n3, st));
+    TRY(launch_fill(cells, CellSet{n, 0, nullptr}, CellSet{n, 0, nullptr}, k0, quirks, false, mf ? nullptr : Z, ldz,
+                    w32, n3, st));
     TRY(launch_rhs(cells, CellSet{n, 0, nullptr}, ntr, dth, dph, k0, quirks, false, ws.V.as<double2>(), n3, st));
     double tf = tm.ms();
     if (t_fill_ms) *t_fill_ms = tf;
+    if (mf) {
+        ws.mf_cells = &cells;
+        ws.mf_k0 = k0;
+        ws.mf_pi = host_pi(quirks);
+        ws.mf_quirks = quirks;
+    }
     // The MoM matrix (1 - Znnpp on the diagonal, Green's couplings scaled by
     // the cells' contrast off it) takes its pivots on the diagonal under
     // partial pivoting (LAPACK's zgetrf makes no interchange on the dielectric
@@ -990,6 +1033,7 @@ static int mom_solve_dev(midas_ctx *ctx, SolveWs &ws, const DevCells &cells, dou
     const int r = solve_dev(ctx, n3, 2 * ntr, Z, ldz, w32, n3, ws.V.as<double2>(), n3, X, ldx, mode, opts, info, true,
                             &ws);
     ws.pivot_search = 0;
+    ws.mf_cells = nullptr;
     return r;
 }
 
@@ -1005,7 +1049,8 @@ int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double
     DevCellsOwned dc;
     TRY(dc.from_host(cells, k0, quirks, st));
     DBuf Z, X, dth, dph;
-    TRY(Z.alloc(sizeof(double2) * (size_t)n3 * n3));
+    // (a matrix-free mixed solve needs no FP64 matrix unless it falls back)
+    if (!mom_matrix_free(ctx->ws, mode, ntr)) TRY(Z.alloc(sizeof(double2) * (size_t)n3 * n3));
     TRY(X.alloc(sizeof(double2) * (size_t)n3 * 2 * ntr));
     TRY(upload(dth, tr_th, sizeof(double) * ntr, st));
     TRY(upload(dph, tr_ph, sizeof(double) * ntr, st));

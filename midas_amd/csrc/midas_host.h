@@ -176,12 +176,20 @@ struct SolveWs {
     DBuf V;            // excitation of the MoM solve
     DBuf l21;          // diagonal-block pivot search: L21 staging (n x nb) and max |L21|
     int pivot_search = 0;   // the next mixed solve restricts its pivot search to diagonal blocks (MoM)
+    // matrix-free FP64 residuals (MoM mixed solves, <= 4 right-hand sides): the
+    // cells the matrix is recomputed from; the FP64 matrix is then filled only
+    // for the FP64 fallback
+    const DevCells *mf_cells = nullptr;
+    double mf_k0 = 0.0, mf_pi = 0.0;
+    int mf_quirks = 0;
+    bool mf_active = false;   // set by the solve while it uses them
+    DBuf mfs;                 // their per-chunk partial sums
     LuWork<float> luf;
     LuWork<double> lud;
     bool lent = false; // every buffer lent from a fixed scratch: nothing may be allocated
     void release()
     {
-        DBuf *b[] = {&W, &trail, &trailp, &ipiv, &perm, &norms, &zx, &R, &D, &t2, &xbest, &zcopy, &V, &l21};
+        DBuf *b[] = {&W, &trail, &trailp, &ipiv, &perm, &norms, &zx, &R, &D, &t2, &xbest, &zcopy, &V, &l21, &mfs};
         for (DBuf *x : b) x->release();
         luf.release();
         lud.release();
@@ -242,6 +250,12 @@ size_t zx_scratch_bytes(int n, int nrhs);
 int launch_residual_few(const double2 *Z, long ldz, int n, const double2 *X, long ldx, int nrhs, const double2 *B,
                         long ldb, double2 *R, long ldr, void *scratch, hipStream_t st);
 int launch_rowabs(const double2 *Z, long ldz, int n, double *rowsum, void *scratch, hipStream_t st);
+// the same two from the cells (the MoM matrix recomputed, never stored)
+size_t mf_scratch_bytes(int ncells, int nrhs);
+int launch_residual_free(const DevCells &cells, int nc, double k0, double pi, const double2 *X, long ldx, int nrhs,
+                         const double2 *B, long ldb, double2 *R, long ldr, void *scratch, hipStream_t st);
+int launch_rowabs_free(const DevCells &cells, int nc, double k0, double pi, double *rowsum, void *scratch,
+                       hipStream_t st);
 }  // namespace midas
 
 // Dense solve on the device (FP64 LU or mixed-precision LU + FP64 refinement):

@@ -17,6 +17,7 @@ enum {
     TUNE_CGEMM_TILE,
     TUNE_CBFM_GRAPH,
     TUNE_LU_PIVOT_SEARCH,
+    TUNE_MOM_MATRIX_FREE,
     TUNE_COUNT
 };
 int tune(int which);

@@ -277,6 +277,52 @@ def test_mom_diag_block_pivot_search(mid, mode):
     assert _rel(E1, E0) < 1e-12
 
 
+@pytest.mark.parametrize("mode", [api.SOLVE_F16_IR, api.SOLVE_C32_IR])
+@pytest.mark.parametrize("ntr", [1, 2])
+def test_mom_matrix_free_residuals(mid, mode, ntr):
+    """MoM mixed solves with few right-hand sides take ||Z||_inf and the FP64
+    residuals of refinement from the cells (mom_matrix_free = 1, no FP64
+    matrix stored): same norm, same sweeps and the same field as with the
+    stored matrix."""
+    s = g.gaussian_random_sphere(size_param=3.0, target_cells=1200, eps=1.7 + 0.05j, seed=11)
+    _case(f"MoM matrix-free residuals {api.MODE_NAMES[mode]} ntr={ntr} cells={s.n}")
+    tth, tph = np.array([0.0, 50.0])[:ntr], np.array([0.0, 20.0])[:ntr]
+    rx = (np.array([0.0, 90.0]), np.array([0.0, 0.0]))
+    old = api.tune_set("mom_matrix_free", 1)
+    try:
+        _, ce1, _, E1, i1 = mid.compute_efields_mom(s, s.k0, (tth, tph), rx, mode=mode)
+        api.tune_set("mom_matrix_free", 0)
+        _, ce0, _, E0, i0 = mid.compute_efields_mom(s, s.k0, (tth, tph), rx, mode=mode)
+    finally:
+        api.tune_set("mom_matrix_free", old)
+    print("matrix-free", i1, "stored", i0, "field diff", _rel(E1, E0))
+    assert abs(i1["anorm"] - i0["anorm"]) <= 1e-13 * i0["anorm"]
+    assert i1["status"] == i0["status"] == api.STATUS_CONVERGED
+    assert i1["iterations"] == i0["iterations"] and i1["berr"] <= 1e-14
+    assert _rel(E1, E0) < 1e-13
+    assert _rel(ce1, ce0) < 1e-13
+
+
+def test_mom_matrix_free_fallback(mid):
+    """A matrix-free mixed solve that cannot reach its tolerance (one sweep,
+    tol 1e-30) fills the FP64 matrix only then and returns the FP64 LU's
+    field, as the stored-matrix solve does."""
+    s = g.gaussian_random_sphere(size_param=3.0, target_cells=900, eps=1.5 + 0.01j, seed=12)
+    _case(f"MoM matrix-free fallback cells={s.n}")
+    tth, tph = np.array([0.0]), np.array([0.0])
+    rx = (np.array([0.0]), np.array([0.0]))
+    old = api.tune_set("mom_matrix_free", 1)
+    try:
+        _, _, _, Ef, inf = mid.compute_efields_mom(s, s.k0, (tth, tph), rx, mode=api.SOLVE_F16_IR, max_iter=1,
+                                                   tol=1e-30)
+    finally:
+        api.tune_set("mom_matrix_free", old)
+    _, _, _, E64, _ = mid.compute_efields_mom(s, s.k0, (tth, tph), rx, mode=api.SOLVE_FP64)
+    print(inf, "vs FP64", _rel(Ef, E64))
+    assert inf["status"] == api.STATUS_FP64_FALLBACK
+    assert _rel(Ef, E64) < 1e-13
+
+
 @pytest.mark.parametrize("nrhs", [2, 8])
 def test_refinement_graph_replays_match_eager(mid, oracle, nrhs):
     """The refinement sweeps replayed from one captured hipGraph (several
