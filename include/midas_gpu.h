@@ -118,7 +118,12 @@ void *midas_stream(midas_ctx *ctx);           /* hipStream_t of the context */
  * "cbfm_graph" (1: each batch of CBF block systems -- grouped fills, SR masks,
  * excitations, batched FP64 LU and solve -- recorded into a hipGraph and
  * launched as one, the executable graph kept and updated in place by the next
- * call), "zgemm_tile", "cgemm_tile", "fp64_gemm_library", "cbfm_block_mode".
+ * call), "zgemm_tile", "cgemm_tile", "fp64_gemm_library", "cbfm_block_mode",
+ * "lu_pivot_search" (mixed LU: 1 the MoM solves search each panel's pivots in
+ * its diagonal block, certified by max |L21| in the report; 2 every solve;
+ * 0 whole columns), "mom_matrix_free" (1: MoM mixed solves with at most two
+ * directions recompute the FP64 matrix for their residuals instead of storing
+ * it).
  * They change speed, not results beyond rounding.  No reference counterpart
  * (the reference fixes its 32 x 32 ScaLAPACK blocks, Initialization.f90:66-67). */
 int midas_tune_set(const char *name, int value);
@@ -349,9 +354,11 @@ int midas_extend_blocks(const midas_cells_t *cells, int nblocks, const int *bloc
 
 /* ---------------- device-pointer variants (inputs resident in HBM) ----------------
  * Same semantics as above; the pointers named below are device memory; ordered
- * on midas_stream(ctx).  midas_dev_mom_solve fills Z (FP64, the refinement's
- * residual matrix) and overwrites it with its LU factors in FP64 mode and in
- * the FP64 fallback. */
+ * on midas_stream(ctx).  midas_dev_mom_solve fills Z (FP64) and, in FP64 mode
+ * and in the FP64 fallback, overwrites it with its LU factors.  A mixed solve
+ * with at most two directions (schedule parameter "mom_matrix_free", on by
+ * default) computes its FP64 residuals and ||Z||_inf from the cells instead
+ * and leaves Z untouched unless the fallback runs. */
 int midas_dev_mom_fill(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, double *Z, long ldz,
                        float *W32, long ldw);
 int midas_dev_mom_solve(midas_ctx *ctx, const midas_cells_t *dcells, double k0, int quirks, int ntr,
