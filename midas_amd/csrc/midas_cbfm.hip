@@ -198,7 +198,7 @@ __global__ void k_blk_colmax3(const double2 *R, const double2 *X, const double2 
 }
 
 struct BlockMixedWs {
-    DBuf W, E, R, D, tmp, scr, rs, anorm, cm;
+    DBuf W, E, R, D, tmp, scr, rs, anorm, cm, l21;
 };
 
 // X (nc blocks of mpc x T2c at stride mpc T2c, ld mpc) holds E on entry and
@@ -231,6 +231,16 @@ static int block_solve_mixed(int mode, int nc, int mpc, int T2c, const double2 *
     if (tg.mode != TRAIL_NATIVE) {
         TRY(bw.scr.reserve(trail16_batch_bytes(mpc, mpc, nb, nc)));
         tg.scratch = bw.scr.p;
+    }
+    // the block systems are MoM matrices of the extended blocks: the pivot
+    // search stays in the panels' diagonal blocks (refinement and the FP64
+    // fallback below keep each block's answer whatever the pivots)
+    w.diag_pivot = tune(TUNE_LU_PIVOT_SEARCH) >= 1;
+    if (w.diag_pivot) {
+        const size_t sl = sizeof(float2) * (size_t)mpc * nb * nc;
+        TRY(bw.l21.reserve(sl + 256));
+        w.l21 = bw.l21.as<c32>();
+        w.lmax = (unsigned *)((char *)bw.l21.p + sl);
     }
     TRY(getrf<float>(bw.W.as<float2>(), mpc, mpc, ipiv, w, st, tg));
     TRY(ipiv_to_perm_dev(ipiv, perm, mpc, nc, st));
@@ -745,7 +755,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         TRY(Xc.alloc(sizeof(double2) * xmax));
         MIDAS_HIP(hipMemsetAsync(Xall.p, 0, sizeof(double2) * mp * T2c * nb, st));
         std::vector<int> hinfo(smax_c);
-        DBuf djob;
+        DBuf djob, l21f;
         TRY(djob.alloc(sizeof(BlockJob) * smax_c));
         const BlockJob *jd = djob.as<BlockJob>();
         const int *idx = didx.as<int>();
@@ -788,6 +798,16 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                 w.defer_fault_check = true;   // checked after the batch (no synchronisation inside a graph)
                 tg.batch = nc;
                 tg.stride = mpc * mpc;
+                // pivot search in the panels' diagonal blocks, kept only where
+                // certified (max |L21| <= 1/sqrt 2: partial pivoting's pivots);
+                // otherwise the batch is built and factored again below
+                w.diag_pivot = tune(TUNE_LU_PIVOT_SEARCH) >= 1;
+                if (w.diag_pivot) {
+                    const size_t sl = sizeof(double2) * (size_t)mpc * nb_lu * nc;
+                    TRY(l21f.reserve(sl + 256));
+                    w.l21 = l21f.as<z64>();
+                    w.lmax = (unsigned *)((char *)l21f.p + sl);
+                }
             }
             double2 *Z = Zall.as<double2>(), *X = Xc.as<double2>();
             // One batch: grouped block fills, SR masks, padding, excitations;
@@ -818,6 +838,16 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                 TRY(enqueue());
             if (fp64) {
                 TRY(lu_fault_check<double>(w, st));
+                if (w.diag_pivot) {
+                    unsigned lm = 0;
+                    TRY(copy_sync(&lm, w.lmax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+                    if (!((double)__builtin_bit_cast(float, lm) <= 0.70710678)) {
+                        // not certified: the batch again, with the whole-column search
+                        w.diag_pivot = false;
+                        TRY(enqueue());
+                        TRY(lu_fault_check<double>(w, st));
+                    }
+                }
                 TRY(copy_sync(hinfo.data(), w.info, sizeof(int) * nc, hipMemcpyDeviceToHost, st));
             } else {
                 TRY(block_solve_mixed(block_mode, nc, (int)mpc, T2c, Z, X, ipiv.as<int>(), perm.as<int>(), hinfo, bmw,

@@ -31,8 +31,9 @@ template <typename T> struct LuWork {
     size_t cap[10] = {};              // bytes held by each buffer above (grow-only)
     bool borrowed = false;           // buffers, stream and events lent by the caller (lend())
     bool defer_fault_check = false;  // getrf leaves the wait-fault check to lu_fault_check (stream capture)
-    // pivot search restricted to each panel's diagonal block (batch 1): L21 =
-    // A21 U11^-1 staged in l21 (n x nb), the largest |L21| in *lmax (float bits)
+    // pivot search restricted to each panel's diagonal block: L21 = A21 U11^-1
+    // staged in l21 (n x nb per matrix of the batch), the largest |L21| of the
+    // batch in *lmax (float bits)
     bool diag_pivot = false;
     cplx<T> *l21 = nullptr;
     unsigned *lmax = nullptr;

@@ -661,10 +661,12 @@ __global__ __launch_bounds__(LOC_T) void k_getf2_local(typename cvec<T>::t *A, l
 // the largest modulus of its entries into *amax (float bits, atomic max: the
 // order of non-negative floats is that of their bits).
 template <typename T>
-__global__ __launch_bounds__(256) void k_copy_amax(const typename cvec<T>::t *S, long lds,
-                                                   typename cvec<T>::t *D, long ldd, int rows, int cols,
+__global__ __launch_bounds__(256) void k_copy_amax(const typename cvec<T>::t *S, long lds, long sS,
+                                                   typename cvec<T>::t *D, long ldd, long sD, int rows, int cols,
                                                    unsigned *amax)
 {
+    S += blockIdx.z * sS;   // matrix of a batch
+    D += blockIdx.z * sD;
     // a few thousand workgroups, each over a share of rows and columns, so
     // that few atomics meet on *amax
     __shared__ float wm[4];
@@ -844,13 +846,13 @@ __global__ void k_trsm_lunit_small(typename cvec<T>::t *A, long lda, long sA, in
 // rule with a pivot search that keeps the diagonal).
 constexpr int LASWP_LDS = 1024;
 template <typename T>
-__global__ __launch_bounds__(256) void k_laswp(typename cvec<T>::t *A, long lda, long sA, const int *ipiv, int n,
+__global__ __launch_bounds__(256) void k_laswp(typename cvec<T>::t *A, long lda, long sA, const int *ipiv, long sP,
                                                int k0, int k1, int c0, int c1, int c2, int c3)
 {
     typedef typename cvec<T>::t V;
     __shared__ int pv[LASWP_LDS];
     A += blockIdx.y * sA;
-    ipiv += (long)blockIdx.y * n;
+    ipiv += blockIdx.y * sP;
     int moved = 0;
     for (int j = threadIdx.x; j < k1 - k0; j += blockDim.x) {
         const int p = ipiv[k0 + j];
@@ -965,11 +967,12 @@ __global__ __launch_bounds__(256) void k_trinv_wave(const typename cvec<T>::t *D
 // Both inverses of one LU-factored diagonal block in one launch: grid z = 0
 // the unit-lower L^-1 into Li, z = 1 the upper U^-1 into Ui.
 template <typename T, int R>
-__global__ __launch_bounds__(256) void k_trinv_lu(const typename cvec<T>::t *D, long lda, int bs,
-                                                  typename cvec<T>::t *Li, typename cvec<T>::t *Ui, long ldi)
+__global__ __launch_bounds__(256) void k_trinv_lu(const typename cvec<T>::t *D, long lda, long sD, int bs,
+                                                  typename cvec<T>::t *Li, typename cvec<T>::t *Ui, long ldi, long sI)
 {
-    if (blockIdx.z == 0) trinv_wave<T, true, R>(D, lda, bs, Li, ldi);
-    else trinv_wave<T, false, R>(D, lda, bs, Ui, ldi);
+    D += blockIdx.y * sD;
+    if (blockIdx.z == 0) trinv_wave<T, true, R>(D, lda, bs, Li + blockIdx.y * sI, ldi);
+    else trinv_wave<T, false, R>(D, lda, bs, Ui + blockIdx.y * sI, ldi);
 }
 
 // Inverses of bs x bs triangular blocks: grid y over the matrices of a batch
@@ -991,19 +994,20 @@ static int trinv(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T
     return 0;
 }
 
-// L^-1 and U^-1 of an LU-factored bs x bs block (one matrix), one launch.
+// L^-1 and U^-1 of an LU-factored bs x bs block, one launch for a strided
+// batch of matrices (sD, sI).
 template <typename T>
 static int trinv_lu(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T>::t *Li, typename cvec<T>::t *Ui,
-                    long ldi, hipStream_t st)
+                    long ldi, hipStream_t st, int batch = 1, long sD = 0, long sI = 0)
 {
-    MIDAS_CHECK(bs <= 512, -1, "trinv: block %d above 512", bs);
-    const dim3 g((bs + 3) / 4, 1, 2), b(256);
+    MIDAS_CHECK(bs <= 512 && batch <= 65535, -1, "trinv: block %d above 512 or batch %d", bs, batch);
+    const dim3 g((bs + 3) / 4, batch, 2), b(256);
     if (bs <= 128)
-        hipLaunchKernelGGL((k_trinv_lu<T, 2>), g, b, 0, st, D, lda, bs, Li, Ui, ldi);
+        hipLaunchKernelGGL((k_trinv_lu<T, 2>), g, b, 0, st, D, lda, sD, bs, Li, Ui, ldi, sI);
     else if (bs <= 256)
-        hipLaunchKernelGGL((k_trinv_lu<T, 4>), g, b, 0, st, D, lda, bs, Li, Ui, ldi);
+        hipLaunchKernelGGL((k_trinv_lu<T, 4>), g, b, 0, st, D, lda, sD, bs, Li, Ui, ldi, sI);
     else
-        hipLaunchKernelGGL((k_trinv_lu<T, 8>), g, b, 0, st, D, lda, bs, Li, Ui, ldi);
+        hipLaunchKernelGGL((k_trinv_lu<T, 8>), g, b, 0, st, D, lda, sD, bs, Li, Ui, ldi, sI);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
@@ -1064,7 +1068,7 @@ static int panel_leaf(typename cvec<T>::t *A, long lda, int n, int k, int nb, in
         if (!column_steps && n - jb <= LOC_T) {
             // (its interchanges and U12 solve on the other panel columns included)
             hipLaunchKernelGGL(k_getf2_local<T>, dim3(1, w.batch), dim3(LOC_T), 0, st, A, lda, w.sA, n, jb, jend - jb,
-                               ipiv, (long)n, w.info, k, k + nb);
+                               ipiv, (long)w.n, w.info, k, k + nb);
             MIDAS_HIP(hipGetLastError());
         } else if (fused) {
             const int nblk = (n - jb + BLK_T - 1) / BLK_T;
@@ -1127,7 +1131,7 @@ static int panel_rec(typename cvec<T>::t *A, long lda, int n, int k, int nb, int
     const int h = ((nb / 2 + LU_IB - 1) / LU_IB) * LU_IB, h2 = nb - h;
     int r = panel_rec<T>(A, lda, n, k, h, ipiv, w, st);
     if (r) return r;
-    hipLaunchKernelGGL(k_laswp<T>, dim3((h2 + 255) / 256, w.batch), dim3(256), 0, st, A, lda, w.sA, ipiv, n, k, k + h,
+    hipLaunchKernelGGL(k_laswp<T>, dim3((h2 + 255) / 256, w.batch), dim3(256), 0, st, A, lda, w.sA, ipiv, (long)w.n, k, k + h,
                        k + h, k + nb, 0, 0);
     MIDAS_HIP(hipGetLastError());
     // A12 <- L11^-1 A12 through the staging buffer (nb x nb per matrix)
@@ -1150,7 +1154,7 @@ static int panel_rec(typename cvec<T>::t *A, long lda, int n, int k, int nb, int
     }
     r = panel_rec<T>(A, lda, n, k + h, h2, ipiv, w, st);
     if (r) return r;
-    hipLaunchKernelGGL(k_laswp<T>, dim3((h + 255) / 256, w.batch), dim3(256), 0, st, A, lda, w.sA, ipiv, n, k + h,
+    hipLaunchKernelGGL(k_laswp<T>, dim3((h + 255) / 256, w.batch), dim3(256), 0, st, A, lda, w.sA, ipiv, (long)w.n, k + h,
                        k + nb, k, k + h, 0, 0);
     MIDAS_HIP(hipGetLastError());
     return 0;
@@ -1181,14 +1185,18 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
     int r = panel_rec<T>(A, lda, k + nb, k, nb, ipiv, w, st);
     if (r) return r;
     V *D = A + k + (long)k * lda;
-    V *Ui = (V *)w.uinv + (long)(k / w.nb) * w.nb * w.nb;
-    if ((r = trinv_lu<T>(D, lda, nb, (V *)w.linv + (long)(k / w.nb) * w.nb * w.nb, Ui, w.nb, st))) return r;
+    const long sI = w.inv_stride(), off = (long)(k / w.nb) * w.nb * w.nb;
+    V *Ui = (V *)w.uinv + off;
+    if ((r = trinv_lu<T>(D, lda, nb, (V *)w.linv + off, Ui, w.nb, st, w.batch, w.sA, sI))) return r;
     const int mr = n - k - nb;
     if (mr <= 0) return 0;
     V *A21 = A + (k + nb) + (long)k * lda, *S = (V *)w.l21;
-    if ((r = launch_cgemm<T>(mr, nb, nb, cone<T>(), A21, lda, 0, Ui, w.nb, 0, czero<T>(), S, mr, st))) return r;
-    hipLaunchKernelGGL(k_copy_amax<T>, dim3(min((mr + 255) / 256, 64), min(nb, 32)), dim3(256), 0, st, S, (long)mr,
-                       A21, lda, mr, nb, w.lmax);
+    const long sS = (long)mr * nb;
+    if ((r = launch_cgemm<T>(mr, nb, nb, cone<T>(), A21, lda, 0, Ui, w.nb, 0, czero<T>(), S, mr, st, w.batch, w.sA, sI,
+                             sS)))
+        return r;
+    hipLaunchKernelGGL(k_copy_amax<T>, dim3(min((mr + 255) / 256, 64), min(nb, 32), w.batch), dim3(256), 0, st, S,
+                       (long)mr, sS, A21, lda, w.sA, mr, nb, w.lmax);
     MIDAS_HIP(hipGetLastError());
     return 0;
 }
@@ -1225,7 +1233,7 @@ static int outer_panel(typename cvec<T>::t *A, long lda, int n, int k, int W, in
     const int h1 = min(w.nb, W), h2 = W - h1;
     int r = panel<T>(A, lda, n, k, h1, ipiv, w, ps);
     if (r || h2 <= 0) return r;
-    hipLaunchKernelGGL(k_laswp<T>, dim3((h2 + 255) / 256, w.batch), dim3(256), 0, ps, A, lda, w.sA, ipiv, n, k,
+    hipLaunchKernelGGL(k_laswp<T>, dim3((h2 + 255) / 256, w.batch), dim3(256), 0, ps, A, lda, w.sA, ipiv, (long)w.n, k,
                        k + h1, k + h1, k + W, 0, 0);
     MIDAS_HIP(hipGetLastError());
     r = u12_solve<T>(A, lda, k, h1, k + h1, k + W, w, ps, w.u12p, (long)w.nb * w.nb);
@@ -1235,7 +1243,7 @@ static int outer_panel(typename cvec<T>::t *A, long lda, int n, int k, int W, in
     if (r) return r;
     r = panel<T>(A, lda, n, k + h1, h2, ipiv, w, ps);
     if (r) return r;
-    hipLaunchKernelGGL(k_laswp<T>, dim3((h1 + 255) / 256, w.batch), dim3(256), 0, ps, A, lda, w.sA, ipiv, n, k + h1,
+    hipLaunchKernelGGL(k_laswp<T>, dim3((h1 + 255) / 256, w.batch), dim3(256), 0, ps, A, lda, w.sA, ipiv, (long)w.n, k + h1,
                        k + W, k, k + h1, 0, 0);
     MIDAS_HIP(hipGetLastError());
     return 0;
@@ -1280,8 +1288,7 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
     MIDAS_CHECK(w.batch == 1 || tg.batch == w.batch, -1, "getrf: trailing GEMM not batched like the factorisation");
     MIDAS_CHECK(w.outer == 1 || (w.outer == 2 && tgp.scratch != tg.scratch) || sizeof(T) == 8 || tg.mode == TRAIL_NATIVE,
                 -1, "getrf: two-panel steps need a panel-stream GEMM scratch of its own");
-    MIDAS_CHECK(!w.diag_pivot || (w.batch == 1 && w.l21 && w.lmax), -1,
-                "getrf: the diagonal-block pivot search needs batch 1 and its staging buffers");
+    MIDAS_CHECK(!w.diag_pivot || (w.l21 && w.lmax), -1, "getrf: the diagonal-block pivot search needs its staging buffers");
     const int nbo = w.nb * (w.outer == 2 ? 2 : 1);
     // last-workgroup counters, info, the fault flag and the step tags; the
     // tagged records and words (a fill kernel: safe under stream capture)
@@ -1302,7 +1309,7 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
         // swaps outside the outer panel: columns [0, k) and [k+W, n)
         int ncols = k + (n - k - W);
         if (ncols > 0) {
-            hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + 255) / 256, w.batch), dim3(256), 0, st, A, lda, w.sA, ipiv, n,
+            hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + 255) / 256, w.batch), dim3(256), 0, st, A, lda, w.sA, ipiv, (long)w.n,
                                k, k + W, 0, k, k + W, n);
             MIDAS_HIP(hipGetLastError());
         }
