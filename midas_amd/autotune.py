@@ -15,6 +15,26 @@ repetitions), the parameters midas_tune_set exposes:
   trail_group_kib     L2 share of a row-tile group of the split GEMM
   lu_nb_fp64          panel width of the FP64 LU
   refine_graph        refinement sweeps replayed from a captured hipGraph
+  lu_pivot_search     the MoM solve's pivot search: whole columns or each
+                      panel's diagonal block (certified)
+  mom_matrix_free     FP64 residuals of the MoM solve from the cells instead
+                      of a stored FP64 matrix (timed over the whole call:
+                      fill, solve, far field)
+
+and, on a DLCA aggregate solved by CBFM-E with ACA (midas_compute_efields_cbfm,
+wall time of the call):
+
+  cbfm_block_mode     the CBF block systems by block size (5: the f16 LU +
+                      FP64 refinement for blocks at least 16 times taller than
+                      the excitation count, the FP64 LU below) or always by
+                      the f16 LU + refinement (2).  (0, the FP64 LU for all, is
+                      the same code as 5 on this case, whose blocks are below
+                      that size; C5 measured 5 against 0: 1.85 s against
+                      1.98 s for its block phase.)
+
+(cbfm_graph, the batches' hipGraph launch, is not swept: it is the design's
+launch form, and measures equal to eager issue on the C3 and a0006 benches,
+the phase being GPU-bound.)
 
 It writes every timing to midas_amd/tuning.json and the chosen values to
 midas_amd/csrc/midas_tuned.h, the library's compiled-in defaults (rebuild to
@@ -72,8 +92,10 @@ def _solve_time(m, s, mode):
 def sweep(m, name, values, s, mode, reps, log):
     """Time every value of one parameter, the values interleaved over `reps`
     rounds (so that slow drifts of the device hit them alike); keep the
-    fastest (best of its rounds); return it."""
+    fastest (best of its rounds) unless the current value is within 1 % of
+    it; return it."""
     best_ms = {v: np.inf for v in values}
+    start = api.tune_get(name)
     for _ in range(reps):
         for v in values:
             api.tune_set(name, v)
@@ -83,7 +105,44 @@ def sweep(m, name, values, s, mode, reps, log):
                         "factor_ms": info["t_factor_ms"], "refine_ms": info["t_refine_ms"]})
     for v in values:
         print(f"  {name} = {v:6d}  n = {3 * s.n:6d}  {api.MODE_NAMES[mode]:8s}  {best_ms[v]:9.1f} ms", flush=True)
+    best = _pick(values, best_ms, start)
+    api.tune_set(name, best)
+    return best, best_ms
+
+
+def _pick(values, best_ms, current):
+    """The fastest value, unless the current one is within 1 % of it (timing
+    noise does not move a default)."""
     best = min(values, key=lambda v: best_ms[v])
+    if current in best_ms and best_ms[current] <= 1.01 * best_ms[best]:
+        return current
+    return best
+
+
+def _wall(f, reps):
+    """Best wall time (ms) of reps calls of f."""
+    best = np.inf
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        f()
+        best = min(best, 1e3 * (time.perf_counter() - t0))
+    return best
+
+
+def sweep_wall(name, values, f, reps, log, tag, keep=True):
+    """As sweep, for a timing function f() of the whole call (ms of wall
+    time); with keep=False the parameter is restored after the timings."""
+    old = api.tune_get(name)
+    best_ms = {v: np.inf for v in values}
+    for _ in range(reps):
+        for v in values:
+            api.tune_set(name, v)
+            t = _wall(f, 1)
+            best_ms[v] = min(best_ms[v], t)
+            log.append({"parameter": name, "value": v, "case": tag, "wall_ms": t})
+    for v in values:
+        print(f"  {name} = {v:6d}  {tag:24s}  {best_ms[v]:9.1f} ms", flush=True)
+    best = _pick(values, best_ms, old) if keep else old
     api.tune_set(name, best)
     return best, best_ms
 
@@ -117,6 +176,18 @@ def main():
     start = {k: api.tune_get(k) for k in api.TUNE_NAMES}
     print("start", start, flush=True)
     f16 = api.SOLVE_F16_IR
+    # the CBFM first: a DLCA aggregate, SR blocks, ACA, the reference's uniform
+    # CBF grid (timed before the MoM solves: once the process has allocated and
+    # freed their 57.5 GB matrices, the runtime's later multi-GB allocations --
+    # the CBFM's, made per call -- slow down)
+    tth, tph = np.array([0.0]), np.array([0.0])
+    ag = g.dlca_aggregate(nmono=16 if args.quick else 64, cells_per_mono=300, seed=3)
+    th_c, ph_c = g.cbf_directions_uniform(422)
+    cbfm_call = lambda: m.compute_efields_cbfme(ag, ag.k0, ag.blocks, ag.ext, (th_c, ph_c), (tth, tph), use_aca=True,
+                                                sr=True, res_sr=0.01, mode=f16)
+    cbfm_call()
+    tag = f"CBFM DLCA {ag.n} cells"
+    sweep_wall("cbfm_block_mode", [5, 2], cbfm_call, reps + 1, log, tag)
     # the wider LU step applies from lu_two_panel_min_n on: open it for the sweep
     api.tune_set("lu_two_panel_min_n", 0)
     sweep(m, "lu_nb_split", [512, 768, 1024], big, f16, reps, log)
@@ -137,6 +208,12 @@ def main():
     api.tune_set("lu_two_panel_min_n", min_n)
     mid = spheres[sizes[-2]]
     sweep(m, "lu_nb_fp64", [64, 128, 256], mid, api.SOLVE_FP64, reps, log)
+    # the MoM solve's pivot search and residual form, on the largest sphere
+    sweep(m, "lu_pivot_search", [0, 1], big, f16, reps, log)
+    rx = (np.array([0.0, 90.0]), np.array([0.0, 0.0]))
+    mom_call = lambda: m.compute_efields_mom(big, big.k0, (tth, tph), rx, mode=f16, want_E=False)
+    mom_call()
+    sweep_wall("mom_matrix_free", [0, 1], mom_call, reps, log, f"MoM {big.n} cells")
     m.close()
     final = {k: api.tune_get(k) for k in api.TUNE_NAMES}
     print("chosen", final, flush=True)

@@ -177,6 +177,7 @@ static int fp64_lu_solve(midas_ctx *ctx, SolveWs &ws, int n, int nrhs, double2 *
 {
     hipStream_t st = ctx->st;
     LuWork<double> &w = ws.lud;
+    if (!ws.lent) w.share_stream(ctx->ps, ctx->pev[0], ctx->pev[1]);
     TRY(w.alloc(n, nb));
     TrailingGemm<double> tg;
     tg.library = !ws.lent;
@@ -385,6 +386,7 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
                 "solve: panel width %d: above 512 needs a single-term split mode, at most 1024", nb);
     const int nbi = two ? (nb + 1) / 2 : nb;
     LuWork<float> &w = ws.luf;
+    if (!ws.lent) w.share_stream(ctx->ps, ctx->pev[0], ctx->pev[1]);
     TRY(w.alloc(n, nbi));
     w.outer = two ? 2 : 1;
     TRY(ws.trail.reserve(trail16_scratch_bytes(n, n, nb)));
@@ -542,9 +544,21 @@ int midas_init(int device, midas_ctx **ctx)
     if (e == hipSuccess) e = hipGetSymbolAddress(&base, HIP_SYMBOL(scratch));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->nev[0], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->nev[1], hipEventDisableTiming);
+    // the panel stream and the second CBFM stream, right after the main one
+    if (e == hipSuccess) e = hipStreamCreate(&c->ps);
+    if (e == hipSuccess) e = hipStreamCreate(&c->s2);
+    for (hipEvent_t &ev : c->pev)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    for (hipEvent_t &ev : c->aev)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
     if (e != hipSuccess) {
-        if (c->st) hipStreamDestroy(c->st);
+        for (hipStream_t s : {c->st, c->ps, c->s2})
+            if (s) hipStreamDestroy(s);
         for (hipEvent_t ev : c->nev)
+            if (ev) hipEventDestroy(ev);
+        for (hipEvent_t ev : c->pev)
+            if (ev) hipEventDestroy(ev);
+        for (hipEvent_t ev : c->aev)
             if (ev) hipEventDestroy(ev);
         delete c;
         set_error("midas_init: %s", hipGetErrorString(e));
@@ -564,8 +578,15 @@ int midas_finalize(midas_ctx *ctx)
     ctx->nws.release();
     for (hipGraphExec_t g : ctx->cbfm_graphs)
         if (g) hipGraphExecDestroy(g);
-    hipStreamDestroy(ctx->st);
+    hipStreamSynchronize(ctx->ps);
+    hipStreamSynchronize(ctx->s2);
+    for (hipStream_t s : {ctx->st, ctx->ps, ctx->s2})
+        hipStreamDestroy(s);
     for (hipEvent_t ev : ctx->nev)
+        if (ev) hipEventDestroy(ev);
+    for (hipEvent_t ev : ctx->pev)
+        if (ev) hipEventDestroy(ev);
+    for (hipEvent_t ev : ctx->aev)
         if (ev) hipEventDestroy(ev);
     delete ctx;
     return 0;

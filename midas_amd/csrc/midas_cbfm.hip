@@ -203,8 +203,9 @@ struct BlockMixedWs {
 
 // X (nc blocks of mpc x T2c at stride mpc T2c, ld mpc) holds E on entry and
 // X on return; Z (nc blocks of mpc x mpc, ld mpc) stays unmodified.
-static int block_solve_mixed(int mode, int nc, int mpc, int T2c, const double2 *Z, double2 *X, int *ipiv, int *perm,
-                             std::vector<int> &hinfo, BlockMixedWs &bw, int nb, int nb64, hipStream_t st)
+static int block_solve_mixed(midas_ctx *ctx, int mode, int nc, int mpc, int T2c, const double2 *Z, double2 *X,
+                             int *ipiv, int *perm, std::vector<int> &hinfo, BlockMixedWs &bw, int nb, int nb64,
+                             hipStream_t st)
 {
     const long zs = (long)mpc * mpc, xs = (long)mpc * T2c, nx = xs * nc;
     const dim3 fg(2048), fb(256);
@@ -223,6 +224,7 @@ static int block_solve_mixed(int mode, int nc, int mpc, int T2c, const double2 *
     MIDAS_HIP(hipGetLastError());
     // batched complex-FP32 LU
     LuWork<float> w;
+    w.share_stream(ctx->ps, ctx->pev[0], ctx->pev[1]);
     TRY(w.alloc(mpc, nb, nc, zs));
     TrailingGemm<float> tg;
     tg.mode = mode == MIDAS_SOLVE_C32_IR ? TRAIL_NATIVE : mode == MIDAS_SOLVE_BF16_IR ? TRAIL_BF16_1 : TRAIL_F16_1;
@@ -287,6 +289,7 @@ static int block_solve_mixed(int mode, int nc, int mpc, int T2c, const double2 *
     for (int q = 0; q < nc; ++q) {
         if (berr[q] <= tol) continue;
         LuWork<double> w64;
+        w64.share_stream(ctx->ps, ctx->pev[0], ctx->pev[1]);
         TRY(w64.alloc(mpc, nb64));
         TrailingGemm<double> tg64;
         DBuf Ab;
@@ -794,6 +797,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
             LuRelease<double> wguard{w};
             TrailingGemm<double> tg;
             if (fp64) {
+                w.share_stream(ctx->ps, ctx->pev[0], ctx->pev[1]);
                 TRY(w.alloc((int)mpc, nb_lu, nc, mpc * mpc));
                 w.defer_fault_check = true;   // checked after the batch (no synchronisation inside a graph)
                 tg.batch = nc;
@@ -850,8 +854,8 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                 }
                 TRY(copy_sync(hinfo.data(), w.info, sizeof(int) * nc, hipMemcpyDeviceToHost, st));
             } else {
-                TRY(block_solve_mixed(block_mode, nc, (int)mpc, T2c, Z, X, ipiv.as<int>(), perm.as<int>(), hinfo, bmw,
-                                      nb_mixed, nb_lu, st));
+                TRY(block_solve_mixed(ctx, block_mode, nc, (int)mpc, T2c, Z, X, ipiv.as<int>(), perm.as<int>(), hinfo,
+                                      bmw, nb_mixed, nb_lu, st));
                 hipLaunchKernelGGL(k_copy_out_group, dim3((int)((mpc + 255) / 256), T2c, nc), dim3(256), 0, st, jd, X,
                                    mpc, mpc * T2c, Xall.as<double2>(), mp, mp * T2c);
                 MIDAS_HIP(hipGetLastError());
@@ -1129,6 +1133,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
             uo_max = std::max(uo_max, bt.uo);
             vo_max = std::max(vo_max, bt.vo);
         }
+        // the second stream and its events are the context's (made once)
         struct Pipe {
             hipStream_t s2 = nullptr;
             hipEvent_t ev[5] = {};   // start, rank[2], products[2]
@@ -1136,17 +1141,14 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
             ~Pipe()
             {
                 if (s2) hipStreamSynchronize(s2);
-                for (hipEvent_t e : ev)
-                    if (e) hipEventDestroy(e);
                 for (int *h : hk)
                     if (h) hipHostFree(h);
-                if (s2) hipStreamDestroy(s2);
             }
         } pp;
         DBuf dj[2], Ub[2], Vb[2], dk[2], Gb[2], LR[2], dgj[2];
         if (!batches.empty()) {
-            MIDAS_HIP(hipStreamCreate(&pp.s2));
-            for (hipEvent_t &e : pp.ev) MIDAS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            pp.s2 = ctx->s2;
+            for (int q = 0; q < 5; ++q) pp.ev[q] = ctx->aev[q];
             for (int q = 0; q < 2; ++q) {
                 MIDAS_HIP(hipHostMalloc((void **)&pp.hk[q], sizeof(int) * nj_max));
                 TRY(dj[q].reserve(sizeof(AcaJob) * nj_max));

@@ -199,6 +199,11 @@ struct SolveWs {
 struct midas_ctx {
     int device = 0;
     hipStream_t st = nullptr;
+    // the factorisations' panel stream and the CBFM pipeline's second stream,
+    // with their events, made once by midas_init: streams made call after call
+    // can land on the main stream's hardware queue and serialise with it
+    hipStream_t ps = nullptr, s2 = nullptr;
+    hipEvent_t pev[2] = {}, aev[5] = {};
     SolveWs ws;
     DevCellsOwned cells;          // cell constants of the device-pointer entry points
     // the allocation-free entry point (midas_dev_mom_solve_noalloc): its

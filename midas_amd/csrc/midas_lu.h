@@ -28,6 +28,7 @@ template <typename T> struct LuWork {
     int outer = 1;                   // inner panels per outer step: 1, or 2 (trailing updates of 2 nb columns)
     hipStream_t pstream = nullptr;   // panel stream (look-ahead), default priority
     hipEvent_t ev_panel = nullptr, ev_next = nullptr;
+    bool own_stream = true;          // pstream and the events made (and destroyed) here
     size_t cap[10] = {};              // bytes held by each buffer above (grow-only)
     bool borrowed = false;           // buffers, stream and events lent by the caller (lend())
     bool defer_fault_check = false;  // getrf leaves the wait-fault check to lu_fault_check (stream capture)
@@ -48,6 +49,11 @@ template <typename T> struct LuWork {
     // look-ahead overlap) and events: alloc() then only checks the sizes.
     static size_t bytes_needed(int n, int nb, int batch = 1);
     int lend(void *mem, size_t bytes, int n, int nb, int batch, hipStream_t ps, hipEvent_t e1, hipEvent_t e2);
+    // Use the caller's panel stream and events (kept for the life of its
+    // context) instead of making them: streams made and destroyed call after
+    // call are placed on the process's few hardware queues anew each time and
+    // can share the main stream's, which serialises the look-ahead.
+    void share_stream(hipStream_t ps, hipEvent_t e1, hipEvent_t e2);
     long inv_stride() const { return (long)((n + nb - 1) / nb) * nb * nb; }   // diagonal-block inverses per matrix
 };
 

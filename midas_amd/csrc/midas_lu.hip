@@ -1698,7 +1698,7 @@ void LuWork<T>::release()
     if (!borrowed) {
         if (pstream) {
             hipStreamSynchronize(pstream);
-            hipStreamDestroy(pstream);
+            if (own_stream) hipStreamDestroy(pstream);
         }
         hipFree(pmax);
         hipFree(pidx);
@@ -1710,10 +1710,13 @@ void LuWork<T>::release()
         hipFree(u12p);
         hipFree(cand);
         hipFree(rowj);
-        if (ev_panel) hipEventDestroy(ev_panel);
-        if (ev_next) hipEventDestroy(ev_next);
+        if (own_stream) {
+            if (ev_panel) hipEventDestroy(ev_panel);
+            if (ev_next) hipEventDestroy(ev_next);
+        }
     }
     borrowed = false;
+    own_stream = true;
     pstream = nullptr;
     ev_panel = ev_next = nullptr;
     pmax = nullptr;
@@ -1722,6 +1725,24 @@ void LuWork<T>::release()
     pinv = nullptr;
     linv = uinv = u12 = u12p = cand = rowj = nullptr;
     for (size_t &c : cap) c = 0;
+}
+
+template <typename T>
+void LuWork<T>::share_stream(hipStream_t ps, hipEvent_t e1, hipEvent_t e2)
+{
+    if (!ps || pstream == ps) return;
+    if (own_stream && !borrowed) {
+        if (pstream) {
+            hipStreamSynchronize(pstream);
+            hipStreamDestroy(pstream);
+        }
+        if (ev_panel) hipEventDestroy(ev_panel);
+        if (ev_next) hipEventDestroy(ev_next);
+    }
+    pstream = ps;
+    ev_panel = e1;
+    ev_next = e2;
+    own_stream = false;
 }
 
 template struct LuWork<double>;
