@@ -21,6 +21,15 @@ extern "C" {
 int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double *alpha, const double *A, long lda,
                     int transA, const double *B, long ldb, int transB, const double *beta, double *C, long ldc);
 
+/* LU with partial pivoting of one n x n complex matrix by the library's
+ * factorisation: A (complex128, column-major, lda) receives the unit-lower L
+ * and U; ipiv[j] (0-based) the row interchanged with row j.  prec 64 (FP64)
+ * or 32 (FP32: A rounded to complex64, the factors returned widened); nb the
+ * panel width (<= 512); pivot_search 1 restricts each panel's search to its
+ * diagonal block and returns the largest |L21| in *max_l21 (else 0). */
+int midas_test_getrf(midas_ctx *ctx, int prec, int n, double *A, long lda, int nb, int pivot_search, int *ipiv,
+                     double *max_l21);
+
 /* Eigen-decomposition of a batch of Hermitian n x n complex matrices G
  * (batch matrices one after another, column-major) by the CBF subspaces'
  * batched eigensolver: D (n per matrix) ascending, V (n x n per matrix) the

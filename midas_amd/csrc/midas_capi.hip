@@ -1299,6 +1299,61 @@ int midas_test_gemm(midas_ctx *ctx, int prec, int m, int n, int k, const double 
     return 0;
 }
 
+}   // extern "C"
+
+template <typename T>
+static int test_getrf_t(midas_ctx *ctx, int n, typename cvec<T>::t *dA, int nb, int pivot_search, int *dipiv,
+                        double *max_l21)
+{
+    LuWork<T> w;
+    w.share_stream(ctx->ps, ctx->pev[0], ctx->pev[1]);
+    int r = w.alloc(n, nb);
+    DBuf l21;
+    if (!r && pivot_search) {
+        const size_t sl = sizeof(cplx<T>) * (size_t)n * nb;
+        r = l21.alloc(sl + 256);
+        w.diag_pivot = true;
+        w.l21 = l21.as<cplx<T>>();
+        w.lmax = (unsigned *)((char *)l21.p + sl);
+    }
+    TrailingGemm<T> tg;
+    if (!r) r = getrf<T>(dA, n, n, dipiv, w, ctx->st, tg);
+    unsigned lm = 0;
+    if (!r && pivot_search) r = copy_sync(&lm, w.lmax, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st);
+    if (max_l21) *max_l21 = (double)__builtin_bit_cast(float, lm);
+    w.release();
+    return r;
+}
+
+extern "C" {
+
+int midas_test_getrf(midas_ctx *ctx, int prec, int n, double *A, long lda, int nb, int pivot_search, int *ipiv,
+                     double *max_l21)
+{
+    MIDAS_CHECK(ctx && A && ipiv && n > 0 && lda >= n && nb >= 16 && nb <= 512 && (prec == 64 || prec == 32), -1,
+                "test_getrf: bad arguments");
+    MIDAS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->st;
+    DBuf dA, dp;
+    TRY(upload_mat(dA, A, n, n, lda, st));
+    TRY(dp.alloc(sizeof(int) * n));
+    if (prec == 64) {
+        TRY(test_getrf_t<double>(ctx, n, dA.as<double2>(), nb, pivot_search, dp.as<int>(), max_l21));
+    } else {
+        DBuf a32;
+        TRY(to_c64(a32, dA, (long)n * n, st));
+        TRY(test_getrf_t<float>(ctx, n, a32.as<float2>(), nb, pivot_search, dp.as<int>(), max_l21));
+        hipLaunchKernelGGL(k_c2z, dim3(((long)n * n + 255) / 256), dim3(256), 0, st, a32.as<float2>(),
+                           dA.as<double2>(), (long)n * n);
+        MIDAS_HIP(hipGetLastError());
+    }
+    MIDAS_HIP(hipMemcpy2DAsync(A, sizeof(double2) * lda, dA.p, sizeof(double2) * n, sizeof(double2) * n, n,
+                               hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipMemcpyAsync(ipiv, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    MIDAS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
 int midas_test_heev(midas_ctx *ctx, int n, int batch, const double *G, double *D, double *V, double cut, int path)
 {
     MIDAS_CHECK(ctx && G && D && V && n > 0 && batch > 0, -1, "test_heev: bad arguments");

@@ -240,15 +240,45 @@ template <typename T> __device__ inline bool pivot_better(T v, int i, T bv, int 
     return v > bv || (v == bv && i < bi);
 }
 
+// Argmax over the wave of (v, i), v >= 0 or v < 0 for no candidate, the
+// smallest i on ties when i grows with the lane (a wave's rows do).  FP32: the
+// key "float bits + 1" (0 for none) is reduced by DPP row shifts and row
+// broadcasts -- VALU only, where lane shuffles (ds_bpermute) queue behind the
+// trailing update's LDS fragment reads -- and the first lane holding the
+// maximum gives (v, i).  FP64: lane shuffles.
+template <typename T> __device__ inline void wave_argmax(T &v, int &i)
+{
+    if constexpr (sizeof(T) == 4) {
+        const unsigned key = v < (T)0 ? 0u : __float_as_uint((float)v) + 1u;
+        unsigned m = key;
+        auto up = [](unsigned a, unsigned b) { return a > b ? a : b; };
+        m = up(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x111, 0xf, 0xf, true));   // row_shr:1
+        m = up(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x112, 0xf, 0xf, true));   // row_shr:2
+        m = up(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x114, 0xf, 0xf, true));   // row_shr:4
+        m = up(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x118, 0xf, 0xf, true));   // row_shr:8
+        m = up(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x142, 0xa, 0xf, false));   // row_bcast:15
+        m = up(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x143, 0xc, 0xf, false));   // row_bcast:31
+        const unsigned w = __builtin_amdgcn_readlane(m, 63);
+        const int lane = __ffsll((long long)__ballot(key == w)) - 1;
+        v = __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+        i = __builtin_amdgcn_readlane(i, lane);
+    } else {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const T ov = __shfl_xor(v, off, 64);
+            const int oi = __shfl_xor(i, off, 64);
+            if (pivot_better(ov, oi, v, i)) {
+                v = ov;
+                i = oi;
+            }
+        }
+    }
+}
+
 // argmax over the workgroup of NW waves; every thread gets the result.
 template <typename T, int NW = BLK_T / 64> __device__ inline void argmax_block(T &v, int &i, T *rv, int *ri)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const T ov = __shfl_xor(v, off, 64);
-        const int oi = __shfl_xor(i, off, 64);
-        if (pivot_better(ov, oi, v, i)) { v = ov; i = oi; }
-    }
+    wave_argmax(v, i);
     if ((threadIdx.x & 63) == 0) { rv[threadIdx.x >> 6] = v; ri[threadIdx.x >> 6] = i; }
     __syncthreads();
 #pragma unroll
@@ -484,25 +514,28 @@ __global__ __launch_bounds__(BLK_T, WPE) void k_getf2_block(typename cvec<T>::t 
 // every thread picks the winner among the waves' slots and reads the pivot
 // row from it.  Slots alternate by column parity, so a slot is rewritten only
 // after every thread has passed the next column's barrier.  Then, as
-// k_swap_trsm would (the same workgroup, no launch): the block's interchanges
-// on the panel's columns [k, jb) and [jend, k1), and the unit-lower solve with
-// L11 (from LDS) on [jend, k1).  It takes the inner blocks of the last rows of
+// k_swap_trsm would (the same workgroup, no launch): the block's interchanges,
+// in order, on the panel's columns [k, jb) and [jend, k1), and the unit-lower
+// solve with L11 (from LDS) on [jend, k1).  It takes the inner blocks of the last rows of
 // a factorisation and the diagonal blocks of panels whose pivot search is
 // restricted to them.
+// NT threads: 512, or 256 for at most 256 rows (four waves of at most 96
+// registers fit a CU beside the trailing update's two workgroups; eight
+// waves wait for a CU it leaves).
 constexpr int LOC_T = 512;
-template <typename T>
-__global__ __launch_bounds__(LOC_T) void k_getf2_local(typename cvec<T>::t *A, long lda, long sA, int rend, int jb,
-                                                       int ncols, int *ipiv, long sP, int *info, int k, int k1)
+template <typename T, int NT>
+__global__ __launch_bounds__(NT, NT == 256 ? 5 : 1) void k_getf2_local(typename cvec<T>::t *A, long lda, long sA,
+                                                                       int rend, int jb, int ncols, int *ipiv, long sP,
+                                                                       int *info, int k, int k1)
 {
     typedef typename cvec<T>::t V;
-    constexpr int NW = LOC_T / 64;
+    constexpr int NW = NT / 64;
     __shared__ V crow[2][NW][LU_IB];   // each wave's candidate row
     __shared__ V jrow[2][LU_IB];       // row j
     __shared__ T cval[2][NW];
     __shared__ int cidx[2][NW];
     __shared__ V l11[LU_IB][LU_IB];    // the block's diagonal 16 x 16, [row][column]
-    __shared__ int piv[LU_IB], pos[2 * LU_IB], src[2 * LU_IB];
-    __shared__ int npair;
+    __shared__ int piv[LU_IB];
     const int tid = threadIdx.x, zb = blockIdx.y, wv = tid >> 6, lane = tid & 63;
     A += zb * sA;
     ipiv += zb * sP;
@@ -522,15 +555,7 @@ __global__ __launch_bounds__(LOC_T) void k_getf2_local(typename cvec<T>::t *A, l
                 best = fabs(a[0].x) + fabs(a[0].y);
                 bi = i;
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const T ov = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (pivot_better(ov, oi, best, bi)) {
-                    best = ov;
-                    bi = oi;
-                }
-            }
+            wave_argmax(best, bi);
             if (i == bi) {
                 cval[par][wv] = best;
                 cidx[par][wv] = bi;
@@ -603,41 +628,23 @@ __global__ __launch_bounds__(LOC_T) void k_getf2_local(typename cvec<T>::t *A, l
     }
     stores_done();
     __syncthreads();
-    // the interchanges composed into at most 2 LU_IB (position, source) moves
-    if (tid == 0) {
-        int m = 0;
-        for (int j = jb; j < jend; ++j) {
-            const int p = piv[j - jb];
-            if (p == j) continue;
-            int sj = -1, sp = -1;
-            for (int s = 0; s < m; ++s) {
-                if (pos[s] == j) sj = s;
-                if (pos[s] == p) sp = s;
-            }
-            if (sj < 0) { pos[m] = j; src[m] = j; sj = m++; }
-            if (sp < 0) { pos[m] = p; src[m] = p; sp = m++; }
-            const int t = src[sj];
-            src[sj] = src[sp];
-            src[sp] = t;
-        }
-        npair = m;
-    }
-    __syncthreads();
-    const int m = npair;
-    for (int t = tid; t < nl + nr; t += LOC_T) {
+    // the interchanges, in order (the pivots are in LDS; a row pair at a time
+    // keeps the registers to those of the column steps)
+    bool moved = false;
+#pragma unroll
+    for (int r = 0; r < LU_IB; ++r) moved |= r < ncols && piv[r] != jb + r;
+    for (int t = tid; t < nl + nr; t += NT) {
         const int c = t < nl ? k + t : jend + (t - nl);
         V *col = A + (long)c * lda;
-        if (m > 0) {
-            cplx<T> v[2 * LU_IB];
-#pragma unroll
-            for (int s = 0; s < 2 * LU_IB; ++s)
-                if (s < m) {
-                    const V x = col[src[s]];
-                    v[s] = cplx<T>{x.x, x.y};
+        if (moved) {
+            for (int r = 0; r < ncols; ++r) {
+                const int p = piv[r];
+                if (p != jb + r) {
+                    const V u = col[jb + r], v = col[p];
+                    col[jb + r] = v;
+                    col[p] = u;
                 }
-#pragma unroll
-            for (int s = 0; s < 2 * LU_IB; ++s)
-                if (s < m) col[pos[s]] = V{v[s].r, v[s].i};
+            }
             stores_done();
         }
         if (c < jend) continue;
@@ -1067,8 +1074,12 @@ static int panel_leaf(typename cvec<T>::t *A, long lda, int n, int k, int nb, in
         int rc = k + nb - jend;
         if (!column_steps && n - jb <= LOC_T) {
             // (its interchanges and U12 solve on the other panel columns included)
-            hipLaunchKernelGGL(k_getf2_local<T>, dim3(1, w.batch), dim3(LOC_T), 0, st, A, lda, w.sA, n, jb, jend - jb,
-                               ipiv, (long)w.n, w.info, k, k + nb);
+            if (n - jb <= 256)
+                hipLaunchKernelGGL((k_getf2_local<T, 256>), dim3(1, w.batch), dim3(256), 0, st, A, lda, w.sA, n, jb,
+                                   jend - jb, ipiv, (long)w.n, w.info, k, k + nb);
+            else
+                hipLaunchKernelGGL((k_getf2_local<T, LOC_T>), dim3(1, w.batch), dim3(LOC_T), 0, st, A, lda, w.sA, n, jb,
+                                   jend - jb, ipiv, (long)w.n, w.info, k, k + nb);
             MIDAS_HIP(hipGetLastError());
         } else if (fused) {
             const int nblk = (n - jb + BLK_T - 1) / BLK_T;

@@ -362,3 +362,60 @@ def test_pchol_rank_reduction(mid, n):
         err = np.abs(Gq - Lq @ Lq.conj().T).max() / nrm
         print(f"  case {q}: rank {rk[q]}, |G - L L^H| / max diag {err:.1e}")
         assert err < 100 * n * np.finfo(float).eps
+
+
+def _getrf(mid, prec, A, nb, search=0):
+    n = A.shape[0]
+    F = np.asfortranarray(A.astype(np.complex128))
+    piv = np.zeros(n, dtype=np.int32)
+    lm = C.c_double(-1.0)
+    r = mid.lib.midas_test_getrf(mid.ctx, C.c_int(prec), C.c_int(n), _p(F), C.c_long(n), C.c_int(nb),
+                                 C.c_int(search), _p(piv), C.byref(lm))
+    assert r == 0, mid.lib.midas_last_error()
+    return F, piv, lm.value
+
+
+@pytest.mark.parametrize("n,nb", [(37, 64), (300, 64), (700, 128), (1100, 256)])
+def test_getrf_pivots_match_lapack(mid, n, nb):
+    """FP64 factorisation: the same pivot rows as LAPACK's zgetrf (search by
+    |re| + |im|) and the same factors."""
+    import scipy.linalg as sl
+    print(f"\n[case] getrf fp64 n={n} nb={nb}", flush=True)
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    F, piv, _ = _getrf(mid, 64, A, nb)
+    lu, p = sl.lu_factor(A)
+    assert np.array_equal(piv, p)
+    assert np.abs(F - lu).max() <= 1e-10 * np.abs(lu).max()
+
+
+@pytest.mark.parametrize("n,nb", [(300, 64), (1100, 256), (1500, 512)])
+def test_getrf_fp32_partial_pivoting(mid, n, nb):
+    """FP32 factorisation of a real matrix: every multiplier |l_ij| <= 1 (the
+    pivot is the column's largest entry) and P A = L U to FP32 rounding."""
+    print(f"\n[case] getrf fp32 n={n} nb={nb}", flush=True)
+    rng = np.random.default_rng(7 + n)
+    A = rng.standard_normal((n, n))
+    F, piv, _ = _getrf(mid, 32, A, nb)
+    L = np.tril(F, -1) + np.eye(n)
+    U = np.triu(F)
+    assert np.abs(L).max() <= 1 + 1e-6
+    PA = A.astype(np.complex128).copy()
+    for j in range(n):
+        PA[[j, piv[j]]] = PA[[piv[j], j]]
+    assert np.abs(L @ U - PA).max() <= 2e-4 * np.abs(A).max()
+
+
+def test_getrf_diag_block_search(mid):
+    """Diagonal-block pivot search on a diagonally dominant matrix: certified
+    (max |L21| <= 1/sqrt(2)) and equal to LAPACK's factors, no interchanges."""
+    import scipy.linalg as sl
+    n, nb = 900, 128
+    print(f"\n[case] getrf diag-block search n={n} nb={nb}", flush=True)
+    rng = np.random.default_rng(3)
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / n + np.diag(np.full(n, 4.0 + 1j))
+    F, piv, lm = _getrf(mid, 64, A, nb, search=1)
+    lu, p = sl.lu_factor(A)
+    assert np.array_equal(p, np.arange(n)) and np.array_equal(piv, p)
+    assert 0 < lm <= 0.70710678
+    assert np.abs(F - lu).max() <= 1e-10 * np.abs(lu).max()
