@@ -1019,6 +1019,229 @@ static int trinv_lu(const typename cvec<T>::t *D, long lda, int bs, typename cve
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Inverses of the triangular factors of a bs x bs diagonal block by recursive
+// doubling.  32 x 32 leaves are inverted one per wave (lane c carries column c
+// of the leaf's inverse in registers; the leaf's columns are read with
+// wave-uniform addresses).  Each level then merges pairs of b x b inverses
+// into 2b x 2b ones,
+//   L = [A 0; C B]:  L^-1 = [A^-1 0; -B^-1 C A^-1  B^-1]
+//   U = [A E; 0 G]:  U^-1 = [A^-1  -A^-1 E G^-1; 0 G^-1]
+// as two small products (T = C A^-1 then -B^-1 T; T = E G^-1 then -A^-1 T),
+// every merge of the level, both factors and every matrix of a batch in one
+// launch per product: 1 + 2 log2(bs / 32) launches in all, where one wave per
+// column of the inverse walking the whole triangle took bs dependent steps and
+// read the triangle once per column.  The blocks opposite the inverses'
+// triangles are written as zeros.
+// ---------------------------------------------------------------------------
+constexpr int TI_LEAF = 32;
+
+// (FP32 within 96 registers: it runs beside the trailing update)
+template <typename T>
+__global__ __launch_bounds__(64, sizeof(T) == 4 ? 5 : 1) void k_trinv_leaf(const typename cvec<T>::t *__restrict__ D,
+                                                                           long lda, long sD, int bs,
+                                                                           typename cvec<T>::t *Li,
+                                                                           typename cvec<T>::t *Ui, long ldi, long sI)
+{
+    typedef typename cvec<T>::t V;
+    V *Inv = blockIdx.z ? Ui : Li;
+    if (!Inv) return;
+    const int p = blockIdx.x * TI_LEAF, b = min(TI_LEAF, bs - p);
+    D += blockIdx.y * sD + p + (long)p * lda;
+    Inv += blockIdx.y * sI + p + (long)p * ldi;
+    // lane: column c of the leaf's inverse, rows [16 h, 16 h + 16) of it
+    // (lanes of columns >= b compute along and store nothing)
+    const int lane = threadIdx.x, c = lane & 31, h = lane >> 5;
+    // the factor's triangle staged in LDS without its diagonal (zero
+    // elsewhere and outside the b x b block), U's diagonal apart (one past b)
+    __shared__ cplx<T> sl[TI_LEAF][TI_LEAF];   // [column][row]
+    __shared__ cplx<T> sd[TI_LEAF];
+    const bool lower = blockIdx.z == 0;
+    for (int e = lane; e < TI_LEAF * TI_LEAF; e += 64) {
+        const int r = e % TI_LEAF, q = e / TI_LEAF;
+        const bool in = r < b && q < b && (lower ? r > q : r < q);
+        const V v = in ? D[r + (long)q * lda] : V{0, 0};
+        sl[q][r] = cplx<T>{v.x, v.y};
+    }
+    if (lane < TI_LEAF) {
+        const V d = lane < b ? D[lane + (long)lane * lda] : V{1, 0};
+        sd[lane] = cplx<T>{d.x, d.y};
+    }
+    __syncthreads();
+    cplx<T> x[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = cplx<T>{(T)(16 * h + j == c ? 1 : 0), (T)0};
+    // x_q (final once step q begins) from the lane holding row q of column c
+    auto xq_of = [&](int q) {
+        const cplx<T> v = x[q & 15];
+        const int src = c + 32 * (q >> 4);
+        return cplx<T>{__shfl(v.r, src, 64), __shfl(v.i, src, 64)};
+    };
+    if (lower) {
+        // unit lower: x_r -= L(r, q) x_q (r > q), q ascending
+#pragma unroll
+        for (int q = 0; q < TI_LEAF - 1; ++q) {
+            const cplx<T> xq = xq_of(q);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) x[j] = csub(x[j], cmul(sl[q][16 * h + j], xq));
+        }
+    } else {
+        // upper: x_q /= U(q, q), then x_r -= U(r, q) x_q (r < q), q descending
+#pragma unroll
+        for (int q = TI_LEAF - 1; q >= 0; --q) {
+            const cplx<T> xq = cmul(xq_of(q), cinv(sd[q]));
+            if (h == (q >> 4)) x[q & 15] = xq;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) x[j] = csub(x[j], cmul(sl[q][16 * h + j], xq));
+        }
+    }
+    if (c < b) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (16 * h + j < b) Inv[16 * h + j + (long)c * ldi] = V{x[j].r, x[j].i};
+    }
+}
+
+// Merges at level b (pairs of b x b inverses at p = 2 m b): stage 0 forms T
+// for every merge m into Tw (per merge at m b^2, ld = rows of T), stage 1 the
+// off-diagonal block of the merged inverse and the zero block opposite it.
+// Grid x: merges x (ceil(b / 32))^2 tiles of 32 x 32, y: matrices, z: L / U.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trinv_merge(const typename cvec<T>::t *__restrict__ D, long lda, long sD,
+                                                     int bs, int b, int stage, typename cvec<T>::t *Li,
+                                                     typename cvec<T>::t *Ui, long ldi, long sI,
+                                                     typename cvec<T>::t *Tw, long sT, long sTz)
+{
+    typedef typename cvec<T>::t V;
+    V *Inv = blockIdx.z ? Ui : Li;
+    if (!Inv) return;
+    const int tpm = (b + 31) / 32;
+    const int m = blockIdx.x / (tpm * tpm), t = blockIdx.x % (tpm * tpm);
+    const int p = 2 * m * b, b1 = b, b2 = min(b, bs - p - b);
+    if (b2 <= 0) return;
+    D += blockIdx.y * sD;
+    Inv += blockIdx.y * sI;
+    V *Tm = Tw + blockIdx.y * sT + blockIdx.z * sTz + (long)m * b * b;
+    // Out (M x N) = alpha P (M x K) Q (K x N)
+    int M, N, K;
+    const V *P, *Q;
+    long ldp, ldq, ldo;
+    V *O, *Z;
+    T alpha;
+    if (blockIdx.z == 0) {
+        M = b2;
+        N = b1;
+        Z = Inv + p + (long)(p + b1) * ldi;   // b1 x b2 zeros above the merged block's diagonal
+        if (stage == 0) {
+            K = b1, P = D + (p + b1) + (long)p * lda, ldp = lda, Q = Inv + p + (long)p * ldi, ldq = ldi;
+            O = Tm, ldo = b2, alpha = 1;
+        } else {
+            K = b2, P = Inv + (p + b1) + (long)(p + b1) * ldi, ldp = ldi, Q = Tm, ldq = b2;
+            O = Inv + (p + b1) + (long)p * ldi, ldo = ldi, alpha = -1;
+        }
+    } else {
+        M = b1;
+        N = b2;
+        Z = Inv + (p + b1) + (long)p * ldi;   // b2 x b1 zeros below it
+        if (stage == 0) {
+            K = b2, P = D + p + (long)(p + b1) * lda, ldp = lda, Q = Inv + (p + b1) + (long)(p + b1) * ldi, ldq = ldi;
+            O = Tm, ldo = b1, alpha = 1;
+        } else {
+            K = b1, P = Inv + p + (long)p * ldi, ldp = ldi, Q = Tm, ldq = b1;
+            O = Inv + p + (long)(p + b1) * ldi, ldo = ldi, alpha = -1;
+        }
+    }
+    const int i0 = (t % tpm) * 32, j0 = (t / tpm) * 32;
+    const int tid = threadIdx.x, row = tid & 31, jc = tid >> 5;
+    if (stage == 1) {
+        // zero block: N x M, tile rows [j0, j0 + 32), columns [i0, i0 + 32)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int zr = j0 + row, zc = i0 + jc + 8 * u;
+            if (zr < N && zc < M) Z[zr + (long)zc * ldi] = V{0, 0};
+        }
+    }
+    if (i0 >= M || j0 >= N) return;   // block-uniform
+    __shared__ cplx<T> sp[32][33], sq[32][33];
+    cplx<T> acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = cplx<T>{0, 0};
+    // the next 32-deep slices of P and Q are loaded while the current one is
+    // multiplied (one load latency per product, not one per slice)
+    V na[4], nq[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = row, c = jc + 8 * u;
+            na[u] = (i0 + r < M && k0 + c < K) ? P[(i0 + r) + (long)(k0 + c) * ldp] : V{0, 0};
+            nq[u] = (k0 + r < K && j0 + c < N) ? Q[(k0 + r) + (long)(j0 + c) * ldq] : V{0, 0};
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            sp[jc + 8 * u][row] = cplx<T>{na[u].x, na[u].y};
+            sq[jc + 8 * u][row] = cplx<T>{nq[u].x, nq[u].y};
+        }
+        __syncthreads();
+        if (k0 + 32 < K) load(k0 + 32);
+#pragma unroll 8
+        for (int kk = 0; kk < 32; ++kk) {
+            const cplx<T> a = sp[kk][row];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] = cadd(acc[u], cmul(a, sq[jc + 8 * u][kk]));
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int r = i0 + row, c = j0 + jc + 8 * u;
+        if (r < M && c < N) O[r + (long)c * ldo] = V{alpha * acc[u].r, alpha * acc[u].i};
+    }
+}
+
+// Recursive doubling pays from 512 columns on: below, its 1 + 2 log2(bs / 32)
+// launches (each waiting for room beside the trailing update) cost more than
+// the one launch of the column-per-wave inverse.
+static bool use_trinv_rec(int bs) { return bs > 256; }
+
+// Elements of the product scratch per factor and matrix for a bs x bs block.
+static long trinv_rec_scratch(int bs)
+{
+    long need = 0;
+    for (int b = TI_LEAF; b < bs; b *= 2) {
+        const int nm = (bs - b + 2 * b - 1) / (2 * b);
+        need = std::max(need, (long)(nm - 1) * b * b + (long)b * std::min(b, bs - 2 * (nm - 1) * b - b));
+    }
+    return need;
+}
+
+// L^-1 (Li, unless null) and U^-1 (Ui, unless null) of the LU-factored
+// bs x bs block D by recursive doubling; scr: trinv_rec_scratch(bs) elements
+// per factor, the L part first, at stride sT per matrix.
+template <typename T>
+static int trinv_rec(const typename cvec<T>::t *D, long lda, int bs, typename cvec<T>::t *Li,
+                     typename cvec<T>::t *Ui, long ldi, typename cvec<T>::t *scr, hipStream_t st, int batch, long sD,
+                     long sI, long sT)
+{
+    MIDAS_CHECK(bs >= 1 && batch >= 1 && batch <= 65535, -1, "trinv: block %d, batch %d", bs, batch);
+    const int nz = Ui ? 2 : 1;
+    hipLaunchKernelGGL(k_trinv_leaf<T>, dim3((bs + TI_LEAF - 1) / TI_LEAF, batch, nz), dim3(64), 0, st, D, lda, sD, bs,
+                       Li, Ui, ldi, sI);
+    MIDAS_HIP(hipGetLastError());
+    const long sTz = trinv_rec_scratch(bs);
+    for (int b = TI_LEAF; b < bs; b *= 2) {
+        const int nm = (bs - b + 2 * b - 1) / (2 * b), tpm = (b + 31) / 32;
+        for (int stage = 0; stage < 2; ++stage) {
+            hipLaunchKernelGGL(k_trinv_merge<T>, dim3(nm * tpm * tpm, batch, nz), dim3(256), 0, st, D, lda, sD, bs, b,
+                               stage, Li, Ui, ldi, sI, scr, sT, sTz);
+            MIDAS_HIP(hipGetLastError());
+        }
+    }
+    return 0;
+}
+
 // Copy a (rows x cols) block between column-major arrays.
 template <typename T>
 __global__ void k_copy2d(const typename cvec<T>::t *S, long lds, typename cvec<T>::t *D, long ldd, int rows, int cols,
@@ -1148,7 +1371,11 @@ static int panel_rec(typename cvec<T>::t *A, long lda, int n, int k, int nb, int
     // A12 <- L11^-1 A12 through the staging buffer (nb x nb per matrix)
     V *Li = (V *)w.u12p, *P = Li + (long)h * h;
     const long sU = (long)w.nb * w.nb;
-    r = trinv<T, true>(A + k + (long)k * lda, lda, h, Li, h, st, w.batch, w.sA, sU);
+    // L11^-1 by recursive doubling, its product scratch after Li and P
+    if (use_trinv_rec(h) && trinv_rec_scratch(h) <= sU - (long)h * nb)
+        r = trinv_rec<T>(A + k + (long)k * lda, lda, h, Li, nullptr, h, Li + (long)h * nb, st, w.batch, w.sA, sU, sU);
+    else
+        r = trinv<T, true>(A + k + (long)k * lda, lda, h, Li, h, st, w.batch, w.sA, sU);
     if (r) return r;
     r = launch_cgemm<T>(h, h2, h, cone<T>(), Li, h, 0, A + k + (long)(k + h) * lda, lda, 0, czero<T>(), P, h, st,
                         w.batch, sU, w.sA, sU);
@@ -1181,8 +1408,12 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
     if (!w.diag_pivot) {
         int r = panel_rec<T>(A, lda, n, k, nb, ipiv, w, st);
         if (r) return r;
-        return trinv<T, true>(A + k + (long)k * lda, lda, nb, (V *)w.linv + (long)(k / w.nb) * w.nb * w.nb, w.nb, st,
-                              w.batch, w.sA, w.inv_stride());
+        V *Li = (V *)w.linv + (long)(k / w.nb) * w.nb * w.nb;
+        const long sU = (long)w.nb * w.nb;
+        if (use_trinv_rec(nb) && trinv_rec_scratch(nb) <= sU)
+            return trinv_rec<T>(A + k + (long)k * lda, lda, nb, Li, nullptr, w.nb, (V *)w.u12p, st, w.batch, w.sA,
+                                w.inv_stride(), sU);
+        return trinv<T, true>(A + k + (long)k * lda, lda, nb, Li, w.nb, st, w.batch, w.sA, w.inv_stride());
     }
     // Pivot search restricted to the diagonal block: rows [k, k + nb) factored
     // with partial pivoting among themselves (P11 A11 = L11 U11: the recursive
@@ -1198,7 +1429,13 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
     V *D = A + k + (long)k * lda;
     const long sI = w.inv_stride(), off = (long)(k / w.nb) * w.nb * w.nb;
     V *Ui = (V *)w.uinv + off;
-    if ((r = trinv_lu<T>(D, lda, nb, (V *)w.linv + off, Ui, w.nb, st, w.batch, w.sA, sI))) return r;
+    // both inverses by recursive doubling (product scratch: the panel's staging buffer)
+    const long sU = (long)w.nb * w.nb;
+    if (use_trinv_rec(nb) && 2 * trinv_rec_scratch(nb) <= sU)
+        r = trinv_rec<T>(D, lda, nb, (V *)w.linv + off, Ui, w.nb, (V *)w.u12p, st, w.batch, w.sA, sI, sU);
+    else
+        r = trinv_lu<T>(D, lda, nb, (V *)w.linv + off, Ui, w.nb, st, w.batch, w.sA, sI);
+    if (r) return r;
     const int mr = n - k - nb;
     if (mr <= 0) return 0;
     V *A21 = A + (k + nb) + (long)k * lda, *S = (V *)w.l21;

@@ -410,7 +410,7 @@ def test_getrf_diag_block_search(mid):
     """Diagonal-block pivot search on a diagonally dominant matrix: certified
     (max |L21| <= 1/sqrt(2)) and equal to LAPACK's factors, no interchanges."""
     import scipy.linalg as sl
-    n, nb = 900, 128
+    n, nb = 1200, 512
     print(f"\n[case] getrf diag-block search n={n} nb={nb}", flush=True)
     rng = np.random.default_rng(3)
     A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / n + np.diag(np.full(n, 4.0 + 1j))
