@@ -123,7 +123,9 @@ void *midas_stream(midas_ctx *ctx);           /* hipStream_t of the context */
  * its diagonal block, certified by max |L21| in the report; 2 every solve;
  * 0 whole columns), "mom_matrix_free" (1: MoM mixed solves with at most two
  * directions recompute the FP64 matrix for their residuals instead of storing
- * it).
+ * it), "u12_split" (1: LUs with a single-term split trailing update form the
+ * U12 block rows on the same split-precision matrix cores, in place as
+ * A12 -= (I - L11^-1) A12; 0 on the complex-FP32 GEMM).
  * They change speed, not results beyond rounding.  No reference counterpart
  * (the reference fixes its 32 x 32 ScaLAPACK blocks, Initialization.f90:66-67). */
 int midas_tune_set(const char *name, int value);

@@ -20,6 +20,9 @@ repetitions), the parameters midas_tune_set exposes:
   mom_matrix_free     FP64 residuals of the MoM solve from the cells instead
                       of a stored FP64 matrix (timed over the whole call:
                       fill, solve, far field)
+  u12_split           U12 = L11^-1 A12 of the split-mode LU on the split
+                      GEMM (in place, A12 -= (I - L11^-1) A12) or on the
+                      complex-FP32 GEMM
 
 and, on a DLCA aggregate solved by CBFM-E with ACA (midas_compute_efields_cbfm,
 wall time of the call):
@@ -74,6 +77,7 @@ COMMENTS = {
     "cbfm_graph": "CBFM block batches (fills, excitations, FP64 LU, solve) launched as one hipGraph each",
     "lu_pivot_search": "mixed LU pivot search in each panel's diagonal block (certified): 1 MoM solves, 2 all; 0 whole columns",
     "mom_matrix_free": "MoM mixed solves (<= 2 directions): FP64 residuals from the cells, the FP64 matrix not stored",
+    "u12_split": "single-term split LUs: U12 = L11^-1 A12 on the split-precision matrix cores",
 }
 
 
@@ -210,6 +214,7 @@ def main():
     sweep(m, "lu_nb_fp64", [64, 128, 256], mid, api.SOLVE_FP64, reps, log)
     # the MoM solve's pivot search and residual form, on the largest sphere
     sweep(m, "lu_pivot_search", [0, 1], big, f16, reps, log)
+    sweep(m, "u12_split", [0, 1], big, f16, reps, log)
     rx = (np.array([0.0, 90.0]), np.array([0.0, 0.0]))
     mom_call = lambda: m.compute_efields_mom(big, big.k0, (tth, tph), rx, mode=f16, want_E=False)
     mom_call()

@@ -67,10 +67,10 @@ __host__ __device__ inline long plane_off(int pc, int row, int k, int rtiles, in
     return ((((long)pc * rtiles + row / T_M) * kst + k / T_K) * T_M + row % T_M) * T_K + k % T_K;
 }
 
-// A (m x K, lda) -> planes; thread per (row, group of 8 k)
+// A (m x K, lda) -> planes (of I - A with eye_minus); thread per (row, group of 8 k)
 template <bool BF, int P>
 __global__ void k_split_a(const float2 *__restrict__ A, long lda, int m, int K, int rtiles, int kst, u16 *__restrict__ out,
-                          long sA = 0, long sOut = 0)
+                          long sA = 0, long sOut = 0, int eye_minus = 0)
 {
     A += blockIdx.z * sA;      // matrix of a batch
     out += blockIdx.z * sOut;
@@ -81,6 +81,7 @@ __global__ void k_split_a(const float2 *__restrict__ A, long lda, int m, int K, 
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
         float2 v = (row < m && k0 + t < K) ? A[(long)row + (long)(k0 + t) * lda] : float2{0.f, 0.f};
+        if (eye_minus) v = float2{(row == k0 + t && row < m && row < K ? 1.f : 0.f) - v.x, -v.y};
         u16 hr = half_t<BF>::cvt(v.x), hi = half_t<BF>::cvt(v.y);
         h[0][0][t] = hr;
         h[0][1][t] = hi;
@@ -659,7 +660,7 @@ static size_t trail16_matrix_bytes(int m, int n, int k)
 
 template <bool BF, int P, int TERMS>
 static int run_trail(int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C, long ldc,
-                     char *scratch, hipStream_t st, int batch, long stride)
+                     char *scratch, hipStream_t st, int batch, long stride, bool eye_minus_a)
 {
     const int mt = (m + T_M - 1) / T_M, nt = (n + T_N - 1) / T_N, kst = kstages(k);
     u16 *Ap = (u16 *)scratch;
@@ -680,7 +681,7 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
             const long mb = (long)trail16_matrix_bytes(m, n, k);
             const long sP = mb / 2, sS = mb / 4;
             hipLaunchKernelGGL((k_split_a<BF, 1>), dim3(mt * T_M / 256 + 1, ks * 2, batch), dim3(256), 0, st, A, lda, m,
-                               k, mt, ks, Ap, stride, sP);
+                               k, mt, ks, Ap, stride, sP, (int)eye_minus_a);
             hipLaunchKernelGGL((k_split_b<BF, 1>), dim3(nt * T_N, batch), dim3(64), 0, st, B, ldb, k, n, nt, ks, Bq, sq,
                                stride, sP, sS);
             // algorithmic bytes: C read and written, both operands once in FP16
@@ -695,7 +696,7 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
     }
     MIDAS_CHECK(kst * T_K <= 512, -1, "trail16: k = %d above 512", k);
     hipLaunchKernelGGL((k_split_a<BF, P>), dim3(mt * T_M / 256 + 1, kst * 2), dim3(256), 0, st, A, lda, m, k, mt, kst,
-                       Ap);
+                       Ap, 0L, 0L, (int)eye_minus_a);
     hipLaunchKernelGGL((k_split_b<BF, P>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, kst, Bp, sc);
     // live accounting of the GEMM kernel alone: algorithmic flops 8 m n k
     // (complex multiply-add); bytes = C read and written (complex FP32) plus
@@ -711,16 +712,16 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
 }
 
 int launch_trail16(int mode, int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
-                   long ldc, void *scratch, hipStream_t st, int batch, long stride)
+                   long ldc, void *scratch, hipStream_t st, int batch, long stride, bool eye_minus_a)
 {
     if (m <= 0 || n <= 0 || k <= 0) return 0;
     MIDAS_CHECK(scratch, -1, "trail16: no scratch");
     char *s = (char *)scratch;
     switch (mode) {
-    case TRAIL_F16_1: return run_trail<false, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride);
-    case TRAIL_F16_3: return run_trail<false, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride);
-    case TRAIL_BF16_1: return run_trail<true, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride);
-    case TRAIL_BF16_3: return run_trail<true, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride);
+    case TRAIL_F16_1: return run_trail<false, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, eye_minus_a);
+    case TRAIL_F16_3: return run_trail<false, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, eye_minus_a);
+    case TRAIL_BF16_1: return run_trail<true, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, eye_minus_a);
+    case TRAIL_BF16_3: return run_trail<true, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, eye_minus_a);
     }
     MIDAS_CHECK(false, -1, "trail16: bad mode %d", mode);
 }
@@ -788,7 +789,7 @@ int TrailingGemm<float>::operator()(int m, int n, int k, const void *A, long lda
     MIDAS_CHECK(batch == 1 || mode == TRAIL_F16_1 || mode == TRAIL_BF16_1, -1,
                 "trail16: batched split-precision updates are single-term only");
     return launch_trail16(mode, m, n, k, (const float2 *)A, lda, (const float2 *)B, ldb, (float2 *)C, ldc, scratch,
-                          st, batch, stride);
+                          st, batch, stride, eye_minus_a);
 }
 
 }  // namespace midas

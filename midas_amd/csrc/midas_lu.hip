@@ -1450,15 +1450,25 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
 }
 
 // U12(:, c0:c1) = L11^-1 A12(:, c0:c1) through a staging buffer (w.u12, or
-// stage: nb x (c1 - c0) per matrix at stride sst), for the panel at k.
+// stage: nb x (c1 - c0) per matrix at stride sst), for the panel at k.  With
+// a single-term split trailing GEMM (ts) and u12_split set, in place on its
+// matrix cores instead: A12 -= (I - L11^-1) A12, the operand I - L11^-1
+// formed while it is split, A12 split before the update overwrites it.
 template <typename T>
 static int u12_solve(typename cvec<T>::t *A, long lda, int k, int nb, int c0, int c1, LuWork<T> &w, hipStream_t st,
-                     cplx<T> *stage = nullptr, long sst = 0)
+                     cplx<T> *stage = nullptr, long sst = 0, const TrailingGemm<T> *ts = nullptr)
 {
     typedef typename cvec<T>::t V;
     const int nc = c1 - c0;
     if (nc <= 0) return 0;
     const V *Li = (const V *)w.linv + (long)(k / w.nb) * w.nb * w.nb;
+    if (ts && sizeof(T) == 4 && w.batch == 1 && (ts->mode == TRAIL_F16_1 || ts->mode == TRAIL_BF16_1) &&
+        tune(TUNE_U12_SPLIT)) {
+        TrailingGemm<T> t = *ts;
+        t.eye_minus_a = true;
+        V *A12 = A + k + (long)c0 * lda;
+        return t(nb, nc, nb, Li, w.nb, A12, lda, A12, lda, st);
+    }
     V *tmp = (V *)(stage ? stage : w.u12);
     const long st12 = stage ? sst : (long)w.nb * w.n;
     int r = launch_cgemm<T>(nb, nc, nb, cone<T>(), Li, w.nb, 0, A + k + (long)c0 * lda, lda, 0, czero<T>(), tmp, nb,
@@ -1484,7 +1494,7 @@ static int outer_panel(typename cvec<T>::t *A, long lda, int n, int k, int W, in
     hipLaunchKernelGGL(k_laswp<T>, dim3((h2 + 255) / 256, w.batch), dim3(256), 0, ps, A, lda, w.sA, ipiv, (long)w.n, k,
                        k + h1, k + h1, k + W, 0, 0);
     MIDAS_HIP(hipGetLastError());
-    r = u12_solve<T>(A, lda, k, h1, k + h1, k + W, w, ps, w.u12p, (long)w.nb * w.nb);
+    r = u12_solve<T>(A, lda, k, h1, k + h1, k + W, w, ps, w.u12p, (long)w.nb * w.nb, &tgp);
     if (r) return r;
     r = tgp(n - k - h1, h2, h1, A + (k + h1) + (long)k * lda, lda, A + k + (long)(k + h1) * lda, lda,
             A + (k + h1) + (long)(k + h1) * lda, lda, ps);
@@ -1506,13 +1516,13 @@ static int outer_update(typename cvec<T>::t *A, long lda, int n, int k, int W, i
 {
     if (c1 <= c0) return 0;
     const int h1 = min(w.nb, W), h2 = W - h1;
-    int r = u12_solve<T>(A, lda, k, h1, c0, c1, w, st);
+    int r = u12_solve<T>(A, lda, k, h1, c0, c1, w, st, nullptr, 0, &tg);
     if (r) return r;
     if (h2 > 0) {
         r = tg(h2, c1 - c0, h1, A + (k + h1) + (long)k * lda, lda, A + k + (long)c0 * lda, lda,
                A + (k + h1) + (long)c0 * lda, lda, st);
         if (r) return r;
-        r = u12_solve<T>(A, lda, k + h1, h2, c0, c1, w, st);
+        r = u12_solve<T>(A, lda, k + h1, h2, c0, c1, w, st, nullptr, 0, &tg);
         if (r) return r;
     }
     const int nr = n - k - W;

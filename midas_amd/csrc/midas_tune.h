@@ -18,6 +18,7 @@ enum {
     TUNE_CBFM_GRAPH,
     TUNE_LU_PIVOT_SEARCH,
     TUNE_MOM_MATRIX_FREE,
+    TUNE_U12_SPLIT,
     TUNE_COUNT
 };
 int tune(int which);

@@ -17,3 +17,4 @@
 #define MIDAS_TUNED_CBFM_GRAPH             1        // CBFM block batches (fills, excitations, FP64 LU, solve) launched as one hipGraph each
 #define MIDAS_TUNED_LU_PIVOT_SEARCH        1        // mixed LU pivot search in each panel's diagonal block (certified): 1 MoM solves, 2 all; 0 whole columns
 #define MIDAS_TUNED_MOM_MATRIX_FREE        1        // MoM mixed solves (<= 2 directions): FP64 residuals from the cells, the FP64 matrix not stored
+#define MIDAS_TUNED_U12_SPLIT              1        // single-term split LUs: U12 = L11^-1 A12 on the split-precision matrix cores
