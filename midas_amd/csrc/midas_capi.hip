@@ -518,7 +518,8 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
 namespace midas {
 int mfma_probe(int which, double *dout, hipStream_t st);
 int launch_trail16(int mode, int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
-                   long ldc, void *scratch, hipStream_t st, int batch = 1, long stride = 0, bool eye_minus_a = false);
+                   long ldc, void *scratch, hipStream_t st, int batch = 1, long stride = 0, bool eye_minus_a = false,
+                   bool eye_minus_b = false);
 }
 
 // ===========================================================================

@@ -107,8 +107,9 @@ constexpr int SPLIT_B_KMAX = 2048;
 template <bool BF, int P>
 __global__ __launch_bounds__(64) void k_split_b(const float2 *__restrict__ B, long ldb, int K, int n, int rtiles,
                                                 int kst, u16 *__restrict__ out, float *__restrict__ scale,
-                                                long sB = 0, long sOut = 0, long sScale = 0)
+                                                long sB = 0, long sOut = 0, long sScale = 0, int eye_minus = 0)
 {
+    // (eye_minus: the planes of I - B, the identity of B's rows)
     B += blockIdx.y * sB;      // matrix of a batch
     out += blockIdx.y * sOut;
     scale += blockIdx.y * sScale;
@@ -124,6 +125,7 @@ __global__ __launch_bounds__(64) void k_split_b(const float2 *__restrict__ B, lo
         for (int q = 0; q < 8; ++q) {
             const int kk = 512 * g + 8 * t + q;
             v[g][q] = (col < n && kk < K) ? B[(long)kk + (long)col * ldb] : float2{0.f, 0.f};
+            if (eye_minus) v[g][q] = float2{(col < n && kk < K && kk == col ? 1.f : 0.f) - v[g][q].x, -v[g][q].y};
             mx = fmaxf(mx, fmaxf(fabsf(v[g][q].x), fabsf(v[g][q].y)));
         }
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
@@ -660,7 +662,7 @@ static size_t trail16_matrix_bytes(int m, int n, int k)
 
 template <bool BF, int P, int TERMS>
 static int run_trail(int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C, long ldc,
-                     char *scratch, hipStream_t st, int batch, long stride, bool eye_minus_a)
+                     char *scratch, hipStream_t st, int batch, long stride, bool eye_minus_a, bool eye_minus_b)
 {
     const int mt = (m + T_M - 1) / T_M, nt = (n + T_N - 1) / T_N, kst = kstages(k);
     u16 *Ap = (u16 *)scratch;
@@ -683,7 +685,7 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
             hipLaunchKernelGGL((k_split_a<BF, 1>), dim3(mt * T_M / 256 + 1, ks * 2, batch), dim3(256), 0, st, A, lda, m,
                                k, mt, ks, Ap, stride, sP, (int)eye_minus_a);
             hipLaunchKernelGGL((k_split_b<BF, 1>), dim3(nt * T_N, batch), dim3(64), 0, st, B, ldb, k, n, nt, ks, Bq, sq,
-                               stride, sP, sS);
+                               stride, sP, sS, (int)eye_minus_b);
             // algorithmic bytes: C read and written, both operands once in FP16
             ProfScope ps(st, PROF_TRAIL, 8.0 * m * n * k * batch, (16.0 * m * n + 4.0 * ((double)m + n) * k) * batch);
             const long a_tile = (long)T_M * ks * T_K * 2 * 2;
@@ -697,7 +699,8 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
     MIDAS_CHECK(kst * T_K <= 512, -1, "trail16: k = %d above 512", k);
     hipLaunchKernelGGL((k_split_a<BF, P>), dim3(mt * T_M / 256 + 1, kst * 2), dim3(256), 0, st, A, lda, m, k, mt, kst,
                        Ap, 0L, 0L, (int)eye_minus_a);
-    hipLaunchKernelGGL((k_split_b<BF, P>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, kst, Bp, sc);
+    hipLaunchKernelGGL((k_split_b<BF, P>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, kst, Bp, sc, 0L, 0L, 0L,
+                       (int)eye_minus_b);
     // live accounting of the GEMM kernel alone: algorithmic flops 8 m n k
     // (complex multiply-add); bytes = C read and written (complex FP32) plus
     // one read of the operand planes (P planes of 2 B per real component)
@@ -712,16 +715,17 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
 }
 
 int launch_trail16(int mode, int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
-                   long ldc, void *scratch, hipStream_t st, int batch, long stride, bool eye_minus_a)
+                   long ldc, void *scratch, hipStream_t st, int batch, long stride, bool eye_minus_a, bool eye_minus_b)
 {
     if (m <= 0 || n <= 0 || k <= 0) return 0;
     MIDAS_CHECK(scratch, -1, "trail16: no scratch");
     char *s = (char *)scratch;
+    const bool ea = eye_minus_a, eb = eye_minus_b;
     switch (mode) {
-    case TRAIL_F16_1: return run_trail<false, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, eye_minus_a);
-    case TRAIL_F16_3: return run_trail<false, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, eye_minus_a);
-    case TRAIL_BF16_1: return run_trail<true, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, eye_minus_a);
-    case TRAIL_BF16_3: return run_trail<true, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, eye_minus_a);
+    case TRAIL_F16_1: return run_trail<false, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb);
+    case TRAIL_F16_3: return run_trail<false, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb);
+    case TRAIL_BF16_1: return run_trail<true, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb);
+    case TRAIL_BF16_3: return run_trail<true, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb);
     }
     MIDAS_CHECK(false, -1, "trail16: bad mode %d", mode);
 }
@@ -789,7 +793,7 @@ int TrailingGemm<float>::operator()(int m, int n, int k, const void *A, long lda
     MIDAS_CHECK(batch == 1 || mode == TRAIL_F16_1 || mode == TRAIL_BF16_1, -1,
                 "trail16: batched split-precision updates are single-term only");
     return launch_trail16(mode, m, n, k, (const float2 *)A, lda, (const float2 *)B, ldb, (float2 *)C, ldc, scratch,
-                          st, batch, stride, eye_minus_a);
+                          st, batch, stride, eye_minus_a, eye_minus_b);
 }
 
 }  // namespace midas

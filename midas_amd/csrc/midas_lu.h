@@ -67,6 +67,7 @@ template <typename T> struct TrailingGemm {
     long stride = 0;
     bool library = true;            // FP64: rocBLAS ZGEMM allowed (where fp64_gemm_library asks for it)
     bool eye_minus_a = false;       // split modes: C -= (I - A) B instead (the identity of A's order)
+    bool eye_minus_b = false;       // split modes: C -= A (I - B) instead
     int operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C, long ldc,
                    hipStream_t st) const;
 };

@@ -664,9 +664,10 @@ __global__ __launch_bounds__(NT, NT == 256 ? 5 : 1) void k_getf2_local(typename 
     }
 }
 
-// L21 from the staging buffer S (rows x cols, lds) back into the matrix, and
-// the largest modulus of its entries into *amax (float bits, atomic max: the
-// order of non-negative floats is that of their bits).
+// L21 from the staging buffer S (rows x cols, lds) back into the matrix (D;
+// null: L21 is in place already), and the largest modulus of its entries into
+// *amax (float bits, atomic max: the order of non-negative floats is that of
+// their bits).
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy_amax(const typename cvec<T>::t *S, long lds, long sS,
                                                    typename cvec<T>::t *D, long ldd, long sD, int rows, int cols,
@@ -681,7 +682,7 @@ __global__ __launch_bounds__(256) void k_copy_amax(const typename cvec<T>::t *S,
     for (int c = blockIdx.y; c < cols; c += gridDim.y)
         for (int r = blockIdx.x * 256 + threadIdx.x; r < rows; r += gridDim.x * 256) {
             const typename cvec<T>::t v = S[(long)r + (long)c * lds];
-            D[(long)r + (long)c * ldd] = v;
+            if (D) D[(long)r + (long)c * ldd] = v;
             m = fmaxf(m, (float)sqrt((double)v.x * v.x + (double)v.y * v.y));
         }
 #pragma unroll
@@ -1402,7 +1403,8 @@ static int panel_rec(typename cvec<T>::t *A, long lda, int n, int k, int nb, int
 // diagonal block L11: U12 = L11^-1 A12 becomes one GEMM on the main stream,
 // and getrs reuses it.
 template <typename T>
-static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ipiv, LuWork<T> &w, hipStream_t st)
+static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ipiv, LuWork<T> &w, hipStream_t st,
+                 const TrailingGemm<T> *ts = nullptr)
 {
     typedef typename cvec<T>::t V;
     if (!w.diag_pivot) {
@@ -1440,6 +1442,17 @@ static int panel(typename cvec<T>::t *A, long lda, int n, int k, int nb, int *ip
     if (mr <= 0) return 0;
     V *A21 = A + (k + nb) + (long)k * lda, *S = (V *)w.l21;
     const long sS = (long)mr * nb;
+    if (ts && ts->scratch && sizeof(T) == 4 && w.batch == 1 && (ts->mode == TRAIL_F16_1 || ts->mode == TRAIL_BF16_1) &&
+        tune(TUNE_L21_SPLIT)) {
+        // on the split-precision matrix cores, in place: A21 -= A21 (I - U11^-1)
+        TrailingGemm<T> t = *ts;
+        t.eye_minus_b = true;
+        if ((r = t(mr, nb, nb, A21, lda, Ui, w.nb, A21, lda, st))) return r;
+        hipLaunchKernelGGL(k_copy_amax<T>, dim3(min((mr + 255) / 256, 64), min(nb, 32), 1), dim3(256), 0, st, A21, lda,
+                           0L, (V *)nullptr, lda, 0L, mr, nb, w.lmax);
+        MIDAS_HIP(hipGetLastError());
+        return 0;
+    }
     if ((r = launch_cgemm<T>(mr, nb, nb, cone<T>(), A21, lda, 0, Ui, w.nb, 0, czero<T>(), S, mr, st, w.batch, w.sA, sI,
                              sS)))
         return r;
@@ -1489,7 +1502,7 @@ static int outer_panel(typename cvec<T>::t *A, long lda, int n, int k, int W, in
                        const TrailingGemm<T> &tgp)
 {
     const int h1 = min(w.nb, W), h2 = W - h1;
-    int r = panel<T>(A, lda, n, k, h1, ipiv, w, ps);
+    int r = panel<T>(A, lda, n, k, h1, ipiv, w, ps, &tgp);
     if (r || h2 <= 0) return r;
     hipLaunchKernelGGL(k_laswp<T>, dim3((h2 + 255) / 256, w.batch), dim3(256), 0, ps, A, lda, w.sA, ipiv, (long)w.n, k,
                        k + h1, k + h1, k + W, 0, 0);
@@ -1499,7 +1512,7 @@ static int outer_panel(typename cvec<T>::t *A, long lda, int n, int k, int W, in
     r = tgp(n - k - h1, h2, h1, A + (k + h1) + (long)k * lda, lda, A + k + (long)(k + h1) * lda, lda,
             A + (k + h1) + (long)(k + h1) * lda, lda, ps);
     if (r) return r;
-    r = panel<T>(A, lda, n, k + h1, h2, ipiv, w, ps);
+    r = panel<T>(A, lda, n, k + h1, h2, ipiv, w, ps, &tgp);
     if (r) return r;
     hipLaunchKernelGGL(k_laswp<T>, dim3((h1 + 255) / 256, w.batch), dim3(256), 0, ps, A, lda, w.sA, ipiv, (long)w.n, k + h1,
                        k + W, k, k + h1, 0, 0);

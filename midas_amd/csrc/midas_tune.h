@@ -19,6 +19,7 @@ enum {
     TUNE_LU_PIVOT_SEARCH,
     TUNE_MOM_MATRIX_FREE,
     TUNE_U12_SPLIT,
+    TUNE_L21_SPLIT,
     TUNE_COUNT
 };
 int tune(int which);

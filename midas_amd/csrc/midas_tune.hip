@@ -29,6 +29,7 @@ static TuneEntry g_tune[TUNE_COUNT] = {
     {"lu_pivot_search", MIDAS_TUNED_LU_PIVOT_SEARCH, 0, 2},
     {"mom_matrix_free", MIDAS_TUNED_MOM_MATRIX_FREE, 0, 1},
     {"u12_split", MIDAS_TUNED_U12_SPLIT, 0, 1},
+    {"l21_split", MIDAS_TUNED_L21_SPLIT, 0, 1},
 };
 
 int tune(int which) { return g_tune[which].value; }

@@ -18,3 +18,4 @@
 #define MIDAS_TUNED_LU_PIVOT_SEARCH        1        // mixed LU pivot search in each panel's diagonal block (certified): 1 MoM solves, 2 all; 0 whole columns
 #define MIDAS_TUNED_MOM_MATRIX_FREE        1        // MoM mixed solves (<= 2 directions): FP64 residuals from the cells, the FP64 matrix not stored
 #define MIDAS_TUNED_U12_SPLIT              1        // single-term split LUs: U12 = L11^-1 A12 on the split-precision matrix cores
+#define MIDAS_TUNED_L21_SPLIT              1        // ... and L21 = A21 U11^-1 of the diagonal-block pivot search (two-panel steps)

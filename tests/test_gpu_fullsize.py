@@ -117,7 +117,8 @@ def test_c4_multi_rhs_full_size(mid):
     S, ce, ca, E, info, av = out[api.SOLVE_F16_IR]
     S64, ce64, ca64, E64, _, av64 = out[api.SOLVE_FP64]
     print("field err", _rel(E, E64), "S err", _rel(S, S64), "Q_av err", _rel(av, av64), flush=True)
-    assert info["status"] == api.STATUS_CONVERGED and info["berr"] <= 1e-14
+    # refinement's stopping rule (ZCGESV's): max(1e-14, sqrt(n) eps), 4.2e-14 at n = 36,000
+    assert info["status"] == api.STATUS_CONVERGED and info["berr"] <= max(1e-14, np.sqrt(3 * s.n) * np.finfo(float).eps)
     assert _rel(E, E64) <= 1e-12
     assert _rel(S, S64) <= 1e-12
     assert _rel(ce, ce64) <= 1e-12 and _rel(ca, ca64) <= 1e-12

@@ -391,11 +391,13 @@ def test_cbfm_graph_matches_eager(mid, dlca6, sr, iws, one_per_batch):
 
 
 @pytest.mark.parametrize("name,value", [("panel_leaf", 32), ("panel_leaf", 128), ("trail_group_kib", 512),
-                                        ("lu_nb_split", 768), ("lu_nb_fp64", 256)])
+                                        ("lu_nb_split", 768), ("lu_nb_fp64", 256), ("u12_split", 0),
+                                        ("l21_split", 0), ("lu_pivot_search", 2)])
 def test_tuned_parameters_keep_results(mid, name, value):
     """Every schedule parameter the autotuner varies changes the time, not the
     solution: FP64 for lu_nb_fp64, the FP16 split mode (with two-panel steps
-    forced on) for the others."""
+    forced on) for the others (lu_pivot_search = 2: the diagonal-block search
+    with its split-precision L21)."""
     rng = np.random.default_rng(40)
     n = 2600
     A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 2.5 * np.sqrt(2 * n) * np.eye(n)
