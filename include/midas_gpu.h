@@ -125,7 +125,9 @@ void *midas_stream(midas_ctx *ctx);           /* hipStream_t of the context */
  * directions recompute the FP64 matrix for their residuals instead of storing
  * it), "u12_split" (1: LUs with a single-term split trailing update form the
  * U12 block rows on the same split-precision matrix cores, in place as
- * A12 -= (I - L11^-1) A12; 0 on the complex-FP32 GEMM).
+ * A12 -= (I - L11^-1) A12; 0 on the complex-FP32 GEMM), "l21_split" (1: in
+ * their two-panel steps the diagonal-block pivot search's L21 = A21 U11^-1 the
+ * same way, A21 -= A21 (I - U11^-1)).
  * They change speed, not results beyond rounding.  No reference counterpart
  * (the reference fixes its 32 x 32 ScaLAPACK blocks, Initialization.f90:66-67). */
 int midas_tune_set(const char *name, int value);

@@ -23,6 +23,8 @@ repetitions), the parameters midas_tune_set exposes:
   u12_split           U12 = L11^-1 A12 of the split-mode LU on the split
                       GEMM (in place, A12 -= (I - L11^-1) A12) or on the
                       complex-FP32 GEMM
+  l21_split           the same for L21 = A21 U11^-1 of the diagonal-block
+                      pivot search (A21 -= A21 (I - U11^-1))
 
 and, on a DLCA aggregate solved by CBFM-E with ACA (midas_compute_efields_cbfm,
 wall time of the call):
@@ -78,6 +80,7 @@ COMMENTS = {
     "lu_pivot_search": "mixed LU pivot search in each panel's diagonal block (certified): 1 MoM solves, 2 all; 0 whole columns",
     "mom_matrix_free": "MoM mixed solves (<= 2 directions): FP64 residuals from the cells, the FP64 matrix not stored",
     "u12_split": "single-term split LUs: U12 = L11^-1 A12 on the split-precision matrix cores",
+    "l21_split": "... and L21 = A21 U11^-1 of the diagonal-block pivot search (two-panel steps)",
 }
 
 
@@ -215,6 +218,7 @@ def main():
     # the MoM solve's pivot search and residual form, on the largest sphere
     sweep(m, "lu_pivot_search", [0, 1], big, f16, reps, log)
     sweep(m, "u12_split", [0, 1], big, f16, reps, log)
+    sweep(m, "l21_split", [0, 1], big, f16, reps, log)
     rx = (np.array([0.0, 90.0]), np.array([0.0, 0.0]))
     mom_call = lambda: m.compute_efields_mom(big, big.k0, (tth, tph), rx, mode=f16, want_E=False)
     mom_call()

@@ -4,7 +4,7 @@
 // midas_tune_set (include/midas_gpu.h).
 #pragma once
 #define MIDAS_TUNED_LU_NB_SPLIT            1024     // LU outer step, single-term split modes, n >= lu_two_panel_min_n
-#define MIDAS_TUNED_LU_TWO_PANEL_MIN_N     36000    // from this order on the split modes take two-panel steps
+#define MIDAS_TUNED_LU_TWO_PANEL_MIN_N     12000    // from this order on the split modes take two-panel steps
 #define MIDAS_TUNED_LU_NB_MIXED            512      // LU panel, the other mixed modes (and smaller n)
 #define MIDAS_TUNED_LU_NB_FP64             256      // LU panel, FP64
 #define MIDAS_TUNED_PANEL_LEAF             128      // columns of a recursive panel's leaves
