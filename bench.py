@@ -37,7 +37,7 @@ MODE_TERMS = {"fp64": 1, "c32": 1, "f16": 1, "f16x3": 3, "bf16": 1, "bf16x3": 3}
 # WRITE_SIZE on this workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors
 # a 1 GiB device copy reads on the same counters), keyed by (cells, mode, nb);
 # other configurations report null.  Source: profiles/lu_trail16_f16_grs20k_hbm_traffic.md
-TRAFFIC_PER_LAUNCH = {(20000, "f16", 0): 7.730e9}
+TRAFFIC_PER_LAUNCH = {(20000, "f16", 0): 3.344e9}
 
 
 def log(*a):

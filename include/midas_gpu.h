@@ -127,7 +127,9 @@ void *midas_stream(midas_ctx *ctx);           /* hipStream_t of the context */
  * U12 block rows on the same split-precision matrix cores, in place as
  * A12 -= (I - L11^-1) A12; 0 on the complex-FP32 GEMM), "l21_split" (1: in
  * their two-panel steps the diagonal-block pivot search's L21 = A21 U11^-1 the
- * same way, A21 -= A21 (I - U11^-1)).
+ * same way, A21 -= A21 (I - U11^-1)), "lu_lookahead" (1: LU panels factored
+ * one step ahead on a stream of their own; 0: on the main stream, one stream
+ * in all, for profilers that serialise dispatches).
  * They change speed, not results beyond rounding.  No reference counterpart
  * (the reference fixes its 32 x 32 ScaLAPACK blocks, Initialization.f90:66-67). */
 int midas_tune_set(const char *name, int value);

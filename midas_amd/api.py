@@ -111,7 +111,7 @@ def _p(a):
 TUNE_NAMES = ("lu_nb_split", "lu_two_panel_min_n", "lu_nb_mixed", "lu_nb_fp64", "panel_leaf", "trail_group_kib",
               "refine_graph", "zgemm_tile", "fp64_gemm_library", "cbfm_block_mode", "cgemm_tile", "cbfm_graph",
               "lu_pivot_search", "mom_matrix_free", "u12_split",
-              "l21_split")
+              "l21_split", "lu_lookahead")
 
 
 def tune_get(name):

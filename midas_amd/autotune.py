@@ -81,6 +81,7 @@ COMMENTS = {
     "mom_matrix_free": "MoM mixed solves (<= 2 directions): FP64 residuals from the cells, the FP64 matrix not stored",
     "u12_split": "single-term split LUs: U12 = L11^-1 A12 on the split-precision matrix cores",
     "l21_split": "... and L21 = A21 U11^-1 of the diagonal-block pivot search (two-panel steps)",
+    "lu_lookahead": "LU panels on their own stream, one step ahead (0: on the main stream)",
 }
 
 

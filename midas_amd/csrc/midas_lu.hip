@@ -1555,7 +1555,9 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
           TrailingGemm<T> tgp)
 {
     typedef typename cvec<T>::t V;
-    hipStream_t ps = w.pstream;
+    // lu_lookahead = 0 keeps the panels on the main stream (one stream, no
+    // cross-stream waits: for profilers that serialise dispatches)
+    hipStream_t ps = tune(TUNE_LU_LOOKAHEAD) ? w.pstream : st;
     MIDAS_CHECK(w.batch == 1 || tg.batch == w.batch, -1, "getrf: trailing GEMM not batched like the factorisation");
     MIDAS_CHECK(w.outer == 1 || (w.outer == 2 && tgp.scratch != tg.scratch) || sizeof(T) == 8 || tg.mode == TRAIL_NATIVE,
                 -1, "getrf: two-panel steps need a panel-stream GEMM scratch of its own");

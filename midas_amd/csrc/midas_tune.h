@@ -20,6 +20,7 @@ enum {
     TUNE_MOM_MATRIX_FREE,
     TUNE_U12_SPLIT,
     TUNE_L21_SPLIT,
+    TUNE_LU_LOOKAHEAD,
     TUNE_COUNT
 };
 int tune(int which);

@@ -30,6 +30,7 @@ static TuneEntry g_tune[TUNE_COUNT] = {
     {"mom_matrix_free", MIDAS_TUNED_MOM_MATRIX_FREE, 0, 1},
     {"u12_split", MIDAS_TUNED_U12_SPLIT, 0, 1},
     {"l21_split", MIDAS_TUNED_L21_SPLIT, 0, 1},
+    {"lu_lookahead", MIDAS_TUNED_LU_LOOKAHEAD, 0, 1},
 };
 
 int tune(int which) { return g_tune[which].value; }

@@ -19,3 +19,4 @@
 #define MIDAS_TUNED_MOM_MATRIX_FREE        1        // MoM mixed solves (<= 2 directions): FP64 residuals from the cells, the FP64 matrix not stored
 #define MIDAS_TUNED_U12_SPLIT              1        // single-term split LUs: U12 = L11^-1 A12 on the split-precision matrix cores
 #define MIDAS_TUNED_L21_SPLIT              1        // ... and L21 = A21 U11^-1 of the diagonal-block pivot search (two-panel steps)
+#define MIDAS_TUNED_LU_LOOKAHEAD           1        // LU panels on their own stream, one step ahead (0: on the main stream)

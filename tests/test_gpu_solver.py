@@ -392,7 +392,7 @@ def test_cbfm_graph_matches_eager(mid, dlca6, sr, iws, one_per_batch):
 
 @pytest.mark.parametrize("name,value", [("panel_leaf", 32), ("panel_leaf", 128), ("trail_group_kib", 512),
                                         ("lu_nb_split", 768), ("lu_nb_fp64", 256), ("u12_split", 0),
-                                        ("l21_split", 0), ("lu_pivot_search", 2)])
+                                        ("l21_split", 0), ("lu_pivot_search", 2), ("lu_lookahead", 0)])
 def test_tuned_parameters_keep_results(mid, name, value):
     """Every schedule parameter the autotuner varies changes the time, not the
     solution: FP64 for lu_nb_fp64, the FP16 split mode (with two-panel steps
