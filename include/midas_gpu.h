@@ -96,9 +96,10 @@ typedef struct {
 int midas_init(int device, midas_ctx **ctx);
 int midas_finalize(midas_ctx *ctx);
 /* The solvers keep their device workspace (the FP32 copy of the matrix, LU
- * work buffers, refinement vectors) in the context and grow it on demand, so
- * that repeated solves of one size allocate nothing; this frees it.  No
- * reference counterpart (ScaLAPACK workspaces are the caller's). */
+ * work buffers, refinement vectors; the CBFM call's block matrices, CBFs,
+ * Gram and reduced matrices, ACA factors) in the context and grow it on
+ * demand, so that repeated calls of one size allocate nothing; this frees it.
+ * No reference counterpart (ScaLAPACK workspaces are the caller's). */
 int midas_release_workspace(midas_ctx *ctx);
 /* Memory budgets (bytes) of the CBFM batching: the padded block matrices of one
  * batched block-LU chunk, the column strips of the dense reduced-matrix

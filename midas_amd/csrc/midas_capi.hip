@@ -577,6 +577,7 @@ int midas_finalize(midas_ctx *ctx)
     hipStreamSynchronize(ctx->st);
     ctx->ws.release();
     ctx->nws.release();
+    ctx->release_cbfm();
     for (hipGraphExec_t g : ctx->cbfm_graphs)
         if (g) hipGraphExecDestroy(g);
     hipStreamSynchronize(ctx->ps);
@@ -599,6 +600,7 @@ int midas_release_workspace(midas_ctx *ctx)
     MIDAS_HIP(hipSetDevice(ctx->device));
     MIDAS_HIP(hipStreamSynchronize(ctx->st));
     ctx->ws.release();
+    ctx->release_cbfm();
     DBuf *b[] = {&ctx->cells.x, &ctx->cells.y, &ctx->cells.z, &ctx->cells.sc, &ctx->cells.eps, &ctx->cells.kappa,
                  &ctx->cells.che, &ctx->cells.znnpp, &ctx->cells.kchi, &ctx->cells.id, &ctx->cells.a};
     for (DBuf *x : b) x->release();

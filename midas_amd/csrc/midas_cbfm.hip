@@ -197,10 +197,6 @@ __global__ void k_blk_colmax3(const double2 *R, const double2 *X, const double2 
     }
 }
 
-struct BlockMixedWs {
-    DBuf W, E, R, D, tmp, scr, rs, anorm, cm, l21;
-};
-
 // X (nc blocks of mpc x T2c at stride mpc T2c, ld mpc) holds E on entry and
 // X on return; Z (nc blocks of mpc x mpc, ld mpc) stays unmodified.
 static int block_solve_mixed(midas_ctx *ctx, int mode, int nc, int mpc, int T2c, const double2 *Z, double2 *X,
@@ -222,8 +218,8 @@ static int block_solve_mixed(midas_ctx *ctx, int mode, int nc, int mpc, int T2c,
                        bw.rs.as<double>());
     hipLaunchKernelGGL(k_blk_max, dim3(nc), dim3(256), 0, st, bw.rs.as<double>(), mpc, bw.anorm.as<double>());
     MIDAS_HIP(hipGetLastError());
-    // batched complex-FP32 LU
-    LuWork<float> w;
+    // batched complex-FP32 LU (workspace kept with the batch buffers)
+    LuWork<float> &w = bw.lu;
     w.share_stream(ctx->ps, ctx->pev[0], ctx->pev[1]);
     TRY(w.alloc(mpc, nb, nc, zs));
     TrailingGemm<float> tg;
@@ -284,7 +280,6 @@ static int block_solve_mixed(midas_ctx *ctx, int mode, int nc, int mpc, int T2c,
         }
         if (!any) break;
     }
-    w.release();
     // blocks above tol: FP64 LU of their Z (unmodified) and E
     for (int q = 0; q < nc; ++q) {
         if (berr[q] <= tol) continue;
@@ -643,13 +638,14 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
     auto phase = [&](const char *name) {
         if (tm) std::fprintf(stderr, "[cbfm] %-28s %10.2f ms\n", name, tm->ms());
     };
-    DevCellsOwned dc;
+    // every device buffer of the call is the context's, grown when needed
+    DevCellsOwned &dc = ctx->cbfm_cells;
     TRY(dc.from_host(cells, k0, quirks, st));
-    DBuf dthc, dphc, dtt, dtp;
-    TRY(upload(dthc, th_cbf, sizeof(double) * ntr_cbf, st));
-    TRY(upload(dphc, ph_cbf, sizeof(double) * ntr_cbf, st));
-    TRY(upload(dtt, tr_th, sizeof(double) * ntr, st));
-    TRY(upload(dtp, tr_ph, sizeof(double) * ntr, st));
+    DBuf &dthc = pooled(ctx, "thc"), &dphc = pooled(ctx, "phc"), &dtt = pooled(ctx, "tt"), &dtp = pooled(ctx, "tp");
+    TRY(upload_keep(dthc, th_cbf, sizeof(double) * ntr_cbf, st));
+    TRY(upload_keep(dphc, ph_cbf, sizeof(double) * ntr_cbf, st));
+    TRY(upload_keep(dtt, tr_th, sizeof(double) * ntr, st));
+    TRY(upload_keep(dtp, tr_ph, sizeof(double) * ntr, st));
 
     // ---- extended cell lists of every block (block cells first)
     std::vector<int> nbk(nb), sb(nb), ioff(nb + 1, 0);
@@ -668,8 +664,8 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         for (int i = 0; i < nbk[b]; ++i) allidx[ioff[b] + i] = bs[b] + i;
         for (int e = es[b]; e < es[b + 1]; ++e) allidx[ioff[b] + nbk[b] + (e - es[b])] = blocks->ext_cells[e];
     }
-    DBuf didx;
-    TRY(upload(didx, allidx.data(), sizeof(int) * allidx.size(), st));
+    DBuf &didx = pooled(ctx, "idx");
+    TRY(upload_keep(didx, allidx.data(), sizeof(int) * allidx.size(), st));
 
     // ---- 1. characteristic basis functions
     // 1a. block solutions X_b = Z_b^-1 E_b (FP64 LU).  All blocks are solved
@@ -681,17 +677,17 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
     const long mp = mmax;
     int rpad = 1;
     for (int b = 0; b < nb; ++b) rpad = std::max(rpad, std::min(3 * sb[b], T2c));
-    DBuf Call, Xall;
-    TRY(Call.alloc(sizeof(double2) * std::max<long>(coff[nb], 1)));
-    TRY(Xall.alloc(sizeof(double2) * mp * T2c * nb));
+    DBuf &Call = pooled(ctx, "C"), &Xall = pooled(ctx, "X");
+    TRY(Call.reserve(sizeof(double2) * std::max<long>(coff[nb], 1)));
+    TRY(Xall.reserve(sizeof(double2) * mp * T2c * nb));
     auto Xb = [&](int b) { return Xall.as<double2>() + (long)b * mp * T2c; };
     int worst_info = 0;
     // SR: the fSR of every block from one pass of norms (setfSR)
     std::vector<double> fsr(nb, 0.0);
     if (sr) {
-        DBuf stats;
-        TRY(stats.alloc(sizeof(SrStats) * nb));
-        MIDAS_HIP(hipMemsetAsync(stats.p, 0, stats.bytes, st));
+        DBuf &stats = pooled(ctx, "srstats");
+        TRY(stats.reserve(sizeof(SrStats) * nb));
+        MIDAS_HIP(hipMemsetAsync(stats.p, 0, sizeof(SrStats) * nb, st));
         for (int b = 0; b < nb; ++b) {
             CellSet S{sb[b], 0, didx.as<int>() + ioff[b]};
             hipLaunchKernelGGL(k_sr_stats, dim3((sb[b] + 63) / 64, sb[b]), dim3(64), 0, st, dc.v, S, k0,
@@ -737,7 +733,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         // the block phase; a0006 at ~9 lost 60 %), else the FP64 LU.
         const int bm_set = tune(TUNE_CBFM_BLOCK_MODE);
         const int nb_mixed = 256;
-        BlockMixedWs bmw;
+        BlockMixedWs &bmw = ctx->cbfm_mixed;
         for (int pos = 0; pos < nb;) {
             const long mpc = 3L * sb[order[pos]];
             const long zb = (long)sizeof(double2) * mpc * mpc;
@@ -750,16 +746,17 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
             smax_c = std::max(smax_c, (long)nc);
             pos += nc;
         }
-        DBuf Zall, ipiv, perm, tmp, Xc;
-        TRY(Zall.alloc(zmax));
-        TRY(ipiv.alloc(sizeof(int) * pmax_c));
-        TRY(perm.alloc(sizeof(int) * pmax_c));
-        TRY(tmp.alloc(sizeof(double2) * tmax));
-        TRY(Xc.alloc(sizeof(double2) * xmax));
+        DBuf &Zall = pooled(ctx, "Zb"), &ipiv = pooled(ctx, "ipiv"), &perm = pooled(ctx, "perm"),
+             &tmp = pooled(ctx, "getrs_tmp"), &Xc = pooled(ctx, "Xc");
+        TRY(Zall.reserve(zmax));
+        TRY(ipiv.reserve(sizeof(int) * pmax_c));
+        TRY(perm.reserve(sizeof(int) * pmax_c));
+        TRY(tmp.reserve(sizeof(double2) * tmax));
+        TRY(Xc.reserve(sizeof(double2) * xmax));
         MIDAS_HIP(hipMemsetAsync(Xall.p, 0, sizeof(double2) * mp * T2c * nb, st));
         std::vector<int> hinfo(smax_c);
-        DBuf djob, l21f;
-        TRY(djob.alloc(sizeof(BlockJob) * smax_c));
+        DBuf &djob = pooled(ctx, "jobs"), &l21f = pooled(ctx, "l21");
+        TRY(djob.reserve(sizeof(BlockJob) * smax_c));
         const BlockJob *jd = djob.as<BlockJob>();
         const int *idx = didx.as<int>();
         const bool use_graph = tune(TUNE_CBFM_GRAPH) != 0;
@@ -793,8 +790,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
             const int block_mode = bm_set == 5 ? (mpc >= 16L * T2c ? MIDAS_SOLVE_F16_IR : MIDAS_SOLVE_FP64)
                                    : bm_set == 3 ? MIDAS_SOLVE_F16_IR : bm_set;
             const bool fp64 = block_mode == MIDAS_SOLVE_FP64;
-            LuWork<double> w;
-            LuRelease<double> wguard{w};
+            LuWork<double> &w = ctx->cbfm_lu;
             TrailingGemm<double> tg;
             if (fp64) {
                 w.share_stream(ctx->ps, ctx->pev[0], ctx->pev[1]);
@@ -875,13 +871,15 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         // launch of the batched Hermitian eigensolver (midas_heev.hip):
         // B w = lambda w gives G (L w) = lambda (L w), |L w|^2 = lambda.
         const long gs = (long)rpad * rpad;
-        DBuf G, Lc, dr, D, dinfo, Wk, Wr, HW, Bm;
-        TRY(G.alloc(sizeof(double2) * gs * nb));
-        TRY(Lc.alloc(sizeof(double2) * gs * nb));
-        TRY(dr.alloc(sizeof(int) * nb));
-        TRY(dinfo.alloc(sizeof(int) * nb));
-        TRY(Wk.alloc(sizeof(double2) * (size_t)T2c * T2c));
-        MIDAS_HIP(hipMemsetAsync(G.p, 0, G.bytes, st));
+        DBuf &G = pooled(ctx, "gram"), &Lc = pooled(ctx, "chol"), &dr = pooled(ctx, "rank"),
+             &dinfo = pooled(ctx, "info"), &Wk = pooled(ctx, "wk"), &Wr = pooled(ctx, "wr"), &HW = pooled(ctx, "heev"),
+             &Bm = pooled(ctx, "lhl"), &D = pooled(ctx, "eig");
+        TRY(G.reserve(sizeof(double2) * gs * nb));
+        TRY(Lc.reserve(sizeof(double2) * gs * nb));
+        TRY(dr.reserve(sizeof(int) * nb));
+        TRY(dinfo.reserve(sizeof(int) * nb));
+        TRY(Wk.reserve(sizeof(double2) * (size_t)T2c * T2c));
+        MIDAS_HIP(hipMemsetAsync(G.p, 0, sizeof(double2) * gs * nb, st));
         bool all_tall = true;
         for (int b = 0; b < nb; ++b) all_tall = all_tall && T2c <= 3 * sb[b];
         if (all_tall) {
@@ -904,14 +902,13 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         std::vector<int> rk(nb);
         TRY(copy_sync(rk.data(), dr.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
         const int rr = *std::max_element(rk.begin(), rk.end());
-        G.release();
         std::vector<double> hd((size_t)std::max(rr, 1) * nb, 0.0);
         if (rr > 0) {
             const long bs2 = (long)rr * rr;
-            TRY(Bm.alloc(sizeof(double2) * bs2 * nb));
-            TRY(D.alloc(sizeof(double) * (size_t)rr * nb));
-            TRY(HW.alloc(heev_work_bytes(rr, nb)));
-            TRY(Wr.alloc(sizeof(double2) * (size_t)rr * T2c));
+            TRY(Bm.reserve(sizeof(double2) * bs2 * nb));
+            TRY(D.reserve(sizeof(double) * (size_t)rr * nb));
+            TRY(HW.reserve(heev_work_bytes(rr, nb)));
+            TRY(Wr.reserve(sizeof(double2) * (size_t)rr * T2c));
             TRY(launch_cgemm<double>(rr, rr, rpad, ONE, Lc.p, rpad, 2, Lc.p, rpad, 0, ZERO, Bm.p, rr, st, nb, gs, gs,
                                      bs2));
             // (large orders compute only the pairs at or above a quarter of
@@ -980,7 +977,6 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         }
     }
     phase("CBF singular subspaces");
-    Xall.release();
     std::vector<long> offs(nb + 1, 0);
     int kbmax = 0;
     for (int b = 0; b < nb; ++b) {
@@ -992,11 +988,11 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
     auto Cb = [&](int b) { return Call.as<double2>() + coff[b]; };
 
     // ---- 2. reduced matrix
-    DBuf Zc;
-    TRY(Zc.alloc(sizeof(double2) * (size_t)Kt * Kt));
+    DBuf &Zc = pooled(ctx, "Zc");
+    TRY(Zc.reserve(sizeof(double2) * (size_t)Kt * Kt));
     double2 *zc_ = Zc.as<double2>();
     const CellSet ALL{N, 0, nullptr};
-    DBuf djobs;
+    DBuf &djobs = pooled(ctx, "gjobs");
     std::vector<GJob> jobs;
     if (!use_aca) {
         // T(:, block j) = Z(:, block j) C_j for a chunk of block columns (one
@@ -1004,9 +1000,9 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         // one grouped launch
         const long tbudget = (long)strip_budget;
         const int tcols = (int)std::max<long>(kbmax, std::min<long>(Kt, tbudget / ((long)sizeof(double2) * N3)));
-        DBuf Zs, Tt;
-        TRY(Zs.alloc(sizeof(double2) * N3 * 3 * nmax));
-        TRY(Tt.alloc(sizeof(double2) * N3 * std::max(tcols, 1)));
+        DBuf &Zs = pooled(ctx, "strip"), &Tt = pooled(ctx, "strip_t");
+        TRY(Zs.reserve(sizeof(double2) * N3 * 3 * nmax));
+        TRY(Tt.reserve(sizeof(double2) * N3 * std::max(tcols, 1)));
         int j0 = 0;
         while (j0 < nb) {
             int j1 = j0, used = 0;
@@ -1039,7 +1035,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         std::vector<std::pair<int, int>> dense;
         // reused across batches (allocating multi-GB buffers per batch costs
         // milliseconds each and serialises the stream)
-        DBuf Zd, Md;
+        DBuf &Zd = pooled(ctx, "dense_z"), &Md = pooled(ctx, "dense_m");
         auto dense_pairs = [&](hipStream_t sp) -> int {
             const long zbudget = (long)dense_budget;
             size_t q0 = 0;
@@ -1137,20 +1133,21 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         struct Pipe {
             hipStream_t s2 = nullptr;
             hipEvent_t ev[5] = {};   // start, rank[2], products[2]
-            int *hk[2] = {};         // pinned: the ranks of a batch
+            int *hk[2] = {};         // pinned (the context's): the ranks of a batch
             ~Pipe()
             {
                 if (s2) hipStreamSynchronize(s2);
-                for (int *h : hk)
-                    if (h) hipHostFree(h);
             }
         } pp;
-        DBuf dj[2], Ub[2], Vb[2], dk[2], Gb[2], LR[2], dgj[2];
+        AcaBufs &ab = ctx->cbfm_aca;
+        DBuf(&dj)[2] = ab.dj, (&Ub)[2] = ab.Ub, (&Vb)[2] = ab.Vb, (&dk)[2] = ab.dk, (&Gb)[2] = ab.Gb,
+            (&LR)[2] = ab.LR, (&dgj)[2] = ab.dgj;
         if (!batches.empty()) {
             pp.s2 = ctx->s2;
             for (int q = 0; q < 5; ++q) pp.ev[q] = ctx->aev[q];
+            TRY(ab.reserve_ranks(nj_max));
             for (int q = 0; q < 2; ++q) {
-                MIDAS_HIP(hipHostMalloc((void **)&pp.hk[q], sizeof(int) * nj_max));
+                pp.hk[q] = ab.hk[q];
                 TRY(dj[q].reserve(sizeof(AcaJob) * nj_max));
                 TRY(Ub[q].reserve(sizeof(double2) * uo_max));
                 TRY(Vb[q].reserve(sizeof(double2) * vo_max));
@@ -1246,10 +1243,10 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
     }
     phase(use_aca ? "reduced matrix (ACA)" : "reduced matrix (dense)");
     // ---- 3. excitation, reduced solve, fields
-    DBuf Einc, Vc, Al, dE;
-    TRY(Einc.alloc(sizeof(double2) * N3 * T2));
-    TRY(Vc.alloc(sizeof(double2) * (size_t)Kt * T2));
-    TRY(Al.alloc(sizeof(double2) * (size_t)Kt * T2));
+    DBuf &Einc = pooled(ctx, "Einc"), &Vc = pooled(ctx, "Vc"), &Al = pooled(ctx, "alpha"), &dE = pooled(ctx, "E");
+    TRY(Einc.reserve(sizeof(double2) * N3 * T2));
+    TRY(Vc.reserve(sizeof(double2) * (size_t)Kt * T2));
+    TRY(Al.reserve(sizeof(double2) * (size_t)Kt * T2));
     TRY(launch_rhs(dc.v, ALL, ntr, dtt.as<double>(), dtp.as<double>(), k0, quirks, ref32, Einc.as<double2>(), N3,
                    st));
     // reduced excitation V_i = C_i^T E_i of every block as one grouped launch
@@ -1264,10 +1261,8 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
                       &inf, true);
     if (r < 0) return r;
     phase("reduced solve");
-    Zc.release();
-    Vc.release();
-    TRY(dE.alloc(sizeof(double2) * N3 * T2));
-    MIDAS_HIP(hipMemsetAsync(dE.p, 0, dE.bytes, st));
+    TRY(dE.reserve(sizeof(double2) * N3 * T2));
+    MIDAS_HIP(hipMemsetAsync(dE.p, 0, sizeof(double2) * N3 * T2, st));
     // E_i = C_i alpha_i of every block as one grouped launch
     jobs.clear();
     for (int i = 0; i < nb; ++i)

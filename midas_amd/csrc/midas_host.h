@@ -6,6 +6,8 @@
 #include "../../include/midas_gpu.h"
 
 #include <algorithm>
+#include <map>
+#include <string>
 #include <vector>
 
 namespace midas {
@@ -83,6 +85,13 @@ inline int upload(DBuf &b, const void *h, size_t bytes, hipStream_t st)
     if (h && bytes) MIDAS_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, st));
     return 0;
 }
+// the same into a grow-only buffer (one kept between calls)
+inline int upload_keep(DBuf &b, const void *h, size_t bytes, hipStream_t st)
+{
+    TRY(b.reserve(bytes));
+    if (h && bytes) MIDAS_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, st));
+    return 0;
+}
 
 int launch_iota1(int *id, int n, hipStream_t st);   // id[i] = i + 1 (midas_resid.hip)
 
@@ -95,26 +104,27 @@ struct DevCellsOwned {
     DBuf x, y, z, sc, eps, kappa, che, znnpp, kchi, id, a;
     DevCells v{};
     // host cells -> device; constants computed on the GPU unless supplied
+    // (grow-only buffers: a DevCellsOwned kept by the context serves repeated calls)
     int from_host(const midas_cells_t *c, double k0, int quirks, hipStream_t st)
     {
         MIDAS_CHECK(c && c->n > 0 && c->x && c->y && c->z, -1, "cells: empty or missing coordinates");
         const int n = c->n;
         const size_t d = sizeof(double) * n, z2 = 2 * d;
-        TRY(upload(x, c->x, d, st));
-        TRY(upload(y, c->y, d, st));
-        TRY(upload(this->z, c->z, d, st));
-        TRY(upload(sc, c->sc, d, st));
+        TRY(upload_keep(x, c->x, d, st));
+        TRY(upload_keep(y, c->y, d, st));
+        TRY(upload_keep(this->z, c->z, d, st));
+        TRY(upload_keep(sc, c->sc, d, st));
         std::vector<int> ids;
         if (!c->id) {
             ids.resize(n);
             for (int i = 0; i < n; ++i) ids[i] = i + 1;
         }
-        TRY(upload(id, c->id ? (const void *)c->id : (const void *)ids.data(), sizeof(int) * n, st));
-        TRY(kappa.alloc(d));
-        TRY(che.alloc(z2));
-        TRY(znnpp.alloc(z2));
-        TRY(kchi.alloc(z2));
-        TRY(a.alloc(d));
+        TRY(upload_keep(id, c->id ? (const void *)c->id : (const void *)ids.data(), sizeof(int) * n, st));
+        TRY(kappa.reserve(d));
+        TRY(che.reserve(z2));
+        TRY(znnpp.reserve(z2));
+        TRY(kchi.reserve(z2));
+        TRY(a.reserve(d));
         if (c->kappa && c->che && c->znnpp) {
             MIDAS_HIP(hipMemcpyAsync(kappa.p, c->kappa, d, hipMemcpyHostToDevice, st));
             MIDAS_HIP(hipMemcpyAsync(che.p, c->che, z2, hipMemcpyHostToDevice, st));
@@ -128,7 +138,7 @@ struct DevCellsOwned {
             MIDAS_HIP(hipStreamSynchronize(st));
         } else {
             MIDAS_CHECK(c->sc && c->eps, -1, "cells: need sc and eps (or kappa, che, znnpp)");
-            TRY(upload(eps, c->eps, z2, st));
+            TRY(upload_keep(eps, c->eps, z2, st));
             TRY(launch_cell_params(n, sc.as<double>(), eps.as<double>(), k0, quirks, a.as<double>(),
                                    che.as<double>(), znnpp.as<double>(), kappa.as<double>(), kchi.as<double>(), st));
         }
@@ -196,6 +206,48 @@ struct SolveWs {
     }
 };
 
+// Buffers of the CBF block batches' mixed-precision solves
+struct BlockMixedWs {
+    DBuf W, E, R, D, tmp, scr, rs, anorm, cm, l21;
+    LuWork<float> lu;
+    void release()
+    {
+        for (DBuf *b : {&W, &E, &R, &D, &tmp, &scr, &rs, &anorm, &cm, &l21}) b->release();
+        lu.release();
+    }
+};
+
+// The ACA pipeline's buffers of its two batch parities, and the pinned host
+// words that bring a batch's ranks back
+struct AcaBufs {
+    DBuf dj[2], Ub[2], Vb[2], dk[2], Gb[2], LR[2], dgj[2];
+    int *hk[2] = {};
+    size_t hk_n = 0;
+    int reserve_ranks(size_t n)
+    {
+        if (n <= hk_n) return 0;
+        release_ranks();
+        for (int q = 0; q < 2; ++q) MIDAS_HIP(hipHostMalloc((void **)&hk[q], sizeof(int) * n));
+        hk_n = n;
+        return 0;
+    }
+    void release_ranks()
+    {
+        for (int *&h : hk)
+            if (h) {
+                hipHostFree(h);
+                h = nullptr;
+            }
+        hk_n = 0;
+    }
+    void release()
+    {
+        for (int q = 0; q < 2; ++q)
+            for (DBuf *b : {&dj[q], &Ub[q], &Vb[q], &dk[q], &Gb[q], &LR[q], &dgj[q]}) b->release();
+        release_ranks();
+    }
+};
+
 struct midas_ctx {
     int device = 0;
     hipStream_t st = nullptr;
@@ -217,7 +269,30 @@ struct midas_ctx {
     // executable graphs of the CBFM block batches (one per batch position),
     // updated in place by the next call whose batch has the same launches
     std::vector<hipGraphExec_t> cbfm_graphs;
+    // the CBFM call's device buffers, kept between calls and grown when a call
+    // needs more (hipMalloc and hipFree wait for the device: a call that made
+    // and freed its buffers left the GPU idle around each): named buffers, the
+    // cells, the block batches' LU workspaces (midas_release_workspace frees them)
+    std::map<std::string, DBuf> cbfm_pool;
+    DevCellsOwned cbfm_cells;
+    BlockMixedWs cbfm_mixed;
+    LuWork<double> cbfm_lu;
+    AcaBufs cbfm_aca;
+    void release_cbfm()
+    {
+        cbfm_pool.clear();
+        for (DBuf *b : {&cbfm_cells.x, &cbfm_cells.y, &cbfm_cells.z, &cbfm_cells.sc, &cbfm_cells.eps,
+                        &cbfm_cells.kappa, &cbfm_cells.che, &cbfm_cells.znnpp, &cbfm_cells.kchi, &cbfm_cells.id,
+                        &cbfm_cells.a})
+            b->release();
+        cbfm_mixed.release();
+        cbfm_lu.release();
+        cbfm_aca.release();
+    }
 };
+
+// a named CBFM buffer of the context (grow it with reserve)
+inline DBuf &pooled(midas_ctx *ctx, const char *name) { return ctx->cbfm_pool[name]; }
 
 // Event timer on a stream; a disabled timer creates no events and reads 0.
 struct Timer {
