@@ -577,7 +577,7 @@ int midas_finalize(midas_ctx *ctx)
     hipStreamSynchronize(ctx->st);
     ctx->ws.release();
     ctx->nws.release();
-    ctx->release_cbfm();
+    ctx->release_calls();
     for (hipGraphExec_t g : ctx->cbfm_graphs)
         if (g) hipGraphExecDestroy(g);
     hipStreamSynchronize(ctx->ps);
@@ -600,7 +600,7 @@ int midas_release_workspace(midas_ctx *ctx)
     MIDAS_HIP(hipSetDevice(ctx->device));
     MIDAS_HIP(hipStreamSynchronize(ctx->st));
     ctx->ws.release();
-    ctx->release_cbfm();
+    ctx->release_calls();
     DBuf *b[] = {&ctx->cells.x, &ctx->cells.y, &ctx->cells.z, &ctx->cells.sc, &ctx->cells.eps, &ctx->cells.kappa,
                  &ctx->cells.che, &ctx->cells.znnpp, &ctx->cells.kchi, &ctx->cells.id, &ctx->cells.a};
     for (DBuf *x : b) x->release();
@@ -945,16 +945,18 @@ int midas_scattering_quantities(midas_ctx *ctx, const double *S, long lds, int n
                 "scattering_quantities: nrx_tot %d has no forward / backward receivers", nrx_tot);
     MIDAS_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->st;
-    DBuf dS, drt, drp, dw, dtt, dtp, dout;
-    TRY(dS.alloc(sizeof(double2) * (size_t)nrx * 4 * ntr));
+    // the context's buffers, grown when needed
+    DBuf &dS = pooled(ctx, "q.S"), &drt = pooled(ctx, "q.rth"), &drp = pooled(ctx, "q.rph"), &dw = pooled(ctx, "q.w"),
+         &dtt = pooled(ctx, "q.tth"), &dtp = pooled(ctx, "q.tph"), &dout = pooled(ctx, "q.sums");
+    TRY(dS.reserve(sizeof(double2) * (size_t)nrx * 4 * ntr));
     MIDAS_HIP(hipMemcpy2DAsync(dS.p, sizeof(double2) * nrx, S, sizeof(double2) * lds, sizeof(double2) * nrx, 4 * ntr,
                                hipMemcpyHostToDevice, st));
-    TRY(upload(drt, rx_th, sizeof(double) * nrx, st));
-    TRY(upload(drp, rx_ph, sizeof(double) * nrx, st));
-    if (rx_type == MIDAS_RX_LB) TRY(upload(dw, rx_w, sizeof(double) * nrx, st));
-    TRY(upload(dtt, tr_th, sizeof(double) * ntr, st));
-    TRY(upload(dtp, tr_ph, sizeof(double) * ntr, st));
-    TRY(dout.alloc(sizeof(double) * 2 * ntr));
+    TRY(upload_keep(drt, rx_th, sizeof(double) * nrx, st));
+    TRY(upload_keep(drp, rx_ph, sizeof(double) * nrx, st));
+    if (rx_type == MIDAS_RX_LB) TRY(upload_keep(dw, rx_w, sizeof(double) * nrx, st));
+    TRY(upload_keep(dtt, tr_th, sizeof(double) * ntr, st));
+    TRY(upload_keep(dtp, tr_ph, sizeof(double) * ntr, st));
+    TRY(dout.reserve(sizeof(double) * 2 * ntr));
     TRY(scatq_sums(dS.as<double2>(), nrx, nrx, ntr, drt.as<double>(), drp.as<double>(),
                    rx_type == MIDAS_RX_LB ? dw.as<double>() : nullptr, dtt.as<double>(), dtp.as<double>(), quirks,
                    dout.as<double>(), st));
@@ -1070,14 +1072,17 @@ int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double
     MIDAS_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->st;
     const int n = cells->n, n3 = 3 * n;
-    DevCellsOwned dc;
+    // the context's buffers, grown when needed (none made or freed per call)
+    DevCellsOwned &dc = ctx->call_cells;
     TRY(dc.from_host(cells, k0, quirks, st));
-    DBuf Z, X, dth, dph;
+    DBuf &X = pooled(ctx, "mom.X"), &dth = pooled(ctx, "mom.th"), &dph = pooled(ctx, "mom.ph");
+    DBuf Zloc;
+    DBuf &Z = mom_matrix_free(ctx->ws, mode, ntr) ? Zloc : pooled(ctx, "mom.Z");
     // (a matrix-free mixed solve needs no FP64 matrix unless it falls back)
-    if (!mom_matrix_free(ctx->ws, mode, ntr)) TRY(Z.alloc(sizeof(double2) * (size_t)n3 * n3));
-    TRY(X.alloc(sizeof(double2) * (size_t)n3 * 2 * ntr));
-    TRY(upload(dth, tr_th, sizeof(double) * ntr, st));
-    TRY(upload(dph, tr_ph, sizeof(double) * ntr, st));
+    if (!mom_matrix_free(ctx->ws, mode, ntr)) TRY(Z.reserve(sizeof(double2) * (size_t)n3 * n3));
+    TRY(X.reserve(sizeof(double2) * (size_t)n3 * 2 * ntr));
+    TRY(upload_keep(dth, tr_th, sizeof(double) * ntr, st));
+    TRY(upload_keep(dph, tr_ph, sizeof(double) * ntr, st));
     int r = mom_solve_dev(ctx, ctx->ws, dc.v, k0, quirks, ntr, dth.as<double>(), dph.as<double>(), mode, opts,
                           Z.as<double2>(), n3, X.as<double2>(), n3, info, nullptr);
     // a refinement that missed its tolerance (no_fallback) still delivers the
@@ -1085,26 +1090,27 @@ int midas_compute_efields_mom(midas_ctx *ctx, const midas_cells_t *cells, double
     if (r < 0 && r != MIDAS_ERR_NOT_CONVERGED) return r;
     int singular = r;
     std::string nc_msg = r == MIDAS_ERR_NOT_CONVERGED ? midas::last_error() : "";
-    Z.release();
     if (S_total && nrx > 0) {
-        DBuf drt, drp, db, dS, FF, E3, G;
+        DBuf &drt = pooled(ctx, "ff.rth"), &drp = pooled(ctx, "ff.rph"), &db = pooled(ctx, "ff.beta"),
+             &dS = pooled(ctx, "ff.S"), &FF = pooled(ctx, "ff.coef"), &E3 = pooled(ctx, "ff.E3"),
+             &G = pooled(ctx, "ff.G");
         double b0 = 0.0;
-        TRY(upload(drt, rx_th, sizeof(double) * nrx, st));
-        TRY(upload(drp, rx_ph, sizeof(double) * nrx, st));
-        TRY(upload(db, &b0, sizeof(double), st));
-        TRY(dS.alloc(sizeof(double2) * (size_t)nrx * 4 * ntr));
-        TRY(FF.alloc(sizeof(double2) * (size_t)nrx * n));
-        TRY(E3.alloc(sizeof(double2) * (size_t)n * 6 * ntr));
-        TRY(G.alloc(sizeof(double2) * (size_t)nrx * 6 * ntr));
+        TRY(upload_keep(drt, rx_th, sizeof(double) * nrx, st));
+        TRY(upload_keep(drp, rx_ph, sizeof(double) * nrx, st));
+        TRY(upload_keep(db, &b0, sizeof(double), st));
+        TRY(dS.reserve(sizeof(double2) * (size_t)nrx * 4 * ntr));
+        TRY(FF.reserve(sizeof(double2) * (size_t)nrx * n));
+        TRY(E3.reserve(sizeof(double2) * (size_t)n * 6 * ntr));
+        TRY(G.reserve(sizeof(double2) * (size_t)nrx * 6 * ntr));
         TRY(far_field(dc.v, X.as<double2>(), n3, ntr, nrx, drt.as<double>(), drp.as<double>(), 1, db.as<double>(), k0,
                       quirks, dS.as<double2>(), nrx, FF.as<double2>(), E3.as<double2>(), G.as<double2>(), st));
         MIDAS_HIP(hipMemcpyAsync(S_total, dS.p, sizeof(double2) * (size_t)nrx * 4 * ntr, hipMemcpyDeviceToHost, st));
         MIDAS_HIP(hipStreamSynchronize(st));
     }
     if (C_ext && C_abs) {
-        DBuf dce, dca;
-        TRY(dce.alloc(sizeof(double2) * ntr));
-        TRY(dca.alloc(sizeof(double2) * ntr));
+        DBuf &dce = pooled(ctx, "cext"), &dca = pooled(ctx, "cabs");
+        TRY(dce.reserve(sizeof(double2) * ntr));
+        TRY(dca.reserve(sizeof(double2) * ntr));
         TRY(cext_cabs(dc.v, X.as<double2>(), n3, ntr, dth.as<double>(), dph.as<double>(), k0, quirks, dce.as<double>(),
                       dca.as<double>(), st));
         MIDAS_HIP(hipMemcpyAsync(C_ext, dce.p, sizeof(double2) * ntr, hipMemcpyDeviceToHost, st));

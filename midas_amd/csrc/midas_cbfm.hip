@@ -639,7 +639,7 @@ static int compute_efields_cbfm(midas_ctx *ctx, const midas_cells_t *cells, cons
         if (tm) std::fprintf(stderr, "[cbfm] %-28s %10.2f ms\n", name, tm->ms());
     };
     // every device buffer of the call is the context's, grown when needed
-    DevCellsOwned &dc = ctx->cbfm_cells;
+    DevCellsOwned &dc = ctx->call_cells;
     TRY(dc.from_host(cells, k0, quirks, st));
     DBuf &dthc = pooled(ctx, "thc"), &dphc = pooled(ctx, "phc"), &dtt = pooled(ctx, "tt"), &dtp = pooled(ctx, "tp");
     TRY(upload_keep(dthc, th_cbf, sizeof(double) * ntr_cbf, st));

@@ -269,21 +269,23 @@ struct midas_ctx {
     // executable graphs of the CBFM block batches (one per batch position),
     // updated in place by the next call whose batch has the same launches
     std::vector<hipGraphExec_t> cbfm_graphs;
-    // the CBFM call's device buffers, kept between calls and grown when a call
-    // needs more (hipMalloc and hipFree wait for the device: a call that made
-    // and freed its buffers left the GPU idle around each): named buffers, the
-    // cells, the block batches' LU workspaces (midas_release_workspace frees them)
-    std::map<std::string, DBuf> cbfm_pool;
-    DevCellsOwned cbfm_cells;
+    // the device buffers of the MoM and CBFM calls (host-pointer entry points),
+    // kept between calls and grown when a call needs more (hipMalloc and
+    // hipFree wait for the device: a call that made and freed its buffers left
+    // the GPU idle around each): named buffers, the cells, the CBF block
+    // batches' LU workspaces and the ACA pipeline's buffers
+    // (midas_release_workspace frees them)
+    std::map<std::string, DBuf> call_pool;
+    DevCellsOwned call_cells;
     BlockMixedWs cbfm_mixed;
     LuWork<double> cbfm_lu;
     AcaBufs cbfm_aca;
-    void release_cbfm()
+    void release_calls()
     {
-        cbfm_pool.clear();
-        for (DBuf *b : {&cbfm_cells.x, &cbfm_cells.y, &cbfm_cells.z, &cbfm_cells.sc, &cbfm_cells.eps,
-                        &cbfm_cells.kappa, &cbfm_cells.che, &cbfm_cells.znnpp, &cbfm_cells.kchi, &cbfm_cells.id,
-                        &cbfm_cells.a})
+        call_pool.clear();
+        for (DBuf *b : {&call_cells.x, &call_cells.y, &call_cells.z, &call_cells.sc, &call_cells.eps,
+                        &call_cells.kappa, &call_cells.che, &call_cells.znnpp, &call_cells.kchi, &call_cells.id,
+                        &call_cells.a})
             b->release();
         cbfm_mixed.release();
         cbfm_lu.release();
@@ -291,8 +293,8 @@ struct midas_ctx {
     }
 };
 
-// a named CBFM buffer of the context (grow it with reserve)
-inline DBuf &pooled(midas_ctx *ctx, const char *name) { return ctx->cbfm_pool[name]; }
+// a named call buffer of the context (grow it with reserve)
+inline DBuf &pooled(midas_ctx *ctx, const char *name) { return ctx->call_pool[name]; }
 
 // Event timer on a stream; a disabled timer creates no events and reads 0.
 struct Timer {
