@@ -396,6 +396,11 @@ int solve_dev(midas_ctx *ctx, int n, int nrhs, double2 *Z, long ldz, float2 *W32
     if (two) {
         TRY(ws.trailp.reserve(trail16_scratch_bytes(n, nbi, nbi)));
         tgp.scratch = ws.trailp.p;
+        // (lent workspaces have no room for it: their steps split A twice)
+        if (!ws.lent && tg.mode != TRAIL_NATIVE) {
+            TRY(ws.trail2.reserve(trail16_scratch_bytes(nbi, n, nbi)));
+            tg.scratch2 = ws.trail2.p;
+        }
     }
     // pivot search in the diagonal blocks (MoM solves; not with lent buffers)
     w.diag_pivot = ws.pivot_search && !ws.lent;
@@ -519,7 +524,7 @@ namespace midas {
 int mfma_probe(int which, double *dout, hipStream_t st);
 int launch_trail16(int mode, int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
                    long ldc, void *scratch, hipStream_t st, int batch = 1, long stride = 0, bool eye_minus_a = false,
-                   bool eye_minus_b = false);
+                   bool eye_minus_b = false, bool a_ready = false);
 }
 
 // ===========================================================================

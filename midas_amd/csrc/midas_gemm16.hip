@@ -662,7 +662,8 @@ static size_t trail16_matrix_bytes(int m, int n, int k)
 
 template <bool BF, int P, int TERMS>
 static int run_trail(int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C, long ldc,
-                     char *scratch, hipStream_t st, int batch, long stride, bool eye_minus_a, bool eye_minus_b)
+                     char *scratch, hipStream_t st, int batch, long stride, bool eye_minus_a, bool eye_minus_b,
+                     bool a_ready)
 {
     const int mt = (m + T_M - 1) / T_M, nt = (n + T_N - 1) / T_N, kst = kstages(k);
     u16 *Ap = (u16 *)scratch;
@@ -682,8 +683,9 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
             // of trail16_matrix_bytes each
             const long mb = (long)trail16_matrix_bytes(m, n, k);
             const long sP = mb / 2, sS = mb / 4;
-            hipLaunchKernelGGL((k_split_a<BF, 1>), dim3(mt * T_M / 256 + 1, ks * 2, batch), dim3(256), 0, st, A, lda, m,
-                               k, mt, ks, Ap, stride, sP, (int)eye_minus_a);
+            if (!a_ready)
+                hipLaunchKernelGGL((k_split_a<BF, 1>), dim3(mt * T_M / 256 + 1, ks * 2, batch), dim3(256), 0, st, A,
+                                   lda, m, k, mt, ks, Ap, stride, sP, (int)eye_minus_a);
             hipLaunchKernelGGL((k_split_b<BF, 1>), dim3(nt * T_N, batch), dim3(64), 0, st, B, ldb, k, n, nt, ks, Bq, sq,
                                stride, sP, sS, (int)eye_minus_b);
             // algorithmic bytes: C read and written, both operands once in FP16
@@ -697,8 +699,9 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
         }
     }
     MIDAS_CHECK(kst * T_K <= 512, -1, "trail16: k = %d above 512", k);
-    hipLaunchKernelGGL((k_split_a<BF, P>), dim3(mt * T_M / 256 + 1, kst * 2), dim3(256), 0, st, A, lda, m, k, mt, kst,
-                       Ap, 0L, 0L, (int)eye_minus_a);
+    if (!a_ready)
+        hipLaunchKernelGGL((k_split_a<BF, P>), dim3(mt * T_M / 256 + 1, kst * 2), dim3(256), 0, st, A, lda, m, k, mt,
+                           kst, Ap, 0L, 0L, (int)eye_minus_a);
     hipLaunchKernelGGL((k_split_b<BF, P>), dim3(nt * T_N), dim3(64), 0, st, B, ldb, k, n, nt, kst, Bp, sc, 0L, 0L, 0L,
                        (int)eye_minus_b);
     // live accounting of the GEMM kernel alone: algorithmic flops 8 m n k
@@ -715,17 +718,18 @@ static int run_trail(int m, int n, int k, const float2 *A, long lda, const float
 }
 
 int launch_trail16(int mode, int m, int n, int k, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
-                   long ldc, void *scratch, hipStream_t st, int batch, long stride, bool eye_minus_a, bool eye_minus_b)
+                   long ldc, void *scratch, hipStream_t st, int batch, long stride, bool eye_minus_a, bool eye_minus_b,
+                   bool a_ready)
 {
     if (m <= 0 || n <= 0 || k <= 0) return 0;
     MIDAS_CHECK(scratch, -1, "trail16: no scratch");
     char *s = (char *)scratch;
-    const bool ea = eye_minus_a, eb = eye_minus_b;
+    const bool ea = eye_minus_a, eb = eye_minus_b, ar = a_ready;
     switch (mode) {
-    case TRAIL_F16_1: return run_trail<false, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb);
-    case TRAIL_F16_3: return run_trail<false, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb);
-    case TRAIL_BF16_1: return run_trail<true, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb);
-    case TRAIL_BF16_3: return run_trail<true, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb);
+    case TRAIL_F16_1: return run_trail<false, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb, ar);
+    case TRAIL_F16_3: return run_trail<false, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb, false);
+    case TRAIL_BF16_1: return run_trail<true, 1, 1>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb, ar);
+    case TRAIL_BF16_3: return run_trail<true, 2, 3>(m, n, k, A, lda, B, ldb, C, ldc, s, st, batch, stride, ea, eb, false);
     }
     MIDAS_CHECK(false, -1, "trail16: bad mode %d", mode);
 }
@@ -793,7 +797,7 @@ int TrailingGemm<float>::operator()(int m, int n, int k, const void *A, long lda
     MIDAS_CHECK(batch == 1 || mode == TRAIL_F16_1 || mode == TRAIL_BF16_1, -1,
                 "trail16: batched split-precision updates are single-term only");
     return launch_trail16(mode, m, n, k, (const float2 *)A, lda, (const float2 *)B, ldb, (float2 *)C, ldc, scratch,
-                          st, batch, stride, eye_minus_a, eye_minus_b);
+                          st, batch, stride, eye_minus_a, eye_minus_b, a_ready && batch == 1);
 }
 
 }  // namespace midas

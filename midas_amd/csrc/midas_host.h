@@ -177,6 +177,7 @@ struct SolveWs {
     DBuf W;            // complex-FP32 copy of the matrix (mixed modes)
     DBuf trail;        // split-precision operand planes of the trailing update
     DBuf trailp;       // the same for the panel stream's update (two-panel steps)
+    DBuf trail2;       // the main stream's U12 and second-panel products (two-panel steps)
     DBuf ipiv, perm;   // interchanges and the final permutation
     DBuf norms;        // row sums, column maxima, scalars
     DBuf zx;           // one-pass residual / row-sum scratch
@@ -199,7 +200,7 @@ struct SolveWs {
     bool lent = false; // every buffer lent from a fixed scratch: nothing may be allocated
     void release()
     {
-        DBuf *b[] = {&W, &trail, &trailp, &ipiv, &perm, &norms, &zx, &R, &D, &t2, &xbest, &zcopy, &V, &l21, &mfs};
+        DBuf *b[] = {&W, &trail, &trailp, &trail2, &ipiv, &perm, &norms, &zx, &R, &D, &t2, &xbest, &zcopy, &V, &l21, &mfs};
         for (DBuf *x : b) x->release();
         luf.release();
         lud.release();

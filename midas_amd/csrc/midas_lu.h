@@ -68,6 +68,9 @@ template <typename T> struct TrailingGemm {
     bool library = true;            // FP64: rocBLAS ZGEMM allowed (where fp64_gemm_library asks for it)
     bool eye_minus_a = false;       // split modes: C -= (I - A) B instead (the identity of A's order)
     bool eye_minus_b = false;       // split modes: C -= A (I - B) instead
+    bool a_ready = false;           // split modes: A's planes are in scratch already (a call with the same A)
+    void *scratch2 = nullptr;       // two-panel steps: scratch of the U12 and second-panel products, so that
+                                    // the main update's A planes outlive them
     int operator()(int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C, long ldc,
                    hipStream_t st) const;
 };

@@ -1525,22 +1525,29 @@ static int outer_panel(typename cvec<T>::t *A, long lda, int n, int k, int W, in
 // first's update before their own triangular solve.
 template <typename T>
 static int outer_update(typename cvec<T>::t *A, long lda, int n, int k, int W, int c0, int c1, LuWork<T> &w,
-                        hipStream_t st, const TrailingGemm<T> &tg)
+                        hipStream_t st, const TrailingGemm<T> &tg, bool reuse_a = false)
 {
     if (c1 <= c0) return 0;
     const int h1 = min(w.nb, W), h2 = W - h1;
-    int r = u12_solve<T>(A, lda, k, h1, c0, c1, w, st, nullptr, 0, &tg);
+    // the smaller products on a scratch of their own (when there is one), so
+    // that the main update's A planes, split by the step's first call, serve
+    // its second (reuse_a) as well
+    TrailingGemm<T> ts = tg;
+    if (tg.scratch2) ts.scratch = tg.scratch2;
+    int r = u12_solve<T>(A, lda, k, h1, c0, c1, w, st, nullptr, 0, &ts);
     if (r) return r;
     if (h2 > 0) {
-        r = tg(h2, c1 - c0, h1, A + (k + h1) + (long)k * lda, lda, A + k + (long)c0 * lda, lda,
+        r = ts(h2, c1 - c0, h1, A + (k + h1) + (long)k * lda, lda, A + k + (long)c0 * lda, lda,
                A + (k + h1) + (long)c0 * lda, lda, st);
         if (r) return r;
-        r = u12_solve<T>(A, lda, k + h1, h2, c0, c1, w, st, nullptr, 0, &tg);
+        r = u12_solve<T>(A, lda, k + h1, h2, c0, c1, w, st, nullptr, 0, &ts);
         if (r) return r;
     }
     const int nr = n - k - W;
     if (nr <= 0) return 0;
-    return tg(nr, c1 - c0, W, A + (k + W) + (long)k * lda, lda, A + k + (long)c0 * lda, lda,
+    TrailingGemm<T> tm = tg;
+    tm.a_ready = reuse_a && tg.scratch2 != nullptr;
+    return tm(nr, c1 - c0, W, A + (k + W) + (long)k * lda, lda, A + k + (long)c0 * lda, lda,
               A + (k + W) + (long)c0 * lda, lda, st);
 }
 
@@ -1597,7 +1604,7 @@ int getrf(typename cvec<T>::t *A, long lda, int n, int *ipiv, LuWork<T> &w, hipS
         // The rest of the update is enqueued before the panel: a panel is
         // hundreds of small launches, and the host takes milliseconds to issue
         // them; issued first, the big update runs while they are issued.
-        r = outer_update<T>(A, lda, n, k, W, k + W + W1, n, w, st, tg);
+        r = outer_update<T>(A, lda, n, k, W, k + W + W1, n, w, st, tg, true);
         if (r) return r;
         MIDAS_HIP(hipStreamWaitEvent(ps, w.ev_next, 0));
         r = outer_panel<T>(A, lda, n, k + W, W1, ipiv, w, ps, tgp);
